@@ -1,0 +1,157 @@
+/*
+ * wbfmm_hip.h -- thin C ABI between the host layer (C, wbfmm_api.c, which
+ * implements the reference API of wbfmm.h) and the hand-written HIP
+ * kernels for gfx950.  Plain pointers and sizes only.  Unless stated,
+ * pointers are device pointers, every call is asynchronous on the
+ * library's stream and returns 0 on success or a negative status.
+ *
+ * None of these entry points has a one-to-one counterpart in the
+ * reference (whose passes are C loops); each comment names the reference
+ * routine whose work the kernel performs.
+ */
+
+#ifndef WBFMM_HIP_H_INCLUDED
+#define WBFMM_HIP_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WBFMM_HIP_OK 0
+#define WBFMM_HIP_ERR_RUNTIME (-1)
+#define WBFMM_HIP_ERR_ARG (-2)
+#define WBFMM_HIP_ERR_ORDER (-3)
+#define WBFMM_HIP_ERR_DOMAIN (-4)
+#define WBFMM_HIP_ERR_NOMEM (-5)
+
+/* largest expansion order the kernels accept */
+#define WBFMM_HIP_MAX_ORDER 40
+
+/* ---- runtime ---------------------------------------------------------- */
+int wbfmm_hip_init(void);
+int wbfmm_hip_malloc(void **p, size_t bytes);
+int wbfmm_hip_free(void *p);
+/* copy of any kind (host/device either side), ordered on the stream;
+   returns after the copy when a host buffer is involved */
+int wbfmm_hip_memcpy(void *dst, const void *src, size_t bytes);
+int wbfmm_hip_memset(void *p, int v, size_t bytes);
+/* 1 when p is device (or managed) memory, 0 for host memory */
+int wbfmm_hip_is_device(const void *p);
+int wbfmm_hip_sync(void);
+const char *wbfmm_hip_last_error(void);
+
+/* per-kernel timing with HIP events on the library stream:
+   enable(1) starts recording every launch; query returns, for kernel
+   name, the number of launches and the summed milliseconds. */
+int wbfmm_hip_timing_enable(int on);
+int wbfmm_hip_timing_reset(void);
+int wbfmm_hip_timing_query(const char *name, int *launches, double *ms);
+
+/* ---- tree (src/boxes.c:118-277, src/indexing.c) ----------------------- */
+/* Morton codes of n points read at byte stride pstr (host or device);
+   writes xyz (4 doubles per point), codes and identity indices; *nbad
+   (host) gets the number of points outside the open domain cube. */
+int wbfmm_hip_morton(const void *pts, size_t pstr, unsigned n,
+                     const double *x0, double D, double *xyz,
+                     uint64_t *codes, uint32_t *idx, unsigned *nbad);
+/* n rows of 3 doubles read at byte stride pstr (host or device) copied
+   to 4-double device rows (4th entry 0) */
+int wbfmm_hip_stage4(const void *pts, size_t pstr, unsigned n, double *out);
+/* stable sort of (codes, idx) pairs by code */
+int wbfmm_hip_sort(uint64_t *codes, uint32_t *idx, unsigned n);
+/* out[4i..4i+3] = in[4 idx[i] ..] */
+int wbfmm_hip_gather4(const double *in, const uint32_t *idx, unsigned n,
+                      double *out);
+/* out[i] = src[idx[i]*sstr] (src host or device) */
+int wbfmm_hip_gather_strided(const double *src, int sstr, int ncomp,
+                             const uint32_t *idx, unsigned n, double *out);
+/* dense [start,end) of every box at `level` from sorted codes (both
+   arrays 8^level, zeroed here first) */
+int wbfmm_hip_box_ranges(const uint64_t *codes, unsigned n, unsigned level,
+                         uint32_t *start, uint32_t *end);
+/* compact list of the boxes with end>start (or flag!=0 when flag is not
+   NULL) in Morton order; *count (host) gets the list length */
+int wbfmm_hip_compact_boxes(const uint32_t *start, const uint32_t *end,
+                            const uint32_t *flag, unsigned level,
+                            uint32_t *list, unsigned *count);
+/* flag[b] |= 1 for every box at `level` holding one of the n codes */
+int wbfmm_hip_mark_boxes(const uint64_t *codes, unsigned n, unsigned level,
+                         uint32_t *flag);
+
+/* ---- operator tables --------------------------------------------------- */
+/* Real rotation matrices of every list-4 offset (343 slots), of the 8
+   child->parent and 8 parent->child directions, forward and back, up to
+   order L, built on the host in FP64 from the Gumerov-Duraiswami
+   recursions and uploaded.  Returned as one device block; *layout gets
+   the element offsets (see wbfmm_tables.c). */
+typedef struct {
+  int L;                 /* order of the tables */
+  size_t nrot;           /* doubles per rotation: sum_{n<=L} (2n+1)^2 */
+  size_t m2l_fwd, m2l_back;   /* 343 * nrot each */
+  size_t up_fwd, up_back;     /* 8 * nrot each (child -> parent) */
+  size_t dn_fwd, dn_back;     /* 8 * nrot each (parent -> child) */
+  size_t total;
+  double r[343];         /* offset length in box widths (0: not list 4) */
+} wbfmm_rot_layout_t;
+int wbfmm_hip_rotation_tables(int L, double **dev, wbfmm_rot_layout_t *lay);
+/* Laplace coaxial coefficient tables (unit distance), kinds SS, RR, SR,
+   c[kind][(m*(L+1)+n)*(L+1)+nu], closed forms of src/laplace.c:542-725 */
+int wbfmm_hip_laplace_coaxial_tables(int L, double **dev);
+
+/* ---- Laplace passes ------------------------------------------------------ */
+/* P2M into mps at `level` for the listed leaf boxes (src/laplace.c:854-989) */
+int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
+                      const uint32_t *start, const uint32_t *end,
+                      const double *xyz, const double *q,
+                      const double *nrm, const double *dip,
+                      unsigned level, const double *x0, double D, int N,
+                      double *mps);
+/* M2M: parents (list at level-1) from children at level
+   (src/pass-laplace.c:433-486, src/shift-laplace.c:225-442) */
+int wbfmm_hip_lap_m2m(const uint32_t *parents, unsigned npar,
+                      const uint32_t *cstart, const uint32_t *cend,
+                      unsigned level, double D, int Nc, int Np,
+                      const double *mps_child, double *mps_parent,
+                      const double *rot, const wbfmm_rot_layout_t *lay,
+                      const double *coax);
+/* M2L for the listed target boxes at `level`, accumulating into mpr
+   (src/pass-laplace.c:35-408) */
+int wbfmm_hip_lap_m2l(const uint32_t *targets, unsigned ntgt,
+                      const uint32_t *sstart, const uint32_t *send,
+                      unsigned level, double D, int Ns, int Nr,
+                      const double *mps, double *mpr, const double *rot,
+                      const wbfmm_rot_layout_t *lay, const double *coax);
+/* L2L from the listed parents at `level` into their children with
+   cflag != 0 (all children when cflag is NULL) (src/shift-laplace.c:
+   562-789) */
+int wbfmm_hip_lap_l2l(const uint32_t *parents, unsigned npar,
+                      const uint32_t *cflag, unsigned level, double D,
+                      int Np, int Nc, const double *mpr_parent,
+                      double *mpr_child, const double *rot,
+                      const wbfmm_rot_layout_t *lay, const double *coax);
+/* L2P + near field for n targets (xyz: 4 doubles each, device) writing
+   field values to f[tid[i]*fstr + ...] (accumulated); field bits 1
+   potential, 2 gradient; layout: potential first when both are asked
+   (src/local-laplace.c:333-630, :1091-1336) */
+int wbfmm_hip_lap_field(const double *xt, const uint32_t *tid, unsigned n,
+                        const uint32_t *start, const uint32_t *end,
+                        const double *xyz, const double *q,
+                        const double *nrm, const double *dip,
+                        unsigned level, const double *x0, double D, int N,
+                        const double *mpr, int field, double *f, int fstr);
+/* FP64 direct sum on the GPU (src/direct-laplace.c:200-336): sources in
+   xyz (4 doubles each), q and/or (nrm,dip); cutoff: pairs with r<=cutoff
+   skipped (0: none); f[i*fstr + ...] accumulated as in _lap_field */
+int wbfmm_hip_lap_direct(const double *xyz, const double *q,
+                         const double *nrm, const double *dip,
+                         unsigned nsrc, const double *xt, unsigned ntgt,
+                         int field, double cutoff, double *f, int fstr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

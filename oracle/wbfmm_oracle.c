@@ -1,0 +1,1098 @@
+/*
+ * wbfmm_oracle.c -- CPU restatement of the wbfmm octree FMM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: pytest, the smoke
+ * check and the CPU-baseline leg of bench.py call it; the product library
+ * (libwbfmm.so) never links or calls it.
+ *
+ * Each function names the reference file:line whose algorithm it restates.
+ * The code is written from the mathematics those functions implement
+ * (Gumerov & Duraiswami rotation/coaxial recurrences, normalised Legendre
+ * recursions); it is not a copy of the reference source.
+ */
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+#include <time.h>
+
+#include <quadmath.h>
+
+#include "wbfmm_oracle.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+#define CUTOFF 1e-6  /* src/wbfmm.h:36 */
+
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* index of (n,m), 0<=m<=n, in a triangular Legendre table */
+#define PIDX(n, m) ((n) * ((n) + 1) / 2 + (m))
+/* Laplace real-packed index, src/wbfmm-private.h:76 */
+#define LIDX(n, m) ((n) * (n) + 2 * (m) - 1)
+/* Helmholtz complex index, src/wbfmm.h:242 */
+#define HIDX(n, m) ((n) * ((n) + 1) + (m))
+/* rotation table index (nu,m,n), src/wbfmm-private.h:73-74 */
+#define RIDX(nu, m, n) ((n) * (4 * (n) - 1) * ((n) + 1) / 6 + ((nu) + (n)) * ((n) + 1) + (m))
+
+/* ======================================================================
+ * special functions
+ * ====================================================================== */
+
+/*
+ * Normalised associated Legendre functions, Gumerov-Duraiswami sign
+ * convention (no Condon-Shortley phase) and 1/sqrt(4 pi) scaling:
+ *   Pbar_n^m = sqrt((2n+1)/(4pi) (n-m)!/(n+m)!) P_n^m(C)
+ * Restates src/util.c:67-116 (three-term recursion in n, diagonal
+ * recursion for m=n).  P[PIDX(n,m)] for 0<=m<=n<=N.
+ */
+void orc_legendre(int N, double C, double S, double *P)
+{
+  int n, m;
+  const double s4pi = 1.0 / sqrt(4.0 * M_PI);
+  P[0] = s4pi;
+  if (N == 0) return;
+  P[PIDX(1, 0)] = C * sqrt(3.0) * s4pi;
+  P[PIDX(1, 1)] = S * sqrt(1.5) * s4pi;
+  for (n = 1; n < N; n++) {
+    /* diagonal */
+    P[PIDX(n + 1, n + 1)] = sqrt((2.0 * n + 3) / (2.0 * n + 2)) * S * P[PIDX(n, n)];
+    for (m = 0; m <= n; m++) {
+      double pm1 = (m <= n - 1) ? P[PIDX(n - 1, m)] : 0.0;
+      double v = C * sqrt(2.0 * n + 1) * P[PIDX(n, m)] -
+                 sqrt((double)(n * n - m * m) / (2 * n - 1)) * pm1;
+      P[PIDX(n + 1, m)] = v / sqrt((double)((n + 1) * (n + 1) - m * m) / (2 * n + 3));
+    }
+  }
+}
+
+/* series for j_n(x), used when x < n+1 (src/util.c:118-145) */
+static double bessel_j_series(int n, double x)
+{
+  double sc = 1.0 / (2 * n + 1), term = 1.0, s = 1.0, x2 = x * x;
+  int q;
+  for (q = 0; q < n; q++) sc *= x / (2 * q + 1);
+  for (q = 0; q < 64 && fabs(term) > 1e-15; q++) {
+    term *= -x2 / ((2 * q + 2) * (2 * n + 2 * (q + 1) + 1));
+    s += term;
+  }
+  return s * sc;
+}
+
+/*
+ * spherical Bessel j_0..j_N: upward recursion while x > n+1, series
+ * otherwise (src/util.c:147-175).
+ */
+void orc_bessel_j(int N, double x, double *j)
+{
+  int n;
+  if (x > 0.0) {
+    j[0] = sin(x) / x;
+    if (N >= 1) j[1] = j[0] / x - cos(x) / x;
+  } else {
+    j[0] = 1.0;
+    if (N >= 1) j[1] = 0.0;
+  }
+  for (n = 1; n < N; n++) {
+    if (x > n + 1)
+      j[n + 1] = j[n] * (2 * n + 1) / x - j[n - 1];
+    else
+      j[n + 1] = bessel_j_series(n + 1, x);
+  }
+}
+
+/*
+ * spherical Hankel h_n^(1)(x) = j_n + i y_n by upward recursion,
+ * h[2n], h[2n+1] = re, im (src/util.c:177-202).
+ */
+void orc_bessel_h(int N, double x, double *h)
+{
+  int n;
+  h[0] = sin(x) / x;
+  h[1] = -cos(x) / x;
+  if (N >= 1) {
+    h[2] = h[0] / x + h[1];
+    h[3] = h[1] / x - h[0];
+  }
+  for (n = 1; n < N; n++) {
+    h[2 * (n + 1) + 0] = h[2 * n + 0] * (2 * n + 1) / x - h[2 * (n - 1) + 0];
+    h[2 * (n + 1) + 1] = h[2 * n + 1] * (2 * n + 1) / x - h[2 * (n - 1) + 1];
+  }
+}
+
+/*
+ * Truncation number of Gumerov & Duraiswami (JASA 2009) as used by the
+ * reference, src/util.c:474-503.
+ */
+int orc_truncation_number(double k, double D, int level, double tol)
+{
+  /* p = (p_lo^4 + p_hi^4)^(1/4): low-frequency bound for separation
+     ratio 2 and the high-frequency estimate ka + (3 ln(1/tol))^(2/3)
+     (ka)^(1/3) / 2, ka = k * circumradius of a box at this level */
+  const double del = 2.0;
+  double ka = 0.5 * sqrt(3.0) * k * D / (1 << level);
+  double plo = 1.0 - log(tol * (1.0 - 1.0 / del) * sqrt(1.0 - 1.0 / del)) / log(del);
+  double phi = cbrt(3.0 * log(1.0 / tol));
+  phi = ka + 0.5 * phi * phi * cbrt(ka);
+  return (int)ceil(sqrt(sqrt(plo * plo * plo * plo + phi * phi * phi * phi)));
+}
+
+/* ======================================================================
+ * Morton indexing (src/indexing.c:68-163, src/boxes.c:324-339)
+ * ====================================================================== */
+
+static uint64_t spread3(uint32_t v)
+{
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+static uint32_t compact3(uint64_t x)
+{
+  x &= 0x1249249249249249ULL;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ULL;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00fULL;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffULL;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
+  x = (x ^ (x >> 32)) & 0x1fffff;
+  return (uint32_t)x;
+}
+
+uint64_t orc_morton_encode(uint32_t x, uint32_t y, uint32_t z)
+{
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+void orc_morton_decode(uint64_t m, uint32_t *x, uint32_t *y, uint32_t *z)
+{
+  *x = compact3(m);
+  *y = compact3(m >> 1);
+  *z = compact3(m >> 2);
+}
+
+/* src/boxes.c:324-339: 20-bit grid coordinates, truncation toward zero */
+uint64_t orc_point_index(const double *x, const double *x0, double D)
+{
+  uint32_t i = (uint32_t)((x[0] - x0[0]) / D * (1u << 20));
+  uint32_t j = (uint32_t)((x[1] - x0[1]) / D * (1u << 20));
+  uint32_t k = (uint32_t)((x[2] - x0[2]) / D * (1u << 20));
+  return orc_morton_encode(i, j, k);
+}
+
+/* 27-neighbourhood clipped to the domain, src/indexing.c:250-286 */
+int orc_box_neighbours(int level, uint64_t idx, uint64_t *nb)
+{
+  uint32_t i, j, k;
+  int nbox = 1 << level, n = 0, ii, jj, kk;
+  orc_morton_decode(idx, &i, &j, &k);
+  for (ii = (int)i - 1; ii <= (int)i + 1; ii++) {
+    if (ii < 0 || ii >= nbox) continue;
+    for (jj = (int)j - 1; jj <= (int)j + 1; jj++) {
+      if (jj < 0 || jj >= nbox) continue;
+      for (kk = (int)k - 1; kk <= (int)k + 1; kk++) {
+        if (kk < 0 || kk >= nbox) continue;
+        nb[n++] = orc_morton_encode(ii, jj, kk);
+      }
+    }
+  }
+  return n;
+}
+
+/*
+ * interaction list 4: children of the parent's neighbours that are not
+ * neighbours of the box (src/indexing.c:330-377).  off[3n..3n+2] gets the
+ * source-minus-target offset in box widths.
+ */
+int orc_box_list4(int level, uint64_t idx, uint64_t *list, int *off)
+{
+  uint32_t i, j, k;
+  int nbox = 1 << level, n = 0, ii, jj, kk;
+  int i0, j0, k0;
+  orc_morton_decode(idx, &i, &j, &k);
+  i0 = (int)i - (int)(i % 2) - 2;
+  j0 = (int)j - (int)(j % 2) - 2;
+  k0 = (int)k - (int)(k % 2) - 2;
+  for (ii = i0; ii <= i0 + 5; ii++) {
+    if (ii < 0 || ii >= nbox) continue;
+    for (jj = j0; jj <= j0 + 5; jj++) {
+      if (jj < 0 || jj >= nbox) continue;
+      for (kk = k0; kk <= k0 + 5; kk++) {
+        if (kk < 0 || kk >= nbox) continue;
+        if (abs(ii - (int)i) <= 1 && abs(jj - (int)j) <= 1 &&
+            abs(kk - (int)k) <= 1)
+          continue;
+        list[n] = orc_morton_encode(ii, jj, kk);
+        if (off) {
+          off[3 * n + 0] = ii - (int)i;
+          off[3 * n + 1] = jj - (int)j;
+          off[3 * n + 2] = kk - (int)k;
+        }
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+/* ======================================================================
+ * rotations
+ * ====================================================================== */
+
+int orc_rotation_size(int N) { return RIDX(N + 1, 0, N + 1); }
+
+/* G&D recursion coefficients (src/translations.c:182-203), evaluated in
+   quad precision: the recursion below loses about 0.6 decimal digits per
+   order in FP64 (2e-10 at N=12, 7e-6 at N=20 against a 60-digit run), so
+   the oracle runs the reference's recursion with a 113-bit mantissa */
+typedef __float128 qreal;
+
+static qreal rec_a(int n, int m)
+{
+  m = abs(m);
+  if (n < m) return 0;
+  return sqrtq((qreal)(n + 1 + m) * (n + 1 - m) / ((qreal)(2 * n + 1) * (2 * n + 3)));
+}
+
+static qreal rec_b(int n, int m)
+{
+  qreal v;
+  if (n < abs(m)) return 0;
+  v = sqrtq((qreal)(n - m - 1) * (n - m) / ((qreal)(2 * n - 1) * (2 * n + 1)));
+  return (m >= 0) ? v : -v;
+}
+
+/*
+ * Rotation coefficients H_n^{nu,m}(theta), 0<=m<=n, -n<=nu<=n, n<=N.
+ * m=0 entries from the normalised Legendre functions (G&D 5.48), then the
+ * recursion G&D 5.55 advancing m; restates src/rotations.c:131-207.
+ * H must hold orc_rotation_size(N) doubles.
+ */
+void orc_H_rotation(double *H, int N, double th)
+{
+  int L = 2 * N, n, m, nu, i;
+  qreal C = cosq((qreal)th), S = sinq((qreal)th);
+  int nrot = RIDX(L + 1, 0, L + 1);
+  qreal *h = (qreal *)calloc(nrot, sizeof(qreal));
+  qreal *P = (qreal *)malloc(sizeof(qreal) * (L + 2) * (L + 3) / 2);
+  const qreal s4pi = 1 / sqrtq(4 * M_PIq);
+  P[0] = s4pi;
+  if (L >= 1) {
+    P[PIDX(1, 0)] = C * sqrtq((qreal)3) * s4pi;
+    P[PIDX(1, 1)] = S * sqrtq((qreal)1.5Q) * s4pi;
+  }
+  for (n = 1; n < L; n++) {
+    P[PIDX(n + 1, n + 1)] = sqrtq((qreal)(2 * n + 3) / (2 * n + 2)) * S * P[PIDX(n, n)];
+    for (m = 0; m <= n; m++) {
+      qreal pm1 = (m <= n - 1) ? P[PIDX(n - 1, m)] : 0;
+      P[PIDX(n + 1, m)] = (C * sqrtq((qreal)(2 * n + 1)) * P[PIDX(n, m)] -
+                           sqrtq((qreal)(n * n - m * m) / (2 * n - 1)) * pm1) /
+                          sqrtq((qreal)((n + 1) * (n + 1) - m * m) / (2 * n + 3));
+    }
+  }
+  for (n = 0; n <= L; n++) {
+    for (nu = 0; nu <= n; nu++) {
+      qreal v = P[PIDX(n, nu)] / sqrtq((qreal)(2 * n + 1)) * sqrtq(4 * M_PIq);
+      h[RIDX(nu, 0, n)] = v;
+      h[RIDX(-nu, 0, n)] = v;
+    }
+  }
+  for (m = 0; m <= N; m++) {
+    for (n = m + 2; n <= L - m; n++) {
+      qreal b = rec_b(n, m);
+      for (nu = -n + 1; nu <= n - 1; nu++) {
+        qreal b1 = rec_b(n, -nu - 1), b2 = rec_b(n, nu - 1);
+        qreal a3 = rec_a(n - 1, nu);
+        h[RIDX(nu, m + 1, n - 1)] =
+            ((b1 * (1 - C) * h[RIDX(nu + 1, m, n)] -
+              b2 * (1 + C) * h[RIDX(nu - 1, m, n)]) / 2 -
+             a3 * S * h[RIDX(nu, m, n)]) / b;
+      }
+    }
+  }
+  for (i = 0; i < orc_rotation_size(N); i++) H[i] = (double)h[i];
+  free(h);
+  free(P);
+}
+
+/*
+ * Rotation angles for a translation along d = target - source, expressed
+ * in box widths from the source-minus-target offset (dx,dy,dz).  The frame
+ * is that of src/util.c:224-262 (z' along d, x' = (d_y,-d_x,0)/|.|) and the
+ * angles follow src/rotations.c:99-128; for offsets off the z axis
+ * phi = -pi/2 and chi = atan2(d_y,d_x), which is what the reference's
+ * tabulated angles hold (src/shifts.c:23-138, src/shift.c:628-724).
+ */
+void orc_shift_angles(int dx, int dy, int dz, double *th, double *ph,
+                      double *ch, double *r)
+{
+  double ex = -dx, ey = -dy, ez = -dz;
+  double rr = sqrt(ex * ex + ey * ey + ez * ez);
+  *r = rr;
+  *th = acos(ez / rr);
+  if (dx == 0 && dy == 0) {
+    *ph = 0.0;
+    *ch = 0.0;
+  } else {
+    *ph = -0.5 * M_PI;
+    *ch = atan2(ey, ex);
+  }
+}
+
+/* ======================================================================
+ * Laplace operators
+ * ====================================================================== */
+
+static void sph(const double *x0, const double *x, double *r, double *C,
+                double *S, double *cp, double *sp)
+{
+  double dx = x[0] - x0[0], dy = x[1] - x0[1], dz = x[2] - x0[2];
+  double rr = sqrt(dx * dx + dy * dy + dz * dz), rho;
+  *r = rr;
+  if (rr == 0.0) {
+    *C = 1.0; *S = 0.0; *cp = 1.0; *sp = 0.0;
+    return;
+  }
+  *C = dz / rr;
+  rho = sqrt(dx * dx + dy * dy);
+  *S = rho / rr;
+  if (rho == 0.0) { *cp = 1.0; *sp = 0.0; }
+  else { *cp = dx / rho; *sp = dy / rho; }
+}
+
+/*
+ * P2M, monopole: M_n^m += q r^n Pbar_n^m e^{-i m phi} / (2n+1)
+ * (src/laplace.c:35-129).  M dense, real-packed.
+ */
+void orc_lap_p2m(int N, const double *x0, const double *xs, double q,
+                 double *M)
+{
+  double r, C, S, cp, sp, rn = 1.0, cm, sm, t;
+  double P[(64 + 1) * (64 + 2) / 2];
+  int n, m;
+  sph(x0, xs, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  for (n = 0; n <= N; n++) {
+    M[n * n] += q * rn * P[PIDX(n, 0)] / (2 * n + 1);
+    cm = 1.0; sm = 0.0;
+    for (m = 1; m <= n; m++) {
+      t = cm * cp - sm * sp; sm = sm * cp + cm * sp; cm = t;
+      M[LIDX(n, m) + 0] += q * rn * P[PIDX(n, m)] * cm / (2 * n + 1);
+      M[LIDX(n, m) + 1] -= q * rn * P[PIDX(n, m)] * sm / (2 * n + 1);
+    }
+    rn *= r;
+  }
+}
+
+/*
+ * P2M, normal dipole of strength d and unit normal nrm: the expansion of
+ * d n.grad_y [1/(4 pi |x-y|)], i.e. the directional derivative of the
+ * monopole expansion with respect to the source position (the reference
+ * builds the same coefficients by its own recurrences,
+ * src/laplace.c:188-423).  Computed here by differentiating the regular
+ * solid harmonics analytically.
+ */
+void orc_lap_p2m_normal(int N, const double *x0, const double *xs,
+                        const double *nrm, double d, double *M)
+{
+  /* conj(R_n^m(y)) derivative along nrm, using the identities
+     dz R_n^m = a R_{n-1}^m, (dx + i dy) R_n^m = -b1 R_{n-1}^{m+1},
+     (dx - i dy) R_n^m = b2 R_{n-1}^{m-1} (see orc_lap_l_eval). */
+  double r, C, S, cp, sp, rn, cm, sm, t;
+  double P[(64 + 1) * (64 + 2) / 2];
+  double Rre[(64 + 1) * (64 + 2) / 2], Rim[(64 + 1) * (64 + 2) / 2];
+  int n, m;
+  sph(x0, xs, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  rn = 1.0;
+  for (n = 0; n <= N; n++) {
+    cm = 1.0; sm = 0.0;
+    for (m = 0; m <= n; m++) {
+      if (m > 0) { t = cm * cp - sm * sp; sm = sm * cp + cm * sp; cm = t; }
+      Rre[PIDX(n, m)] = rn * P[PIDX(n, m)] * cm;
+      Rim[PIDX(n, m)] = rn * P[PIDX(n, m)] * sm;
+    }
+    rn *= r;
+  }
+  for (n = 1; n <= N; n++) {
+    double f = sqrt((2.0 * n + 1) / (2.0 * n - 1));
+    for (m = 0; m <= n; m++) {
+      /* derivative of R_n^m(y) = A(y) + iB(y) */
+      double gzr = 0, gzi = 0, dpr = 0, dpi = 0, dmr = 0, dmi = 0;
+      double a = f * sqrt((double)(n - m) * (n + m));
+      double b1 = f * sqrt((double)(n - m) * (n - m - 1));
+      double b2 = f * sqrt((double)(n + m) * (n + m - 1));
+      double gxr, gxi, gyr, gyi, dr, di;
+      if (m <= n - 1) { gzr = a * Rre[PIDX(n - 1, m)]; gzi = a * Rim[PIDX(n - 1, m)]; }
+      if (m + 1 <= n - 1) { dpr = -b1 * Rre[PIDX(n - 1, m + 1)]; dpi = -b1 * Rim[PIDX(n - 1, m + 1)]; }
+      if (m >= 1) {
+        dmr = b2 * Rre[PIDX(n - 1, m - 1)]; dmi = b2 * Rim[PIDX(n - 1, m - 1)];
+      } else {
+        /* R_{n-1}^{-1} = -conj(R_{n-1}^1) scaled: use D- R_n^0 = conj(D+ R_n^0) */
+        dmr = dpr; dmi = -dpi;
+      }
+      /* dx = (D+ + D-)/2, dy = (D+ - D-)/(2i) */
+      gxr = 0.5 * (dpr + dmr); gxi = 0.5 * (dpi + dmi);
+      gyr = 0.5 * (dpi - dmi); gyi = -0.5 * (dpr - dmr);
+      dr = nrm[0] * gxr + nrm[1] * gyr + nrm[2] * gzr;
+      di = nrm[0] * gxi + nrm[1] * gyi + nrm[2] * gzi;
+      /* M_n^m += d conj(R_n^m) / (2n+1) */
+      if (m == 0)
+        M[n * n] += d * dr / (2 * n + 1);
+      else {
+        M[LIDX(n, m) + 0] += d * dr / (2 * n + 1);
+        M[LIDX(n, m) + 1] -= d * di / (2 * n + 1);
+      }
+    }
+  }
+}
+
+/*
+ * singular-expansion evaluation, src/laplace.c:426-494:
+ * f = sum r^{-n-1} [M_n^0 Pbar_n^0 + 2 sum_m Pbar_n^m Re(M_n^m e^{i m phi})]
+ */
+double orc_lap_m_eval(int N, const double *x0, const double *M,
+                      const double *xf)
+{
+  double r, C, S, cp, sp, rn, cm, sm, t, f = 0.0;
+  double P[(64 + 1) * (64 + 2) / 2];
+  int n, m;
+  sph(x0, xf, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  rn = 1.0 / r;
+  for (n = 0; n <= N; n++) {
+    f += M[n * n] * rn * P[PIDX(n, 0)];
+    cm = 1.0; sm = 0.0;
+    for (m = 1; m <= n; m++) {
+      t = cm * cp - sm * sp; sm = sm * cp + cm * sp; cm = t;
+      f += 2.0 * P[PIDX(n, m)] * rn *
+           (M[LIDX(n, m)] * cm - M[LIDX(n, m) + 1] * sm);
+    }
+    rn /= r;
+  }
+  return f;
+}
+
+/*
+ * local-expansion evaluation (L2P).  field&1: potential into f[0]
+ * (src/local-laplace.c:333-398); field&2: gradient into f[1..3]
+ * (src/local-laplace.c:525-630 with the derivative identities of
+ * src/wbfmm-derivatives.h:255-314):
+ *   dz R_n^m       =  a_n^m  R_{n-1}^m,     a  = sqrt((2n+1)/(2n-1)(n-m)(n+m))
+ *   (dx+i dy) R_n^m = -b1_n^m R_{n-1}^{m+1}, b1 = sqrt((2n+1)/(2n-1)(n-m)(n-m-1))
+ *   (dx-i dy) R_n^m =  b2_n^m R_{n-1}^{m-1}, b2 = sqrt((2n+1)/(2n-1)(n+m)(n+m-1))
+ * f is accumulated into.
+ */
+void orc_lap_l_eval(int N, const double *x0, const double *L,
+                    const double *xf, int field, double *f)
+{
+  double r, C, S, cp, sp, rn, cm, sm, t;
+  double P[(64 + 1) * (64 + 2) / 2];
+  double Rre[(64 + 1) * (64 + 2) / 2], Rim[(64 + 1) * (64 + 2) / 2];
+  int n, m;
+  sph(x0, xf, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  rn = 1.0;
+  for (n = 0; n <= N; n++) {
+    cm = 1.0; sm = 0.0;
+    for (m = 0; m <= n; m++) {
+      if (m > 0) { t = cm * cp - sm * sp; sm = sm * cp + cm * sp; cm = t; }
+      Rre[PIDX(n, m)] = rn * P[PIDX(n, m)] * cm;
+      Rim[PIDX(n, m)] = rn * P[PIDX(n, m)] * sm;
+    }
+    rn *= r;
+  }
+  if (field & 1) {
+    double phi = 0.0;
+    for (n = 0; n <= N; n++) {
+      phi += L[n * n] * Rre[PIDX(n, 0)];
+      for (m = 1; m <= n; m++)
+        phi += 2.0 * (L[LIDX(n, m)] * Rre[PIDX(n, m)] -
+                      L[LIDX(n, m) + 1] * Rim[PIDX(n, m)]);
+    }
+    f[0] += phi;
+  }
+  if (field & 2) {
+    double gx = 0, gy = 0, gz = 0;
+    for (n = 1; n <= N; n++) {
+      double fc = sqrt((2.0 * n + 1) / (2.0 * n - 1));
+      for (m = 0; m <= n; m++) {
+        double lr, li, a, b1, b2, dpr = 0, dpi = 0;
+        if (m == 0) { lr = L[n * n]; li = 0.0; }
+        else { lr = L[LIDX(n, m)]; li = L[LIDX(n, m) + 1]; }
+        a = fc * sqrt((double)(n - m) * (n + m));
+        b1 = fc * sqrt((double)(n - m) * (n - m - 1));
+        b2 = fc * sqrt((double)(n + m) * (n + m - 1));
+        /* d/dz of L R_n^m (+ conj for m>0) */
+        if (m <= n - 1) {
+          double v = a * (lr * Rre[PIDX(n - 1, m)] - li * Rim[PIDX(n - 1, m)]);
+          gz += (m == 0) ? v : 2.0 * v;
+        }
+        /* D+ of L R_n^m: -b1 L R_{n-1}^{m+1} */
+        if (m + 1 <= n - 1) {
+          double rr = Rre[PIDX(n - 1, m + 1)], ri = Rim[PIDX(n - 1, m + 1)];
+          dpr += -b1 * (lr * rr - li * ri);
+          dpi += -b1 * (lr * ri + li * rr);
+        }
+        /* D+ of conj(L) conj(R_n^m), m>=1: b2 conj(L) conj(R_{n-1}^{m-1}) */
+        if (m >= 1) {
+          double rr = Rre[PIDX(n - 1, m - 1)], ri = Rim[PIDX(n - 1, m - 1)];
+          dpr += b2 * (lr * rr - li * ri);
+          dpi += -b2 * (lr * ri + li * rr);
+        }
+        gx += dpr;
+        gy += dpi;
+      }
+    }
+    f[1] += gx;
+    f[2] += gy;
+    f[3] += gz;
+  }
+}
+
+/*
+ * closed-form Laplace coaxial coefficients (src/laplace.c:542-569,
+ * :630-649, :698-725); kind 0 = SS, 1 = RR, 2 = SR.  The powers of the
+ * translation distance are applied by orc_lap_coaxial.
+ */
+double orc_lap_coaxial_cfft(int kind, int n, int nu, int m)
+{
+  double c;
+  if (kind == 0) {       /* SS, n >= nu */
+    c = lgamma(n - m + 1) + lgamma(n + m + 1) - lgamma(nu - m + 1) -
+        lgamma(nu + m + 1) - 2.0 * lgamma(n - nu + 1);
+    c += log((2.0 * nu + 1) / (2.0 * n + 1));
+    return exp(0.5 * c);
+  }
+  if (kind == 1) {       /* RR, nu >= n */
+    c = lgamma(nu - m + 1) + lgamma(nu + m + 1) - lgamma(n - m + 1) -
+        lgamma(n + m + 1) - 2.0 * lgamma(nu - n + 1);
+    c += log((2.0 * nu + 1) / (2.0 * n + 1));
+    return exp(0.5 * c);
+  }
+  c = 2.0 * lgamma(n + nu + 1) - lgamma(n - m + 1) - lgamma(n + m + 1) -
+      lgamma(nu - m + 1) - lgamma(nu + m + 1);
+  c += log((2.0 * nu + 1) / (2.0 * n + 1));
+  c = exp(0.5 * c);
+  return ((n + m) % 2) ? -c : c;
+}
+
+/* coefficient cache, the counterpart of the reference's global tables
+   filled by wbfmm_laplace_coaxial_translate_init (src/laplace.c:727-792);
+   cf[kind][(m*(LC+1)+n)*(LC+1)+nu] */
+#define LC 64
+static double *lap_cf[3];
+static int lap_cf_N = -1;
+
+static void lap_cf_init(int N)
+{
+  int kind, n, nu, m;
+  if (N <= lap_cf_N) return;
+  if (N > LC) N = LC;
+  for (kind = 0; kind < 3; kind++) {
+    free(lap_cf[kind]);
+    lap_cf[kind] = calloc((size_t)(LC + 1) * (LC + 1) * (LC + 1), sizeof(double));
+    for (m = 0; m <= N; m++)
+      for (n = m; n <= N; n++)
+        for (nu = m; nu <= N; nu++) {
+          if (kind == 0 && nu > n) continue;
+          if (kind == 1 && nu < n) continue;
+          lap_cf[kind][((size_t)m * (LC + 1) + n) * (LC + 1) + nu] =
+              orc_lap_coaxial_cfft(kind, n, nu, m);
+        }
+  }
+  lap_cf_N = N;
+}
+
+/*
+ * coaxial translation along z by t (src/laplace.c:496-696):
+ *   SS: Co_n^m = sc Co + sum_{nu<=n} c (-t)^{n-nu} Ci_nu^m
+ *   RR: Co_n^m = sc Co + sum_{nu>=n} c t^{nu-n} Ci_nu^m
+ *   SR: Co_n^m = sc Co + sum_{nu} c / t^{n+nu+1} Ci_nu^m   (t signed)
+ * Co and Ci dense real-packed.
+ */
+void orc_lap_coaxial(int kind, double *Co, int No, const double *Ci, int Ni,
+                     double t, double sc)
+{
+  int n, m, nu, c;
+  double tp[160];
+  int nc = (No + 1) * (No + 1);
+  double out[(LC + 1) * (LC + 1)];
+  lap_cf_init(No > Ni ? No : Ni);
+  memset(out, 0, sizeof(double) * nc);
+  tp[0] = 1.0;
+  for (n = 1; n < 160; n++) tp[n] = tp[n - 1] * t;
+  for (m = 0; m <= No; m++) {
+    for (n = m; n <= No; n++) {
+      int lo, hi;
+      const double *cf = lap_cf[kind] + ((size_t)m * (LC + 1) + n) * (LC + 1);
+      double sr = 0.0, si = 0.0;
+      if (kind == 0) { lo = m; hi = (n < Ni) ? n : Ni; }
+      else if (kind == 1) { lo = (n > m) ? n : m; hi = Ni; }
+      else { lo = m; hi = Ni; }
+      for (nu = lo; nu <= hi; nu++) {
+        double c1 = cf[nu];
+        if (kind == 0) c1 *= ((n - nu) % 2 ? -1.0 : 1.0) * tp[n - nu];
+        else if (kind == 1) c1 *= tp[nu - n];
+        else c1 /= tp[n + nu + 1];
+        if (m == 0)
+          sr += c1 * Ci[nu * nu];
+        else {
+          sr += c1 * Ci[LIDX(nu, m) + 0];
+          si += c1 * Ci[LIDX(nu, m) + 1];
+        }
+      }
+      if (m == 0) out[n * n] = sr;
+      else { out[LIDX(n, m)] = sr; out[LIDX(n, m) + 1] = si; }
+    }
+  }
+  /* sc == 0 assigns (the output buffer may hold garbage) */
+  for (c = 0; c < nc; c++) Co[c] = (sc == 0.0) ? out[c] : sc * Co[c] + out[c];
+}
+
+/*
+ * rotation of a real-packed expansion, src/rotations-laplace.c:36-118:
+ *   Co_n^nu = sc Co + e^{-i nu ph} sum_{m=-n}^{n} H_n^{nu,m} e^{i m ch} Ci_n^m
+ * with Ci_n^{-m} = conj(Ci_n^m) and H_n^{nu,-m} = H_n^{-nu,m}.
+ */
+void orc_lap_rotate(double *Co, const double *Ci, int N, const double *H,
+                    double ph, double ch, double sc)
+{
+  int n, m, nu;
+  double cc[LC + 2], sc_[LC + 2], cp[LC + 2], sp[LC + 2];
+  cc[0] = 1.0; sc_[0] = 0.0; cp[0] = 1.0; sp[0] = 0.0;
+  for (m = 1; m <= N; m++) {
+    cc[m] = cos(m * ch); sc_[m] = sin(m * ch);
+    cp[m] = cos(m * ph); sp[m] = sin(m * ph);
+  }
+  for (n = 0; n <= N; n++) {
+    double zr[LC + 1], zi[LC + 1];
+    for (m = 1; m <= n; m++) {
+      double cr = Ci[LIDX(n, m)], ci = Ci[LIDX(n, m) + 1];
+      zr[m] = cr * cc[m] - ci * sc_[m];
+      zi[m] = cr * sc_[m] + ci * cc[m];
+    }
+    for (nu = 0; nu <= n; nu++) {
+      const double *hp = H + RIDX(nu, 0, n), *hm = H + RIDX(-nu, 0, n);
+      double tr = hp[0] * Ci[n * n], ti = 0.0;
+      for (m = 1; m <= n; m++) {
+        tr += (hp[m] + hm[m]) * zr[m];
+        ti += (hp[m] - hm[m]) * zi[m];
+      }
+      if (nu == 0) {
+        Co[n * n] = (sc == 0.0) ? tr : sc * Co[n * n] + tr;
+      } else {
+        double vr = cp[nu] * tr + sp[nu] * ti, vi = cp[nu] * ti - sp[nu] * tr;
+        if (sc == 0.0) {
+          Co[LIDX(n, nu) + 0] = vr;
+          Co[LIDX(n, nu) + 1] = vi;
+        } else {
+          Co[LIDX(n, nu) + 0] = sc * Co[LIDX(n, nu) + 0] + vr;
+          Co[LIDX(n, nu) + 1] = sc * Co[LIDX(n, nu) + 1] + vi;
+        }
+      }
+    }
+  }
+}
+
+/* ======================================================================
+ * FP64 direct sums
+ * ====================================================================== */
+
+/*
+ * Laplace direct sum (src/direct-laplace.c:200-336).  field&1 -> f[4i],
+ * field&2 -> f[4i+1..3].  Sources: monopoles q (may be NULL) and normal
+ * dipoles d with unit normals nrm (may be NULL).  Pairs with r <= cutoff
+ * are skipped (cutoff 0: none skipped, as in the reference direct sum).
+ */
+void orc_lap_direct(int nsrc, const double *xs, const double *q,
+                    const double *nrm, const double *d,
+                    int ntgt, const double *xt, int field, double cutoff,
+                    double *f)
+{
+  int i, j;
+  const double c4 = 0.25 / M_PI;
+  for (i = 0; i < ntgt; i++) {
+    double phi = 0, gx = 0, gy = 0, gz = 0;
+    for (j = 0; j < nsrc; j++) {
+      double rx = xt[3 * i] - xs[3 * j], ry = xt[3 * i + 1] - xs[3 * j + 1];
+      double rz = xt[3 * i + 2] - xs[3 * j + 2];
+      double r2 = rx * rx + ry * ry + rz * rz, r, ir, ir3;
+      if (r2 <= cutoff * cutoff) continue;
+      r = sqrt(r2); ir = 1.0 / r; ir3 = ir * ir * ir;
+      if (q) {
+        phi += q[j] * ir;
+        gx -= q[j] * rx * ir3; gy -= q[j] * ry * ir3; gz -= q[j] * rz * ir3;
+      }
+      if (d) {
+        double nr = nrm[3 * j] * rx + nrm[3 * j + 1] * ry + nrm[3 * j + 2] * rz;
+        double ir5 = ir3 * ir * ir;
+        phi += d[j] * nr * ir3;
+        gx += d[j] * (nrm[3 * j] * ir3 - 3.0 * nr * rx * ir5);
+        gy += d[j] * (nrm[3 * j + 1] * ir3 - 3.0 * nr * ry * ir5);
+        gz += d[j] * (nrm[3 * j + 2] * ir3 - 3.0 * nr * rz * ir5);
+      }
+    }
+    if (field & 1) f[4 * i] += phi * c4;
+    if (field & 2) {
+      f[4 * i + 1] += gx * c4; f[4 * i + 2] += gy * c4; f[4 * i + 3] += gz * c4;
+    }
+  }
+}
+
+/* ======================================================================
+ * octree
+ * ====================================================================== */
+
+typedef struct {
+  int depth, nsrc;
+  double x0[3], D;
+  uint32_t *ip;          /* Morton-sorted source indices */
+  uint64_t *code;        /* Morton codes of sorted sources */
+  int *bstart[17], *bcount[17];  /* dense per level */
+} orc_tree;
+
+typedef struct { uint64_t c; uint32_t i; } code_idx;
+
+static int cmp_code(const void *a, const void *b)
+{
+  const code_idx *x = a, *y = b;
+  if (x->c < y->c) return -1;
+  if (x->c > y->c) return 1;
+  /* stable: ties keep input order (GLib qsort is stable, src/boxes.c:158) */
+  return (x->i < y->i) ? -1 : (x->i > y->i);
+}
+
+/* tree build: src/boxes.c:118-277 (sort by Morton code, box ranges) */
+static int tree_build(orc_tree *t, int nsrc, const double *xs)
+{
+  int i, l;
+  code_idx *ci = malloc(sizeof(code_idx) * (nsrc > 0 ? nsrc : 1));
+  for (i = 0; i < nsrc; i++) {
+    const double *x = xs + 3 * i;
+    if (x[0] <= t->x0[0] || x[0] >= t->x0[0] + t->D ||
+        x[1] <= t->x0[1] || x[1] >= t->x0[1] + t->D ||
+        x[2] <= t->x0[2] || x[2] >= t->x0[2] + t->D) {
+      free(ci);
+      return -1;
+    }
+    ci[i].c = orc_point_index(x, t->x0, t->D);
+    ci[i].i = i;
+  }
+  qsort(ci, nsrc, sizeof(code_idx), cmp_code);
+  t->nsrc = nsrc;
+  t->ip = malloc(sizeof(uint32_t) * (nsrc + 1));
+  t->code = malloc(sizeof(uint64_t) * (nsrc + 1));
+  for (i = 0; i < nsrc; i++) { t->ip[i] = ci[i].i; t->code[i] = ci[i].c; }
+  free(ci);
+  for (l = 0; l <= t->depth; l++) {
+    int nb = 1 << (3 * l);
+    t->bstart[l] = calloc(nb, sizeof(int));
+    t->bcount[l] = calloc(nb, sizeof(int));
+    for (i = 0; i < nsrc; i++) {
+      uint64_t b = t->code[i] >> (3 * (20 - l));
+      if (t->bcount[l][b] == 0) t->bstart[l][b] = i;
+      t->bcount[l][b]++;
+    }
+  }
+  return 0;
+}
+
+static void tree_free(orc_tree *t)
+{
+  int l;
+  free(t->ip); free(t->code);
+  for (l = 0; l <= t->depth; l++) { free(t->bstart[l]); free(t->bcount[l]); }
+}
+
+static void box_centre(const orc_tree *t, int level, uint64_t b, double *xc)
+{
+  uint32_t i, j, k;
+  double w = t->D / (1 << level);
+  orc_morton_decode(b, &i, &j, &k);
+  xc[0] = t->x0[0] + w * (i + 0.5);
+  xc[1] = t->x0[1] + w * (j + 0.5);
+  xc[2] = t->x0[2] + w * (k + 0.5);
+}
+
+/* ======================================================================
+ * Laplace solve: src/laplace.c:854-989 (P2M), src/pass-laplace.c:433-486
+ * with src/shift-laplace.c:225-442 (M2M), src/pass-laplace.c:35-408 (M2L)
+ * with src/shift-laplace.c:562-789 (L2L), src/local-laplace.c:1293-1336
+ * (L2P + near field); call order of tools/wbfmm-laplace-tree-calc.c.in
+ * :380-460.
+ * ====================================================================== */
+
+#include <pthread.h>
+
+typedef struct {
+  const orc_tree *t;
+  const orc_fmm_cfg *cfg;
+  int level, nth, ith;
+  double **mps, **mpr;
+  double *Htab;   /* per (dx,dy,dz) offset rotation tables */
+  int nrot;
+} m2l_job;
+
+static void *lap_m2l_worker(void *arg)
+{
+  m2l_job *J = arg;
+  const orc_tree *t = J->t;
+  int l = J->level, Ns = J->cfg->order_s[l], Nr = J->cfg->order_r[l];
+  int ncs = (Ns + 1) * (Ns + 1), ncr = (Nr + 1) * (Nr + 1);
+  uint64_t nb = 1ull << (3 * l), b;
+  double w = t->D / (1 << l);
+  double *wks = malloc(sizeof(double) * ncs), *wkr = malloc(sizeof(double) * ncr);
+  uint64_t list[216];
+  int off[3 * 216];
+  for (b = J->ith; b < nb; b += J->nth) {
+    int n = orc_box_list4(l, b, list, off), j;
+    double *L = J->mpr[l] + b * ncr;
+    for (j = 0; j < n; j++) {
+      double th, ph, ch, r;
+      const double *H;
+      if (t->bcount[l][list[j]] == 0) continue;  /* zero expansion */
+      orc_shift_angles(off[3 * j], off[3 * j + 1], off[3 * j + 2], &th, &ph, &ch, &r);
+      H = J->Htab + (size_t)((off[3 * j] + 3) * 49 + (off[3 * j + 1] + 3) * 7 +
+                             off[3 * j + 2] + 3) * J->nrot;
+      orc_lap_rotate(wks, J->mps[l] + list[j] * ncs, Ns, H, ph, ch, 0.0);
+      orc_lap_coaxial(2, wkr, Nr, wks, Ns, r * w, 0.0);
+      orc_lap_rotate(L, wkr, Nr, H, ch, ph, 1.0);
+    }
+  }
+  free(wks); free(wkr);
+  return NULL;
+}
+
+static void run_m2l(orc_tree *t, orc_fmm_cfg *cfg, int level, double **mps,
+                    double **mpr, double *Htab, int nrot,
+                    void *(*worker)(void *))
+{
+  int nth = cfg->nthreads > 1 ? cfg->nthreads : 1, i;
+  pthread_t th[64];
+  m2l_job jobs[64];
+  if (nth > 64) nth = 64;
+  for (i = 0; i < nth; i++) {
+    jobs[i].t = t; jobs[i].cfg = cfg; jobs[i].level = level;
+    jobs[i].nth = nth; jobs[i].ith = i; jobs[i].mps = mps; jobs[i].mpr = mpr;
+    jobs[i].Htab = Htab; jobs[i].nrot = nrot;
+  }
+  if (nth == 1) { worker(&jobs[0]); return; }
+  for (i = 0; i < nth; i++) pthread_create(&th[i], NULL, worker, &jobs[i]);
+  for (i = 0; i < nth; i++) pthread_join(th[i], NULL);
+}
+
+/* rotation tables for all 343 offsets up to order N (entries for
+   offsets in the near field are left unused) */
+static double *offset_rotations(int N, int *nrot)
+{
+  int dx, dy, dz;
+  double *Htab;
+  *nrot = orc_rotation_size(N);
+  Htab = calloc((size_t)343 * (*nrot), sizeof(double));
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        double th, ph, ch, r;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        orc_shift_angles(dx, dy, dz, &th, &ph, &ch, &r);
+        orc_H_rotation(Htab + (size_t)((dx + 3) * 49 + (dy + 3) * 7 + dz + 3) * (*nrot),
+                       N, th);
+      }
+  return Htab;
+}
+
+/* child/parent direction angles: child Morton digit c, direction from
+   child centre to parent centre (sgn=+1) or reverse (sgn=-1) */
+static void child_angles(int c, int sgn, double *th, double *ph, double *ch)
+{
+  double ex = sgn * (0.5 - (c & 1)), ey = sgn * (0.5 - ((c >> 1) & 1));
+  double ez = sgn * (0.5 - ((c >> 2) & 1));
+  double r = sqrt(ex * ex + ey * ey + ez * ez);
+  *th = acos(ez / r);
+  *ph = -0.5 * M_PI;
+  *ch = atan2(ey, ex);
+}
+
+static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
+                     const double *q, const double *nrm, const double *d,
+                     int ntgt, const double *xt, double *f)
+{
+  int depth = t->depth, l, Lmax = 0, nrot, i;
+  double *mps[17] = {0}, *mpr[17] = {0}, *Htab, tm;
+  double *Hud[2][8];
+  for (l = 2; l <= depth; l++) {
+    if (cfg->order_s[l] > Lmax) Lmax = cfg->order_s[l];
+    if (cfg->order_r[l] > Lmax) Lmax = cfg->order_r[l];
+  }
+  Htab = offset_rotations(Lmax, &nrot);
+  for (i = 0; i < 8; i++) {
+    double th, ph, ch;
+    Hud[0][i] = malloc(sizeof(double) * nrot);
+    Hud[1][i] = malloc(sizeof(double) * nrot);
+    child_angles(i, 1, &th, &ph, &ch); orc_H_rotation(Hud[0][i], Lmax, th);
+    child_angles(i, -1, &th, &ph, &ch); orc_H_rotation(Hud[1][i], Lmax, th);
+  }
+  for (l = 2; l <= depth; l++) {
+    size_t nb = (size_t)1 << (3 * l);
+    mps[l] = calloc(nb * (cfg->order_s[l] + 1) * (cfg->order_s[l] + 1), sizeof(double));
+    mpr[l] = calloc(nb * (cfg->order_r[l] + 1) * (cfg->order_r[l] + 1), sizeof(double));
+  }
+  /* P2M at the leaves */
+  tm = now_s();
+  {
+    int Ns = cfg->order_s[depth], ncs = (Ns + 1) * (Ns + 1);
+    uint64_t nb = 1ull << (3 * depth), b;
+    for (b = 0; b < nb; b++) {
+      double xc[3];
+      int j;
+      if (t->bcount[depth][b] == 0) continue;
+      box_centre(t, depth, b, xc);
+      for (j = 0; j < t->bcount[depth][b]; j++) {
+        int s = t->ip[t->bstart[depth][b] + j];
+        if (d) orc_lap_p2m_normal(Ns, xc, xs + 3 * s, nrm + 3 * s, d[s], mps[depth] + b * ncs);
+        if (q) orc_lap_p2m(Ns, xc, xs + 3 * s, q[s], mps[depth] + b * ncs);
+      }
+    }
+  }
+  cfg->t_phase[1] = now_s() - tm;
+  /* upward pass, levels depth..3 */
+  tm = now_s();
+  for (l = depth; l >= 3; l--) {
+    int Nc = cfg->order_s[l], Np = cfg->order_s[l - 1];
+    int ncc = (Nc + 1) * (Nc + 1), ncp = (Np + 1) * (Np + 1);
+    double wc = t->D / (1 << l), tt = sqrt(3.0) * 0.5 * wc;
+    uint64_t np = 1ull << (3 * (l - 1)), p;
+    double *wk1 = malloc(sizeof(double) * (ncc > ncp ? ncc : ncp));
+    double *wk2 = malloc(sizeof(double) * ncp);
+    for (p = 0; p < np; p++) {
+      int c;
+      double *Cp = mps[l - 1] + p * ncp;
+      memset(Cp, 0, sizeof(double) * ncp);
+      if (t->bcount[l - 1][p] == 0) continue;
+      for (c = 0; c < 8; c++) {
+        double th, ph, ch;
+        if (t->bcount[l][8 * p + c] == 0) continue;
+        child_angles(c, 1, &th, &ph, &ch);
+        orc_lap_rotate(wk1, mps[l] + (8 * p + c) * ncc, Nc, Hud[0][c], ph, ch, 0.0);
+        orc_lap_coaxial(0, wk2, Np, wk1, Nc, tt, 0.0);
+        orc_lap_rotate(Cp, wk2, Np, Hud[0][c], ch, ph, 1.0);
+      }
+    }
+    free(wk1); free(wk2);
+  }
+  cfg->t_phase[2] = now_s() - tm;
+  /* downward pass, levels 2..depth */
+  tm = now_s();
+  for (l = 2; l <= depth; l++) {
+    run_m2l(t, cfg, l, mps, mpr, Htab, nrot, lap_m2l_worker);
+    if (l < depth) {
+      int Np = cfg->order_r[l], Nc = cfg->order_r[l + 1];
+      int ncp = (Np + 1) * (Np + 1), ncc = (Nc + 1) * (Nc + 1);
+      double wc = t->D / (1 << (l + 1)), tt = sqrt(3.0) * 0.5 * wc;
+      uint64_t np = 1ull << (3 * l), p;
+      double *wk1 = malloc(sizeof(double) * ncp), *wk2 = malloc(sizeof(double) * ncc);
+      for (p = 0; p < np; p++) {
+        int c;
+        for (c = 0; c < 8; c++) {
+          double th, ph, ch;
+          child_angles(c, -1, &th, &ph, &ch);
+          orc_lap_rotate(wk1, mpr[l] + p * ncp, Np, Hud[1][c], ph, ch, 0.0);
+          orc_lap_coaxial(1, wk2, Nc, wk1, Np, tt, 0.0);
+          orc_lap_rotate(mpr[l + 1] + (8 * p + c) * ncc, wk2, Nc, Hud[1][c], ch, ph, 1.0);
+        }
+      }
+      free(wk1); free(wk2);
+    }
+  }
+  cfg->t_phase[3] = now_s() - tm;
+  /* L2P + near field */
+  tm = now_s();
+  {
+    int Nr = cfg->order_r[depth], ncr = (Nr + 1) * (Nr + 1);
+    for (i = 0; i < ntgt; i++) {
+      const double *x = xt + 3 * i;
+      uint64_t b = orc_point_index(x, t->x0, t->D) >> (3 * (20 - depth));
+      uint64_t nb[27];
+      double xc[3], loc[4] = {0, 0, 0, 0};
+      int nn, a, j;
+      box_centre(t, depth, b, xc);
+      orc_lap_l_eval(Nr, xc, mpr[depth] + b * ncr, x, cfg->field, loc);
+      nn = orc_box_neighbours(depth, b, nb);
+      for (a = 0; a < nn; a++) {
+        for (j = 0; j < t->bcount[depth][nb[a]]; j++) {
+          int s = t->ip[t->bstart[depth][nb[a]] + j];
+          double rx = x[0] - xs[3 * s], ry = x[1] - xs[3 * s + 1], rz = x[2] - xs[3 * s + 2];
+          double r2 = rx * rx + ry * ry + rz * rz, r, g;
+          if (r2 <= CUTOFF * CUTOFF) continue;
+          r = sqrt(r2);
+          g = 0.25 / M_PI / r;
+          if (q) {
+            loc[0] += q[s] * g;
+            loc[1] -= q[s] * g * rx / r2; loc[2] -= q[s] * g * ry / r2;
+            loc[3] -= q[s] * g * rz / r2;
+          }
+          if (d) {
+            const double *nv = nrm + 3 * s;
+            double nr = nv[0] * rx + nv[1] * ry + nv[2] * rz;
+            loc[0] += d[s] * nr * g / r2;
+            loc[1] += d[s] * g / r2 * (nv[0] - 3.0 * nr * rx / r2);
+            loc[2] += d[s] * g / r2 * (nv[1] - 3.0 * nr * ry / r2);
+            loc[3] += d[s] * g / r2 * (nv[2] - 3.0 * nr * rz / r2);
+          }
+        }
+      }
+      if (cfg->field & 1) f[4 * i] += loc[0];
+      if (cfg->field & 2) { f[4 * i + 1] += loc[1]; f[4 * i + 2] += loc[2]; f[4 * i + 3] += loc[3]; }
+    }
+  }
+  cfg->t_phase[4] = now_s() - tm;
+  if (cfg->dump_level >= 2 && cfg->dump_level <= depth) {
+    int l2 = cfg->dump_level;
+    size_t nb = (size_t)1 << (3 * l2);
+    if (cfg->dump_mps)
+      memcpy(cfg->dump_mps, mps[l2], sizeof(double) * nb * (cfg->order_s[l2] + 1) * (cfg->order_s[l2] + 1));
+    if (cfg->dump_mpr)
+      memcpy(cfg->dump_mpr, mpr[l2], sizeof(double) * nb * (cfg->order_r[l2] + 1) * (cfg->order_r[l2] + 1));
+  }
+  for (l = 2; l <= depth; l++) { free(mps[l]); free(mpr[l]); }
+  for (i = 0; i < 8; i++) { free(Hud[0][i]); free(Hud[1][i]); }
+  free(Htab);
+  return 0;
+}
+
+int orc_fmm(orc_fmm_cfg *cfg, int nsrc, const double *xs, const double *q,
+            const double *nrm, const double *d, int ntgt, const double *xt,
+            double *f)
+{
+  orc_tree t;
+  int rc;
+  double t0 = now_s(), t1;
+  memset(&t, 0, sizeof(t));
+  t.depth = cfg->depth;
+  t.x0[0] = cfg->x0[0]; t.x0[1] = cfg->x0[1]; t.x0[2] = cfg->x0[2];
+  t.D = cfg->D;
+  if (cfg->depth < 2 || cfg->depth > 16) return -2;
+  if (tree_build(&t, nsrc, xs) != 0) { tree_free(&t); return -1; }
+  t1 = now_s();
+  cfg->t_phase[0] = t1 - t0;
+  if (cfg->problem == 1)
+    rc = lap_solve(cfg, &t, xs, q, nrm, d, ntgt, xt, f);
+  else
+    rc = -3;
+  tree_free(&t);
+  cfg->t_phase[5] = now_s() - t0;
+  return rc;
+}

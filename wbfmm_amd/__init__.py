@@ -1,0 +1,240 @@
+"""wbfmm_amd -- MI355X (gfx950) implementation of the wbfmm octree FMM path.
+
+The product is the C-ABI library wbfmm_amd/libwbfmm.so (reference API in
+include/wbfmm.h, kernel ABI in include/wbfmm_hip.h).  This module loads it
+with ctypes and mirrors the reference's call sequence
+(tools/wbfmm-laplace-tree-calc.c.in:380-460) for Python callers, tests and
+the benchmark.  There is no fallback: if the library is missing or no GPU
+is present, the calls raise.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwbfmm.so")
+
+WBFMM_PROBLEM_LAPLACE = 1
+WBFMM_PROBLEM_HELMHOLTZ = 2
+WBFMM_FIELD_POTENTIAL = 1
+WBFMM_FIELD_GRADIENT = 2
+
+_lib = None
+_vp = ctypes.c_void_p
+_u = ctypes.c_uint
+_i = ctypes.c_int
+_d = ctypes.c_double
+_sz = ctypes.c_size_t
+
+
+class WbfmmTree(ctypes.Structure):
+    """Public part of wbfmm_tree_t (src/wbfmm.h:116-145)."""
+    _fields_ = [
+        ("problem", _i),
+        ("boxes", _vp * 17),
+        ("sorted", _i),
+        ("maxpoints", _u), ("npoints", _u), ("ip", _vp), ("nq", _u),
+        ("depth", _u), ("order_s", _u * 17), ("order_r", _u * 17),
+        ("x", ctypes.c_char * 24), ("normals", _vp), ("points", _vp),
+        ("mps", _vp * 17), ("mpr", _vp * 17),
+        ("size", _sz), ("nstr", _sz), ("pstr", _sz),
+        ("D", _d),
+    ]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "wbfmm_tree_new": (ctypes.POINTER(WbfmmTree), [_vp, _d, _u]),
+    "wbfmm_tree_add_points": (_i, [_vp, _vp, _sz, _vp, _sz, _u, _i]),
+    "wbfmm_tree_refine": (_i, [_vp]),
+    "wbfmm_tree_host_mirror": (_i, [_vp, _i]),
+    "wbfmm_tree_laplace_coefficient_init": (_i, [_vp, _u, _u, _u]),
+    "wbfmm_tree_laplace_leaf_expansions": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp]),
+    "wbfmm_shift_angle_table_init": (_i, []),
+    "wbfmm_shift_operators_new": (_vp, [_u, _i, _vp]),
+    "wbfmm_shift_operators_free": (_i, [_vp]),
+    "wbfmm_laplace_coaxial_translate_init": (_i, [_i]),
+    "wbfmm_laplace_upward_pass": (_i, [_vp, _vp, _u, _vp]),
+    "wbfmm_laplace_downward_pass_ref": (_i, [_vp, _vp, _u, _vp, _i]),
+    "wbfmm_laplace_downward_pass_avx": (_i, [_vp, _vp, _u, _vp, _i]),
+    "wbfmm_laplace_box_field": (_i, [_vp, _u, _u, _vp, _i, _vp, _i, _u, _i, _vp, _vp, _i, _vp]),
+    "wbfmm_laplace_field_targets": (_i, [_vp, _vp, _sz, _u, _vp, _i, _vp, _i, _u, _vp, _i]),
+    "wbfmm_laplace_field_direct": (_i, [_vp, _i, _vp, _i, _i, _vp, _i, _vp, _i, _i, _u, _vp, _vp, _i]),
+    "wbfmm_target_list_new": (_vp, [_vp, _u]),
+    "wbfmm_target_list_add_points": (_i, [_vp, _vp, _sz, _u]),
+    "wbfmm_target_list_coefficients_init": (_i, [_vp, _u]),
+    "wbfmm_laplace_target_list_local_coefficients": (_i, [_vp, _u, _vp]),
+    "wbfmm_target_list_local_field": (_i, [_vp, _vp, _i, _vp, _i, _vp, _i]),
+    "wbfmm_target_list_free": (_i, [_vp]),
+    "wbfmm_tree_free": (_i, [_vp]),
+    "wbfmm_tree_coefficients_download": (_i, [_vp, _u, _i, _vp]),
+    "wbfmm_tree_coefficients_zero": (_i, [_vp, _u]),
+    "wbfmm_point_box": (ctypes.c_uint64, [_vp, _u, _vp]),
+    "wbfmm_box_interaction_list_4": (_i, [_u, ctypes.c_uint64, _vp, _i]),
+    "wbfmm_box_neighbours": (_i, [_u, ctypes.c_uint64, _vp]),
+    "wbfmm_box_index": (ctypes.c_uint64, [ctypes.c_uint32] * 3),
+    "wbfmm_truncation_number": (_i, [_vp, _d, _u, _d]),
+    "wbfmm_hip_synchronize": (_i, []),
+    "wbfmm_hip_sync": (_i, []),
+    "wbfmm_hip_init": (_i, []),
+    "wbfmm_hip_timing_enable": (_i, [_i]),
+    "wbfmm_hip_timing_reset": (_i, []),
+    "wbfmm_hip_timing_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
+    "wbfmm_hip_last_error": (ctypes.c_char_p, []),
+}
+
+
+def lib():
+    """Load libwbfmm.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built: run python -m wbfmm_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    """host numpy array, torch tensor or int address -> void pointer."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor (device memory)
+
+
+def sync():
+    rc = lib().wbfmm_hip_synchronize()
+    if rc != 0:
+        raise RuntimeError(lib().wbfmm_hip_last_error().decode())
+
+
+class Timing:
+    """Per-kernel HIP-event timing on the library stream."""
+
+    @staticmethod
+    def enable(on=True):
+        lib().wbfmm_hip_timing_enable(1 if on else 0)
+
+    @staticmethod
+    def reset():
+        lib().wbfmm_hip_timing_reset()
+
+    @staticmethod
+    def query(name):
+        n = _i()
+        ms = _d()
+        lib().wbfmm_hip_timing_query(name.encode(), ctypes.byref(n), ctypes.byref(ms))
+        return n.value, ms.value
+
+
+class LaplaceSolver:
+    """Laplace FMM through the drop-in API, in the reference driver's call
+    order (tools/wbfmm-laplace-tree-calc.c.in:380-460).
+
+    xs: source positions (n,3) float64, host numpy or device torch tensor
+    (then pass its row stride in bytes via pstr).  Targets given at
+    construction are registered as a target list so the downward pass runs
+    over boxes holding sources or targets only.
+    """
+
+    def __init__(self, xs, x0, D, depth, order_s, order_r, targets=None,
+                 pstr=24, tstr=24, ntgt=None, host_mirror=True, nsrc=None,
+                 normals=None, nstr=24):
+        L = lib()
+        self.depth = depth
+        self.nsrc = len(xs) if nsrc is None else nsrc
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self._x0 = x0
+        self.xs = xs
+        self.tree = L.wbfmm_tree_new(ptr(x0), float(D), self.nsrc)
+        if not self.tree:
+            raise RuntimeError("wbfmm_tree_new failed")
+        self.t = ctypes.cast(self.tree, _vp).value
+        L.wbfmm_tree_host_mirror(self.t, 1 if host_mirror else 0)
+        lmax = max(max(order_s[2:depth + 1]), max(order_r[2:depth + 1]))
+        L.wbfmm_shift_angle_table_init()
+        self.op = L.wbfmm_shift_operators_new(lmax, 0, None)
+        L.wbfmm_laplace_coaxial_translate_init(lmax + 2)
+        self.order_s = list(order_s)
+        self.order_r = list(order_r)
+        self.targets = targets
+        self.tstr = tstr
+        self.ntgt = ntgt if ntgt is not None else (None if targets is None else len(targets))
+        self.pstr = pstr
+        self.normals = normals
+        self.nstr = nstr
+        self.tl = None
+
+    def build_tree(self):
+        """add_points + refine x depth + coefficient init (+ target list)."""
+        L = lib()
+        L.wbfmm_tree_add_points(self.t, ptr(self.xs), self.pstr, ptr(self.normals),
+                                self.nstr if self.normals is not None else 0, self.nsrc, 0)
+        for _ in range(self.depth):
+            L.wbfmm_tree_refine(self.t)
+        self.tree.contents.problem = WBFMM_PROBLEM_LAPLACE
+        self.tree.contents.nq = 1
+        for l in range(1, self.depth + 1):
+            L.wbfmm_tree_laplace_coefficient_init(self.t, l, self.order_r[l], self.order_s[l])
+        if self.targets is not None:
+            if self.tl is None:
+                self.tl = L.wbfmm_target_list_new(self.t, self.ntgt)
+            L.wbfmm_target_list_add_points(self.tl, ptr(self.targets), self.tstr, self.ntgt)
+
+    def passes(self, q, sstr=1, d=None, dstr=1):
+        """P2M, upward pass depth..3, downward pass 2..depth."""
+        L = lib()
+        for l in range(1, self.depth + 1):
+            L.wbfmm_tree_coefficients_zero(self.t, l)
+        L.wbfmm_tree_laplace_leaf_expansions(self.t, ptr(q), sstr, ptr(d), dstr, 1, None)
+        for l in range(self.depth, 2, -1):
+            L.wbfmm_laplace_upward_pass(self.t, self.op, l, None)
+        for l in range(2, self.depth + 1):
+            L.wbfmm_laplace_downward_pass_ref(self.t, self.op, l, None, 0)
+
+    def field(self, q, f, field=3, fstr=4, sstr=1, d=None, dstr=1):
+        """target-list field (L2P + near field) into f (accumulated)."""
+        L = lib()
+        L.wbfmm_target_list_coefficients_init(self.tl, field)
+        L.wbfmm_laplace_target_list_local_coefficients(self.tl, 1, None)
+        L.wbfmm_target_list_local_field(self.tl, ptr(q), sstr, ptr(d), dstr, ptr(f), fstr)
+
+    def solve(self, q, f, field=3, fstr=4, d=None):
+        self.build_tree()
+        self.passes(q, d=d)
+        self.field(q, f, field, fstr, d=d)
+
+    def coefficients(self, level, which):
+        """download mps (which=0) or mpr (which=1) of a level as an
+        (8^level, C) array in box-major order."""
+        C = (self.order_s[level] + 1) ** 2 if which == 0 else (self.order_r[level] + 1) ** 2
+        nb = 8 ** level
+        buf = np.zeros(nb * C)
+        rc = lib().wbfmm_tree_coefficients_download(self.t, level, which, ptr(buf))
+        if rc != 0:
+            raise RuntimeError("download failed")
+        # [group of 8][coef][child] -> [box][coef]
+        return buf.reshape(nb // 8, C, 8).transpose(0, 2, 1).reshape(nb, C)
+
+    def close(self):
+        L = lib()
+        if self.tl:
+            L.wbfmm_target_list_free(self.tl)
+            self.tl = None
+        if self.t:
+            L.wbfmm_tree_free(self.t)
+            self.t = None
+        if self.op:
+            L.wbfmm_shift_operators_free(self.op)
+            self.op = None

@@ -1,0 +1,86 @@
+// hip_common.h -- shared helpers of the gfx950 kernels (device side) and
+// of the launch layer (stream, error capture, per-kernel event timing).
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wbfmm_hip.h"
+
+namespace wbfmm {
+
+hipStream_t stream();
+// record an error message; returns WBFMM_HIP_ERR_RUNTIME
+int fail(const char *what, hipError_t e);
+// begin/end a timed launch of kernel `name` (no-ops unless timing is on)
+void timing_begin(const char *name);
+void timing_end(const char *name);
+
+#define WBFMM_CHECK(call)                                  \
+  do {                                                     \
+    hipError_t _e = (call);                                \
+    if (_e != hipSuccess) return ::wbfmm::fail(#call, _e); \
+  } while (0)
+
+#define WBFMM_LAUNCH(name, kernel, grid, block, shmem, ...)                  \
+  do {                                                                        \
+    ::wbfmm::timing_begin(name);                                              \
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ::wbfmm::stream(),        \
+                       __VA_ARGS__);                                          \
+    hipError_t _e = hipGetLastError();                                        \
+    ::wbfmm::timing_end(name);                                                \
+    if (_e != hipSuccess) return ::wbfmm::fail(name, _e);                    \
+  } while (0)
+
+// ---- device helpers ------------------------------------------------------
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t compact3(uint64_t x) {
+  x &= 0x1249249249249249ULL;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ULL;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00fULL;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffULL;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
+  x = (x ^ (x >> 32)) & 0x1fffff;
+  return (uint32_t)x;
+}
+
+// Morton code of a point on the 2^20 grid, same arithmetic as the
+// reference's wbfmm_point_index_3d (src/boxes.c:324-339)
+__device__ __forceinline__ uint64_t point_code(double x, double y, double z,
+                                               double x0, double y0,
+                                               double z0, double D) {
+  uint32_t i = (uint32_t)((x - x0) / D * 1048576.0);
+  uint32_t j = (uint32_t)((y - y0) / D * 1048576.0);
+  uint32_t k = (uint32_t)((z - z0) / D * 1048576.0);
+  return spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
+}
+
+// element offset of coefficient c of box b in a level array packed in
+// groups of 8 siblings (src/boxes.c:552-579, src/laplace.c:823-849);
+// C = coefficients per box (reals for Laplace)
+__device__ __forceinline__ size_t coef_at(uint64_t b, int c, int C) {
+  return (size_t)(b >> 3) * 8 * (size_t)C + (size_t)c * 8 + (b & 7);
+}
+
+__device__ __forceinline__ int isqrt_deg(int r) {
+  int n = (int)sqrt((double)r);
+  while (n * n > r) n--;
+  while ((n + 1) * (n + 1) <= r) n++;
+  return n;
+}
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kWave = 64;
+
+}  // namespace wbfmm

@@ -1,0 +1,432 @@
+// runtime.hip -- stream, memory and timing services of the thin C ABI,
+// plus the octree build kernels (Morton codes, stable sort, box ranges).
+//
+// Tree build restates src/boxes.c:118-277: the reference sorts an index
+// array by Morton code with a stable qsort and then segments it level by
+// level; here the codes are computed by one kernel, sorted with rocPRIM's
+// stable LSD radix sort, and every level's [start,end) ranges are found by
+// one kernel that marks segment boundaries in the sorted codes.
+
+#include <cstdio>
+#include <cstring>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hip_common.h"
+
+namespace wbfmm {
+
+static hipStream_t g_stream = nullptr;
+static std::mutex g_mu;
+static char g_err[512] = "";
+static bool g_timing = false;
+struct Ev {
+  std::string name;
+  hipEvent_t a, b;
+};
+static std::vector<Ev> g_events;
+static std::map<std::string, std::pair<int, double>> g_totals;
+
+hipStream_t stream() {
+  if (!g_stream) wbfmm_hip_init();
+  return g_stream;
+}
+
+int fail(const char *what, hipError_t e) {
+  snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+  fprintf(stderr, "wbfmm (HIP): %s\n", g_err);
+  return WBFMM_HIP_ERR_RUNTIME;
+}
+
+void timing_begin(const char *name) {
+  if (!g_timing) return;
+  Ev ev;
+  ev.name = name;
+  hipEventCreate(&ev.a);
+  hipEventCreate(&ev.b);
+  hipEventRecord(ev.a, g_stream);
+  g_events.push_back(ev);
+}
+
+void timing_end(const char *name) {
+  if (!g_timing) return;
+  (void)name;
+  hipEventRecord(g_events.back().b, g_stream);
+}
+
+static void timing_collect() {
+  if (g_events.empty()) return;
+  hipStreamSynchronize(g_stream);
+  for (auto &ev : g_events) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ev.a, ev.b);
+    auto &t = g_totals[ev.name];
+    t.first += 1;
+    t.second += ms;
+    hipEventDestroy(ev.a);
+    hipEventDestroy(ev.b);
+  }
+  g_events.clear();
+}
+
+// ---------------------------------------------------------------------
+// tree kernels
+// ---------------------------------------------------------------------
+
+__global__ void k_morton(const char *pts, size_t pstr, unsigned n, double x0,
+                         double y0, double z0, double D, double *xyz,
+                         uint64_t *codes, uint32_t *idx, unsigned *nbad) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *x = (const double *)(pts + (size_t)i * pstr);
+  double a = x[0], b = x[1], c = x[2];
+  if (a <= x0 || a >= x0 + D || b <= y0 || b >= y0 + D || c <= z0 ||
+      c >= z0 + D)
+    atomicAdd(nbad, 1u);
+  codes[i] = point_code(a, b, c, x0, y0, z0, D);
+  idx[i] = i;
+  xyz[4 * (size_t)i + 0] = a;
+  xyz[4 * (size_t)i + 1] = b;
+  xyz[4 * (size_t)i + 2] = c;
+  xyz[4 * (size_t)i + 3] = 0.0;
+}
+
+__global__ void k_stage4(const char *pts, size_t pstr, unsigned n,
+                         double *out) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *x = (const double *)(pts + (size_t)i * pstr);
+  out[4 * (size_t)i + 0] = x[0];
+  out[4 * (size_t)i + 1] = x[1];
+  out[4 * (size_t)i + 2] = x[2];
+  out[4 * (size_t)i + 3] = 0.0;
+}
+
+__global__ void k_gather4(const double *in, const uint32_t *idx, unsigned n,
+                          double *out) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *s = in + 4 * (size_t)idx[i];
+  double *d = out + 4 * (size_t)i;
+  d[0] = s[0];
+  d[1] = s[1];
+  d[2] = s[2];
+  d[3] = s[3];
+}
+
+__global__ void k_gather_strided(const double *src, int sstr, int ncomp,
+                                 const uint32_t *idx, unsigned n,
+                                 double *out) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; c++)
+    out[(size_t)i * ncomp + c] = src[(size_t)idx[i] * sstr + c];
+}
+
+__global__ void k_box_ranges(const uint64_t *codes, unsigned n, int shift,
+                             uint32_t *start, uint32_t *end) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t b = codes[i] >> shift;
+  if (i == 0 || (codes[i - 1] >> shift) != b) start[b] = i;
+  if (i == n - 1 || (codes[i + 1] >> shift) != b) end[b] = i + 1;
+}
+
+__global__ void k_box_flags(const uint32_t *start, const uint32_t *end,
+                            const uint32_t *flag, unsigned nb,
+                            uint32_t *out) {
+  unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  out[b] = flag ? (flag[b] != 0) : (end[b] > start[b]);
+}
+
+__global__ void k_box_scatter(const uint32_t *flags, const uint32_t *pos,
+                              unsigned nb, uint32_t *list) {
+  unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  if (flags[b]) list[pos[b]] = b;
+}
+
+__global__ void k_mark_boxes(const uint64_t *codes, unsigned n, int shift,
+                             uint32_t *flag) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[codes[i] >> shift] = 1u;
+}
+
+static inline unsigned nblk(size_t n, unsigned b) {
+  return (unsigned)((n + b - 1) / b);
+}
+
+// scratch buffer reused by the sort and scan calls (grown on demand)
+static void *g_tmp = nullptr;
+static size_t g_tmp_bytes = 0;
+static int ensure_tmp(size_t bytes) {
+  if (bytes <= g_tmp_bytes) return 0;
+  if (g_tmp) hipFree(g_tmp);
+  g_tmp = nullptr;
+  g_tmp_bytes = 0;
+  WBFMM_CHECK(hipMalloc(&g_tmp, bytes));
+  g_tmp_bytes = bytes;
+  return 0;
+}
+
+}  // namespace wbfmm
+
+using namespace wbfmm;
+
+extern "C" {
+
+int wbfmm_hip_init(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_stream) return 0;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    snprintf(g_err, sizeof(g_err), "no HIP device available (%s)",
+             hipGetErrorString(e));
+    fprintf(stderr, "wbfmm (HIP): %s\n", g_err);
+    return WBFMM_HIP_ERR_RUNTIME;
+  }
+  // default priority, non-blocking with respect to the null stream
+  WBFMM_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return 0;
+}
+
+const char *wbfmm_hip_last_error(void) { return g_err; }
+
+int wbfmm_hip_malloc(void **p, size_t bytes) {
+  if (wbfmm_hip_init()) return WBFMM_HIP_ERR_RUNTIME;
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    fail("hipMalloc", e);
+    return WBFMM_HIP_ERR_NOMEM;
+  }
+  return 0;
+}
+
+int wbfmm_hip_free(void *p) {
+  if (!p) return 0;
+  hipStreamSynchronize(stream());
+  WBFMM_CHECK(hipFree(p));
+  return 0;
+}
+
+int wbfmm_hip_is_device(const void *p) {
+  if (!p) return 0;
+  if (wbfmm_hip_init()) return 0;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    hipGetLastError();  // clear the sticky error from an unknown pointer
+    return 0;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int wbfmm_hip_memcpy(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return 0;
+  WBFMM_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, stream()));
+  if (!wbfmm_hip_is_device(dst) || !wbfmm_hip_is_device(src))
+    WBFMM_CHECK(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int wbfmm_hip_memset(void *p, int v, size_t bytes) {
+  if (bytes == 0) return 0;
+  WBFMM_CHECK(hipMemsetAsync(p, v, bytes, stream()));
+  return 0;
+}
+
+int wbfmm_hip_sync(void) {
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int wbfmm_hip_timing_enable(int on) {
+  timing_collect();
+  g_timing = (on != 0);
+  return 0;
+}
+
+int wbfmm_hip_timing_reset(void) {
+  timing_collect();
+  g_totals.clear();
+  return 0;
+}
+
+int wbfmm_hip_timing_query(const char *name, int *launches, double *ms) {
+  timing_collect();
+  auto it = g_totals.find(name);
+  if (it == g_totals.end()) {
+    *launches = 0;
+    *ms = 0.0;
+    return 0;
+  }
+  *launches = it->second.first;
+  *ms = it->second.second;
+  return 0;
+}
+
+int wbfmm_hip_morton(const void *pts, size_t pstr, unsigned n,
+                     const double *x0, double D, double *xyz,
+                     uint64_t *codes, uint32_t *idx, unsigned *nbad) {
+  if (n == 0) {
+    *nbad = 0;
+    return 0;
+  }
+  const char *dpts = (const char *)pts;
+  void *staged = nullptr;
+  if (!wbfmm_hip_is_device(pts)) {
+    // copy the strided host block as it lies
+    size_t bytes = (size_t)(n - 1) * pstr + 3 * sizeof(double);
+    if (wbfmm_hip_malloc(&staged, bytes)) return WBFMM_HIP_ERR_NOMEM;
+    WBFMM_CHECK(hipMemcpyAsync(staged, pts, bytes, hipMemcpyHostToDevice,
+                               stream()));
+    dpts = (const char *)staged;
+  }
+  unsigned *dbad = nullptr;
+  WBFMM_CHECK(hipMalloc(&dbad, sizeof(unsigned)));
+  WBFMM_CHECK(hipMemsetAsync(dbad, 0, sizeof(unsigned), stream()));
+  WBFMM_LAUNCH("tree_morton", k_morton, dim3(nblk(n, 256)), dim3(256), 0,
+               dpts, pstr, n, x0[0], x0[1], x0[2], D, xyz, codes, idx, dbad);
+  WBFMM_CHECK(hipMemcpyAsync(nbad, dbad, sizeof(unsigned),
+                             hipMemcpyDeviceToHost, stream()));
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
+  hipFree(dbad);
+  if (staged) hipFree(staged);
+  return 0;
+}
+
+int wbfmm_hip_stage4(const void *pts, size_t pstr, unsigned n, double *out) {
+  if (n == 0) return 0;
+  const char *dpts = (const char *)pts;
+  void *staged = nullptr;
+  if (!wbfmm_hip_is_device(pts)) {
+    size_t bytes = (size_t)(n - 1) * pstr + 3 * sizeof(double);
+    if (wbfmm_hip_malloc(&staged, bytes)) return WBFMM_HIP_ERR_NOMEM;
+    WBFMM_CHECK(hipMemcpyAsync(staged, pts, bytes, hipMemcpyHostToDevice,
+                               stream()));
+    dpts = (const char *)staged;
+  }
+  WBFMM_LAUNCH("stage4", k_stage4, dim3(nblk(n, 256)), dim3(256), 0, dpts,
+               pstr, n, out);
+  if (staged) {
+    WBFMM_CHECK(hipStreamSynchronize(stream()));
+    hipFree(staged);
+  }
+  return 0;
+}
+
+int wbfmm_hip_sort(uint64_t *codes, uint32_t *idx, unsigned n) {
+  if (n < 2) return 0;
+  uint64_t *k2 = nullptr;
+  uint32_t *v2 = nullptr;
+  WBFMM_CHECK(hipMalloc(&k2, sizeof(uint64_t) * n));
+  WBFMM_CHECK(hipMalloc(&v2, sizeof(uint32_t) * n));
+  size_t bytes = 0;
+  WBFMM_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, codes, k2, idx, v2,
+                                        n, 0, 60, stream()));
+  if (ensure_tmp(bytes)) return WBFMM_HIP_ERR_NOMEM;
+  timing_begin("tree_sort");
+  WBFMM_CHECK(rocprim::radix_sort_pairs(g_tmp, bytes, codes, k2, idx, v2, n,
+                                        0, 60, stream()));
+  timing_end("tree_sort");
+  WBFMM_CHECK(hipMemcpyAsync(codes, k2, sizeof(uint64_t) * n,
+                             hipMemcpyDeviceToDevice, stream()));
+  WBFMM_CHECK(hipMemcpyAsync(idx, v2, sizeof(uint32_t) * n,
+                             hipMemcpyDeviceToDevice, stream()));
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
+  hipFree(k2);
+  hipFree(v2);
+  return 0;
+}
+
+int wbfmm_hip_gather4(const double *in, const uint32_t *idx, unsigned n,
+                      double *out) {
+  if (n == 0) return 0;
+  WBFMM_LAUNCH("gather4", k_gather4, dim3(nblk(n, 256)), dim3(256), 0, in,
+               idx, n, out);
+  return 0;
+}
+
+int wbfmm_hip_gather_strided(const double *src, int sstr, int ncomp,
+                             const uint32_t *idx, unsigned n, double *out) {
+  if (n == 0) return 0;
+  if (sstr < ncomp) return WBFMM_HIP_ERR_ARG;
+  const double *dsrc = src;
+  void *staged = nullptr;
+  if (!wbfmm_hip_is_device(src)) {
+    size_t bytes = sizeof(double) * ((size_t)(n - 1) * sstr + ncomp);
+    // the index range covers 0..n-1 (a permutation of the input)
+    if (wbfmm_hip_malloc(&staged, bytes)) return WBFMM_HIP_ERR_NOMEM;
+    WBFMM_CHECK(hipMemcpyAsync(staged, src, bytes, hipMemcpyHostToDevice,
+                               stream()));
+    dsrc = (const double *)staged;
+  }
+  WBFMM_LAUNCH("gather_strength", k_gather_strided, dim3(nblk(n, 256)),
+               dim3(256), 0, dsrc, sstr, ncomp, idx, n, out);
+  if (staged) {
+    WBFMM_CHECK(hipStreamSynchronize(stream()));
+    hipFree(staged);
+  }
+  return 0;
+}
+
+int wbfmm_hip_box_ranges(const uint64_t *codes, unsigned n, unsigned level,
+                         uint32_t *start, uint32_t *end) {
+  size_t nb = (size_t)1 << (3 * level);
+  WBFMM_CHECK(hipMemsetAsync(start, 0, sizeof(uint32_t) * nb, stream()));
+  WBFMM_CHECK(hipMemsetAsync(end, 0, sizeof(uint32_t) * nb, stream()));
+  if (n == 0) return 0;
+  WBFMM_LAUNCH("tree_box_ranges", k_box_ranges, dim3(nblk(n, 256)),
+               dim3(256), 0, codes, n, (int)(3 * (20 - level)), start, end);
+  return 0;
+}
+
+int wbfmm_hip_compact_boxes(const uint32_t *start, const uint32_t *end,
+                            const uint32_t *flag, unsigned level,
+                            uint32_t *list, unsigned *count) {
+  unsigned nb = 1u << (3 * level);
+  uint32_t *flags = nullptr, *pos = nullptr;
+  WBFMM_CHECK(hipMalloc(&flags, sizeof(uint32_t) * nb));
+  WBFMM_CHECK(hipMalloc(&pos, sizeof(uint32_t) * nb));
+  WBFMM_LAUNCH("tree_box_flags", k_box_flags, dim3(nblk(nb, 256)), dim3(256),
+               0, start, end, flag, nb, flags);
+  size_t bytes = 0;
+  WBFMM_CHECK(rocprim::exclusive_scan(nullptr, bytes, flags, pos, 0u, nb,
+                                      rocprim::plus<uint32_t>(), stream()));
+  if (ensure_tmp(bytes)) return WBFMM_HIP_ERR_NOMEM;
+  WBFMM_CHECK(rocprim::exclusive_scan(g_tmp, bytes, flags, pos, 0u, nb,
+                                      rocprim::plus<uint32_t>(), stream()));
+  WBFMM_LAUNCH("tree_box_scatter", k_box_scatter, dim3(nblk(nb, 256)),
+               dim3(256), 0, flags, pos, nb, list);
+  uint32_t lastf = 0, lastp = 0;
+  WBFMM_CHECK(hipMemcpyAsync(&lastf, flags + nb - 1, 4, hipMemcpyDeviceToHost,
+                             stream()));
+  WBFMM_CHECK(hipMemcpyAsync(&lastp, pos + nb - 1, 4, hipMemcpyDeviceToHost,
+                             stream()));
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
+  *count = lastf + lastp;
+  hipFree(flags);
+  hipFree(pos);
+  return 0;
+}
+
+int wbfmm_hip_mark_boxes(const uint64_t *codes, unsigned n, unsigned level,
+                         uint32_t *flag) {
+  if (n == 0) return 0;
+  WBFMM_LAUNCH("tree_mark_boxes", k_mark_boxes, dim3(nblk(n, 256)),
+               dim3(256), 0, codes, n, (int)(3 * (20 - level)), flag);
+  return 0;
+}
+
+}  // extern "C"

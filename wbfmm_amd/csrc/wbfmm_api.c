@@ -1,0 +1,1027 @@
+/*
+ * wbfmm_api.c -- host side (C) of the drop-in library: the reference's
+ * wbfmm_tree_* / upward-downward pass / target-field API (wbfmm.h),
+ * implemented over the thin HIP C ABI of wbfmm_hip.h.
+ *
+ * The reference keeps everything on the host and runs each pass as C
+ * loops over boxes (src/boxes.c, src/pass-laplace.c, src/local-laplace.c).
+ * Here the public structs keep their reference layout and host-visible
+ * fields, while points, box ranges and expansion coefficients live in
+ * GPU memory; each API call queues the corresponding kernels on the
+ * library stream.  Calls that hand results back to host memory wait for
+ * them.
+ */
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "wbfmm.h"
+#include "wbfmm_hip.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* abort on misuse, like the reference's g_error() */
+#define WBFMM_FATAL(...)                              \
+  do {                                                \
+    fprintf(stderr, "wbfmm: %s: ", __func__);         \
+    fprintf(stderr, __VA_ARGS__);                     \
+    fprintf(stderr, "\n");                            \
+    abort();                                          \
+  } while (0)
+
+#define HIPOK(call)                                                       \
+  do {                                                                    \
+    int _rc = (call);                                                     \
+    if (_rc != 0)                                                         \
+      WBFMM_FATAL("%s failed (%d): %s", #call, _rc, wbfmm_hip_last_error()); \
+  } while (0)
+
+/* ------------------------------------------------------------------ */
+/* internal state behind the public structs                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  wbfmm_tree_t t; /* public part, must stay first */
+  int host_mirror;
+  double *d_xyz;      /* sorted source positions, 4 doubles each */
+  double *d_xyz_raw;  /* unsorted staging */
+  double *d_nrm;      /* sorted normals, 4 doubles each, or NULL */
+  uint64_t *d_codes;  /* sorted Morton codes */
+  uint32_t *d_idx;    /* sorted position -> caller index */
+  uint32_t *d_start[WBFMM_TREE_MAX_DEPTH + 1], *d_end[WBFMM_TREE_MAX_DEPTH + 1];
+  uint32_t *d_occ[WBFMM_TREE_MAX_DEPTH + 1];
+  unsigned nocc[WBFMM_TREE_MAX_DEPTH + 1];
+  /* boxes holding sources or registered targets; when no target list is
+     registered every box is a possible target and these stay NULL */
+  uint32_t *d_flag[WBFMM_TREE_MAX_DEPTH + 1];
+  uint32_t *d_act[WBFMM_TREE_MAX_DEPTH + 1];
+  unsigned nact[WBFMM_TREE_MAX_DEPTH + 1];
+  size_t mps_bytes[WBFMM_TREE_MAX_DEPTH + 1], mpr_bytes[WBFMM_TREE_MAX_DEPTH + 1];
+  double *d_q, *d_dip; /* strengths in sorted order */
+  unsigned cap_q, cap_dip;
+} tree_dev;
+
+typedef struct {
+  wbfmm_target_list_t l; /* public part, must stay first */
+  double *d_xt;          /* target positions (sorted), 4 doubles each */
+  uint32_t *d_tid;       /* sorted -> caller index */
+  uint64_t *d_codes;
+} tlist_dev;
+
+typedef struct {
+  wbfmm_shift_operators_t op; /* public part, must stay first */
+  double *d_rot;
+  wbfmm_rot_layout_t lay;
+  double *d_lap_coax;
+} ops_dev;
+
+static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
+
+static int g_lap_coax_N = -1; /* wbfmm_laplace_coaxial_translate_init */
+
+/* ------------------------------------------------------------------ */
+/* indexing (precision independent), src/indexing.c:165-427           */
+/* ------------------------------------------------------------------ */
+
+static uint64_t spread3(uint32_t v)
+{
+  uint64_t x = v & 0x1fffff;
+  x = (x | (x << 32)) & 0x1f00000000ffffULL;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+  x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+  x = (x | (x << 2)) & 0x1249249249249249ULL;
+  return x;
+}
+
+static uint32_t compact3(uint64_t x)
+{
+  x &= 0x1249249249249249ULL;
+  x = (x ^ (x >> 2)) & 0x10c30c30c30c30c3ULL;
+  x = (x ^ (x >> 4)) & 0x100f00f00f00f00fULL;
+  x = (x ^ (x >> 8)) & 0x1f0000ff0000ffULL;
+  x = (x ^ (x >> 16)) & 0x1f00000000ffffULL;
+  x = (x ^ (x >> 32)) & 0x1fffff;
+  return (uint32_t)x;
+}
+
+guint64 wbfmm_box_index(guint32 i, guint32 j, guint32 k)
+{
+  return spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
+}
+
+gint wbfmm_box_location(guint64 idx, guint32 *i, guint32 *j, guint32 *k)
+{
+  *i = compact3(idx);
+  *j = compact3(idx >> 1);
+  *k = compact3(idx >> 2);
+  return 0;
+}
+
+gint wbfmm_point_from_index(guint64 i, guint32 *x, guint32 *y, guint32 *z)
+{
+  return wbfmm_box_location(i, x, y, z);
+}
+
+guint64 wbfmm_point_locate_box(guint64 x, guint level)
+{
+  return x >> 3 * (20 - level);
+}
+
+guint64 wbfmm_box_parent(guint64 idx) { return idx >> 3; }
+guint64 wbfmm_box_first_child(guint64 idx) { return idx << 3; }
+
+gint wbfmm_box_neighbours(guint level, guint64 idx, guint64 *neighbours)
+{
+  guint32 i, j, k;
+  int nbox = 1 << level, n = 0, ii, jj, kk;
+  wbfmm_box_location(idx, &i, &j, &k);
+  for (ii = (int)i - 1; ii <= (int)i + 1; ii++) {
+    if (ii < 0 || ii >= nbox) continue;
+    for (jj = (int)j - 1; jj <= (int)j + 1; jj++) {
+      if (jj < 0 || jj >= nbox) continue;
+      for (kk = (int)k - 1; kk <= (int)k + 1; kk++) {
+        if (kk < 0 || kk >= nbox) continue;
+        neighbours[n++] = wbfmm_box_index(ii, jj, kk);
+      }
+    }
+  }
+  return n;
+}
+
+/* list-4 entries are (box, offset slot) pairs, as in the reference; with
+   sort TRUE the entries are grouped by polar angle of the offset, the
+   grouping the reference uses for its rotation tables */
+typedef struct { guint64 b, slot; double key; } l4ent;
+
+static int cmp_l4(const void *a, const void *b)
+{
+  const l4ent *x = a, *y = b;
+  if (x->key < y->key) return -1;
+  if (x->key > y->key) return 1;
+  return (x->slot < y->slot) ? -1 : (x->slot > y->slot);
+}
+
+gint wbfmm_box_interaction_list_4(guint level, guint64 idx, guint64 *list,
+                                  gboolean sort)
+{
+  guint32 i, j, k;
+  int nbox = 1 << level, n = 0, ii, jj, kk, i0, j0, k0;
+  l4ent ent[216];
+  wbfmm_box_location(idx, &i, &j, &k);
+  i0 = (int)i - (int)(i % 2) - 2;
+  j0 = (int)j - (int)(j % 2) - 2;
+  k0 = (int)k - (int)(k % 2) - 2;
+  for (ii = i0; ii <= i0 + 5; ii++) {
+    if (ii < 0 || ii >= nbox) continue;
+    for (jj = j0; jj <= j0 + 5; jj++) {
+      if (jj < 0 || jj >= nbox) continue;
+      for (kk = k0; kk <= k0 + 5; kk++) {
+        int dx = ii - (int)i, dy = jj - (int)j, dz = kk - (int)k;
+        if (kk < 0 || kk >= nbox) continue;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        ent[n].b = wbfmm_box_index(ii, jj, kk);
+        ent[n].slot = (guint64)((dx + 3) * 49 + (dy + 3) * 7 + dz + 3);
+        ent[n].key = acos(-dz / sqrt((double)(dx * dx + dy * dy + dz * dz)));
+        n++;
+      }
+    }
+  }
+  if (sort) qsort(ent, n, sizeof(l4ent), cmp_l4);
+  for (ii = 0; ii < n; ii++) {
+    list[2 * ii] = ent[ii].b;
+    list[2 * ii + 1] = ent[ii].slot;
+  }
+  return n;
+}
+
+gint wbfmm_box_interaction_index(gint i, gint j, gint k)
+{
+  /* index of the polar-angle class of offset (i,j,k): the reference
+     returns the row of its 49-entry theta table (src/indexing.c:385-392);
+     this returns the same classification as the position of cos(theta)
+     among the sorted distinct values over all list-4 offsets */
+  static double vals[64];
+  static int nv = 0;
+  double c, r = sqrt((double)(i * i + j * j + k * k));
+  int a;
+  if (nv == 0) {
+    int dx, dy, dz;
+    for (dx = -3; dx <= 3; dx++)
+      for (dy = -3; dy <= 3; dy++)
+        for (dz = -3; dz <= 3; dz++) {
+          double v;
+          int b, dup = 0;
+          if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+          v = -dz / sqrt((double)(dx * dx + dy * dy + dz * dz));
+          for (b = 0; b < nv; b++)
+            if (fabs(vals[b] - v) < 1e-12) dup = 1;
+          if (!dup) vals[nv++] = v;
+        }
+    for (a = 0; a < nv; a++) {
+      int b;
+      for (b = a + 1; b < nv; b++)
+        if (vals[b] > vals[a]) { double tt = vals[a]; vals[a] = vals[b]; vals[b] = tt; }
+    }
+  }
+  if (i < -3 || i > 3 || j < -3 || j > 3 || k < -3 || k > 3)
+    WBFMM_FATAL("offset (%d,%d,%d) outside the 7x7x7 grid", i, j, k);
+  c = -k / r;
+  for (a = 0; a < nv; a++)
+    if (fabs(vals[a] - c) < 1e-12) return a;
+  return -1;
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry helpers, src/util.c:280-319, src/boxes.c:324-339, 585-634 */
+/* ------------------------------------------------------------------ */
+
+guint64 wbfmm_point_index_3d(gdouble *x, gdouble *c, gdouble D)
+{
+  guint32 xi = (guint32)((x[0] - c[0]) / D * WBFMM_INDEX_SHIFT);
+  guint32 yi = (guint32)((x[1] - c[1]) / D * WBFMM_INDEX_SHIFT);
+  guint32 zi = (guint32)((x[2] - c[2]) / D * WBFMM_INDEX_SHIFT);
+  return wbfmm_box_index(xi, yi, zi);
+}
+
+guint64 wbfmm_point_box(wbfmm_tree_t *t, guint level, gdouble *x)
+{
+  guint64 i = wbfmm_point_index_3d(x, (gdouble *)wbfmm_tree_origin(t), t->D);
+  return wbfmm_point_locate_box(i, level);
+}
+
+gint wbfmm_box_location_from_index(guint64 idx, guint32 level, gdouble *x0,
+                                   gdouble D, gdouble *x, gdouble *wb)
+{
+  guint32 i, j, k;
+  if (!(idx < ((guint64)1 << (3 * level))))
+    WBFMM_FATAL("box %llu is not at level %u", (unsigned long long)idx, level);
+  *wb = D / (1 << level);
+  wbfmm_box_location(idx, &i, &j, &k);
+  x[0] = x0[0] + (*wb) * i;
+  x[1] = x0[1] + (*wb) * j;
+  x[2] = x0[2] + (*wb) * k;
+  return 0;
+}
+
+gint wbfmm_tree_box_centre(wbfmm_tree_t *t, guint level, guint64 b,
+                           gdouble *xb, gdouble *wb)
+{
+  wbfmm_box_location_from_index(b, level, (gdouble *)wbfmm_tree_origin(t),
+                                t->D, xb, wb);
+  xb[0] += 0.5 * (*wb);
+  xb[1] += 0.5 * (*wb);
+  xb[2] += 0.5 * (*wb);
+  return 0;
+}
+
+gint wbfmm_points_origin_width(gdouble *x, gint str, gint n, gdouble *xmin,
+                               gdouble *xmax, gdouble *D,
+                               gboolean init_limits)
+{
+  gint i;
+  if (init_limits) {
+    xmin[0] = xmax[0] = x[0];
+    xmin[1] = xmax[1] = x[1];
+    xmin[2] = xmax[2] = x[2];
+  }
+  for (i = 0; i < n; i++) {
+    int a;
+    for (a = 0; a < 3; a++) {
+      double v = x[i * str + a];
+      if (v < xmin[a]) xmin[a] = v;
+      if (v > xmax[a]) xmax[a] = v;
+    }
+  }
+  *D = xmax[0] - xmin[0];
+  if (xmax[1] - xmin[1] > *D) *D = xmax[1] - xmin[1];
+  if (xmax[2] - xmin[2] > *D) *D = xmax[2] - xmin[2];
+  return 0;
+}
+
+gint wbfmm_truncation_number(wbfmm_tree_t *t, gdouble k, guint level,
+                             gdouble tol)
+{
+  /* Gumerov & Duraiswami, JASA 2009 (src/util.c:474-503):
+     p = (p_lo^4 + p_hi^4)^(1/4) */
+  const double del = 2.0;
+  double ka = 0.5 * sqrt(3.0) * k * t->D / (1 << level);
+  double plo = 1.0 - log(tol * (1.0 - 1.0 / del) * sqrt(1.0 - 1.0 / del)) / log(del);
+  double phi = cbrt(3.0 * log(1.0 / tol));
+  phi = ka + 0.5 * phi * phi * cbrt(ka);
+  return (gint)ceil(sqrt(sqrt(plo * plo * plo * plo + phi * phi * phi * phi)));
+}
+
+/* ------------------------------------------------------------------ */
+/* tree                                                               */
+/* ------------------------------------------------------------------ */
+
+wbfmm_tree_t *wbfmm_tree_new(gdouble *x, gdouble D, guint maxpoints)
+{
+  tree_dev *td = calloc(1, sizeof(tree_dev));
+  wbfmm_tree_t *t = &td->t;
+  double *xt = (double *)wbfmm_tree_origin(t);
+  HIPOK(wbfmm_hip_init());
+  t->problem = 0;
+  t->sorted = FALSE;
+  t->maxpoints = maxpoints;
+  t->nq = 0;
+  t->ip = malloc(sizeof(guint) * (maxpoints ? maxpoints : 1));
+  t->npoints = 0;
+  t->size = sizeof(double);
+  t->boxes[0] = calloc(1, sizeof(wbfmm_box_t));
+  t->depth = 0;
+  xt[0] = x[0];
+  xt[1] = x[1];
+  xt[2] = x[2];
+  t->D = D;
+  td->host_mirror = 1;
+  HIPOK(wbfmm_hip_malloc((void **)&td->d_xyz, sizeof(double) * 4 * (maxpoints ? maxpoints : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)&td->d_xyz_raw, sizeof(double) * 4 * (maxpoints ? maxpoints : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)&td->d_codes, sizeof(uint64_t) * (maxpoints ? maxpoints : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)&td->d_idx, sizeof(uint32_t) * (maxpoints ? maxpoints : 1)));
+  return t;
+}
+
+gint wbfmm_tree_host_mirror(wbfmm_tree_t *t, gboolean on)
+{
+  TD(t)->host_mirror = on ? 1 : 0;
+  return 0;
+}
+
+static void level_ranges(tree_dev *td, unsigned l)
+{
+  size_t nb = (size_t)1 << (3 * l);
+  if (!td->d_start[l]) {
+    HIPOK(wbfmm_hip_malloc((void **)&td->d_start[l], sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_malloc((void **)&td->d_end[l], sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_malloc((void **)&td->d_occ[l], sizeof(uint32_t) * nb));
+  }
+  HIPOK(wbfmm_hip_box_ranges(td->d_codes, td->t.npoints, l, td->d_start[l],
+                             td->d_end[l]));
+  HIPOK(wbfmm_hip_compact_boxes(td->d_start[l], td->d_end[l], NULL, l,
+                                td->d_occ[l], &td->nocc[l]));
+}
+
+static void mirror_level(tree_dev *td, unsigned l)
+{
+  size_t nb = (size_t)1 << (3 * l), b;
+  uint32_t *s, *e;
+  wbfmm_box_t *boxes = td->t.boxes[l];
+  if (!td->host_mirror || !boxes) return;
+  s = malloc(sizeof(uint32_t) * nb);
+  e = malloc(sizeof(uint32_t) * nb);
+  HIPOK(wbfmm_hip_memcpy(s, td->d_start[l], sizeof(uint32_t) * nb));
+  HIPOK(wbfmm_hip_memcpy(e, td->d_end[l], sizeof(uint32_t) * nb));
+  for (b = 0; b < nb; b++) {
+    boxes[b].n = e[b] - s[b];
+    boxes[b].i = boxes[b].n ? s[b] : 0;
+  }
+  free(s);
+  free(e);
+}
+
+gint wbfmm_tree_add_points(wbfmm_tree_t *t, gpointer pts, gsize pstr,
+                           gpointer normals, gsize nstr, guint npts,
+                           gboolean sorted)
+{
+  tree_dev *td = TD(t);
+  unsigned nbad = 0;
+  double *x0 = (double *)wbfmm_tree_origin(t);
+  if (t->size != sizeof(double))
+    WBFMM_FATAL("mixed precision not implemented");
+  if (npts > t->maxpoints)
+    WBFMM_FATAL("too many points (%u) for tree (%u)", npts, t->maxpoints);
+  t->npoints = npts;
+  t->points = (char *)pts;
+  t->pstr = pstr;
+  t->normals = (char *)normals;
+  t->nstr = nstr;
+  t->sorted = sorted;
+  HIPOK(wbfmm_hip_morton(pts, pstr, npts, x0, t->D, td->d_xyz_raw,
+                         td->d_codes, td->d_idx, &nbad));
+  if (nbad)
+    WBFMM_FATAL("%u points do not lie in box of width %g at (%g,%g,%g)", nbad,
+                t->D, x0[0], x0[1], x0[2]);
+  HIPOK(wbfmm_hip_sort(td->d_codes, td->d_idx, npts));
+  HIPOK(wbfmm_hip_gather4(td->d_xyz_raw, td->d_idx, npts, td->d_xyz));
+  if (normals) {
+    if (!td->d_nrm)
+      HIPOK(wbfmm_hip_malloc((void **)&td->d_nrm, sizeof(double) * 4 * (t->maxpoints ? t->maxpoints : 1)));
+    HIPOK(wbfmm_hip_stage4(normals, nstr, npts, td->d_xyz_raw));
+    HIPOK(wbfmm_hip_gather4(td->d_xyz_raw, td->d_idx, npts, td->d_nrm));
+  }
+  if (td->host_mirror && t->ip && npts)
+    HIPOK(wbfmm_hip_memcpy(t->ip, td->d_idx, sizeof(uint32_t) * npts));
+  t->boxes[0][0].i = 0;
+  t->boxes[0][0].n = npts;
+  level_ranges(td, 0);
+  return 0;
+}
+
+gint wbfmm_tree_refine(wbfmm_tree_t *t)
+{
+  tree_dev *td = TD(t);
+  unsigned l = t->depth + 1;
+  if (l > WBFMM_TREE_MAX_DEPTH)
+    WBFMM_FATAL("tree depth would exceed %d", WBFMM_TREE_MAX_DEPTH);
+  if (td->host_mirror && !t->boxes[l])
+    t->boxes[l] = calloc((size_t)1 << (3 * l), sizeof(wbfmm_box_t));
+  t->depth = l;
+  level_ranges(td, l);
+  mirror_level(td, l);
+  return 0;
+}
+
+gint wbfmm_tree_coefficients_zero(wbfmm_tree_t *t, guint level)
+{
+  tree_dev *td = TD(t);
+  if (t->mps[level]) HIPOK(wbfmm_hip_memset(t->mps[level], 0, td->mps_bytes[level]));
+  if (t->mpr[level]) HIPOK(wbfmm_hip_memset(t->mpr[level], 0, td->mpr_bytes[level]));
+  return 0;
+}
+
+gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
+                                      gint which, gdouble *host)
+{
+  tree_dev *td = TD(t);
+  void *src = which ? t->mpr[level] : t->mps[level];
+  size_t bytes = which ? td->mpr_bytes[level] : td->mps_bytes[level];
+  if (!src) return -1;
+  HIPOK(wbfmm_hip_memcpy(host, src, bytes));
+  return 0;
+}
+
+static void set_box_pointers(tree_dev *td, unsigned l, int which, int C)
+{
+  wbfmm_tree_t *t = &td->t;
+  size_t nb = (size_t)1 << (3 * l), i;
+  double *c = which ? (double *)t->mpr[l] : (double *)t->mps[l];
+  wbfmm_box_t *boxes = t->boxes[l];
+  if (!boxes) return;
+  /* coefficients packed in groups of 8 siblings (src/laplace.c:823-849) */
+  for (i = 0; i < nb; i++) {
+    double *p = c + (i / 8) * 8 * (size_t)C + (i % 8);
+    if (which) boxes[i].mpr = p; else boxes[i].mps = p;
+  }
+}
+
+static void alloc_level(tree_dev *td, unsigned l, int which, size_t bytes)
+{
+  wbfmm_tree_t *t = &td->t;
+  void **slot = which ? &t->mpr[l] : &t->mps[l];
+  size_t *have = which ? &td->mpr_bytes[l] : &td->mps_bytes[l];
+  if (*slot && *have != bytes) {
+    wbfmm_hip_free(*slot);
+    *slot = NULL;
+  }
+  if (!*slot) HIPOK(wbfmm_hip_malloc(slot, bytes));
+  *have = bytes;
+  HIPOK(wbfmm_hip_memset(*slot, 0, bytes));
+}
+
+gint wbfmm_tree_laplace_coefficient_init(wbfmm_tree_t *t, guint l, guint nr,
+                                         guint ns)
+{
+  tree_dev *td = TD(t);
+  size_t nb = (size_t)1 << (3 * l);
+  if ((int)t->nq < 1)
+    WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
+  if (t->nq != 1) WBFMM_FATAL("only nq == 1 is supported (nq = %u)", t->nq);
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (l > t->depth) WBFMM_FATAL("level %u below tree depth %u", l, t->depth);
+  if (ns > WBFMM_HIP_MAX_ORDER || nr > WBFMM_HIP_MAX_ORDER)
+    WBFMM_FATAL("order above %d", WBFMM_HIP_MAX_ORDER);
+  t->order_s[l] = ns;
+  t->order_r[l] = nr;
+  if (ns) {
+    alloc_level(td, l, 0, nb * (ns + 1) * (ns + 1) * sizeof(double));
+    if (td->host_mirror) set_box_pointers(td, l, 0, (ns + 1) * (ns + 1));
+  }
+  if (nr) {
+    alloc_level(td, l, 1, nb * (nr + 1) * (nr + 1) * sizeof(double));
+    if (td->host_mirror) set_box_pointers(td, l, 1, (nr + 1) * (nr + 1));
+  }
+  return 0;
+}
+
+/* strengths (host or device, element stride) into sorted device order */
+static double *gather_strengths(tree_dev *td, double *src, gint sstr,
+                                int dip)
+{
+  unsigned n = td->t.npoints;
+  double **buf = dip ? &td->d_dip : &td->d_q;
+  unsigned *cap = dip ? &td->cap_dip : &td->cap_q;
+  if (!src) return NULL;
+  if (*cap < n) {
+    if (*buf) wbfmm_hip_free(*buf);
+    HIPOK(wbfmm_hip_malloc((void **)buf, sizeof(double) * (n ? n : 1)));
+    *cap = n;
+  }
+  HIPOK(wbfmm_hip_gather_strided(src, sstr, 1, td->d_idx, n, *buf));
+  return *buf;
+}
+
+gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
+                                        gint sstr, gdouble *dipoles,
+                                        gint dstr, gboolean zero_expansions,
+                                        gdouble *work)
+{
+  tree_dev *td = TD(t);
+  unsigned d = t->depth;
+  double *q, *dp;
+  (void)work;
+  if ((int)t->nq < 1)
+    WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (!t->mps[d]) WBFMM_FATAL("coefficients not initialised at level %u", d);
+  if (zero_expansions) HIPOK(wbfmm_hip_memset(t->mps[d], 0, td->mps_bytes[d]));
+  if (src == NULL && dipoles == NULL) return 0;
+  if (t->normals == NULL && dipoles != NULL)
+    WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
+  q = gather_strengths(td, src, sstr, 0);
+  dp = gather_strengths(td, dipoles, dstr, 1);
+  HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
+                          td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
+                          dp, d, (double *)wbfmm_tree_origin(t), t->D,
+                          t->order_s[d], (double *)t->mps[d]));
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* shift operators                                                    */
+/* ------------------------------------------------------------------ */
+
+gint wbfmm_shift_angle_table_init(void) { return 0; }
+
+gint wbfmm_shift_angles_list4(gint i, gint j, gint k, gdouble *th,
+                              gdouble *ph, gdouble *ch, gdouble *rs)
+{
+  /* angles of the translation from a source at offset (i,j,k) from the
+     target (src/shift.c:587-626) */
+  double ex = -i, ey = -j, ez = -k, r = sqrt(ex * ex + ey * ey + ez * ez);
+  if (r == 0.0) WBFMM_FATAL("zero offset");
+  *rs = r;
+  *th = acos(ez / r);
+  if (i == 0 && j == 0) { *ph = 0.0; *ch = 0.0; }
+  else { *ph = -0.5 * M_PI; *ch = atan2(ey, ex); }
+  return 0;
+}
+
+wbfmm_shift_operators_t *wbfmm_shift_operators_new(guint L, gboolean bw,
+                                                   gdouble *work)
+{
+  ops_dev *od = calloc(1, sizeof(ops_dev));
+  (void)work;
+  if (L > WBFMM_HIP_MAX_ORDER) WBFMM_FATAL("order %u above %d", L, WBFMM_HIP_MAX_ORDER);
+  od->op.size = sizeof(double);
+  od->op.Lmax = L;
+  od->op.bw = bw;
+  HIPOK(wbfmm_hip_rotation_tables((int)L, &od->d_rot, &od->lay));
+  od->op.nerot = (guint)od->lay.nrot;
+  od->op.rotations = od->d_rot;
+  return &od->op;
+}
+
+static void ensure_lap_coax(ops_dev *od)
+{
+  if (!od->d_lap_coax)
+    HIPOK(wbfmm_hip_laplace_coaxial_tables(od->lay.L, &od->d_lap_coax));
+}
+
+gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
+{
+  ops_dev *od = (ops_dev *)op;
+  if (!op) return 0;
+  wbfmm_hip_free(od->d_rot);
+  wbfmm_hip_free(od->d_lap_coax);
+  free(od);
+  return 0;
+}
+
+gint wbfmm_laplace_coaxial_translate_init(gint N)
+{
+  if (N > g_lap_coax_N) g_lap_coax_N = N;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Laplace passes                                                     */
+/* ------------------------------------------------------------------ */
+
+gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                               guint level, gdouble *work)
+{
+  tree_dev *td = TD(t);
+  ops_dev *od = (ops_dev *)op;
+  (void)work;
+  if (level <= 1) WBFMM_FATAL("level %u <= 1", level);
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (!t->mps[level] || !t->mps[level - 1])
+    WBFMM_FATAL("coefficients not initialised at levels %u, %u", level, level - 1);
+  if ((int)t->order_s[level] > od->lay.L || (int)t->order_s[level - 1] > od->lay.L)
+    WBFMM_FATAL("operators built to order %d, tree needs %u", od->lay.L,
+                t->order_s[level] > t->order_s[level - 1] ? t->order_s[level] : t->order_s[level - 1]);
+  ensure_lap_coax(od);
+  HIPOK(wbfmm_hip_lap_m2m(td->d_occ[level - 1], td->nocc[level - 1],
+                          td->d_start[level], td->d_end[level], level, t->D,
+                          t->order_s[level], t->order_s[level - 1],
+                          (double *)t->mps[level], (double *)t->mps[level - 1],
+                          od->d_rot, &od->lay, od->d_lap_coax));
+  return 0;
+}
+
+/* all boxes of a level, used as the target list when no target list is
+   registered (every box can hold a target) */
+static uint32_t *g_all_boxes = NULL;
+static unsigned g_all_n = 0;
+
+static uint32_t *all_boxes(unsigned level, unsigned *n)
+{
+  unsigned nb = 1u << (3 * level);
+  if (nb > g_all_n) {
+    uint32_t *h = malloc(sizeof(uint32_t) * nb);
+    unsigned i;
+    for (i = 0; i < nb; i++) h[i] = i;
+    if (g_all_boxes) wbfmm_hip_free(g_all_boxes);
+    HIPOK(wbfmm_hip_malloc((void **)&g_all_boxes, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_memcpy(g_all_boxes, h, sizeof(uint32_t) * nb));
+    free(h);
+    g_all_n = nb;
+  }
+  *n = nb;
+  return g_all_boxes;
+}
+
+static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                         guint level)
+{
+  tree_dev *td = TD(t);
+  ops_dev *od = (ops_dev *)op;
+  uint32_t *tl;
+  unsigned nt;
+  if (level <= 1) WBFMM_FATAL("level %u <= 1", level);
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (!t->mpr[level] || !t->mps[level])
+    WBFMM_FATAL("coefficients not initialised at level %u", level);
+  if ((int)t->order_s[level] > od->lay.L || (int)t->order_r[level] > od->lay.L)
+    WBFMM_FATAL("operators built to order %d", od->lay.L);
+  ensure_lap_coax(od);
+  if (td->d_act[level]) {
+    tl = td->d_act[level];
+    nt = td->nact[level];
+  } else {
+    tl = all_boxes(level, &nt);
+  }
+  HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
+                          t->D, t->order_s[level], t->order_r[level],
+                          (double *)t->mps[level], (double *)t->mpr[level],
+                          od->d_rot, &od->lay, od->d_lap_coax));
+  if (level == t->depth) return 0;
+  if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
+  if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
+  HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->d_flag[level + 1], level, t->D,
+                          t->order_r[level], t->order_r[level + 1],
+                          (double *)t->mpr[level], (double *)t->mpr[level + 1],
+                          od->d_rot, &od->lay, od->d_lap_coax));
+  return 0;
+}
+
+gint wbfmm_laplace_downward_pass_ref(wbfmm_tree_t *t,
+                                     wbfmm_shift_operators_t *op,
+                                     guint level, gdouble *work,
+                                     gint nthreads)
+{
+  (void)work;
+  (void)nthreads;
+  return lap_downward(t, op, level);
+}
+
+gint wbfmm_laplace_downward_pass_avx(wbfmm_tree_t *t,
+                                     wbfmm_shift_operators_t *op,
+                                     guint level, gdouble *work,
+                                     gint nthreads)
+{
+  (void)work;
+  (void)nthreads;
+  return lap_downward(t, op, level);
+}
+
+/* stage n points (host or device, byte stride) as 4-double device rows
+   with identity ids; returns device buffers that the caller frees */
+static void stage_targets(const void *xt, size_t tstr, unsigned n,
+                          double **dx, uint32_t **did, uint64_t **dcodes,
+                          const double *x0, double D, int sort_them)
+{
+  unsigned nbad = 0;
+  double *raw;
+  HIPOK(wbfmm_hip_malloc((void **)&raw, sizeof(double) * 4 * (n ? n : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)dx, sizeof(double) * 4 * (n ? n : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)did, sizeof(uint32_t) * (n ? n : 1)));
+  HIPOK(wbfmm_hip_malloc((void **)dcodes, sizeof(uint64_t) * (n ? n : 1)));
+  HIPOK(wbfmm_hip_morton(xt, tstr, n, x0, D, raw, *dcodes, *did, &nbad));
+  if (nbad)
+    WBFMM_FATAL("%u target points do not lie in the tree's domain", nbad);
+  if (sort_them) HIPOK(wbfmm_hip_sort(*dcodes, *did, n));
+  HIPOK(wbfmm_hip_gather4(raw, *did, n, *dx));
+  HIPOK(wbfmm_hip_sync());
+  wbfmm_hip_free(raw);
+}
+
+/* device view of a caller field array: f itself when it is device memory,
+   else a staged copy that is written back by field_finish */
+static double *field_begin(double *f, size_t nelem, int *staged)
+{
+  double *d;
+  if (wbfmm_hip_is_device(f)) { *staged = 0; return f; }
+  HIPOK(wbfmm_hip_malloc((void **)&d, sizeof(double) * (nelem ? nelem : 1)));
+  HIPOK(wbfmm_hip_memcpy(d, f, sizeof(double) * nelem));
+  *staged = 1;
+  return d;
+}
+
+static void field_finish(double *f, double *d, size_t nelem, int staged)
+{
+  if (!staged) return;
+  HIPOK(wbfmm_hip_memcpy(f, d, sizeof(double) * nelem));
+  wbfmm_hip_free(d);
+}
+
+static void lap_field_run(tree_dev *td, const double *dxt,
+                          const uint32_t *dtid, unsigned n, double *src,
+                          gint sstr, double *d, gint dstr, int field,
+                          double *f, gint fstr, size_t nf_rows)
+{
+  wbfmm_tree_t *t = &td->t;
+  unsigned lv = t->depth;
+  double *q, *dp, *df;
+  int staged;
+  size_t nelem;
+  if (!t->mpr[lv]) WBFMM_FATAL("coefficients not initialised at level %u", lv);
+  if (t->normals == NULL && d != NULL)
+    WBFMM_FATAL("no normals in tree but dipole strengths specified (d != NULL)");
+  q = gather_strengths(td, src, sstr, 0);
+  dp = gather_strengths(td, d, dstr, 1);
+  nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((field & 1) ? 1 : 0) + ((field & 2) ? 3 : 0) : 0;
+  df = field_begin(f, nelem, &staged);
+  HIPOK(wbfmm_hip_lap_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
+                            td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
+                            (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
+                            (double *)t->mpr[lv], field, df, fstr));
+  field_finish(f, df, nelem, staged);
+}
+
+gint wbfmm_laplace_field_targets(wbfmm_tree_t *t, gdouble *xt, gsize tstr,
+                                 guint ntgt, gdouble *src, gint sstr,
+                                 gdouble *d, gint dstr, guint field,
+                                 gdouble *f, gint fstr)
+{
+  tree_dev *td = TD(t);
+  double *dx;
+  uint32_t *did;
+  uint64_t *dc;
+  int fl = field & 3;
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) return -2;
+  if (fl == 0 || (field & ~3u)) return -2;
+  if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0)) return -2;
+  if (ntgt == 0) return 0;
+  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, (double *)wbfmm_tree_origin(t),
+                t->D, 1);
+  lap_field_run(td, dx, did, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
+  HIPOK(wbfmm_hip_sync());
+  wbfmm_hip_free(dx);
+  wbfmm_hip_free(did);
+  wbfmm_hip_free(dc);
+  return 0;
+}
+
+gint wbfmm_laplace_box_field(wbfmm_tree_t *t, guint level, guint b,
+                             gdouble *src, gint sstr, gdouble *d, gint dstr,
+                             guint field, gboolean eval_neighbours,
+                             gdouble *x, gdouble *f, gint fstr,
+                             gdouble *work)
+{
+  tree_dev *td = TD(t);
+  double *dx;
+  uint32_t *did;
+  uint64_t *dc;
+  int fl;
+  (void)work;
+  (void)b;
+  if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (level != t->depth) WBFMM_FATAL("field evaluation is at the leaf level only");
+  switch (field) {
+    case WBFMM_FIELD_POTENTIAL: fl = 1; break;
+    case WBFMM_FIELD_GRADIENT:
+    case WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 2; break;
+    default: WBFMM_FATAL("unsupported field type %u", field);
+  }
+  if (fl == 2 && fstr < 3) WBFMM_FATAL("field data stride (%d) must be greater than two", fstr);
+  stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc,
+                (double *)wbfmm_tree_origin(t), t->D, 0);
+  if (!eval_neighbours) { src = NULL; d = NULL; }
+  {
+    /* f holds 1 (potential) or 3 (gradient) values at unit stride */
+    double tmp[4] = {0, 0, 0, 0};
+    int k;
+    lap_field_run(td, dx, did, 1, src, sstr, d, dstr, fl, tmp, 4, 1);
+    if (fl == 1) f[0] += tmp[0];
+    else for (k = 0; k < 3; k++) f[k] += tmp[k];
+  }
+  wbfmm_hip_free(dx);
+  wbfmm_hip_free(did);
+  wbfmm_hip_free(dc);
+  return 0;
+}
+
+gint wbfmm_laplace_field_direct(gdouble *xs, gint xstride, gdouble *n,
+                                gint nstr, gint nsrc, gdouble *src,
+                                gint sstr, gdouble *d, gint dstr, gint nq,
+                                guint field, gdouble *xf, gdouble *f,
+                                gint fstr)
+{
+  /* FP64 direct sum on the GPU for one field point, as the reference's
+     wbfmm_laplace_field_direct (src/direct-laplace.c:274-336) */
+  double *dxs, *dq = NULL, *dd = NULL, *dn = NULL, *dxt, *df;
+  uint32_t *ids = NULL;
+  int fl, i;
+  double tmp[4] = {0, 0, 0, 0};
+  if (nq != 1) WBFMM_FATAL("only nq == 1 is supported");
+  switch (field) {
+    case WBFMM_FIELD_POTENTIAL: fl = 1; break;
+    case WBFMM_FIELD_GRADIENT:
+    case WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 2; break;
+    default: WBFMM_FATAL("unsupported field type %u", field);
+  }
+  (void)fstr;
+  HIPOK(wbfmm_hip_malloc((void **)&dxs, sizeof(double) * 4 * (nsrc ? nsrc : 1)));
+  HIPOK(wbfmm_hip_stage4(xs, xstride * sizeof(double), nsrc, dxs));
+  if (src) {
+    HIPOK(wbfmm_hip_malloc((void **)&dq, sizeof(double) * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_malloc((void **)&ids, sizeof(uint32_t) * (nsrc ? nsrc : 1)));
+  }
+  if (src || d) {
+    uint32_t *h = malloc(sizeof(uint32_t) * (nsrc ? nsrc : 1));
+    gint i2;
+    for (i2 = 0; i2 < nsrc; i2++) h[i2] = i2;
+    if (!ids) HIPOK(wbfmm_hip_malloc((void **)&ids, sizeof(uint32_t) * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_memcpy(ids, h, sizeof(uint32_t) * nsrc));
+    free(h);
+  }
+  if (src) HIPOK(wbfmm_hip_gather_strided(src, sstr, 1, ids, nsrc, dq));
+  if (d) {
+    HIPOK(wbfmm_hip_malloc((void **)&dd, sizeof(double) * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_gather_strided(d, dstr, 1, ids, nsrc, dd));
+    HIPOK(wbfmm_hip_malloc((void **)&dn, sizeof(double) * 4 * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_stage4(n, nstr * sizeof(double), nsrc, dn));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&dxt, sizeof(double) * 4));
+  {
+    double h[4] = {xf[0], xf[1], xf[2], 0.0};
+    HIPOK(wbfmm_hip_memcpy(dxt, h, sizeof(h)));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&df, sizeof(double) * 4));
+  HIPOK(wbfmm_hip_memset(df, 0, sizeof(double) * 4));
+  HIPOK(wbfmm_hip_lap_direct(dxs, dq, dn, dd, nsrc, dxt, 1, fl, 0.0, df, 4));
+  HIPOK(wbfmm_hip_memcpy(tmp, df, sizeof(double) * 4));
+  if (fl == 1) f[0] += tmp[0];
+  else for (i = 0; i < 3; i++) f[i] += tmp[i];
+  wbfmm_hip_free(dxs); wbfmm_hip_free(ids);
+  wbfmm_hip_free(dq); wbfmm_hip_free(dd); wbfmm_hip_free(dn);
+  wbfmm_hip_free(dxt); wbfmm_hip_free(df);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* target lists                                                       */
+/* ------------------------------------------------------------------ */
+
+wbfmm_target_list_t *wbfmm_target_list_new(wbfmm_tree_t *t, guint npts)
+{
+  tlist_dev *ld = calloc(1, sizeof(tlist_dev));
+  wbfmm_target_list_t *l = &ld->l;
+  l->t = t;
+  l->maxpoints = npts;
+  l->npoints = 0;
+  l->size = sizeof(double);
+  l->complex = (t->problem == WBFMM_PROBLEM_HELMHOLTZ);
+  return l;
+}
+
+/* mark the boxes holding targets at every level and rebuild the active
+   lists (boxes with sources or targets) the downward pass runs over */
+static void register_targets(tree_dev *td, const uint64_t *dcodes, unsigned n)
+{
+  unsigned l;
+  for (l = 0; l <= td->t.depth; l++) {
+    size_t nb = (size_t)1 << (3 * l);
+    if (!td->d_flag[l]) {
+      HIPOK(wbfmm_hip_malloc((void **)&td->d_flag[l], sizeof(uint32_t) * nb));
+      HIPOK(wbfmm_hip_malloc((void **)&td->d_act[l], sizeof(uint32_t) * nb));
+    }
+    HIPOK(wbfmm_hip_memset(td->d_flag[l], 0, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_mark_boxes(td->d_codes, td->t.npoints, l, td->d_flag[l]));
+    HIPOK(wbfmm_hip_mark_boxes(dcodes, n, l, td->d_flag[l]));
+    HIPOK(wbfmm_hip_compact_boxes(td->d_start[l], td->d_end[l], td->d_flag[l],
+                                  l, td->d_act[l], &td->nact[l]));
+  }
+}
+
+gint wbfmm_target_list_add_points(wbfmm_target_list_t *l, gpointer pts,
+                                  gsize pstr, guint npts)
+{
+  tlist_dev *ld = (tlist_dev *)l;
+  tree_dev *td = TD(l->t);
+  if (npts > l->maxpoints)
+    WBFMM_FATAL("too many points (%u) for target list (%u)", npts, l->maxpoints);
+  if (ld->d_xt) {
+    wbfmm_hip_free(ld->d_xt);
+    wbfmm_hip_free(ld->d_tid);
+    wbfmm_hip_free(ld->d_codes);
+  }
+  l->points = (char *)pts;
+  l->pstr = pstr;
+  l->npoints = npts;
+  stage_targets(pts, pstr, npts, &ld->d_xt, &ld->d_tid, &ld->d_codes,
+                (double *)wbfmm_tree_origin(l->t), l->t->D, 1);
+  register_targets(td, ld->d_codes, npts);
+  return 0;
+}
+
+gint wbfmm_target_list_coefficients_init(wbfmm_target_list_t *l, guint field)
+{
+  l->field = field;
+  return 0;
+}
+
+gint wbfmm_laplace_target_list_local_coefficients(wbfmm_target_list_t *l,
+                                                  guint source, gdouble *work)
+{
+  (void)work;
+  l->source = source;
+  if (l->t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  return 0;
+}
+
+gint wbfmm_target_list_local_field(wbfmm_target_list_t *l, gdouble *src,
+                                   gint sstr, gdouble *d, gint dstr,
+                                   gdouble *f, gint fstr)
+{
+  tlist_dev *ld = (tlist_dev *)l;
+  tree_dev *td = TD(l->t);
+  int fl = (int)(l->field & 3);
+  if (fl == 0) WBFMM_FATAL("unsupported field type %u", l->field);
+  if (l->t->problem != WBFMM_PROBLEM_LAPLACE)
+    WBFMM_FATAL("only Laplace target lists are implemented");
+  if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0))
+    WBFMM_FATAL("field stride %d too small", fstr);
+  lap_field_run(td, ld->d_xt, ld->d_tid, l->npoints, src, sstr, d, dstr, fl,
+                f, fstr, l->npoints);
+  return 0;
+}
+
+gint wbfmm_target_list_free(wbfmm_target_list_t *l)
+{
+  tlist_dev *ld = (tlist_dev *)l;
+  if (!l) return 0;
+  wbfmm_hip_free(ld->d_xt);
+  wbfmm_hip_free(ld->d_tid);
+  wbfmm_hip_free(ld->d_codes);
+  free(ld);
+  return 0;
+}
+
+gint wbfmm_tree_free(wbfmm_tree_t *t)
+{
+  tree_dev *td = TD(t);
+  int l;
+  if (!t) return 0;
+  wbfmm_hip_sync();
+  for (l = 0; l <= WBFMM_TREE_MAX_DEPTH; l++) {
+    wbfmm_hip_free(td->d_start[l]);
+    wbfmm_hip_free(td->d_end[l]);
+    wbfmm_hip_free(td->d_occ[l]);
+    wbfmm_hip_free(td->d_flag[l]);
+    wbfmm_hip_free(td->d_act[l]);
+    wbfmm_hip_free(t->mps[l]);
+    wbfmm_hip_free(t->mpr[l]);
+    free(t->boxes[l]);
+  }
+  wbfmm_hip_free(td->d_xyz);
+  wbfmm_hip_free(td->d_xyz_raw);
+  wbfmm_hip_free(td->d_nrm);
+  wbfmm_hip_free(td->d_codes);
+  wbfmm_hip_free(td->d_idx);
+  wbfmm_hip_free(td->d_q);
+  wbfmm_hip_free(td->d_dip);
+  free(t->ip);
+  free(td);
+  return 0;
+}
+
+gint wbfmm_hip_synchronize(void) { return wbfmm_hip_sync(); }
