@@ -1,0 +1,247 @@
+"""Benchmark: Laplace FMM in FP64 through the drop-in API on one MI355X.
+
+Workload (the plain-Laplace GPU drop-in configuration): 10^6 sources
+uniform in the unit cube (SplitMix64 stream, seed 3), real charges iid
+N(0,1), octree depth 7, expansion order 12 at every level, targets = the
+source points, potential plus gradient.  A step is one whole solve with the
+inputs already resident in HBM: tree build (Morton codes, sort, box
+ranges, coefficient arrays, target list), P2M, upward pass (M2M), downward
+pass (M2L + L2L) and field evaluation (L2P + near field), in the reference
+driver's call order (tools/wbfmm-laplace-tree-calc.c.in:380-460).
+
+Prints one JSON line: targets/s, per-step latency p50/p99, the relative
+error against an FP64 direct sum on sampled targets, the roofline entry of
+the dominant kernel (M2L) measured with HIP events on the library stream,
+and a CPU baseline from the oracle on a bounded sample of the same
+workload.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def splitmix64(seed, n):
+    """SplitMix64 stream, vectorised: n uint64 values."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, n):
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, n):
+    """Box-Muller on two SplitMix64 streams."""
+    u1 = uniform(seed, n)
+    u2 = uniform(seed + 0x5bd1e995, n)
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def workload(n=1_000_000, seed=3):
+    xs = uniform(seed, 3 * n).reshape(n, 3)
+    q = normal(seed + 17, n)
+    lo, hi = xs.min(0), xs.max(0)
+    x0 = lo - 0.01
+    D = float((hi - lo).max()) + 0.02
+    return xs, q, x0, D
+
+
+def m2l_pairs(xs, x0, D, depth):
+    """Occupied-target x occupied-source list-4 pairs per level (targets =
+    sources).  A target at grid (i,j,k) interacts with the sources in the
+    6x6x6 block of its parent's neighbours' children, near neighbours
+    excluded."""
+    pairs = {}
+    for l in range(2, depth + 1):
+        nb = 1 << l
+        g = np.floor((xs - x0) / D * nb).astype(np.int64)
+        g = np.clip(g, 0, nb - 1)
+        occ = np.zeros((nb + 6, nb + 6, nb + 6), dtype=np.int32)
+        occ[g[:, 0] + 3, g[:, 1] + 3, g[:, 2] + 3] = 1
+        core = occ[3:nb + 3, 3:nb + 3, 3:nb + 3]
+        tot = 0
+        for px in range(2):
+            for py in range(2):
+                for pz in range(2):
+                    t = core[px::2, py::2, pz::2]
+                    for dx in range(-2 - px, 4 - px):
+                        for dy in range(-2 - py, 4 - py):
+                            for dz in range(-2 - pz, 4 - pz):
+                                if abs(dx) <= 1 and abs(dy) <= 1 and abs(dz) <= 1:
+                                    continue
+                                s = occ[3 + px + dx:3 + px + dx + nb:2,
+                                        3 + py + dy:3 + py + dy + nb:2,
+                                        3 + pz + dz:3 + pz + dz + nb:2]
+                                tot += int(np.sum(t * s))
+        pairs[l] = tot
+    return pairs
+
+
+def cpu_baseline(p, seconds_hint=20.0):
+    """Oracle (CPU restatement of the reference path) on a bounded sample
+    with the same leaf density: 1e6/64 sources, depth 5, same order, M2L
+    threaded over the host cores like the reference's downward pass."""
+    import oracle
+    n = 1_000_000 // 64
+    xs, q, x0, D = workload(n, seed=3)
+    nth = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    f, ph, _ = oracle.fmm(1, xs, xs, x0, D, 5, [p] * 17, [p] * 17, q=q, field=3, nthreads=nth)
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "targets/s", "cores": nth, "kind": "port",
+            "sample": f"oracle FMM, {n} uniform sources = targets, depth 5 (same leaf "
+                      f"density as 1e6 at depth 7), order {p}, potential+gradient, "
+                      f"{el:.1f} s wall, M2L on {nth} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--depth", type=int, default=7)
+    ap.add_argument("--order", type=int, default=12)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check-targets", type=int, default=2000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        sys.exit(2)
+
+    import wbfmm_amd as W
+    L = W.lib()
+    n, depth, p = args.n, args.depth, args.order
+    xs, q, x0, D = workload(n)
+    print(f"bench: workload ready n={n} depth={depth} p={p}", file=sys.stderr, flush=True)
+
+    # inputs resident in HBM before the timed region
+    d_xs = W.DeviceBuffer.from_array(xs)
+    d_q = W.DeviceBuffer.from_array(q)
+    d_f = W.DeviceBuffer(n * 4 * 8)
+    S = W.LaplaceSolver(d_xs, x0, D, depth, [p] * 17, [p] * 17, targets=d_xs,
+                        ntgt=n, nsrc=n, host_mirror=False)
+
+    def step():
+        d_f.zero()
+        S.build_tree()
+        S.passes(d_q)
+        S.field(d_q, d_f, field=3, fstr=4)
+
+    for _ in range(args.warmup):
+        step()
+    W.sync()
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    try:
+        import torch
+        torch_sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        torch_sync = lambda: None
+
+    W.Timing.reset()
+    W.Timing.enable(True)
+    lat = []
+    torch_sync()
+    W.sync()
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        W.sync()
+        lat.append(time.perf_counter() - t0)
+    W.sync()
+    torch_sync()
+    total = time.perf_counter() - t_all
+    W.Timing.enable(False)
+    kern = {}
+    for name in ("lap_m2l", "lap_m2m", "lap_l2l", "lap_p2m", "lap_l2p_p2p", "tree_sort",
+                 "tree_morton", "tree_box_ranges", "tree_box_flags", "tree_box_scatter",
+                 "tree_mark_boxes", "gather4", "gather_strength", "stage4"):
+        cnt, ms = W.Timing.query(name)
+        if cnt:
+            kern[name] = {"launches_per_step": cnt / args.steps, "ms_per_step": ms / args.steps,
+                          "avg_ms": ms / cnt}
+    print(f"bench: timed {args.steps} steps, {1e3 * total / args.steps:.2f} ms/step",
+          file=sys.stderr, flush=True)
+
+    # accuracy against the FP64 direct sum on sampled targets
+    f = d_f.download(np.zeros((n, 4)))
+    rng = np.random.default_rng(99)
+    idx = np.sort(rng.choice(n, args.check_targets, replace=False))
+    d_src4 = W.DeviceBuffer(n * 32)
+    L.wbfmm_hip_stage4(d_xs.ptr, 24, n, d_src4.ptr)
+    d_t4 = W.DeviceBuffer(len(idx) * 32)
+    L.wbfmm_hip_stage4(W.DeviceBuffer.from_array(np.ascontiguousarray(xs[idx])).ptr, 24,
+                       len(idx), d_t4.ptr)
+    d_fd = W.DeviceBuffer(len(idx) * 32)
+    d_fd.zero()
+    L.wbfmm_hip_lap_direct(d_src4.ptr, d_q.ptr, None, None, n, d_t4.ptr, len(idx), 3, 1e-6,
+                           d_fd.ptr, 4)
+    fd = d_fd.download(np.zeros((len(idx), 4)))
+    err_pot = float(np.linalg.norm(f[idx, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0]))
+    err_grad = float(np.linalg.norm(f[idx, 1:] - fd[:, 1:]) / np.linalg.norm(fd[:, 1:]))
+    maxd = [float(np.abs(f[idx, c] - fd[:, c]).max() / np.abs(fd[:, c]).max()) for c in range(4)]
+
+    # roofline of the dominant kernel
+    pairs = m2l_pairs(xs, x0, D, depth)
+    R = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
+    C = (p + 1) ** 2 + 2 * sum((p + 1 - m) ** 2 for m in range(1, p + 1))
+    flops_pair = 2 * (2 * R) + 2 * C
+    m2l_flops_step = flops_pair * sum(pairs.values())
+    m2l = kern.get("lap_m2l", {"ms_per_step": float("nan"), "launches_per_step": 1})
+    achieved = m2l_flops_step / (m2l["ms_per_step"] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "lap_m2l", "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
+                "pairs_per_step": int(sum(pairs.values())), "flops_per_pair": flops_pair}
+
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
+        "value": n * args.steps / total,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 3, uniform cube, N(0,1) charges)",
+        "config": {"workload": "laplace_fp64_dropin: 1e6 sources = targets, unit cube, "
+                               "depth 7, order 12, potential+gradient",
+                   "n_sources": n, "n_targets": n, "depth": depth, "order": p,
+                   "parallelism": "single GPU"},
+        "error_vs_fp64_direct": {"targets_checked": int(len(idx)), "rel_l2_potential": err_pot,
+                                 "rel_l2_gradient": err_grad, "max_rel_per_component": maxd},
+        "roofline": roofline,
+        "kernels": kern,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(p)
+    S.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -168,7 +168,10 @@ gint wbfmm_box_interaction_index(gint i, gint j, gint k);             /* src/wbf
 wbfmm_tree_t *wbfmm_tree_new(gdouble *x, gdouble D, guint maxpoints);
 /* src/wbfmm.h:810-813.  pts/normals: host or device, byte strides.  The
    pointers are kept (borrowed, as in the reference); a device copy of the
-   coordinates is taken, sorted by Morton index on the GPU (stable). */
+   coordinates is taken, sorted by Morton index on the GPU (stable).
+   Adding points to a tree that already holds some starts it afresh
+   (depth 0, target lists to be added again), so one tree and its device
+   buffers can serve repeated solves. */
 gint wbfmm_tree_add_points(wbfmm_tree_t *t, gpointer pts, gsize pstr,
                            gpointer normals, gsize nstr, guint npts,
                            gboolean sorted);

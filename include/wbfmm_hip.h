@@ -101,6 +101,45 @@ int wbfmm_hip_rotation_tables(int L, double **dev, wbfmm_rot_layout_t *lay);
    c[kind][(m*(L+1)+n)*(L+1)+nu], closed forms of src/laplace.c:542-725 */
 int wbfmm_hip_laplace_coaxial_tables(int L, double **dev);
 
+/* Plan of the MFMA form of the Laplace M2L at order N (rows padded to
+   16-row tiles; each tile's inputs span the whole degree blocks it
+   touches).  Stage 1 rotates into the offset's frame (n-major rows),
+   stage 2 applies the coaxial S|R shift (m-major rows), stage 3 rotates
+   back and accumulates (n-major rows). */
+#define WBFMM_M2L_MAXT 64
+#define WBFMM_M2L_MAXC 448
+typedef struct {
+  int N, C, NT, CP, T13, T2;
+  int kb1[WBFMM_M2L_MAXT], nk1[WBFMM_M2L_MAXT], off1[WBFMM_M2L_MAXT];
+  int kb2[WBFMM_M2L_MAXT], nk2[WBFMM_M2L_MAXT], off2[WBFMM_M2L_MAXT];
+  int mpos[WBFMM_M2L_MAXC];  /* n-major row -> m-major row */
+  int npos[WBFMM_M2L_MAXC];  /* m-major row -> n-major row */
+  int degn[WBFMM_M2L_MAXC];  /* degree of an n-major row */
+  int degm[WBFMM_M2L_MAXC];  /* degree of an m-major row */
+  double r[343];             /* offset lengths in box widths */
+} wbfmm_m2l_plan_t;
+/* builds the plan and the MFMA A-operand tables (in lane order) for
+   orders up to WBFMM_M2L_MAXC coefficients: A1/A3 343 slots x T13 x 64,
+   A2 T2 x 64; *plan_dev is a device copy of *plan */
+int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *plan, void **plan_dev,
+                           double **A1, double **A3, double **A2);
+/* MFMA M2L for target boxes grouped by Morton child digit: cls holds the
+   targets of class c at [cstart[c], cstart[c]+ccount[c]); 16 targets of
+   one class per wavefront, accumulating into mpr */
+int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
+                           const unsigned *ccount, const uint32_t *sstart,
+                           const uint32_t *send, unsigned level, double D,
+                           int N, const double *mps, double *mpr,
+                           const void *plan_dev, const double *A1,
+                           const double *A3, const double *A2);
+/* one v_mfma_f64_16x16x4f64 with A[i][k] = 1+i+100k, B[k][j] = 1+3k+1000j;
+   host_out (256 doubles) gets the four result values of each lane */
+int wbfmm_hip_mfma_layout_probe(double *host_out);
+/* sort a box list by Morton child digit (stable); returns the per-class
+   starts and counts on the host */
+int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,
+                            unsigned *cstart, unsigned *ccount);
+
 /* ---- Laplace passes ------------------------------------------------------ */
 /* P2M into mps at `level` for the listed leaf boxes (src/laplace.c:854-989) */
 int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,

@@ -81,6 +81,17 @@ def lib():
                                      ctypes.c_int, ctypes.c_double, _dp]
         L.orc_fmm.argtypes = [ctypes.POINTER(FmmCfg), ctypes.c_int, _dp, _dp, _dp, _dp,
                               ctypes.c_int, _dp, _dp]
+        L.orc_hel_p2m.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_hel_p2m_normal.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_hel_h_eval.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_hel_j_eval.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_hel_coaxial_table.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_hel_coaxial.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double]
+        L.orc_hel_rotate.argtypes = [_dp, _dp, ctypes.c_int, _dp, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double]
+        L.orc_hel_direct.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                     ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -163,6 +174,69 @@ def lap_direct(xs, q, xt, field=3, nrm=None, d=None, cutoff=0.0):
                          P(None if d is None else np.ascontiguousarray(d, float)),
                          len(xt), P(xt), field, cutoff, P(f))
     return f
+
+
+def _c(a):
+    return np.ascontiguousarray(a, float)
+
+
+def hel_p2m(k, N, x0, xs, q, M=None):
+    """q complex scalar"""
+    M = np.zeros(2 * (N + 1) ** 2) if M is None else M
+    lib().orc_hel_p2m(k, N, P(_c(x0)), P(_c(xs)), P(_c([q.real, q.imag])), P(M))
+    return M
+
+
+def hel_p2m_normal(k, N, x0, xs, nrm, d, M=None):
+    M = np.zeros(2 * (N + 1) ** 2) if M is None else M
+    lib().orc_hel_p2m_normal(k, N, P(_c(x0)), P(_c(xs)), P(_c(nrm)), P(_c([d.real, d.imag])), P(M))
+    return M
+
+
+def hel_h_eval(k, N, x0, M, xf):
+    f = np.zeros(2)
+    lib().orc_hel_h_eval(k, N, P(_c(x0)), P(M), P(_c(xf)), P(f))
+    return f[0] + 1j * f[1]
+
+
+def hel_j_eval(k, N, x0, L, xf, field=3):
+    f = np.zeros(8)
+    lib().orc_hel_j_eval(k, N, P(_c(x0)), P(_c(L)), P(_c(xf)), field, P(f))
+    return f[0::2] + 1j * f[1::2]
+
+
+def coaxial_size(L):
+    return (L + 1) * (L + 2) * (L + 3) // 6 + 8
+
+
+def hel_coaxial_table(kind, L, kr):
+    E = np.zeros(2 * ((L + 2) * (L + 3) * (L + 4) // 6))
+    lib().orc_hel_coaxial_table(kind, L, kr, P(E))
+    return E
+
+
+def hel_coaxial(Ci, Ni, No, E, L, is_complex):
+    Co = np.zeros(2 * (No + 1) ** 2)
+    lib().orc_hel_coaxial(P(Co), No, P(_c(Ci)), Ni, P(E), L, int(is_complex), 0.0)
+    return Co
+
+
+def hel_rotate(Ci, N, H, ph, ch):
+    Co = np.zeros(2 * (N + 1) ** 2)
+    lib().orc_hel_rotate(P(Co), P(_c(Ci)), N, P(H), ph, ch, 0.0)
+    return Co
+
+
+def hel_direct(k, xs, q, xt, field=3, nrm=None, d=None, cutoff=0.0):
+    """q, d complex arrays; returns (ntgt, 4) complex: potential, gradient"""
+    xs = _c(xs)
+    xt = _c(xt)
+    f = np.zeros((len(xt), 8))
+    qq = None if q is None else _c(np.stack([np.real(q), np.imag(q)], 1))
+    dd = None if d is None else _c(np.stack([np.real(d), np.imag(d)], 1))
+    lib().orc_hel_direct(k, len(xs), P(xs), P(qq), P(None if nrm is None else _c(nrm)), P(dd),
+                         len(xt), P(xt), field, cutoff, P(f))
+    return f[:, 0::2] + 1j * f[:, 1::2]
 
 
 def fmm(problem, xs, xt, x0, D, depth, order_s, order_r, q=None, nrm=None, d=None,

@@ -835,6 +835,336 @@ static void box_centre(const orc_tree *t, int level, uint64_t b, double *xc)
  * :380-460.
  * ====================================================================== */
 
+
+/* ======================================================================
+ * Helmholtz operators.  Coefficients complex, C_n^m at 2*HIDX(n,m),
+ * -n<=m<=n (src/wbfmm.h:242).  Kernel G = h0(kr)/(4pi) = e^{ikr}/(4pi ikr)
+ * (src/direct.c:58-70).
+ * ====================================================================== */
+
+#include <complex.h>
+typedef double complex cplx;
+
+/* R_n^m = j_n(kr) Pbar_n^|m| e^{i m phi} for n<=N, all m, plus (when grad)
+   the ingredients for n<=N+1 */
+static void hel_regular(double k, int N, const double *x0, const double *x,
+                        cplx *R)
+{
+  double r, C, S, cp, sp, P[(66 + 1) * (66 + 2) / 2], j[70];
+  int n, m;
+  sph(x0, x, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  orc_bessel_j(N, k * r, j);
+  for (n = 0; n <= N; n++)
+    for (m = 0; m <= n; m++) {
+      cplx e = cpow(cp + I * sp, m);
+      double a = j[n] * P[PIDX(n, m)];
+      R[HIDX(n, m)] = a * e;
+      R[HIDX(n, -m)] = a * conj(e);
+    }
+}
+
+/* P2M monopole (src/expansions.c:52-123): M_n^m += q conj(R_n^m(xs-x0)) */
+void orc_hel_p2m(double k, int N, const double *x0, const double *xs,
+                 const double *q, double *M)
+{
+  cplx R[(66 + 1) * (66 + 1)], qq = q[0] + I * q[1];
+  int i;
+  hel_regular(k, N, x0, xs, R);
+  for (i = 0; i < (N + 1) * (N + 1); i++) {
+    cplx v = qq * conj(R[i]);
+    M[2 * i] += creal(v);
+    M[2 * i + 1] += cimag(v);
+  }
+}
+
+/* derivative coefficients of the regular wave functions, G&D (2004)
+   eq. 3.7 (used by src/gradient.c:220-280):
+     dz R_n^m = k [a_{n-1}^m R_{n-1}^m - a_n^m R_{n+1}^m]
+     D+ R_n^m = k [b_{n+1}^{-m-1} R_{n+1}^{m+1} - b_n^m R_{n-1}^{m+1}]
+     D- R_n^m = k [b_{n+1}^{m-1} R_{n+1}^{m-1} - b_n^{-m} R_{n-1}^{m-1}]
+   with a, b the recursion coefficients of src/translations.c:182-203 */
+static double dab_a(int n, int m) { return (double)rec_a(n, m); }
+static double dab_b(int n, int m) { return (double)rec_b(n, m); }
+
+static cplx Rget(const cplx *R, int N, int n, int m)
+{
+  if (n < 0 || n > N || m < -n || m > n) return 0.0;
+  return R[HIDX(n, m)];
+}
+
+/* gradient of R_n^m at the point whose R table (to order N+1) is given */
+static void hel_regular_grad(double k, int N1, const cplx *R, int n, int m,
+                             cplx *gx, cplx *gy, cplx *gz)
+{
+  cplx dz = k * (dab_a(n - 1, m) * Rget(R, N1, n - 1, m) - dab_a(n, m) * Rget(R, N1, n + 1, m));
+  cplx dp = k * (dab_b(n + 1, -m - 1) * Rget(R, N1, n + 1, m + 1) - dab_b(n, m) * Rget(R, N1, n - 1, m + 1));
+  cplx dm = k * (dab_b(n + 1, m - 1) * Rget(R, N1, n + 1, m - 1) - dab_b(n, -m) * Rget(R, N1, n - 1, m - 1));
+  *gz = dz;
+  *gx = 0.5 * (dp + dm);
+  *gy = -0.5 * I * (dp - dm);
+}
+
+/* P2M normal dipole: the source d nrm.grad_x G = -d nrm.grad_y G
+   (src/local.c:155-166) expands as M_n^m += -d conj(nrm . grad R_n^m(xs-x0))
+   (src/expansions.c:545-631) */
+void orc_hel_p2m_normal(double k, int N, const double *x0, const double *xs,
+                        const double *nrm, const double *d, double *M)
+{
+  cplx R[(67 + 1) * (67 + 1)], dd = d[0] + I * d[1];
+  int n, m;
+  hel_regular(k, N + 1, x0, xs, R);
+  for (n = 0; n <= N; n++)
+    for (m = -n; m <= n; m++) {
+      cplx gx, gy, gz, v;
+      hel_regular_grad(k, N + 1, R, n, m, &gx, &gy, &gz);
+      v = -dd * conj(nrm[0] * gx + nrm[1] * gy + nrm[2] * gz);
+      M[2 * HIDX(n, m)] += creal(v);
+      M[2 * HIDX(n, m) + 1] += cimag(v);
+    }
+}
+
+/* singular evaluation (src/expansions.c:147-228):
+   f += sum M_n^m h_n(kr) Pbar_n^|m| e^{i m phi} */
+void orc_hel_h_eval(double k, int N, const double *x0, const double *M,
+                    const double *xf, double *f)
+{
+  double r, C, S, cp, sp, P[(66 + 1) * (66 + 2) / 2], h[2 * 70];
+  int n, m;
+  cplx acc = 0.0;
+  sph(x0, xf, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  orc_bessel_h(N, k * r, h);
+  for (n = 0; n <= N; n++) {
+    cplx hn = h[2 * n] + I * h[2 * n + 1];
+    for (m = -n; m <= n; m++) {
+      cplx e = cpow(cp + I * sp, m);
+      acc += (M[2 * HIDX(n, m)] + I * M[2 * HIDX(n, m) + 1]) * hn * P[PIDX(n, abs(m))] * e;
+    }
+  }
+  f[0] += creal(acc);
+  f[1] += cimag(acc);
+}
+
+/* regular evaluation (L2P): field&1 potential f[0..1], field&2 gradient
+   f[2..7] (src/expansions.c:275-344, src/gradient.c:282-399) */
+void orc_hel_j_eval(double k, int N, const double *x0, const double *L,
+                    const double *xf, int field, double *f)
+{
+  cplx R[(67 + 1) * (67 + 1)];
+  cplx pot = 0, gx = 0, gy = 0, gz = 0;
+  int n, m;
+  hel_regular(k, N + 1, x0, xf, R);
+  for (n = 0; n <= N; n++)
+    for (m = -n; m <= n; m++) {
+      cplx c = L[2 * HIDX(n, m)] + I * L[2 * HIDX(n, m) + 1];
+      if (field & 1) pot += c * R[HIDX(n, m)];
+      if (field & 2) {
+        cplx ax, ay, az;
+        hel_regular_grad(k, N + 1, R, n, m, &ax, &ay, &az);
+        gx += c * ax; gy += c * ay; gz += c * az;
+      }
+    }
+  if (field & 1) { f[0] += creal(pot); f[1] += cimag(pot); }
+  if (field & 2) {
+    f[2] += creal(gx); f[3] += cimag(gx);
+    f[4] += creal(gy); f[5] += cimag(gy);
+    f[6] += creal(gz); f[7] += cimag(gz);
+  }
+}
+
+#define CIDX(l, m, n) ((l) * ((l) + 1) * ((l) + 2) / 6 + (n) * ((n) + 1) / 2 + (m))
+
+/*
+ * coaxial coefficients (E|F)_{l n}^m, l >= n >= m >= 0, at distance kr
+ * (kind 0: R|R = S|S from j_l, kind 1: S|R from h_l), G&D 4.79/4.83/4.84,
+ * restating src/translations.c:205-409; computed in quad precision (the
+ * recursion runs to degree 2L).  E holds 2 doubles (re, im) per entry,
+ * entries CIDX(l,m,n) for l <= L.
+ */
+void orc_hel_coaxial_table(int kind, int L, double kr, double *E)
+{
+  int l, m, n, sw = 1, LL = 2 * L;
+  size_t ne = (size_t)CIDX(LL + 1, 0, 0) + 8;
+  qreal *er = calloc(ne, sizeof(qreal)), *ei = calloc(ne, sizeof(qreal));
+  qreal x, f0r[2 * 64 + 4], f0i[2 * 64 + 4];
+  if (kr < 0) { kr = -kr; sw = -1; }
+  x = (qreal)kr;
+  /* f_l(x): j_l (kind 0) or h_l = j_l + i y_l (kind 1), quad precision,
+     j_l by downward recursion normalised to j_0, y_l upward */
+  {
+    int top = LL + 40;
+    qreal jp = 0, jc = 1e-300Q, jm, *jj = calloc(top + 2, sizeof(qreal));
+    qreal y0 = -cosq(x) / x, y1 = -cosq(x) / (x * x) - sinq(x) / x, ym, yc, yn;
+    int i;
+    for (i = top; i >= 1; i--) {
+      jm = (2 * i + 1) / x * jc - jp;
+      jp = jc; jc = jm;
+      jj[i - 1] = jc;
+      if (fabsq(jc) > 1e250Q) {
+        int t2;
+        for (t2 = i - 1; t2 <= top; t2++) jj[t2] *= 1e-250Q;
+        jp *= 1e-250Q; jc *= 1e-250Q;
+      }
+    }
+    {
+      qreal scale = (sinq(x) / x) / jj[0];
+      for (i = 0; i <= LL + 1; i++) f0r[i] = jj[i] * scale;
+    }
+    free(jj);
+    ym = y0; yc = y1;
+    f0i[0] = y0;
+    if (LL >= 1) f0i[1] = y1;
+    for (i = 1; i < LL + 1; i++) {
+      yn = (2 * i + 1) / x * yc - ym;
+      ym = yc; yc = yn;
+      f0i[i + 1] = yc;
+    }
+  }
+  for (l = 0; l <= LL; l++) {
+    qreal sg = (sw < 0 && (l % 2)) ? -1 : 1;
+    er[CIDX(l, 0, 0)] = sg * sqrtq((qreal)(2 * l + 1)) * f0r[l];
+    ei[CIDX(l, 0, 0)] = kind == 1 ? sg * sqrtq((qreal)(2 * l + 1)) * f0i[l] : 0;
+  }
+  /* G&D 4.79, m = 0: (E|F)_{l,n+1}^0 from n-1, n and l+-1 */
+  m = 0;
+  for (n = 0; n <= L; n++)
+    for (l = n + 1; l <= LL - n; l++) {
+      qreal a1 = rec_a(n - 1, m), a2 = rec_a(n, m), a3 = rec_a(l, m), a4 = rec_a(l - 1, m);
+      qreal e1r = n >= 1 ? er[CIDX(l, m, n - 1)] : 0, e1i = n >= 1 ? ei[CIDX(l, m, n - 1)] : 0;
+      er[CIDX(l, m, n + 1)] = (a1 * e1r - a3 * er[CIDX(l + 1, m, n)] + a4 * er[CIDX(l - 1, m, n)]) / a2;
+      ei[CIDX(l, m, n + 1)] = (a1 * e1i - a3 * ei[CIDX(l + 1, m, n)] + a4 * ei[CIDX(l - 1, m, n)]) / a2;
+    }
+  /* G&D 4.84: (E|F)_{l,m+1}^{m+1} */
+  for (m = 0; m <= L; m++)
+    for (l = m + 1; l <= LL - m; l++) {
+      qreal a1 = rec_b(m + 1, -m - 1), a2 = rec_b(l, -m - 1), a3 = rec_b(l + 1, m);
+      er[CIDX(l, m + 1, m + 1)] = (a2 * er[CIDX(l - 1, m, m)] - a3 * er[CIDX(l + 1, m, m)]) / a1;
+      ei[CIDX(l, m + 1, m + 1)] = (a2 * ei[CIDX(l - 1, m, m)] - a3 * ei[CIDX(l + 1, m, m)]) / a1;
+    }
+  /* G&D 4.79 for m >= 1 */
+  for (m = 1; m <= L; m++)
+    for (n = m; n <= L; n++)
+      for (l = n + 1; l <= LL - n; l++) {
+        qreal a1 = rec_a(n - 1, m), a2 = rec_a(n, m), a3 = rec_a(l, m), a4 = rec_a(l - 1, m);
+        qreal e1r = (n - 1 >= m) ? er[CIDX(l, m, n - 1)] : 0;
+        qreal e1i = (n - 1 >= m) ? ei[CIDX(l, m, n - 1)] : 0;
+        er[CIDX(l, m, n + 1)] = (a1 * e1r - a3 * er[CIDX(l + 1, m, n)] + a4 * er[CIDX(l - 1, m, n)]) / a2;
+        ei[CIDX(l, m, n + 1)] = (a1 * e1i - a3 * ei[CIDX(l + 1, m, n)] + a4 * ei[CIDX(l - 1, m, n)]) / a2;
+      }
+  for (l = 0; l <= L; l++)
+    for (n = 0; n <= l; n++)
+      for (m = 0; m <= n; m++) {
+        E[2 * CIDX(l, m, n)] = (double)er[CIDX(l, m, n)];
+        E[2 * CIDX(l, m, n) + 1] = (double)ei[CIDX(l, m, n)];
+      }
+  free(er);
+  free(ei);
+}
+
+/*
+ * coaxial translation (src/translations.c:411-604):
+ *   Co_n^m = sc Co + sum_l T(n,l,|m|) Ci_l^m,
+ *   T = E_{l n}^{|m|} (l >= n), (-1)^{n+l} E_{n l}^{|m|} (l < n)
+ */
+void orc_hel_coaxial(double *Co, int No, const double *Ci, int Ni,
+                     const double *E, int L, int is_complex, double sc)
+{
+  int n, m, l;
+  (void)L;
+  for (n = 0; n <= No; n++)
+    for (m = -n; m <= n; m++) {
+      int am = abs(m);
+      cplx acc = 0;
+      for (l = am; l <= Ni; l++) {
+        int idx = (l >= n) ? CIDX(l, am, n) : CIDX(n, am, l);
+        double sg = (l < n && ((n + l) % 2)) ? -1.0 : 1.0;
+        cplx e = E[2 * idx] + I * (is_complex ? E[2 * idx + 1] : 0.0);
+        acc += sg * e * (Ci[2 * HIDX(l, m)] + I * Ci[2 * HIDX(l, m) + 1]);
+      }
+      if (sc == 0.0) {
+        Co[2 * HIDX(n, m)] = creal(acc);
+        Co[2 * HIDX(n, m) + 1] = cimag(acc);
+      } else {
+        Co[2 * HIDX(n, m)] = sc * Co[2 * HIDX(n, m)] + creal(acc);
+        Co[2 * HIDX(n, m) + 1] = sc * Co[2 * HIDX(n, m) + 1] + cimag(acc);
+      }
+    }
+}
+
+/* rotation (src/rotations.c:419-593):
+   Co_n^nu = sc Co + e^{-i nu ph} sum_m H_n^{nu,m} e^{i m ch} Ci_n^m,
+   H_n^{nu,-m} = H_n^{-nu,m} */
+void orc_hel_rotate(double *Co, const double *Ci, int N, const double *H,
+                    double ph, double ch, double sc)
+{
+  int n, m, nu;
+  for (n = 0; n <= N; n++)
+    for (nu = -n; nu <= n; nu++) {
+      cplx acc = 0;
+      for (m = -n; m <= n; m++) {
+        double h = (m >= 0) ? H[RIDX(nu, m, n)] : H[RIDX(-nu, -m, n)];
+        acc += h * cexp(I * m * ch) * (Ci[2 * HIDX(n, m)] + I * Ci[2 * HIDX(n, m) + 1]);
+      }
+      acc *= cexp(-I * nu * ph);
+      if (sc == 0.0) {
+        Co[2 * HIDX(n, nu)] = creal(acc);
+        Co[2 * HIDX(n, nu) + 1] = cimag(acc);
+      } else {
+        Co[2 * HIDX(n, nu)] = sc * Co[2 * HIDX(n, nu)] + creal(acc);
+        Co[2 * HIDX(n, nu) + 1] = sc * Co[2 * HIDX(n, nu) + 1] + cimag(acc);
+      }
+    }
+}
+
+/* FP64 direct sum (src/direct.c:35-294), G = h0(kr)/(4 pi), complex
+   monopoles q and normal dipoles d: f[8i..8i+7] = potential, gradient
+   (re,im pairs).  Pairs with r <= cutoff skipped. */
+void orc_hel_direct(double k, int nsrc, const double *xs, const double *q,
+                    const double *nrm, const double *d,
+                    int ntgt, const double *xt, int field, double cutoff,
+                    double *f)
+{
+  int i, j;
+  for (i = 0; i < ntgt; i++) {
+    cplx pot = 0, g[3] = {0, 0, 0};
+    for (j = 0; j < nsrc; j++) {
+      double rv[3] = {xt[3 * i] - xs[3 * j], xt[3 * i + 1] - xs[3 * j + 1],
+                      xt[3 * i + 2] - xs[3 * j + 2]};
+      double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+      cplx G, dG, ddG;
+      int a;
+      if (r <= cutoff) continue;
+      /* G(r) = e^{ikr}/(4 pi i k r); G' = dG/dr; G'' */
+      G = cexp(I * k * r) / (4.0 * M_PI * I * k * r);
+      dG = G * (I * k - 1.0 / r);
+      ddG = G * ((I * k - 1.0 / r) * (I * k - 1.0 / r) + 1.0 / (r * r));
+      if (q) {
+        cplx qq = q[2 * j] + I * q[2 * j + 1];
+        pot += qq * G;
+        for (a = 0; a < 3; a++) g[a] += qq * dG * rv[a] / r;
+      }
+      if (d) {
+        cplx dd = d[2 * j] + I * d[2 * j + 1];
+        const double *nv = nrm + 3 * j;
+        double nr = (nv[0] * rv[0] + nv[1] * rv[1] + nv[2] * rv[2]) / r;
+        /* the Helmholtz dipole is d n.grad_x G = d G'(r) n.(x-y)/r, i.e.
+           -k h1(kr) d n.(x-y)/(4 pi r) (src/local.c:155-166) */
+        pot += dd * dG * nr;
+        for (a = 0; a < 3; a++) {
+          double ra = rv[a] / r;
+          g[a] += dd * (ddG * nr * ra + dG * (nv[a] - nr * ra) / r);
+        }
+      }
+    }
+    if (field & 1) { f[8 * i] += creal(pot); f[8 * i + 1] += cimag(pot); }
+    if (field & 2) {
+      int a;
+      for (a = 0; a < 3; a++) { f[8 * i + 2 + 2 * a] += creal(g[a]); f[8 * i + 3 + 2 * a] += cimag(g[a]); }
+    }
+  }
+}
+
 #include <pthread.h>
 
 typedef struct {
@@ -1073,6 +1403,247 @@ static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
   return 0;
 }
 
+
+/* ======================================================================
+ * Helmholtz solve: src/boxes.c:341-490 (P2M), src/pass.c:581-632 with
+ * src/shift.c:244-401 (M2M), src/pass.c:56-105, :448-537 (M2L),
+ * src/shift.c:403-585 (L2L), src/local.c:32-262 (L2P + near field); call
+ * order of tools/wbfmm-tree-calc.c.in:306-417.  Translation tables per
+ * level: S|R at the 15 list-4 distances k w r_i, R|R (= S|S) at
+ * k w sqrt(3)/2 (src/shift.c:777-861).
+ * ====================================================================== */
+
+typedef struct {
+  const orc_tree *t;
+  const orc_fmm_cfg *cfg;
+  int level, nth, ith;
+  double **mps, **mpr;
+  double *Htab;
+  int nrot;
+  double **SR;      /* per offset slot (343), table or NULL */
+} hel_job;
+
+static void *hel_m2l_worker(void *arg)
+{
+  hel_job *J = arg;
+  const orc_tree *t = J->t;
+  int l = J->level, Ns = J->cfg->order_s[l], Nr = J->cfg->order_r[l];
+  int L = Ns > Nr ? Ns : Nr;
+  int ncs = 2 * (Ns + 1) * (Ns + 1), ncr = 2 * (Nr + 1) * (Nr + 1);
+  uint64_t nb = 1ull << (3 * l), b;
+  double *wks = malloc(sizeof(double) * 2 * (L + 1) * (L + 1));
+  double *wkr = malloc(sizeof(double) * 2 * (L + 1) * (L + 1));
+  uint64_t list[216];
+  int off[3 * 216];
+  for (b = J->ith; b < nb; b += J->nth) {
+    int n = orc_box_list4(l, b, list, off), j;
+    double *Lc = J->mpr[l] + b * ncr;
+    for (j = 0; j < n; j++) {
+      double th, ph, ch, r;
+      int o = (off[3 * j] + 3) * 49 + (off[3 * j + 1] + 3) * 7 + off[3 * j + 2] + 3;
+      if (t->bcount[l][list[j]] == 0) continue;
+      orc_shift_angles(off[3 * j], off[3 * j + 1], off[3 * j + 2], &th, &ph, &ch, &r);
+      orc_hel_rotate(wks, J->mps[l] + list[j] * ncs, Ns, J->Htab + (size_t)o * J->nrot, ph, ch, 0.0);
+      orc_hel_coaxial(wkr, Nr, wks, Ns, J->SR[o], L, 1, 0.0);
+      orc_hel_rotate(Lc, wkr, Nr, J->Htab + (size_t)o * J->nrot, ch, ph, 1.0);
+    }
+  }
+  free(wks); free(wkr);
+  return NULL;
+}
+
+static int hel_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
+                     const double *q, const double *nrm, const double *d,
+                     int ntgt, const double *xt, double *f)
+{
+  int depth = t->depth, l, Lmax = 0, nrot, i;
+  double *mps[17] = {0}, *mpr[17] = {0}, *Htab, tm, k = cfg->k;
+  double *Hud[2][8];
+  for (l = 2; l <= depth; l++) {
+    if (cfg->order_s[l] > Lmax) Lmax = cfg->order_s[l];
+    if (cfg->order_r[l] > Lmax) Lmax = cfg->order_r[l];
+  }
+  Htab = offset_rotations(Lmax, &nrot);
+  for (i = 0; i < 8; i++) {
+    double th, ph, ch;
+    Hud[0][i] = malloc(sizeof(double) * nrot);
+    Hud[1][i] = malloc(sizeof(double) * nrot);
+    child_angles(i, 1, &th, &ph, &ch); orc_H_rotation(Hud[0][i], Lmax, th);
+    child_angles(i, -1, &th, &ph, &ch); orc_H_rotation(Hud[1][i], Lmax, th);
+  }
+  for (l = 2; l <= depth; l++) {
+    size_t nb = (size_t)1 << (3 * l);
+    mps[l] = calloc(nb * 2 * (cfg->order_s[l] + 1) * (cfg->order_s[l] + 1), sizeof(double));
+    mpr[l] = calloc(nb * 2 * (cfg->order_r[l] + 1) * (cfg->order_r[l] + 1), sizeof(double));
+  }
+  tm = now_s();
+  {
+    int Ns = cfg->order_s[depth], ncs = 2 * (Ns + 1) * (Ns + 1);
+    uint64_t nb = 1ull << (3 * depth), b;
+    for (b = 0; b < nb; b++) {
+      double xc[3];
+      int j;
+      if (t->bcount[depth][b] == 0) continue;
+      box_centre(t, depth, b, xc);
+      for (j = 0; j < t->bcount[depth][b]; j++) {
+        int s = t->ip[t->bstart[depth][b] + j];
+        if (q) orc_hel_p2m(k, Ns, xc, xs + 3 * s, q + 2 * s, mps[depth] + b * ncs);
+        if (d) orc_hel_p2m_normal(k, Ns, xc, xs + 3 * s, nrm + 3 * s, d + 2 * s, mps[depth] + b * ncs);
+      }
+    }
+  }
+  cfg->t_phase[1] = now_s() - tm;
+  tm = now_s();
+  for (l = depth; l >= 3; l--) {
+    int Nc = cfg->order_s[l], Np = cfg->order_s[l - 1], L = Nc > Np ? Nc : Np;
+    int ncc = 2 * (Nc + 1) * (Nc + 1), ncp = 2 * (Np + 1) * (Np + 1);
+    double wc = t->D / (1 << l);
+    double *E = calloc(2 * (size_t)((L + 2) * (L + 3) * (L + 4) / 6), sizeof(double));
+    double *wk1 = malloc(sizeof(double) * 2 * (L + 1) * (L + 1));
+    double *wk2 = malloc(sizeof(double) * 2 * (L + 1) * (L + 1));
+    uint64_t np = 1ull << (3 * (l - 1)), p;
+    orc_hel_coaxial_table(0, L, k * wc * 0.5 * sqrt(3.0), E);
+    for (p = 0; p < np; p++) {
+      int c;
+      double *Cp = mps[l - 1] + p * ncp;
+      memset(Cp, 0, sizeof(double) * ncp);
+      if (t->bcount[l - 1][p] == 0) continue;
+      for (c = 0; c < 8; c++) {
+        double th, ph, ch;
+        if (t->bcount[l][8 * p + c] == 0) continue;
+        child_angles(c, 1, &th, &ph, &ch);
+        orc_hel_rotate(wk1, mps[l] + (8 * p + c) * ncc, Nc, Hud[0][c], ph, ch, 0.0);
+        orc_hel_coaxial(wk2, Np, wk1, Nc, E, L, 0, 0.0);
+        orc_hel_rotate(Cp, wk2, Np, Hud[0][c], ch, ph, 1.0);
+      }
+    }
+    free(E); free(wk1); free(wk2);
+  }
+  cfg->t_phase[2] = now_s() - tm;
+  tm = now_s();
+  for (l = 2; l <= depth; l++) {
+    int Ns = cfg->order_s[l], Nr = cfg->order_r[l], L = Ns > Nr ? Ns : Nr, o;
+    double w = t->D / (1 << l);
+    double *SR[343] = {0};
+    hel_job jobs[64];
+    pthread_t th[64];
+    int nth = cfg->nthreads > 1 ? cfg->nthreads : 1;
+    if (nth > 64) nth = 64;
+    for (o = 0; o < 343; o++) {
+      int dx = o / 49 - 3, dy = (o / 7) % 7 - 3, dz = o % 7 - 3, o2;
+      double r2;
+      if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+      r2 = sqrt((double)(dx * dx + dy * dy + dz * dz));
+      /* share the table between offsets of equal length */
+      for (o2 = 0; o2 < o; o2++) {
+        int ex = o2 / 49 - 3, ey = (o2 / 7) % 7 - 3, ez = o2 % 7 - 3;
+        if (SR[o2] && ex * ex + ey * ey + ez * ez == dx * dx + dy * dy + dz * dz) {
+          SR[o] = SR[o2];
+          break;
+        }
+      }
+      if (!SR[o]) {
+        SR[o] = calloc(2 * (size_t)((L + 2) * (L + 3) * (L + 4) / 6), sizeof(double));
+        orc_hel_coaxial_table(1, L, k * w * r2, SR[o]);
+      }
+    }
+    for (i = 0; i < nth; i++) {
+      jobs[i].t = t; jobs[i].cfg = cfg; jobs[i].level = l; jobs[i].nth = nth;
+      jobs[i].ith = i; jobs[i].mps = mps; jobs[i].mpr = mpr; jobs[i].Htab = Htab;
+      jobs[i].nrot = nrot; jobs[i].SR = SR;
+    }
+    if (nth == 1) hel_m2l_worker(&jobs[0]);
+    else {
+      for (i = 0; i < nth; i++) pthread_create(&th[i], NULL, hel_m2l_worker, &jobs[i]);
+      for (i = 0; i < nth; i++) pthread_join(th[i], NULL);
+    }
+    for (o = 0; o < 343; o++) {
+      int o2, shared = 0;
+      if (!SR[o]) continue;
+      for (o2 = o + 1; o2 < 343; o2++) if (SR[o2] == SR[o]) SR[o2] = NULL, shared = 1;
+      (void)shared;
+      free(SR[o]);
+    }
+    if (l < depth) {
+      int Np = cfg->order_r[l], Nc = cfg->order_r[l + 1], L2 = Np > Nc ? Np : Nc;
+      int ncp = 2 * (Np + 1) * (Np + 1), ncc = 2 * (Nc + 1) * (Nc + 1);
+      double wc = t->D / (1 << (l + 1));
+      double *E = calloc(2 * (size_t)((L2 + 2) * (L2 + 3) * (L2 + 4) / 6), sizeof(double));
+      double *wk1 = malloc(sizeof(double) * 2 * (L2 + 1) * (L2 + 1));
+      double *wk2 = malloc(sizeof(double) * 2 * (L2 + 1) * (L2 + 1));
+      uint64_t np = 1ull << (3 * l), p;
+      orc_hel_coaxial_table(0, L2, k * wc * 0.5 * sqrt(3.0), E);
+      for (p = 0; p < np; p++) {
+        int c;
+        for (c = 0; c < 8; c++) {
+          double th, ph, ch;
+          child_angles(c, -1, &th, &ph, &ch);
+          orc_hel_rotate(wk1, mpr[l] + p * ncp, Np, Hud[1][c], ph, ch, 0.0);
+          orc_hel_coaxial(wk2, Nc, wk1, Np, E, L2, 0, 0.0);
+          orc_hel_rotate(mpr[l + 1] + (8 * p + c) * ncc, wk2, Nc, Hud[1][c], ch, ph, 1.0);
+        }
+      }
+      free(E); free(wk1); free(wk2);
+    }
+  }
+  cfg->t_phase[3] = now_s() - tm;
+  tm = now_s();
+  {
+    int Nr = cfg->order_r[depth], ncr = 2 * (Nr + 1) * (Nr + 1);
+    for (i = 0; i < ntgt; i++) {
+      const double *x = xt + 3 * i;
+      uint64_t b = orc_point_index(x, t->x0, t->D) >> (3 * (20 - depth));
+      uint64_t nb[27];
+      double xc[3], loc[8] = {0};
+      int nn, a, j, c;
+      box_centre(t, depth, b, xc);
+      orc_hel_j_eval(k, Nr, xc, mpr[depth] + b * ncr, x, cfg->field, loc);
+      nn = orc_box_neighbours(depth, b, nb);
+      for (a = 0; a < nn; a++)
+        for (j = 0; j < t->bcount[depth][nb[a]]; j++) {
+          int s = t->ip[t->bstart[depth][nb[a]] + j];
+          double rv[3] = {x[0] - xs[3 * s], x[1] - xs[3 * s + 1], x[2] - xs[3 * s + 2]};
+          double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+          cplx G, dG, ddG, pot = 0, g[3] = {0, 0, 0};
+          if (r <= CUTOFF) continue;
+          G = cexp(I * k * r) / (4.0 * M_PI * I * k * r);
+          dG = G * (I * k - 1.0 / r);
+          ddG = G * ((I * k - 1.0 / r) * (I * k - 1.0 / r) + 1.0 / (r * r));
+          if (q) {
+            cplx qq = q[2 * s] + I * q[2 * s + 1];
+            pot += qq * G;
+            for (c = 0; c < 3; c++) g[c] += qq * dG * rv[c] / r;
+          }
+          if (d) {
+            cplx dd = d[2 * s] + I * d[2 * s + 1];
+            const double *nv = nrm + 3 * s;
+            double nr = (nv[0] * rv[0] + nv[1] * rv[1] + nv[2] * rv[2]) / r;
+            pot += dd * dG * nr;
+            for (c = 0; c < 3; c++)
+              g[c] += dd * (ddG * nr * rv[c] / r + dG * (nv[c] - nr * rv[c] / r) / r);
+          }
+          loc[0] += creal(pot); loc[1] += cimag(pot);
+          for (c = 0; c < 3; c++) { loc[2 + 2 * c] += creal(g[c]); loc[3 + 2 * c] += cimag(g[c]); }
+        }
+      if (cfg->field & 1) { f[8 * i] += loc[0]; f[8 * i + 1] += loc[1]; }
+      if (cfg->field & 2) for (c = 2; c < 8; c++) f[8 * i + c] += loc[c];
+    }
+  }
+  cfg->t_phase[4] = now_s() - tm;
+  if (cfg->dump_level >= 2 && cfg->dump_level <= depth) {
+    int l2 = cfg->dump_level;
+    size_t nb = (size_t)1 << (3 * l2);
+    if (cfg->dump_mps)
+      memcpy(cfg->dump_mps, mps[l2], sizeof(double) * nb * 2 * (cfg->order_s[l2] + 1) * (cfg->order_s[l2] + 1));
+    if (cfg->dump_mpr)
+      memcpy(cfg->dump_mpr, mpr[l2], sizeof(double) * nb * 2 * (cfg->order_r[l2] + 1) * (cfg->order_r[l2] + 1));
+  }
+  for (l = 2; l <= depth; l++) { free(mps[l]); free(mpr[l]); }
+  for (i = 0; i < 8; i++) { free(Hud[0][i]); free(Hud[1][i]); }
+  free(Htab);
+  return 0;
+}
+
 int orc_fmm(orc_fmm_cfg *cfg, int nsrc, const double *xs, const double *q,
             const double *nrm, const double *d, int ntgt, const double *xt,
             double *f)
@@ -1090,6 +1661,8 @@ int orc_fmm(orc_fmm_cfg *cfg, int nsrc, const double *xs, const double *q,
   cfg->t_phase[0] = t1 - t0;
   if (cfg->problem == 1)
     rc = lap_solve(cfg, &t, xs, q, nrm, d, ntgt, xt, f);
+  else if (cfg->problem == 2)
+    rc = hel_solve(cfg, &t, xs, q, nrm, d, ntgt, xt, f);
   else
     rc = -3;
   tree_free(&t);

@@ -192,7 +192,9 @@ def test_high_order_against_direct(wb, oracle):
     fd = oracle.lap_direct(xs, q, xs, cutoff=1e-6)
     for c in range(4):
         e = np.linalg.norm(fg[:, c] - fd[:, c]) / np.linalg.norm(fd[:, c])
-        assert e < 1e-10, (c, e)
+        # truncation error at p=24 is ~3e-10 (potential), ~2e-9 (gradient);
+        # FP64-recursion rotation tables would leave ~1e-4 here
+        assert e < (1e-9 if c == 0 else 1e-8), (c, e)
 
 
 def test_full_size_properties(wb):
@@ -253,3 +255,33 @@ def test_point_outside_domain_aborts(wb):
                        timeout=120)
     assert r.returncode != 0
     assert "do not lie in box" in r.stderr
+
+
+def test_mfma_f64_operand_layout(wb):
+    # the M2L kernel assumes lane l supplies A[l%16][l/16], B[l/16][l%16]
+    # and receives D[l/16 + 4q][l%16]; check it against numpy
+    out = np.zeros(256)
+    assert wb.lib().wbfmm_hip_mfma_layout_probe(wb.ptr(out)) == 0
+    i = np.arange(16)[:, None]
+    k = np.arange(4)[None, :]
+    A = 1.0 + i + 100.0 * k
+    Bm = 1.0 + 3.0 * np.arange(4)[:, None] + 1000.0 * np.arange(16)[None, :]
+    Dm = A @ Bm
+    got = np.zeros((16, 16))
+    for l in range(64):
+        for q in range(4):
+            got[l // 16 + 4 * q, l % 16] = out[4 * l + q]
+    assert np.array_equal(got, Dm)
+
+
+@pytest.mark.parametrize("p", [4, 7, 12, 16])
+def test_m2l_mfma_orders_match_oracle(wb, oracle, p):
+    # the matrix-core M2L over the orders it serves (2..16)
+    rng = np.random.default_rng(20 + p)
+    xs = rng.uniform(0, 1, (1800, 3))
+    q = rng.normal(size=1800)
+    x0, D = fit_box(xs)
+    fo, _, _ = oracle.fmm(1, xs, xs, x0, D, 3, [p] * 17, [p] * 17, q=q, field=3)
+    fg = gpu_fmm(wb, xs, q, xs, x0, D, 3, p)
+    for c in range(4):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (p, c, rel(fg[:, c], fo[:, c]))

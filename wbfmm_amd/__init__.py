@@ -83,7 +83,53 @@ _SIGS = {
     "wbfmm_hip_timing_reset": (_i, []),
     "wbfmm_hip_timing_query": (_i, [ctypes.c_char_p, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
     "wbfmm_hip_last_error": (ctypes.c_char_p, []),
+    "wbfmm_hip_malloc": (_i, [ctypes.POINTER(_vp), _sz]),
+    "wbfmm_hip_free": (_i, [_vp]),
+    "wbfmm_hip_memcpy": (_i, [_vp, _vp, _sz]),
+    "wbfmm_hip_memset": (_i, [_vp, _i, _sz]),
+    "wbfmm_hip_stage4": (_i, [_vp, _sz, _u, _vp]),
+    "wbfmm_hip_lap_direct": (_i, [_vp, _vp, _vp, _vp, _u, _vp, _u, _i, _d, _vp, _i]),
+    "wbfmm_hip_mfma_layout_probe": (_i, [_vp]),
 }
+
+
+class DeviceBuffer:
+    """Device memory from the library's allocator (no torch needed)."""
+
+    def __init__(self, nbytes):
+        p = _vp()
+        rc = lib().wbfmm_hip_malloc(ctypes.byref(p), nbytes)
+        if rc != 0:
+            raise MemoryError(lib().wbfmm_hip_last_error().decode())
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if lib().wbfmm_hip_memcpy(self.ptr, a.ctypes.data, a.nbytes) != 0:
+            raise RuntimeError(lib().wbfmm_hip_last_error().decode())
+
+    def download(self, a):
+        assert a.flags["C_CONTIGUOUS"] and a.nbytes <= self.nbytes
+        if lib().wbfmm_hip_memcpy(a.ctypes.data, self.ptr, a.nbytes) != 0:
+            raise RuntimeError(lib().wbfmm_hip_last_error().decode())
+        return a
+
+    def zero(self):
+        lib().wbfmm_hip_memset(self.ptr, 0, self.nbytes)
+
+    def free(self):
+        if self.ptr:
+            lib().wbfmm_hip_free(self.ptr)
+            self.ptr = None
 
 
 def lib():
@@ -107,6 +153,8 @@ def ptr(a):
         return None
     if isinstance(a, int):
         return a
+    if isinstance(a, DeviceBuffer):
+        return a.ptr
     if isinstance(a, np.ndarray):
         assert a.flags["C_CONTIGUOUS"]
         return a.ctypes.data

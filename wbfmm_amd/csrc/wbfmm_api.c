@@ -61,6 +61,7 @@ typedef struct {
   uint32_t *d_flag[WBFMM_TREE_MAX_DEPTH + 1];
   uint32_t *d_act[WBFMM_TREE_MAX_DEPTH + 1];
   unsigned nact[WBFMM_TREE_MAX_DEPTH + 1];
+  int have_targets; /* d_flag/d_act describe the current points */
   size_t mps_bytes[WBFMM_TREE_MAX_DEPTH + 1], mpr_bytes[WBFMM_TREE_MAX_DEPTH + 1];
   double *d_q, *d_dip; /* strengths in sorted order */
   unsigned cap_q, cap_dip;
@@ -78,7 +79,17 @@ typedef struct {
   double *d_rot;
   wbfmm_rot_layout_t lay;
   double *d_lap_coax;
+  /* MFMA M2L plans, one per order (built on first use) */
+  int m2l_have[WBFMM_HIP_MAX_ORDER + 1];
+  wbfmm_m2l_plan_t *m2l_plan[WBFMM_HIP_MAX_ORDER + 1];
+  void *m2l_plan_dev[WBFMM_HIP_MAX_ORDER + 1];
+  double *m2l_A1[WBFMM_HIP_MAX_ORDER + 1], *m2l_A3[WBFMM_HIP_MAX_ORDER + 1],
+      *m2l_A2[WBFMM_HIP_MAX_ORDER + 1];
 } ops_dev;
+
+/* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
+#define M2L_MFMA_MIN 2
+#define M2L_MFMA_MAX 16
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
 
@@ -397,6 +408,10 @@ gint wbfmm_tree_add_points(wbfmm_tree_t *t, gpointer pts, gsize pstr,
     WBFMM_FATAL("mixed precision not implemented");
   if (npts > t->maxpoints)
     WBFMM_FATAL("too many points (%u) for tree (%u)", npts, t->maxpoints);
+  /* a new point set starts a new tree: levels are rebuilt by refine and
+     any registered target list must be added again */
+  t->depth = 0;
+  td->have_targets = 0;
   t->npoints = npts;
   t->points = (char *)pts;
   t->pstr = pstr;
@@ -597,11 +612,29 @@ static void ensure_lap_coax(ops_dev *od)
 gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
 {
   ops_dev *od = (ops_dev *)op;
+  int n;
   if (!op) return 0;
   wbfmm_hip_free(od->d_rot);
   wbfmm_hip_free(od->d_lap_coax);
+  for (n = 0; n <= WBFMM_HIP_MAX_ORDER; n++) {
+    if (!od->m2l_have[n]) continue;
+    wbfmm_hip_free(od->m2l_plan_dev[n]);
+    wbfmm_hip_free(od->m2l_A1[n]);
+    wbfmm_hip_free(od->m2l_A3[n]);
+    wbfmm_hip_free(od->m2l_A2[n]);
+    free(od->m2l_plan[n]);
+  }
   free(od);
   return 0;
+}
+
+static void ensure_m2l_plan(ops_dev *od, int N)
+{
+  if (od->m2l_have[N]) return;
+  od->m2l_plan[N] = calloc(1, sizeof(wbfmm_m2l_plan_t));
+  HIPOK(wbfmm_hip_lap_m2l_plan(N, od->m2l_plan[N], &od->m2l_plan_dev[N],
+                               &od->m2l_A1[N], &od->m2l_A3[N], &od->m2l_A2[N]));
+  od->m2l_have[N] = 1;
 }
 
 gint wbfmm_laplace_coaxial_translate_init(gint N)
@@ -672,20 +705,38 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if ((int)t->order_s[level] > od->lay.L || (int)t->order_r[level] > od->lay.L)
     WBFMM_FATAL("operators built to order %d", od->lay.L);
   ensure_lap_coax(od);
-  if (td->d_act[level]) {
+  if (td->have_targets) {
     tl = td->d_act[level];
     nt = td->nact[level];
   } else {
     tl = all_boxes(level, &nt);
   }
-  HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
-                          t->D, t->order_s[level], t->order_r[level],
-                          (double *)t->mps[level], (double *)t->mpr[level],
-                          od->d_rot, &od->lay, od->d_lap_coax));
+  if (t->order_s[level] == t->order_r[level] &&
+      (int)t->order_s[level] >= M2L_MFMA_MIN && (int)t->order_s[level] <= M2L_MFMA_MAX) {
+    /* matrix-core form: targets grouped by Morton child digit */
+    int N = (int)t->order_s[level];
+    unsigned cstart[8], ccount[8];
+    uint32_t *cls;
+    ensure_m2l_plan(od, N);
+    HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
+    HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
+    HIPOK(wbfmm_hip_lap_m2l_mfma(cls, cstart, ccount, td->d_start[level],
+                                 td->d_end[level], level, t->D, N,
+                                 (double *)t->mps[level], (double *)t->mpr[level],
+                                 od->m2l_plan_dev[N], od->m2l_A1[N],
+                                 od->m2l_A3[N], od->m2l_A2[N]));
+    wbfmm_hip_free(cls);
+  } else {
+    HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
+                            t->D, t->order_s[level], t->order_r[level],
+                            (double *)t->mps[level], (double *)t->mpr[level],
+                            od->d_rot, &od->lay, od->d_lap_coax));
+  }
   if (level == t->depth) return 0;
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
-  HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->d_flag[level + 1], level, t->D,
+  HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
+                          level, t->D,
                           t->order_r[level], t->order_r[level + 1],
                           (double *)t->mpr[level], (double *)t->mpr[level + 1],
                           od->d_rot, &od->lay, od->d_lap_coax));
@@ -930,6 +981,7 @@ static void register_targets(tree_dev *td, const uint64_t *dcodes, unsigned n)
     HIPOK(wbfmm_hip_compact_boxes(td->d_start[l], td->d_end[l], td->d_flag[l],
                                   l, td->d_act[l], &td->nact[l]));
   }
+  td->have_targets = 1;
 }
 
 gint wbfmm_target_list_add_points(wbfmm_target_list_t *l, gpointer pts,

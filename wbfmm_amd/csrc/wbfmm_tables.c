@@ -270,3 +270,158 @@ int wbfmm_hip_laplace_coaxial_tables(int L, double **dev)
   free(host);
   return rc;
 }
+
+/* ------------------------------------------------------------------
+ * MFMA form of the Laplace M2L (v_mfma_f64_16x16x4f64 operands).
+ *
+ * The three factors of one translation (rotate, coaxial S|R, rotate
+ * back; src/pass-laplace.c:73-101) are applied to 16 boxes at a time as
+ * 16x16x4 FP64 matrix products.  Rows are padded to 16-row tiles; the
+ * inputs of a tile span whole degree blocks (rotations) or order blocks
+ * (coaxial), so each tile needs a short run of 4-wide k steps.  The A
+ * operand of each (tile, k step) is stored in lane order: lane l holds
+ * A[l % 16][l / 16].  The coaxial factor is stored at unit distance; the
+ * kernel scales its inputs by t^-nu and outputs by t^-(n+1).
+ * ------------------------------------------------------------------ */
+
+static void tile_ranges(int C, int NT, const int *blk_lo, const int *blk_hi,
+                        int *kb, int *nk, int *off, int *total)
+{
+  int i, acc = 0;
+  for (i = 0; i < NT; i++) {
+    int r0 = 16 * i, r1 = 16 * i + 15, lo, hi;
+    if (r1 >= C) r1 = C - 1;
+    lo = blk_lo[r0];
+    hi = blk_hi[r1];
+    kb[i] = lo;
+    nk[i] = (hi - lo + 3) / 4;
+    off[i] = acc;
+    acc += nk[i];
+  }
+  *total = acc;
+}
+
+/* store rows [16i,16i+16) x cols [kb+4j, kb+4j+4) of the dense C x C
+   matrix M (row-major, zero outside) in MFMA lane order */
+static void swizzle(double *dst, const double *M, int C, int NT,
+                    const int *kb, const int *nk)
+{
+  int i, j, l;
+  for (i = 0; i < NT; i++)
+    for (j = 0; j < nk[i]; j++)
+      for (l = 0; l < 64; l++) {
+        int r = 16 * i + (l % 16), c = kb[i] + 4 * j + (l / 16);
+        *dst++ = (r < C && c < C) ? M[(size_t)r * C + c] : 0.0;
+      }
+}
+
+int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *P, void **plan_dev,
+                           double **A1, double **A3, double **A2)
+{
+  int C = (N + 1) * (N + 1), NT = (C + 15) / 16, n, m, r, i, dx, dy, dz, rc;
+  int lo_n[WBFMM_M2L_MAXC], hi_n[WBFMM_M2L_MAXC];
+  int lo_m[WBFMM_M2L_MAXC], hi_m[WBFMM_M2L_MAXC];
+  double *M, *h1, *h3, *h2, *R, *Rb, *H;
+  hcache hc;
+  size_t per;
+  if (C > WBFMM_M2L_MAXC || NT > WBFMM_M2L_MAXT) return WBFMM_HIP_ERR_ORDER;
+  memset(P, 0, sizeof(*P));
+  P->N = N;
+  P->C = C;
+  P->NT = NT;
+  P->CP = 16 * NT;
+  /* n-major rows: degree blocks [n^2, (n+1)^2) */
+  for (n = 0; n <= N; n++)
+    for (r = n * n; r < (n + 1) * (n + 1); r++) {
+      lo_n[r] = n * n;
+      hi_n[r] = (n + 1) * (n + 1);
+      P->degn[r] = n;
+    }
+  /* m-major rows: m=0 (n=0..N), then for m>=1 re (n=m..N), im (n=m..N) */
+  r = 0;
+  for (m = 0; m <= N; m++) {
+    int part, nparts = (m == 0) ? 1 : 2;
+    for (part = 0; part < nparts; part++) {
+      int start = r, len = N + 1 - m;
+      for (n = m; n <= N; n++, r++) {
+        int nrow = (m == 0) ? n * n : n * n + 2 * m - 1 + part;
+        P->npos[r] = nrow;
+        P->mpos[nrow] = r;
+        P->degm[r] = n;
+        lo_m[r] = start;
+        hi_m[r] = start + len;
+      }
+    }
+  }
+  tile_ranges(C, NT, lo_n, hi_n, P->kb1, P->nk1, P->off1, &P->T13);
+  tile_ranges(C, NT, lo_m, hi_m, P->kb2, P->nk2, P->off2, &P->T2);
+
+  /* stage 2: unit-distance S|R in m-major order */
+  M = calloc((size_t)C * C, sizeof(double));
+  for (r = 0; r < C; r++) {
+    int n1 = P->degm[r], c;
+    for (c = lo_m[r]; c < hi_m[r]; c++) {
+      int nu = P->degm[c], mm;
+      /* order m of this block: rows of one block share m */
+      int nrow = P->npos[r];
+      mm = (nrow == n1 * n1) ? 0 : (nrow - n1 * n1 + 1) / 2;
+      M[(size_t)r * C + c] = lap_cfft(2, n1, nu, mm);
+    }
+  }
+  h2 = malloc(sizeof(double) * (size_t)P->T2 * 64);
+  swizzle(h2, M, C, NT, P->kb2, P->nk2);
+
+  /* stages 1 and 3: rotation blocks of every list-4 offset */
+  per = (size_t)P->T13 * 64;
+  h1 = calloc(343 * per, sizeof(double));
+  h3 = calloc(343 * per, sizeof(double));
+  R = malloc(sizeof(double) * (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3);
+  Rb = malloc(sizeof(double) * (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3);
+  hc.n = 0;
+  hc.L = N;
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+        double th, ph, ch;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        P->r[o] = sqrt((double)(dx * dx + dy * dy + dz * dz));
+        dir_angles(-dx, -dy, -dz, &th, &ph, &ch);
+        H = (double *)cached_H(&hc, th);
+        real_blocks(R, H, N, ph, ch);
+        real_blocks(Rb, H, N, ch, ph);
+        /* expand the degree blocks into dense C x C (row-major) */
+        memset(M, 0, sizeof(double) * (size_t)C * C);
+        for (n = 0; n <= N; n++) {
+          int w = 2 * n + 1, a, b;
+          const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+          for (a = 0; a < w; a++)
+            for (b = 0; b < w; b++) M[(size_t)(n * n + a) * C + n * n + b] = B[a * w + b];
+        }
+        swizzle(h1 + o * per, M, C, NT, P->kb1, P->nk1);
+        memset(M, 0, sizeof(double) * (size_t)C * C);
+        for (n = 0; n <= N; n++) {
+          int w = 2 * n + 1, a, b;
+          const double *B = Rb + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+          for (a = 0; a < w; a++)
+            for (b = 0; b < w; b++) M[(size_t)(n * n + a) * C + n * n + b] = B[a * w + b];
+        }
+        swizzle(h3 + o * per, M, C, NT, P->kb1, P->nk1);
+      }
+  for (i = 0; i < hc.n; i++) free(hc.H[i]);
+  free(R);
+  free(Rb);
+  free(M);
+  rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * 343 * per);
+  if (!rc) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * 343 * per);
+  if (!rc) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * 343 * per);
+  if (!rc) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * 343 * per);
+  if (!rc) rc = wbfmm_hip_malloc((void **)A2, sizeof(double) * (size_t)P->T2 * 64);
+  if (!rc) rc = wbfmm_hip_memcpy(*A2, h2, sizeof(double) * (size_t)P->T2 * 64);
+  if (!rc) rc = wbfmm_hip_malloc(plan_dev, sizeof(*P));
+  if (!rc) rc = wbfmm_hip_memcpy(*plan_dev, P, sizeof(*P));
+  free(h1);
+  free(h2);
+  free(h3);
+  return rc;
+}
