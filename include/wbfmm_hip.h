@@ -131,7 +131,8 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const uint32_t *send, unsigned level, double D,
                            int N, const double *mps, double *mpr,
                            const void *plan_dev, const double *A1,
-                           const double *A3, const double *A2);
+                           const double *A3, const double *A2,
+                           const wbfmm_m2l_plan_t *plan_host);
 /* one v_mfma_f64_16x16x4f64 with A[i][k] = 1+i+100k, B[k][j] = 1+3k+1000j;
    host_out (256 doubles) gets the four result values of each lane */
 int wbfmm_hip_mfma_layout_probe(double *host_out);

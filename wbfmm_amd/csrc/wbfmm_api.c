@@ -724,7 +724,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                  td->d_end[level], level, t->D, N,
                                  (double *)t->mps[level], (double *)t->mpr[level],
                                  od->m2l_plan_dev[N], od->m2l_A1[N],
-                                 od->m2l_A3[N], od->m2l_A2[N]));
+                                 od->m2l_A3[N], od->m2l_A2[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
     HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
