@@ -66,11 +66,14 @@ __device__ __forceinline__ uint64_t point_code(double x, double y, double z,
   return spread3(i) | (spread3(j) << 1) | (spread3(k) << 2);
 }
 
-// element offset of coefficient c of box b in a level array packed in
-// groups of 8 siblings (src/boxes.c:552-579, src/laplace.c:823-849);
-// C = coefficients per box (reals for Laplace)
+// element offset of coefficient c of box b in a level array.  On the
+// device the levels are stored box-major (box b at b*C, C coefficients,
+// reals for Laplace) so that one box's expansion is one contiguous run for
+// the gathers of M2L; the reference's 8-sibling interleave
+// (src/boxes.c:552-579, src/laplace.c:823-849) is restored when
+// coefficients are copied to the host (wbfmm_tree_coefficients_download).
 __device__ __forceinline__ size_t coef_at(uint64_t b, int c, int C) {
-  return (size_t)(b >> 3) * 8 * (size_t)C + (size_t)c * 8 + (b & 7);
+  return (size_t)b * (size_t)C + (size_t)c;
 }
 
 __device__ __forceinline__ int isqrt_deg(int r) {

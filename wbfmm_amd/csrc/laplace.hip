@@ -1,8 +1,8 @@
 // laplace.hip -- gfx950 kernels of the Laplace FMM path (FP64).
 //
 // Coefficients use the reference's real packing (n,0) at n^2, (n,m>0)
-// re/im at n^2+2m-1, n^2+2m (src/wbfmm-private.h:76) and its 8-sibling
-// interleaved level arrays (src/laplace.c:794-852).  Every translation is
+// re/im at n^2+2m-1, n^2+2m (src/wbfmm-private.h:76); level arrays are
+// box-major on the device (coef_at, hip_common.h).  Every translation is
 // the reference's rotate -> coaxial shift -> rotate back sequence
 // (src/pass-laplace.c:35-105, src/shift-laplace.c:225-789); the rotations
 // are applied as real (2n+1)x(2n+1) blocks built on the host from the

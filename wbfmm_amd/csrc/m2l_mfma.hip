@@ -2,21 +2,27 @@
 //
 // The reference translates one source box at a time: rotate the multipole
 // into the frame of the offset, coaxial S|R shift, rotate back and add into
-// the target's local expansion (src/pass-laplace.c:35-105).  Every box pair
-// with the same offset shares the two rotations and every pair with the
-// same distance shares the shift, so here a wavefront takes 16 target
-// boxes with the same Morton child digit (hence the same 189 list-4
-// offsets), and for each offset applies the three factors to the 16
-// source expansions at once with v_mfma_f64_16x16x4f64:
+// the target's local expansion (src/pass-laplace.c:35-105).  Here a
+// wavefront takes 16 target boxes with the same Morton child digit (hence
+// the same list-4 offsets) and, for each offset, applies the translation
+// to the 16 source expansions at once:
 //
-//   stage 1  Y = R_o X        rows n-major, A from the offset's table
-//   stage 2  Z = S Y          rows m-major, unit-distance S|R, inputs
-//                             scaled by t^-nu and outputs by t^-(n+1)
-//   stage 3  L += R_o^-1 Z    rows n-major, accumulated in registers
+//   Zin     rotate each (re, im) pair of order m by m chi   (vector units)
+//   stage 1 Y = H1(theta) X      bin-packed cos/sin blocks  (MFMA)
+//   stage 2 Z = S Y              coaxial S|R per order      (MFMA)
+//             inputs scaled by t^-n, outputs by t^-(n+1)
+//   stage 3 W = H3(theta) Z      (MFMA)
+//   Zout    L += rotation of each (tr, ti) pair by -nu chi (vector units)
 //
-// X, Y and Z pass through one LDS buffer of (CP+4) x 16 doubles; the local
-// expansions of the 16 targets stay in registers (NT double4 per lane)
-// for all 189 offsets and are added to mpr once at the end.
+// The matrices depend on the polar angle only (49 tables) and are read
+// from L2; the plan (slot maps, per-offset theta/chi/r) is built on the
+// host by wbfmm_hip_lap_m2l_tables.  One LDS buffer R of S slots x 16
+// columns carries the data between stages (slot s of column j at
+// R[16 s + j]).  Source expansions are box-major (box b at b*C); the next
+// offset's 16 sources are loaded into registers while the current offset
+// is computed (orders up to 12) and deposited, rotated by Zin, into R at
+// the end of the iteration.  The local expansions of the 16 targets stay
+// in registers for all offsets and are added to mpr once at the end.
 //
 // MFMA operand layout (16x16x4 f64, checked by wbfmm_hip_mfma_layout_probe):
 // lane l supplies A[l%16][l/16] and B[l/16][l%16]; the result holds
@@ -39,200 +45,323 @@ struct ClassArgs {
   unsigned chunk_prefix[9];
 };
 
-// Compile-time copy of the tiling built on the host by
-// wbfmm_hip_lap_m2l_plan (the A tables are stored in this order); with it
-// every k loop has a constant trip count and unrolls, so the A loads of a
-// whole stage can be issued ahead of their products.
-template <int N>
-struct Tiling {
-  static constexpr int C = (N + 1) * (N + 1);
-  static constexpr int NT = (C + 15) / 16;
-  int kb1[NT], nk1[NT], off1[NT], kb2[NT], nk2[NT], off2[NT];
-  int T13, T2;
+// Compile-time copy of the bin packing of wbfmm_tables.c (pack_bins over
+// the cos/sin blocks, sizes n+1 and n): number of bins and the k steps of
+// each bin, so that every MFMA loop has a constant trip count.
+struct Bins {
+  int nb, kt;
+  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB];
 };
 
-constexpr int tl_deg(int r) {
-  int n = 0;
-  while ((n + 1) * (n + 1) <= r) n++;
-  return n;
-}
-
-// bounds of the m-major block holding m-major row r (order N)
-constexpr int tl_mlo(int N, int r) {
-  int start = 0, len = N + 1;
-  if (r < start + len) return start;
-  start += len;
-  for (int m = 1; m <= N; m++)
-    for (int part = 0; part < 2; part++) {
-      len = N + 1 - m;
-      if (r < start + len) return start;
-      start += len;
-    }
-  return start;
-}
-
-constexpr int tl_mhi(int N, int r) {
-  int start = 0, len = N + 1;
-  if (r < start + len) return start + len;
-  start += len;
-  for (int m = 1; m <= N; m++)
-    for (int part = 0; part < 2; part++) {
-      len = N + 1 - m;
-      if (r < start + len) return start + len;
-      start += len;
-    }
-  return start;
+constexpr Bins make_bins(int N) {
+  Bins B{};
+  int sz[2 * WBFMM_M2L_MAXN + 2] = {}, order[2 * WBFMM_M2L_MAXN + 2] = {};
+  int fill[WBFMM_M2L_MAXB] = {};
+  const int nblk = 2 * N + 1;
+  for (int i = 0; i < nblk; i++) {
+    sz[i] = i <= N ? i + 1 : i - N;
+    order[i] = i;
+  }
+  for (int i = 1; i < nblk; i++) {
+    int v = order[i], j = i;
+    for (; j > 0 && sz[order[j - 1]] < sz[v]; j--) order[j] = order[j - 1];
+    order[j] = v;
+  }
+  int nb = 0;
+  for (int i = 0; i < nblk; i++) {
+    int k = order[i], b = 0;
+    for (; b < nb; b++)
+      if (fill[b] + sz[k] <= 16) break;
+    if (b == nb) fill[nb++] = 0;
+    fill[b] += sz[k];
+  }
+  B.nb = nb;
+  for (int b = 0; b < nb; b++) {
+    B.nk[b] = (fill[b] + 3) / 4;
+    B.koff[b] = B.kt;
+    B.kt += B.nk[b];
+  }
+  return B;
 }
 
 template <int N>
-constexpr Tiling<N> make_tiling() {
-  Tiling<N> t{};
-  constexpr int C = Tiling<N>::C, NT = Tiling<N>::NT;
-  int a1 = 0, a2 = 0;
-  for (int i = 0; i < NT; i++) {
-    int r0 = 16 * i, r1 = (16 * i + 15 < C - 1) ? 16 * i + 15 : C - 1;
-    int lo = tl_deg(r0) * tl_deg(r0), hi = (tl_deg(r1) + 1) * (tl_deg(r1) + 1);
-    t.kb1[i] = lo;
-    t.nk1[i] = (hi - lo + 3) / 4;
-    t.off1[i] = a1;
-    a1 += t.nk1[i];
-    lo = tl_mlo(N, r0);
-    hi = tl_mhi(N, r1);
-    t.kb2[i] = lo;
-    t.nk2[i] = (hi - lo + 3) / 4;
-    t.off2[i] = a2;
-    a2 += t.nk2[i];
+struct M2LShape {
+  static constexpr Bins BN = make_bins(N);
+  static constexpr int NB = BN.nb;
+  static constexpr int KT = BN.kt;
+  static constexpr int C = (N + 1) * (N + 1);
+  static constexpr int NU = (N + 1) * (N + 2) / 2;
+  static constexpr int S = (16 * NB > 2 * NU) ? 16 * NB : 2 * NU;  // = plan S
+  static constexpr int NPU = (16 * NU + 63) / 64;  // source units per lane
+  static constexpr int NAU = (NU + 3) / 4;         // target units per lane
+  // register prefetch of the next offset's sources while it fits
+  static constexpr bool PREFETCH = N <= 12;
+};
+
+// one stage: y[b] = tile(b) * R[bin b's k slots].  The A operands (lane
+// order) run through a ring of AD registers: step t is requested AD steps
+// before its product, so a product waits on a load issued long before.
+// `a` is wave-uniform (scalar base) and the lane index is added per load:
+// the loads use a scalar base plus a 32-bit lane offset.
+template <int NB, int KT>
+__device__ __forceinline__ void stage(const Bins &BN, const double *__restrict__ a,
+                                      const double *R, int lane, int grp, int col,
+                                      d4 *y) {
+  constexpr int AD = KT < 16 ? KT : 16;
+  double av[AD];
+#pragma unroll
+  for (int t = 0; t < AD; t++) av[t] = a[t * 64 + lane];
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    d4 r4 = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int t = BN.koff[b] + j;
+      r4 = __builtin_amdgcn_mfma_f64_16x16x4f64(
+          av[t % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4, 0, 0, 0);
+      if (t + AD < KT) av[t % AD] = a[(t + AD) * 64 + lane];
+    }
+    y[b] = r4;
   }
-  t.T13 = a1;
-  t.T2 = a2;
-  return t;
 }
 
 template <int N>
 __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const uint32_t *__restrict__ cls, ClassArgs ca,
-    const uint32_t *__restrict__ sstart, const uint32_t *__restrict__ send,
-    int level, double w, const double *__restrict__ mps,
-    double *__restrict__ mpr, const wbfmm_m2l_plan_t *__restrict__ P,
-    const double *__restrict__ A1, const double *__restrict__ A3,
-    const double *__restrict__ A2) {
-  constexpr Tiling<N> TP = make_tiling<N>();
-  constexpr int C = Tiling<N>::C;
-  constexpr int NT = Tiling<N>::NT;
-  constexpr int ROWS = 16 * NT + 4;
-  __shared__ double buf[ROWS * 16];
-  __shared__ double ipw[2 * N + 4];
-  __shared__ int s_mpos[C], s_npos[C], s_degn[C], s_degm[C];
+    const uint32_t *__restrict__ occ, int level, double w,
+    const double *__restrict__ mps, double *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr Bins BN = SH::BN;
+  constexpr int NB = SH::NB, KT = SH::KT, C = SH::C, NU = SH::NU, S = SH::S;
+  constexpr int NPU = SH::NPU, NAU = SH::NAU;
+  __shared__ double R[(S + 1) * 16];  // slot S: dummy destination
+  __shared__ int map1[16 * NB], map2[16 * NB], map3[16 * NB];
+  __shared__ int2 uinf[NU];  // x: packed position of re | m << 16; y: stage-1 slots
+  __shared__ double ipw[N + 2];
+  __shared__ double csc[2 * (N + 1)], csn[2 * (N + 1)];
+  __shared__ unsigned vm[16][7];
+  __shared__ unsigned anym[7];
+  __shared__ const double *srcp[16];
+
   const int lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
 
-  for (int r = lane; r < C; r += 64) {
-    s_mpos[r] = P->mpos[r];
-    s_npos[r] = P->npos[r];
-    s_degn[r] = P->degn[r];
-    s_degm[r] = P->degm[r];
+  for (int i = lane; i < 16 * NB; i += 64) {
+    map1[i] = P->map1[i];
+    map2[i] = P->map2[i];
+    map3[i] = P->map3[i];
   }
+  for (int u = lane; u < NU; u += 64) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int2((n * n + (m ? 2 * m - 1 : 0)) | (m << 16), P->s1[u]);
+  }
+  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = 0.0;
+  for (int i = lane; i < 16 * 7; i += 64) vm[i / 7][i % 7] = 0u;
 
-  int b = blockIdx.x, c = 0;
-  while (c < 7 && (unsigned)b >= ca.chunk_prefix[c + 1]) c++;
-  const unsigned k16 = (unsigned)(b - ca.chunk_prefix[c]) * 16 + col;
+  // block -> chunk of 16 targets; consecutive chunks stay on one XCD
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x, q8 = nch / 8;
+  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+  int c = 0;
+  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  const unsigned k16 = (chunk - ca.chunk_prefix[c]) * 16 + col;
   const bool tvalid = k16 < ca.count[c];
   const uint32_t tb = tvalid ? cls[ca.start[c] + k16] : 0u;
   const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
   const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
   const int nbox = 1 << level;
+  __syncthreads();
 
-  d4 acc[NT];
-#pragma unroll
-  for (int i = 0; i < NT; i++) acc[i] = d4{0.0, 0.0, 0.0, 0.0};
+  // list-4 sources of each target, as bits over the 6x6x6 cube of the
+  // parent's neighbours' children (k = 36 a + 6 b + c)
+  for (int k = grp; k < 216; k += 4) {
+    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int si = ti + dx, sj = tj + dy, sk = tk + dz;
+    if (!tvalid || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox)
+      continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[col][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  if (lane < 7) {
+    unsigned a = 0;
+    for (int j = 0; j < 16; j++) a |= vm[j][lane];
+    anym[lane] = a;
+  }
+  __syncthreads();
 
-  for (int dx = -2 - px; dx <= 3 - px; dx++)
-    for (int dy = -2 - py; dy <= 3 - py; dy++)
-      for (int dz = -2 - pz; dz <= 3 - pz; dz++) {
-        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1)
-          continue;
-        const int si = ti + dx, sj = tj + dy, sk = tk + dz;
-        bool v = tvalid && si >= 0 && si < nbox && sj >= 0 && sj < nbox &&
-                 sk >= 0 && sk < nbox;
-        uint64_t sb = 0;
-        if (v) {
-          sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
-          v = send[sb] > sstart[sb];
-        }
-        if (!__any(v)) continue;
-        const int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
-        const double lt = log(P->r[o] * w);
-        __syncthreads();
-        if (lane < 2 * N + 4) ipw[lane] = exp(-(double)lane * lt);
+  auto next_k = [&](int k) {
+    for (k++; k < 216; k++)
+      if ((anym[k >> 5] >> (k & 31)) & 1u) break;
+    return k;
+  };
+  auto offset_of = [&](int k) {
+    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+  };
+  // source pointer of each column for cube index k (zeros when empty)
+  auto set_sources = [&](int k) {
+    if (grp == 0) {
+      const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+      const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
+      const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) |
+                               (spread3(tk + dz) << 2))
+                            : 0;
+      srcp[col] = v ? mps + sb * (uint64_t)C : zero;
+    }
+    if (lane <= N) {
+      double sn, cn;
+      sincos((double)lane * P->chi[offset_of(k)], &sn, &cn);
+      csn[2 * lane] = cn;
+      csn[2 * lane + 1] = sn;
+    }
+  };
+  double pre_re[NPU], pre_im[NPU];
+  // Branch-free: the last lanes repeat the last unit (same value to the
+  // same slot); the im part is always loaded (for m = 0 it is the next
+  // coefficient, multiplied by sin 0 = 0 and written to the dummy slot).
+  auto load_sources = [&](const int2 *tu, int ln) {
 #pragma unroll
-        for (int row = grp; row < ROWS; row += 4)
-          buf[row * 16 + col] = (v && row < C) ? mps[coef_at(sb, row, C)] : 0.0;
-        __syncthreads();
+    for (int i = 0; i < NPU; i++) {
+      const int U = min(ln + 64 * i, 16 * NU - 1);
+      const int j = U / NU, u = U % NU;
+      const double *p = srcp[j] + (tu[u].x & 0xffff);
+      pre_re[i] = p[0];
+      pre_im[i] = p[1];
+    }
+  };
+  // rotated by Zin into the stage-1 slots
+  auto deposit = [&](const int2 *tu, int ln) {
+#pragma unroll
+    for (int i = 0; i < NPU; i++) {
+      const int U = min(ln + 64 * i, 16 * NU - 1);
+      const int j = U / NU, u = U % NU;
+      const int2 inf = tu[u];
+      const int m = inf.x >> 16;
+      const double cn = csn[2 * m], sn = csn[2 * m + 1];
+      R[(inf.y & 0xffff) * 16 + j] = cn * pre_re[i] - sn * pre_im[i];
+      R[(inf.y >> 16) * 16 + j] = sn * pre_re[i] + cn * pre_im[i];
+    }
+  };
 
-        // stage 1: rotate into the offset's frame
-        d4 y[NT];
-        const double *a1 = A1 + (size_t)o * TP.T13 * 64 + lane;
+  double acc_re[NAU], acc_im[NAU];
 #pragma unroll
-        for (int i = 0; i < NT; i++) {
-          d4 r4 = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int j = 0; j < TP.nk1[i]; j++)
-            r4 = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                a1[(TP.off1[i] + j) * 64],
-                buf[(TP.kb1[i] + 4 * j + grp) * 16 + col], r4, 0, 0, 0);
-          y[i] = r4;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NT; i++)
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int r = 16 * i + grp + 4 * q;
-            if (r < C) buf[s_mpos[r] * 16 + col] = y[i][q] * ipw[s_degn[r]];
-          }
-        __syncthreads();
+  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = 0.0;
 
-        // stage 2: coaxial S|R shift, m-major
-        const double *a2 = A2 + lane;
-#pragma unroll
-        for (int i = 0; i < NT; i++) {
-          d4 r4 = d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int j = 0; j < TP.nk2[i]; j++)
-            r4 = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                a2[(TP.off2[i] + j) * 64],
-                buf[(TP.kb2[i] + 4 * j + grp) * 16 + col], r4, 0, 0, 0);
-          y[i] = r4;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NT; i++)
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int r = 16 * i + grp + 4 * q;
-            if (r < C) buf[s_npos[r] * 16 + col] = y[i][q] * ipw[s_degm[r] + 1];
-          }
-        __syncthreads();
+  int k = next_k(-1);
+  if (k < 216) {
+    set_sources(k);
+    load_sources(uinf, lane);
+    deposit(uinf, lane);
+  }
+  while (k < 216) {
+    // Views of the constant LDS tables through an offset the compiler cannot
+    // see (0, from inline asm, fresh every iteration): the ~200 per-lane
+    // table values are then re-read where used instead of being hoisted out
+    // of the loop into registers.
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int *tm1 = map1 + z0, *tm2 = map2 + z0, *tm3 = map3 + z0;
+    const int2 *tu = uinf + z0;
+    // lane-derived indices are recomputed each iteration for the same reason
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4;
+    const int o = offset_of(k);
+    const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
+    const double chi = P->chi[o];
+    if (lane <= N + 1) ipw[lane] = exp(-(double)lane * log(P->r[o] * w));
+    if (lane <= N) {
+      double sn, cn;
+      sincos((double)lane * chi, &sn, &cn);
+      csc[2 * lane] = cn;
+      csc[2 * lane + 1] = sn;
+    }
+    const int kn = next_k(k);
 
-        // stage 3: rotate back and accumulate
-        const double *a3 = A3 + (size_t)o * TP.T13 * 64 + lane;
+    // stage 1: rotate into the offset's frame
+    d4 y[NB];
+    stage<NB, KT>(BN, A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
-        for (int i = 0; i < NT; i++) {
+    for (int b = 0; b < NB; b++)
 #pragma unroll
-          for (int j = 0; j < TP.nk1[i]; j++)
-            acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                a3[(TP.off1[i] + j) * 64],
-                buf[(TP.kb1[i] + 4 * j + grp) * 16 + col], acc[i], 0, 0, 0);
-        }
+      for (int q = 0; q < 4; q++) {
+        const int mp = tm1[16 * b + gp + 4 * q];
+        R[(mp >> 6) * 16 + cl] = y[b][q] * ipw[mp & 63];
+      }
+    if (SH::PREFETCH && kn < 216) {
+      set_sources(kn);
+      load_sources(tu, ln);
+    }
+
+    // stage 2: coaxial S|R
+    stage<NB, KT>(BN, A2 + z0, R, ln, gp, cl, y);
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int mp = tm2[16 * b + gp + 4 * q];
+        R[(mp >> 6) * 16 + cl] = y[b][q] * ipw[mp & 63];
       }
 
-  if (!tvalid) return;
+    // stage 3: rotate back (without the azimuthal factor)
+    stage<NB, KT>(BN, A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
-  for (int i = 0; i < NT; i++)
+    for (int b = 0; b < NB; b++)
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int r = 16 * i + grp + 4 * q;
-      if (r < C) mpr[coef_at(tb, r, C)] += acc[i][q];
+      for (int q = 0; q < 4; q++) {
+        const int mp = tm3[16 * b + gp + 4 * q];
+        R[(mp >> 6) * 16 + cl] = y[b][q];
+      }
+
+    // Zout and accumulate: unit u = (n, nu) of this lane's target
+#pragma unroll
+    for (int i = 0; i < NAU; i++) {
+      // the last lanes repeat unit NU-1 and are not written out
+      const int u = min(gp + 4 * i, NU - 1);
+      const int m = tu[u].x >> 16;
+      const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+      const double cn = csc[2 * m], sn = csc[2 * m + 1];
+      acc_re[i] += cn * tr + sn * tq;
+      acc_im[i] += cn * tq - sn * tr;
     }
+
+    if (kn < 216) {
+      if (!SH::PREFETCH) {
+        set_sources(kn);
+        load_sources(tu, ln);
+      }
+      deposit(tu, ln);
+    }
+    k = kn;
+  }
+
+  if (!tvalid) return;
+  double *out = mpr + (size_t)tb * C;
+#pragma unroll
+  for (int i = 0; i < NAU; i++) {
+    const int u = grp + 4 * i;
+    if (u < NU) {
+      const int2 inf = uinf[u];
+      out[inf.x & 0xffff] += acc_re[i];
+      if (inf.x >> 16) out[(inf.x & 0xffff) + 1] += acc_im[i];
+    }
+  }
+}
+
+// occupancy bitmap of a level: bit b set when box b holds points
+__global__ void k_occ_bits(const uint32_t *start, const uint32_t *end,
+                           uint64_t nbox, uint32_t *bits) {
+  const uint64_t wd = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (wd * 32 >= nbox) return;
+  uint32_t v = 0;
+  for (int j = 0; j < 32; j++) {
+    const uint64_t b = wd * 32 + j;
+    if (b < nbox && end[b] > start[b]) v |= 1u << j;
+  }
+  bits[wd] = v;
 }
 
 // one 16x16x4 product with known operands; out gets D in lane order
@@ -253,31 +382,34 @@ __global__ void k_class_count(const uint32_t *list, unsigned n,
 }
 
 template <int N>
-static bool tiling_matches(const wbfmm_m2l_plan_t *hp) {
-  constexpr Tiling<N> t = make_tiling<N>();
-  if (hp->N != N || hp->T13 != t.T13 || hp->T2 != t.T2) return false;
-  for (int i = 0; i < Tiling<N>::NT; i++)
-    if (hp->kb1[i] != t.kb1[i] || hp->nk1[i] != t.nk1[i] || hp->off1[i] != t.off1[i] ||
-        hp->kb2[i] != t.kb2[i] || hp->nk2[i] != t.nk2[i] || hp->off2[i] != t.off2[i])
-      return false;
+static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
+  constexpr Bins B = make_bins(N);
+  if (hp->N != N || hp->NB != B.nb || hp->KT != B.kt || hp->S != M2LShape<N>::S) return false;
+  for (int b = 0; b < B.nb; b++)
+    if (hp->nk[b] != B.nk[b] || hp->koff[b] != B.koff[b]) return false;
   return true;
 }
 
 template <int N>
 static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
-                      const uint32_t *sstart, const uint32_t *send, int level,
-                      double w, const double *mps, double *mpr,
-                      const void *plan, const double *A1, const double *A3,
-                      const double *A2, const wbfmm_m2l_plan_t *hplan) {
-  if (!tiling_matches<N>(hplan)) {
-    fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's tiling\n", N);
+                      const uint32_t *occ, int level, double w,
+                      const double *mps, double *mpr, const void *plan,
+                      const double *A1, const double *A3, const double *A2,
+                      const double *zero, const wbfmm_m2l_plan_t *hplan) {
+  if (!plan_matches<N>(hplan)) {
+    fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
   WBFMM_LAUNCH("lap_m2l", k_lap_m2l_mfma<N>, dim3(nchunks), dim3(64), 0, cls,
-               ca, sstart, send, level, w, mps, mpr,
-               (const wbfmm_m2l_plan_t *)plan, A1, A3, A2);
+               ca, occ, level, w, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+               A1, A3, A2, zero);
   return 0;
 }
+
+// device scratch kept between calls: occupancy bits and a zero expansion
+static uint32_t *g_occ = nullptr;
+static size_t g_occ_words = 0;
+static double *g_zero = nullptr;
 
 }  // namespace wbfmm
 
@@ -342,13 +474,28 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
   }
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
-  double w = D / (double)(1u << level);
+  if (level > 20) return WBFMM_HIP_ERR_ARG;
+  const double w = D / (double)(1u << level);
+  const uint64_t nbox = (uint64_t)1 << (3 * level);
+  const size_t words = (size_t)((nbox + 31) / 32);
+  if (words > g_occ_words) {
+    if (g_occ) (void)hipFree(g_occ);
+    g_occ = nullptr;
+    WBFMM_CHECK(hipMalloc(&g_occ, words * sizeof(uint32_t)));
+    g_occ_words = words;
+  }
+  if (!g_zero) {
+    WBFMM_CHECK(hipMalloc(&g_zero, sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1)));
+    WBFMM_CHECK(hipMemsetAsync(g_zero, 0, sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1), stream()));
+  }
+  WBFMM_LAUNCH("m2l_occ_bits", k_occ_bits, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+               sstart, send, nbox, g_occ);
   switch (N) {
 #define CASE(n) \
   case n:       \
-    return launch_m2l<n>(cls, ca, nch, sstart, send, (int)level, w, mps, mpr, plan_dev, A1, A3, A2, plan_host);
+    return launch_m2l<n>(cls, ca, nch, g_occ, (int)level, w, mps, mpr, plan_dev, A1, A3, A2, g_zero, plan_host);
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
 #undef CASE
     default:
       return WBFMM_HIP_ERR_ORDER;

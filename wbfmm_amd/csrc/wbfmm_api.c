@@ -89,7 +89,7 @@ typedef struct {
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
 #define M2L_MFMA_MIN 2
-#define M2L_MFMA_MAX 16
+#define M2L_MFMA_MAX 15
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
 
@@ -467,8 +467,23 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
   tree_dev *td = TD(t);
   void *src = which ? t->mpr[level] : t->mps[level];
   size_t bytes = which ? td->mpr_bytes[level] : td->mps_bytes[level];
+  size_t nb = (size_t)1 << (3 * level), C, g, c, k;
+  double *tmp;
   if (!src) return -1;
-  HIPOK(wbfmm_hip_memcpy(host, src, bytes));
+  C = bytes / sizeof(double) / nb;
+  tmp = malloc(bytes);
+  if (!tmp) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_memcpy(tmp, src, bytes));
+  /* device box-major -> reference groups of 8 siblings
+     (src/boxes.c:552-579): [group][coefficient][child]; level 0 has a
+     single box */
+  if (nb < 8) memcpy(host, tmp, bytes);
+  else
+    for (g = 0; g < nb / 8; g++)
+      for (k = 0; k < 8; k++)
+        for (c = 0; c < C; c++)
+          host[(g * C + c) * 8 + k] = tmp[(8 * g + k) * C + c];
+  free(tmp);
   return 0;
 }
 
@@ -479,9 +494,10 @@ static void set_box_pointers(tree_dev *td, unsigned l, int which, int C)
   double *c = which ? (double *)t->mpr[l] : (double *)t->mps[l];
   wbfmm_box_t *boxes = t->boxes[l];
   if (!boxes) return;
-  /* coefficients packed in groups of 8 siblings (src/laplace.c:823-849) */
+  /* device storage is box-major (the reference interleaves groups of 8
+     siblings, src/laplace.c:823-849; the download restores that order) */
   for (i = 0; i < nb; i++) {
-    double *p = c + (i / 8) * 8 * (size_t)C + (i % 8);
+    double *p = c + i * (size_t)C;
     if (which) boxes[i].mpr = p; else boxes[i].mps = p;
   }
 }

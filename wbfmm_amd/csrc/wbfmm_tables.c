@@ -274,150 +274,252 @@ int wbfmm_hip_laplace_coaxial_tables(int L, double **dev)
 /* ------------------------------------------------------------------
  * MFMA form of the Laplace M2L (v_mfma_f64_16x16x4f64 operands).
  *
- * The three factors of one translation (rotate, coaxial S|R, rotate
- * back; src/pass-laplace.c:73-101) are applied to 16 boxes at a time as
- * 16x16x4 FP64 matrix products.  Rows are padded to 16-row tiles; the
- * inputs of a tile span whole degree blocks (rotations) or order blocks
- * (coaxial), so each tile needs a short run of 4-wide k steps.  The A
- * operand of each (tile, k step) is stored in lane order: lane l holds
- * A[l % 16][l / 16].  The coaxial factor is stored at unit distance; the
- * kernel scales its inputs by t^-nu and outputs by t^-(n+1).
+ * One translation of the reference (rotate, coaxial S|R, rotate back;
+ * src/pass-laplace.c:73-101, src/rotations-laplace.c:36-216,
+ * src/laplace.c:651-696) is factored as
+ *   L += Zout(chi) H3(theta) T(t) H1(theta) Zin(chi) M
+ * where Zin/Zout are the azimuthal phases (applied by the kernel on the
+ * vector units), H1 = real blocks of e^{-i nu phi} H(theta) with
+ * phi = -pi/2 and H3 those of H(theta) e^{i m phi'} with phi' = -pi/2: the
+ * inverse rotation swaps the two azimuths, so the pair is exact for any
+ * choice of phi.  With phi = -pi/2 the real (2n+1)^2 block of degree n
+ * maps the real parts (m = 0..n) and the imaginary parts (m = 1..n)
+ * separately: a cos block of size n+1 and a sin block of size n.  T is the
+ * unit-distance coaxial S|R shift (one block per order m, shared by the
+ * two parts); the kernel scales its inputs by t^-n and outputs by
+ * t^-(n+1).  Blocks are bin-packed into 16-row MFMA tiles.
  * ------------------------------------------------------------------ */
 
-static void tile_ranges(int C, int NT, const int *blk_lo, const int *blk_hi,
-                        int *kb, int *nk, int *off, int *total)
+/* first-fit decreasing packing of nblk blocks (sizes sz) into bins of 16
+   rows; ties in size keep block order.  bin[i] gets the bin of block i,
+   pos[i] its first slot within the bin; returns the number of bins and
+   fills fill[b] (rows used by bin b). */
+static int pack_bins(int nblk, const int *sz, int *bin, int *pos, int *fill)
 {
-  int i, acc = 0;
-  for (i = 0; i < NT; i++) {
-    int r0 = 16 * i, r1 = 16 * i + 15, lo, hi;
-    if (r1 >= C) r1 = C - 1;
-    lo = blk_lo[r0];
-    hi = blk_hi[r1];
-    kb[i] = lo;
-    nk[i] = (hi - lo + 3) / 4;
-    off[i] = acc;
-    acc += nk[i];
+  int order[2 * WBFMM_M2L_MAXN + 2], i, j, nb = 0;
+  for (i = 0; i < nblk; i++) order[i] = i;
+  for (i = 1; i < nblk; i++) {        /* stable insertion sort, size desc */
+    int v = order[i];
+    for (j = i; j > 0 && sz[order[j - 1]] < sz[v]; j--) order[j] = order[j - 1];
+    order[j] = v;
   }
-  *total = acc;
+  for (i = 0; i < nblk; i++) {
+    int k = order[i], b;
+    for (b = 0; b < nb; b++)
+      if (fill[b] + sz[k] <= 16) break;
+    if (b == nb) fill[nb++] = 0;
+    bin[k] = b;
+    pos[k] = fill[b];
+    fill[b] += sz[k];
+  }
+  return nb;
 }
 
-/* store rows [16i,16i+16) x cols [kb+4j, kb+4j+4) of the dense C x C
-   matrix M (row-major, zero outside) in MFMA lane order */
-static void swizzle(double *dst, const double *M, int C, int NT,
-                    const int *kb, const int *nk)
+/* blocks of stages 1 and 3: 0..N cos part of degree n (size n+1),
+   N+1..2N sin part of degree n = i-N (size n); stage 2: 0..N order m
+   cos part (size N+1-m), N+1..2N order m = i-N sin part */
+static int blk_size13(int N, int i) { return i <= N ? i + 1 : i - N; }
+static int blk_size2(int N, int i) { return i <= N ? N + 1 - i : N + 1 - (i - N); }
+
+typedef struct {
+  int nb, bin[2 * WBFMM_M2L_MAXN + 2], pos[2 * WBFMM_M2L_MAXN + 2];
+  int fill[WBFMM_M2L_MAXB];
+} packing;
+
+static int do_pack(int N, int stage2, packing *pk)
 {
-  int i, j, l;
-  for (i = 0; i < NT; i++)
-    for (j = 0; j < nk[i]; j++)
+  int sz[2 * WBFMM_M2L_MAXN + 2], i, nblk = 2 * N + 1;
+  for (i = 0; i < nblk; i++) sz[i] = stage2 ? blk_size2(N, i) : blk_size13(N, i);
+  memset(pk->fill, 0, sizeof(pk->fill));
+  pk->nb = pack_bins(nblk, sz, pk->bin, pk->pos, pk->fill);
+  return pk->nb;
+}
+
+/* slot of entity (n, nu, part) (part 0 cos, 1 sin) as an input (k) or
+   output (row) of each stage; blocks hold their entities in order */
+static int slot13(const packing *p, int N, int n, int idx, int part)
+{
+  int blk = part ? N + n : n;             /* sin part of degree n: N + n */
+  return 16 * p->bin[blk] + p->pos[blk] + (part ? idx - 1 : idx);
+}
+static int slot2(const packing *p, int N, int n, int m, int part)
+{
+  int blk = part ? N + m : m;
+  return 16 * p->bin[blk] + p->pos[blk] + (n - m);
+}
+
+/* row of the cos/sin part of degree n block of real_blocks() output
+   (stage 1, phi = -pi/2): cos rows re(nu) for even nu, im(nu) odd nu */
+static int h1_row(int nu, int part)
+{
+  if (!part) return nu == 0 ? 0 : (nu % 2 == 0 ? 2 * nu - 1 : 2 * nu);
+  return nu % 2 == 0 ? 2 * nu : 2 * nu - 1;
+}
+static int h1_col(int m, int part) { return part ? 2 * m : (m == 0 ? 0 : 2 * m - 1); }
+/* stage 3 (phi' = -pi/2 on the input side): rows tr/ti, cols cos inputs
+   re(m) for even m, im(m) odd m; sin inputs im(m) even, re(m) odd */
+static int h3_row(int nu, int part) { return part ? 2 * nu : (nu == 0 ? 0 : 2 * nu - 1); }
+static int h3_col(int m, int part)
+{
+  if (!part) return m == 0 ? 0 : (m % 2 == 0 ? 2 * m - 1 : 2 * m);
+  return m % 2 == 0 ? 2 * m : 2 * m - 1;
+}
+
+/* store the 16 x 4nk tile of bin b from the dense slot matrix T
+   (16*NB x 16*NB, row-major) in lane order */
+static void put_tiles(double *dst, const double *T, const wbfmm_m2l_plan_t *P)
+{
+  int b, j, l, S = 16 * P->NB;
+  for (b = 0; b < P->NB; b++)
+    for (j = 0; j < P->nk[b]; j++)
       for (l = 0; l < 64; l++) {
-        int r = 16 * i + (l % 16), c = kb[i] + 4 * j + (l / 16);
-        *dst++ = (r < C && c < C) ? M[(size_t)r * C + c] : 0.0;
+        int r = 16 * b + l % 16, c = 16 * b + 4 * j + l / 16;
+        dst[((size_t)P->koff[b] + j) * 64 + l] = T[(size_t)r * S + c];
       }
+}
+
+int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
+                             double *A3, double *A2)
+{
+  packing p13, p2;
+  int b, n, nu, m, part, i, dx, dy, dz, S;
+  double *T, *R, *Rb, ths[343];
+  hcache hc;
+  /* every block must fit one 16-row bin: cos block of degree N has N+1 rows */
+  if (N < 1 || N > 15) return WBFMM_HIP_ERR_ORDER;
+  memset(P, 0, sizeof(*P));
+  P->N = N;
+  P->C = (N + 1) * (N + 1);
+  P->NU = (N + 1) * (N + 2) / 2;
+  do_pack(N, 0, &p13);
+  do_pack(N, 1, &p2);
+  /* the two block multisets are equal, so are the bin fills */
+  if (p13.nb != p2.nb || p13.nb > WBFMM_M2L_MAXB) return WBFMM_HIP_ERR_ORDER;
+  for (b = 0; b < p13.nb; b++)
+    if (p13.fill[b] != p2.fill[b]) return WBFMM_HIP_ERR_ORDER;
+  P->NB = p13.nb;
+  for (b = 0; b < P->NB; b++) {
+    P->nk[b] = (p13.fill[b] + 3) / 4;
+    P->koff[b] = P->KT;
+    P->KT += P->nk[b];
+  }
+  S = 16 * P->NB;
+  P->S = S > 2 * P->NU ? S : 2 * P->NU;
+  for (i = 0; i < S; i++) P->map1[i] = P->map2[i] = P->map3[i] = P->S << 6;
+  for (n = 0; n <= N; n++)
+    for (part = 0; part < 2; part++)
+      for (nu = part; nu <= n; nu++) {
+        /* stage-1 output (n,nu,part) -> stage-2 input, scale t^-n */
+        P->map1[slot13(&p13, N, n, nu, part)] = (slot2(&p2, N, n, nu, part) << 6) | n;
+        /* stage-2 output -> stage-3 input, scale t^-(n+1) */
+        P->map2[slot2(&p2, N, n, nu, part)] = (slot13(&p13, N, n, nu, part) << 6) | (n + 1);
+        /* stage-3 output tr/ti(n,nu) -> slot u or NU + u */
+        P->map3[slot13(&p13, N, n, nu, part)] =
+            ((part ? P->NU : 0) + n * (n + 1) / 2 + nu) << 6;
+      }
+  for (n = 0; n <= N; n++)
+    for (m = 0; m <= n; m++)
+      P->s1[n * (n + 1) / 2 + m] =
+          slot13(&p13, N, n, m, 0) | ((m ? slot13(&p13, N, n, m, 1) : P->S) << 16);
+
+  /* theta of each list-4 offset; e = target - source = -(dx,dy,dz) */
+  P->NTH = 0;
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3, k;
+        double th, ph, ch;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        P->r[o] = sqrt((double)(dx * dx + dy * dy + dz * dz));
+        dir_angles(-dx, -dy, -dz, &th, &ph, &ch);
+        P->chi[o] = (dx == 0 && dy == 0) ? 0.0 : ch;
+        /* collinear offsets, e.g. (2,2,2) and (3,3,3), share theta up to
+           rounding: 49 distinct angles as in src/shift.c:637-686 */
+        for (k = 0; k < P->NTH; k++)
+          if (fabs(ths[k] - th) < 1e-12) break;
+        if (k == P->NTH) ths[P->NTH++] = th;
+        P->th[o] = k;
+      }
+  if (!A1 && !A2 && !A3) return 0;
+
+  T = calloc((size_t)S * S, sizeof(double));
+  if (!T) return WBFMM_HIP_ERR_NOMEM;
+  if (A2) {   /* stage 2: unit-distance S|R, same block for both parts */
+    memset(T, 0, sizeof(double) * (size_t)S * S);
+    for (m = 0; m <= N; m++)
+      for (part = 0; part < (m ? 2 : 1); part++)
+        for (n = m; n <= N; n++)
+          for (nu = m; nu <= N; nu++)
+            T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] =
+                lap_cfft(2, n, nu, m);
+    put_tiles(A2, T, P);
+  }
+  if (A1 || A3) {
+    size_t per = (size_t)P->KT * 64, nr = (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3;
+    int k;
+    R = malloc(sizeof(double) * nr);
+    Rb = malloc(sizeof(double) * nr);
+    hc.n = 0;
+    hc.L = N;
+    for (k = 0; k < P->NTH; k++) {
+      const double *H = cached_H(&hc, ths[k]);
+      real_blocks(R, H, N, -0.5 * M_PI, 0.0);
+      real_blocks(Rb, H, N, 0.0, -0.5 * M_PI);
+      if (A1) {
+        memset(T, 0, sizeof(double) * (size_t)S * S);
+        for (n = 0; n <= N; n++) {
+          const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+          int w = 2 * n + 1;
+          for (part = 0; part < 2; part++)
+            for (nu = part; nu <= n; nu++)
+              for (m = part; m <= n; m++)
+                T[(size_t)slot13(&p13, N, n, nu, part) * S + slot13(&p13, N, n, m, part)] =
+                    B[h1_row(nu, part) * w + h1_col(m, part)];
+        }
+        put_tiles(A1 + k * per, T, P);
+      }
+      if (A3) {
+        memset(T, 0, sizeof(double) * (size_t)S * S);
+        for (n = 0; n <= N; n++) {
+          const double *B = Rb + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+          int w = 2 * n + 1;
+          for (part = 0; part < 2; part++)
+            for (nu = part; nu <= n; nu++)
+              for (m = part; m <= n; m++)
+                T[(size_t)slot13(&p13, N, n, nu, part) * S + slot13(&p13, N, n, m, part)] =
+                    B[h3_row(nu, part) * w + h3_col(m, part)];
+        }
+        put_tiles(A3 + k * per, T, P);
+      }
+    }
+    for (i = 0; i < hc.n; i++) free(hc.H[i]);
+    free(R);
+    free(Rb);
+  }
+  free(T);
+  return 0;
 }
 
 int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *P, void **plan_dev,
                            double **A1, double **A3, double **A2)
 {
-  int C = (N + 1) * (N + 1), NT = (C + 15) / 16, n, m, r, i, dx, dy, dz, rc;
-  int lo_n[WBFMM_M2L_MAXC], hi_n[WBFMM_M2L_MAXC];
-  int lo_m[WBFMM_M2L_MAXC], hi_m[WBFMM_M2L_MAXC];
-  double *M, *h1, *h3, *h2, *R, *Rb, *H;
-  hcache hc;
-  size_t per;
-  if (C > WBFMM_M2L_MAXC || NT > WBFMM_M2L_MAXT) return WBFMM_HIP_ERR_ORDER;
-  memset(P, 0, sizeof(*P));
-  P->N = N;
-  P->C = C;
-  P->NT = NT;
-  P->CP = 16 * NT;
-  /* n-major rows: degree blocks [n^2, (n+1)^2) */
-  for (n = 0; n <= N; n++)
-    for (r = n * n; r < (n + 1) * (n + 1); r++) {
-      lo_n[r] = n * n;
-      hi_n[r] = (n + 1) * (n + 1);
-      P->degn[r] = n;
-    }
-  /* m-major rows: m=0 (n=0..N), then for m>=1 re (n=m..N), im (n=m..N) */
-  r = 0;
-  for (m = 0; m <= N; m++) {
-    int part, nparts = (m == 0) ? 1 : 2;
-    for (part = 0; part < nparts; part++) {
-      int start = r, len = N + 1 - m;
-      for (n = m; n <= N; n++, r++) {
-        int nrow = (m == 0) ? n * n : n * n + 2 * m - 1 + part;
-        P->npos[r] = nrow;
-        P->mpos[nrow] = r;
-        P->degm[r] = n;
-        lo_m[r] = start;
-        hi_m[r] = start + len;
-      }
-    }
-  }
-  tile_ranges(C, NT, lo_n, hi_n, P->kb1, P->nk1, P->off1, &P->T13);
-  tile_ranges(C, NT, lo_m, hi_m, P->kb2, P->nk2, P->off2, &P->T2);
-
-  /* stage 2: unit-distance S|R in m-major order */
-  M = calloc((size_t)C * C, sizeof(double));
-  for (r = 0; r < C; r++) {
-    int n1 = P->degm[r], c;
-    for (c = lo_m[r]; c < hi_m[r]; c++) {
-      int nu = P->degm[c], mm;
-      /* order m of this block: rows of one block share m */
-      int nrow = P->npos[r];
-      mm = (nrow == n1 * n1) ? 0 : (nrow - n1 * n1 + 1) / 2;
-      M[(size_t)r * C + c] = lap_cfft(2, n1, nu, mm);
-    }
-  }
-  h2 = malloc(sizeof(double) * (size_t)P->T2 * 64);
-  swizzle(h2, M, C, NT, P->kb2, P->nk2);
-
-  /* stages 1 and 3: rotation blocks of every list-4 offset */
-  per = (size_t)P->T13 * 64;
-  h1 = calloc(343 * per, sizeof(double));
-  h3 = calloc(343 * per, sizeof(double));
-  R = malloc(sizeof(double) * (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3);
-  Rb = malloc(sizeof(double) * (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3);
-  hc.n = 0;
-  hc.L = N;
-  for (dx = -3; dx <= 3; dx++)
-    for (dy = -3; dy <= 3; dy++)
-      for (dz = -3; dz <= 3; dz++) {
-        int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
-        double th, ph, ch;
-        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
-        P->r[o] = sqrt((double)(dx * dx + dy * dy + dz * dz));
-        dir_angles(-dx, -dy, -dz, &th, &ph, &ch);
-        H = (double *)cached_H(&hc, th);
-        real_blocks(R, H, N, ph, ch);
-        real_blocks(Rb, H, N, ch, ph);
-        /* expand the degree blocks into dense C x C (row-major) */
-        memset(M, 0, sizeof(double) * (size_t)C * C);
-        for (n = 0; n <= N; n++) {
-          int w = 2 * n + 1, a, b;
-          const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
-          for (a = 0; a < w; a++)
-            for (b = 0; b < w; b++) M[(size_t)(n * n + a) * C + n * n + b] = B[a * w + b];
-        }
-        swizzle(h1 + o * per, M, C, NT, P->kb1, P->nk1);
-        memset(M, 0, sizeof(double) * (size_t)C * C);
-        for (n = 0; n <= N; n++) {
-          int w = 2 * n + 1, a, b;
-          const double *B = Rb + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
-          for (a = 0; a < w; a++)
-            for (b = 0; b < w; b++) M[(size_t)(n * n + a) * C + n * n + b] = B[a * w + b];
-        }
-        swizzle(h3 + o * per, M, C, NT, P->kb1, P->nk1);
-      }
-  for (i = 0; i < hc.n; i++) free(hc.H[i]);
-  free(R);
-  free(Rb);
-  free(M);
-  rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * 343 * per);
-  if (!rc) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * 343 * per);
-  if (!rc) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * 343 * per);
-  if (!rc) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * 343 * per);
-  if (!rc) rc = wbfmm_hip_malloc((void **)A2, sizeof(double) * (size_t)P->T2 * 64);
-  if (!rc) rc = wbfmm_hip_memcpy(*A2, h2, sizeof(double) * (size_t)P->T2 * 64);
+  double *h1, *h2, *h3;
+  size_t n13, n2;
+  int rc = wbfmm_hip_lap_m2l_tables(N, P, NULL, NULL, NULL);
+  if (rc) return rc;
+  n13 = (size_t)P->NTH * P->KT * 64;
+  n2 = (size_t)P->KT * 64;
+  h1 = malloc(sizeof(double) * n13);
+  h3 = malloc(sizeof(double) * n13);
+  h2 = malloc(sizeof(double) * n2);
+  if (!h1 || !h2 || !h3) rc = WBFMM_HIP_ERR_NOMEM;
+  if (!rc) rc = wbfmm_hip_lap_m2l_tables(N, P, h1, h3, h2);
+  if (!rc) rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * n13);
+  if (!rc) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * n13);
+  if (!rc) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * n13);
+  if (!rc) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * n13);
+  if (!rc) rc = wbfmm_hip_malloc((void **)A2, sizeof(double) * n2);
+  if (!rc) rc = wbfmm_hip_memcpy(*A2, h2, sizeof(double) * n2);
   if (!rc) rc = wbfmm_hip_malloc(plan_dev, sizeof(*P));
   if (!rc) rc = wbfmm_hip_memcpy(*plan_dev, P, sizeof(*P));
   free(h1);
