@@ -106,59 +106,70 @@ int wbfmm_hip_laplace_coaxial_tables(int L, double **dev);
      rotate  = Zout(phi) H(theta) Zin(chi),  phi = -pi/2 fixed,
    so the matrices depend on the polar angle theta only (49 values over
    the list-4 offsets) and the azimuthal factors Zin/Zout (2x2 rotations
-   of each (re, im) pair) are applied on the vector units.  With phi fixed,
-   each real degree block of H splits into a cos part ((n+1)x(n+1)) and a
-   sin part (n x n); the coaxial S|R shift is one (N+1-m)^2 block per
-   order m shared by both parts.  The blocks of each stage are bin-packed
-   (first fit, decreasing) into NB bins of at most 16 rows, each bin one
-   16 x (4 nk) MFMA tile (v_mfma_f64_16x16x4f64).
-   Slots: rows and k positions of bin b are slots 16 b + i.  Tables:
+   of each (re, im) pair by m chi) are applied on the vector units.  With
+   phi fixed, each real degree block of H splits into a cos part
+   ((n+1)x(n+1)) and a sin part (n x n); the coaxial S|R shift is one
+   (N+1-m)^2 block per order m shared by both parts.  The blocks of each
+   stage are bin-packed (first fit, decreasing) into NB bins of at most 16
+   rows, each bin one 16 x (4 nk) MFMA tile (v_mfma_f64_16x16x4f64).
+   Slots: rows and k positions of bin b are slots 16 b + i.
      S = max(16 NB, 2 NU) slots carry data between stages, slot S is a
        dummy destination for padding rows;
      map1/map2/map3[slot]: destination slot of a stage's output row in the
        next stage's k order (map3: tr(n,nu) -> u, ti(n,nu) -> NU + u with
-       u = n(n+1)/2 + nu), shifted left by 6, plus the degree used for the
-       t^-k scaling (padding rows: slot S, degree 0);
+       u = n(n+1)/2 + nu), shifted left by 6, plus the degree n (map1) or
+       n+1 (map2) of the t^-k factor the coaxial tables carry (padding
+       rows: slot S);
      s1[u]: stage-1 k slots of the rotated re and im parts of unit
        u = (n, m) (low and high 16 bits, high = S for m = 0);
-     th/chi/r: theta-table index, azimuth and length (box widths) of each
-       list-4 offset o = (dx+3)*49 + (dy+3)*7 + dz+3 (dx = source - target).
-   A operands in MFMA lane order (lane l holds A[l % 16][4 j + l / 16]):
-     A1, A3: NTH x KT x 64 (stages 1 and 3, per theta), A2: KT x 64. */
+     th/rix/chi/r: theta-table index, distance index, azimuth and length
+       (box widths) of each list-4 offset o = (dx+3)*49 + (dy+3)*7 + dz+3
+       (dx = source - target); rr[NR]: the distinct lengths (15).
+   Tables (MFMA lane order: lane l holds A[l % 16][4 j + l / 16]):
+     A1, A3: NTH x KT x 64 (stages 1 and 3, per theta);
+     A2: NR x KT x 64, the coaxial S|R at distance rr[i] w for one box
+       width w (wbfmm_hip_lap_m2l_coax);
+     CS: 343 x (N+1) x 2, cos and sin of m chi for each offset. */
 #define WBFMM_M2L_MAXN 20
 #define WBFMM_M2L_MAXB 32
 #define WBFMM_M2L_MAXU 231
 typedef struct {
-  int N, C, NU, NB, KT, NTH, S;
+  int N, C, NU, NB, KT, NTH, S, NR;
   int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB];
   int map1[16 * WBFMM_M2L_MAXB], map2[16 * WBFMM_M2L_MAXB],
       map3[16 * WBFMM_M2L_MAXB];
   int s1[WBFMM_M2L_MAXU];
-  int th[343];
+  int th[343], rix[343];
   double chi[343];
   double r[343];
+  double rr[32];
 } wbfmm_m2l_plan_t;
 /* builds the plan of order N (1 <= N <= 15, so that every block fits one
-   16-row bin) and, when the pointers
-   are not NULL, fills the host tables A1, A3 (NTH*KT*64 doubles each) and
-   A2 (KT*64); call once with NULL tables to learn the sizes */
+   16-row bin) and, for the pointers that are not NULL, fills the host
+   tables A1, A3 (NTH*KT*64 doubles each) and CS (343*(N+1)*2); call once
+   with NULL tables to learn the sizes */
 int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *plan, double *A1,
-                             double *A3, double *A2);
+                             double *A3, double *CS);
+/* host coaxial table A2 (NR*KT*64 doubles) of a plan for box width w */
+int wbfmm_hip_lap_m2l_coax(const wbfmm_m2l_plan_t *plan, double w,
+                           double *A2);
 /* plan and tables of order N uploaded to the device (plan_dev: device copy
    of *plan) */
 int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *plan, void **plan_dev,
-                           double **A1, double **A3, double **A2);
+                           double **A1, double **A3, double **CS);
 /* MFMA M2L for target boxes grouped by Morton child digit: cls holds the
    targets of class c at [cstart[c], cstart[c]+ccount[c]); 16 targets of
    one class per wavefront, accumulating into mpr.  Expansions are stored
    box-major (box b at b*C).  sstart/send: dense point ranges of the
-   level's boxes (empty boxes are skipped). */
+   level's boxes (empty boxes are skipped); A2: device coaxial table for
+   this level's box width D/2^level. */
 int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const unsigned *ccount, const uint32_t *sstart,
                            const uint32_t *send, unsigned level, double D,
                            int N, const double *mps, double *mpr,
                            const void *plan_dev, const double *A1,
                            const double *A3, const double *A2,
+                           const double *CS,
                            const wbfmm_m2l_plan_t *plan_host);
 /* one v_mfma_f64_16x16x4f64 with A[i][k] = 1+i+100k, B[k][j] = 1+3k+1000j;
    host_out (256 doubles) gets the four result values of each lane */

@@ -14,18 +14,19 @@ import numpy as np
 import pytest
 
 
-def emulate(P, A1, A3, A2, M, o, w):
+def emulate(P, A1, A3, A2, CS, M, o):
     N, NU, NB = P.N, P.NU, P.NB
     S = P.S + 1              # slot P.S is the dummy destination
     assert P.S == max(16 * NB, 2 * NU)
-    th, chi, t = P.th[o], P.chi[o], P.r[o] * w
+    th, ri = P.th[o], P.rix[o]
+    assert P.rr[ri] == P.r[o]
     x = np.zeros(S)
     for n in range(N + 1):
         for m in range(n + 1):
             u = n * (n + 1) // 2 + m
             re = M[n * n + max(2 * m - 1, 0)]
             im = M[n * n + 2 * m] if m else 0.0
-            c, s = math.cos(m * chi), math.sin(m * chi)
+            c, s = CS[o, m]
             x[P.s1[u] & 0xffff] = c * re - s * im
             x[(P.s1[u] >> 16) & 0xffff] = s * re + c * im   # m = 0: dummy slot
 
@@ -37,22 +38,21 @@ def emulate(P, A1, A3, A2, M, o, w):
                 y[16 * b:16 * b + 16] += a @ v[16 * b + 4 * j:16 * b + 4 * j + 4]
         return y
 
-    def move(y, mp, shift):
+    def move(y, mp):
         z = np.zeros(S)
         for r in range(16 * NB):
-            k = mp[r] & 63
-            z[mp[r] >> 6] = y[r] * (t ** -k if shift else 1.0)
+            z[mp[r] >> 6] = y[r]
         return z
 
-    y = move(tiles(A1[th], x), P.map1, True)
-    z = move(tiles(A2, y), P.map2, True)
-    a = move(tiles(A3[th], z), P.map3, False)
+    y = move(tiles(A1[th], x), P.map1)
+    z = move(tiles(A2[ri], y), P.map2)
+    a = move(tiles(A3[th], z), P.map3)
     L = np.zeros((N + 1) ** 2)
     for n in range(N + 1):
         for nu in range(n + 1):
             u = n * (n + 1) // 2 + nu
             tr, ti = a[u], a[NU + u]
-            c, s = math.cos(nu * chi), math.sin(nu * chi)
+            c, s = CS[o, nu]
             if nu == 0:
                 L[n * n] += tr
             else:
@@ -63,7 +63,8 @@ def emulate(P, A1, A3, A2, M, o, w):
 
 @pytest.mark.parametrize("N", [1, 2, 5, 8, 12, 13, 15])
 def test_plan_matches_oracle_m2l(wb, oracle, N):
-    P, A1, A3, A2 = wb.m2l_tables(N)
+    w = 0.37
+    P, A1, A3, A2, CS = wb.m2l_tables(N, w)
     assert P.N == N and P.C == (N + 1) ** 2
     # every bin holds at most 16 rows and the k steps cover it
     assert sum(P.nk[b] for b in range(P.NB)) == P.KT and all(1 <= P.nk[b] <= 4 for b in range(P.NB))
@@ -71,8 +72,8 @@ def test_plan_matches_oracle_m2l(wb, oracle, N):
     for mp in (P.map1, P.map2, P.map3):
         dest = [v >> 6 for v in mp[:16 * P.NB] if (v >> 6) != P.S]
         assert len(dest) == len(set(dest)) == P.C and max(dest) < P.S
+    assert P.NR == 15
     rng = np.random.default_rng(N)
-    w = 0.37
     offs = [(2, 0, 0), (-3, 1, 2), (0, 0, 2), (0, 0, -3), (3, -3, 3), (2, 3, -2), (-2, -2, 0), (0, 3, 1)]
     for off in offs:
         o = (off[0] + 3) * 49 + (off[1] + 3) * 7 + off[2] + 3
@@ -85,7 +86,7 @@ def test_plan_matches_oracle_m2l(wb, oracle, N):
         H = oracle.H_rotation(N, th)
         Lref = oracle.lap_rotate(oracle.lap_coaxial(2, oracle.lap_rotate(M, N, H, ph, ch), N, N, r * w),
                                  N, H, ch, ph)
-        L = emulate(P, A1, A3, A2, M, o, w)
+        L = emulate(P, A1, A3, A2, CS, M, o)
         assert np.abs(L - Lref).max() < 1e-12 * np.abs(Lref).max(), off
 
 
@@ -96,8 +97,9 @@ def test_orders_beyond_one_bin_are_refused(wb):
 
 
 def test_theta_table_has_49_angles(wb):
-    P, _, _, _ = wb.m2l_tables(4)
+    P, _, _, _, CS = wb.m2l_tables(4, 1.0)
     assert P.NTH == 49
     # offsets on the z axis use chi = 0; r is the offset length
     o = (0 + 3) * 49 + (0 + 3) * 7 + (2 + 3)
     assert P.chi[o] == 0.0 and P.r[o] == 2.0
+    assert np.allclose(CS[o], [[1.0, 0.0]] * 5)

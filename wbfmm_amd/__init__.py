@@ -52,28 +52,34 @@ class M2LPlan(ctypes.Structure):
     """wbfmm_m2l_plan_t (include/wbfmm_hip.h): plan of the MFMA M2L."""
     _fields_ = [
         ("N", _i), ("C", _i), ("NU", _i), ("NB", _i), ("KT", _i), ("NTH", _i), ("S", _i),
+        ("NR", _i),
         ("nk", _i * M2L_MAXB), ("koff", _i * M2L_MAXB),
         ("map1", _i * (16 * M2L_MAXB)), ("map2", _i * (16 * M2L_MAXB)),
         ("map3", _i * (16 * M2L_MAXB)),
         ("s1", _i * M2L_MAXU),
-        ("th", _i * 343), ("chi", _d * 343), ("r", _d * 343),
+        ("th", _i * 343), ("rix", _i * 343), ("chi", _d * 343), ("r", _d * 343),
+        ("rr", _d * 32),
     ]
 
 
-def m2l_tables(N):
-    """host copy of the M2L plan and A tables of order N:
-    (plan, A1 (NTH, KT, 64), A3 (NTH, KT, 64), A2 (KT, 64))"""
+def m2l_tables(N, w):
+    """host copy of the M2L plan and tables of order N for box width w:
+    (plan, A1 (NTH, KT, 64), A3 (NTH, KT, 64), A2 (NR, KT, 64),
+    CS (343, N+1, 2))"""
     P = M2LPlan()
     rc = lib().wbfmm_hip_lap_m2l_tables(N, ctypes.byref(P), None, None, None)
     if rc != 0:
         raise RuntimeError(f"wbfmm_hip_lap_m2l_tables({N}) = {rc}")
     A1 = np.zeros((P.NTH, P.KT, 64))
     A3 = np.zeros((P.NTH, P.KT, 64))
-    A2 = np.zeros((P.KT, 64))
-    rc = lib().wbfmm_hip_lap_m2l_tables(N, ctypes.byref(P), ptr(A1), ptr(A3), ptr(A2))
+    CS = np.zeros((343, N + 1, 2))
+    A2 = np.zeros((P.NR, P.KT, 64))
+    rc = lib().wbfmm_hip_lap_m2l_tables(N, ctypes.byref(P), ptr(A1), ptr(A3), ptr(CS))
+    if rc == 0:
+        rc = lib().wbfmm_hip_lap_m2l_coax(ctypes.byref(P), ctypes.c_double(w), ptr(A2))
     if rc != 0:
-        raise RuntimeError(f"wbfmm_hip_lap_m2l_tables({N}) = {rc}")
-    return P, A1, A3, A2
+        raise RuntimeError(f"M2L tables of order {N}: status {rc}")
+    return P, A1, A3, A2, CS
 
 
 _SIGS = {
@@ -122,6 +128,7 @@ _SIGS = {
     "wbfmm_hip_lap_direct": (_i, [_vp, _vp, _vp, _vp, _u, _vp, _u, _i, _d, _vp, _i]),
     "wbfmm_hip_mfma_layout_probe": (_i, [_vp]),
     "wbfmm_hip_lap_m2l_tables": (_i, [_i, _vp, _vp, _vp, _vp]),
+    "wbfmm_hip_lap_m2l_coax": (_i, [_vp, _d, _vp]),
 }
 
 

@@ -14,15 +14,19 @@
 //   stage 3 W = H3(theta) Z      (MFMA)
 //   Zout    L += rotation of each (tr, ti) pair by -nu chi (vector units)
 //
-// The matrices depend on the polar angle only (49 tables) and are read
-// from L2; the plan (slot maps, per-offset theta/chi/r) is built on the
-// host by wbfmm_hip_lap_m2l_tables.  One LDS buffer R of S slots x 16
-// columns carries the data between stages (slot s of column j at
-// R[16 s + j]).  Source expansions are box-major (box b at b*C); the next
-// offset's 16 sources are loaded into registers while the current offset
-// is computed (orders up to 12) and deposited, rotated by Zin, into R at
-// the end of the iteration.  The local expansions of the 16 targets stay
-// in registers for all offsets and are added to mpr once at the end.
+// The rotation matrices depend on the polar angle only (49 tables), the
+// coaxial ones on the offset length (15 tables per box width, t^-k factors
+// folded in), and all are read from L2; the per-offset azimuthal factors
+// (cos, sin of m chi) come from a table.  The plan (bins, slot maps,
+// per-offset theta/distance indices) is built on the host by
+// wbfmm_hip_lap_m2l_tables.  One LDS buffer R of S slots x 16 columns
+// carries the data between stages (slot s of column j at R[16 s + j]).
+// Source expansions are box-major (box b at b*C); lane (j, g) handles the
+// units u = g + 4 i of column j: it loads them for the next offset while
+// the current one is computed (orders up to 12), rotates them by Zin into
+// R, and accumulates the same units of its target after Zout.  The local
+// expansions of the 16 targets stay in registers for all offsets and are
+// added to mpr once at the end.
 //
 // MFMA operand layout (16x16x4 f64, checked by wbfmm_hip_mfma_layout_probe):
 // lane l supplies A[l%16][l/16] and B[l/16][l%16]; the result holds
@@ -92,8 +96,7 @@ struct M2LShape {
   static constexpr int C = (N + 1) * (N + 1);
   static constexpr int NU = (N + 1) * (N + 2) / 2;
   static constexpr int S = (16 * NB > 2 * NU) ? 16 * NB : 2 * NU;  // = plan S
-  static constexpr int NPU = (16 * NU + 63) / 64;  // source units per lane
-  static constexpr int NAU = (NU + 3) / 4;         // target units per lane
+  static constexpr int NAU = (NU + 3) / 4;         // units per lane
   // register prefetch of the next offset's sources while it fits
   static constexpr bool PREFETCH = N <= 12;
 };
@@ -128,36 +131,44 @@ __device__ __forceinline__ void stage(const Bins &BN, const double *__restrict__
 template <int N>
 __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const uint32_t *__restrict__ cls, ClassArgs ca,
-    const uint32_t *__restrict__ occ, int level, double w,
+    const uint32_t *__restrict__ occ, int level,
     const double *__restrict__ mps, double *__restrict__ mpr,
     const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
     const double *__restrict__ A3, const double *__restrict__ A2,
-    const double *__restrict__ zero) {
+    const double *__restrict__ CS, const double *__restrict__ zero) {
   using SH = M2LShape<N>;
   constexpr Bins BN = SH::BN;
   constexpr int NB = SH::NB, KT = SH::KT, C = SH::C, NU = SH::NU, S = SH::S;
-  constexpr int NPU = SH::NPU, NAU = SH::NAU;
+  constexpr int NAU = SH::NAU;
   __shared__ double R[(S + 1) * 16];  // slot S: dummy destination
-  __shared__ int map1[16 * NB], map2[16 * NB], map3[16 * NB];
-  __shared__ int2 uinf[NU];  // x: packed position of re | m << 16; y: stage-1 slots
-  __shared__ double ipw[N + 2];
-  __shared__ double csc[2 * (N + 1)], csn[2 * (N + 1)];
+  // slot maps transposed so that one lane reads its four rows (q = 0..3)
+  // of a bin at once: tmapX[4 b + grp] = slots of rows 16 b + grp + 4 q
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];  // packed position of re, m, stage-1 re slot, im slot
+  __shared__ double2 cs[2][N + 1];  // cos, sin of m chi: current / next offset
   __shared__ unsigned vm[16][7];
   __shared__ unsigned anym[7];
-  __shared__ const double *srcp[16];
 
   const int lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
 
-  for (int i = lane; i < 16 * NB; i += 64) {
-    map1[i] = P->map1[i];
-    map2[i] = P->map2[i];
-    map3[i] = P->map3[i];
+  for (int i = lane; i < 4 * NB; i += 64) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      t1[q] = P->map1[16 * b + g + 4 * q] >> 6;
+      t2[q] = P->map2[16 * b + g + 4 * q] >> 6;
+      t3[q] = P->map3[16 * b + g + 4 * q] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
   }
   for (int u = lane; u < NU; u += 64) {
     int n = 0;
     while ((n + 1) * (n + 2) / 2 <= u) n++;
     const int m = u - n * (n + 1) / 2;
-    uinf[u] = make_int2((n * n + (m ? 2 * m - 1 : 0)) | (m << 16), P->s1[u]);
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
   }
   for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = 0.0;
   for (int i = lane; i < 16 * 7; i += 64) vm[i / 7][i % 7] = 0u;
@@ -203,48 +214,52 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
     return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
   };
-  // source pointer of each column for cube index k (zeros when empty)
-  auto set_sources = [&](int k) {
-    if (grp == 0) {
-      const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
-      const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
-      const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) |
-                               (spread3(tk + dz) << 2))
-                            : 0;
-      srcp[col] = v ? mps + sb * (uint64_t)C : zero;
-    }
+  // this lane's source expansion for cube index k (zeros when empty) and
+  // the offset's azimuthal factors into cs[p]
+  auto source_of = [&](int k, int p) {
+    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
     if (lane <= N) {
-      double sn, cn;
-      sincos((double)lane * P->chi[offset_of(k)], &sn, &cn);
-      csn[2 * lane] = cn;
-      csn[2 * lane + 1] = sn;
+      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = make_double2(t[0], t[1]);
+    }
+    const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
+    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
+                          : 0;
+    return v ? mps + sb * (uint64_t)C : zero;
+  };
+  // Units u = grp + 4 i of column col: the lane loads, rotates (Zin) and
+  // accumulates the same units.  The last lanes repeat unit NU-1 (same
+  // value to the same slot) and are not written out.  The im part is
+  // always loaded (for m = 0 it is the next coefficient, multiplied by
+  // sin 0 = 0 and written to the dummy slot).
+  double pre_re[NAU], pre_im[NAU];
+  // units [i0, i1) of this lane
+  auto load_sources = [&](const double *src, const int4 *tu, int gp, int i0, int i1) {
+#pragma unroll
+    for (int i = i0; i < i1; i++) {
+      const int u = min(gp + 4 * i, NU - 1);
+      const double *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
     }
   };
-  double pre_re[NPU], pre_im[NPU];
-  // Branch-free: the last lanes repeat the last unit (same value to the
-  // same slot); the im part is always loaded (for m = 0 it is the next
-  // coefficient, multiplied by sin 0 = 0 and written to the dummy slot).
-  auto load_sources = [&](const int2 *tu, int ln) {
+  auto deposit = [&](const int4 *tu, int gp, int cl, int p, int i0, int i1) {
 #pragma unroll
-    for (int i = 0; i < NPU; i++) {
-      const int U = min(ln + 64 * i, 16 * NU - 1);
-      const int j = U / NU, u = U % NU;
-      const double *p = srcp[j] + (tu[u].x & 0xffff);
-      pre_re[i] = p[0];
-      pre_im[i] = p[1];
+    for (int i = i0; i < i1; i++) {
+      const int u = min(gp + 4 * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 e = cs[p][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
     }
   };
-  // rotated by Zin into the stage-1 slots
-  auto deposit = [&](const int2 *tu, int ln) {
+  // without the register prefetch: load and deposit in groups of 8 units
+  auto load_deposit = [&](const double *src, const int4 *tu, int gp, int cl, int p) {
 #pragma unroll
-    for (int i = 0; i < NPU; i++) {
-      const int U = min(ln + 64 * i, 16 * NU - 1);
-      const int j = U / NU, u = U % NU;
-      const int2 inf = tu[u];
-      const int m = inf.x >> 16;
-      const double cn = csn[2 * m], sn = csn[2 * m + 1];
-      R[(inf.y & 0xffff) * 16 + j] = cn * pre_re[i] - sn * pre_im[i];
-      R[(inf.y >> 16) * 16 + j] = sn * pre_re[i] + cn * pre_im[i];
+    for (int i0 = 0; i0 < NAU; i0 += 8) {
+      const int i1 = i0 + 8 < NAU ? i0 + 8 : NAU;
+      load_sources(src, tu, gp, i0, i1);
+      deposit(tu, gp, cl, p, i0, i1);
     }
   };
 
@@ -252,89 +267,74 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 #pragma unroll
   for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = 0.0;
 
-  int k = next_k(-1);
-  if (k < 216) {
-    set_sources(k);
-    load_sources(uinf, lane);
-    deposit(uinf, lane);
-  }
+  int k = next_k(-1), p = 0;
+  if (k < 216) load_deposit(source_of(k, 0), uinf, grp, col, 0);
   while (k < 216) {
-    // Views of the constant LDS tables through an offset the compiler cannot
-    // see (0, from inline asm, fresh every iteration): the ~200 per-lane
-    // table values are then re-read where used instead of being hoisted out
-    // of the loop into registers.
+    // Views of the constant LDS tables and lane indices through an offset
+    // the compiler cannot see (0, from inline asm, fresh every iteration):
+    // the per-lane table values are then re-read where they are used
+    // instead of being hoisted out of the loop into registers.
     int z0;
     asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
-    const int *tm1 = map1 + z0, *tm2 = map2 + z0, *tm3 = map3 + z0;
-    const int2 *tu = uinf + z0;
-    // lane-derived indices are recomputed each iteration for the same reason
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
     const int ln = lane + z0, cl = ln & 15, gp = ln >> 4;
     const int o = offset_of(k);
     const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
-    const double chi = P->chi[o];
-    if (lane <= N + 1) ipw[lane] = exp(-(double)lane * log(P->r[o] * w));
-    if (lane <= N) {
-      double sn, cn;
-      sincos((double)lane * chi, &sn, &cn);
-      csc[2 * lane] = cn;
-      csc[2 * lane + 1] = sn;
-    }
+    const int ri = __builtin_amdgcn_readfirstlane(P->rix[o]);
     const int kn = next_k(k);
 
     // stage 1: rotate into the offset's frame
     d4 y[NB];
     stage<NB, KT>(BN, A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int mp = tm1[16 * b + gp + 4 * q];
-        R[(mp >> 6) * 16 + cl] = y[b][q] * ipw[mp & 63];
-      }
-    if (SH::PREFETCH && kn < 216) {
-      set_sources(kn);
-      load_sources(tu, ln);
+    for (int b = 0; b < NB; b++) {
+      const int4 d = t1[4 * b + gp];
+      R[d.x * 16 + cl] = y[b][0];
+      R[d.y * 16 + cl] = y[b][1];
+      R[d.z * 16 + cl] = y[b][2];
+      R[d.w * 16 + cl] = y[b][3];
     }
+    if (SH::PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, gp, 0, NAU);
 
-    // stage 2: coaxial S|R
-    stage<NB, KT>(BN, A2 + z0, R, ln, gp, cl, y);
+    // stage 2: coaxial S|R at this offset's distance
+    stage<NB, KT>(BN, A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int mp = tm2[16 * b + gp + 4 * q];
-        R[(mp >> 6) * 16 + cl] = y[b][q] * ipw[mp & 63];
-      }
+    for (int b = 0; b < NB; b++) {
+      const int4 d = t2[4 * b + gp];
+      R[d.x * 16 + cl] = y[b][0];
+      R[d.y * 16 + cl] = y[b][1];
+      R[d.z * 16 + cl] = y[b][2];
+      R[d.w * 16 + cl] = y[b][3];
+    }
 
     // stage 3: rotate back (without the azimuthal factor)
     stage<NB, KT>(BN, A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int mp = tm3[16 * b + gp + 4 * q];
-        R[(mp >> 6) * 16 + cl] = y[b][q];
-      }
+    for (int b = 0; b < NB; b++) {
+      const int4 d = t3[4 * b + gp];
+      R[d.x * 16 + cl] = y[b][0];
+      R[d.y * 16 + cl] = y[b][1];
+      R[d.z * 16 + cl] = y[b][2];
+      R[d.w * 16 + cl] = y[b][3];
+    }
 
-    // Zout and accumulate: unit u = (n, nu) of this lane's target
+    // Zout and accumulate
 #pragma unroll
     for (int i = 0; i < NAU; i++) {
-      // the last lanes repeat unit NU-1 and are not written out
       const int u = min(gp + 4 * i, NU - 1);
-      const int m = tu[u].x >> 16;
+      const double2 e = cs[p][tu[u].y];
       const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
-      const double cn = csc[2 * m], sn = csc[2 * m + 1];
-      acc_re[i] += cn * tr + sn * tq;
-      acc_im[i] += cn * tq - sn * tr;
+      acc_re[i] += e.x * tr + e.y * tq;
+      acc_im[i] += e.x * tq - e.y * tr;
     }
 
     if (kn < 216) {
-      if (!SH::PREFETCH) {
-        set_sources(kn);
-        load_sources(tu, ln);
-      }
-      deposit(tu, ln);
+      if (SH::PREFETCH)
+        deposit(tu, gp, cl, p ^ 1, 0, NAU);
+      else
+        load_deposit(source_of(kn, p ^ 1), tu, gp, cl, p ^ 1);
     }
+    p ^= 1;
     k = kn;
   }
 
@@ -344,9 +344,9 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   for (int i = 0; i < NAU; i++) {
     const int u = grp + 4 * i;
     if (u < NU) {
-      const int2 inf = uinf[u];
-      out[inf.x & 0xffff] += acc_re[i];
-      if (inf.x >> 16) out[(inf.x & 0xffff) + 1] += acc_im[i];
+      const int4 inf = uinf[u];
+      out[inf.x] += acc_re[i];
+      if (inf.y) out[inf.x + 1] += acc_im[i];
     }
   }
 }
@@ -392,17 +392,17 @@ static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
 
 template <int N>
 static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
-                      const uint32_t *occ, int level, double w,
-                      const double *mps, double *mpr, const void *plan,
-                      const double *A1, const double *A3, const double *A2,
+                      const uint32_t *occ, int level, const double *mps,
+                      double *mpr, const void *plan, const double *A1,
+                      const double *A3, const double *A2, const double *CS,
                       const double *zero, const wbfmm_m2l_plan_t *hplan) {
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
   WBFMM_LAUNCH("lap_m2l", k_lap_m2l_mfma<N>, dim3(nchunks), dim3(64), 0, cls,
-               ca, occ, level, w, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
-               A1, A3, A2, zero);
+               ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+               A1, A3, A2, CS, zero);
   return 0;
 }
 
@@ -463,6 +463,7 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            int N, const double *mps, double *mpr,
                            const void *plan_dev, const double *A1,
                            const double *A3, const double *A2,
+                           const double *CS,
                            const wbfmm_m2l_plan_t *plan_host) {
   ClassArgs ca;
   unsigned nch = 0;
@@ -475,7 +476,7 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
   if (level > 20) return WBFMM_HIP_ERR_ARG;
-  const double w = D / (double)(1u << level);
+  (void)D;
   const uint64_t nbox = (uint64_t)1 << (3 * level);
   const size_t words = (size_t)((nbox + 31) / 32);
   if (words > g_occ_words) {
@@ -493,7 +494,7 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
   switch (N) {
 #define CASE(n) \
   case n:       \
-    return launch_m2l<n>(cls, ca, nch, g_occ, (int)level, w, mps, mpr, plan_dev, A1, A3, A2, g_zero, plan_host);
+    return launch_m2l<n>(cls, ca, nch, g_occ, (int)level, mps, mpr, plan_dev, A1, A3, A2, CS, g_zero, plan_host);
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
     CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
 #undef CASE

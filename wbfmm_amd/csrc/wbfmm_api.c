@@ -74,6 +74,8 @@ typedef struct {
   uint64_t *d_codes;
 } tlist_dev;
 
+#define M2L_A2_CACHE 64
+
 typedef struct {
   wbfmm_shift_operators_t op; /* public part, must stay first */
   double *d_rot;
@@ -84,7 +86,12 @@ typedef struct {
   wbfmm_m2l_plan_t *m2l_plan[WBFMM_HIP_MAX_ORDER + 1];
   void *m2l_plan_dev[WBFMM_HIP_MAX_ORDER + 1];
   double *m2l_A1[WBFMM_HIP_MAX_ORDER + 1], *m2l_A3[WBFMM_HIP_MAX_ORDER + 1],
-      *m2l_A2[WBFMM_HIP_MAX_ORDER + 1];
+      *m2l_CS[WBFMM_HIP_MAX_ORDER + 1];
+  /* coaxial tables of the MFMA M2L, one per (order, box width) in use */
+  int m2l_nA2;
+  int m2l_A2_N[M2L_A2_CACHE];
+  double m2l_A2_w[M2L_A2_CACHE];
+  double *m2l_A2[M2L_A2_CACHE];
 } ops_dev;
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
@@ -637,9 +644,10 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     wbfmm_hip_free(od->m2l_plan_dev[n]);
     wbfmm_hip_free(od->m2l_A1[n]);
     wbfmm_hip_free(od->m2l_A3[n]);
-    wbfmm_hip_free(od->m2l_A2[n]);
+    wbfmm_hip_free(od->m2l_CS[n]);
     free(od->m2l_plan[n]);
   }
+  for (n = 0; n < od->m2l_nA2; n++) wbfmm_hip_free(od->m2l_A2[n]);
   free(od);
   return 0;
 }
@@ -649,8 +657,38 @@ static void ensure_m2l_plan(ops_dev *od, int N)
   if (od->m2l_have[N]) return;
   od->m2l_plan[N] = calloc(1, sizeof(wbfmm_m2l_plan_t));
   HIPOK(wbfmm_hip_lap_m2l_plan(N, od->m2l_plan[N], &od->m2l_plan_dev[N],
-                               &od->m2l_A1[N], &od->m2l_A3[N], &od->m2l_A2[N]));
+                               &od->m2l_A1[N], &od->m2l_A3[N], &od->m2l_CS[N]));
   od->m2l_have[N] = 1;
+}
+
+/* device coaxial table of the MFMA M2L for order N and box width w (the
+   tables depend on the tree's width; a few widths are kept) */
+static const double *m2l_coax(ops_dev *od, int N, double w)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  size_t n = (size_t)P->NR * P->KT * 64;
+  double *h;
+  int i;
+  for (i = 0; i < od->m2l_nA2; i++)
+    if (od->m2l_A2_N[i] == N && od->m2l_A2_w[i] == w) return od->m2l_A2[i];
+  if (od->m2l_nA2 == M2L_A2_CACHE) {   /* full: drop the oldest */
+    wbfmm_hip_sync();
+    wbfmm_hip_free(od->m2l_A2[0]);
+    memmove(od->m2l_A2_N, od->m2l_A2_N + 1, sizeof(int) * (M2L_A2_CACHE - 1));
+    memmove(od->m2l_A2_w, od->m2l_A2_w + 1, sizeof(double) * (M2L_A2_CACHE - 1));
+    memmove(od->m2l_A2, od->m2l_A2 + 1, sizeof(double *) * (M2L_A2_CACHE - 1));
+    od->m2l_nA2--;
+  }
+  h = malloc(sizeof(double) * n);
+  if (!h) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_lap_m2l_coax(P, w, h));
+  i = od->m2l_nA2++;
+  od->m2l_A2_N[i] = N;
+  od->m2l_A2_w[i] = w;
+  HIPOK(wbfmm_hip_malloc((void **)&od->m2l_A2[i], sizeof(double) * n));
+  HIPOK(wbfmm_hip_memcpy(od->m2l_A2[i], h, sizeof(double) * n));
+  free(h);
+  return od->m2l_A2[i];
 }
 
 gint wbfmm_laplace_coaxial_translate_init(gint N)
@@ -739,8 +777,9 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     HIPOK(wbfmm_hip_lap_m2l_mfma(cls, cstart, ccount, td->d_start[level],
                                  td->d_end[level], level, t->D, N,
                                  (double *)t->mps[level], (double *)t->mpr[level],
-                                 od->m2l_plan_dev[N], od->m2l_A1[N],
-                                 od->m2l_A3[N], od->m2l_A2[N], od->m2l_plan[N]));
+                                 od->m2l_plan_dev[N], od->m2l_A1[N], od->m2l_A3[N],
+                                 m2l_coax(od, N, t->D / (double)(1u << level)),
+                                 od->m2l_CS[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
     HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,

@@ -379,7 +379,7 @@ static void put_tiles(double *dst, const double *T, const wbfmm_m2l_plan_t *P)
 }
 
 int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
-                             double *A3, double *A2)
+                             double *A3, double *CS)
 {
   packing p13, p2;
   int b, n, nu, m, part, i, dx, dy, dz, S;
@@ -433,6 +433,10 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
         P->r[o] = sqrt((double)(dx * dx + dy * dy + dz * dz));
         dir_angles(-dx, -dy, -dz, &th, &ph, &ch);
         P->chi[o] = (dx == 0 && dy == 0) ? 0.0 : ch;
+        for (k = 0; k < P->NR; k++)
+          if (P->rr[k] == P->r[o]) break;
+        if (k == P->NR) P->rr[P->NR++] = P->r[o];
+        P->rix[o] = k;
         /* collinear offsets, e.g. (2,2,2) and (3,3,3), share theta up to
            rounding: 49 distinct angles as in src/shift.c:637-686 */
         for (k = 0; k < P->NTH; k++)
@@ -440,20 +444,16 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
         if (k == P->NTH) ths[P->NTH++] = th;
         P->th[o] = k;
       }
-  if (!A1 && !A2 && !A3) return 0;
+  if (CS)
+    for (i = 0; i < 343; i++)
+      for (m = 0; m <= N; m++) {
+        CS[((size_t)i * (N + 1) + m) * 2] = cos(m * P->chi[i]);
+        CS[((size_t)i * (N + 1) + m) * 2 + 1] = sin(m * P->chi[i]);
+      }
+  if (!A1 && !A3) return 0;
 
   T = calloc((size_t)S * S, sizeof(double));
   if (!T) return WBFMM_HIP_ERR_NOMEM;
-  if (A2) {   /* stage 2: unit-distance S|R, same block for both parts */
-    memset(T, 0, sizeof(double) * (size_t)S * S);
-    for (m = 0; m <= N; m++)
-      for (part = 0; part < (m ? 2 : 1); part++)
-        for (n = m; n <= N; n++)
-          for (nu = m; nu <= N; nu++)
-            T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] =
-                lap_cfft(2, n, nu, m);
-    put_tiles(A2, T, P);
-  }
   if (A1 || A3) {
     size_t per = (size_t)P->KT * 64, nr = (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3;
     int k;
@@ -500,30 +500,56 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
   return 0;
 }
 
-int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *P, void **plan_dev,
-                           double **A1, double **A3, double **A2)
+/* stage 2 at box width w: S|R(n, nu, m) / t^(n+nu+1), t = rr[i] w, the
+   same block for the cos and sin parts (src/laplace.c:651-696) */
+int wbfmm_hip_lap_m2l_coax(const wbfmm_m2l_plan_t *P, double w, double *A2)
 {
-  double *h1, *h2, *h3;
-  size_t n13, n2;
+  packing p2;
+  int N = P->N, S = 16 * P->NB, i, m, n, nu, part;
+  double *T;
+  if (w <= 0.0) return WBFMM_HIP_ERR_ARG;
+  do_pack(N, 1, &p2);
+  T = calloc((size_t)S * S, sizeof(double));
+  if (!T) return WBFMM_HIP_ERR_NOMEM;
+  for (i = 0; i < P->NR; i++) {
+    double t = P->rr[i] * w;
+    memset(T, 0, sizeof(double) * (size_t)S * S);
+    for (m = 0; m <= N; m++)
+      for (part = 0; part < (m ? 2 : 1); part++)
+        for (n = m; n <= N; n++)
+          for (nu = m; nu <= N; nu++)
+            T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] =
+                lap_cfft(2, n, nu, m) * pow(t, -(double)(n + nu + 1));
+    put_tiles(A2 + (size_t)i * P->KT * 64, T, P);
+  }
+  free(T);
+  return 0;
+}
+
+int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *P, void **plan_dev,
+                           double **A1, double **A3, double **CS)
+{
+  double *h1, *h3, *hc;
+  size_t n13, ncs;
   int rc = wbfmm_hip_lap_m2l_tables(N, P, NULL, NULL, NULL);
   if (rc) return rc;
   n13 = (size_t)P->NTH * P->KT * 64;
-  n2 = (size_t)P->KT * 64;
+  ncs = (size_t)343 * (N + 1) * 2;
   h1 = malloc(sizeof(double) * n13);
   h3 = malloc(sizeof(double) * n13);
-  h2 = malloc(sizeof(double) * n2);
-  if (!h1 || !h2 || !h3) rc = WBFMM_HIP_ERR_NOMEM;
-  if (!rc) rc = wbfmm_hip_lap_m2l_tables(N, P, h1, h3, h2);
+  hc = malloc(sizeof(double) * ncs);
+  if (!h1 || !h3 || !hc) rc = WBFMM_HIP_ERR_NOMEM;
+  if (!rc) rc = wbfmm_hip_lap_m2l_tables(N, P, h1, h3, hc);
   if (!rc) rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * n13);
   if (!rc) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * n13);
   if (!rc) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * n13);
   if (!rc) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * n13);
-  if (!rc) rc = wbfmm_hip_malloc((void **)A2, sizeof(double) * n2);
-  if (!rc) rc = wbfmm_hip_memcpy(*A2, h2, sizeof(double) * n2);
+  if (!rc) rc = wbfmm_hip_malloc((void **)CS, sizeof(double) * ncs);
+  if (!rc) rc = wbfmm_hip_memcpy(*CS, hc, sizeof(double) * ncs);
   if (!rc) rc = wbfmm_hip_malloc(plan_dev, sizeof(*P));
   if (!rc) rc = wbfmm_hip_memcpy(*plan_dev, P, sizeof(*P));
   free(h1);
-  free(h2);
   free(h3);
+  free(hc);
   return rc;
 }
