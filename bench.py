@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per lap_m2l launch (mean over the 6 launches of one solve) from
+# rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size;
+# measured with the commands in profiles/lap_m2l_laplace_1e6_depth7_p12.md
+M2L_TRAFFIC_BYTES_PER_LAUNCH = 24.63e9
 
 
 def splitmix64(seed, n):
@@ -119,6 +123,9 @@ def main():
     ap.add_argument("--order", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-targets", type=int, default=2000)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="end with a copy kernel of known size (wbfmm_hip_stage4 over 8Mi rows: "
+                         "192 MiB read, 256 MiB written) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -207,7 +214,8 @@ def main():
     m2l = kern.get("lap_m2l", {"ms_per_step": float("nan"), "launches_per_step": 1})
     achieved = m2l_flops_step / (m2l["ms_per_step"] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": M2L_TRAFFIC_BYTES_PER_LAUNCH if (n, depth, p) == (1_000_000, 7, 12) else None,
                 "kernel": "lap_m2l", "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
                 "pairs_per_step": int(sum(pairs.values())), "flops_per_pair": flops_pair}
 
@@ -240,6 +248,15 @@ def main():
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)
     S.close()
+    if args.calibrate:
+        rows = 8 << 20
+        d_a = W.DeviceBuffer(rows * 24)
+        d_b = W.DeviceBuffer(rows * 32)
+        d_a.zero()
+        W.sync()
+        L.wbfmm_hip_stage4(d_a.ptr, 24, rows, d_b.ptr)
+        W.sync()
+        print("bench: calibration copy done", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

@@ -187,9 +187,12 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   __syncthreads();
 
   // list-4 sources of each target, as bits over the 6x6x6 cube of the
-  // parent's neighbours' children (k = 36 a + 6 b + c)
+  // parent's neighbours' children, k = 36 (dz+2+pz) + 6 (dy+2+py) + dx+2+px.
+  // dx varies fastest: the 16 targets of a wave are consecutive in Morton
+  // order, i.e. spread along x, so offsets two steps apart in dx read
+  // mostly the same source boxes while they are still in L2.
   for (int k = grp; k < 216; k += 4) {
-    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
     const int si = ti + dx, sj = tj + dy, sk = tk + dz;
     if (!tvalid || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox)
@@ -211,13 +214,13 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     return k;
   };
   auto offset_of = [&](int k) {
-    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
   };
   // this lane's source expansion for cube index k (zeros when empty) and
   // the offset's azimuthal factors into cs[p]
   auto source_of = [&](int k, int p) {
-    const int dx = k / 36 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k % 6 - 2 - pz;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     if (lane <= N) {
       const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
       cs[p][lane] = make_double2(t[0], t[1]);
