@@ -260,6 +260,55 @@ gint wbfmm_laplace_field_direct(gdouble *xs, gint xstride, gdouble *n,
 /* ------------------------------------------------------------------ */
 /* target lists (src/wbfmm.h, src/tree.c:86-122, src/targets.c)       */
 /* ------------------------------------------------------------------ */
+/* ---- Helmholtz path (complex coefficients, FP64) ----------------------- */
+/* src/wbfmm.h:836-838: complex coefficient arrays of level l (device) */
+gint wbfmm_tree_coefficient_init(wbfmm_tree_t *t, guint l, guint nr, guint ns);
+/* src/wbfmm.h:839-843: P2M on the GPU; src, dipoles: complex strengths
+   (2 doubles) at element stride, host or device; dipoles need normals */
+gint wbfmm_tree_leaf_expansions(wbfmm_tree_t *t, gdouble k, gdouble *src,
+                                gint sstr, gdouble *dipoles, gint dstr,
+                                gboolean zero_expansions, gdouble *work);
+/* src/wbfmm.h:792-797: coaxial tables of a level, built on the host in
+   quad precision and kept on the device (w->SR[level], w->SS[level] are
+   device pointers); one wavenumber per operator set */
+gint wbfmm_shift_operators_coaxial_SR_init(wbfmm_shift_operators_t *w,
+                                           gdouble D, guint level, guint L,
+                                           gdouble k, gdouble *work);
+gint wbfmm_shift_operators_coaxial_SS_init(wbfmm_shift_operators_t *w,
+                                           gdouble D, guint level, guint L,
+                                           gdouble k, gdouble *work);
+/* src/wbfmm.h:798-806: M2M from level into level-1; M2L at level then L2L
+   into level+1 (nthreads and work are not used) */
+gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                       guint level, gdouble *work);
+gint wbfmm_downward_pass_ref(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                             guint level, gdouble *work, gint nthreads);
+gint wbfmm_downward_pass_avx(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                             guint level, gdouble *work, gint nthreads);
+#define wbfmm_downward_pass(_t, _op, _level, _work, _nth) \
+  wbfmm_downward_pass_ref(_t, _op, _level, _work, _nth)
+/* src/wbfmm.h:810-818: field at one point of leaf box b; POTENTIAL:
+   f[0..1], GRADIENT: f[0..5]; POTENTIAL|GRADIENT is refused as in the
+   reference (src/local.c:297-300) */
+gint wbfmm_tree_box_local_field(wbfmm_tree_t *t, guint level, guint b,
+                                gdouble k, gdouble *x, gdouble *f, gint fstr,
+                                gdouble *src, gint sstr, gdouble *d,
+                                gint dstr, gboolean eval_neighbours,
+                                guint field, gdouble *work);
+/* src/wbfmm.h:317-330: FP64 direct sum on the GPU at one point (potential
+   f[0..1], or gradient f[0..5] for _grad), G = h0(kr)/(4 pi), dipole term
+   d nrm.grad_x G; values are added to field (the reference's mixed path
+   divides the whole field array by 4 pi instead, src/direct.c:134) */
+gint wbfmm_total_field(gdouble k, gdouble *xs, gint xstride, gdouble *src,
+                       gint sstride, gdouble *normals, gint nstr,
+                       gdouble *dipoles, gint dstr, gint nq, gint nsrc,
+                       gdouble *xf, gdouble *field, gint fstr);
+gint wbfmm_total_field_grad(gdouble k, gdouble *xs, gint xstride,
+                            gdouble *src, gint sstride, gdouble *normals,
+                            gint nstr, gdouble *dipoles, gint dstr, gint nq,
+                            gint nsrc, gdouble *xf, gdouble *field,
+                            gint fstr);
+
 /* src/tree.c:86 */
 wbfmm_target_list_t *wbfmm_target_list_new(wbfmm_tree_t *t, guint npts);
 /* src/targets.c:87: targets are located and sorted on the GPU; the
@@ -300,6 +349,14 @@ gint wbfmm_laplace_field_targets(wbfmm_tree_t *t, gdouble *xt, gsize tstr,
                                  guint ntgt, gdouble *src, gint sstr,
                                  gdouble *d, gint dstr, guint field,
                                  gdouble *f, gint fstr);
+/* Helmholtz potential and/or gradient at ntgt points (host or device, byte
+   stride tstr) in one launch: f gets 2 (potential), 6 (gradient) or 8
+   (both, potential first) doubles per target at stride fstr (elements),
+   accumulated */
+gint wbfmm_helmholtz_field_targets(wbfmm_tree_t *t, gdouble k, gdouble *xt,
+                                   gsize tstr, guint ntgt, gdouble *src,
+                                   gint sstr, gdouble *d, gint dstr,
+                                   guint field, gdouble *f, gint fstr);
 /* wait for all GPU work queued by the library */
 gint wbfmm_hip_synchronize(void);
 /* release a tree and its device memory (the reference has no destroy) */

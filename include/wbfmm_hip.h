@@ -228,6 +228,73 @@ int wbfmm_hip_lap_direct(const double *xyz, const double *q,
                          unsigned nsrc, const double *xt, unsigned ntgt,
                          int field, double cutoff, double *f, int fstr);
 
+/* ---- Helmholtz tables (host, wbfmm_tables.c) --------------------------- */
+/* geometry of the 343 list-4 offset slots (dx = source - target, slot
+   (dx+3)*49 + (dy+3)*7 + dz+3): index of the polar angle in thetas[] (49
+   distinct), azimuth chi (0 on the z axis, phi = -pi/2 throughout) and
+   index of the length in rr[] (15 distinct); thetas needs 64 slots, rr 16 */
+int wbfmm_hip_offset_geometry(double *thetas, int *nth, int *th_idx,
+                              double *chi, double *rr, int *nr, int *rix);
+/* raw Gumerov-Duraiswami H_n^{nu,m}(theta) (nu = -n..n, m = 0..n, n <= L)
+   for nth angles in the reference's index order (src/wbfmm-private.h:
+   73-74), uploaded as one device block; *stride gets the doubles per
+   angle */
+int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
+                       size_t *stride);
+/* Helmholtz coaxial coefficients (E|F)_{l n}^m, l <= L, at signed distance
+   kr into host E: 2 doubles (re, im) per entry l(l+1)(l+2)/6 + n(n+1)/2 +
+   m; kind 0 R|R (= S|S), kind 1 S|R (src/translations.c:205-409) */
+int wbfmm_hip_hel_coaxial_table(int kind, int L, double kr, double *E);
+
+/* ---- Helmholtz passes (FP64, complex coefficients, box-major) ---------- */
+/* P2M into mps at `level` for the listed leaf boxes; q, dip: complex
+   strengths in sorted order (2 doubles each); nrm: sorted normals, 4
+   doubles each (src/boxes.c:341-490) */
+int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox,
+                      const uint32_t *start, const uint32_t *end,
+                      const double *xyz, const double *q, const double *nrm,
+                      const double *dip, unsigned level, const double *x0,
+                      double D, int N, double k, double *mps);
+/* M2M: each listed parent from its children (child level ranges
+   cstart/cend); Hud: H tables of acos(1/sqrt3) and acos(-1/sqrt3), nrot
+   doubles each; SS: R|R table of the child level (src/shift.c:244-401) */
+int wbfmm_hip_hel_m2m(const uint32_t *parents, unsigned npar,
+                      const uint32_t *cstart, const uint32_t *cend, int Nc,
+                      int Np, const double *mps_child, double *mps_parent,
+                      const double *Hud, size_t nrot, const double *SS);
+/* L2L: each listed parent's local expansion added to its children with
+   cflag != 0 (all when NULL); SS: R|R table of the child level
+   (src/shift.c:403-585) */
+int wbfmm_hip_hel_l2l(const uint32_t *parents, unsigned npar,
+                      const uint32_t *cflag, int Np, int Nc,
+                      const double *mpr_parent, double *mpr_child,
+                      const double *Hud, size_t nrot, const double *SS);
+/* M2L for the listed target boxes, accumulating into mpr; Hth: H tables
+   of the offset polar angles; th_idx/chi/rix: per-slot geometry (device);
+   SR: one S|R table of ne complex entries per offset length
+   (src/pass.c:56-105, :448-537) */
+int wbfmm_hip_hel_m2l(const uint32_t *targets, unsigned ntgt,
+                      const uint32_t *sstart, const uint32_t *send,
+                      unsigned level, int Ns, int Nr, const double *mps,
+                      double *mpr, const double *Hth, size_t nrot,
+                      const int *th_idx, const double *chi, const int *rix,
+                      const double *SR, size_t ne);
+/* L2P + near field for n targets (4 doubles each), accumulated into
+   f[tid[i]*fstr + ...]: field bit 1 potential (2 doubles), bit 2
+   gradient (6), potential first (src/local.c:32-310) */
+int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
+                        const uint32_t *start, const uint32_t *end,
+                        const double *xyz, const double *q, const double *nrm,
+                        const double *dip, unsigned level, const double *x0,
+                        double D, int N, double k, const double *mpr, int field,
+                        double *f, int fstr);
+/* FP64 Helmholtz direct sum (src/direct.c:35-294); pairs with r <= cutoff
+   skipped; same output layout as wbfmm_hip_hel_field */
+int wbfmm_hip_hel_direct(const double *xyz, const double *q, const double *nrm,
+                         const double *dip, unsigned nsrc, const double *xt,
+                         unsigned ntgt, double k, int field, double cutoff,
+                         double *f, int fstr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,6 +129,19 @@ _SIGS = {
     "wbfmm_hip_mfma_layout_probe": (_i, [_vp]),
     "wbfmm_hip_lap_m2l_tables": (_i, [_i, _vp, _vp, _vp, _vp]),
     "wbfmm_hip_lap_m2l_coax": (_i, [_vp, _d, _vp]),
+    "wbfmm_tree_coefficient_init": (_i, [_vp, _u, _u, _u]),
+    "wbfmm_tree_leaf_expansions": (_i, [_vp, _d, _vp, _i, _vp, _i, _i, _vp]),
+    "wbfmm_shift_operators_coaxial_SR_init": (_i, [_vp, _d, _u, _u, _d, _vp]),
+    "wbfmm_shift_operators_coaxial_SS_init": (_i, [_vp, _d, _u, _u, _d, _vp]),
+    "wbfmm_upward_pass": (_i, [_vp, _vp, _u, _vp]),
+    "wbfmm_downward_pass_ref": (_i, [_vp, _vp, _u, _vp, _i]),
+    "wbfmm_downward_pass_avx": (_i, [_vp, _vp, _u, _vp, _i]),
+    "wbfmm_tree_box_local_field": (_i, [_vp, _u, _u, _d, _vp, _vp, _i, _vp, _i, _vp, _i, _i, _u, _vp]),
+    "wbfmm_helmholtz_field_targets": (_i, [_vp, _d, _vp, _sz, _u, _vp, _i, _vp, _i, _u, _vp, _i]),
+    "wbfmm_total_field": (_i, [_d, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _i]),
+    "wbfmm_total_field_grad": (_i, [_d, _vp, _i, _vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _i]),
+    "wbfmm_hip_hel_direct": (_i, [_vp, _vp, _vp, _vp, _u, _vp, _u, _d, _i, _d, _vp, _i]),
+    "wbfmm_hip_hel_coaxial_table": (_i, [_i, _i, _d, _vp]),
 }
 
 
@@ -319,6 +332,93 @@ class LaplaceSolver:
         if self.tl:
             L.wbfmm_target_list_free(self.tl)
             self.tl = None
+        if self.t:
+            L.wbfmm_tree_free(self.t)
+            self.t = None
+        if self.op:
+            L.wbfmm_shift_operators_free(self.op)
+            self.op = None
+
+
+class HelmholtzSolver:
+    """Helmholtz FMM through the drop-in API, in the reference driver's
+    call order (tools/wbfmm-tree-calc.c.in:306-417): shift operators,
+    S|R tables for levels 2..depth and S|S tables for levels 3..depth,
+    tree, coefficients, P2M, upward pass, downward pass, field at the
+    targets.  Strengths are complex: (n, 2) float64 arrays (re, im)."""
+
+    def __init__(self, xs, x0, D, depth, order_s, order_r, k, pstr=24,
+                 normals=None, nstr=24, nsrc=None):
+        L = lib()
+        self.depth = depth
+        self.k = float(k)
+        self.nsrc = len(xs) if nsrc is None else nsrc
+        self._x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        self.xs = xs
+        self.pstr = pstr
+        self.normals = normals
+        self.nstr = nstr
+        self.order_s = list(order_s)
+        self.order_r = list(order_r)
+        self.tree = L.wbfmm_tree_new(ptr(self._x0), float(D), self.nsrc)
+        if not self.tree:
+            raise RuntimeError("wbfmm_tree_new failed")
+        self.t = ctypes.cast(self.tree, _vp).value
+        L.wbfmm_tree_host_mirror(self.t, 0)
+        lmax = max(max(order_s[2:depth + 1]), max(order_r[2:depth + 1]))
+        L.wbfmm_shift_angle_table_init()
+        self.op = L.wbfmm_shift_operators_new(lmax, 0, None)
+        for l in range(2, depth + 1):
+            lo = max(order_s[l], order_r[l])
+            if L.wbfmm_shift_operators_coaxial_SR_init(self.op, float(D), l, lo, self.k, None):
+                raise RuntimeError("S|R init failed")
+        for l in range(3, depth + 1):
+            lo = max(order_s[l], order_s[l - 1], order_r[l], order_r[l - 1])
+            if L.wbfmm_shift_operators_coaxial_SS_init(self.op, float(D), l, lo, self.k, None):
+                raise RuntimeError("S|S init failed")
+
+    def build_tree(self):
+        L = lib()
+        L.wbfmm_tree_add_points(self.t, ptr(self.xs), self.pstr, ptr(self.normals),
+                                self.nstr if self.normals is not None else 0, self.nsrc, 0)
+        for _ in range(self.depth):
+            L.wbfmm_tree_refine(self.t)
+        self.tree.contents.problem = WBFMM_PROBLEM_HELMHOLTZ
+        self.tree.contents.nq = 1
+        for l in range(1, self.depth + 1):
+            L.wbfmm_tree_coefficient_init(self.t, l, self.order_r[l], self.order_s[l])
+
+    def passes(self, q, sstr=2, d=None, dstr=2):
+        L = lib()
+        for l in range(1, self.depth + 1):
+            L.wbfmm_tree_coefficients_zero(self.t, l)
+        L.wbfmm_tree_leaf_expansions(self.t, self.k, ptr(q), sstr, ptr(d), dstr, 1, None)
+        for l in range(self.depth, 2, -1):
+            L.wbfmm_upward_pass(self.t, self.op, l, None)
+        for l in range(2, self.depth + 1):
+            L.wbfmm_downward_pass_ref(self.t, self.op, l, None, 0)
+
+    def field(self, xt, q, f, field=3, fstr=8, tstr=24, ntgt=None, sstr=2, d=None, dstr=2):
+        n = len(xt) if ntgt is None else ntgt
+        rc = lib().wbfmm_helmholtz_field_targets(self.t, self.k, ptr(xt), tstr, n, ptr(q), sstr,
+                                                 ptr(d), dstr, field, ptr(f), fstr)
+        if rc != 0:
+            raise RuntimeError(f"wbfmm_helmholtz_field_targets = {rc}")
+
+    def coefficients(self, level, which):
+        """mps (0) or mpr (1) of a level as an (8^level, C) complex array"""
+        N = self.order_s[level] if which == 0 else self.order_r[level]
+        C = (N + 1) ** 2
+        nb = 8 ** level
+        buf = np.zeros(nb * C * 2)
+        if lib().wbfmm_tree_coefficients_download(self.t, level, which, ptr(buf)) != 0:
+            raise RuntimeError("download failed")
+        # [group of 8][coef][child][re, im] -> [box][coef]
+        z = buf[0::2] + 1j * buf[1::2]
+        return z.reshape(nb // 8, C, 8).transpose(0, 2, 1).reshape(nb, C)
+
+    def close(self):
+        L = lib()
         if self.t:
             L.wbfmm_tree_free(self.t)
             self.t = None

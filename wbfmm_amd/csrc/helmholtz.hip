@@ -1,0 +1,699 @@
+// helmholtz.hip -- gfx950 kernels of the Helmholtz FMM path (FP64).
+//
+// Kernel G = h0(kr)/(4 pi) = e^{ikr}/(4 pi i k r) (src/direct.c:35-134).
+// Coefficients are complex, C_n^m for -n <= m <= n at index n(n+1)+m
+// (src/wbfmm.h:242), two doubles (re, im) each; level arrays are box-major
+// on the device (box b at b * 2 (N+1)^2 doubles).
+//
+// Translations follow the reference's rotate -> coaxial shift -> rotate
+// back sequence (src/pass.c:56-105, src/shift.c:244-585) with the
+// Gumerov-Duraiswami H(theta) coefficients (src/rotations.c:131-207) and
+// coaxial tables (E|F)_{l n}^m (src/translations.c:205-604):
+//   rotate:  Co_n^nu = e^{-i nu phi} sum_m H_n^{nu,m}(theta) e^{i m chi} Ci_n^m
+//   coaxial: Co_n^m  = sum_l T(n, l, |m|) Ci_l^m,
+//            T = E_{l n}^{|m|} (l >= n), (-1)^{n+l} E_{n l}^{|m|} (l < n).
+// The inverse rotation swaps phi and chi.
+//
+// Kernel shapes (first version, one wavefront per box, vectors in LDS):
+//   P2M      one wave per leaf box; per point the regular functions
+//            R_n^m = j_n(kr) Pbar_n^|m| e^{i m phi} go to LDS (lane m
+//            computes column m) and lane m accumulates orders +-m
+//   M2M/L2L  one wave per parent, the 8 children in turn
+//   M2L      one wave per target box, its list-4 sources in turn, the
+//            local expansion accumulated in LDS
+//   L2P+P2P  one thread per target, regular functions streamed by order
+//            with the gradient in the re-indexed form of G&D (2004) eq 3.7
+
+#include <cstdio>
+#include <cstring>
+
+#include <hip/hip_runtime.h>
+
+#include "hip_common.h"
+
+namespace wbfmm {
+
+__device__ __forceinline__ int hidx(int n, int m) { return n * (n + 1) + m; }
+__device__ __forceinline__ int ridx(int nu, int m, int n) {
+  return n * (4 * n - 1) * (n + 1) / 6 + (nu + n) * (n + 1) + m;
+}
+__device__ __forceinline__ int cxi(int l, int m, int n) {
+  return l * (l + 1) * (l + 2) / 6 + n * (n + 1) / 2 + m;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) {
+  return make_double2(a.x * s, a.y * s);
+}
+
+// Gumerov-Duraiswami recursion coefficients a_n^m, b_n^m (G&D 2003)
+__device__ __forceinline__ double rec_a(int n, int m) {
+  m = m < 0 ? -m : m;
+  if (n < m || n < 0) return 0.0;
+  return sqrt((double)(n + 1 + m) * (n + 1 - m) / ((double)(2 * n + 1) * (2 * n + 3)));
+}
+__device__ __forceinline__ double rec_b(int n, int m) {
+  if (n < 0 || n < (m < 0 ? -m : m)) return 0.0;
+  const double v = sqrt((double)(n - m - 1) * (n - m) / ((double)(2 * n - 1) * (2 * n + 1)));
+  return m >= 0 ? v : -v;
+}
+
+// j_n(x), n = 0..N into j[]: upward recursion while x > n+1, the power
+// series otherwise (src/util.c:118-175)
+__device__ inline double bessel_j_series(int n, double x) {
+  double sc = 1.0 / (2 * n + 1), term = 1.0, s = 1.0;
+  const double x2 = x * x;
+  for (int q = 0; q < n; q++) sc *= x / (2 * q + 1);
+  for (int q = 0; q < 80 && fabs(term) > 1e-17; q++) {
+    term *= -x2 / ((2 * q + 2) * (double)(2 * n + 2 * (q + 1) + 1));
+    s += term;
+  }
+  return s * sc;
+}
+
+__device__ inline void bessel_j(int N, double x, double *j) {
+  if (x > 0.0) {
+    j[0] = sin(x) / x;
+    if (N >= 1) j[1] = j[0] / x - cos(x) / x;
+  } else {
+    j[0] = 1.0;
+    if (N >= 1) j[1] = 0.0;
+  }
+  for (int n = 1; n < N; n++)
+    j[n + 1] = (x > n + 1) ? j[n] * (2 * n + 1) / x - j[n - 1] : bessel_j_series(n + 1, x);
+}
+
+// spherical coordinates of x - x0: r, cos/sin theta, cos/sin phi
+__device__ __forceinline__ void sph(double dx, double dy, double dz, double &r,
+                                    double &C, double &S, double &cp, double &sp) {
+  r = sqrt(dx * dx + dy * dy + dz * dz);
+  if (r == 0.0) {
+    C = 1.0; S = 0.0; cp = 1.0; sp = 0.0;
+    return;
+  }
+  C = dz / r;
+  const double rho = sqrt(dx * dx + dy * dy);
+  S = rho / r;
+  if (rho == 0.0) {
+    cp = 1.0; sp = 0.0;
+  } else {
+    cp = dx / rho; sp = dy / rho;
+  }
+}
+
+// Pbar_m^m (G&D normalisation, no Condon-Shortley phase)
+__device__ __forceinline__ double pbar_mm(int m, double S) {
+  double p = 1.0 / sqrt(4.0 * kPi);
+  for (int i = 1; i <= m; i++) p *= sqrt((2.0 * i + 1) / (2.0 * i)) * S;
+  return p;
+}
+// Pbar_{n+1}^m from Pbar_n^m, Pbar_{n-1}^m (src/util.c:81-115)
+__device__ __forceinline__ double pbar_up(int n, int m, double C, double pn, double pnm1) {
+  const double v = C * sqrt(2.0 * n + 1) * pn - sqrt((double)(n * n - m * m) / (2 * n - 1)) * pnm1;
+  return v / sqrt((double)((n + 1) * (n + 1) - m * m) / (2 * n + 3));
+}
+
+// ---------------------------------------------------------------------
+// wave-cooperative operators on LDS vectors (double2, index hidx)
+// ---------------------------------------------------------------------
+
+// out (+)= e^{-i nu ph} sum_m H e^{i m ch} in, degrees 0..N; tmp: (N+1)^2,
+// ep/ec: N+1 phase slots
+__device__ void rotate_wave(const double2 *in, double2 *out, double2 *tmp,
+                            double2 *ep, double2 *ec, int N,
+                            const double *__restrict__ H, double ph,
+                            double ch, bool add) {
+  const int lane = threadIdx.x;
+  if (lane <= N) {
+    double s, c;
+    sincos((double)lane * ch, &s, &c);
+    ec[lane] = make_double2(c, s);
+    sincos(-(double)lane * ph, &s, &c);
+    ep[lane] = make_double2(c, s);
+  }
+  __syncthreads();
+  const int nc = (N + 1) * (N + 1);
+  for (int r = lane; r < nc; r += kWave) {
+    const int n = isqrt_deg(r), m = r - n * (n + 1);
+    const double2 e = m >= 0 ? ec[m] : make_double2(ec[-m].x, -ec[-m].y);
+    tmp[r] = cmul(e, in[r]);
+  }
+  __syncthreads();
+  for (int r = lane; r < nc; r += kWave) {
+    const int n = isqrt_deg(r), nu = r - n * (n + 1);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int m = -n; m <= n; m++) {
+      const double h = m >= 0 ? H[ridx(nu, m, n)] : H[ridx(-nu, -m, n)];
+      const double2 v = tmp[hidx(n, m)];
+      acc.x += h * v.x;
+      acc.y += h * v.y;
+    }
+    const double2 e = nu >= 0 ? ep[nu] : make_double2(ep[-nu].x, -ep[-nu].y);
+    acc = cmul(e, acc);
+    out[r] = add ? cadd(out[r], acc) : acc;
+  }
+  __syncthreads();
+}
+
+// out (+)= coaxial translation of in (degrees 0..Ni) to degrees 0..No
+__device__ void coax_wave(const double2 *in, int Ni, double2 *out, int No,
+                          const double2 *__restrict__ E, bool add) {
+  const int lane = threadIdx.x, nc = (No + 1) * (No + 1);
+  for (int r = lane; r < nc; r += kWave) {
+    const int n = isqrt_deg(r), m = r - n * (n + 1), am = m < 0 ? -m : m;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int l = am; l <= Ni; l++) {
+      const int idx = l >= n ? cxi(l, am, n) : cxi(n, am, l);
+      double2 e = E[idx];
+      if (l < n && ((n + l) & 1)) e = make_double2(-e.x, -e.y);
+      acc = cadd(acc, cmul(e, in[hidx(l, m)]));
+    }
+    out[r] = add ? cadd(out[r], acc) : acc;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------
+// P2M (src/boxes.c:341-490, src/expansions.c:52-123, :545-631)
+// ---------------------------------------------------------------------
+// monopole:  M_n^m += q conj(R_n^m(y - c))
+// dipole:    M_n^m += -d conj(nrm . grad R_n^m(y - c))  (the source
+//            d nrm.grad_x G, src/local.c:155-166)
+__global__ __launch_bounds__(64) void k_hel_p2m(
+    const uint32_t *__restrict__ boxes, unsigned nbox,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
+    const double *__restrict__ xyz, const double2 *__restrict__ q,
+    const double *__restrict__ nrm, const double2 *__restrict__ dip,
+    unsigned level, double x0, double y0, double z0, double D, int N,
+    double k, double2 *__restrict__ mps) {
+  extern __shared__ double2 smem[];
+  const int N1 = N + 1, lane = threadIdx.x;
+  double2 *R = smem;                      // (N+2)^2 (n <= N+1, m >= 0 used)
+  double2 *acc = R + (N1 + 1) * (N1 + 1); // (N+1)^2
+  double *jn = (double *)(acc + N1 * N1); // N+2
+  const unsigned bi = blockIdx.x;
+  if (bi >= nbox) return;
+  const uint32_t b = boxes[bi];
+  const double w = D / (double)(1u << level);
+  const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
+               zc = z0 + (compact3(b >> 2) + 0.5) * w;
+  for (int r = lane; r < N1 * N1; r += kWave) acc[r] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (uint32_t s = start[b]; s < end[b]; s++) {
+    double r, C, S, cp, sp;
+    sph(xyz[4 * s] - xc, xyz[4 * s + 1] - yc, xyz[4 * s + 2] - zc, r, C, S, cp, sp);
+    if (lane == 0) bessel_j(N1, k * r, jn);
+    __syncthreads();
+    // column m of R for n = m..N+1
+    if (lane <= N1) {
+      const int m = lane;
+      double sn, cs;
+      sincos((double)m * atan2(sp, cp), &sn, &cs);
+      double pmm = pbar_mm(m, S), pprev = 0.0, p = pmm;
+      for (int n = m; n <= N1; n++) {
+        R[hidx(n, m)] = make_double2(jn[n] * p * cs, jn[n] * p * sn);
+        const double pn = (n + 1 <= N1) ? pbar_up(n, m, C, p, pprev) : 0.0;
+        pprev = p;
+        p = pn;
+      }
+    }
+    __syncthreads();
+    if (lane <= N) {
+      const int m = lane;
+      const double2 qq = q ? q[s] : make_double2(0.0, 0.0);
+      const double2 dd = dip ? dip[s] : make_double2(0.0, 0.0);
+      const double nx = dip ? nrm[4 * s] : 0.0, ny = dip ? nrm[4 * s + 1] : 0.0,
+                   nz = dip ? nrm[4 * s + 2] : 0.0;
+      auto Rg = [&](int n, int mm) -> double2 {
+        if (n < 0 || n > N1 || mm > n || -mm > n) return make_double2(0.0, 0.0);
+        const double2 v = R[hidx(n, mm < 0 ? -mm : mm)];
+        return mm >= 0 ? v : make_double2(v.x, -v.y);
+      };
+      for (int sgn = 1; sgn >= -1; sgn -= 2) {
+        if (m == 0 && sgn < 0) break;
+        const int mm = sgn * m;
+        for (int n = m; n <= N; n++) {
+          double2 v = make_double2(0.0, 0.0);
+          if (q) v = cmulc(qq, Rg(n, mm));
+          if (dip) {
+            // G&D 2004 eq 3.7
+            const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), rec_a(n - 1, mm)),
+                                           cscale(Rg(n + 1, mm), -rec_a(n, mm))), k);
+            const double2 dp = cscale(cadd(cscale(Rg(n + 1, mm + 1), rec_b(n + 1, -mm - 1)),
+                                           cscale(Rg(n - 1, mm + 1), -rec_b(n, mm))), k);
+            const double2 dm = cscale(cadd(cscale(Rg(n + 1, mm - 1), rec_b(n + 1, mm - 1)),
+                                           cscale(Rg(n - 1, mm - 1), -rec_b(n, -mm))), k);
+            // gx = (dp + dm)/2, gy = -i (dp - dm)/2
+            const double2 gx = cscale(cadd(dp, dm), 0.5);
+            const double2 dd2 = cscale(cadd(dp, cscale(dm, -1.0)), 0.5);
+            const double2 gy = make_double2(dd2.y, -dd2.x);
+            const double2 g = cadd(cadd(cscale(gx, nx), cscale(gy, ny)), cscale(dz, nz));
+            const double2 t = cmulc(dd, g);
+            v = cadd(v, make_double2(-t.x, -t.y));
+          }
+          acc[hidx(n, mm)] = cadd(acc[hidx(n, mm)], v);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double2 *out = mps + (size_t)b * N1 * N1;
+  for (int r = lane; r < N1 * N1; r += kWave) out[r] = cadd(out[r], acc[r]);
+}
+
+// ---------------------------------------------------------------------
+// M2M (src/pass.c:581-632, src/shift.c:244-401): the 8 children of each
+// parent, rotated along the child -> parent direction, R|R shift by
+// sqrt(3)/2 w_child, rotated back and summed into the parent
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void child_dir(int c, int sgn, double &th, double &ch) {
+  const double ex = sgn * (0.5 - (c & 1)), ey = sgn * (0.5 - ((c >> 1) & 1)),
+               ez = sgn * (0.5 - ((c >> 2) & 1));
+  th = acos(ez / sqrt(ex * ex + ey * ey + ez * ez));
+  ch = atan2(ey, ex);
+}
+
+// Hud: H tables of the two polar angles acos(+-1/sqrt 3) (index 0: +, 1: -)
+__global__ __launch_bounds__(64) void k_hel_m2m(
+    const uint32_t *__restrict__ parents, unsigned npar,
+    const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ cend,
+    int Nc, int Np, const double2 *__restrict__ mps_c, double2 *__restrict__ mps_p,
+    const double *__restrict__ Hud, size_t nrot, const double2 *__restrict__ SS) {
+  extern __shared__ double2 smem[];
+  const int L = Nc > Np ? Nc : Np, L1 = (L + 1) * (L + 1);
+  double2 *in = smem, *t1 = in + L1, *t2 = t1 + L1, *tmp = t2 + L1, *acc = tmp + L1;
+  double2 *ep = acc + L1, *ec = ep + L + 1;
+  const int lane = threadIdx.x;
+  if (blockIdx.x >= npar) return;
+  const uint32_t p = parents[blockIdx.x];
+  for (int r = lane; r < (Np + 1) * (Np + 1); r += kWave) acc[r] = make_double2(0.0, 0.0);
+  for (int c = 0; c < 8; c++) {
+    const uint32_t cb = 8 * p + c;
+    if (cend[cb] <= cstart[cb]) continue;
+    const double2 *src = mps_c + (size_t)cb * (Nc + 1) * (Nc + 1);
+    for (int r = lane; r < (Nc + 1) * (Nc + 1); r += kWave) in[r] = src[r];
+    __syncthreads();
+    double th, ch;
+    child_dir(c, 1, th, ch);
+    const double *H = Hud + ((c >> 2) & 1 ? nrot : 0);
+    rotate_wave(in, t1, tmp, ep, ec, Nc, H, -0.5 * kPi, ch, false);
+    coax_wave(t1, Nc, t2, Np, SS, false);
+    rotate_wave(t2, acc, tmp, ep, ec, Np, H, ch, -0.5 * kPi, true);
+  }
+  double2 *out = mps_p + (size_t)p * (Np + 1) * (Np + 1);
+  for (int r = lane; r < (Np + 1) * (Np + 1); r += kWave) out[r] = acc[r];
+}
+
+// ---------------------------------------------------------------------
+// L2L (src/pass.c:550-576, src/shift.c:403-585): parent local expansion
+// to each flagged child (parent -> child direction)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_hel_l2l(
+    const uint32_t *__restrict__ parents, unsigned npar,
+    const uint32_t *__restrict__ cflag, int Np, int Nc,
+    const double2 *__restrict__ mpr_p, double2 *__restrict__ mpr_c,
+    const double *__restrict__ Hud, size_t nrot, const double2 *__restrict__ SS) {
+  extern __shared__ double2 smem[];
+  const int L = Nc > Np ? Nc : Np, L1 = (L + 1) * (L + 1);
+  double2 *in = smem, *t1 = in + L1, *t2 = t1 + L1, *tmp = t2 + L1, *out = tmp + L1;
+  double2 *ep = out + L1, *ec = ep + L + 1;
+  const int lane = threadIdx.x;
+  if (blockIdx.x >= npar) return;
+  const uint32_t p = parents[blockIdx.x];
+  const double2 *src = mpr_p + (size_t)p * (Np + 1) * (Np + 1);
+  for (int r = lane; r < (Np + 1) * (Np + 1); r += kWave) in[r] = src[r];
+  __syncthreads();
+  for (int c = 0; c < 8; c++) {
+    const uint32_t cb = 8 * p + c;
+    if (cflag && !cflag[cb]) continue;
+    double th, ch;
+    child_dir(c, -1, th, ch);
+    // parent -> child: e = -(0.5 - c bits); the z component flips the polar
+    // angle, so the table index is that of the opposite z bit
+    const double *H = Hud + (((c >> 2) & 1) ? 0 : nrot);
+    rotate_wave(in, t1, tmp, ep, ec, Np, H, -0.5 * kPi, ch, false);
+    coax_wave(t1, Np, t2, Nc, SS, false);
+    rotate_wave(t2, out, tmp, ep, ec, Nc, H, ch, -0.5 * kPi, false);
+    double2 *dst = mpr_c + (size_t)cb * (Nc + 1) * (Nc + 1);
+    for (int r = lane; r < (Nc + 1) * (Nc + 1); r += kWave) dst[r] = cadd(dst[r], out[r]);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------
+// M2L (src/pass.c:56-105, :448-537): one wave per target box, its list-4
+// sources in turn.  Offset geometry per slot o (dx = source - target):
+// theta index, azimuth chi and length index; SR: one (E|F) table per
+// length.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_hel_m2l(
+    const uint32_t *__restrict__ targets, unsigned ntgt,
+    const uint32_t *__restrict__ sstart, const uint32_t *__restrict__ send,
+    int level, int Ns, int Nr, const double2 *__restrict__ mps,
+    double2 *__restrict__ mpr, const double *__restrict__ Hth, size_t nrot,
+    const int *__restrict__ th_idx, const double *__restrict__ chi,
+    const int *__restrict__ rix, const double2 *__restrict__ SR, size_t ne) {
+  extern __shared__ double2 smem[];
+  const int L = Ns > Nr ? Ns : Nr, L1 = (L + 1) * (L + 1);
+  double2 *in = smem, *t1 = in + L1, *t2 = t1 + L1, *tmp = t2 + L1, *acc = tmp + L1;
+  double2 *ep = acc + L1, *ec = ep + L + 1;
+  const int lane = threadIdx.x;
+  if (blockIdx.x >= ntgt) return;
+  const uint32_t tb = targets[blockIdx.x];
+  const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
+  const int px = ti & 1, py = tj & 1, pz = tk & 1, nbox = 1 << level;
+  for (int r = lane; r < (Nr + 1) * (Nr + 1); r += kWave) acc[r] = make_double2(0.0, 0.0);
+  for (int dz = -2 - pz; dz <= 3 - pz; dz++)
+    for (int dy = -2 - py; dy <= 3 - py; dy++)
+      for (int dx = -2 - px; dx <= 3 - px; dx++) {
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+        const int si = ti + dx, sj = tj + dy, sk = tk + dz;
+        if (si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox) continue;
+        const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+        if (send[sb] <= sstart[sb]) continue;
+        const int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+        const double *H = Hth + (size_t)th_idx[o] * nrot;
+        const double c = chi[o];
+        const double2 *src = mps + sb * (Ns + 1) * (Ns + 1);
+        for (int r = lane; r < (Ns + 1) * (Ns + 1); r += kWave) in[r] = src[r];
+        __syncthreads();
+        rotate_wave(in, t1, tmp, ep, ec, Ns, H, -0.5 * kPi, c, false);
+        coax_wave(t1, Ns, t2, Nr, SR + (size_t)rix[o] * ne, false);
+        rotate_wave(t2, acc, tmp, ep, ec, Nr, H, c, -0.5 * kPi, true);
+      }
+  double2 *out = mpr + (size_t)tb * (Nr + 1) * (Nr + 1);
+  for (int r = lane; r < (Nr + 1) * (Nr + 1); r += kWave) out[r] = cadd(out[r], acc[r]);
+}
+
+// ---------------------------------------------------------------------
+// L2P + near field (src/local.c:32-310, src/expansions.c:275-344,
+// src/gradient.c:282-399), one thread per target.  With
+// R_n^m = j_n(kr) Pbar_n^|m| e^{i m phi} and the G&D eq 3.7 derivatives
+// re-indexed so that every R_n^m (n <= N+1) is used once:
+//   dz f = k sum R_n^m [a_n^m L_{n+1}^m - a_{n-1}^m L_{n-1}^m]
+//   D+ f = k sum R_n^m [b_n^{-m} L_{n-1}^{m-1} - b_{n+1}^{m-1} L_{n+1}^{m-1}]
+//   D- f = k sum R_n^m [b_n^m L_{n-1}^{m+1} - b_{n+1}^{-m-1} L_{n+1}^{m+1}]
+//   grad_x = (D+ + D-)/2, grad_y = -i (D+ - D-)/2.
+// Near field: G = e^{ikr}/(4 pi i k r), grad_x G = G (ik - 1/r) (x-y)/r,
+// dipole term d nrm.grad_x G; pairs with r <= 1e-6 skipped.  Normals are
+// sorted rows of 4 doubles like the positions.
+// f[tid*fstr + ...]: potential (2 doubles) then gradient (6) when both
+// are asked, accumulated.
+// ---------------------------------------------------------------------
+template <int NMAX>
+__global__ __launch_bounds__(128) void k_hel_field(
+    const double *__restrict__ xt, const uint32_t *__restrict__ tid, unsigned n,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
+    const double *__restrict__ xyz, const double2 *__restrict__ q,
+    const double *__restrict__ nrm, const double2 *__restrict__ dip,
+    unsigned level, double x0, double y0, double z0, double D, int N, double k,
+    const double2 *__restrict__ mpr, int field, double *__restrict__ f, int fstr) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xt[4 * i], y = xt[4 * i + 1], z = xt[4 * i + 2];
+  const uint64_t code = point_code(x, y, z, x0, y0, z0, D);
+  const uint64_t b = code >> (3 * (20 - level));
+  const double w = D / (double)(1u << level);
+  const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
+               zc = z0 + (compact3(b >> 2) + 0.5) * w;
+  double2 pot = make_double2(0.0, 0.0), gz = pot, dpl = pot, dmi = pot;
+  const double2 *Lc = mpr + b * (size_t)(N + 1) * (N + 1);
+  auto Lg = [&](int nn, int mm) -> double2 {
+    if (nn < 0 || nn > N || mm > nn || -mm > nn) return make_double2(0.0, 0.0);
+    return Lc[hidx(nn, mm)];
+  };
+  {
+    double r, C, S, cp, sp, jn[NMAX + 2];
+    sph(x - xc, y - yc, z - zc, r, C, S, cp, sp);
+    bessel_j(N + 1, k * r, jn);
+    const double phi = atan2(sp, cp);
+    double pmm = 1.0 / sqrt(4.0 * kPi);
+    for (int m = 0; m <= N + 1; m++) {
+      if (m > 0) pmm *= sqrt((2.0 * m + 1) / (2.0 * m)) * S;
+      double sn, cs;
+      sincos((double)m * phi, &sn, &cs);
+      double p = pmm, pprev = 0.0;
+      for (int nn = m; nn <= N + 1; nn++) {
+        const double a = jn[nn] * p;
+        for (int sg = 1; sg >= -1; sg -= 2) {
+          if (m == 0 && sg < 0) break;
+          const int mm = sg * m;
+          const double2 R = make_double2(a * cs, sg * a * sn);
+          if ((field & 1) && nn <= N) pot = cadd(pot, cmul(Lg(nn, mm), R));
+          if (field & 2) {
+            const double2 cz = cadd(cscale(Lg(nn + 1, mm), rec_a(nn, mm)),
+                                    cscale(Lg(nn - 1, mm), -rec_a(nn - 1, mm)));
+            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), rec_b(nn, -mm)),
+                                     cscale(Lg(nn + 1, mm - 1), -rec_b(nn + 1, mm - 1)));
+            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), rec_b(nn, mm)),
+                                     cscale(Lg(nn + 1, mm + 1), -rec_b(nn + 1, -mm - 1)));
+            gz = cadd(gz, cmul(cz, R));
+            dpl = cadd(dpl, cmul(cpz, R));
+            dmi = cadd(dmi, cmul(cmz, R));
+          }
+        }
+        const double pn = pbar_up(nn, m, C, p, pprev);
+        pprev = p;
+        p = pn;
+      }
+    }
+  }
+  double2 gx = cscale(cadd(dpl, dmi), 0.5 * k);
+  const double2 dd = cscale(cadd(dpl, cscale(dmi, -1.0)), 0.5 * k);
+  double2 gy = make_double2(dd.y, -dd.x);
+  gz = cscale(gz, k);
+  // near field: the 27 neighbours of the target's box
+  const int bi = compact3(b), bj = compact3(b >> 1), bk = compact3(b >> 2), nb = 1 << level;
+  for (int a = -1; a <= 1; a++)
+    for (int bb = -1; bb <= 1; bb++)
+      for (int c = -1; c <= 1; c++) {
+        const int si = bi + a, sj = bj + bb, sk = bk + c;
+        if (si < 0 || si >= nb || sj < 0 || sj >= nb || sk < 0 || sk >= nb) continue;
+        const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+        for (uint32_t s = start[sb]; s < end[sb]; s++) {
+          const double rx = x - xyz[4 * s], ry = y - xyz[4 * s + 1], rz = z - xyz[4 * s + 2];
+          const double r = sqrt(rx * rx + ry * ry + rz * rz);
+          if (r <= 1e-6) continue;
+          double sn, cs;
+          sincos(k * r, &sn, &cs);
+          // G = e^{ikr} / (4 pi i k r) = (sin - i cos) / (4 pi k r)
+          const double g0 = 1.0 / (4.0 * kPi * k * r);
+          const double2 G = make_double2(sn * g0, -cs * g0);
+          const double2 dG = cmul(G, make_double2(-1.0 / r, k));  // G (ik - 1/r)
+          const double2 G2 = cmul(G, cadd(cmul(make_double2(-1.0 / r, k), make_double2(-1.0 / r, k)),
+                                          make_double2(1.0 / (r * r), 0.0)));
+          const double ux = rx / r, uy = ry / r, uz = rz / r;
+          if (q) {
+            const double2 qq = q[s];
+            const double2 qG = cmul(qq, G), qd = cmul(qq, dG);
+            pot = cadd(pot, qG);
+            gx = cadd(gx, cscale(qd, ux));
+            gy = cadd(gy, cscale(qd, uy));
+            gz = cadd(gz, cscale(qd, uz));
+          }
+          if (dip) {
+            const double2 dq = dip[s];
+            const double nx = nrm[4 * s], ny = nrm[4 * s + 1], nz = nrm[4 * s + 2];
+            const double nr = nx * ux + ny * uy + nz * uz;
+            pot = cadd(pot, cscale(cmul(dq, dG), nr));
+            const double2 t1 = cscale(cmul(dq, G2), nr), t2 = cscale(cmul(dq, dG), 1.0 / r);
+            gx = cadd(gx, cadd(cscale(t1, ux), cscale(t2, nx - nr * ux)));
+            gy = cadd(gy, cadd(cscale(t1, uy), cscale(t2, ny - nr * uy)));
+            gz = cadd(gz, cadd(cscale(t1, uz), cscale(t2, nz - nr * uz)));
+          }
+        }
+      }
+  double *o = f + (size_t)tid[i] * fstr;
+  int at = 0;
+  if (field & 1) {
+    o[0] += pot.x;
+    o[1] += pot.y;
+    at = 2;
+  }
+  if (field & 2) {
+    o[at + 0] += gx.x;
+    o[at + 1] += gx.y;
+    o[at + 2] += gy.x;
+    o[at + 3] += gy.y;
+    o[at + 4] += gz.x;
+    o[at + 5] += gz.y;
+  }
+}
+
+// ---------------------------------------------------------------------
+// FP64 direct sum (src/direct.c:35-294): thread per target, sources
+// streamed through LDS; same output layout as k_hel_field
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_hel_direct(
+    const double *__restrict__ xyz, const double2 *__restrict__ q,
+    const double *__restrict__ nrm, const double2 *__restrict__ dip,
+    unsigned nsrc, const double *__restrict__ xt, unsigned ntgt, double k,
+    int field, double cutoff, double *__restrict__ f, int fstr) {
+  __shared__ double sx[256][4];
+  __shared__ double2 sq[256], sd[256];
+  __shared__ double sn3[256][3];
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < ntgt;
+  const double x = act ? xt[4 * i] : 0.0, y = act ? xt[4 * i + 1] : 0.0,
+               z = act ? xt[4 * i + 2] : 0.0;
+  double2 pot = make_double2(0.0, 0.0), gx = pot, gy = pot, gz = pot;
+  for (unsigned s0 = 0; s0 < nsrc; s0 += 256) {
+    const unsigned s = s0 + threadIdx.x;
+    __syncthreads();
+    if (s < nsrc) {
+      sx[threadIdx.x][0] = xyz[4 * s];
+      sx[threadIdx.x][1] = xyz[4 * s + 1];
+      sx[threadIdx.x][2] = xyz[4 * s + 2];
+      sq[threadIdx.x] = q ? q[s] : make_double2(0.0, 0.0);
+      sd[threadIdx.x] = dip ? dip[s] : make_double2(0.0, 0.0);
+      sn3[threadIdx.x][0] = dip ? nrm[4 * s] : 0.0;
+      sn3[threadIdx.x][1] = dip ? nrm[4 * s + 1] : 0.0;
+      sn3[threadIdx.x][2] = dip ? nrm[4 * s + 2] : 0.0;
+    }
+    __syncthreads();
+    const unsigned ns = nsrc - s0 < 256 ? nsrc - s0 : 256;
+    for (unsigned j = 0; j < ns; j++) {
+      const double rx = x - sx[j][0], ry = y - sx[j][1], rz = z - sx[j][2];
+      const double r = sqrt(rx * rx + ry * ry + rz * rz);
+      if (r <= cutoff) continue;
+      double sn, cs;
+      sincos(k * r, &sn, &cs);
+      const double g0 = 1.0 / (4.0 * kPi * k * r);
+      const double2 G = make_double2(sn * g0, -cs * g0);
+      const double2 dG = cmul(G, make_double2(-1.0 / r, k));
+      const double ux = rx / r, uy = ry / r, uz = rz / r;
+      const double2 qG = cmul(sq[j], G), qd = cmul(sq[j], dG);
+      pot = cadd(pot, qG);
+      gx = cadd(gx, cscale(qd, ux));
+      gy = cadd(gy, cscale(qd, uy));
+      gz = cadd(gz, cscale(qd, uz));
+      if (dip) {
+        const double2 G2 = cmul(G, cadd(cmul(make_double2(-1.0 / r, k), make_double2(-1.0 / r, k)),
+                                        make_double2(1.0 / (r * r), 0.0)));
+        const double nx = sn3[j][0], ny = sn3[j][1], nz = sn3[j][2];
+        const double nr = nx * ux + ny * uy + nz * uz;
+        pot = cadd(pot, cscale(cmul(sd[j], dG), nr));
+        const double2 t1 = cscale(cmul(sd[j], G2), nr), t2 = cscale(cmul(sd[j], dG), 1.0 / r);
+        gx = cadd(gx, cadd(cscale(t1, ux), cscale(t2, nx - nr * ux)));
+        gy = cadd(gy, cadd(cscale(t1, uy), cscale(t2, ny - nr * uy)));
+        gz = cadd(gz, cadd(cscale(t1, uz), cscale(t2, nz - nr * uz)));
+      }
+    }
+  }
+  if (!act) return;
+  double *o = f + (size_t)i * fstr;
+  int at = 0;
+  if (field & 1) {
+    o[0] += pot.x;
+    o[1] += pot.y;
+    at = 2;
+  }
+  if (field & 2) {
+    o[at + 0] += gx.x;
+    o[at + 1] += gx.y;
+    o[at + 2] += gy.x;
+    o[at + 3] += gy.y;
+    o[at + 4] += gz.x;
+    o[at + 5] += gz.y;
+  }
+}
+
+}  // namespace wbfmm
+
+using namespace wbfmm;
+
+extern "C" {
+
+static size_t lds_vectors(int L, int nvec) {
+  return sizeof(double2) * ((size_t)nvec * (L + 1) * (L + 1) + 2 * (L + 1));
+}
+
+int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                      const uint32_t *end, const double *xyz, const double *q,
+                      const double *nrm, const double *dip, unsigned level,
+                      const double *x0, double D, int N, double k, double *mps) {
+  if (N < 0 || N > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
+  if (nbox == 0) return 0;
+  const size_t sh = sizeof(double2) * ((size_t)(N + 2) * (N + 2) + (size_t)(N + 1) * (N + 1)) +
+                    sizeof(double) * (N + 3);
+  WBFMM_LAUNCH("hel_p2m", k_hel_p2m, dim3(nbox), dim3(64), sh, boxes, nbox, start, end,
+               xyz, (const double2 *)q, nrm, (const double2 *)dip, level, x0[0], x0[1],
+               x0[2], D, N, k, (double2 *)mps);
+  return 0;
+}
+
+int wbfmm_hip_hel_m2m(const uint32_t *parents, unsigned npar, const uint32_t *cstart,
+                      const uint32_t *cend, int Nc, int Np, const double *mps_child,
+                      double *mps_parent, const double *Hud, size_t nrot,
+                      const double *SS) {
+  if (npar == 0) return 0;
+  const int L = Nc > Np ? Nc : Np;
+  WBFMM_LAUNCH("hel_m2m", k_hel_m2m, dim3(npar), dim3(64), lds_vectors(L, 5), parents,
+               npar, cstart, cend, Nc, Np, (const double2 *)mps_child,
+               (double2 *)mps_parent, Hud, nrot, (const double2 *)SS);
+  return 0;
+}
+
+int wbfmm_hip_hel_l2l(const uint32_t *parents, unsigned npar, const uint32_t *cflag,
+                      int Np, int Nc, const double *mpr_parent, double *mpr_child,
+                      const double *Hud, size_t nrot, const double *SS) {
+  if (npar == 0) return 0;
+  const int L = Nc > Np ? Nc : Np;
+  WBFMM_LAUNCH("hel_l2l", k_hel_l2l, dim3(npar), dim3(64), lds_vectors(L, 5), parents,
+               npar, cflag, Np, Nc, (const double2 *)mpr_parent, (double2 *)mpr_child,
+               Hud, nrot, (const double2 *)SS);
+  return 0;
+}
+
+int wbfmm_hip_hel_m2l(const uint32_t *targets, unsigned ntgt, const uint32_t *sstart,
+                      const uint32_t *send, unsigned level, int Ns, int Nr,
+                      const double *mps, double *mpr, const double *Hth, size_t nrot,
+                      const int *th_idx, const double *chi, const int *rix,
+                      const double *SR, size_t ne) {
+  if (ntgt == 0) return 0;
+  const int L = Ns > Nr ? Ns : Nr;
+  WBFMM_LAUNCH("hel_m2l", k_hel_m2l, dim3(ntgt), dim3(64), lds_vectors(L, 5), targets,
+               ntgt, sstart, send, (int)level, Ns, Nr, (const double2 *)mps,
+               (double2 *)mpr, Hth, nrot, th_idx, chi, rix, (const double2 *)SR, ne);
+  return 0;
+}
+
+int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
+                        const uint32_t *start, const uint32_t *end, const double *xyz,
+                        const double *q, const double *nrm, const double *dip,
+                        unsigned level, const double *x0, double D, int N, double k,
+                        const double *mpr, int field, double *f, int fstr) {
+  if (n == 0) return 0;
+  if (N < 0) return WBFMM_HIP_ERR_ORDER;
+#define HF(NM)                                                                          \
+  WBFMM_LAUNCH("hel_l2p_p2p", k_hel_field<NM>, dim3((n + 127) / 128), dim3(128), 0, xt, \
+               tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,  \
+               level, x0[0], x0[1], x0[2], D, N, k, (const double2 *)mpr, field, f, fstr)
+  if (N <= 16) HF(16);
+  else if (N <= 32) HF(32);
+  else if (N <= WBFMM_HIP_MAX_ORDER) HF(WBFMM_HIP_MAX_ORDER);
+  else return WBFMM_HIP_ERR_ORDER;
+#undef HF
+  return 0;
+}
+
+int wbfmm_hip_hel_direct(const double *xyz, const double *q, const double *nrm,
+                         const double *dip, unsigned nsrc, const double *xt,
+                         unsigned ntgt, double k, int field, double cutoff, double *f,
+                         int fstr) {
+  if (ntgt == 0) return 0;
+  WBFMM_LAUNCH("hel_direct", k_hel_direct, dim3((ntgt + 255) / 256), dim3(256), 0, xyz,
+               (const double2 *)q, nrm, (const double2 *)dip, nsrc, xt, ntgt, k, field,
+               cutoff, f, fstr);
+  return 0;
+}
+
+}  // extern "C"

@@ -92,6 +92,16 @@ typedef struct {
   int m2l_A2_N[M2L_A2_CACHE];
   double m2l_A2_w[M2L_A2_CACHE];
   double *m2l_A2[M2L_A2_CACHE];
+  /* Helmholtz: wavenumber of the coaxial tables, their orders per level
+     (op.L for S|R, ss_L for R|R), H tables of the 49 list-4 polar angles
+     and of the two child directions (order op.Lmax), offset geometry */
+  double hel_k;
+  int hel_k_set;
+  guint ss_L[WBFMM_TREE_MAX_DEPTH + 1];
+  double *d_hel_Hth, *d_hel_Hud;
+  size_t hel_nrot;
+  int *d_th_idx, *d_rix;
+  double *d_chi;
 } ops_dev;
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
@@ -485,7 +495,16 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
      (src/boxes.c:552-579): [group][coefficient][child]; level 0 has a
      single box */
   if (nb < 8) memcpy(host, tmp, bytes);
-  else
+  else if (t->problem == WBFMM_PROBLEM_HELMHOLTZ) {
+    /* complex coefficients: (re, im) pairs interleaved by child */
+    size_t Cc = C / 2;
+    for (g = 0; g < nb / 8; g++)
+      for (k = 0; k < 8; k++)
+        for (c = 0; c < Cc; c++) {
+          host[((g * Cc + c) * 8 + k) * 2] = tmp[(8 * g + k) * C + 2 * c];
+          host[((g * Cc + c) * 8 + k) * 2 + 1] = tmp[(8 * g + k) * C + 2 * c + 1];
+        }
+  } else
     for (g = 0; g < nb / 8; g++)
       for (k = 0; k < 8; k++)
         for (c = 0; c < C; c++)
@@ -549,20 +568,27 @@ gint wbfmm_tree_laplace_coefficient_init(wbfmm_tree_t *t, guint l, guint nr,
 }
 
 /* strengths (host or device, element stride) into sorted device order */
-static double *gather_strengths(tree_dev *td, double *src, gint sstr,
-                                int dip)
+/* strengths (host or device, element stride) into sorted device order;
+   ncomp doubles per point (1 Laplace, 2 Helmholtz) */
+static double *gather_strengths_n(tree_dev *td, double *src, gint sstr,
+                                  int dip, int ncomp)
 {
   unsigned n = td->t.npoints;
   double **buf = dip ? &td->d_dip : &td->d_q;
   unsigned *cap = dip ? &td->cap_dip : &td->cap_q;
   if (!src) return NULL;
-  if (*cap < n) {
+  if (*cap < ncomp * n) {
     if (*buf) wbfmm_hip_free(*buf);
-    HIPOK(wbfmm_hip_malloc((void **)buf, sizeof(double) * (n ? n : 1)));
-    *cap = n;
+    HIPOK(wbfmm_hip_malloc((void **)buf, sizeof(double) * ncomp * (n ? n : 1)));
+    *cap = ncomp * n;
   }
-  HIPOK(wbfmm_hip_gather_strided(src, sstr, 1, td->d_idx, n, *buf));
+  HIPOK(wbfmm_hip_gather_strided(src, sstr, ncomp, td->d_idx, n, *buf));
   return *buf;
+}
+
+static double *gather_strengths(tree_dev *td, double *src, gint sstr, int dip)
+{
+  return gather_strengths_n(td, src, sstr, dip, 1);
 }
 
 gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
@@ -648,6 +674,15 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     free(od->m2l_plan[n]);
   }
   for (n = 0; n < od->m2l_nA2; n++) wbfmm_hip_free(od->m2l_A2[n]);
+  for (n = 0; n <= WBFMM_TREE_MAX_DEPTH; n++) {
+    wbfmm_hip_free(od->op.SR[n]);
+    wbfmm_hip_free(od->op.SS[n]);
+  }
+  wbfmm_hip_free(od->d_hel_Hth);
+  wbfmm_hip_free(od->d_hel_Hud);
+  wbfmm_hip_free(od->d_th_idx);
+  wbfmm_hip_free(od->d_rix);
+  wbfmm_hip_free(od->d_chi);
   free(od);
   return 0;
 }
@@ -1132,3 +1167,386 @@ gint wbfmm_tree_free(wbfmm_tree_t *t)
 }
 
 gint wbfmm_hip_synchronize(void) { return wbfmm_hip_sync(); }
+
+/* ------------------------------------------------------------------ */
+/* Helmholtz path                                                     */
+/* ------------------------------------------------------------------ */
+
+/* src/boxes.c:536-583: complex coefficient arrays of level l, (n+1)^2
+   complex per box (box-major on the device) */
+gint wbfmm_tree_coefficient_init(wbfmm_tree_t *t, guint l, guint nr, guint ns)
+{
+  tree_dev *td = TD(t);
+  size_t nb = (size_t)1 << (3 * l);
+  if ((int)t->nq < 1)
+    WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
+  if (t->nq != 1) WBFMM_FATAL("only nq == 1 is supported (nq = %u)", t->nq);
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  if (l > t->depth) WBFMM_FATAL("level %u below tree depth %u", l, t->depth);
+  if (ns > WBFMM_HIP_MAX_ORDER || nr > WBFMM_HIP_MAX_ORDER)
+    WBFMM_FATAL("order above %d", WBFMM_HIP_MAX_ORDER);
+  t->order_s[l] = ns;
+  t->order_r[l] = nr;
+  if (ns) {
+    alloc_level(td, l, 0, nb * 2 * (ns + 1) * (ns + 1) * sizeof(double));
+    if (td->host_mirror) set_box_pointers(td, l, 0, 2 * (ns + 1) * (ns + 1));
+  }
+  if (nr) {
+    alloc_level(td, l, 1, nb * 2 * (nr + 1) * (nr + 1) * sizeof(double));
+    if (td->host_mirror) set_box_pointers(td, l, 1, 2 * (nr + 1) * (nr + 1));
+  }
+  return 0;
+}
+
+/* src/boxes.c:341-490: P2M at the leaf level; src, dipoles: complex
+   strengths (2 doubles) at element stride sstr, dstr */
+gint wbfmm_tree_leaf_expansions(wbfmm_tree_t *t, gdouble k, gdouble *src,
+                                gint sstr, gdouble *dipoles, gint dstr,
+                                gboolean zero_expansions, gdouble *work)
+{
+  tree_dev *td = TD(t);
+  unsigned d = t->depth;
+  double *q, *dp;
+  (void)work;
+  if ((int)t->nq < 1)
+    WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  if (!t->mps[d]) WBFMM_FATAL("coefficients not initialised at level %u", d);
+  if (zero_expansions) HIPOK(wbfmm_hip_memset(t->mps[d], 0, td->mps_bytes[d]));
+  if (src == NULL && dipoles == NULL) return 0;
+  if (t->normals == NULL && dipoles != NULL)
+    WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
+  q = gather_strengths_n(td, src, sstr, 0, 2);
+  dp = gather_strengths_n(td, dipoles, dstr, 1, 2);
+  HIPOK(wbfmm_hip_hel_p2m(td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d],
+                          td->d_xyz, q, dp ? td->d_nrm : NULL, dp, d,
+                          (double *)wbfmm_tree_origin(t), t->D, t->order_s[d], k,
+                          (double *)t->mps[d]));
+  return 0;
+}
+
+static void hel_k_check(ops_dev *od, double k)
+{
+  if (od->hel_k_set && od->hel_k != k)
+    WBFMM_FATAL("coaxial operators built for k = %g, not %g", od->hel_k, k);
+  od->hel_k = k;
+  od->hel_k_set = 1;
+}
+
+static size_t coax_entries(guint L) { return (size_t)(L + 1) * (L + 2) * (L + 3) / 6; }
+
+/* src/shift.c:777-827: S|R tables of level `level' (box width D/2^level)
+   at the 15 list-4 distances, order L, kept on the device */
+gint wbfmm_shift_operators_coaxial_SR_init(wbfmm_shift_operators_t *w, gdouble D,
+                                           guint level, guint L, gdouble k,
+                                           gdouble *work)
+{
+  ops_dev *od = (ops_dev *)w;
+  double thetas[64], chi[343], rr[16], wb = D / (double)(1u << level), *h;
+  int th_idx[343], rix[343], nth, nr, i;
+  size_t ne = coax_entries(L);
+  (void)work;
+  if (level > WBFMM_TREE_MAX_DEPTH) WBFMM_FATAL("level %u too deep", level);
+  if (L > w->Lmax) WBFMM_FATAL("order %u above the operators' %u", L, w->Lmax);
+  hel_k_check(od, k);
+  wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
+  h = malloc(sizeof(double) * 2 * ne * nr);
+  if (!h) WBFMM_FATAL("out of host memory");
+  for (i = 0; i < nr; i++)
+    HIPOK(wbfmm_hip_hel_coaxial_table(1, (int)L, k * wb * rr[i], h + 2 * ne * i));
+  if (w->SR[level]) wbfmm_hip_free(w->SR[level]);
+  HIPOK(wbfmm_hip_malloc(&w->SR[level], sizeof(double) * 2 * ne * nr));
+  HIPOK(wbfmm_hip_memcpy(w->SR[level], h, sizeof(double) * 2 * ne * nr));
+  free(h);
+  w->L[level] = L;
+  return 0;
+}
+
+/* src/shift.c:829-861: R|R (= S|S) table for shifts between level
+   `level' and its parents, distance sqrt(3)/2 D/2^level, order L */
+gint wbfmm_shift_operators_coaxial_SS_init(wbfmm_shift_operators_t *w, gdouble D,
+                                           guint level, guint L, gdouble k,
+                                           gdouble *work)
+{
+  ops_dev *od = (ops_dev *)w;
+  double wb = D / (double)(1u << level), *h;
+  size_t ne = coax_entries(L);
+  (void)work;
+  if (level > WBFMM_TREE_MAX_DEPTH) WBFMM_FATAL("level %u too deep", level);
+  if (L > w->Lmax) WBFMM_FATAL("order %u above the operators' %u", L, w->Lmax);
+  hel_k_check(od, k);
+  h = malloc(sizeof(double) * 2 * ne);
+  if (!h) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_hel_coaxial_table(0, (int)L, k * wb * 0.5 * sqrt(3.0), h));
+  if (w->SS[level]) wbfmm_hip_free(w->SS[level]);
+  HIPOK(wbfmm_hip_malloc(&w->SS[level], sizeof(double) * 2 * ne));
+  HIPOK(wbfmm_hip_memcpy(w->SS[level], h, sizeof(double) * 2 * ne));
+  free(h);
+  od->ss_L[level] = L;
+  return 0;
+}
+
+/* H tables and offset geometry of the Helmholtz passes, order Lmax */
+static void ensure_hel_rotations(ops_dev *od)
+{
+  double thetas[64], chi[343], rr[16], tud[2];
+  int th_idx[343], rix[343], nth, nr;
+  int L = (int)od->op.Lmax;
+  if (od->d_hel_Hth) return;
+  wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
+  HIPOK(wbfmm_hip_H_tables(L, nth, thetas, &od->d_hel_Hth, &od->hel_nrot));
+  tud[0] = acos(1.0 / sqrt(3.0));
+  tud[1] = acos(-1.0 / sqrt(3.0));
+  HIPOK(wbfmm_hip_H_tables(L, 2, tud, &od->d_hel_Hud, &od->hel_nrot));
+  HIPOK(wbfmm_hip_malloc((void **)&od->d_th_idx, sizeof(th_idx)));
+  HIPOK(wbfmm_hip_memcpy(od->d_th_idx, th_idx, sizeof(th_idx)));
+  HIPOK(wbfmm_hip_malloc((void **)&od->d_rix, sizeof(rix)));
+  HIPOK(wbfmm_hip_memcpy(od->d_rix, rix, sizeof(rix)));
+  HIPOK(wbfmm_hip_malloc((void **)&od->d_chi, sizeof(chi)));
+  HIPOK(wbfmm_hip_memcpy(od->d_chi, chi, sizeof(chi)));
+}
+
+/* src/pass.c:581-632: M2M from level into level-1 */
+gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                       gdouble *work)
+{
+  tree_dev *td = TD(t);
+  ops_dev *od = (ops_dev *)op;
+  guint Nc, Np;
+  (void)work;
+  if (level <= 1) WBFMM_FATAL("level %u <= 1", level);
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  if (!t->mps[level] || !t->mps[level - 1])
+    WBFMM_FATAL("coefficients not initialised at levels %u, %u", level, level - 1);
+  Nc = t->order_s[level];
+  Np = t->order_s[level - 1];
+  if (!op->SS[level]) WBFMM_FATAL("S|S operators not initialised at level %u", level);
+  if ((Nc > Np ? Nc : Np) > od->ss_L[level] || (Nc > Np ? Nc : Np) > op->Lmax)
+    WBFMM_FATAL("operators at level %u built to order %u", level, od->ss_L[level]);
+  ensure_hel_rotations(od);
+  HIPOK(wbfmm_hip_hel_m2m(td->d_occ[level - 1], td->nocc[level - 1], td->d_start[level],
+                          td->d_end[level], (int)Nc, (int)Np, (double *)t->mps[level],
+                          (double *)t->mps[level - 1], od->d_hel_Hud, od->hel_nrot,
+                          (double *)op->SS[level]));
+  return 0;
+}
+
+static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level)
+{
+  tree_dev *td = TD(t);
+  ops_dev *od = (ops_dev *)op;
+  uint32_t *tl;
+  unsigned nt;
+  guint Ns, Nr;
+  if (level <= 1) WBFMM_FATAL("level %u <= 1", level);
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  if (!t->mpr[level] || !t->mps[level])
+    WBFMM_FATAL("coefficients not initialised at level %u", level);
+  Ns = t->order_s[level];
+  Nr = t->order_r[level];
+  if (!op->SR[level]) WBFMM_FATAL("S|R operators not initialised at level %u", level);
+  if ((Ns > Nr ? Ns : Nr) > op->L[level])
+    WBFMM_FATAL("S|R operators at level %u built to order %u", level, op->L[level]);
+  if ((Ns > Nr ? Ns : Nr) > op->Lmax) WBFMM_FATAL("rotations built to order %u", op->Lmax);
+  ensure_hel_rotations(od);
+  if (td->have_targets) {
+    tl = td->d_act[level];
+    nt = td->nact[level];
+  } else {
+    tl = all_boxes(level, &nt);
+  }
+  HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
+                          (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
+                          od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
+                          (double *)op->SR[level], coax_entries(op->L[level])));
+  if (level == t->depth) return 0;
+  if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
+  if (!op->SS[level + 1]) WBFMM_FATAL("S|S operators not initialised at level %u", level + 1);
+  {
+    guint Nc = t->order_r[level + 1];
+    if ((Nr > Nc ? Nr : Nc) > od->ss_L[level + 1] || (Nr > Nc ? Nr : Nc) > op->Lmax)
+      WBFMM_FATAL("S|S operators at level %u built to order %u", level + 1, od->ss_L[level + 1]);
+    HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
+                            (int)Nr, (int)Nc, (double *)t->mpr[level],
+                            (double *)t->mpr[level + 1], od->d_hel_Hud, od->hel_nrot,
+                            (double *)op->SS[level + 1]));
+  }
+  return 0;
+}
+
+/* src/pass.c:448-537 (and the AVX variant, src/pass-avx.c): M2L at level,
+   then L2L into level+1; work and nthreads are not used */
+gint wbfmm_downward_pass_ref(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                             guint level, gdouble *work, gint nthreads)
+{
+  (void)work;
+  (void)nthreads;
+  return hel_downward(t, op, level);
+}
+
+gint wbfmm_downward_pass_avx(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                             guint level, gdouble *work, gint nthreads)
+{
+  (void)work;
+  (void)nthreads;
+  return hel_downward(t, op, level);
+}
+
+/* field values of n staged targets (Helmholtz): fl bit 1 potential (2
+   doubles), bit 2 gradient (6), potential first */
+static void hel_field_run(tree_dev *td, double k, const double *dxt,
+                          const uint32_t *dtid, unsigned n, double *src, gint sstr,
+                          double *d, gint dstr, int fl, double *f, gint fstr,
+                          size_t nf_rows)
+{
+  wbfmm_tree_t *t = &td->t;
+  unsigned lv = t->depth;
+  double *q, *dp, *df;
+  int staged;
+  size_t nelem;
+  if (!t->mpr[lv]) WBFMM_FATAL("coefficients not initialised at level %u", lv);
+  if (t->normals == NULL && d != NULL)
+    WBFMM_FATAL("no normals in tree but dipole strengths specified (d != NULL)");
+  q = gather_strengths_n(td, src, sstr, 0, 2);
+  dp = gather_strengths_n(td, d, dstr, 1, 2);
+  nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0) : 0;
+  df = field_begin(f, nelem, &staged);
+  HIPOK(wbfmm_hip_hel_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
+                            dp ? td->d_nrm : NULL, dp, lv, (double *)wbfmm_tree_origin(t),
+                            t->D, t->order_r[lv], k, (double *)t->mpr[lv], fl, df, fstr));
+  field_finish(f, df, nelem, staged);
+}
+
+/* src/local.c:264-310: field at one point from the leaf box's local
+   expansion and (eval_neighbours) the 27 neighbour boxes' sources;
+   POTENTIAL: f[0..1], GRADIENT: f[0..5] (POTENTIAL|GRADIENT is refused as
+   in the reference) */
+gint wbfmm_tree_box_local_field(wbfmm_tree_t *t, guint level, guint b, gdouble k,
+                                gdouble *x, gdouble *f, gint fstr, gdouble *src,
+                                gint sstr, gdouble *d, gint dstr,
+                                gboolean eval_neighbours, guint field,
+                                gdouble *work)
+{
+  tree_dev *td = TD(t);
+  double *dx, tmp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t *did;
+  uint64_t *dc;
+  int fl, i;
+  (void)work;
+  (void)b;
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  if (level != t->depth) WBFMM_FATAL("field evaluation is at the leaf level only");
+  switch (field) {
+    case WBFMM_FIELD_POTENTIAL:
+      if (fstr < 2) WBFMM_FATAL("field stride (%d) too small for field calculation", fstr);
+      fl = 1;
+      break;
+    case WBFMM_FIELD_GRADIENT:
+      if (fstr < 6) WBFMM_FATAL("field stride (%d) too small for gradient calculation", fstr);
+      fl = 2;
+      break;
+    default:
+      WBFMM_FATAL("unrecognized field identifier (%u)", field);
+  }
+  stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc, (double *)wbfmm_tree_origin(t),
+                t->D, 0);
+  if (!eval_neighbours) { src = NULL; d = NULL; }
+  hel_field_run(td, k, dx, did, 1, src, sstr, d, dstr, fl, tmp, 8, 1);
+  for (i = 0; i < (fl == 1 ? 2 : 6); i++) f[i] += tmp[i];
+  wbfmm_hip_free(dx);
+  wbfmm_hip_free(did);
+  wbfmm_hip_free(dc);
+  return 0;
+}
+
+/* addition: potential and/or gradient at ntgt points in one launch;
+   f gets 2 (potential), 6 (gradient) or 8 (both, potential first)
+   doubles per target at element stride fstr, accumulated */
+gint wbfmm_helmholtz_field_targets(wbfmm_tree_t *t, gdouble k, gdouble *xt,
+                                   gsize tstr, guint ntgt, gdouble *src, gint sstr,
+                                   gdouble *d, gint dstr, guint field, gdouble *f,
+                                   gint fstr)
+{
+  tree_dev *td = TD(t);
+  double *dx;
+  uint32_t *did;
+  uint64_t *dc;
+  int fl = field & 3;
+  if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) return -2;
+  if (fl == 0 || (field & ~3u)) return -2;
+  if (fstr < ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0)) return -2;
+  if (ntgt == 0) return 0;
+  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, (double *)wbfmm_tree_origin(t), t->D, 1);
+  hel_field_run(td, k, dx, did, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
+  HIPOK(wbfmm_hip_sync());
+  wbfmm_hip_free(dx);
+  wbfmm_hip_free(did);
+  wbfmm_hip_free(dc);
+  return 0;
+}
+
+/* src/direct.c:35-294: FP64 direct sum on the GPU at one field point;
+   grad: gradient (6 doubles) instead of potential (2) */
+static gint hel_total(gdouble k, gdouble *xs, gint xstride, gdouble *src, gint sstride,
+                      gdouble *normals, gint nstr, gdouble *dipoles, gint dstr, gint nq,
+                      gint nsrc, gdouble *xf, gdouble *field, int grad)
+{
+  double *dxs, *dq = NULL, *dd = NULL, *dn = NULL, *dxt, *df, tmp[8];
+  uint32_t *ids;
+  uint32_t *h;
+  gint i;
+  if (nq != 1) WBFMM_FATAL("only nq == 1 is supported");
+  if (nsrc <= 0) return 0;
+  HIPOK(wbfmm_hip_malloc((void **)&dxs, sizeof(double) * 4 * nsrc));
+  HIPOK(wbfmm_hip_stage4(xs, xstride * sizeof(double), nsrc, dxs));
+  HIPOK(wbfmm_hip_malloc((void **)&ids, sizeof(uint32_t) * nsrc));
+  h = malloc(sizeof(uint32_t) * nsrc);
+  for (i = 0; i < nsrc; i++) h[i] = i;
+  HIPOK(wbfmm_hip_memcpy(ids, h, sizeof(uint32_t) * nsrc));
+  free(h);
+  if (src) {
+    HIPOK(wbfmm_hip_malloc((void **)&dq, sizeof(double) * 2 * nsrc));
+    HIPOK(wbfmm_hip_gather_strided(src, sstride, 2, ids, nsrc, dq));
+  }
+  if (dipoles) {
+    if (!normals) WBFMM_FATAL("dipole strengths without normals");
+    HIPOK(wbfmm_hip_malloc((void **)&dd, sizeof(double) * 2 * nsrc));
+    HIPOK(wbfmm_hip_gather_strided(dipoles, dstr, 2, ids, nsrc, dd));
+    HIPOK(wbfmm_hip_malloc((void **)&dn, sizeof(double) * 4 * nsrc));
+    HIPOK(wbfmm_hip_stage4(normals, nstr * sizeof(double), nsrc, dn));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&dxt, sizeof(double) * 4));
+  {
+    double p4[4] = {xf[0], xf[1], xf[2], 0.0};
+    HIPOK(wbfmm_hip_memcpy(dxt, p4, sizeof(p4)));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&df, sizeof(double) * 8));
+  HIPOK(wbfmm_hip_memset(df, 0, sizeof(double) * 8));
+  HIPOK(wbfmm_hip_hel_direct(dxs, dq, dn, dd, (unsigned)nsrc, dxt, 1, k, grad ? 2 : 1, 0.0,
+                             df, 8));
+  HIPOK(wbfmm_hip_memcpy(tmp, df, sizeof(double) * 8));
+  for (i = 0; i < (grad ? 6 : 2); i++) field[i] += tmp[i];
+  wbfmm_hip_free(dxs); wbfmm_hip_free(ids); wbfmm_hip_free(dq);
+  wbfmm_hip_free(dd); wbfmm_hip_free(dn); wbfmm_hip_free(dxt); wbfmm_hip_free(df);
+  return 0;
+}
+
+gint wbfmm_total_field(gdouble k, gdouble *xs, gint xstride, gdouble *src,
+                       gint sstride, gdouble *normals, gint nstr, gdouble *dipoles,
+                       gint dstr, gint nq, gint nsrc, gdouble *xf, gdouble *field,
+                       gint fstr)
+{
+  (void)fstr;
+  return hel_total(k, xs, xstride, src, sstride, normals, nstr, dipoles, dstr, nq, nsrc,
+                   xf, field, 0);
+}
+
+gint wbfmm_total_field_grad(gdouble k, gdouble *xs, gint xstride, gdouble *src,
+                            gint sstride, gdouble *normals, gint nstr, gdouble *dipoles,
+                            gint dstr, gint nq, gint nsrc, gdouble *xf, gdouble *field,
+                            gint fstr)
+{
+  (void)fstr;
+  return hel_total(k, xs, xstride, src, sstride, normals, nstr, dipoles, dstr, nq, nsrc,
+                   xf, field, 1);
+}
