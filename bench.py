@@ -113,6 +113,138 @@ def cpu_baseline(p, seconds_hint=20.0):
                       f"{el:.1f} s wall, M2L on {nth} threads"}
 
 
+def sphere_points(seed, n):
+    """n points on the unit sphere: normalised Gaussian triples (Box-Muller
+    on SplitMix64 streams)"""
+    g = np.stack([normal(seed + 101 * c, n) for c in range(3)], 1)
+    return g / np.linalg.norm(g, axis=1)[:, None]
+
+
+def hel_workload(kind):
+    """(xs, q, nrm, d, xt, k, depth, orders, x0, D, description)"""
+    if kind == "helmholtz-cube":
+        # 1e4 monopoles uniform in [-1,1]^3, complex N(0,1) strengths, k = 1,
+        # depth 4, order ceil(k w) + 10 = 11, 1e4 separate uniform targets
+        n = 10_000
+        xs = 2.0 * uniform(1, 3 * n).reshape(n, 3) - 1.0
+        xt = 2.0 * uniform(2, 3 * n).reshape(n, 3) - 1.0
+        q = normal(11, n) + 1j * normal(12, n)
+        k, depth, orders = 1.0, 4, [11] * 17
+        nrm = d = None
+        desc = ("helmholtz_fp64_cube: 1e4 monopoles uniform in [-1,1]^3, complex N(0,1), k=1, "
+                "depth 4, order 11, 1e4 separate targets, potential+gradient")
+    else:
+        # 1e6 points on the unit sphere, half monopoles and half radial normal
+        # dipoles (complex N(0,1)), k = 16, depth 7, orders from the
+        # truncation number at tol 1e-6 (25, 23, 23, 23, 23, 23 at levels
+        # 2..7), targets = the points
+        n = 1_000_000
+        xs = sphere_points(4, n)
+        half = np.arange(n) % 2 == 0
+        q = np.where(half, normal(13, n) + 1j * normal(14, n), 0.0)
+        d = np.where(~half, normal(15, n) + 1j * normal(16, n), 0.0)
+        nrm = xs.copy()
+        xt = xs
+        k, depth = 16.0, 7
+        orders = [25, 25, 25, 23, 23, 23, 23, 23] + [23] * 9
+        desc = ("helmholtz_fp64_sphere: 1e6 points on the unit sphere, 50% monopoles / 50% radial "
+                "dipoles, complex N(0,1), k=16, depth 7, orders 25,23,23,23,23,23 (levels 2..7), "
+                "targets = sources, potential+gradient")
+    lo = np.minimum(xs.min(0), xt.min(0))
+    hi = np.maximum(xs.max(0), xt.max(0))
+    x0 = lo - 0.01
+    D = float((hi - lo).max()) + 0.02
+    return xs, q, nrm, d, xt, k, depth, orders, x0, D, desc
+
+
+def run_helmholtz(args):
+    import wbfmm_amd as W
+    L = W.lib()
+    xs, q, nrm, d, xt, k, depth, orders, x0, D, desc = hel_workload(args.workload)
+    n, nt = len(xs), len(xt)
+    c2 = lambda z: np.ascontiguousarray(np.stack([z.real, z.imag], 1))
+    print(f"bench: workload {args.workload} ready", file=sys.stderr, flush=True)
+    d_xs = W.DeviceBuffer.from_array(np.ascontiguousarray(xs))
+    d_xt = d_xs if xt is xs else W.DeviceBuffer.from_array(np.ascontiguousarray(xt))
+    d_q = W.DeviceBuffer.from_array(c2(q))
+    d_d = None if d is None else W.DeviceBuffer.from_array(c2(d))
+    d_n = None if nrm is None else W.DeviceBuffer.from_array(np.ascontiguousarray(nrm))
+    d_f = W.DeviceBuffer(nt * 64)
+    t0 = time.perf_counter()
+    S = W.HelmholtzSolver(d_xs, x0, D, depth, orders, orders, k, normals=d_n, nsrc=n,
+                          targets=d_xt, ntgt=nt)
+    W.sync()
+    t_tables = time.perf_counter() - t0
+
+    def step():
+        d_f.zero()
+        S.build_tree()
+        S.passes(d_q, d=d_d)
+        S.field(d_xt, d_q, d_f, field=3, fstr=8, ntgt=nt, d=d_d)
+
+    for _ in range(args.warmup):
+        step()
+    W.sync()
+    W.Timing.reset()
+    W.Timing.enable(True)
+    lat = []
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        step()
+        W.sync()
+        lat.append(time.perf_counter() - t1)
+    total = time.perf_counter() - t_all
+    W.Timing.enable(False)
+    kern = {}
+    for name in ("hel_m2l", "hel_m2m", "hel_l2l", "hel_p2m", "hel_l2p_p2p", "tree_sort"):
+        cnt, ms = W.Timing.query(name)
+        if cnt:
+            kern[name] = {"launches_per_step": cnt / args.steps, "ms_per_step": ms / args.steps,
+                          "avg_ms": ms / cnt}
+    print(f"bench: timed {args.steps} steps, {1e3 * total / args.steps:.2f} ms/step",
+          file=sys.stderr, flush=True)
+    f = d_f.download(np.zeros((nt, 8)))
+    rng = np.random.default_rng(5)
+    idx = np.sort(rng.choice(nt, min(args.check_targets, nt), replace=False))
+    d_s4 = W.DeviceBuffer(n * 32)
+    L.wbfmm_hip_stage4(d_xs.ptr, 24, n, d_s4.ptr)
+    d_n4 = None
+    if d_n is not None:
+        d_n4 = W.DeviceBuffer(n * 32)
+        L.wbfmm_hip_stage4(d_n.ptr, 24, n, d_n4.ptr)
+    d_t4 = W.DeviceBuffer(len(idx) * 32)
+    L.wbfmm_hip_stage4(W.DeviceBuffer.from_array(np.ascontiguousarray(xt[idx])).ptr, 24,
+                       len(idx), d_t4.ptr)
+    d_fd = W.DeviceBuffer(len(idx) * 64)
+    d_fd.zero()
+    L.wbfmm_hip_hel_direct(d_s4.ptr, d_q.ptr, None if d_n4 is None else d_n4.ptr,
+                           None if d_d is None else d_d.ptr, n, d_t4.ptr, len(idx), k, 3, 1e-6,
+                           d_fd.ptr, 8)
+    fd = d_fd.download(np.zeros((len(idx), 8)))
+    fc = f[idx][:, 0::2] + 1j * f[idx][:, 1::2]
+    fdc = fd[:, 0::2] + 1j * fd[:, 1::2]
+    err_pot = float(np.linalg.norm(fc[:, 0] - fdc[:, 0]) / np.linalg.norm(fdc[:, 0]))
+    err_grad = float(np.linalg.norm(fc[:, 1:] - fdc[:, 1:]) / np.linalg.norm(fdc[:, 1:]))
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
+        "value": nt * args.steps / total, "unit": "targets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps,
+        "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 streams)",
+        "config": {"workload": desc, "n_sources": n, "n_targets": nt, "depth": depth,
+                   "k": k, "orders": orders[2:depth + 1], "parallelism": "single GPU"},
+        "error_vs_fp64_direct": {"targets_checked": int(len(idx)), "rel_l2_potential": err_pot,
+                                 "rel_l2_gradient": err_grad},
+        "tables_s": t_tables,
+        "kernels": kern,
+    }
+    S.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +255,10 @@ def main():
     ap.add_argument("--order", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-targets", type=int, default=2000)
+    ap.add_argument("--workload", default="laplace",
+                    choices=["laplace", "helmholtz-cube", "helmholtz-sphere"],
+                    help="laplace: the headline configuration (default); helmholtz-cube / "
+                         "helmholtz-sphere: the Helmholtz configurations")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with a copy kernel of known size (wbfmm_hip_stage4 over 8Mi rows: "
                          "192 MiB read, 256 MiB written) to scale HBM counters")
@@ -131,6 +267,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "laplace":
+        run_helmholtz(args)
+        return
 
     import wbfmm_amd as W
     L = W.lib()

@@ -323,9 +323,15 @@ gint wbfmm_target_list_coefficients_init(wbfmm_target_list_t *l,
 gint wbfmm_laplace_target_list_local_coefficients(wbfmm_target_list_t *l,
                                                   guint source,
                                                   gdouble *work);
-/* src/targets.c:424-603: fused L2P + near field for every target.  For
-   Laplace, field POTENTIAL|GRADIENT gives 4 values per target (potential
-   then gradient), unlike the reference, which asserts. */
+/* src/wbfmm.h:1099-1101 (Helmholtz precompute): keeps the wavenumber; the
+   GPU evaluates the regular functions on the fly */
+gint wbfmm_target_list_local_coefficients(wbfmm_target_list_t *l, gdouble k,
+                                          gdouble *work);
+/* src/targets.c:424-603: fused L2P + near field for every target.  Laplace:
+   1, 3 or 4 values per target (potential, gradient, both with the
+   potential first); Helmholtz: 2, 6 or 8 doubles (complex).  Gradients and
+   POTENTIAL|GRADIENT are evaluated, where the reference asserts
+   (src/targets.c:280, :562). */
 gint wbfmm_target_list_local_field(wbfmm_target_list_t *l, gdouble *src,
                                    gint sstr, gdouble *d, gint dstr,
                                    gdouble *f, gint fstr);

@@ -184,3 +184,28 @@ def test_hel_config1_against_direct(wb, oracle):
         e = np.linalg.norm(fc[:, c] - fd[:, c]) / np.linalg.norm(fd[:, c])
         print(f"config1 component {c}: rel L2 error {e:.3e}", flush=True)
         assert e < 1e-4, (c, e)
+
+
+def test_hel_target_list_matches_field_targets(wb, oracle):
+    # the target-list path (src/targets.c) gives the batched field
+    import ctypes
+    rng = np.random.default_rng(12)
+    xs = rng.uniform(-1, 1, (1500, 3))
+    xt = np.ascontiguousarray(rng.uniform(-1, 1, (300, 3)))
+    q = rng.normal(size=1500) + 1j * rng.normal(size=1500)
+    x0, D = fit_box(xs, xt)
+    k = 1.7
+    f, S = gpu_hel(wb, xs, q, xt, x0, D, 3, 9, k, keep=True)
+    L = wb.lib()
+    tl = L.wbfmm_target_list_new(S.t, len(xt))
+    L.wbfmm_target_list_add_points(tl, wb.ptr(xt), 24, len(xt))
+    L.wbfmm_target_list_coefficients_init(tl, 3)
+    L.wbfmm_target_list_local_coefficients.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                                       ctypes.c_void_p]
+    L.wbfmm_target_list_local_coefficients(tl, k, None)
+    g = np.zeros((len(xt), 8))
+    L.wbfmm_target_list_local_field(tl, wb.ptr(c2(q)), 2, None, 2, wb.ptr(g), 8)
+    wb.sync()
+    L.wbfmm_target_list_free(tl)
+    S.close()
+    assert np.abs(g - f).max() < 1e-12 * np.abs(f).max()

@@ -348,8 +348,12 @@ class HelmholtzSolver:
     targets.  Strengths are complex: (n, 2) float64 arrays (re, im)."""
 
     def __init__(self, xs, x0, D, depth, order_s, order_r, k, pstr=24,
-                 normals=None, nstr=24, nsrc=None):
+                 normals=None, nstr=24, nsrc=None, targets=None, ntgt=None, tstr=24):
         L = lib()
+        self.targets = targets
+        self.ntgt = ntgt if ntgt is not None else (None if targets is None else len(targets))
+        self.tstr = tstr
+        self.tl = None
         self.depth = depth
         self.k = float(k)
         self.nsrc = len(xs) if nsrc is None else nsrc
@@ -387,6 +391,12 @@ class HelmholtzSolver:
         self.tree.contents.nq = 1
         for l in range(1, self.depth + 1):
             L.wbfmm_tree_coefficient_init(self.t, l, self.order_r[l], self.order_s[l])
+        if self.targets is not None:
+            # registered targets restrict the downward pass to boxes holding
+            # sources or targets
+            if self.tl is None:
+                self.tl = L.wbfmm_target_list_new(self.t, self.ntgt)
+            L.wbfmm_target_list_add_points(self.tl, ptr(self.targets), self.tstr, self.ntgt)
 
     def passes(self, q, sstr=2, d=None, dstr=2):
         L = lib()
@@ -419,6 +429,9 @@ class HelmholtzSolver:
 
     def close(self):
         L = lib()
+        if self.tl:
+            L.wbfmm_target_list_free(self.tl)
+            self.tl = None
         if self.t:
             L.wbfmm_tree_free(self.t)
             self.t = None

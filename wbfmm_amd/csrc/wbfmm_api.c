@@ -69,6 +69,7 @@ typedef struct {
 
 typedef struct {
   wbfmm_target_list_t l; /* public part, must stay first */
+  double hel_k;          /* wavenumber (Helmholtz lists) */
   double *d_xt;          /* target positions (sorted), 4 doubles each */
   uint32_t *d_tid;       /* sorted -> caller index */
   uint64_t *d_codes;
@@ -1110,6 +1111,27 @@ gint wbfmm_laplace_target_list_local_coefficients(wbfmm_target_list_t *l,
   return 0;
 }
 
+/* src/targets.c:424-603 (Helmholtz precompute): the GPU evaluates the
+   regular functions on the fly, so only the wavenumber is kept */
+gint wbfmm_target_list_local_coefficients(wbfmm_target_list_t *l, gdouble k,
+                                          gdouble *work)
+{
+  (void)work;
+  if (l->t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
+  ((tlist_dev *)l)->hel_k = k;
+  return 0;
+}
+
+static void hel_field_run(tree_dev *td, double k, const double *dxt,
+                          const uint32_t *dtid, unsigned n, double *src, gint sstr,
+                          double *d, gint dstr, int fl, double *f, gint fstr,
+                          size_t nf_rows);
+
+/* fused L2P + near field for every target of the list.  Laplace: f holds
+   1 (potential) or 3 (gradient) values, 4 for both (potential first);
+   Helmholtz: 2, 6 or 8 doubles (complex values).  The reference's target
+   lists stop at gradients (src/targets.c:280, :562); here both kinds
+   evaluate them. */
 gint wbfmm_target_list_local_field(wbfmm_target_list_t *l, gdouble *src,
                                    gint sstr, gdouble *d, gint dstr,
                                    gdouble *f, gint fstr)
@@ -1118,8 +1140,13 @@ gint wbfmm_target_list_local_field(wbfmm_target_list_t *l, gdouble *src,
   tree_dev *td = TD(l->t);
   int fl = (int)(l->field & 3);
   if (fl == 0) WBFMM_FATAL("unsupported field type %u", l->field);
-  if (l->t->problem != WBFMM_PROBLEM_LAPLACE)
-    WBFMM_FATAL("only Laplace target lists are implemented");
+  if (l->t->problem == WBFMM_PROBLEM_HELMHOLTZ) {
+    if (fstr < ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0))
+      WBFMM_FATAL("field stride %d too small", fstr);
+    hel_field_run(td, ld->hel_k, ld->d_xt, ld->d_tid, l->npoints, src, sstr, d, dstr, fl, f,
+                  fstr, l->npoints);
+    return 0;
+  }
   if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0))
     WBFMM_FATAL("field stride %d too small", fstr);
   lap_field_run(td, ld->d_xt, ld->d_tid, l->npoints, src, sstr, d, dstr, fl,
