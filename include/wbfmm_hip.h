@@ -171,6 +171,28 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const double *A3, const double *A2,
                            const double *CS,
                            const wbfmm_m2l_plan_t *plan_host);
+/* tables of the MFMA M2M / L2L for plan P (same bins and slot maps):
+   A1, A3: 2 x KT x 64 (polar angles acos(1/sqrt3), acos(-1/sqrt3));
+   CSup, CSdn: 8 x (N+1) x 2, cos and sin of m chi of child c's
+   child -> parent and parent -> child directions */
+int wbfmm_hip_lap_shift_tables(const wbfmm_m2l_plan_t *plan, double *A1,
+                               double *A3, double *CSup, double *CSdn);
+/* stage-2 tiles (KT x 64): kind 0 S|S c (-t)^(n-nu) (M2M), kind 1 R|R
+   c t^(nu-n) (L2L), t = sqrt(3)/2 times the child box width */
+int wbfmm_hip_lap_shift_coax(const wbfmm_m2l_plan_t *plan, int kind, double t,
+                             double *A2);
+/* MFMA M2M (mode 0: each listed parent from its children, child ranges
+   cstart/cend; the parent is overwritten) or L2L (mode 1: each listed
+   parent's local expansion added to its children with cflag != 0, all
+   when NULL); in/out: source/destination level arrays (box-major), orders
+   equal (2..15) (src/shift-laplace.c:225-789) */
+int wbfmm_hip_lap_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
+                             const uint32_t *cstart, const uint32_t *cend,
+                             const uint32_t *cflag, int N, const double *in,
+                             double *out, const void *plan_dev,
+                             const double *A1, const double *A3,
+                             const double *A2, const double *CS,
+                             const wbfmm_m2l_plan_t *plan_host);
 /* one v_mfma_f64_16x16x4f64 with A[i][k] = 1+i+100k, B[k][j] = 1+3k+1000j;
    host_out (256 doubles) gets the four result values of each lane */
 int wbfmm_hip_mfma_layout_probe(double *host_out);

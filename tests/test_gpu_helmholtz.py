@@ -136,9 +136,10 @@ def test_hel_total_field_api(wb, oracle):
     f = np.zeros(2)
     g = np.zeros(6)
     L = wb.lib()
-    assert L.wbfmm_total_field(k, wb.ptr(xs), 3, wb.ptr(c2(q)), 2, None, 0, None, 0, 1, 200,
+    qq = c2(q)  # kept alive across the calls that read it
+    assert L.wbfmm_total_field(k, wb.ptr(xs), 3, wb.ptr(qq), 2, None, 0, None, 0, 1, 200,
                                wb.ptr(xf), wb.ptr(f), 2) == 0
-    assert L.wbfmm_total_field_grad(k, wb.ptr(xs), 3, wb.ptr(c2(q)), 2, None, 0, None, 0, 1,
+    assert L.wbfmm_total_field_grad(k, wb.ptr(xs), 3, wb.ptr(qq), 2, None, 0, None, 0, 1,
                                     200, wb.ptr(xf), wb.ptr(g), 6) == 0
     fo = oracle.hel_direct(k, xs, q, xf[None])[0]
     assert abs(f[0] + 1j * f[1] - fo[0]) < 1e-12 * abs(fo[0])
@@ -153,14 +154,15 @@ def test_hel_box_local_field_api(wb, oracle):
     x0, D = fit_box(xs)
     f, S = gpu_hel(wb, xs, q, xs[:5], x0, D, 3, 8, 1.1, keep=True)
     L = wb.lib()
+    qq = c2(q)  # kept alive across the calls that read it
     for i in range(5):
         x = np.ascontiguousarray(xs[i])
         b = L.wbfmm_point_box(S.t, 3, wb.ptr(x))
         fp = np.zeros(2)
         fgr = np.zeros(6)
-        L.wbfmm_tree_box_local_field(S.t, 3, b, 1.1, wb.ptr(x), wb.ptr(fp), 2, wb.ptr(c2(q)), 2,
+        L.wbfmm_tree_box_local_field(S.t, 3, b, 1.1, wb.ptr(x), wb.ptr(fp), 2, wb.ptr(qq), 2,
                                      None, 0, 1, 1, None)
-        L.wbfmm_tree_box_local_field(S.t, 3, b, 1.1, wb.ptr(x), wb.ptr(fgr), 6, wb.ptr(c2(q)), 2,
+        L.wbfmm_tree_box_local_field(S.t, 3, b, 1.1, wb.ptr(x), wb.ptr(fgr), 6, wb.ptr(qq), 2,
                                      None, 0, 1, 2, None)
         assert np.abs(fp - f[i, :2]).max() < 1e-12 * np.abs(f[i, :2]).max()
         assert np.abs(fgr - f[i, 2:]).max() < 1e-12 * np.abs(f[i, 2:]).max()
@@ -204,7 +206,8 @@ def test_hel_target_list_matches_field_targets(wb, oracle):
                                                        ctypes.c_void_p]
     L.wbfmm_target_list_local_coefficients(tl, k, None)
     g = np.zeros((len(xt), 8))
-    L.wbfmm_target_list_local_field(tl, wb.ptr(c2(q)), 2, None, 2, wb.ptr(g), 8)
+    qq = c2(q)  # kept alive across the call that reads it
+    L.wbfmm_target_list_local_field(tl, wb.ptr(qq), 2, None, 2, wb.ptr(g), 8)
     wb.sync()
     L.wbfmm_target_list_free(tl)
     S.close()

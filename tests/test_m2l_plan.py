@@ -103,3 +103,111 @@ def test_theta_table_has_49_angles(wb):
     o = (0 + 3) * 49 + (0 + 3) * 7 + (2 + 3)
     assert P.chi[o] == 0.0 and P.r[o] == 2.0
     assert np.allclose(CS[o], [[1.0, 0.0]] * 5)
+
+
+def emulate_shift(P, A1, A3, A2, CS, th, M):
+    """the MFMA M2M / L2L kernels' arithmetic for one vector: Zin, three
+    tile products, Zout (same slots and maps as the M2L)"""
+    N, NU, NB = P.N, P.NU, P.NB
+    S = P.S + 1
+    x = np.zeros(S)
+    for n in range(N + 1):
+        for m in range(n + 1):
+            u = n * (n + 1) // 2 + m
+            re = M[n * n + max(2 * m - 1, 0)]
+            im = M[n * n + 2 * m] if m else 0.0
+            c, s = CS[m]
+            x[P.s1[u] & 0xffff] = c * re - s * im
+            x[(P.s1[u] >> 16) & 0xffff] = s * re + c * im
+
+    def tiles(A, v):
+        y = np.zeros(S)
+        for b in range(NB):
+            for j in range(P.nk[b]):
+                a = A[P.koff[b] + j].reshape(4, 16).T
+                y[16 * b:16 * b + 16] += a @ v[16 * b + 4 * j:16 * b + 4 * j + 4]
+        return y
+
+    def move(y, mp):
+        z = np.zeros(S)
+        for r in range(16 * NB):
+            z[mp[r] >> 6] = y[r]
+        return z
+
+    a = move(tiles(A3[th], move(tiles(A2, move(tiles(A1[th], x), P.map1)), P.map2)), P.map3)
+    out = np.zeros((N + 1) ** 2)
+    for n in range(N + 1):
+        for nu in range(n + 1):
+            u = n * (n + 1) // 2 + nu
+            tr, ti = a[u], a[NU + u]
+            c, s = CS[nu]
+            if nu == 0:
+                out[n * n] = tr
+            else:
+                out[n * n + 2 * nu - 1] = c * tr + s * ti
+                out[n * n + 2 * nu] = c * ti - s * tr
+    return out
+
+
+def shift_tables(wb, N, t):
+    import ctypes
+    P = wb.M2LPlan()
+    assert wb.lib().wbfmm_hip_lap_m2l_tables(N, ctypes.byref(P), None, None, None) == 0
+    A1 = np.zeros((2, P.KT, 64))
+    A3 = np.zeros((2, P.KT, 64))
+    CSu = np.zeros((8, N + 1, 2))
+    CSd = np.zeros((8, N + 1, 2))
+    L = wb.lib()
+    L.wbfmm_hip_lap_shift_tables.argtypes = [ctypes.c_void_p] * 5
+    L.wbfmm_hip_lap_shift_coax.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_void_p]
+    assert L.wbfmm_hip_lap_shift_tables(ctypes.byref(P), wb.ptr(A1), wb.ptr(A3), wb.ptr(CSu),
+                                        wb.ptr(CSd)) == 0
+    SS = np.zeros((P.KT, 64))
+    RR = np.zeros((P.KT, 64))
+    assert L.wbfmm_hip_lap_shift_coax(ctypes.byref(P), 0, t, wb.ptr(SS)) == 0
+    assert L.wbfmm_hip_lap_shift_coax(ctypes.byref(P), 1, t, wb.ptr(RR)) == 0
+    return P, A1, A3, CSu, CSd, SS, RR
+
+
+@pytest.mark.parametrize("N", [3, 8, 12, 15])
+def test_shift_tables_m2m_l2l(wb, oracle, N):
+    # M2M: child multipole -> parent equals the multipole of the same
+    # sources about the parent centre (the Laplace M2M is exact degree by
+    # degree); L2L: parent local expansion -> child gives the same field in
+    # the child box (exact for the truncated expansion)
+    wc = 0.25
+    t = 0.5 * math.sqrt(3.0) * wc
+    P, A1, A3, CSu, CSd, SS, RR = shift_tables(wb, N, t)
+    rng = np.random.default_rng(N + 40)
+    for c in range(8):
+        off = (np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], float) - 0.5) * wc
+        src = off + rng.uniform(-0.5, 0.5, (4, 3)) * wc
+        q = rng.normal(size=4)
+        M = np.zeros((N + 1) ** 2)
+        for s_, qq in zip(src, q):
+            oracle.lap_p2m(N, off, s_, qq, M)
+        Mp = emulate_shift(P, A1, A3, SS, CSu[c], (c >> 2) & 1, M)
+        Mref = np.zeros((N + 1) ** 2)
+        for s_, qq in zip(src, q):
+            oracle.lap_p2m(N, np.zeros(3), s_, qq, Mref)
+        assert np.abs(Mp - Mref).max() < 1e-12 * np.abs(Mref).max(), c
+        # a local expansion about the parent centre from far sources
+        far = np.array([2.0, 1.0, -1.5]) + rng.uniform(-0.1, 0.1, (3, 3))
+        qf = rng.normal(size=3)
+        Mf = np.zeros((N + 1) ** 2)
+        xc = np.array([2.0, 1.0, -1.5])
+        for s_, qq in zip(far, qf):
+            oracle.lap_p2m(N, xc, s_, qq, Mf)
+        # M2L from xc to the origin: rotate, S|R, rotate back
+        rr = np.linalg.norm(xc)
+        the = math.acos(-xc[2] / rr)
+        chi = math.atan2(-xc[1], -xc[0])
+        H = oracle.H_rotation(N, the)
+        Lp = oracle.lap_rotate(oracle.lap_coaxial(2, oracle.lap_rotate(Mf, N, H, -0.5 * math.pi, chi),
+                                                  N, N, rr), N, H, chi, -0.5 * math.pi)
+        Lc = emulate_shift(P, A1, A3, RR, CSd[c], 1 - ((c >> 2) & 1), Lp)
+        xp = off + rng.uniform(-0.4, 0.4, 3) * wc
+        fl = oracle.lap_l_eval(N, off, Lc, xp, 1)[0]
+        fp = oracle.lap_l_eval(N, np.zeros(3), Lp, xp, 1)[0]
+        assert abs(fl - fp) < 1e-12 * abs(fp), (c, fl, fp)

@@ -354,6 +354,169 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   }
 }
 
+// ---------------------------------------------------------------------
+// M2M (MODE 0, src/shift-laplace.c:225-442) and L2L (MODE 1, :562-789) in
+// the same split MFMA form: a wavefront takes 16 parents and, for each of
+// the 8 child directions, translates the 16 vectors at once.  M2M sums
+// the children of each parent in registers and writes the parent; L2L
+// translates each parent's local expansion to child c and adds it there.
+// Child direction c uses polar-angle table th(c) (0: acos(1/sqrt3), 1:
+// acos(-1/sqrt3)) and the azimuthal factors CS[c].
+// ---------------------------------------------------------------------
+template <int N, int MODE>
+__global__ __launch_bounds__(64) void k_lap_shift_mfma(
+    const uint32_t *__restrict__ parents, unsigned npar,
+    const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ cend,
+    const uint32_t *__restrict__ cflag, const double *__restrict__ in,
+    double *__restrict__ out, const wbfmm_m2l_plan_t *__restrict__ P,
+    const double *__restrict__ A1, const double *__restrict__ A3,
+    const double *__restrict__ A2, const double *__restrict__ CS,
+    const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr Bins BN = SH::BN;
+  constexpr int NB = SH::NB, KT = SH::KT, C = SH::C, NU = SH::NU, S = SH::S;
+  constexpr int NAU = SH::NAU;
+  __shared__ double R[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[N + 1];
+
+  const int lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
+  for (int i = lane; i < 4 * NB; i += 64) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      t1[q] = P->map1[16 * b + g + 4 * q] >> 6;
+      t2[q] = P->map2[16 * b + g + 4 * q] >> 6;
+      t3[q] = P->map3[16 * b + g + 4 * q] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = lane; u < NU; u += 64) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = 0.0;
+  const unsigned pi = blockIdx.x * 16 + col;
+  const bool pvalid = pi < npar;
+  const uint64_t p = pvalid ? parents[pi] : 0;
+  __syncthreads();
+
+  double pre_re[NAU], pre_im[NAU], acc_re[NAU], acc_im[NAU];
+  auto load = [&](const double *src, int i0, int i1) {
+#pragma unroll
+    for (int i = i0; i < i1; i++) {
+      const int u = min(grp + 4 * i, NU - 1);
+      const double *q = src + uinf[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  auto deposit = [&](int i0, int i1) {
+#pragma unroll
+    for (int i = i0; i < i1; i++) {
+      const int u = min(grp + 4 * i, NU - 1);
+      const int4 inf = uinf[u];
+      const double2 e = cs[inf.y];
+      R[inf.z * 16 + col] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + col] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
+#pragma unroll
+  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = 0.0;
+  if (MODE == 1) load(pvalid ? in + p * (uint64_t)C : zero, 0, NAU);
+
+  for (int c = 0; c < 8; c++) {
+    const uint64_t cb = 8 * p + c;
+    const bool v = pvalid && (MODE == 0 ? cend[cb] > cstart[cb] : (!cflag || cflag[cb] != 0));
+    if (!__any(v)) continue;
+    const int th = MODE == 0 ? (c >> 2) & 1 : 1 - ((c >> 2) & 1);
+    if (lane <= N) {
+      const double *t = CS + ((size_t)c * (N + 1) + lane) * 2;
+      cs[lane] = make_double2(t[0], t[1]);
+    }
+    if (MODE == 0) {
+      const double *src = v ? in + cb * (uint64_t)C : zero;
+#pragma unroll
+      for (int i0 = 0; i0 < NAU; i0 += 8) {
+        const int i1 = i0 + 8 < NAU ? i0 + 8 : NAU;
+        load(src, i0, i1);
+        deposit(i0, i1);
+      }
+    } else {
+      deposit(0, NAU);
+    }
+    d4 y[NB];
+    stage<NB, KT>(BN, A1 + (size_t)th * KT * 64, R, lane, grp, col, y);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int4 d = tmap1[4 * b + grp];
+      R[d.x * 16 + col] = y[b][0];
+      R[d.y * 16 + col] = y[b][1];
+      R[d.z * 16 + col] = y[b][2];
+      R[d.w * 16 + col] = y[b][3];
+    }
+    stage<NB, KT>(BN, A2, R, lane, grp, col, y);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int4 d = tmap2[4 * b + grp];
+      R[d.x * 16 + col] = y[b][0];
+      R[d.y * 16 + col] = y[b][1];
+      R[d.z * 16 + col] = y[b][2];
+      R[d.w * 16 + col] = y[b][3];
+    }
+    stage<NB, KT>(BN, A3 + (size_t)th * KT * 64, R, lane, grp, col, y);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int4 d = tmap3[4 * b + grp];
+      R[d.x * 16 + col] = y[b][0];
+      R[d.y * 16 + col] = y[b][1];
+      R[d.z * 16 + col] = y[b][2];
+      R[d.w * 16 + col] = y[b][3];
+    }
+    if (MODE == 0) {
+#pragma unroll
+      for (int i = 0; i < NAU; i++) {
+        const int u = min(grp + 4 * i, NU - 1);
+        const double2 e = cs[uinf[u].y];
+        const double tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
+        acc_re[i] += e.x * tr + e.y * tq;
+        acc_im[i] += e.x * tq - e.y * tr;
+      }
+    } else if (v) {
+      double *dst = out + cb * (uint64_t)C;
+#pragma unroll
+      for (int i = 0; i < NAU; i++) {
+        const int u = grp + 4 * i;
+        if (u < NU) {
+          const int4 inf = uinf[u];
+          const double2 e = cs[inf.y];
+          const double tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
+          dst[inf.x] += e.x * tr + e.y * tq;
+          if (inf.y) dst[inf.x + 1] += e.x * tq - e.y * tr;
+        }
+      }
+    }
+  }
+  if (MODE == 0 && pvalid) {
+    double *dst = out + p * (uint64_t)C;
+#pragma unroll
+    for (int i = 0; i < NAU; i++) {
+      const int u = grp + 4 * i;
+      if (u < NU) {
+        const int4 inf = uinf[u];
+        dst[inf.x] = acc_re[i];
+        if (inf.y) dst[inf.x + 1] = acc_im[i];
+      }
+    }
+  }
+}
+
 // occupancy bitmap of a level: bit b set when box b holds points
 __global__ void k_occ_bits(const uint32_t *start, const uint32_t *end,
                            uint64_t nbox, uint32_t *bits) {
@@ -409,6 +572,22 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   return 0;
 }
 
+template <int N, int MODE>
+static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *cstart,
+                        const uint32_t *cend, const uint32_t *cflag, const double *in,
+                        double *out, const void *plan, const double *A1, const double *A3,
+                        const double *A2, const double *CS, const double *zero,
+                        const wbfmm_m2l_plan_t *hplan) {
+  if (!plan_matches<N>(hplan)) {
+    fprintf(stderr, "wbfmm (HIP): plan for order %d does not match the kernel's bins\n", N);
+    return WBFMM_HIP_ERR_ARG;
+  }
+  WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma<N, MODE>),
+               dim3((npar + 15) / 16), dim3(64), 0, parents, npar, cstart, cend, cflag, in,
+               out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+  return 0;
+}
+
 // device scratch kept between calls: occupancy bits and a zero expansion
 static uint32_t *g_occ = nullptr;
 static size_t g_occ_words = 0;
@@ -447,6 +626,41 @@ int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,
     s += ccount[c];
   }
   return 0;
+}
+
+static int ensure_zero() {
+  if (!g_zero) {
+    const size_t bytes = sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1);
+    WBFMM_CHECK(hipMalloc(&g_zero, bytes));
+    WBFMM_CHECK(hipMemsetAsync(g_zero, 0, bytes, stream()));
+  }
+  return 0;
+}
+
+int wbfmm_hip_lap_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
+                             const uint32_t *cstart, const uint32_t *cend,
+                             const uint32_t *cflag, int N, const double *in, double *out,
+                             const void *plan_dev, const double *A1, const double *A3,
+                             const double *A2, const double *CS,
+                             const wbfmm_m2l_plan_t *plan_host) {
+  if (npar == 0) return 0;
+  if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
+  if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
+  int rc = ensure_zero();
+  if (rc) return rc;
+  switch (N) {
+#define CASE(n)                                                                              \
+  case n:                                                                                    \
+    return mode == 0 ? launch_shift<n, 0>(parents, npar, cstart, cend, cflag, in, out,       \
+                                          plan_dev, A1, A3, A2, CS, g_zero, plan_host)       \
+                     : launch_shift<n, 1>(parents, npar, cstart, cend, cflag, in, out,       \
+                                          plan_dev, A1, A3, A2, CS, g_zero, plan_host);
+    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+#undef CASE
+    default:
+      return WBFMM_HIP_ERR_ORDER;
+  }
 }
 
 int wbfmm_hip_mfma_layout_probe(double *host_out) {
@@ -488,9 +702,9 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
     WBFMM_CHECK(hipMalloc(&g_occ, words * sizeof(uint32_t)));
     g_occ_words = words;
   }
-  if (!g_zero) {
-    WBFMM_CHECK(hipMalloc(&g_zero, sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1)));
-    WBFMM_CHECK(hipMemsetAsync(g_zero, 0, sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1), stream()));
+  {
+    const int rz = ensure_zero();
+    if (rz) return rz;
   }
   WBFMM_LAUNCH("m2l_occ_bits", k_occ_bits, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
                sstart, send, nbox, g_occ);

@@ -88,11 +88,15 @@ typedef struct {
   void *m2l_plan_dev[WBFMM_HIP_MAX_ORDER + 1];
   double *m2l_A1[WBFMM_HIP_MAX_ORDER + 1], *m2l_A3[WBFMM_HIP_MAX_ORDER + 1],
       *m2l_CS[WBFMM_HIP_MAX_ORDER + 1];
-  /* coaxial tables of the MFMA M2L, one per (order, box width) in use */
+  /* coaxial tables of the MFMA M2L (kind 2, S|R at the 15 lengths), M2M
+     (kind 0, S|S) and L2L (kind 1, R|R), one per (order, kind, width) */
   int m2l_nA2;
-  int m2l_A2_N[M2L_A2_CACHE];
+  int m2l_A2_N[M2L_A2_CACHE], m2l_A2_kind[M2L_A2_CACHE];
   double m2l_A2_w[M2L_A2_CACHE];
   double *m2l_A2[M2L_A2_CACHE];
+  /* MFMA M2M / L2L rotation tiles and child-direction phases per order */
+  double *sh_A1[WBFMM_HIP_MAX_ORDER + 1], *sh_A3[WBFMM_HIP_MAX_ORDER + 1];
+  double *sh_CSup[WBFMM_HIP_MAX_ORDER + 1], *sh_CSdn[WBFMM_HIP_MAX_ORDER + 1];
   /* Helmholtz: wavenumber of the coaxial tables, their orders per level
      (op.L for S|R, ss_L for R|R), H tables of the 49 list-4 polar angles
      and of the two child directions (order op.Lmax), offset geometry */
@@ -672,6 +676,10 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     wbfmm_hip_free(od->m2l_A1[n]);
     wbfmm_hip_free(od->m2l_A3[n]);
     wbfmm_hip_free(od->m2l_CS[n]);
+    wbfmm_hip_free(od->sh_A1[n]);
+    wbfmm_hip_free(od->sh_A3[n]);
+    wbfmm_hip_free(od->sh_CSup[n]);
+    wbfmm_hip_free(od->sh_CSdn[n]);
     free(od->m2l_plan[n]);
   }
   for (n = 0; n < od->m2l_nA2; n++) wbfmm_hip_free(od->m2l_A2[n]);
@@ -697,34 +705,72 @@ static void ensure_m2l_plan(ops_dev *od, int N)
   od->m2l_have[N] = 1;
 }
 
-/* device coaxial table of the MFMA M2L for order N and box width w (the
-   tables depend on the tree's width; a few widths are kept) */
-static const double *m2l_coax(ops_dev *od, int N, double w)
+/* device coaxial tiles of the MFMA passes for order N: kind 2 S|R at the
+   15 list-4 lengths for box width w (M2L), kind 0 S|S and kind 1 R|R at
+   distance w (M2M, L2L); a few entries are kept */
+static const double *mfma_coax(ops_dev *od, int N, int kind, double w)
 {
   const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
-  size_t n = (size_t)P->NR * P->KT * 64;
+  size_t n = (size_t)(kind == 2 ? P->NR : 1) * P->KT * 64;
   double *h;
   int i;
   for (i = 0; i < od->m2l_nA2; i++)
-    if (od->m2l_A2_N[i] == N && od->m2l_A2_w[i] == w) return od->m2l_A2[i];
+    if (od->m2l_A2_N[i] == N && od->m2l_A2_kind[i] == kind && od->m2l_A2_w[i] == w)
+      return od->m2l_A2[i];
   if (od->m2l_nA2 == M2L_A2_CACHE) {   /* full: drop the oldest */
     wbfmm_hip_sync();
     wbfmm_hip_free(od->m2l_A2[0]);
     memmove(od->m2l_A2_N, od->m2l_A2_N + 1, sizeof(int) * (M2L_A2_CACHE - 1));
+    memmove(od->m2l_A2_kind, od->m2l_A2_kind + 1, sizeof(int) * (M2L_A2_CACHE - 1));
     memmove(od->m2l_A2_w, od->m2l_A2_w + 1, sizeof(double) * (M2L_A2_CACHE - 1));
     memmove(od->m2l_A2, od->m2l_A2 + 1, sizeof(double *) * (M2L_A2_CACHE - 1));
     od->m2l_nA2--;
   }
   h = malloc(sizeof(double) * n);
   if (!h) WBFMM_FATAL("out of host memory");
-  HIPOK(wbfmm_hip_lap_m2l_coax(P, w, h));
+  if (kind == 2) HIPOK(wbfmm_hip_lap_m2l_coax(P, w, h));
+  else HIPOK(wbfmm_hip_lap_shift_coax(P, kind, w, h));
   i = od->m2l_nA2++;
   od->m2l_A2_N[i] = N;
+  od->m2l_A2_kind[i] = kind;
   od->m2l_A2_w[i] = w;
   HIPOK(wbfmm_hip_malloc((void **)&od->m2l_A2[i], sizeof(double) * n));
   HIPOK(wbfmm_hip_memcpy(od->m2l_A2[i], h, sizeof(double) * n));
   free(h);
   return od->m2l_A2[i];
+}
+
+static const double *m2l_coax(ops_dev *od, int N, double w) { return mfma_coax(od, N, 2, w); }
+
+/* rotation tiles and phases of the MFMA M2M / L2L for order N */
+static void ensure_shift_tables(ops_dev *od, int N)
+{
+  const wbfmm_m2l_plan_t *P;
+  size_t n13, ncs;
+  double *a1, *a3, *cu, *cd;
+  ensure_m2l_plan(od, N);
+  if (od->sh_A1[N]) return;
+  P = od->m2l_plan[N];
+  n13 = (size_t)2 * P->KT * 64;
+  ncs = (size_t)8 * (N + 1) * 2;
+  a1 = malloc(sizeof(double) * n13);
+  a3 = malloc(sizeof(double) * n13);
+  cu = malloc(sizeof(double) * ncs);
+  cd = malloc(sizeof(double) * ncs);
+  if (!a1 || !a3 || !cu || !cd) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_lap_shift_tables(P, a1, a3, cu, cd));
+  HIPOK(wbfmm_hip_malloc((void **)&od->sh_A1[N], sizeof(double) * n13));
+  HIPOK(wbfmm_hip_memcpy(od->sh_A1[N], a1, sizeof(double) * n13));
+  HIPOK(wbfmm_hip_malloc((void **)&od->sh_A3[N], sizeof(double) * n13));
+  HIPOK(wbfmm_hip_memcpy(od->sh_A3[N], a3, sizeof(double) * n13));
+  HIPOK(wbfmm_hip_malloc((void **)&od->sh_CSup[N], sizeof(double) * ncs));
+  HIPOK(wbfmm_hip_memcpy(od->sh_CSup[N], cu, sizeof(double) * ncs));
+  HIPOK(wbfmm_hip_malloc((void **)&od->sh_CSdn[N], sizeof(double) * ncs));
+  HIPOK(wbfmm_hip_memcpy(od->sh_CSdn[N], cd, sizeof(double) * ncs));
+  free(a1);
+  free(a3);
+  free(cu);
+  free(cd);
 }
 
 gint wbfmm_laplace_coaxial_translate_init(gint N)
@@ -750,6 +796,19 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if ((int)t->order_s[level] > od->lay.L || (int)t->order_s[level - 1] > od->lay.L)
     WBFMM_FATAL("operators built to order %d, tree needs %u", od->lay.L,
                 t->order_s[level] > t->order_s[level - 1] ? t->order_s[level] : t->order_s[level - 1]);
+  if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
+      (int)t->order_s[level] <= M2L_MFMA_MAX) {
+    /* matrix-core form: 16 parents per wavefront */
+    int N = (int)t->order_s[level];
+    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << level);
+    ensure_shift_tables(od, N);
+    HIPOK(wbfmm_hip_lap_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                   td->d_start[level], td->d_end[level], NULL, N,
+                                   (double *)t->mps[level], (double *)t->mps[level - 1],
+                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
+                                   mfma_coax(od, N, 0, tt), od->sh_CSup[N], od->m2l_plan[N]));
+    return 0;
+  }
   ensure_lap_coax(od);
   HIPOK(wbfmm_hip_lap_m2m(td->d_occ[level - 1], td->nocc[level - 1],
                           td->d_start[level], td->d_end[level], level, t->D,
@@ -826,6 +885,19 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if (level == t->depth) return 0;
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
+  if (t->order_r[level] == t->order_r[level + 1] && (int)t->order_r[level] >= M2L_MFMA_MIN &&
+      (int)t->order_r[level] <= M2L_MFMA_MAX) {
+    int N = (int)t->order_r[level];
+    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << (level + 1));
+    ensure_shift_tables(od, N);
+    HIPOK(wbfmm_hip_lap_shift_mfma(1, tl, nt, NULL, NULL,
+                                   td->have_targets ? td->d_flag[level + 1] : NULL, N,
+                                   (double *)t->mpr[level], (double *)t->mpr[level + 1],
+                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
+                                   mfma_coax(od, N, 1, tt), od->sh_CSdn[N], od->m2l_plan[N]));
+    return 0;
+  }
+  ensure_lap_coax(od);
   HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
                           level, t->D,
                           t->order_r[level], t->order_r[level + 1],

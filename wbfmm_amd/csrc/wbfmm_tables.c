@@ -378,12 +378,51 @@ static void put_tiles(double *dst, const double *T, const wbfmm_m2l_plan_t *P)
       }
 }
 
+/* stage-1 and stage-3 tiles of one polar angle (H: its H table to order
+   N); T: scratch of (16 NB)^2 doubles */
+static void theta_tiles(const wbfmm_m2l_plan_t *P, const packing *p13,
+                        const double *H, double *A1, double *A3, double *T)
+{
+  int N = P->N, S = 16 * P->NB, n, nu, m, part;
+  size_t nr = (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3;
+  double *R = malloc(sizeof(double) * nr);
+  if (A1) {
+    real_blocks(R, H, N, -0.5 * M_PI, 0.0);
+    memset(T, 0, sizeof(double) * (size_t)S * S);
+    for (n = 0; n <= N; n++) {
+      const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+      int w = 2 * n + 1;
+      for (part = 0; part < 2; part++)
+        for (nu = part; nu <= n; nu++)
+          for (m = part; m <= n; m++)
+            T[(size_t)slot13(p13, N, n, nu, part) * S + slot13(p13, N, n, m, part)] =
+                B[h1_row(nu, part) * w + h1_col(m, part)];
+    }
+    put_tiles(A1, T, P);
+  }
+  if (A3) {
+    real_blocks(R, H, N, 0.0, -0.5 * M_PI);
+    memset(T, 0, sizeof(double) * (size_t)S * S);
+    for (n = 0; n <= N; n++) {
+      const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
+      int w = 2 * n + 1;
+      for (part = 0; part < 2; part++)
+        for (nu = part; nu <= n; nu++)
+          for (m = part; m <= n; m++)
+            T[(size_t)slot13(p13, N, n, nu, part) * S + slot13(p13, N, n, m, part)] =
+                B[h3_row(nu, part) * w + h3_col(m, part)];
+    }
+    put_tiles(A3, T, P);
+  }
+  free(R);
+}
+
 int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
                              double *A3, double *CS)
 {
   packing p13, p2;
   int b, n, nu, m, part, i, dx, dy, dz, S;
-  double *T, *R, *Rb, ths[343];
+  double *T, ths[343];
   hcache hc;
   /* every block must fit one 16-row bin: cos block of degree N has N+1 rows */
   if (N < 1 || N > 15) return WBFMM_HIP_ERR_ORDER;
@@ -455,46 +494,14 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
   T = calloc((size_t)S * S, sizeof(double));
   if (!T) return WBFMM_HIP_ERR_NOMEM;
   if (A1 || A3) {
-    size_t per = (size_t)P->KT * 64, nr = (size_t)(N + 1) * (2 * N + 1) * (2 * N + 3) / 3;
+    size_t per = (size_t)P->KT * 64;
     int k;
-    R = malloc(sizeof(double) * nr);
-    Rb = malloc(sizeof(double) * nr);
     hc.n = 0;
     hc.L = N;
-    for (k = 0; k < P->NTH; k++) {
-      const double *H = cached_H(&hc, ths[k]);
-      real_blocks(R, H, N, -0.5 * M_PI, 0.0);
-      real_blocks(Rb, H, N, 0.0, -0.5 * M_PI);
-      if (A1) {
-        memset(T, 0, sizeof(double) * (size_t)S * S);
-        for (n = 0; n <= N; n++) {
-          const double *B = R + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
-          int w = 2 * n + 1;
-          for (part = 0; part < 2; part++)
-            for (nu = part; nu <= n; nu++)
-              for (m = part; m <= n; m++)
-                T[(size_t)slot13(&p13, N, n, nu, part) * S + slot13(&p13, N, n, m, part)] =
-                    B[h1_row(nu, part) * w + h1_col(m, part)];
-        }
-        put_tiles(A1 + k * per, T, P);
-      }
-      if (A3) {
-        memset(T, 0, sizeof(double) * (size_t)S * S);
-        for (n = 0; n <= N; n++) {
-          const double *B = Rb + (size_t)n * (2 * n - 1) * (2 * n + 1) / 3;
-          int w = 2 * n + 1;
-          for (part = 0; part < 2; part++)
-            for (nu = part; nu <= n; nu++)
-              for (m = part; m <= n; m++)
-                T[(size_t)slot13(&p13, N, n, nu, part) * S + slot13(&p13, N, n, m, part)] =
-                    B[h3_row(nu, part) * w + h3_col(m, part)];
-        }
-        put_tiles(A3 + k * per, T, P);
-      }
-    }
+    for (k = 0; k < P->NTH; k++)
+      theta_tiles(P, &p13, cached_H(&hc, ths[k]), A1 ? A1 + k * per : NULL,
+                  A3 ? A3 + k * per : NULL, T);
     for (i = 0; i < hc.n; i++) free(hc.H[i]);
-    free(R);
-    free(Rb);
   }
   free(T);
   return 0;
@@ -711,5 +718,73 @@ int wbfmm_hip_hel_coaxial_table(int kind, int L, double kr, double *E)
   free(ei);
   free(jq);
   free(yq);
+  return 0;
+}
+
+/* ------------------------------------------------------------------
+ * MFMA form of the Laplace M2M and L2L (src/shift-laplace.c:225-789):
+ * the same split and bins as the M2L plan; the 8 child directions use the
+ * polar angles acos(1/sqrt 3) (index 0) and acos(-1/sqrt 3) (index 1).
+ *   A1, A3: 2 x KT x 64 tiles of the two angles;
+ *   CSup, CSdn: 8 x (N+1) x 2, cos and sin of m chi of the child -> parent
+ *     (M2M) and parent -> child (L2L) directions of child c.
+ * ------------------------------------------------------------------ */
+int wbfmm_hip_lap_shift_tables(const wbfmm_m2l_plan_t *P, double *A1, double *A3,
+                               double *CSup, double *CSdn)
+{
+  packing p13;
+  hcache hc;
+  double *T, th[2];
+  int N = P->N, S = 16 * P->NB, c, m, k;
+  size_t per = (size_t)P->KT * 64;
+  do_pack(N, 0, &p13);
+  T = calloc((size_t)S * S, sizeof(double));
+  if (!T) return WBFMM_HIP_ERR_NOMEM;
+  th[0] = acos(1.0 / sqrt(3.0));
+  th[1] = acos(-1.0 / sqrt(3.0));
+  hc.n = 0;
+  hc.L = N;
+  for (k = 0; k < 2; k++)
+    theta_tiles(P, &p13, cached_H(&hc, th[k]), A1 + k * per, A3 + k * per, T);
+  for (k = 0; k < hc.n; k++) free(hc.H[k]);
+  free(T);
+  for (c = 0; c < 8; c++) {
+    double ex = 0.5 - (c & 1), ey = 0.5 - ((c >> 1) & 1);
+    double cu = atan2(ey, ex), cd = atan2(-ey, -ex);
+    for (m = 0; m <= N; m++) {
+      CSup[((size_t)c * (N + 1) + m) * 2] = cos(m * cu);
+      CSup[((size_t)c * (N + 1) + m) * 2 + 1] = sin(m * cu);
+      CSdn[((size_t)c * (N + 1) + m) * 2] = cos(m * cd);
+      CSdn[((size_t)c * (N + 1) + m) * 2 + 1] = sin(m * cd);
+    }
+  }
+  return 0;
+}
+
+/* stage 2 of the M2M (kind 0: S|S, c (-t)^(n-nu), nu <= n) or L2L (kind 1:
+   R|R, c t^(nu-n), nu >= n) at distance t = sqrt(3)/2 w_child, same block
+   for both parts (src/laplace.c:542-640) */
+int wbfmm_hip_lap_shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, double *A2)
+{
+  packing p2;
+  int N = P->N, S = 16 * P->NB, m, n, nu, part;
+  double *T;
+  if (kind != 0 && kind != 1) return WBFMM_HIP_ERR_ARG;
+  do_pack(N, 1, &p2);
+  T = calloc((size_t)S * S, sizeof(double));
+  if (!T) return WBFMM_HIP_ERR_NOMEM;
+  for (m = 0; m <= N; m++)
+    for (part = 0; part < (m ? 2 : 1); part++)
+      for (n = m; n <= N; n++)
+        for (nu = m; nu <= N; nu++) {
+          double v;
+          if (kind == 0 && nu > n) continue;
+          if (kind == 1 && nu < n) continue;
+          v = kind == 0 ? lap_cfft(0, n, nu, m) * pow(-t, (double)(n - nu))
+                        : lap_cfft(1, n, nu, m) * pow(t, (double)(nu - n));
+          T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] = v;
+        }
+  put_tiles(A2, T, P);
+  free(T);
   return 0;
 }
