@@ -245,6 +245,202 @@ def run_helmholtz(args):
     print(json.dumps(out))
 
 
+def c2(z):
+    return np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], 1))
+
+
+def mixed_problem(seed, n, helm, sphere, k):
+    """one problem of the batched / serving workloads: n points uniform in
+    [-1,1]^3 or on the unit sphere, N(0,1) strengths (complex for
+    Helmholtz), targets = sources"""
+    xs = sphere_points(seed, n) if sphere else 2.0 * uniform(seed, 3 * n).reshape(n, 3) - 1.0
+    q = c2(normal(seed + 1, n) + 1j * normal(seed + 2, n)) if helm else normal(seed + 1, n)
+    return np.ascontiguousarray(xs), q
+
+
+def batched_problems(count=256, seed=6):
+    """256 problems: N = floor(10^U(3,5)), Laplace and Helmholtz alternating
+    (k ~ U(1,20)), cube or sphere with probability 1/2, depth
+    max(2, ceil(log8(N/20))), orders from tol 1e-6"""
+    u = uniform(seed, 4 * count).reshape(count, 4)
+    out = []
+    for i in range(count):
+        n = int(10.0 ** (3.0 + 2.0 * u[i, 0]))
+        helm = i % 2 == 1
+        sphere = u[i, 1] < 0.5
+        k = 1.0 + 19.0 * u[i, 2] if helm else 0.0
+        depth = max(2, int(np.ceil(np.log(n / 20.0) / np.log(8.0))))
+        xs, q = mixed_problem(seed * 1000003 + 17 * i, n, helm, sphere, k)
+        out.append(dict(n=n, helm=helm, sphere=sphere, k=k, depth=depth, tol=1e-6, xs=xs, q=q))
+    return out
+
+
+def serving_requests(count, qps, seed=7):
+    """Poisson arrivals at qps: N = floor(10^U(3,6)), Laplace or Helmholtz
+    (k ~ U(1,32)) with probability 1/2, cube or sphere, accuracy 1e-4 or
+    1e-8 with probability 1/2"""
+    u = uniform(seed, 6 * count).reshape(count, 6)
+    t, out = 0.0, []
+    for i in range(count):
+        n = int(10.0 ** (3.0 + 3.0 * u[i, 0]))
+        helm = u[i, 1] < 0.5
+        k = 1.0 + 31.0 * u[i, 2] if helm else 0.0
+        tol = 1e-4 if u[i, 3] < 0.5 else 1e-8
+        sphere = u[i, 4] < 0.5
+        t += -np.log1p(-u[i, 5]) / qps
+        xs, q = mixed_problem(seed * 1000003 + 17 * i, n, helm, sphere, k)
+        out.append(dict(n=n, helm=helm, sphere=sphere, k=k, tol=tol, t=t, xs=xs, q=q))
+    return out
+
+
+def _upload(W, probs):
+    for p in probs:
+        p["d_xs"] = W.DeviceBuffer.from_array(p["xs"])
+        p["d_q"] = W.DeviceBuffer.from_array(p["q"])
+        p["d_f"] = W.DeviceBuffer(p["n"] * 8 * 8)
+    W.sync()
+
+
+def _request(S, p, field=3, depth=0, check=0):
+    fw = 8 if p["helm"] else 4
+    return S.make_request(2 if p["helm"] else 1, p["d_xs"], q=p["d_q"], f=p["d_f"], fstr=fw,
+                          k=p["k"], tol=p["tol"], field=field, depth=depth, check=check,
+                          nsrc=p["n"])
+
+
+def _release(probs):
+    for p in probs:
+        for key in ("d_xs", "d_q", "d_f"):
+            p[key].free()
+
+
+def run_batched(args):
+    """Config of many independent problems solved as one batch through the
+    serving engine: a step is the whole batch (every problem submitted at
+    once, the engine runs args.streams of them at a time)."""
+    import wbfmm_amd as W
+    import wbfmm_amd.serve as S
+    probs = batched_problems(args.batch)
+    _upload(W, probs)
+    ntot = sum(p["n"] for p in probs)
+    print(f"bench: {len(probs)} problems ready, {ntot} targets", file=sys.stderr, flush=True)
+    srv = S.Server(args.streams)
+
+    def step(check):
+        tk = [srv.submit(_request(S, p, depth=p["depth"], check=check)) for p in probs]
+        return [srv.wait(t) for t in tk]
+
+    for _ in range(args.warmup):
+        step(0)
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    srv.reset_stats()
+    t0 = time.perf_counter()
+    res = []
+    for s_ in range(args.steps):
+        res = step(args.check_targets if s_ == args.steps - 1 else 0)
+    total = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    _release(probs)
+    bad = [r.status for r in res if r.status]
+    ep = np.array([r.err_potential for r in res])
+    eg = np.array([r.err_gradient for r in res])
+    solve = np.array([r.solve_ms for r in res])
+    tables = np.array([r.tables_ms for r in res])
+    out = {
+        "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
+        "value": ntot * args.steps / total, "unit": "targets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps,
+        "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 6)",
+        "config": {"workload": f"batched: {len(probs)} problems, N = 10^U(3,5), Laplace / "
+                               "Helmholtz alternating (k ~ U(1,20)), cube or sphere, depth "
+                               "max(2, ceil(log8(N/20))), orders from tol 1e-6, targets = "
+                               "sources, potential+gradient; one step = the whole batch",
+                   "problems": len(probs), "targets_per_step": ntot, "streams": args.streams,
+                   "parallelism": "single GPU"},
+        "latency_note": "p50/p99 over problems: submission to field written, all problems "
+                        "submitted at once",
+        "solve_ms_p50": float(np.percentile(solve, 50)),
+        "solve_ms_p99": float(np.percentile(solve, 99)),
+        "tables_ms_total": float(tables.sum()),
+        "error_vs_fp64_direct": {"targets_checked_per_problem": args.check_targets,
+                                 "rel_l2_potential_median": float(np.median(ep)),
+                                 "rel_l2_potential_max": float(ep.max()),
+                                 "rel_l2_gradient_median": float(np.median(eg)),
+                                 "rel_l2_gradient_max": float(eg.max()),
+                                 "problems_potential_le_1e-6": int((ep <= 1e-6).sum())},
+        "failed": len(bad),
+    }
+    print(json.dumps(out))
+
+
+def run_serving(args):
+    """Open-loop serving: requests arrive as a Poisson stream at args.qps
+    and are submitted at their arrival times; latency is submission to
+    field written."""
+    import threading
+    import wbfmm_amd as W
+    import wbfmm_amd.serve as S
+    reqs = serving_requests(args.requests, args.qps)
+    _upload(W, reqs)
+    print(f"bench: {len(reqs)} requests ready", file=sys.stderr, flush=True)
+    srv = S.Server(args.streams)
+    # warm the workers' Laplace tables and the kernels with small requests
+    warm = [p for p in reqs if p["n"] < 20000][:2 * args.streams]
+    for t in [srv.submit(_request(S, p)) for p in warm]:
+        srv.wait(t)
+    srv.reset_stats()
+    tickets = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feeder():
+        for i, p in enumerate(reqs):
+            delay = t0 + p["t"] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(_request(S, p, check=args.check_targets))
+
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    res = [srv.wait(t) for t in tickets]
+    total = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    _release(reqs)
+    ntot = sum(p["n"] for p in reqs)
+    lat = np.array([r.latency_ms for r in res])
+    per_tol = {}
+    for tol in (1e-4, 1e-8):
+        sel = [i for i, p in enumerate(reqs) if p["tol"] == tol]
+        if sel:
+            e = np.array([max(res[i].err_potential, 0.0) for i in sel])
+            per_tol[str(tol)] = {"requests": len(sel), "rel_l2_potential_max": float(e.max()),
+                                 "rel_l2_potential_median": float(np.median(e)),
+                                 "latency_p50_ms": float(np.percentile(lat[sel], 50)),
+                                 "latency_p99_ms": float(np.percentile(lat[sel], 99))}
+    out = {
+        "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
+        "value": ntot / total, "unit": "targets/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * total, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64 seed 7)",
+        "config": {"workload": f"serving: {len(reqs)} requests, Poisson arrivals at "
+                               f"{args.qps} req/s, N = 10^U(3,6), Laplace or Helmholtz (k ~ "
+                               "U(1,32)), cube or sphere, accuracy 1e-4 or 1e-8 (FP64 both), "
+                               "targets = sources, potential+gradient",
+                   "requests": len(reqs), "qps": args.qps, "streams": args.streams,
+                   "parallelism": "single GPU"},
+        "latency_ms": {"p50": st.p50_ms, "p99": st.p99_ms, "mean": st.mean_ms, "max": st.max_ms},
+        "by_accuracy": per_tol,
+        "tables_ms_total": float(sum(r.tables_ms for r in res)),
+        "failed": sum(1 for r in res if r.status),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,9 +452,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-targets", type=int, default=2000)
     ap.add_argument("--workload", default="laplace",
-                    choices=["laplace", "helmholtz-cube", "helmholtz-sphere"],
+                    choices=["laplace", "helmholtz-cube", "helmholtz-sphere", "batched",
+                             "serving"],
                     help="laplace: the headline configuration (default); helmholtz-cube / "
-                         "helmholtz-sphere: the Helmholtz configurations")
+                         "helmholtz-sphere: the Helmholtz configurations; batched: 256 "
+                         "independent problems per step; serving: Poisson request stream")
+    ap.add_argument("--streams", type=int, default=8, help="serving engine streams")
+    ap.add_argument("--batch", type=int, default=256, help="problems per batched step")
+    ap.add_argument("--requests", type=int, default=48, help="serving requests")
+    ap.add_argument("--qps", type=float, default=4.0, help="serving arrival rate")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with a copy kernel of known size (wbfmm_hip_stage4 over 8Mi rows: "
                          "192 MiB read, 256 MiB written) to scale HBM counters")
@@ -267,6 +469,12 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        run_batched(args)
+        return
+    if args.workload == "serving":
+        run_serving(args)
+        return
     if args.workload != "laplace":
         run_helmholtz(args)
         return

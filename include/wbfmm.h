@@ -42,6 +42,7 @@ typedef void *gpointer;
 typedef size_t gsize;
 typedef uint32_t guint32;
 typedef uint64_t guint64;
+typedef int64_t gint64;
 #ifndef TRUE
 #define TRUE 1
 #endif
@@ -369,6 +370,10 @@ gint wbfmm_hip_synchronize(void);
 gint wbfmm_tree_free(wbfmm_tree_t *t);
 gint wbfmm_target_list_free(wbfmm_target_list_t *l);
 gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op);
+/* drop the Helmholtz coaxial tables (and their wavenumber) of an operator
+   set, keeping its rotation tables, so the set serves another wavenumber
+   after new coaxial_SR_init / _SS_init calls */
+gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *op);
 
 #ifdef __cplusplus
 }

@@ -32,8 +32,22 @@ extern "C" {
 
 /* ---- runtime ---------------------------------------------------------- */
 int wbfmm_hip_init(void);
+/* device memory from the default pool, ordered on the calling thread's
+   stream (no device-wide wait); free returns it when the stream gets
+   there */
 int wbfmm_hip_malloc(void **p, size_t bytes);
 int wbfmm_hip_free(void *p);
+/* streams: every call of a thread runs on the library stream unless the
+   thread selected one of its own with wbfmm_hip_set_stream (NULL selects
+   the library stream again; *prev, when not NULL, gets the old choice).
+   Objects (trees, target lists, operators) are used by one stream at a
+   time; separate objects may run concurrently on separate streams. */
+int wbfmm_hip_stream_create(void **s);
+int wbfmm_hip_stream_destroy(void *s);
+int wbfmm_hip_set_stream(void *s, void **prev);
+void *wbfmm_hip_get_stream(void);
+/* free and total device memory */
+int wbfmm_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* copy of any kind (host/device either side), ordered on the stream;
    returns after the copy when a host buffer is involved */
 int wbfmm_hip_memcpy(void *dst, const void *src, size_t bytes);
@@ -77,6 +91,13 @@ int wbfmm_hip_box_ranges(const uint64_t *codes, unsigned n, unsigned level,
 int wbfmm_hip_compact_boxes(const uint32_t *start, const uint32_t *end,
                             const uint32_t *flag, unsigned level,
                             uint32_t *list, unsigned *count);
+/* coordinate bounds of n points (host or device, byte stride pstr) into
+   host lo[3], hi[3]; returns after the reduction */
+int wbfmm_hip_bounds(const void *pts, size_t pstr, unsigned n, double *lo,
+                     double *hi);
+/* dst[i*dstr + c] += src[i*sstr + c], c < ncomp, i < n (device arrays) */
+int wbfmm_hip_accumulate(double *dst, int dstr, const double *src, int sstr,
+                         int ncomp, unsigned n);
 /* flag[b] |= 1 for every box at `level` holding one of the n codes */
 int wbfmm_hip_mark_boxes(const uint64_t *codes, unsigned n, unsigned level,
                          uint32_t *flag);

@@ -20,7 +20,7 @@ def declared(header):
 def test_library_exports_declared_symbols(wb):
     lib = ctypes.CDLL(wb.LIB_PATH)
     missing = []
-    for h in ("wbfmm.h", "wbfmm_hip.h"):
+    for h in ("wbfmm.h", "wbfmm_hip.h", "wbfmm_serve.h"):
         for name in sorted(declared(h)):
             try:
                 getattr(lib, name)
@@ -32,7 +32,8 @@ def test_library_exports_declared_symbols(wb):
 def test_headers_compile_as_c(tmp_path):
     import subprocess
     src = tmp_path / "t.c"
-    src.write_text('#include "wbfmm.h"\n#include "wbfmm_hip.h"\nint main(void){return 0;}\n')
+    src.write_text('#include "wbfmm.h"\n#include "wbfmm_hip.h"\n#include "wbfmm_serve.h"\n'
+                   'int main(void){return 0;}\n')
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                         "-c", str(src), "-o", str(tmp_path / "t.o")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

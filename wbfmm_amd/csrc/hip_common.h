@@ -11,6 +11,11 @@
 namespace wbfmm {
 
 hipStream_t stream();
+// stream-ordered device allocation on stream(): the memory comes from the
+// device's default pool and returns there when the stream reaches the free,
+// so neither call waits for the device
+int dmalloc(void **p, size_t bytes);
+void dfree(void *p);
 // record an error message; returns WBFMM_HIP_ERR_RUNTIME
 int fail(const char *what, hipError_t e);
 // begin/end a timed launch of kernel `name` (no-ops unless timing is on)

@@ -38,11 +38,43 @@
 #include <hip/hip_runtime.h>
 #include <rocprim/rocprim.hpp>
 
+#include <mutex>
+
 #include "hip_common.h"
 
 namespace wbfmm {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The kernels run in FP64 (v_mfma_f64_16x16x4f64) or FP32
+// (v_mfma_f32_16x16x4f32).  Both take A[l%16][l/16] and B[l/16][l%16] from
+// lane l; the four results of a lane are rows drow(l/16, q) of column l%16
+// (checked by wbfmm_hip_mfma_layout_probe and _probe_f32).
+template <typename T> struct MF;
+template <> struct MF<double> {
+  typedef d4 v4;
+  typedef double2 t2;
+  static __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static constexpr __host__ __device__ int drow(int g, int q) { return g + 4 * q; }
+};
+template <> struct MF<float> {
+  typedef f4 v4;
+  typedef float2 t2;
+  static __device__ __forceinline__ f4 mma(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static constexpr __host__ __device__ int drow(int g, int q) { return 4 * g + q; }
+};
+template <typename T>
+__device__ __forceinline__ typename MF<T>::t2 mk2(T a, T b) {
+  typename MF<T>::t2 v;
+  v.x = a;
+  v.y = b;
+  return v;
+}
 
 struct ClassArgs {
   unsigned start[8], count[8];
@@ -106,46 +138,47 @@ struct M2LShape {
 // before its product, so a product waits on a load issued long before.
 // `a` is wave-uniform (scalar base) and the lane index is added per load:
 // the loads use a scalar base plus a 32-bit lane offset.
-template <int NB, int KT>
-__device__ __forceinline__ void stage(const Bins &BN, const double *__restrict__ a,
-                                      const double *R, int lane, int grp, int col,
-                                      d4 *y) {
+template <typename T, int NB, int KT>
+__device__ __forceinline__ void stage(const Bins &BN, const T *__restrict__ a,
+                                      const T *R, int lane, int grp, int col,
+                                      typename MF<T>::v4 *y) {
   constexpr int AD = KT < 16 ? KT : 16;
-  double av[AD];
+  T av[AD];
 #pragma unroll
   for (int t = 0; t < AD; t++) av[t] = a[t * 64 + lane];
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    d4 r4 = d4{0.0, 0.0, 0.0, 0.0};
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
       const int t = BN.koff[b] + j;
-      r4 = __builtin_amdgcn_mfma_f64_16x16x4f64(
-          av[t % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4, 0, 0, 0);
+      r4 = MF<T>::mma(av[t % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4);
       if (t + AD < KT) av[t % AD] = a[(t + AD) * 64 + lane];
     }
     y[b] = r4;
   }
 }
 
-template <int N>
+template <typename T, int N>
 __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const uint32_t *__restrict__ cls, ClassArgs ca,
     const uint32_t *__restrict__ occ, int level,
-    const double *__restrict__ mps, double *__restrict__ mpr,
-    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
-    const double *__restrict__ A3, const double *__restrict__ A2,
-    const double *__restrict__ CS, const double *__restrict__ zero) {
+    const T *__restrict__ mps, T *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A1,
+    const T *__restrict__ A3, const T *__restrict__ A2,
+    const T *__restrict__ CS, const T *__restrict__ zero) {
+  typedef typename MF<T>::t2 T2;
+  typedef typename MF<T>::v4 V4;
   using SH = M2LShape<N>;
   constexpr Bins BN = SH::BN;
   constexpr int NB = SH::NB, KT = SH::KT, C = SH::C, NU = SH::NU, S = SH::S;
   constexpr int NAU = SH::NAU;
-  __shared__ double R[(S + 1) * 16];  // slot S: dummy destination
+  __shared__ T R[(S + 1) * 16];  // slot S: dummy destination
   // slot maps transposed so that one lane reads its four rows (q = 0..3)
-  // of a bin at once: tmapX[4 b + grp] = slots of rows 16 b + grp + 4 q
+  // of a bin at once: tmapX[4 b + grp] = slots of rows 16 b + drow(grp, q)
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];  // packed position of re, m, stage-1 re slot, im slot
-  __shared__ double2 cs[2][N + 1];  // cos, sin of m chi: current / next offset
+  __shared__ T2 cs[2][N + 1];  // cos, sin of m chi: current / next offset
   __shared__ unsigned vm[16][7];
   __shared__ unsigned anym[7];
 
@@ -155,9 +188,10 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const int b = i >> 2, g = i & 3;
     int t1[4], t2[4], t3[4];
     for (int q = 0; q < 4; q++) {
-      t1[q] = P->map1[16 * b + g + 4 * q] >> 6;
-      t2[q] = P->map2[16 * b + g + 4 * q] >> 6;
-      t3[q] = P->map3[16 * b + g + 4 * q] >> 6;
+      const int row = 16 * b + MF<T>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
     }
     tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
     tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
@@ -170,7 +204,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
                         (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = 0.0;
+  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = T(0);
   for (int i = lane; i < 16 * 7; i += 64) vm[i / 7][i % 7] = 0u;
 
   // block -> chunk of 16 targets; consecutive chunks stay on one XCD
@@ -222,8 +256,8 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   auto source_of = [&](int k, int p) {
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     if (lane <= N) {
-      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
-      cs[p][lane] = make_double2(t[0], t[1]);
+      const T *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = mk2<T>(t[0], t[1]);
     }
     const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
     const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
@@ -235,13 +269,13 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   // value to the same slot) and are not written out.  The im part is
   // always loaded (for m = 0 it is the next coefficient, multiplied by
   // sin 0 = 0 and written to the dummy slot).
-  double pre_re[NAU], pre_im[NAU];
+  T pre_re[NAU], pre_im[NAU];
   // units [i0, i1) of this lane
-  auto load_sources = [&](const double *src, const int4 *tu, int gp, int i0, int i1) {
+  auto load_sources = [&](const T *src, const int4 *tu, int gp, int i0, int i1) {
 #pragma unroll
     for (int i = i0; i < i1; i++) {
       const int u = min(gp + 4 * i, NU - 1);
-      const double *q = src + tu[u].x;
+      const T *q = src + tu[u].x;
       pre_re[i] = q[0];
       pre_im[i] = q[1];
     }
@@ -251,13 +285,13 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     for (int i = i0; i < i1; i++) {
       const int u = min(gp + 4 * i, NU - 1);
       const int4 inf = tu[u];
-      const double2 e = cs[p][inf.y];
+      const T2 e = cs[p][inf.y];
       R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
       R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
     }
   };
   // without the register prefetch: load and deposit in groups of 8 units
-  auto load_deposit = [&](const double *src, const int4 *tu, int gp, int cl, int p) {
+  auto load_deposit = [&](const T *src, const int4 *tu, int gp, int cl, int p) {
 #pragma unroll
     for (int i0 = 0; i0 < NAU; i0 += 8) {
       const int i1 = i0 + 8 < NAU ? i0 + 8 : NAU;
@@ -266,9 +300,9 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     }
   };
 
-  double acc_re[NAU], acc_im[NAU];
+  T acc_re[NAU], acc_im[NAU];
 #pragma unroll
-  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = 0.0;
+  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = T(0);
 
   int k = next_k(-1), p = 0;
   if (k < 216) load_deposit(source_of(k, 0), uinf, grp, col, 0);
@@ -287,8 +321,8 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const int kn = next_k(k);
 
     // stage 1: rotate into the offset's frame
-    d4 y[NB];
-    stage<NB, KT>(BN, A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    V4 y[NB];
+    stage<T, NB, KT>(BN, A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = t1[4 * b + gp];
@@ -300,7 +334,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     if (SH::PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, gp, 0, NAU);
 
     // stage 2: coaxial S|R at this offset's distance
-    stage<NB, KT>(BN, A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
+    stage<T, NB, KT>(BN, A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = t2[4 * b + gp];
@@ -311,7 +345,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     }
 
     // stage 3: rotate back (without the azimuthal factor)
-    stage<NB, KT>(BN, A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    stage<T, NB, KT>(BN, A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = t3[4 * b + gp];
@@ -325,8 +359,8 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 #pragma unroll
     for (int i = 0; i < NAU; i++) {
       const int u = min(gp + 4 * i, NU - 1);
-      const double2 e = cs[p][tu[u].y];
-      const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+      const T2 e = cs[p][tu[u].y];
+      const T tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
       acc_re[i] += e.x * tr + e.y * tq;
       acc_im[i] += e.x * tq - e.y * tr;
     }
@@ -342,7 +376,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   }
 
   if (!tvalid) return;
-  double *out = mpr + (size_t)tb * C;
+  T *out = mpr + (size_t)tb * C;
 #pragma unroll
   for (int i = 0; i < NAU; i++) {
     const int u = grp + 4 * i;
@@ -363,32 +397,35 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 // Child direction c uses polar-angle table th(c) (0: acos(1/sqrt3), 1:
 // acos(-1/sqrt3)) and the azimuthal factors CS[c].
 // ---------------------------------------------------------------------
-template <int N, int MODE>
+template <typename T, int N, int MODE>
 __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     const uint32_t *__restrict__ parents, unsigned npar,
     const uint32_t *__restrict__ cstart, const uint32_t *__restrict__ cend,
-    const uint32_t *__restrict__ cflag, const double *__restrict__ in,
-    double *__restrict__ out, const wbfmm_m2l_plan_t *__restrict__ P,
-    const double *__restrict__ A1, const double *__restrict__ A3,
-    const double *__restrict__ A2, const double *__restrict__ CS,
-    const double *__restrict__ zero) {
+    const uint32_t *__restrict__ cflag, const T *__restrict__ in,
+    T *__restrict__ out, const wbfmm_m2l_plan_t *__restrict__ P,
+    const T *__restrict__ A1, const T *__restrict__ A3,
+    const T *__restrict__ A2, const T *__restrict__ CS,
+    const T *__restrict__ zero) {
+  typedef typename MF<T>::t2 T2;
+  typedef typename MF<T>::v4 V4;
   using SH = M2LShape<N>;
   constexpr Bins BN = SH::BN;
   constexpr int NB = SH::NB, KT = SH::KT, C = SH::C, NU = SH::NU, S = SH::S;
   constexpr int NAU = SH::NAU;
-  __shared__ double R[(S + 1) * 16];
+  __shared__ T R[(S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
-  __shared__ double2 cs[N + 1];
+  __shared__ T2 cs[N + 1];
 
   const int lane = threadIdx.x, col = lane & 15, grp = lane >> 4;
   for (int i = lane; i < 4 * NB; i += 64) {
     const int b = i >> 2, g = i & 3;
     int t1[4], t2[4], t3[4];
     for (int q = 0; q < 4; q++) {
-      t1[q] = P->map1[16 * b + g + 4 * q] >> 6;
-      t2[q] = P->map2[16 * b + g + 4 * q] >> 6;
-      t3[q] = P->map3[16 * b + g + 4 * q] >> 6;
+      const int row = 16 * b + MF<T>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
     }
     tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
     tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
@@ -401,18 +438,18 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
                         (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = 0.0;
+  for (int i = lane; i < (S + 1) * 16; i += 64) R[i] = T(0);
   const unsigned pi = blockIdx.x * 16 + col;
   const bool pvalid = pi < npar;
   const uint64_t p = pvalid ? parents[pi] : 0;
   __syncthreads();
 
-  double pre_re[NAU], pre_im[NAU], acc_re[NAU], acc_im[NAU];
-  auto load = [&](const double *src, int i0, int i1) {
+  T pre_re[NAU], pre_im[NAU], acc_re[NAU], acc_im[NAU];
+  auto load = [&](const T *src, int i0, int i1) {
 #pragma unroll
     for (int i = i0; i < i1; i++) {
       const int u = min(grp + 4 * i, NU - 1);
-      const double *q = src + uinf[u].x;
+      const T *q = src + uinf[u].x;
       pre_re[i] = q[0];
       pre_im[i] = q[1];
     }
@@ -422,13 +459,13 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     for (int i = i0; i < i1; i++) {
       const int u = min(grp + 4 * i, NU - 1);
       const int4 inf = uinf[u];
-      const double2 e = cs[inf.y];
+      const T2 e = cs[inf.y];
       R[inf.z * 16 + col] = e.x * pre_re[i] - e.y * pre_im[i];
       R[inf.w * 16 + col] = e.y * pre_re[i] + e.x * pre_im[i];
     }
   };
 #pragma unroll
-  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = 0.0;
+  for (int i = 0; i < NAU; i++) acc_re[i] = acc_im[i] = T(0);
   if (MODE == 1) load(pvalid ? in + p * (uint64_t)C : zero, 0, NAU);
 
   for (int c = 0; c < 8; c++) {
@@ -437,11 +474,11 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     if (!__any(v)) continue;
     const int th = MODE == 0 ? (c >> 2) & 1 : 1 - ((c >> 2) & 1);
     if (lane <= N) {
-      const double *t = CS + ((size_t)c * (N + 1) + lane) * 2;
-      cs[lane] = make_double2(t[0], t[1]);
+      const T *t = CS + ((size_t)c * (N + 1) + lane) * 2;
+      cs[lane] = mk2<T>(t[0], t[1]);
     }
     if (MODE == 0) {
-      const double *src = v ? in + cb * (uint64_t)C : zero;
+      const T *src = v ? in + cb * (uint64_t)C : zero;
 #pragma unroll
       for (int i0 = 0; i0 < NAU; i0 += 8) {
         const int i1 = i0 + 8 < NAU ? i0 + 8 : NAU;
@@ -451,8 +488,8 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     } else {
       deposit(0, NAU);
     }
-    d4 y[NB];
-    stage<NB, KT>(BN, A1 + (size_t)th * KT * 64, R, lane, grp, col, y);
+    V4 y[NB];
+    stage<T, NB, KT>(BN, A1 + (size_t)th * KT * 64, R, lane, grp, col, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = tmap1[4 * b + grp];
@@ -461,7 +498,7 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
       R[d.z * 16 + col] = y[b][2];
       R[d.w * 16 + col] = y[b][3];
     }
-    stage<NB, KT>(BN, A2, R, lane, grp, col, y);
+    stage<T, NB, KT>(BN, A2, R, lane, grp, col, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = tmap2[4 * b + grp];
@@ -470,7 +507,7 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
       R[d.z * 16 + col] = y[b][2];
       R[d.w * 16 + col] = y[b][3];
     }
-    stage<NB, KT>(BN, A3 + (size_t)th * KT * 64, R, lane, grp, col, y);
+    stage<T, NB, KT>(BN, A3 + (size_t)th * KT * 64, R, lane, grp, col, y);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int4 d = tmap3[4 * b + grp];
@@ -483,20 +520,20 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
 #pragma unroll
       for (int i = 0; i < NAU; i++) {
         const int u = min(grp + 4 * i, NU - 1);
-        const double2 e = cs[uinf[u].y];
-        const double tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
+        const T2 e = cs[uinf[u].y];
+        const T tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
         acc_re[i] += e.x * tr + e.y * tq;
         acc_im[i] += e.x * tq - e.y * tr;
       }
     } else if (v) {
-      double *dst = out + cb * (uint64_t)C;
+      T *dst = out + cb * (uint64_t)C;
 #pragma unroll
       for (int i = 0; i < NAU; i++) {
         const int u = grp + 4 * i;
         if (u < NU) {
           const int4 inf = uinf[u];
-          const double2 e = cs[inf.y];
-          const double tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
+          const T2 e = cs[inf.y];
+          const T tr = R[u * 16 + col], tq = R[(NU + u) * 16 + col];
           dst[inf.x] += e.x * tr + e.y * tq;
           if (inf.y) dst[inf.x + 1] += e.x * tq - e.y * tr;
         }
@@ -504,7 +541,7 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
     }
   }
   if (MODE == 0 && pvalid) {
-    double *dst = out + p * (uint64_t)C;
+    T *dst = out + p * (uint64_t)C;
 #pragma unroll
     for (int i = 0; i < NAU; i++) {
       const int u = grp + 4 * i;
@@ -541,6 +578,16 @@ __global__ void k_mfma_layout_probe(double *out) {
   for (int q = 0; q < 4; q++) out[4 * l + q] = d[q];
 }
 
+// the same product in FP32 (values are small integers, exact in FP32)
+__global__ void k_mfma_layout_probe_f32(float *out) {
+  const int l = threadIdx.x;
+  const int ai = l % 16, ak = l / 16, bk = l / 16, bj = l % 16;
+  float a = 1.0f + ai + 100.0f * ak, bb = 1.0f + 3.0f * bk + 1000.0f * bj;
+  f4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bb, f4{0.0f, 0.0f, 0.0f, 0.0f},
+                                             0, 0, 0);
+  for (int q = 0; q < 4; q++) out[4 * l + q] = d[q];
+}
+
 __global__ void k_class_count(const uint32_t *list, unsigned n,
                               unsigned *counts) {
   unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -556,42 +603,132 @@ static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
   return true;
 }
 
-template <int N>
+template <typename T, int N>
 static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
-                      const uint32_t *occ, int level, const double *mps,
-                      double *mpr, const void *plan, const double *A1,
-                      const double *A3, const double *A2, const double *CS,
-                      const double *zero, const wbfmm_m2l_plan_t *hplan) {
+                      const uint32_t *occ, int level, const T *mps,
+                      T *mpr, const void *plan, const T *A1,
+                      const T *A3, const T *A2, const T *CS,
+                      const T *zero, const wbfmm_m2l_plan_t *hplan) {
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
-  WBFMM_LAUNCH("lap_m2l", k_lap_m2l_mfma<N>, dim3(nchunks), dim3(64), 0, cls,
+  WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
+               dim3(nchunks), dim3(64), 0, cls,
                ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                A1, A3, A2, CS, zero);
   return 0;
 }
 
-template <int N, int MODE>
+template <typename T, int N, int MODE>
 static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *cstart,
-                        const uint32_t *cend, const uint32_t *cflag, const double *in,
-                        double *out, const void *plan, const double *A1, const double *A3,
-                        const double *A2, const double *CS, const double *zero,
+                        const uint32_t *cend, const uint32_t *cflag, const T *in,
+                        T *out, const void *plan, const T *A1, const T *A3,
+                        const T *A2, const T *CS, const T *zero,
                         const wbfmm_m2l_plan_t *hplan) {
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
-  WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma<N, MODE>),
+  WBFMM_LAUNCH(sizeof(T) == 4 ? (MODE == 0 ? "lap_m2m_f32" : "lap_l2l_f32")
+                              : (MODE == 0 ? "lap_m2m" : "lap_l2l"),
+               (k_lap_shift_mfma<T, N, MODE>),
                dim3((npar + 15) / 16), dim3(64), 0, parents, npar, cstart, cend, cflag, in,
                out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
   return 0;
 }
 
-// device scratch kept between calls: occupancy bits and a zero expansion
-static uint32_t *g_occ = nullptr;
-static size_t g_occ_words = 0;
+// a zero expansion read by the kernels in place of absent boxes (its bytes
+// are zero for either precision); made once and only read afterwards
 static double *g_zero = nullptr;
+static std::mutex g_zero_mu;
+
+static int ensure_zero() {
+  std::lock_guard<std::mutex> lk(g_zero_mu);
+  if (!g_zero) {
+    const size_t bytes = sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1);
+    double *z = nullptr;
+    WBFMM_CHECK(hipMalloc(&z, bytes));
+    WBFMM_CHECK(hipMemsetAsync(z, 0, bytes, stream()));
+    // other streams read it as soon as it is published
+    WBFMM_CHECK(hipStreamSynchronize(stream()));
+    g_zero = z;
+  }
+  return 0;
+}
+
+template <typename T>
+static int shift_entry(int mode, const uint32_t *parents, unsigned npar,
+                       const uint32_t *cstart, const uint32_t *cend,
+                       const uint32_t *cflag, int N, const T *in, T *out,
+                       const void *plan_dev, const T *A1, const T *A3,
+                       const T *A2, const T *CS,
+                       const wbfmm_m2l_plan_t *plan_host) {
+  if (npar == 0) return 0;
+  if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
+  if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
+  int rc = ensure_zero();
+  if (rc) return rc;
+  const T *zero = (const T *)g_zero;
+  switch (N) {
+#define CASE(n)                                                                          \
+  case n:                                                                                \
+    return mode == 0 ? launch_shift<T, n, 0>(parents, npar, cstart, cend, cflag, in, out, \
+                                             plan_dev, A1, A3, A2, CS, zero, plan_host)   \
+                     : launch_shift<T, n, 1>(parents, npar, cstart, cend, cflag, in, out, \
+                                             plan_dev, A1, A3, A2, CS, zero, plan_host);
+    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+#undef CASE
+    default:
+      return WBFMM_HIP_ERR_ORDER;
+  }
+}
+
+template <typename T>
+static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
+                     const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
+                     const T *mps, T *mpr, const void *plan_dev, const T *A1, const T *A3,
+                     const T *A2, const T *CS, const wbfmm_m2l_plan_t *plan_host) {
+  ClassArgs ca;
+  unsigned nch = 0;
+  for (int c = 0; c < 8; c++) {
+    ca.start[c] = cstart[c];
+    ca.count[c] = ccount[c];
+    ca.chunk_prefix[c] = nch;
+    nch += (ccount[c] + 15) / 16;
+  }
+  ca.chunk_prefix[8] = nch;
+  if (nch == 0) return 0;
+  if (level > 20) return WBFMM_HIP_ERR_ARG;
+  if (N < 2 || N > 15) return WBFMM_HIP_ERR_ORDER;
+  const uint64_t nbox = (uint64_t)1 << (3 * level);
+  const size_t words = (size_t)((nbox + 31) / 32);
+  {
+    const int rz = ensure_zero();
+    if (rz) return rz;
+  }
+  const T *zero = (const T *)g_zero;
+  // occupancy bits of the source level, per call (stream-ordered)
+  uint32_t *occ = nullptr;
+  if (dmalloc((void **)&occ, words * sizeof(uint32_t))) return WBFMM_HIP_ERR_NOMEM;
+  WBFMM_LAUNCH("m2l_occ_bits", k_occ_bits, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+               sstart, send, nbox, occ);
+  int rc = WBFMM_HIP_ERR_ORDER;
+  switch (N) {
+#define CASE(n) \
+  case n:       \
+    rc = launch_m2l<T, n>(cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A1, A3, A2, CS, zero, plan_host); \
+    break;
+    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+#undef CASE
+    default:
+      break;
+  }
+  dfree(occ);
+  return rc;
+}
 
 }  // namespace wbfmm
 
@@ -608,8 +745,8 @@ int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,
                                        stream()));
   void *tmp = nullptr;
   unsigned *dcnt = nullptr;
-  WBFMM_CHECK(hipMalloc(&tmp, bytes ? bytes : 16));
-  WBFMM_CHECK(hipMalloc(&dcnt, 8 * sizeof(unsigned)));
+  if (dmalloc(&tmp, bytes) || dmalloc((void **)&dcnt, 8 * sizeof(unsigned)))
+    return WBFMM_HIP_ERR_NOMEM;
   WBFMM_CHECK(rocprim::radix_sort_keys(tmp, bytes, list, out, n, 0, 3,
                                        stream()));
   WBFMM_CHECK(hipMemsetAsync(dcnt, 0, 8 * sizeof(unsigned), stream()));
@@ -617,22 +754,13 @@ int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,
                dim3(256), 0, list, n, dcnt);
   WBFMM_CHECK(hipMemcpyAsync(ccount, dcnt, 8 * sizeof(unsigned),
                              hipMemcpyDeviceToHost, stream()));
+  dfree(tmp);
+  dfree(dcnt);
   WBFMM_CHECK(hipStreamSynchronize(stream()));
-  (void)hipFree(tmp);
-  (void)hipFree(dcnt);
   unsigned s = 0;
   for (int c = 0; c < 8; c++) {
     cstart[c] = s;
     s += ccount[c];
-  }
-  return 0;
-}
-
-static int ensure_zero() {
-  if (!g_zero) {
-    const size_t bytes = sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1);
-    WBFMM_CHECK(hipMalloc(&g_zero, bytes));
-    WBFMM_CHECK(hipMemsetAsync(g_zero, 0, bytes, stream()));
   }
   return 0;
 }
@@ -643,34 +771,39 @@ int wbfmm_hip_lap_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
                              const void *plan_dev, const double *A1, const double *A3,
                              const double *A2, const double *CS,
                              const wbfmm_m2l_plan_t *plan_host) {
-  if (npar == 0) return 0;
-  if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
-  if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
-  int rc = ensure_zero();
-  if (rc) return rc;
-  switch (N) {
-#define CASE(n)                                                                              \
-  case n:                                                                                    \
-    return mode == 0 ? launch_shift<n, 0>(parents, npar, cstart, cend, cflag, in, out,       \
-                                          plan_dev, A1, A3, A2, CS, g_zero, plan_host)       \
-                     : launch_shift<n, 1>(parents, npar, cstart, cend, cflag, in, out,       \
-                                          plan_dev, A1, A3, A2, CS, g_zero, plan_host);
-    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
-#undef CASE
-    default:
-      return WBFMM_HIP_ERR_ORDER;
-  }
+  return shift_entry<double>(mode, parents, npar, cstart, cend, cflag, N, in, out, plan_dev,
+                             A1, A3, A2, CS, plan_host);
+}
+
+int wbfmm_hip_lap_shift_mfma_f32(int mode, const uint32_t *parents, unsigned npar,
+                                 const uint32_t *cstart, const uint32_t *cend,
+                                 const uint32_t *cflag, int N, const float *in, float *out,
+                                 const void *plan_dev, const float *A1, const float *A3,
+                                 const float *A2, const float *CS,
+                                 const wbfmm_m2l_plan_t *plan_host) {
+  return shift_entry<float>(mode, parents, npar, cstart, cend, cflag, N, in, out, plan_dev,
+                            A1, A3, A2, CS, plan_host);
 }
 
 int wbfmm_hip_mfma_layout_probe(double *host_out) {
   double *d = nullptr;
-  WBFMM_CHECK(hipMalloc(&d, 256 * sizeof(double)));
+  if (dmalloc((void **)&d, 256 * sizeof(double))) return WBFMM_HIP_ERR_NOMEM;
   WBFMM_LAUNCH("mfma_probe", k_mfma_layout_probe, dim3(1), dim3(64), 0, d);
   WBFMM_CHECK(hipMemcpyAsync(host_out, d, 256 * sizeof(double),
                              hipMemcpyDeviceToHost, stream()));
+  dfree(d);
   WBFMM_CHECK(hipStreamSynchronize(stream()));
-  (void)hipFree(d);
+  return 0;
+}
+
+int wbfmm_hip_mfma_layout_probe_f32(float *host_out) {
+  float *d = nullptr;
+  if (dmalloc((void **)&d, 256 * sizeof(float))) return WBFMM_HIP_ERR_NOMEM;
+  WBFMM_LAUNCH("mfma_probe_f32", k_mfma_layout_probe_f32, dim3(1), dim3(64), 0, d);
+  WBFMM_CHECK(hipMemcpyAsync(host_out, d, 256 * sizeof(float),
+                             hipMemcpyDeviceToHost, stream()));
+  dfree(d);
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -682,42 +815,19 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const double *A3, const double *A2,
                            const double *CS,
                            const wbfmm_m2l_plan_t *plan_host) {
-  ClassArgs ca;
-  unsigned nch = 0;
-  for (int c = 0; c < 8; c++) {
-    ca.start[c] = cstart[c];
-    ca.count[c] = ccount[c];
-    ca.chunk_prefix[c] = nch;
-    nch += (ccount[c] + 15) / 16;
-  }
-  ca.chunk_prefix[8] = nch;
-  if (nch == 0) return 0;
-  if (level > 20) return WBFMM_HIP_ERR_ARG;
   (void)D;
-  const uint64_t nbox = (uint64_t)1 << (3 * level);
-  const size_t words = (size_t)((nbox + 31) / 32);
-  if (words > g_occ_words) {
-    if (g_occ) (void)hipFree(g_occ);
-    g_occ = nullptr;
-    WBFMM_CHECK(hipMalloc(&g_occ, words * sizeof(uint32_t)));
-    g_occ_words = words;
-  }
-  {
-    const int rz = ensure_zero();
-    if (rz) return rz;
-  }
-  WBFMM_LAUNCH("m2l_occ_bits", k_occ_bits, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
-               sstart, send, nbox, g_occ);
-  switch (N) {
-#define CASE(n) \
-  case n:       \
-    return launch_m2l<n>(cls, ca, nch, g_occ, (int)level, mps, mpr, plan_dev, A1, A3, A2, CS, g_zero, plan_host);
-    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
-#undef CASE
-    default:
-      return WBFMM_HIP_ERR_ORDER;
-  }
+  return m2l_entry<double>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev, A1,
+                           A3, A2, CS, plan_host);
+}
+
+int wbfmm_hip_lap_m2l_mfma_f32(const uint32_t *cls, const unsigned *cstart,
+                               const unsigned *ccount, const uint32_t *sstart,
+                               const uint32_t *send, unsigned level, int N,
+                               const float *mps, float *mpr, const void *plan_dev,
+                               const float *A1, const float *A3, const float *A2,
+                               const float *CS, const wbfmm_m2l_plan_t *plan_host) {
+  return m2l_entry<float>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev, A1,
+                          A3, A2, CS, plan_host);
 }
 
 }  // extern "C"

@@ -22,10 +22,16 @@
 
 namespace wbfmm {
 
+// The library stream, created by wbfmm_hip_init.  A thread may select
+// another stream of its own (wbfmm_hip_set_stream); every launch, copy and
+// stream-ordered allocation of that thread then goes there, which is how
+// the serving engine runs independent solves side by side.
 static hipStream_t g_stream = nullptr;
+static thread_local hipStream_t t_stream = nullptr;
 static std::mutex g_mu;
-static char g_err[512] = "";
+static thread_local char g_err[512] = "";
 static bool g_timing = false;
+static std::mutex g_timing_mu;
 struct Ev {
   std::string name;
   hipEvent_t a, b;
@@ -34,6 +40,7 @@ static std::vector<Ev> g_events;
 static std::map<std::string, std::pair<int, double>> g_totals;
 
 hipStream_t stream() {
+  if (t_stream) return t_stream;
   if (!g_stream) wbfmm_hip_init();
   return g_stream;
 }
@@ -44,8 +51,11 @@ int fail(const char *what, hipError_t e) {
   return WBFMM_HIP_ERR_RUNTIME;
 }
 
+// Timing covers launches made on the library stream only; launches on a
+// thread's own stream are not timed.
 void timing_begin(const char *name) {
-  if (!g_timing) return;
+  if (!g_timing || t_stream) return;
+  std::lock_guard<std::mutex> lk(g_timing_mu);
   Ev ev;
   ev.name = name;
   hipEventCreate(&ev.a);
@@ -55,12 +65,14 @@ void timing_begin(const char *name) {
 }
 
 void timing_end(const char *name) {
-  if (!g_timing) return;
+  if (!g_timing || t_stream) return;
   (void)name;
+  std::lock_guard<std::mutex> lk(g_timing_mu);
   hipEventRecord(g_events.back().b, g_stream);
 }
 
 static void timing_collect() {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
   if (g_events.empty()) return;
   hipStreamSynchronize(g_stream);
   for (auto &ev : g_events) {
@@ -160,21 +172,65 @@ __global__ void k_mark_boxes(const uint64_t *codes, unsigned n, int shift,
   flag[codes[i] >> shift] = 1u;
 }
 
+// per-block minimum and maximum of the 3 coordinates: out[6 b + 0..2] min,
+// out[6 b + 3..5] max
+__global__ void k_bounds(const char *pts, size_t pstr, unsigned n,
+                         double *out) {
+  __shared__ double red[6][256];
+  double lo[3] = {INFINITY, INFINITY, INFINITY};
+  double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const double *x = (const double *)(pts + (size_t)i * pstr);
+    for (int c = 0; c < 3; c++) {
+      lo[c] = fmin(lo[c], x[c]);
+      hi[c] = fmax(hi[c], x[c]);
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    red[c][threadIdx.x] = lo[c];
+    red[3 + c][threadIdx.x] = hi[c];
+  }
+  __syncthreads();
+  for (unsigned h = blockDim.x / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h)
+      for (int c = 0; c < 3; c++) {
+        red[c][threadIdx.x] = fmin(red[c][threadIdx.x], red[c][threadIdx.x + h]);
+        red[3 + c][threadIdx.x] =
+            fmax(red[3 + c][threadIdx.x], red[3 + c][threadIdx.x + h]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_accumulate(double *dst, int dstr, const double *src,
+                             int sstr, int ncomp, unsigned n) {
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < ncomp; c++)
+    dst[(size_t)i * dstr + c] += src[(size_t)i * sstr + c];
+}
+
 static inline unsigned nblk(size_t n, unsigned b) {
   return (unsigned)((n + b - 1) / b);
 }
 
-// scratch buffer reused by the sort and scan calls (grown on demand)
-static void *g_tmp = nullptr;
-static size_t g_tmp_bytes = 0;
-static int ensure_tmp(size_t bytes) {
-  if (bytes <= g_tmp_bytes) return 0;
-  if (g_tmp) hipFree(g_tmp);
-  g_tmp = nullptr;
-  g_tmp_bytes = 0;
-  WBFMM_CHECK(hipMalloc(&g_tmp, bytes));
-  g_tmp_bytes = bytes;
+
+int dmalloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMallocAsync(p, bytes, stream());
+  if (e != hipSuccess) {
+    *p = nullptr;
+    fail("hipMallocAsync", e);
+    return WBFMM_HIP_ERR_NOMEM;
+  }
   return 0;
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFreeAsync(p, stream());
 }
 
 }  // namespace wbfmm
@@ -194,29 +250,58 @@ int wbfmm_hip_init(void) {
     fprintf(stderr, "wbfmm (HIP): %s\n", g_err);
     return WBFMM_HIP_ERR_RUNTIME;
   }
+  // freed blocks stay in the default pool instead of going back to the
+  // driver, so the per-solve allocations are served from the pool
+  int dev = 0;
+  WBFMM_CHECK(hipGetDevice(&dev));
+  hipMemPool_t pool;
+  WBFMM_CHECK(hipDeviceGetDefaultMemPool(&pool, dev));
+  uint64_t keep = UINT64_MAX;
+  WBFMM_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
   // default priority, non-blocking with respect to the null stream
   WBFMM_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   return 0;
 }
 
+int wbfmm_hip_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  if (wbfmm_hip_init()) return WBFMM_HIP_ERR_RUNTIME;
+  WBFMM_CHECK(hipMemGetInfo(free_bytes, total_bytes));
+  return 0;
+}
+
+int wbfmm_hip_stream_create(void **s) {
+  if (wbfmm_hip_init()) return WBFMM_HIP_ERR_RUNTIME;
+  hipStream_t h = nullptr;
+  WBFMM_CHECK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+  *s = (void *)h;
+  return 0;
+}
+
+int wbfmm_hip_stream_destroy(void *s) {
+  if (!s) return 0;
+  WBFMM_CHECK(hipStreamSynchronize((hipStream_t)s));
+  WBFMM_CHECK(hipStreamDestroy((hipStream_t)s));
+  return 0;
+}
+
+int wbfmm_hip_set_stream(void *s, void **prev) {
+  if (prev) *prev = (void *)t_stream;
+  t_stream = (hipStream_t)s;
+  return 0;
+}
+
+void *wbfmm_hip_get_stream(void) { return (void *)stream(); }
+
 const char *wbfmm_hip_last_error(void) { return g_err; }
 
 int wbfmm_hip_malloc(void **p, size_t bytes) {
   if (wbfmm_hip_init()) return WBFMM_HIP_ERR_RUNTIME;
-  *p = nullptr;
-  if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
-  if (e != hipSuccess) {
-    fail("hipMalloc", e);
-    return WBFMM_HIP_ERR_NOMEM;
-  }
-  return 0;
+  return dmalloc(p, bytes);
 }
 
 int wbfmm_hip_free(void *p) {
   if (!p) return 0;
-  hipStreamSynchronize(stream());
-  WBFMM_CHECK(hipFree(p));
+  WBFMM_CHECK(hipFreeAsync(p, stream()));
   return 0;
 }
 
@@ -294,15 +379,15 @@ int wbfmm_hip_morton(const void *pts, size_t pstr, unsigned n,
     dpts = (const char *)staged;
   }
   unsigned *dbad = nullptr;
-  WBFMM_CHECK(hipMalloc(&dbad, sizeof(unsigned)));
+  if (dmalloc((void **)&dbad, sizeof(unsigned))) return WBFMM_HIP_ERR_NOMEM;
   WBFMM_CHECK(hipMemsetAsync(dbad, 0, sizeof(unsigned), stream()));
   WBFMM_LAUNCH("tree_morton", k_morton, dim3(nblk(n, 256)), dim3(256), 0,
                dpts, pstr, n, x0[0], x0[1], x0[2], D, xyz, codes, idx, dbad);
   WBFMM_CHECK(hipMemcpyAsync(nbad, dbad, sizeof(unsigned),
                              hipMemcpyDeviceToHost, stream()));
+  dfree(dbad);
+  dfree(staged);
   WBFMM_CHECK(hipStreamSynchronize(stream()));
-  hipFree(dbad);
-  if (staged) hipFree(staged);
   return 0;
 }
 
@@ -320,8 +405,9 @@ int wbfmm_hip_stage4(const void *pts, size_t pstr, unsigned n, double *out) {
   WBFMM_LAUNCH("stage4", k_stage4, dim3(nblk(n, 256)), dim3(256), 0, dpts,
                pstr, n, out);
   if (staged) {
+    dfree(staged);
+    // the caller's host block may be reused once this returns
     WBFMM_CHECK(hipStreamSynchronize(stream()));
-    hipFree(staged);
   }
   return 0;
 }
@@ -330,23 +416,25 @@ int wbfmm_hip_sort(uint64_t *codes, uint32_t *idx, unsigned n) {
   if (n < 2) return 0;
   uint64_t *k2 = nullptr;
   uint32_t *v2 = nullptr;
-  WBFMM_CHECK(hipMalloc(&k2, sizeof(uint64_t) * n));
-  WBFMM_CHECK(hipMalloc(&v2, sizeof(uint32_t) * n));
+  void *tmp = nullptr;
+  if (dmalloc((void **)&k2, sizeof(uint64_t) * n) ||
+      dmalloc((void **)&v2, sizeof(uint32_t) * n))
+    return WBFMM_HIP_ERR_NOMEM;
   size_t bytes = 0;
   WBFMM_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, codes, k2, idx, v2,
                                         n, 0, 60, stream()));
-  if (ensure_tmp(bytes)) return WBFMM_HIP_ERR_NOMEM;
+  if (dmalloc(&tmp, bytes)) return WBFMM_HIP_ERR_NOMEM;
   timing_begin("tree_sort");
-  WBFMM_CHECK(rocprim::radix_sort_pairs(g_tmp, bytes, codes, k2, idx, v2, n,
+  WBFMM_CHECK(rocprim::radix_sort_pairs(tmp, bytes, codes, k2, idx, v2, n,
                                         0, 60, stream()));
   timing_end("tree_sort");
   WBFMM_CHECK(hipMemcpyAsync(codes, k2, sizeof(uint64_t) * n,
                              hipMemcpyDeviceToDevice, stream()));
   WBFMM_CHECK(hipMemcpyAsync(idx, v2, sizeof(uint32_t) * n,
                              hipMemcpyDeviceToDevice, stream()));
-  WBFMM_CHECK(hipStreamSynchronize(stream()));
-  hipFree(k2);
-  hipFree(v2);
+  dfree(tmp);
+  dfree(k2);
+  dfree(v2);
   return 0;
 }
 
@@ -375,8 +463,8 @@ int wbfmm_hip_gather_strided(const double *src, int sstr, int ncomp,
   WBFMM_LAUNCH("gather_strength", k_gather_strided, dim3(nblk(n, 256)),
                dim3(256), 0, dsrc, sstr, ncomp, idx, n, out);
   if (staged) {
+    dfree(staged);
     WBFMM_CHECK(hipStreamSynchronize(stream()));
-    hipFree(staged);
   }
   return 0;
 }
@@ -397,15 +485,17 @@ int wbfmm_hip_compact_boxes(const uint32_t *start, const uint32_t *end,
                             uint32_t *list, unsigned *count) {
   unsigned nb = 1u << (3 * level);
   uint32_t *flags = nullptr, *pos = nullptr;
-  WBFMM_CHECK(hipMalloc(&flags, sizeof(uint32_t) * nb));
-  WBFMM_CHECK(hipMalloc(&pos, sizeof(uint32_t) * nb));
+  void *tmp = nullptr;
+  if (dmalloc((void **)&flags, sizeof(uint32_t) * nb) ||
+      dmalloc((void **)&pos, sizeof(uint32_t) * nb))
+    return WBFMM_HIP_ERR_NOMEM;
   WBFMM_LAUNCH("tree_box_flags", k_box_flags, dim3(nblk(nb, 256)), dim3(256),
                0, start, end, flag, nb, flags);
   size_t bytes = 0;
   WBFMM_CHECK(rocprim::exclusive_scan(nullptr, bytes, flags, pos, 0u, nb,
                                       rocprim::plus<uint32_t>(), stream()));
-  if (ensure_tmp(bytes)) return WBFMM_HIP_ERR_NOMEM;
-  WBFMM_CHECK(rocprim::exclusive_scan(g_tmp, bytes, flags, pos, 0u, nb,
+  if (dmalloc(&tmp, bytes)) return WBFMM_HIP_ERR_NOMEM;
+  WBFMM_CHECK(rocprim::exclusive_scan(tmp, bytes, flags, pos, 0u, nb,
                                       rocprim::plus<uint32_t>(), stream()));
   WBFMM_LAUNCH("tree_box_scatter", k_box_scatter, dim3(nblk(nb, 256)),
                dim3(256), 0, flags, pos, nb, list);
@@ -414,10 +504,55 @@ int wbfmm_hip_compact_boxes(const uint32_t *start, const uint32_t *end,
                              stream()));
   WBFMM_CHECK(hipMemcpyAsync(&lastp, pos + nb - 1, 4, hipMemcpyDeviceToHost,
                              stream()));
+  dfree(tmp);
+  dfree(flags);
+  dfree(pos);
   WBFMM_CHECK(hipStreamSynchronize(stream()));
   *count = lastf + lastp;
-  hipFree(flags);
-  hipFree(pos);
+  return 0;
+}
+
+int wbfmm_hip_bounds(const void *pts, size_t pstr, unsigned n, double *lo,
+                     double *hi) {
+  for (int c = 0; c < 3; c++) {
+    lo[c] = INFINITY;
+    hi[c] = -INFINITY;
+  }
+  if (n == 0) return 0;
+  if (!wbfmm_hip_is_device(pts)) {
+    for (unsigned i = 0; i < n; i++) {
+      const double *x = (const double *)((const char *)pts + (size_t)i * pstr);
+      for (int c = 0; c < 3; c++) {
+        lo[c] = fmin(lo[c], x[c]);
+        hi[c] = fmax(hi[c], x[c]);
+      }
+    }
+    return 0;
+  }
+  const unsigned nb = nblk(n, 256) < 512 ? nblk(n, 256) : 512;
+  double *d = nullptr;
+  if (dmalloc((void **)&d, sizeof(double) * 6 * nb)) return WBFMM_HIP_ERR_NOMEM;
+  std::vector<double> h(6 * (size_t)nb);
+  WBFMM_LAUNCH("bounds", k_bounds, dim3(nb), dim3(256), 0,
+               (const char *)pts, pstr, n, d);
+  WBFMM_CHECK(hipMemcpyAsync(h.data(), d, sizeof(double) * 6 * nb,
+                             hipMemcpyDeviceToHost, stream()));
+  dfree(d);
+  WBFMM_CHECK(hipStreamSynchronize(stream()));
+  for (unsigned b = 0; b < nb; b++)
+    for (int c = 0; c < 3; c++) {
+      lo[c] = fmin(lo[c], h[6 * b + c]);
+      hi[c] = fmax(hi[c], h[6 * b + 3 + c]);
+    }
+  return 0;
+}
+
+int wbfmm_hip_accumulate(double *dst, int dstr, const double *src, int sstr,
+                         int ncomp, unsigned n) {
+  if (n == 0) return 0;
+  if (dstr < ncomp || sstr < ncomp) return WBFMM_HIP_ERR_ARG;
+  WBFMM_LAUNCH("accumulate", k_accumulate, dim3(nblk(n, 256)), dim3(256), 0,
+               dst, dstr, src, sstr, ncomp, n);
   return 0;
 }
 

@@ -1,0 +1,670 @@
+// serve.cpp -- the serving engine declared in wbfmm_serve.h: a queue of
+// independent FMM solves run by worker threads, one HIP stream each.
+//
+// A worker runs the reference's call sequence for the request's problem
+// (tools/wbfmm-laplace-tree-calc.c.in:380-460 for Laplace,
+// tools/wbfmm-tree-calc.c.in:306-417 for Helmholtz) through the drop-in
+// API with its own stream selected (wbfmm_hip_set_stream), so the solves
+// of different workers overlap on the GPU.  Operator tables are kept per
+// worker: Laplace sets by order, and for Helmholtz one set per largest
+// order whose coaxial tables are rebuilt when the wavenumber or box width
+// changes (the rotation tables depend on the order only).
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "wbfmm_serve.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_between(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// output doubles per target for a field request (0: invalid)
+int field_width(int problem, unsigned field) {
+  const bool p = field & WBFMM_FIELD_POTENTIAL, g = field & WBFMM_FIELD_GRADIENT;
+  if (field & ~(unsigned)(WBFMM_FIELD_POTENTIAL | WBFMM_FIELD_GRADIENT)) return 0;
+  const int c = problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
+  return c * ((p ? 1 : 0) + (g ? 3 : 0));
+}
+
+// Laplace order for a relative accuracy: the error of the octree FMM with
+// one box of separation falls by about 0.3 per order (4.6e-7 potential and
+// 2.3e-6 gradient at order 12 on 1e6 uniform points)
+int laplace_order(double tol) {
+  if (!(tol > 0.0) || tol >= 1.0) return 2;
+  int p = (int)std::ceil(std::log(tol) / std::log(0.3)) + 1;
+  return std::min(std::max(p, 2), WBFMM_HIP_MAX_ORDER);
+}
+
+struct Plan {
+  int depth = 0;
+  int orders[WBFMM_TREE_MAX_DEPTH + 1] = {0};
+  int lmax = 0;
+  double x0[3] = {0, 0, 0};
+  double D = 1.0;
+  size_t footprint = 0;
+};
+
+struct Job {
+  int64_t id = 0;
+  wbfmm_serve_request_t req{};
+  Plan plan;
+  clk::time_point t_submit, t_start, t_done;
+  wbfmm_serve_result_t res{};
+  bool done = false;
+  bool tune = false;  // an autotuner run: not counted in the statistics
+};
+
+struct HelOps {
+  wbfmm_shift_operators_t *op = nullptr;
+  double k = 0.0, D = 0.0;
+  int depth = 0;
+  int orders[WBFMM_TREE_MAX_DEPTH + 1] = {0};
+};
+
+struct Worker {
+  std::thread th;
+  void *stream = nullptr;
+  std::map<int, wbfmm_shift_operators_t *> lap_ops;  // by order
+  std::map<int, HelOps> hel_ops;                     // by largest order
+};
+
+}  // namespace
+
+struct wbfmm_serve_s {
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Job *> queue;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<std::unique_ptr<Worker>> workers;
+  size_t budget = 0, inflight = 0, peak = 0;
+  int64_t next_id = 0;
+  bool stop = false;
+  // statistics since the last reset
+  std::vector<double> lat;
+  uint64_t targets = 0;
+  bool have_first = false;
+  clk::time_point first_submit, last_done;
+  // autotuned depths: (problem, precision, order, log2 n) -> depth
+  std::map<std::tuple<int, int, int, int>, int> tuned;
+};
+
+namespace {
+
+std::tuple<int, int, int, int> tune_key(const wbfmm_serve_request_t &r, int order) {
+  const unsigned n = std::max(r.nsrc, 1u);
+  return std::make_tuple(r.problem, r.precision ? r.precision : 64, order,
+                         (int)std::floor(std::log2((double)n)));
+}
+
+// device bytes of a solve: dense coefficient arrays of every level (as the
+// tree allocates them), box ranges, sorted points and the target field
+size_t footprint(const wbfmm_serve_request_t &r, const Plan &p) {
+  const size_t cplx = r.problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
+  size_t b = 0;
+  for (int l = 1; l <= p.depth; l++) {
+    const size_t nb = (size_t)1 << (3 * l);
+    const size_t C = (size_t)(p.orders[l] + 1) * (p.orders[l] + 1);
+    b += nb * (2 * C * 8 * cplx + 16);
+  }
+  const size_t nt = r.xt ? r.ntgt : r.nsrc;
+  b += (size_t)r.nsrc * (2 * 32 + 12 + 16 * cplx + (r.normals ? 32 : 0));
+  b += nt * (32 + 12 + 8 * (size_t)std::max(r.fstr, 1));
+  if (r.check) b += (size_t)r.nsrc * (32 + 16 * cplx);
+  return b + ((size_t)16 << 20);
+}
+
+int validate(const wbfmm_serve_request_t &r) {
+  if (r.problem != WBFMM_PROBLEM_LAPLACE && r.problem != WBFMM_PROBLEM_HELMHOLTZ)
+    return WBFMM_HIP_ERR_ARG;
+  if (r.precision != 0 && r.precision != 64) return WBFMM_HIP_ERR_ARG;
+  if (r.nsrc == 0 || !r.xs || r.pstr < 3 * sizeof(double)) return WBFMM_HIP_ERR_ARG;
+  if (!r.q && !r.dip) return WBFMM_HIP_ERR_ARG;
+  if (r.dip && (!r.normals || r.nstr < 3 * sizeof(double))) return WBFMM_HIP_ERR_ARG;
+  if (r.xt && r.tstr < 3 * sizeof(double)) return WBFMM_HIP_ERR_ARG;
+  const int fw = field_width(r.problem, r.field);
+  if (fw == 0 || r.fstr < fw || !r.f) return WBFMM_HIP_ERR_ARG;
+  if (r.problem == WBFMM_PROBLEM_HELMHOLTZ && !(r.k > 0.0)) return WBFMM_HIP_ERR_ARG;
+  if (r.order < 0 || r.order > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
+  if (r.depth < 0 || r.depth > 10) return WBFMM_HIP_ERR_ARG;
+  if (r.order == 0 && !(r.tol > 0.0 && r.tol < 1.0)) return WBFMM_HIP_ERR_ARG;
+  return 0;
+}
+
+// default depth: about 16 (Laplace) or 32 (Helmholtz) points per occupied
+// leaf, counting the leaves a uniform cloud would fill
+int default_depth(const wbfmm_serve_request_t &r) {
+  const double per = r.problem == WBFMM_PROBLEM_HELMHOLTZ ? 32.0 : 16.0;
+  const double n = (double)std::max(r.nsrc, r.xt ? r.ntgt : 0u);
+  int d = (int)std::lround(std::log(std::max(n / per, 1.0)) / std::log(8.0));
+  return std::min(std::max(d, 2), 8);
+}
+
+// domain, depth and orders; runs on the submitting thread (the bounds of
+// device points are reduced on its current stream)
+int make_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t &r, Plan &p) {
+  int rc = validate(r);
+  if (rc) return rc;
+  double lo[3], hi[3], lt[3], ht[3];
+  rc = wbfmm_hip_bounds(r.xs, r.pstr, r.nsrc, lo, hi);
+  if (rc) return rc;
+  if (r.xt && r.ntgt) {
+    rc = wbfmm_hip_bounds(r.xt, r.tstr, r.ntgt, lt, ht);
+    if (rc) return rc;
+    for (int c = 0; c < 3; c++) {
+      lo[c] = std::min(lo[c], lt[c]);
+      hi[c] = std::max(hi[c], ht[c]);
+    }
+  }
+  double e = 0.0;
+  for (int c = 0; c < 3; c++) e = std::max(e, hi[c] - lo[c]);
+  if (!(e > 0.0) || !std::isfinite(e)) e = 1.0;
+  // the reference drivers take the bounding cube widened by 0.01 on each
+  // side (tools/wbfmm-tree-calc.c.in:261-267); here 1% of the width
+  for (int c = 0; c < 3; c++) p.x0[c] = lo[c] - 0.01 * e;
+  p.D = 1.02 * e;
+
+  const int order_fixed = r.order;
+  const int lap_order = order_fixed ? order_fixed : laplace_order(r.tol);
+  if (r.depth) {
+    p.depth = r.depth;
+  } else {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tuned.find(tune_key(r, r.problem == WBFMM_PROBLEM_LAPLACE ? lap_order : order_fixed));
+    p.depth = it != s->tuned.end() ? it->second : default_depth(r);
+  }
+  p.lmax = 0;
+  for (int l = 0; l <= p.depth; l++) {
+    int o;
+    if (order_fixed) {
+      o = order_fixed;
+    } else if (r.problem == WBFMM_PROBLEM_LAPLACE) {
+      o = lap_order;
+    } else {
+      // Gumerov & Duraiswami's truncation number of the level
+      // (src/util.c:474-503) for the level's box width
+      wbfmm_tree_t tt{};
+      tt.D = p.D;
+      o = wbfmm_truncation_number(&tt, r.k, (guint)std::max(l, 2), r.tol);
+    }
+    p.orders[l] = std::min(std::max(o, 2), WBFMM_HIP_MAX_ORDER);
+    if (l >= 2) p.lmax = std::max(p.lmax, p.orders[l]);
+  }
+  p.orders[1] = p.orders[2];
+  p.footprint = footprint(r, p);
+  return 0;
+}
+
+wbfmm_shift_operators_t *laplace_ops(Worker &w, int lmax) {
+  auto it = w.lap_ops.find(lmax);
+  if (it != w.lap_ops.end()) return it->second;
+  wbfmm_shift_operators_t *op = wbfmm_shift_operators_new((guint)lmax, FALSE, nullptr);
+  w.lap_ops[lmax] = op;
+  return op;
+}
+
+// Helmholtz operator set of the worker for this plan; returns the
+// milliseconds spent on tables.  Coaxial tables of the levels are built
+// in parallel host threads that share the worker's stream.
+wbfmm_shift_operators_t *helmholtz_ops(Worker &w, const Plan &p, double k,
+                                       double *tables_ms) {
+  const auto t0 = clk::now();
+  HelOps &h = w.hel_ops[p.lmax];
+  if (!h.op) h.op = wbfmm_shift_operators_new((guint)p.lmax, FALSE, nullptr);
+  bool same = h.k == k && h.D == p.D && h.depth == p.depth;
+  for (int l = 2; same && l <= p.depth; l++) same = h.orders[l] == p.orders[l];
+  if (!same) {
+    wbfmm_shift_operators_coaxial_reset(h.op);
+    std::vector<std::thread> th;
+    wbfmm_shift_operators_t *op = h.op;
+    void *st = w.stream;
+    for (int l = 2; l <= p.depth; l++) {
+      const guint lsr = (guint)p.orders[l];
+      const guint lss = (guint)std::max(p.orders[l], p.orders[l - 1]);
+      const int lev = l;
+      const double D = p.D;
+      th.emplace_back([op, st, lsr, lss, lev, D, k]() {
+        wbfmm_hip_set_stream(st, nullptr);
+        wbfmm_shift_operators_coaxial_SR_init(op, D, (guint)lev, lsr, k, nullptr);
+        if (lev >= 3) wbfmm_shift_operators_coaxial_SS_init(op, D, (guint)lev, lss, k, nullptr);
+      });
+    }
+    for (auto &t : th) t.join();
+    h.k = k;
+    h.D = p.D;
+    h.depth = p.depth;
+    for (int l = 0; l <= p.depth; l++) h.orders[l] = p.orders[l];
+  }
+  *tables_ms = ms_between(t0, clk::now());
+  return h.op;
+}
+
+// relative l2 errors of the checked targets against the FP64 direct sum
+int check_errors(const wbfmm_serve_request_t &r, const double *dfield,
+                 unsigned ntgt, int fw, double *ep, double *eg) {
+  *ep = *eg = -1.0;
+  const unsigned m = std::min<unsigned>(r.check, ntgt);
+  if (m == 0) return 0;
+  const unsigned step = ntgt / m;
+  const void *xt = r.xt ? (const void *)r.xt : (const void *)r.xs;
+  const size_t tstr = r.xt ? r.tstr : r.pstr;
+  const bool helm = r.problem == WBFMM_PROBLEM_HELMHOLTZ;
+  const int cplx = helm ? 2 : 1;
+  double *xyz = nullptr, *nrm = nullptr, *tx = nullptr, *fd = nullptr, *fs = nullptr;
+  double *q = nullptr, *dip = nullptr;
+  uint32_t *iota = nullptr;
+  int rc = 0;
+  std::vector<uint32_t> hi(m);
+  for (unsigned i = 0; i < m; i++) hi[i] = i;
+  std::vector<double> hd((size_t)m * fw), hs((size_t)m * fw);
+  if ((rc = wbfmm_hip_malloc((void **)&xyz, sizeof(double) * 4 * r.nsrc))) goto out;
+  if ((rc = wbfmm_hip_stage4(r.xs, r.pstr, r.nsrc, xyz))) goto out;
+  if (r.dip) {
+    if ((rc = wbfmm_hip_malloc((void **)&nrm, sizeof(double) * 4 * r.nsrc))) goto out;
+    if ((rc = wbfmm_hip_stage4(r.normals, r.nstr, r.nsrc, nrm))) goto out;
+  }
+  // strengths on the device (host arrays are copied)
+  if (r.q) {
+    if (wbfmm_hip_is_device(r.q)) {
+      q = const_cast<double *>(r.q);
+    } else {
+      if ((rc = wbfmm_hip_malloc((void **)&q, sizeof(double) * cplx * r.nsrc))) goto out;
+      if ((rc = wbfmm_hip_memcpy(q, r.q, sizeof(double) * cplx * r.nsrc))) goto out;
+    }
+  }
+  if (r.dip) {
+    if (wbfmm_hip_is_device(r.dip)) {
+      dip = const_cast<double *>(r.dip);
+    } else {
+      if ((rc = wbfmm_hip_malloc((void **)&dip, sizeof(double) * cplx * r.nsrc))) goto out;
+      if ((rc = wbfmm_hip_memcpy(dip, r.dip, sizeof(double) * cplx * r.nsrc))) goto out;
+    }
+  }
+  if ((rc = wbfmm_hip_malloc((void **)&tx, sizeof(double) * 4 * m))) goto out;
+  if ((rc = wbfmm_hip_stage4(xt, tstr * step, m, tx))) goto out;
+  if ((rc = wbfmm_hip_malloc((void **)&fd, sizeof(double) * fw * m))) goto out;
+  if ((rc = wbfmm_hip_memset(fd, 0, sizeof(double) * fw * m))) goto out;
+  if (helm)
+    rc = wbfmm_hip_hel_direct(xyz, q, nrm, dip, r.nsrc, tx, m, r.k, (int)r.field, 1e-6, fd, fw);
+  else
+    rc = wbfmm_hip_lap_direct(xyz, q, nrm, dip, r.nsrc, tx, m, (int)r.field, 1e-6, fd, fw);
+  if (rc) goto out;
+  // the FMM values of the same targets
+  if ((rc = wbfmm_hip_malloc((void **)&iota, sizeof(uint32_t) * m))) goto out;
+  if ((rc = wbfmm_hip_memcpy(iota, hi.data(), sizeof(uint32_t) * m))) goto out;
+  if ((rc = wbfmm_hip_malloc((void **)&fs, sizeof(double) * fw * m))) goto out;
+  if ((rc = wbfmm_hip_gather_strided(dfield, fw * (int)step, fw, iota, m, fs))) goto out;
+  if ((rc = wbfmm_hip_memcpy(hd.data(), fd, sizeof(double) * fw * m))) goto out;
+  if ((rc = wbfmm_hip_memcpy(hs.data(), fs, sizeof(double) * fw * m))) goto out;
+  {
+    // potential: the first cplx values when requested; gradient: the rest
+    const int np = (r.field & WBFMM_FIELD_POTENTIAL) ? cplx : 0;
+    double pn = 0, pd = 0, gn = 0, gd = 0;
+    for (unsigned i = 0; i < m; i++)
+      for (int c = 0; c < fw; c++) {
+        const double d = hd[(size_t)i * fw + c], e = hs[(size_t)i * fw + c] - d;
+        if (c < np) {
+          pn += e * e;
+          pd += d * d;
+        } else {
+          gn += e * e;
+          gd += d * d;
+        }
+      }
+    if (np) *ep = pd > 0 ? std::sqrt(pn / pd) : std::sqrt(pn);
+    if (fw > np) *eg = gd > 0 ? std::sqrt(gn / gd) : std::sqrt(gn);
+  }
+out:
+  wbfmm_hip_free(xyz);
+  wbfmm_hip_free(nrm);
+  wbfmm_hip_free(tx);
+  wbfmm_hip_free(fd);
+  wbfmm_hip_free(fs);
+  wbfmm_hip_free(iota);
+  if (q && q != r.q) wbfmm_hip_free(q);
+  if (dip && dip != r.dip) wbfmm_hip_free(dip);
+  return rc;
+}
+
+// one request on the worker's stream
+void run(wbfmm_serve_t *s, Worker &w, Job &j) {
+  const wbfmm_serve_request_t &r = j.req;
+  const Plan &p = j.plan;
+  const bool helm = r.problem == WBFMM_PROBLEM_HELMHOLTZ;
+  const int cplx = helm ? 2 : 1;
+  const int fw = field_width(r.problem, r.field);
+  const unsigned ntgt = r.xt ? r.ntgt : r.nsrc;
+  const void *xt = r.xt ? (const void *)r.xt : (const void *)r.xs;
+  const size_t tstr = r.xt ? r.tstr : r.pstr;
+  double tables_ms = 0.0;
+  (void)s;
+
+  wbfmm_shift_operators_t *op =
+      helm ? helmholtz_ops(w, p, r.k, &tables_ms) : laplace_ops(w, p.lmax);
+  // (Laplace tables are built on first use inside the passes, so their
+  // time is part of the first solve of each order on a worker)
+
+  double x0[3] = {p.x0[0], p.x0[1], p.x0[2]};
+  wbfmm_tree_t *t = wbfmm_tree_new(x0, p.D, r.nsrc);
+  wbfmm_tree_add_points(t, (gpointer)r.xs, r.pstr, (gpointer)r.normals,
+                        r.normals ? r.nstr : 0, r.nsrc, FALSE);
+  for (int l = 0; l < p.depth; l++) wbfmm_tree_refine(t);
+  t->problem = helm ? WBFMM_PROBLEM_HELMHOLTZ : WBFMM_PROBLEM_LAPLACE;
+  t->nq = 1;
+  for (int l = 1; l <= p.depth; l++) {
+    if (helm)
+      wbfmm_tree_coefficient_init(t, (guint)l, (guint)p.orders[l], (guint)p.orders[l]);
+    else
+      wbfmm_tree_laplace_coefficient_init(t, (guint)l, (guint)p.orders[l], (guint)p.orders[l]);
+  }
+  wbfmm_target_list_t *tl = wbfmm_target_list_new(t, ntgt);
+  wbfmm_target_list_add_points(tl, (gpointer)xt, tstr, ntgt);
+
+  // the field is evaluated into a zeroed buffer of the engine and then
+  // added to the caller's array, so the check can read the FMM values
+  double *df = nullptr;
+  int rc = wbfmm_hip_malloc((void **)&df, sizeof(double) * fw * std::max(ntgt, 1u));
+  if (!rc) rc = wbfmm_hip_memset(df, 0, sizeof(double) * fw * std::max(ntgt, 1u));
+
+  for (int l = 1; l <= p.depth; l++) wbfmm_tree_coefficients_zero(t, (guint)l);
+  if (helm) {
+    wbfmm_tree_leaf_expansions(t, r.k, const_cast<double *>(r.q), cplx,
+                               const_cast<double *>(r.dip), cplx, TRUE, nullptr);
+    for (int l = p.depth; l >= 3; l--) wbfmm_upward_pass(t, op, (guint)l, nullptr);
+    for (int l = 2; l <= p.depth; l++) wbfmm_downward_pass_ref(t, op, (guint)l, nullptr, 0);
+    wbfmm_target_list_coefficients_init(tl, r.field);
+    wbfmm_target_list_local_coefficients(tl, r.k, nullptr);
+  } else {
+    wbfmm_tree_laplace_leaf_expansions(t, const_cast<double *>(r.q), 1,
+                                       const_cast<double *>(r.dip), 1, TRUE, nullptr);
+    for (int l = p.depth; l >= 3; l--) wbfmm_laplace_upward_pass(t, op, (guint)l, nullptr);
+    for (int l = 2; l <= p.depth; l++)
+      wbfmm_laplace_downward_pass_ref(t, op, (guint)l, nullptr, 0);
+    wbfmm_target_list_coefficients_init(tl, r.field);
+    wbfmm_laplace_target_list_local_coefficients(tl, 1, nullptr);
+  }
+  if (!rc)
+    wbfmm_target_list_local_field(tl, const_cast<double *>(r.q), cplx,
+                                  const_cast<double *>(r.dip), cplx, df, fw);
+  // add into the caller's output
+  if (!rc) {
+    if (wbfmm_hip_is_device(r.f)) {
+      rc = wbfmm_hip_accumulate(r.f, r.fstr, df, fw, fw, ntgt);
+      if (!rc) rc = wbfmm_hip_sync();
+    } else {
+      std::vector<double> h((size_t)fw * ntgt);
+      rc = wbfmm_hip_memcpy(h.data(), df, sizeof(double) * h.size());
+      if (!rc)
+        for (unsigned i = 0; i < ntgt; i++)
+          for (int c = 0; c < fw; c++) r.f[(size_t)i * r.fstr + c] += h[(size_t)i * fw + c];
+    }
+  }
+  j.t_done = clk::now();
+  j.res.tables_ms = tables_ms;
+  if (!rc && r.check) {
+    const auto c0 = clk::now();
+    rc = check_errors(r, df, ntgt, fw, &j.res.err_potential, &j.res.err_gradient);
+    j.res.check_ms = ms_between(c0, clk::now());
+  } else {
+    j.res.err_potential = j.res.err_gradient = -1.0;
+  }
+  wbfmm_hip_free(df);
+  wbfmm_target_list_free(tl);
+  wbfmm_tree_free(t);
+  wbfmm_hip_sync();
+  j.res.status = rc;
+}
+
+void worker_main(wbfmm_serve_t *s, Worker *w) {
+  wbfmm_hip_set_stream(w->stream, nullptr);
+  for (;;) {
+    Job *j = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] {
+        if (s->stop && s->queue.empty()) return true;
+        if (s->queue.empty()) return false;
+        const size_t need = s->queue.front()->plan.footprint;
+        return s->inflight == 0 || s->inflight + need <= s->budget;
+      });
+      if (s->queue.empty()) break;  // stopping
+      j = s->queue.front();
+      s->queue.pop_front();
+      s->inflight += j->plan.footprint;
+      s->peak = std::max(s->peak, s->inflight);
+      j->t_start = clk::now();
+    }
+    // another waiting worker may fit beside this one
+    s->cv_work.notify_all();
+    run(s, *w, *j);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->inflight -= j->plan.footprint;
+      j->res.queue_ms = ms_between(j->t_submit, j->t_start);
+      j->res.solve_ms = ms_between(j->t_start, j->t_done);
+      j->res.latency_ms = ms_between(j->t_submit, j->t_done);
+      if (!j->tune && j->res.status == 0) {
+        s->lat.push_back(j->res.latency_ms);
+        s->targets += j->req.xt ? j->req.ntgt : j->req.nsrc;
+        s->last_done = std::max(s->last_done, j->t_done);
+      }
+      j->done = true;
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();
+  }
+  // release the worker's operator sets on its own stream
+  for (auto &kv : w->lap_ops) wbfmm_shift_operators_free(kv.second);
+  for (auto &kv : w->hel_ops) wbfmm_shift_operators_free(kv.second.op);
+  wbfmm_hip_sync();
+  wbfmm_hip_set_stream(nullptr, nullptr);
+}
+
+// plan on the caller's thread (the bounds pass runs on the library stream)
+int64_t enqueue(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, bool tune) {
+  auto j = std::make_unique<Job>();
+  j->req = *r;
+  j->tune = tune;
+  j->t_submit = clk::now();
+  int rc = make_plan(s, *r, j->plan);
+  if (rc) return rc;
+  j->res.depth = j->plan.depth;
+  j->res.order = j->plan.lmax;
+  j->res.precision = 64;
+  j->res.footprint = j->plan.footprint;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return WBFMM_HIP_ERR_ARG;
+  const int64_t id = s->next_id++;
+  j->id = id;
+  if (!tune && !s->have_first) {
+    s->have_first = true;
+    s->first_submit = j->t_submit;
+    s->last_done = j->t_submit;
+  }
+  s->queue.push_back(j.get());
+  s->jobs[id] = std::move(j);
+  s->cv_work.notify_one();
+  return id;
+}
+
+}  // namespace
+
+extern "C" {
+
+wbfmm_serve_t *wbfmm_serve_new(gint nstreams, gsize budget) {
+  if (nstreams < 1 || nstreams > 64) return nullptr;
+  if (wbfmm_hip_init()) return nullptr;
+  auto *s = new wbfmm_serve_s();
+  if (budget == 0) {
+    // 90% of the device memory free now
+    size_t fr = 0, tot = 0;
+    if (wbfmm_hip_mem_info(&fr, &tot) == 0) budget = fr / 10 * 9;
+    if (budget == 0) budget = (size_t)64 << 30;
+  }
+  s->budget = budget;
+  // process-wide tables of the drop-in API, set once before any worker
+  wbfmm_shift_angle_table_init();
+  wbfmm_laplace_coaxial_translate_init(WBFMM_HIP_MAX_ORDER + 2);
+  for (int i = 0; i < nstreams; i++) {
+    auto w = std::make_unique<Worker>();
+    if (wbfmm_hip_stream_create(&w->stream)) {
+      s->stop = true;
+      for (auto &x : s->workers) wbfmm_hip_stream_destroy(x->stream);
+      delete s;
+      return nullptr;
+    }
+    s->workers.push_back(std::move(w));
+  }
+  for (auto &w : s->workers) w->th = std::thread(worker_main, s, w.get());
+  return s;
+}
+
+gint64 wbfmm_serve_submit(wbfmm_serve_t *s, const wbfmm_serve_request_t *r) {
+  if (!s || !r) return WBFMM_HIP_ERR_ARG;
+  return enqueue(s, r, false);
+}
+
+gint wbfmm_serve_wait(wbfmm_serve_t *s, gint64 ticket, wbfmm_serve_result_t *res) {
+  if (!s) return WBFMM_HIP_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(ticket);
+  if (it == s->jobs.end()) return WBFMM_HIP_ERR_ARG;
+  Job *j = it->second.get();
+  s->cv_done.wait(lk, [&] { return j->done; });
+  if (res) *res = j->res;
+  const int st = j->res.status;
+  s->jobs.erase(it);
+  return st;
+}
+
+gint wbfmm_serve_stats(wbfmm_serve_t *s, wbfmm_serve_stats_t *st) {
+  if (!s || !st) return WBFMM_HIP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(st, 0, sizeof(*st));
+  st->completed = s->lat.size();
+  st->targets = s->targets;
+  st->peak_bytes = s->peak;
+  if (!s->lat.empty()) {
+    std::vector<double> v = s->lat;
+    std::sort(v.begin(), v.end());
+    // nearest-rank percentiles
+    auto pct = [&](double q) {
+      size_t k = (size_t)std::ceil(q * v.size());
+      return v[std::min(std::max(k, (size_t)1), v.size()) - 1];
+    };
+    st->p50_ms = pct(0.50);
+    st->p99_ms = pct(0.99);
+    double sum = 0;
+    for (double x : v) sum += x;
+    st->mean_ms = sum / v.size();
+    st->max_ms = v.back();
+    st->span_s = ms_between(s->first_submit, s->last_done) / 1e3;
+  }
+  return 0;
+}
+
+gint wbfmm_serve_reset_stats(wbfmm_serve_t *s) {
+  if (!s) return WBFMM_HIP_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->lat.clear();
+  s->targets = 0;
+  s->have_first = false;
+  s->peak = s->inflight;
+  return 0;
+}
+
+gint wbfmm_serve_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, gint *depth,
+                      gint *order, gsize *fp) {
+  if (!s || !r) return WBFMM_HIP_ERR_ARG;
+  Plan p;
+  int rc = make_plan(s, *r, p);
+  if (rc) return rc;
+  if (depth) *depth = p.depth;
+  if (order) *order = p.lmax;
+  if (fp) *fp = p.footprint;
+  return 0;
+}
+
+gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gint *depths,
+                      gint ndepth, gint *best) {
+  if (!s || !r || !depths || ndepth < 1) return WBFMM_HIP_ERR_ARG;
+  int rc = validate(*r);
+  if (rc) return rc;
+  const int fw = field_width(r->problem, r->field);
+  const unsigned ntgt = r->xt ? r->ntgt : r->nsrc;
+  double *scratch = nullptr;
+  rc = wbfmm_hip_malloc((void **)&scratch, sizeof(double) * fw * std::max(ntgt, 1u));
+  if (rc) return rc;
+  double best_ms = INFINITY;
+  int best_d = -1, order = 0;
+  for (int i = 0; i < ndepth; i++) {
+    wbfmm_serve_request_t q = *r;
+    q.depth = depths[i];
+    q.f = scratch;
+    q.fstr = fw;
+    q.check = 0;
+    // two runs: the first builds the worker's tables
+    double ms = INFINITY;
+    for (int rep = 0; rep < 2; rep++) {
+      const int64_t id = enqueue(s, &q, true);
+      if (id < 0) {
+        rc = (int)id;
+        break;
+      }
+      wbfmm_serve_result_t res;
+      rc = wbfmm_serve_wait(s, id, &res);
+      if (rc) break;
+      ms = res.solve_ms - res.tables_ms;
+      order = res.order;
+    }
+    if (rc) break;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_d = depths[i];
+    }
+  }
+  wbfmm_hip_sync();
+  wbfmm_hip_free(scratch);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int key_order = r->problem == WBFMM_PROBLEM_LAPLACE
+                              ? (r->order ? r->order : laplace_order(r->tol))
+                              : r->order;
+    (void)order;
+    s->tuned[tune_key(*r, key_order)] = best_d;
+  }
+  if (best) *best = best_d;
+  return 0;
+}
+
+gint wbfmm_serve_free(wbfmm_serve_t *s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto &w : s->workers) w->th.join();
+  for (auto &w : s->workers) wbfmm_hip_stream_destroy(w->stream);
+  delete s;
+  return 0;
+}
+
+}  // extern "C"

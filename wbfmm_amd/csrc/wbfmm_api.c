@@ -15,6 +15,7 @@
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -233,35 +234,48 @@ gint wbfmm_box_interaction_list_4(guint level, guint64 idx, guint64 *list,
   return n;
 }
 
+/* sorted distinct cos(theta) of the list-4 offsets, built once */
+static double g_l4_vals[64];
+static int g_l4_nv = 0;
+static pthread_once_t g_l4_once = PTHREAD_ONCE_INIT;
+
+static void l4_values_init(void)
+{
+  int dx, dy, dz, a, nv = 0;
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        double v;
+        int b, dup = 0;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        v = -dz / sqrt((double)(dx * dx + dy * dy + dz * dz));
+        for (b = 0; b < nv; b++)
+          if (fabs(g_l4_vals[b] - v) < 1e-12) dup = 1;
+        if (!dup) g_l4_vals[nv++] = v;
+      }
+  for (a = 0; a < nv; a++) {
+    int b;
+    for (b = a + 1; b < nv; b++)
+      if (g_l4_vals[b] > g_l4_vals[a]) {
+        double tt = g_l4_vals[a];
+        g_l4_vals[a] = g_l4_vals[b];
+        g_l4_vals[b] = tt;
+      }
+  }
+  g_l4_nv = nv;
+}
+
 gint wbfmm_box_interaction_index(gint i, gint j, gint k)
 {
   /* index of the polar-angle class of offset (i,j,k): the reference
      returns the row of its 49-entry theta table (src/indexing.c:385-392);
      this returns the same classification as the position of cos(theta)
      among the sorted distinct values over all list-4 offsets */
-  static double vals[64];
-  static int nv = 0;
+  const double *vals = g_l4_vals;
   double c, r = sqrt((double)(i * i + j * j + k * k));
-  int a;
-  if (nv == 0) {
-    int dx, dy, dz;
-    for (dx = -3; dx <= 3; dx++)
-      for (dy = -3; dy <= 3; dy++)
-        for (dz = -3; dz <= 3; dz++) {
-          double v;
-          int b, dup = 0;
-          if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
-          v = -dz / sqrt((double)(dx * dx + dy * dy + dz * dz));
-          for (b = 0; b < nv; b++)
-            if (fabs(vals[b] - v) < 1e-12) dup = 1;
-          if (!dup) vals[nv++] = v;
-        }
-    for (a = 0; a < nv; a++) {
-      int b;
-      for (b = a + 1; b < nv; b++)
-        if (vals[b] > vals[a]) { double tt = vals[a]; vals[a] = vals[b]; vals[b] = tt; }
-    }
-  }
+  int a, nv;
+  pthread_once(&g_l4_once, l4_values_init);
+  nv = g_l4_nv;
   if (i < -3 || i > 3 || j < -3 || j > 3 || k < -3 || k > 3)
     WBFMM_FATAL("offset (%d,%d,%d) outside the 7x7x7 grid", i, j, k);
   c = -k / r;
@@ -819,25 +833,36 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
 }
 
 /* all boxes of a level, used as the target list when no target list is
-   registered (every box can hold a target) */
+   registered (every box can hold a target).  The list only grows; a
+   replaced list is kept, since launches on other streams may still read
+   it, so at most twice the largest list is held. */
 static uint32_t *g_all_boxes = NULL;
 static unsigned g_all_n = 0;
+static uint32_t *g_all_old[WBFMM_TREE_MAX_DEPTH + 1];
+static int g_all_nold = 0;
+static pthread_mutex_t g_all_mu = PTHREAD_MUTEX_INITIALIZER;
 
 static uint32_t *all_boxes(unsigned level, unsigned *n)
 {
   unsigned nb = 1u << (3 * level);
+  uint32_t *list;
+  pthread_mutex_lock(&g_all_mu);
   if (nb > g_all_n) {
-    uint32_t *h = malloc(sizeof(uint32_t) * nb);
+    uint32_t *h = malloc(sizeof(uint32_t) * nb), *d = NULL;
     unsigned i;
+    if (!h) WBFMM_FATAL("out of host memory");
     for (i = 0; i < nb; i++) h[i] = i;
-    if (g_all_boxes) wbfmm_hip_free(g_all_boxes);
-    HIPOK(wbfmm_hip_malloc((void **)&g_all_boxes, sizeof(uint32_t) * nb));
-    HIPOK(wbfmm_hip_memcpy(g_all_boxes, h, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_malloc((void **)&d, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_memcpy(d, h, sizeof(uint32_t) * nb)); /* synchronous */
     free(h);
+    if (g_all_boxes) g_all_old[g_all_nold++] = g_all_boxes;
+    g_all_boxes = d;
     g_all_n = nb;
   }
+  list = g_all_boxes;
+  pthread_mutex_unlock(&g_all_mu);
   *n = nb;
-  return g_all_boxes;
+  return list;
 }
 
 static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
@@ -1382,6 +1407,21 @@ gint wbfmm_shift_operators_coaxial_SS_init(wbfmm_shift_operators_t *w, gdouble D
   HIPOK(wbfmm_hip_memcpy(w->SS[level], h, sizeof(double) * 2 * ne));
   free(h);
   od->ss_L[level] = L;
+  return 0;
+}
+
+gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *w)
+{
+  ops_dev *od = (ops_dev *)w;
+  int n;
+  for (n = 0; n <= WBFMM_TREE_MAX_DEPTH; n++) {
+    wbfmm_hip_free(w->SR[n]);
+    wbfmm_hip_free(w->SS[n]);
+    w->SR[n] = w->SS[n] = NULL;
+    w->L[n] = 0;
+    od->ss_L[n] = 0;
+  }
+  od->hel_k_set = 0;
   return 0;
 }
 

@@ -618,107 +618,50 @@ int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
 /* index of (E|F)_{l n}^m, l >= n >= m >= 0 (src/wbfmm-private.h:149-157) */
 #define CXI(l, m, n) ((size_t)(l) * ((l) + 1) * ((l) + 2) / 6 + (size_t)(n) * ((n) + 1) / 2 + (m))
 
-/* spherical Bessel j_l (downward recursion normalised by j_0, stable for
-   every l) and y_l (upward) in quad precision for l <= LL */
-static void bessel_jy_q(int LL, qreal x, qreal *j, qreal *y)
-{
-  int top = LL + 40 + (int)x, i;
-  qreal jp = 0, jc = 1e-300Q, jm, *jj = calloc(top + 2, sizeof(qreal)), scale;
-  for (i = top; i >= 1; i--) {
-    jm = (2 * i + 1) / x * jc - jp;
-    jp = jc;
-    jc = jm;
-    jj[i - 1] = jc;
-    if (fabsq(jc) > 1e250Q) {
-      int t2;
-      for (t2 = i - 1; t2 <= top; t2++) jj[t2] *= 1e-250Q;
-      jp *= 1e-250Q;
-      jc *= 1e-250Q;
-    }
-  }
-  scale = (sinq(x) / x) / jj[0];
-  for (i = 0; i <= LL; i++) j[i] = jj[i] * scale;
-  free(jj);
-  y[0] = -cosq(x) / x;
-  if (LL >= 1) y[1] = -cosq(x) / (x * x) - sinq(x) / x;
-  for (i = 1; i < LL; i++) y[i + 1] = (2 * i + 1) / x * y[i] - y[i - 1];
-}
+/* index of (E|F)_{l n}^m, l >= n >= m >= 0 (src/wbfmm-private.h:149-157) */
+#define CXI(l, m, n) ((size_t)(l) * ((l) + 1) * ((l) + 2) / 6 + (size_t)(n) * ((n) + 1) / 2 + (m))
 
-static qreal qa(int n, int m)
-{
-  m = abs(m);
-  if (n < m) return 0;
-  return sqrtq((qreal)(n + 1 + m) * (n + 1 - m) / ((qreal)(2 * n + 1) * (2 * n + 3)));
-}
+/* The recursion loses about log10(amplification) digits to cancellation,
+   which grows with |kr|: in x87 extended precision (64-bit mantissa) the
+   tables stay within 2e-13 of the quad-precision ones for |kr| <= 20 at
+   orders up to 37, at a twentieth of the cost; beyond that quad precision
+   is used. */
+#define CR_T long double
+#define CR_NAME(x) x##_ld
+#define CR_C(x) x##L
+#define CR_SQRT sqrtl
+#define CR_FABS fabsl
+#define CR_SIN sinl
+#define CR_COS cosl
+#include "hel_coax.inc"
+#undef CR_T
+#undef CR_NAME
+#undef CR_C
+#undef CR_SQRT
+#undef CR_FABS
+#undef CR_SIN
+#undef CR_COS
 
-static qreal qb(int n, int m)
-{
-  qreal v;
-  if (n < abs(m)) return 0;
-  v = sqrtq((qreal)(n - m - 1) * (n - m) / ((qreal)(2 * n - 1) * (2 * n + 1)));
-  return m >= 0 ? v : -v;
-}
+#define CR_T __float128
+#define CR_NAME(x) x##_q
+#define CR_C(x) x##Q
+#define CR_SQRT sqrtq
+#define CR_FABS fabsq
+#define CR_SIN sinq
+#define CR_COS cosq
+#include "hel_coax.inc"
+#undef CR_T
+#undef CR_NAME
+#undef CR_C
+#undef CR_SQRT
+#undef CR_FABS
+#undef CR_SIN
+#undef CR_COS
 
-/* coaxial translation coefficients (E|F)_{l n}^m for l <= L at signed
-   distance kr (Gumerov & Duraiswami 2004, eqs 4.79, 4.83, 4.84, the
-   recursions of src/translations.c:205-409): kind 0 R|R (= S|S), kind 1
-   S|R.  The recursion needs degrees up to 2L and is run in quad precision.
-   E gets 2 doubles (re, im) per entry CXI(l, m, n). */
 int wbfmm_hip_hel_coaxial_table(int kind, int L, double kr, double *E)
 {
-  int l, m, n, LL = 2 * L, neg = kr < 0.0;
-  size_t ne = CXI(LL + 2, 0, 0);
-  qreal *er, *ei, *jq, *yq, x;
-  if (L < 0 || L > 64 || kr == 0.0) return WBFMM_HIP_ERR_ARG;
-  er = calloc(ne, sizeof(qreal));
-  ei = calloc(ne, sizeof(qreal));
-  jq = calloc(LL + 2, sizeof(qreal));
-  yq = calloc(LL + 2, sizeof(qreal));
-  if (!er || !ei || !jq || !yq) return WBFMM_HIP_ERR_NOMEM;
-  x = fabsq((qreal)kr);
-  bessel_jy_q(LL + 1, x, jq, yq);
-  /* seeds (E|F)_{l 0}^0 = sqrt(2l+1) f_l(kr); f_l(-x) = (-1)^l f_l(x) */
-  for (l = 0; l <= LL; l++) {
-    qreal s = sqrtq((qreal)(2 * l + 1)) * ((neg && (l & 1)) ? -1 : 1);
-    er[CXI(l, 0, 0)] = s * jq[l];
-    ei[CXI(l, 0, 0)] = kind == 1 ? s * yq[l] : 0;
-  }
-  /* 4.79 at m = 0: raise n */
-  for (n = 0; n <= L; n++)
-    for (l = n + 1; l <= LL - n; l++) {
-      qreal a1 = qa(n - 1, 0), a2 = qa(n, 0), a3 = qa(l, 0), a4 = qa(l - 1, 0);
-      qreal pr = n >= 1 ? er[CXI(l, 0, n - 1)] : 0, pi = n >= 1 ? ei[CXI(l, 0, n - 1)] : 0;
-      er[CXI(l, 0, n + 1)] = (a1 * pr - a3 * er[CXI(l + 1, 0, n)] + a4 * er[CXI(l - 1, 0, n)]) / a2;
-      ei[CXI(l, 0, n + 1)] = (a1 * pi - a3 * ei[CXI(l + 1, 0, n)] + a4 * ei[CXI(l - 1, 0, n)]) / a2;
-    }
-  /* 4.84: diagonal (E|F)_{l, m+1}^{m+1} */
-  for (m = 0; m <= L; m++)
-    for (l = m + 1; l <= LL - m; l++) {
-      qreal b1 = qb(m + 1, -m - 1), b2 = qb(l, -m - 1), b3 = qb(l + 1, m);
-      er[CXI(l, m + 1, m + 1)] = (b2 * er[CXI(l - 1, m, m)] - b3 * er[CXI(l + 1, m, m)]) / b1;
-      ei[CXI(l, m + 1, m + 1)] = (b2 * ei[CXI(l - 1, m, m)] - b3 * ei[CXI(l + 1, m, m)]) / b1;
-    }
-  /* 4.79 for m >= 1 */
-  for (m = 1; m <= L; m++)
-    for (n = m; n <= L; n++)
-      for (l = n + 1; l <= LL - n; l++) {
-        qreal a1 = qa(n - 1, m), a2 = qa(n, m), a3 = qa(l, m), a4 = qa(l - 1, m);
-        qreal pr = (n - 1 >= m) ? er[CXI(l, m, n - 1)] : 0;
-        qreal pi = (n - 1 >= m) ? ei[CXI(l, m, n - 1)] : 0;
-        er[CXI(l, m, n + 1)] = (a1 * pr - a3 * er[CXI(l + 1, m, n)] + a4 * er[CXI(l - 1, m, n)]) / a2;
-        ei[CXI(l, m, n + 1)] = (a1 * pi - a3 * ei[CXI(l + 1, m, n)] + a4 * ei[CXI(l - 1, m, n)]) / a2;
-      }
-  for (l = 0; l <= L; l++)
-    for (n = 0; n <= l; n++)
-      for (m = 0; m <= n; m++) {
-        E[2 * CXI(l, m, n)] = (double)er[CXI(l, m, n)];
-        E[2 * CXI(l, m, n) + 1] = (double)ei[CXI(l, m, n)];
-      }
-  free(er);
-  free(ei);
-  free(jq);
-  free(yq);
-  return 0;
+  if (fabs(kr) <= 20.0) return coaxial_ld(kind, L, kr, E);
+  return coaxial_q(kind, L, kr, E);
 }
 
 /* ------------------------------------------------------------------
