@@ -301,11 +301,14 @@ def _upload(W, probs):
     W.sync()
 
 
-def _request(S, p, field=3, depth=0, check=0):
+PRECISION = {"auto": 0, "32": 32, "64": 64}
+
+
+def _request(S, p, field=3, depth=0, check=0, precision=0):
     fw = 8 if p["helm"] else 4
     return S.make_request(2 if p["helm"] else 1, p["d_xs"], q=p["d_q"], f=p["d_f"], fstr=fw,
                           k=p["k"], tol=p["tol"], field=field, depth=depth, check=check,
-                          nsrc=p["n"])
+                          nsrc=p["n"], precision=precision)
 
 
 def _release(probs):
@@ -327,7 +330,8 @@ def run_batched(args):
     srv = S.Server(args.streams)
 
     def step(check):
-        tk = [srv.submit(_request(S, p, depth=p["depth"], check=check)) for p in probs]
+        tk = [srv.submit(_request(S, p, depth=p["depth"], check=check,
+                                  precision=PRECISION[args.precision])) for p in probs]
         return [srv.wait(t) for t in tk]
 
     for _ in range(args.warmup):
@@ -389,7 +393,7 @@ def run_serving(args):
     srv = S.Server(args.streams)
     # warm the workers' Laplace tables and the kernels with small requests
     warm = [p for p in reqs if p["n"] < 20000][:2 * args.streams]
-    for t in [srv.submit(_request(S, p)) for p in warm]:
+    for t in [srv.submit(_request(S, p, precision=PRECISION[args.precision])) for p in warm]:
         srv.wait(t)
     srv.reset_stats()
     tickets = [None] * len(reqs)
@@ -400,7 +404,8 @@ def run_serving(args):
             delay = t0 + p["t"] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            tickets[i] = srv.submit(_request(S, p, check=args.check_targets))
+            tickets[i] = srv.submit(_request(S, p, check=args.check_targets,
+                                             precision=PRECISION[args.precision]))
 
     th = threading.Thread(target=feeder)
     th.start()
@@ -429,13 +434,15 @@ def run_serving(args):
         "data": "synthetic (SplitMix64 seed 7)",
         "config": {"workload": f"serving: {len(reqs)} requests, Poisson arrivals at "
                                f"{args.qps} req/s, N = 10^U(3,6), Laplace or Helmholtz (k ~ "
-                               "U(1,32)), cube or sphere, accuracy 1e-4 or 1e-8 (FP64 both), "
+                               "U(1,32)), cube or sphere, accuracy 1e-4 or 1e-8 (FP32 for Laplace at 1e-4 with --precision auto), "
                                "targets = sources, potential+gradient",
                    "requests": len(reqs), "qps": args.qps, "streams": args.streams,
                    "parallelism": "single GPU"},
         "latency_ms": {"p50": st.p50_ms, "p99": st.p99_ms, "mean": st.mean_ms, "max": st.max_ms},
         "by_accuracy": per_tol,
         "tables_ms_total": float(sum(r.tables_ms for r in res)),
+        "precision_mix": {"fp32": sum(1 for r in res if r.precision == 32),
+                          "fp64": sum(1 for r in res if r.precision == 64)},
         "failed": sum(1 for r in res if r.status),
     }
     print(json.dumps(out))
@@ -461,6 +468,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="problems per batched step")
     ap.add_argument("--requests", type=int, default=48, help="serving requests")
     ap.add_argument("--qps", type=float, default=4.0, help="serving arrival rate")
+    ap.add_argument("--precision", default="auto", choices=["auto", "32", "64"],
+                    help="batched / serving: auto = FP32 for Laplace requests at tol >= 1e-5")
     ap.add_argument("--calibrate", action="store_true",
                     help="end with a copy kernel of known size (wbfmm_hip_stage4 over 8Mi rows: "
                          "192 MiB read, 256 MiB written) to scale HBM counters")

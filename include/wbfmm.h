@@ -374,6 +374,13 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op);
    set, keeping its rotation tables, so the set serves another wavenumber
    after new coaxial_SR_init / _SS_init calls */
 gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *op);
+/* precision of a Laplace tree's expansions, 64 (default) or 32, set
+   before the coefficients are initialised.  32: FP32 coefficients scaled
+   by each level's box width and FP32 matrix-core translations (orders
+   2..15, the same order at every level); Legendre and power recurrences
+   and the near field stay FP64, and inputs and outputs are FP64.  The
+   download returns the coefficients in the reference's scaling. */
+gint wbfmm_tree_set_precision(wbfmm_tree_t *t, gint bits);
 
 #ifdef __cplusplus
 }

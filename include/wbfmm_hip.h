@@ -217,6 +217,43 @@ int wbfmm_hip_lap_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
 /* one v_mfma_f64_16x16x4f64 with A[i][k] = 1+i+100k, B[k][j] = 1+3k+1000j;
    host_out (256 doubles) gets the four result values of each lane */
 int wbfmm_hip_mfma_layout_probe(double *host_out);
+/* the same product with v_mfma_f32_16x16x4f32 (256 floats) */
+int wbfmm_hip_mfma_layout_probe_f32(float *host_out);
+
+/* ---- FP32 Laplace passes ------------------------------------------------
+   Same work as the FP64 entry points above, with FP32 coefficients and
+   FP32 matrix-core products.  The coefficients of each level are scaled
+   by its box width w (M_n / w^n, L_n w^(n+1)) so that FP32 holds them at
+   every depth; the tables are the FP64 ones in that scaling, rounded to
+   FP32: M2L A2 from wbfmm_hip_lap_m2l_coax at w = 1, M2M / L2L A2 from
+   wbfmm_hip_lap_shift_coax_unit (the same at every level).  Legendre and
+   power recurrences (P2M, L2P) run in FP64, and so does the near field. */
+int wbfmm_hip_lap_shift_coax_unit(const wbfmm_m2l_plan_t *plan, int kind, double *A2);
+int wbfmm_hip_lap_m2l_mfma_f32(const uint32_t *cls, const unsigned *cstart,
+                               const unsigned *ccount, const uint32_t *sstart,
+                               const uint32_t *send, unsigned level, int N,
+                               const float *mps, float *mpr, const void *plan_dev,
+                               const float *A1, const float *A3, const float *A2,
+                               const float *CS, const wbfmm_m2l_plan_t *plan_host);
+int wbfmm_hip_lap_shift_mfma_f32(int mode, const uint32_t *parents, unsigned npar,
+                                 const uint32_t *cstart, const uint32_t *cend,
+                                 const uint32_t *cflag, int N, const float *in,
+                                 float *out, const void *plan_dev, const float *A1,
+                                 const float *A3, const float *A2, const float *CS,
+                                 const wbfmm_m2l_plan_t *plan_host);
+int wbfmm_hip_lap_p2m_f32(const uint32_t *boxes, unsigned nbox,
+                          const uint32_t *start, const uint32_t *end,
+                          const double *xyz, const double *q, const double *nrm,
+                          const double *dip, unsigned level, const double *x0,
+                          double D, int N, float *mps);
+int wbfmm_hip_lap_field_f32(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end,
+                            const double *xyz, const double *q, const double *nrm,
+                            const double *dip, unsigned level, const double *x0,
+                            double D, int N, const float *mpr, int field,
+                            double *f, int fstr);
+/* out[i] = (float)in[i] on the device */
+int wbfmm_hip_to_f32(const double *in, float *out, size_t n);
 /* sort a box list by Morton child digit (stable); returns the per-class
    starts and counts on the host */
 int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,

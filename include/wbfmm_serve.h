@@ -39,7 +39,9 @@ typedef struct wbfmm_serve_s wbfmm_serve_t;
    must stay valid until wbfmm_serve_wait returns for the request. */
 typedef struct {
   gint problem;          /* WBFMM_PROBLEM_LAPLACE or WBFMM_PROBLEM_HELMHOLTZ */
-  gint precision;        /* 64: FP64 throughout (0 means 64) */
+  gint precision;        /* 64: FP64 throughout; 32: FP32 expansions and
+                            translations (Laplace, orders 2..15); 0: 32
+                            where allowed and tol >= 1e-5, else 64 */
   gdouble k;             /* wavenumber (Helmholtz) */
   gdouble tol;           /* relative accuracy that picks the orders */
   guint nsrc;

@@ -259,3 +259,19 @@ def test_serve_rejects_invalid(S):
         # a valid request still runs afterwards
         f2, res = solve(S, srv, case, tol=1e-6)
         assert res.status == 0
+
+
+def test_serve_fp32_path(S, oracle):
+    # tol 1e-4 picks the FP32 path by default; with the same order the FP64
+    # path differs by rounding only, and both meet the tolerance
+    case = lap_case(81, 12000, nt=3000, dipoles=True)
+    with S.Server(2) as srv:
+        f32, r32 = solve(S, srv, case, tol=1e-4, precision=0, check=300)
+        f64, r64 = solve(S, srv, case, tol=1e-4, precision=64, check=300)
+        with pytest.raises(RuntimeError):   # FP32 is not offered for Helmholtz
+            solve(S, srv, hel_case(82, 500, 2.0), tol=1e-4, precision=32)
+    assert r32.precision == 32 and r64.precision == 64 and r32.order == r64.order
+    for c in range(4):
+        assert np.abs(f32[:, c] - f64[:, c]).max() <= 3e-5 * np.abs(f64[:, c]).max(), c
+    assert 0 < r32.err_potential < 1e-4 and 0 < r64.err_potential < 1e-4
+    assert abs(r32.err_potential - r64.err_potential) < 0.1 * r64.err_potential

@@ -211,3 +211,51 @@ def test_shift_tables_m2m_l2l(wb, oracle, N):
         fl = oracle.lap_l_eval(N, off, Lc, xp, 1)[0]
         fp = oracle.lap_l_eval(N, np.zeros(3), Lp, xp, 1)[0]
         assert abs(fl - fp) < 1e-12 * abs(fp), (c, fl, fp)
+
+
+def degree_scale(N, w, power):
+    """w^(n + power) for every packed coefficient of degree n <= N"""
+    return np.array([w ** (n + power) for n in range(N + 1) for _ in range(2 * n + 1)])
+
+
+@pytest.mark.parametrize("N", [3, 8, 12, 15])
+def test_width_scaled_tables(wb, oracle, N):
+    # The FP32 passes keep M~_n = M_n / w^n and L~_n = L_n w^(n+1) with
+    # their own tables: M2L at unit width and level-independent M2M / L2L.
+    # Applied to scaled vectors they must give the scaled results of the
+    # FP64 tables (a wrong power of 2, sign or exponent fails).
+    import ctypes
+    wc = 0.25
+    wp = 2 * wc
+    t = 0.5 * math.sqrt(3.0) * wc
+    P, A1, A3, CSu, CSd, SS, RR = shift_tables(wb, N, t)
+    L = wb.lib()
+    L.wbfmm_hip_lap_shift_coax_unit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    SSu = np.zeros((P.KT, 64))
+    RRu = np.zeros((P.KT, 64))
+    assert L.wbfmm_hip_lap_shift_coax_unit(ctypes.byref(P), 0, wb.ptr(SSu)) == 0
+    assert L.wbfmm_hip_lap_shift_coax_unit(ctypes.byref(P), 1, wb.ptr(RRu)) == 0
+    rng = np.random.default_rng(N + 90)
+    for c in (0, 3, 5, 7):
+        M = np.zeros((N + 1) ** 2)
+        off = (np.array([c & 1, (c >> 1) & 1, (c >> 2) & 1], float) - 0.5) * wc
+        for s_ in off + rng.uniform(-0.5, 0.5, (3, 3)) * wc:
+            oracle.lap_p2m(N, off, s_, rng.normal(), M)
+        Mp = emulate_shift(P, A1, A3, SS, CSu[c], (c >> 2) & 1, M)
+        Mpu = emulate_shift(P, A1, A3, SSu, CSu[c], (c >> 2) & 1, M / degree_scale(N, wc, 0))
+        assert np.abs(Mpu * degree_scale(N, wp, 0) - Mp).max() < 1e-12 * np.abs(Mp).max(), c
+        Lp = rng.normal(size=(N + 1) ** 2) / degree_scale(N, wp, 1)
+        Lc = emulate_shift(P, A1, A3, RR, CSd[c], 1 - ((c >> 2) & 1), Lp)
+        Lcu = emulate_shift(P, A1, A3, RRu, CSd[c], 1 - ((c >> 2) & 1),
+                            Lp * degree_scale(N, wp, 1))
+        assert np.abs(Lcu / degree_scale(N, wc, 1) - Lc).max() < 1e-12 * np.abs(Lc).max(), c
+    # M2L: unit-width S|R on M~ gives L~
+    w = 0.37
+    Pm, B1, B3, B2, BCS = wb.m2l_tables(N, w)
+    _, _, _, B2u, _ = wb.m2l_tables(N, 1.0)
+    for off in [(2, 0, 0), (-3, 1, 2), (3, -3, 3)]:
+        o = (off[0] + 3) * 49 + (off[1] + 3) * 7 + off[2] + 3
+        M = rng.normal(size=(N + 1) ** 2) * degree_scale(N, w, 0)
+        Lw = emulate(Pm, B1, B3, B2, BCS, M, o)
+        Lu = emulate(Pm, B1, B3, B2u, BCS, M / degree_scale(N, w, 0), o)
+        assert np.abs(Lu / degree_scale(N, w, 1) - Lw).max() < 1e-12 * np.abs(Lw).max(), off

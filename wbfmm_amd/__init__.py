@@ -127,6 +127,8 @@ _SIGS = {
     "wbfmm_hip_stage4": (_i, [_vp, _sz, _u, _vp]),
     "wbfmm_hip_lap_direct": (_i, [_vp, _vp, _vp, _vp, _u, _vp, _u, _i, _d, _vp, _i]),
     "wbfmm_hip_mfma_layout_probe": (_i, [_vp]),
+    "wbfmm_hip_mfma_layout_probe_f32": (_i, [_vp]),
+    "wbfmm_tree_set_precision": (_i, [_vp, _i]),
     "wbfmm_hip_lap_m2l_tables": (_i, [_i, _vp, _vp, _vp, _vp]),
     "wbfmm_hip_lap_m2l_coax": (_i, [_vp, _d, _vp]),
     "wbfmm_tree_coefficient_init": (_i, [_vp, _u, _u, _u]),
@@ -250,8 +252,10 @@ class LaplaceSolver:
 
     def __init__(self, xs, x0, D, depth, order_s, order_r, targets=None,
                  pstr=24, tstr=24, ntgt=None, host_mirror=True, nsrc=None,
-                 normals=None, nstr=24):
+                 normals=None, nstr=24, precision=64):
         L = lib()
+        if precision == 32:
+            host_mirror = False
         self.depth = depth
         self.nsrc = len(xs) if nsrc is None else nsrc
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
@@ -262,6 +266,8 @@ class LaplaceSolver:
             raise RuntimeError("wbfmm_tree_new failed")
         self.t = ctypes.cast(self.tree, _vp).value
         L.wbfmm_tree_host_mirror(self.t, 1 if host_mirror else 0)
+        if precision != 64:
+            L.wbfmm_tree_set_precision(self.t, precision)
         lmax = max(max(order_s[2:depth + 1]), max(order_r[2:depth + 1]))
         L.wbfmm_shift_angle_table_init()
         self.op = L.wbfmm_shift_operators_new(lmax, 0, None)
@@ -291,12 +297,16 @@ class LaplaceSolver:
             if self.tl is None:
                 self.tl = L.wbfmm_target_list_new(self.t, self.ntgt)
             L.wbfmm_target_list_add_points(self.tl, ptr(self.targets), self.tstr, self.ntgt)
+        self._fresh = True   # coefficient_init leaves every level zeroed
 
     def passes(self, q, sstr=1, d=None, dstr=1):
-        """P2M, upward pass depth..3, downward pass 2..depth."""
+        """P2M, upward pass depth..3, downward pass 2..depth (the levels
+        are zeroed first unless build_tree has just initialised them)."""
         L = lib()
-        for l in range(1, self.depth + 1):
-            L.wbfmm_tree_coefficients_zero(self.t, l)
+        if not getattr(self, "_fresh", False):
+            for l in range(1, self.depth + 1):
+                L.wbfmm_tree_coefficients_zero(self.t, l)
+        self._fresh = False
         L.wbfmm_tree_laplace_leaf_expansions(self.t, ptr(q), sstr, ptr(d), dstr, 1, None)
         for l in range(self.depth, 2, -1):
             L.wbfmm_laplace_upward_pass(self.t, self.op, l, None)
@@ -397,11 +407,14 @@ class HelmholtzSolver:
             if self.tl is None:
                 self.tl = L.wbfmm_target_list_new(self.t, self.ntgt)
             L.wbfmm_target_list_add_points(self.tl, ptr(self.targets), self.tstr, self.ntgt)
+        self._fresh = True   # coefficient_init leaves every level zeroed
 
     def passes(self, q, sstr=2, d=None, dstr=2):
         L = lib()
-        for l in range(1, self.depth + 1):
-            L.wbfmm_tree_coefficients_zero(self.t, l)
+        if not getattr(self, "_fresh", False):
+            for l in range(1, self.depth + 1):
+                L.wbfmm_tree_coefficients_zero(self.t, l)
+        self._fresh = False
         L.wbfmm_tree_leaf_expansions(self.t, self.k, ptr(q), sstr, ptr(d), dstr, 1, None)
         for l in range(self.depth, 2, -1):
             L.wbfmm_upward_pass(self.t, self.op, l, None)

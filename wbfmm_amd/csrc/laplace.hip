@@ -46,12 +46,16 @@ __device__ __forceinline__ const double *coax_row(const double *cf, int L,
 // ---------------------------------------------------------------------
 // P2M (src/laplace.c:35-129, :315-423, :854-989)
 // ---------------------------------------------------------------------
+// T: coefficient type.  rs scales distances (1 for the reference's
+// coefficients; 1/w for coefficients scaled by the box width, M_n / w^n,
+// which the FP32 passes use): the recurrences run in FP64 either way.
+template <typename T>
 __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
                           const uint32_t *start, const uint32_t *end,
                           const double *xyz, const double *q,
                           const double *nrm, const double *dip, int level,
                           double x0, double y0, double z0, double D, int N,
-                          double *mps) {
+                          double rs, T *mps) {
   unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbox) return;
   uint64_t b = boxes[t];
@@ -67,7 +71,9 @@ __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
     double r = sqrt(dx * dx + dy * dy + dz * dz), rho = sqrt(dx * dx + dy * dy);
     double Cth = r > 0.0 ? dz / r : 1.0, Sth = r > 0.0 ? rho / r : 0.0;
     double cp = rho > 0.0 ? dx / rho : 1.0, sp = rho > 0.0 ? dy / rho : 0.0;
-    double qs = q ? q[s] : 0.0, ds = dip ? dip[s] : 0.0;
+    // the dipole terms of M_{n+1} are built from R_n: one factor rs more
+    double qs = q ? q[s] : 0.0, ds = dip ? dip[s] * rs : 0.0;
+    const double rr = r * rs;
     double nx = 0, ny = 0, nz = 0;
     if (dip) {
       nx = nrm[4 * (size_t)s];
@@ -79,7 +85,7 @@ __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
     for (int m = 0; m <= N; m++) {
       if (m > 0) {
         pmm *= sqrt((2.0 * m + 1) / (2.0 * m)) * Sth;
-        rm *= r;
+        rm *= rr;
         double tcm = cm * cp - sm * sp;
         sm = sm * cp + cm * sp;
         cm = tcm;
@@ -92,7 +98,7 @@ __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
                       sqrt((2.0 * (n - 1) + 3) / (double)(n * n - m * m));
           p1 = p0;
           p0 = pn;
-          rn *= r;
+          rn *= rr;
         }
         // R_n^m(y) = rn p0 e^{i m phi}
         double Rr = rn * p0 * cm, Ri = rn * p0 * sm;
@@ -297,7 +303,8 @@ __global__ __launch_bounds__(64) void k_lap_l2l(
 // ---------------------------------------------------------------------
 // L2P + near field (src/local-laplace.c:333-630, :1091-1336)
 // ---------------------------------------------------------------------
-__device__ __forceinline__ void lcoef(const double *mpr, uint64_t b, int C,
+template <typename T>
+__device__ __forceinline__ void lcoef(const T *mpr, uint64_t b, int C,
                                       int n, int m, double &re, double &im) {
   if (m == 0) {
     re = mpr[coef_at(b, n * n, C)];
@@ -308,13 +315,16 @@ __device__ __forceinline__ void lcoef(const double *mpr, uint64_t b, int C,
   }
 }
 
+// T: coefficient type; rs as in k_lap_p2m (1/w for coefficients scaled by
+// the box width, L_n w^(n+1): the expansion then gives phi w and grad w^2)
+template <typename T>
 __global__ void k_lap_field(const double *xt, const uint32_t *tid,
                             unsigned ntgt, const uint32_t *start,
                             const uint32_t *end, const double *xyz,
                             const double *q, const double *nrm,
                             const double *dip, int level, double x0,
                             double y0, double z0, double D, int N,
-                            const double *mpr, int field, double *f,
+                            double rs, const T *mpr, int field, double *f,
                             int fstr) {
   unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntgt) return;
@@ -333,12 +343,13 @@ __global__ void k_lap_field(const double *xt, const uint32_t *tid,
   const double s4pi = 1.0 / sqrt(4.0 * kPi);
   double phi = 0.0, gz = 0.0, gpr = 0.0, gpi = 0.0;  // gp = D+ phi
   const bool want_g = (field & 2) != 0;
+  const double rr = r * rs;
   // regular solid harmonics R_j^k column by column
   double pmm = s4pi, rm = 1.0, cm = 1.0, sm = 0.0;
   for (int k = 0; k <= N; k++) {
     if (k > 0) {
       pmm *= sqrt((2.0 * k + 1) / (2.0 * k)) * Sth;
-      rm *= r;
+      rm *= rr;
       double tcm = cm * cp - sm * sp;
       sm = sm * cp + cm * sp;
       cm = tcm;
@@ -352,7 +363,7 @@ __global__ void k_lap_field(const double *xt, const uint32_t *tid,
                     sqrt((2.0 * (j - 1) + 3) / (double)(j * j - k * k));
         p1 = p0;
         p0 = pn;
-        rn *= r;
+        rn *= rr;
       }
       double Rr = rn * p0 * cm, Ri = rn * p0 * sm;
       double lr, li;
@@ -383,7 +394,9 @@ __global__ void k_lap_field(const double *xt, const uint32_t *tid,
       }
     }
   }
-  double gx = gpr, gy = gpi;
+  phi *= rs;
+  gz *= rs * rs;
+  double gx = gpr * (rs * rs), gy = gpi * (rs * rs);
   // near field over the 27 neighbours, cutoff |x-y| <= 1e-6
   const double c4 = 0.25 / kPi, cut2 = 1e-12;
   int nbox = 1 << level;
@@ -499,6 +512,11 @@ static inline unsigned nblk(size_t n, unsigned b) {
   return (unsigned)((n + b - 1) / b);
 }
 
+__global__ void k_d2f(const double *in, float *out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (float)in[i];
+}
+
 }  // namespace wbfmm
 
 using namespace wbfmm;
@@ -512,9 +530,23 @@ int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
                       double D, int N, double *mps) {
   if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
   if (nbox == 0) return 0;
-  WBFMM_LAUNCH("lap_p2m", k_lap_p2m, dim3(nblk(nbox, 64)), dim3(64), 0,
+  WBFMM_LAUNCH("lap_p2m", k_lap_p2m<double>, dim3(nblk(nbox, 64)), dim3(64), 0,
                boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0],
-               x0[1], x0[2], D, N, mps);
+               x0[1], x0[2], D, N, 1.0, mps);
+  return 0;
+}
+
+int wbfmm_hip_lap_p2m_f32(const uint32_t *boxes, unsigned nbox,
+                          const uint32_t *start, const uint32_t *end,
+                          const double *xyz, const double *q, const double *nrm,
+                          const double *dip, unsigned level, const double *x0,
+                          double D, int N, float *mps) {
+  if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
+  if (nbox == 0) return 0;
+  const double w = D / (double)(1u << level);
+  WBFMM_LAUNCH("lap_p2m_f32", k_lap_p2m<float>, dim3(nblk(nbox, 64)), dim3(64), 0,
+               boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0],
+               x0[1], x0[2], D, N, 1.0 / w, mps);
   return 0;
 }
 
@@ -590,9 +622,30 @@ int wbfmm_hip_lap_field(const double *xt, const uint32_t *tid, unsigned n,
                         double *f, int fstr) {
   if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
   if (n == 0) return 0;
-  WBFMM_LAUNCH("lap_l2p_p2p", k_lap_field, dim3(nblk(n, 128)), dim3(128), 0,
+  WBFMM_LAUNCH("lap_l2p_p2p", k_lap_field<double>, dim3(nblk(n, 128)), dim3(128), 0,
                xt, tid, n, start, end, xyz, q, nrm, dip, (int)level, x0[0],
-               x0[1], x0[2], D, N, mpr, field, f, fstr);
+               x0[1], x0[2], D, N, 1.0, mpr, field, f, fstr);
+  return 0;
+}
+
+int wbfmm_hip_lap_field_f32(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end,
+                            const double *xyz, const double *q, const double *nrm,
+                            const double *dip, unsigned level, const double *x0,
+                            double D, int N, const float *mpr, int field,
+                            double *f, int fstr) {
+  if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
+  if (n == 0) return 0;
+  const double w = D / (double)(1u << level);
+  WBFMM_LAUNCH("lap_l2p_p2p_f32", k_lap_field<float>, dim3(nblk(n, 128)), dim3(128), 0,
+               xt, tid, n, start, end, xyz, q, nrm, dip, (int)level, x0[0],
+               x0[1], x0[2], D, N, 1.0 / w, mpr, field, f, fstr);
+  return 0;
+}
+
+int wbfmm_hip_to_f32(const double *in, float *out, size_t n) {
+  if (n == 0) return 0;
+  WBFMM_LAUNCH("to_f32", k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, in, out, n);
   return 0;
 }
 

@@ -588,10 +588,17 @@ __global__ void k_mfma_layout_probe_f32(float *out) {
   for (int q = 0; q < 4; q++) out[4 * l + q] = d[q];
 }
 
+// per-class counts of a box list: a histogram in LDS per block, then one
+// global add per class and block (integer adds: the counts are exact)
 __global__ void k_class_count(const uint32_t *list, unsigned n,
                               unsigned *counts) {
-  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&counts[list[i] & 7], 1u);
+  __shared__ unsigned h[8];
+  if (threadIdx.x < 8) h[threadIdx.x] = 0u;
+  __syncthreads();
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&h[list[i] & 7], 1u);
+  __syncthreads();
+  if (threadIdx.x < 8 && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
 template <int N>
@@ -750,8 +757,9 @@ int wbfmm_hip_split_classes(const uint32_t *list, unsigned n, uint32_t *out,
   WBFMM_CHECK(rocprim::radix_sort_keys(tmp, bytes, list, out, n, 0, 3,
                                        stream()));
   WBFMM_CHECK(hipMemsetAsync(dcnt, 0, 8 * sizeof(unsigned), stream()));
-  WBFMM_LAUNCH("tree_class_count", k_class_count, dim3((n + 255) / 256),
-               dim3(256), 0, list, n, dcnt);
+  WBFMM_LAUNCH("tree_class_count", k_class_count,
+               dim3((n + 255) / 256 < 512 ? (n + 255) / 256 : 512), dim3(256), 0, list, n,
+               dcnt);
   WBFMM_CHECK(hipMemcpyAsync(ccount, dcnt, 8 * sizeof(unsigned),
                              hipMemcpyDeviceToHost, stream()));
   dfree(tmp);

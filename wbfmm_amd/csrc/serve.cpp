@@ -53,6 +53,7 @@ int laplace_order(double tol) {
 }
 
 struct Plan {
+  int precision = 64;
   int depth = 0;
   int orders[WBFMM_TREE_MAX_DEPTH + 1] = {0};
   int lmax = 0;
@@ -107,21 +108,31 @@ struct wbfmm_serve_s {
 
 namespace {
 
-std::tuple<int, int, int, int> tune_key(const wbfmm_serve_request_t &r, int order) {
+std::tuple<int, int, int, int> tune_key(const wbfmm_serve_request_t &r, int precision,
+                                        int order) {
   const unsigned n = std::max(r.nsrc, 1u);
-  return std::make_tuple(r.problem, r.precision ? r.precision : 64, order,
-                         (int)std::floor(std::log2((double)n)));
+  return std::make_tuple(r.problem, precision, order, (int)std::floor(std::log2((double)n)));
+}
+
+// FP32 (Laplace, orders 2..15) when asked for, or by default when the
+// accuracy asked for is 1e-5 or looser; FP64 otherwise
+int choose_precision(const wbfmm_serve_request_t &r, int lap_order) {
+  const bool fp32_ok = r.problem == WBFMM_PROBLEM_LAPLACE && lap_order >= 2 && lap_order <= 15;
+  if (r.precision == 32) return fp32_ok ? 32 : 0;
+  if (r.precision == 64) return 64;
+  return fp32_ok && r.order == 0 && r.tol >= 1e-5 ? 32 : 64;
 }
 
 // device bytes of a solve: dense coefficient arrays of every level (as the
 // tree allocates them), box ranges, sorted points and the target field
 size_t footprint(const wbfmm_serve_request_t &r, const Plan &p) {
   const size_t cplx = r.problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
+  const size_t es = p.precision == 32 ? 4 : 8;
   size_t b = 0;
   for (int l = 1; l <= p.depth; l++) {
     const size_t nb = (size_t)1 << (3 * l);
     const size_t C = (size_t)(p.orders[l] + 1) * (p.orders[l] + 1);
-    b += nb * (2 * C * 8 * cplx + 16);
+    b += nb * (2 * C * es * cplx + 16);
   }
   const size_t nt = r.xt ? r.ntgt : r.nsrc;
   b += (size_t)r.nsrc * (2 * 32 + 12 + 16 * cplx + (r.normals ? 32 : 0));
@@ -133,7 +144,7 @@ size_t footprint(const wbfmm_serve_request_t &r, const Plan &p) {
 int validate(const wbfmm_serve_request_t &r) {
   if (r.problem != WBFMM_PROBLEM_LAPLACE && r.problem != WBFMM_PROBLEM_HELMHOLTZ)
     return WBFMM_HIP_ERR_ARG;
-  if (r.precision != 0 && r.precision != 64) return WBFMM_HIP_ERR_ARG;
+  if (r.precision != 0 && r.precision != 32 && r.precision != 64) return WBFMM_HIP_ERR_ARG;
   if (r.nsrc == 0 || !r.xs || r.pstr < 3 * sizeof(double)) return WBFMM_HIP_ERR_ARG;
   if (!r.q && !r.dip) return WBFMM_HIP_ERR_ARG;
   if (r.dip && (!r.normals || r.nstr < 3 * sizeof(double))) return WBFMM_HIP_ERR_ARG;
@@ -182,11 +193,14 @@ int make_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t &r, Plan &p) {
 
   const int order_fixed = r.order;
   const int lap_order = order_fixed ? order_fixed : laplace_order(r.tol);
+  p.precision = choose_precision(r, lap_order);
+  if (p.precision == 0) return WBFMM_HIP_ERR_ARG;
   if (r.depth) {
     p.depth = r.depth;
   } else {
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->tuned.find(tune_key(r, r.problem == WBFMM_PROBLEM_LAPLACE ? lap_order : order_fixed));
+    auto it = s->tuned.find(
+        tune_key(r, p.precision, r.problem == WBFMM_PROBLEM_LAPLACE ? lap_order : order_fixed));
     p.depth = it != s->tuned.end() ? it->second : default_depth(r);
   }
   p.lmax = 0;
@@ -362,6 +376,10 @@ void run(wbfmm_serve_t *s, Worker &w, Job &j) {
 
   double x0[3] = {p.x0[0], p.x0[1], p.x0[2]};
   wbfmm_tree_t *t = wbfmm_tree_new(x0, p.D, r.nsrc);
+  // the engine's trees are private: no host box arrays (8^l entries per
+  // level), which FP32 coefficients do not allow anyway
+  wbfmm_tree_host_mirror(t, FALSE);
+  if (p.precision == 32) wbfmm_tree_set_precision(t, 32);
   wbfmm_tree_add_points(t, (gpointer)r.xs, r.pstr, (gpointer)r.normals,
                         r.normals ? r.nstr : 0, r.nsrc, FALSE);
   for (int l = 0; l < p.depth; l++) wbfmm_tree_refine(t);
@@ -486,7 +504,7 @@ int64_t enqueue(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, bool tune) {
   if (rc) return rc;
   j->res.depth = j->plan.depth;
   j->res.order = j->plan.lmax;
-  j->res.precision = 64;
+  j->res.precision = j->plan.precision;
   j->res.footprint = j->plan.footprint;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return WBFMM_HIP_ERR_ARG;
@@ -643,12 +661,15 @@ gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gi
   wbfmm_hip_free(scratch);
   if (rc) return rc;
   {
+    Plan p;
+    rc = make_plan(s, *r, p);
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(s->mu);
     const int key_order = r->problem == WBFMM_PROBLEM_LAPLACE
                               ? (r->order ? r->order : laplace_order(r->tol))
                               : r->order;
     (void)order;
-    s->tuned[tune_key(*r, key_order)] = best_d;
+    s->tuned[tune_key(*r, p.precision, key_order)] = best_d;
   }
   if (best) *best = best_d;
   return 0;

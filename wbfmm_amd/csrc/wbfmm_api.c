@@ -66,6 +66,7 @@ typedef struct {
   size_t mps_bytes[WBFMM_TREE_MAX_DEPTH + 1], mpr_bytes[WBFMM_TREE_MAX_DEPTH + 1];
   double *d_q, *d_dip; /* strengths in sorted order */
   unsigned cap_q, cap_dip;
+  int prec32; /* FP32 coefficients scaled by the box width (Laplace) */
 } tree_dev;
 
 typedef struct {
@@ -98,6 +99,14 @@ typedef struct {
   /* MFMA M2M / L2L rotation tiles and child-direction phases per order */
   double *sh_A1[WBFMM_HIP_MAX_ORDER + 1], *sh_A3[WBFMM_HIP_MAX_ORDER + 1];
   double *sh_CSup[WBFMM_HIP_MAX_ORDER + 1], *sh_CSdn[WBFMM_HIP_MAX_ORDER + 1];
+  /* FP32 tables per order (coefficients scaled by the box width): M2L
+     A1, A3, CS and unit-width A2; M2M / L2L A1, A3, CSup, CSdn and the
+     level-independent A2 of each direction */
+  float *f_A1[WBFMM_HIP_MAX_ORDER + 1], *f_A3[WBFMM_HIP_MAX_ORDER + 1],
+      *f_CS[WBFMM_HIP_MAX_ORDER + 1], *f_A2[WBFMM_HIP_MAX_ORDER + 1];
+  float *f_shA1[WBFMM_HIP_MAX_ORDER + 1], *f_shA3[WBFMM_HIP_MAX_ORDER + 1],
+      *f_CSup[WBFMM_HIP_MAX_ORDER + 1], *f_CSdn[WBFMM_HIP_MAX_ORDER + 1],
+      *f_A2up[WBFMM_HIP_MAX_ORDER + 1], *f_A2dn[WBFMM_HIP_MAX_ORDER + 1];
   /* Helmholtz: wavenumber of the coaxial tables, their orders per level
      (op.L for S|R, ss_L for R|R), H tables of the 49 list-4 polar angles
      and of the two child directions (order op.Lmax), offset geometry */
@@ -503,17 +512,37 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
   tree_dev *td = TD(t);
   void *src = which ? t->mpr[level] : t->mps[level];
   size_t bytes = which ? td->mpr_bytes[level] : td->mps_bytes[level];
-  size_t nb = (size_t)1 << (3 * level), C, g, c, k;
+  size_t nb = (size_t)1 << (3 * level), C, g, c, k, es = td->prec32 ? sizeof(float) : sizeof(double);
   double *tmp;
   if (!src) return -1;
-  C = bytes / sizeof(double) / nb;
-  tmp = malloc(bytes);
+  C = bytes / es / nb;
+  tmp = malloc(nb * C * sizeof(double));
   if (!tmp) WBFMM_FATAL("out of host memory");
-  HIPOK(wbfmm_hip_memcpy(tmp, src, bytes));
+  if (td->prec32) {
+    /* FP32 coefficients scaled by the box width w: M_n = w^n M~_n,
+       L_n = L~_n / w^(n+1) */
+    float *ft = malloc(bytes);
+    double w = t->D / (double)(1u << level), sc[WBFMM_HIP_MAX_ORDER + 2];
+    int n;
+    if (!ft) WBFMM_FATAL("out of host memory");
+    HIPOK(wbfmm_hip_memcpy(ft, src, bytes));
+    for (n = 0; n <= WBFMM_HIP_MAX_ORDER + 1; n++)
+      sc[n] = which ? 1.0 / pow(w, n + 1) : pow(w, n);
+    for (g = 0; g < nb; g++)
+      for (c = 0; c < C; c++) {
+        n = (int)sqrt((double)c);
+        while ((size_t)(n + 1) * (n + 1) <= c) n++;
+        while ((size_t)n * n > c) n--;
+        tmp[g * C + c] = (double)ft[g * C + c] * sc[n];
+      }
+    free(ft);
+  } else {
+    HIPOK(wbfmm_hip_memcpy(tmp, src, bytes));
+  }
   /* device box-major -> reference groups of 8 siblings
      (src/boxes.c:552-579): [group][coefficient][child]; level 0 has a
      single box */
-  if (nb < 8) memcpy(host, tmp, bytes);
+  if (nb < 8) memcpy(host, tmp, nb * C * sizeof(double));
   else if (t->problem == WBFMM_PROBLEM_HELMHOLTZ) {
     /* complex coefficients: (re, im) pairs interleaved by child */
     size_t Cc = C / 2;
@@ -529,6 +558,18 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
         for (c = 0; c < C; c++)
           host[(g * C + c) * 8 + k] = tmp[(8 * g + k) * C + c];
   free(tmp);
+  return 0;
+}
+
+gint wbfmm_tree_set_precision(wbfmm_tree_t *t, gint bits)
+{
+  tree_dev *td = TD(t);
+  int l;
+  if (bits != 32 && bits != 64) WBFMM_FATAL("precision %d is not 32 or 64", bits);
+  for (l = 0; l <= WBFMM_TREE_MAX_DEPTH; l++)
+    if (t->mps[l] || t->mpr[l])
+      WBFMM_FATAL("set the precision before the coefficients are initialised");
+  td->prec32 = bits == 32;
   return 0;
 }
 
@@ -573,15 +614,20 @@ gint wbfmm_tree_laplace_coefficient_init(wbfmm_tree_t *t, guint l, guint nr,
   if (l > t->depth) WBFMM_FATAL("level %u below tree depth %u", l, t->depth);
   if (ns > WBFMM_HIP_MAX_ORDER || nr > WBFMM_HIP_MAX_ORDER)
     WBFMM_FATAL("order above %d", WBFMM_HIP_MAX_ORDER);
+  if (td->prec32 && td->host_mirror)
+    WBFMM_FATAL("FP32 coefficients have no host box pointers (use the download)");
   t->order_s[l] = ns;
   t->order_r[l] = nr;
-  if (ns) {
-    alloc_level(td, l, 0, nb * (ns + 1) * (ns + 1) * sizeof(double));
-    if (td->host_mirror) set_box_pointers(td, l, 0, (ns + 1) * (ns + 1));
-  }
-  if (nr) {
-    alloc_level(td, l, 1, nb * (nr + 1) * (nr + 1) * sizeof(double));
-    if (td->host_mirror) set_box_pointers(td, l, 1, (nr + 1) * (nr + 1));
+  {
+    const size_t es = td->prec32 ? sizeof(float) : sizeof(double);
+    if (ns) {
+      alloc_level(td, l, 0, nb * (ns + 1) * (ns + 1) * es);
+      if (td->host_mirror) set_box_pointers(td, l, 0, (ns + 1) * (ns + 1));
+    }
+    if (nr) {
+      alloc_level(td, l, 1, nb * (nr + 1) * (nr + 1) * es);
+      if (td->host_mirror) set_box_pointers(td, l, 1, (nr + 1) * (nr + 1));
+    }
   }
   return 0;
 }
@@ -629,10 +675,16 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
     WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
   q = gather_strengths(td, src, sstr, 0);
   dp = gather_strengths(td, dipoles, dstr, 1);
-  HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
-                          td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                          dp, d, (double *)wbfmm_tree_origin(t), t->D,
-                          t->order_s[d], (double *)t->mps[d]));
+  if (td->prec32)
+    HIPOK(wbfmm_hip_lap_p2m_f32(td->d_occ[d], td->nocc[d], td->d_start[d],
+                                td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
+                                dp, d, (double *)wbfmm_tree_origin(t), t->D,
+                                t->order_s[d], (float *)t->mps[d]));
+  else
+    HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
+                            td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
+                            dp, d, (double *)wbfmm_tree_origin(t), t->D,
+                            t->order_s[d], (double *)t->mps[d]));
   return 0;
 }
 
@@ -694,6 +746,16 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     wbfmm_hip_free(od->sh_A3[n]);
     wbfmm_hip_free(od->sh_CSup[n]);
     wbfmm_hip_free(od->sh_CSdn[n]);
+    wbfmm_hip_free(od->f_A1[n]);
+    wbfmm_hip_free(od->f_A3[n]);
+    wbfmm_hip_free(od->f_CS[n]);
+    wbfmm_hip_free(od->f_A2[n]);
+    wbfmm_hip_free(od->f_shA1[n]);
+    wbfmm_hip_free(od->f_shA3[n]);
+    wbfmm_hip_free(od->f_CSup[n]);
+    wbfmm_hip_free(od->f_CSdn[n]);
+    wbfmm_hip_free(od->f_A2up[n]);
+    wbfmm_hip_free(od->f_A2dn[n]);
     free(od->m2l_plan[n]);
   }
   for (n = 0; n < od->m2l_nA2; n++) wbfmm_hip_free(od->m2l_A2[n]);
@@ -787,6 +849,60 @@ static void ensure_shift_tables(ops_dev *od, int N)
   free(cd);
 }
 
+/* device FP32 copy of n device doubles */
+static float *dev_f32(const double *d, size_t n)
+{
+  float *f = NULL;
+  HIPOK(wbfmm_hip_malloc((void **)&f, sizeof(float) * n));
+  HIPOK(wbfmm_hip_to_f32(d, f, n));
+  return f;
+}
+
+/* device FP32 copy of n host doubles */
+static float *host_f32(const double *h, size_t n)
+{
+  float *f = NULL, *t = malloc(sizeof(float) * n);
+  size_t i;
+  if (!t) WBFMM_FATAL("out of host memory");
+  for (i = 0; i < n; i++) t[i] = (float)h[i];
+  HIPOK(wbfmm_hip_malloc((void **)&f, sizeof(float) * n));
+  HIPOK(wbfmm_hip_memcpy(f, t, sizeof(float) * n));
+  free(t);
+  return f;
+}
+
+/* FP32 tables of order N for coefficients scaled by the box width */
+static void ensure_f32_tables(ops_dev *od, int N)
+{
+  const wbfmm_m2l_plan_t *P;
+  size_t n13, ncs, na2;
+  double *h;
+  if (od->f_A1[N]) return;
+  ensure_shift_tables(od, N); /* also the M2L plan */
+  P = od->m2l_plan[N];
+  n13 = (size_t)P->NTH * P->KT * 64;
+  ncs = (size_t)343 * (N + 1) * 2;
+  od->f_A1[N] = dev_f32(od->m2l_A1[N], n13);
+  od->f_A3[N] = dev_f32(od->m2l_A3[N], n13);
+  od->f_CS[N] = dev_f32(od->m2l_CS[N], ncs);
+  od->f_shA1[N] = dev_f32(od->sh_A1[N], (size_t)2 * P->KT * 64);
+  od->f_shA3[N] = dev_f32(od->sh_A3[N], (size_t)2 * P->KT * 64);
+  od->f_CSup[N] = dev_f32(od->sh_CSup[N], (size_t)8 * (N + 1) * 2);
+  od->f_CSdn[N] = dev_f32(od->sh_CSdn[N], (size_t)8 * (N + 1) * 2);
+  na2 = (size_t)P->NR * P->KT * 64;
+  h = malloc(sizeof(double) * na2);
+  if (!h) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_lap_m2l_coax(P, 1.0, h));
+  od->f_A2[N] = host_f32(h, na2);
+  HIPOK(wbfmm_hip_lap_shift_coax_unit(P, 0, h));
+  od->f_A2up[N] = host_f32(h, (size_t)P->KT * 64);
+  HIPOK(wbfmm_hip_lap_shift_coax_unit(P, 1, h));
+  od->f_A2dn[N] = host_f32(h, (size_t)P->KT * 64);
+  free(h);
+}
+
+static int mfma_order(int n) { return n >= M2L_MFMA_MIN && n <= M2L_MFMA_MAX; }
+
 gint wbfmm_laplace_coaxial_translate_init(gint N)
 {
   if (N > g_lap_coax_N) g_lap_coax_N = N;
@@ -810,6 +926,18 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if ((int)t->order_s[level] > od->lay.L || (int)t->order_s[level - 1] > od->lay.L)
     WBFMM_FATAL("operators built to order %d, tree needs %u", od->lay.L,
                 t->order_s[level] > t->order_s[level - 1] ? t->order_s[level] : t->order_s[level - 1]);
+  if (td->prec32) {
+    int N = (int)t->order_s[level];
+    if (t->order_s[level - 1] != t->order_s[level] || !mfma_order(N))
+      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, M2L_MFMA_MAX);
+    ensure_f32_tables(od, N);
+    HIPOK(wbfmm_hip_lap_shift_mfma_f32(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                       td->d_start[level], td->d_end[level], NULL, N,
+                                       (const float *)t->mps[level], (float *)t->mps[level - 1],
+                                       od->m2l_plan_dev[N], od->f_shA1[N], od->f_shA3[N],
+                                       od->f_A2up[N], od->f_CSup[N], od->m2l_plan[N]));
+    return 0;
+  }
   if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
       (int)t->order_s[level] <= M2L_MFMA_MAX) {
     /* matrix-core form: 16 parents per wavefront */
@@ -878,13 +1006,38 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     WBFMM_FATAL("coefficients not initialised at level %u", level);
   if ((int)t->order_s[level] > od->lay.L || (int)t->order_r[level] > od->lay.L)
     WBFMM_FATAL("operators built to order %d", od->lay.L);
-  ensure_lap_coax(od);
   if (td->have_targets) {
     tl = td->d_act[level];
     nt = td->nact[level];
   } else {
     tl = all_boxes(level, &nt);
   }
+  if (td->prec32) {
+    int N = (int)t->order_s[level];
+    unsigned cstart[8], ccount[8];
+    uint32_t *cls;
+    if (t->order_r[level] != t->order_s[level] || !mfma_order(N) ||
+        (level < t->depth && t->order_r[level + 1] != t->order_r[level]))
+      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, M2L_MFMA_MAX);
+    ensure_f32_tables(od, N);
+    HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
+    HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
+    HIPOK(wbfmm_hip_lap_m2l_mfma_f32(cls, cstart, ccount, td->d_start[level],
+                                     td->d_end[level], level, N,
+                                     (const float *)t->mps[level], (float *)t->mpr[level],
+                                     od->m2l_plan_dev[N], od->f_A1[N], od->f_A3[N],
+                                     od->f_A2[N], od->f_CS[N], od->m2l_plan[N]));
+    wbfmm_hip_free(cls);
+    if (level == t->depth) return 0;
+    if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
+    HIPOK(wbfmm_hip_lap_shift_mfma_f32(1, tl, nt, NULL, NULL,
+                                       td->have_targets ? td->d_flag[level + 1] : NULL, N,
+                                       (const float *)t->mpr[level], (float *)t->mpr[level + 1],
+                                       od->m2l_plan_dev[N], od->f_shA1[N], od->f_shA3[N],
+                                       od->f_A2dn[N], od->f_CSdn[N], od->m2l_plan[N]));
+    return 0;
+  }
+  ensure_lap_coax(od);
   if (t->order_s[level] == t->order_r[level] &&
       (int)t->order_s[level] >= M2L_MFMA_MIN && (int)t->order_s[level] <= M2L_MFMA_MAX) {
     /* matrix-core form: targets grouped by Morton child digit */
@@ -1008,10 +1161,16 @@ static void lap_field_run(tree_dev *td, const double *dxt,
   dp = gather_strengths(td, d, dstr, 1);
   nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((field & 1) ? 1 : 0) + ((field & 2) ? 3 : 0) : 0;
   df = field_begin(f, nelem, &staged);
-  HIPOK(wbfmm_hip_lap_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
-                            td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
-                            (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
-                            (double *)t->mpr[lv], field, df, fstr));
+  if (td->prec32)
+    HIPOK(wbfmm_hip_lap_field_f32(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
+                                  td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
+                                  (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
+                                  (const float *)t->mpr[lv], field, df, fstr));
+  else
+    HIPOK(wbfmm_hip_lap_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
+                              td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
+                              (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
+                              (double *)t->mpr[lv], field, df, fstr));
   field_finish(f, df, nelem, staged);
 }
 
@@ -1300,6 +1459,7 @@ gint wbfmm_hip_synchronize(void) { return wbfmm_hip_sync(); }
    complex per box (box-major on the device) */
 gint wbfmm_tree_coefficient_init(wbfmm_tree_t *t, guint l, guint nr, guint ns)
 {
+  if (TD(t)->prec32) WBFMM_FATAL("FP32 coefficients are available for Laplace trees only");
   tree_dev *td = TD(t);
   size_t nb = (size_t)1 << (3 * l);
   if ((int)t->nq < 1)

@@ -707,7 +707,31 @@ int wbfmm_hip_lap_shift_tables(const wbfmm_m2l_plan_t *P, double *A1, double *A3
 /* stage 2 of the M2M (kind 0: S|S, c (-t)^(n-nu), nu <= n) or L2L (kind 1:
    R|R, c t^(nu-n), nu >= n) at distance t = sqrt(3)/2 w_child, same block
    for both parts (src/laplace.c:542-640) */
+/* a^n b^nu scaling of the shift entries (1, 1 for the reference's
+   coefficients) */
+static int shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, double a, double b,
+                      double *A2);
+
 int wbfmm_hip_lap_shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, double *A2)
+{
+  return shift_coax(P, kind, t, 1.0, 1.0, A2);
+}
+
+/* Coefficients scaled by the box width (M_n / w^n, L_n w^(n+1)): with
+   t = sqrt(3)/2 w_child and w_parent = 2 w_child the shifts no longer
+   depend on the level,
+     M2M  c0 (-sqrt3/2)^(n-nu) 2^-n       (M~_parent from M~_child)
+     L2L  c1 (sqrt3/2)^(nu-n) 2^-(nu+1)   (L~_child from L~_parent) */
+int wbfmm_hip_lap_shift_coax_unit(const wbfmm_m2l_plan_t *P, int kind, double *A2)
+{
+  const double t = 0.5 * sqrt(3.0);
+  if (kind == 0) return shift_coax(P, 0, t, 0.5, 1.0, A2);
+  if (kind == 1) return shift_coax(P, 1, t, 1.0, 0.5, A2);
+  return WBFMM_HIP_ERR_ARG;
+}
+
+static int shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, double a, double b,
+                      double *A2)
 {
   packing p2;
   int N = P->N, S = 16 * P->NB, m, n, nu, part;
@@ -725,6 +749,8 @@ int wbfmm_hip_lap_shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, doub
           if (kind == 1 && nu < n) continue;
           v = kind == 0 ? lap_cfft(0, n, nu, m) * pow(-t, (double)(n - nu))
                         : lap_cfft(1, n, nu, m) * pow(t, (double)(nu - n));
+          if (a != 1.0) v *= pow(a, (double)n);
+          if (b != 1.0) v *= pow(b, (double)(nu + (kind == 1)));
           T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] = v;
         }
   put_tiles(A2, T, P);

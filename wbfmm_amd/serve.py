@@ -87,12 +87,12 @@ def field_width(problem, field):
 
 def make_request(problem, xs, q=None, xt=None, k=0.0, tol=1e-6, field=3, f=None, fstr=None,
                  normals=None, dip=None, depth=0, order=0, check=0, nsrc=None, ntgt=None,
-                 pstr=24, tstr=24, nstr=24):
+                 pstr=24, tstr=24, nstr=24, precision=64):
     """Request over numpy arrays (host) or torch tensors (device); f is
     the output array (accumulated).  Keep the arrays alive until wait."""
     r = Request()
     r.problem = problem
-    r.precision = 64
+    r.precision = precision
     r.k = k
     r.tol = tol
     r.nsrc = len(xs) if nsrc is None else nsrc
