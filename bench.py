@@ -415,6 +415,11 @@ def run_serving(args):
     st = srv.stats()
     srv.close()
     _release(reqs)
+    for p, r in zip(reqs, res):
+        print(f"bench: request n={p['n']} {'helm k=%.1f' % p['k'] if p['helm'] else 'lap'} "
+              f"tol={p['tol']:g} fp{r.precision} depth={r.depth} order={r.order} "
+              f"queue={r.queue_ms:.1f} tables={r.tables_ms:.1f} solve={r.solve_ms:.1f} "
+              f"err={r.err_potential:.2e}", file=sys.stderr)
     ntot = sum(p["n"] for p in reqs)
     lat = np.array([r.latency_ms for r in res])
     per_tol = {}

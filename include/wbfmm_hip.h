@@ -325,6 +325,9 @@ int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
    kr into host E: 2 doubles (re, im) per entry l(l+1)(l+2)/6 + n(n+1)/2 +
    m; kind 0 R|R (= S|S), kind 1 S|R (src/translations.c:205-409) */
 int wbfmm_hip_hel_coaxial_table(int kind, int L, double kr, double *E);
+/* n tables at kr[0..n-1] on host threads, table i at E + i * 2 ne where
+   ne = (L+1)(L+2)(L+3)/6 */
+int wbfmm_hip_hel_coaxial_tables(int kind, int L, int n, const double *kr, double *E);
 
 /* ---- Helmholtz passes (FP64, complex coefficients, box-major) ---------- */
 /* P2M into mps at `level` for the listed leaf boxes; q, dip: complex

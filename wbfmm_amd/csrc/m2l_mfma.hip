@@ -166,7 +166,8 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
     const T *__restrict__ mps, T *__restrict__ mpr,
     const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A1,
     const T *__restrict__ A3, const T *__restrict__ A2,
-    const T *__restrict__ CS, const T *__restrict__ zero) {
+    const T *__restrict__ CS, const T *__restrict__ zero, int nsplit,
+    T *__restrict__ part, unsigned ntot) {
   typedef typename MF<T>::t2 T2;
   typedef typename MF<T>::v4 V4;
   using SH = M2LShape<N>;
@@ -208,7 +209,11 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   for (int i = lane; i < 16 * 7; i += 64) vm[i / 7][i % 7] = 0u;
 
   // block -> chunk of 16 targets; consecutive chunks stay on one XCD
-  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x, q8 = nch / 8;
+  // nsplit > 1 (levels with few targets): nsplit blocks share a chunk,
+  // block `split` taking the cube indices k = split (mod nsplit), and write
+  // partial sums that k_m2l_reduce adds in a fixed order
+  const int split = (int)(blockIdx.x % (unsigned)nsplit);
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
   const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
   int c = 0;
   while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 
   auto next_k = [&](int k) {
     for (k++; k < 216; k++)
-      if ((anym[k >> 5] >> (k & 31)) & 1u) break;
+      if (((anym[k >> 5] >> (k & 31)) & 1u) && k % nsplit == split) break;
     return k;
   };
   auto offset_of = [&](int k) {
@@ -376,6 +381,20 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   }
 
   if (!tvalid) return;
+  if (nsplit > 1) {
+    // every coefficient of the partial is written (m = 0 has no im part)
+    T *dst = part + ((size_t)split * ntot + ca.start[c] + k16) * C;
+#pragma unroll
+    for (int i = 0; i < NAU; i++) {
+      const int u = grp + 4 * i;
+      if (u < NU) {
+        const int4 inf = uinf[u];
+        dst[inf.x] = acc_re[i];
+        if (inf.y) dst[inf.x + 1] = acc_im[i];
+      }
+    }
+    return;
+  }
   T *out = mpr + (size_t)tb * C;
 #pragma unroll
   for (int i = 0; i < NAU; i++) {
@@ -386,6 +405,19 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
       if (inf.y) out[inf.x + 1] += acc_im[i];
     }
   }
+}
+
+// mpr[cls[i]] += sum over the splits (in split order) of the partials of
+// target i
+template <typename T>
+__global__ void k_m2l_reduce(const uint32_t *__restrict__ cls, unsigned ntot, int nsplit,
+                             int C, const T *__restrict__ part, T *__restrict__ mpr) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)ntot * C) return;
+  const size_t i = e / C, c = e % C;
+  T s = part[i * C + c];
+  for (int sp = 1; sp < nsplit; sp++) s += part[((size_t)sp * ntot + i) * C + c];
+  mpr[(size_t)cls[i] * C + c] += s;
 }
 
 // ---------------------------------------------------------------------
@@ -620,10 +652,28 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
     fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
+  // Few chunks leave most of the chip idle while each wavefront works
+  // through ~189 offsets in series: split the offsets over up to 8 blocks
+  // per chunk.  The split depends on the chunk count only, so a given
+  // problem always sums in the same order.
+  constexpr int C = (N + 1) * (N + 1);
+  unsigned ntot = 0;
+  for (int c = 0; c < 8; c++) ntot += ca.count[c];
+  const int nsplit = nchunks >= 1024 ? 1 : (int)(2048 / nchunks < 8 ? 2048 / nchunks : 8);
+  T *part = nullptr;
+  if (nsplit > 1 &&
+      dmalloc((void **)&part, sizeof(T) * (size_t)nsplit * ntot * C))
+    return WBFMM_HIP_ERR_NOMEM;
   WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
-               dim3(nchunks), dim3(64), 0, cls,
+               dim3(nchunks * nsplit), dim3(64), 0, cls,
                ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
-               A1, A3, A2, CS, zero);
+               A1, A3, A2, CS, zero, nsplit, part, ntot);
+  if (nsplit > 1) {
+    const size_t ne = (size_t)ntot * C;
+    WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<T>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
+                 cls, ntot, nsplit, C, (const T *)part, mpr);
+    dfree(part);
+  }
   return 0;
 }
 

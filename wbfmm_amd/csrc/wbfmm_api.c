@@ -717,16 +717,41 @@ wbfmm_shift_operators_t *wbfmm_shift_operators_new(guint L, gboolean bw,
   od->op.size = sizeof(double);
   od->op.Lmax = L;
   od->op.bw = bw;
-  HIPOK(wbfmm_hip_rotation_tables((int)L, &od->d_rot, &od->lay));
+  /* the rotation tables of the vector passes (718 directions, 2.3 MB per
+     1000 entries of the order's rotation size) are built on first use:
+     the matrix-core passes and the Helmholtz passes have their own */
+  od->lay.L = (int)L;
+  od->lay.nrot = (size_t)(L + 1) * (2 * L + 1) * (2 * L + 3) / 3;
   od->op.nerot = (guint)od->lay.nrot;
-  od->op.rotations = od->d_rot;
+  od->op.rotations = NULL;
   return &od->op;
 }
 
+/* Tables that depend on the order only are built once per process, by the
+   first operator set that needs them, and shared by every operator set
+   until the process ends: the Laplace matrix-core plans and tiles (FP64
+   and FP32) per order, and the rotation tables and coaxial closed forms
+   of the Laplace vector passes per largest order.  They are complete on
+   the device before they are shared. */
+static pthread_mutex_t g_tab_mu = PTHREAD_MUTEX_INITIALIZER;
+static double *g_vec_rot[WBFMM_HIP_MAX_ORDER + 1], *g_vec_coax[WBFMM_HIP_MAX_ORDER + 1];
+static wbfmm_rot_layout_t g_vec_lay[WBFMM_HIP_MAX_ORDER + 1];
+
+/* rotation tables and coaxial closed forms of the Laplace vector passes */
 static void ensure_lap_coax(ops_dev *od)
 {
-  if (!od->d_lap_coax)
-    HIPOK(wbfmm_hip_laplace_coaxial_tables(od->lay.L, &od->d_lap_coax));
+  int L = od->lay.L;
+  if (od->d_rot) return;
+  pthread_mutex_lock(&g_tab_mu);
+  if (!g_vec_rot[L]) {
+    HIPOK(wbfmm_hip_rotation_tables(L, &g_vec_rot[L], &g_vec_lay[L]));
+    HIPOK(wbfmm_hip_laplace_coaxial_tables(L, &g_vec_coax[L]));
+  }
+  od->d_rot = g_vec_rot[L];
+  od->lay = g_vec_lay[L];
+  od->d_lap_coax = g_vec_coax[L];
+  od->op.rotations = od->d_rot;
+  pthread_mutex_unlock(&g_tab_mu);
 }
 
 gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
@@ -734,45 +759,21 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
   ops_dev *od = (ops_dev *)op;
   int n;
   if (!op) return 0;
-  wbfmm_hip_free(od->d_rot);
-  wbfmm_hip_free(od->d_lap_coax);
-  for (n = 0; n <= WBFMM_HIP_MAX_ORDER; n++) {
-    if (!od->m2l_have[n]) continue;
-    wbfmm_hip_free(od->m2l_plan_dev[n]);
-    wbfmm_hip_free(od->m2l_A1[n]);
-    wbfmm_hip_free(od->m2l_A3[n]);
-    wbfmm_hip_free(od->m2l_CS[n]);
-    wbfmm_hip_free(od->sh_A1[n]);
-    wbfmm_hip_free(od->sh_A3[n]);
-    wbfmm_hip_free(od->sh_CSup[n]);
-    wbfmm_hip_free(od->sh_CSdn[n]);
-    wbfmm_hip_free(od->f_A1[n]);
-    wbfmm_hip_free(od->f_A3[n]);
-    wbfmm_hip_free(od->f_CS[n]);
-    wbfmm_hip_free(od->f_A2[n]);
-    wbfmm_hip_free(od->f_shA1[n]);
-    wbfmm_hip_free(od->f_shA3[n]);
-    wbfmm_hip_free(od->f_CSup[n]);
-    wbfmm_hip_free(od->f_CSdn[n]);
-    wbfmm_hip_free(od->f_A2up[n]);
-    wbfmm_hip_free(od->f_A2dn[n]);
-    free(od->m2l_plan[n]);
-  }
+  /* the per-order tables are shared with other operator sets: not freed */
   for (n = 0; n < od->m2l_nA2; n++) wbfmm_hip_free(od->m2l_A2[n]);
   for (n = 0; n <= WBFMM_TREE_MAX_DEPTH; n++) {
     wbfmm_hip_free(od->op.SR[n]);
     wbfmm_hip_free(od->op.SS[n]);
   }
-  wbfmm_hip_free(od->d_hel_Hth);
-  wbfmm_hip_free(od->d_hel_Hud);
-  wbfmm_hip_free(od->d_th_idx);
-  wbfmm_hip_free(od->d_rix);
-  wbfmm_hip_free(od->d_chi);
+  /* d_hel_* point to the process-wide tables of the order: not freed */
   free(od);
   return 0;
 }
 
-static void ensure_m2l_plan(ops_dev *od, int N)
+/* the shared per-order tables live in this set's fields */
+static ops_dev g_tab;
+
+static void build_m2l_plan(ops_dev *od, int N)
 {
   if (od->m2l_have[N]) return;
   od->m2l_plan[N] = calloc(1, sizeof(wbfmm_m2l_plan_t));
@@ -780,6 +781,46 @@ static void ensure_m2l_plan(ops_dev *od, int N)
                                &od->m2l_A1[N], &od->m2l_A3[N], &od->m2l_CS[N]));
   od->m2l_have[N] = 1;
 }
+
+static void build_shift_tables(ops_dev *od, int N);
+static void build_f32_tables(ops_dev *od, int N);
+
+/* every matrix-core table of order N (FP64 M2L plan and tiles, M2M / L2L
+   tiles, FP32 copies) into od, from the shared set */
+static void ensure_order_tables(ops_dev *od, int N)
+{
+  if (od->m2l_have[N]) return;
+  pthread_mutex_lock(&g_tab_mu);
+  if (!g_tab.f_A1[N]) {
+    build_m2l_plan(&g_tab, N);
+    build_shift_tables(&g_tab, N);
+    build_f32_tables(&g_tab, N);
+    HIPOK(wbfmm_hip_sync()); /* conversions queued on this thread's stream */
+  }
+  od->m2l_plan[N] = g_tab.m2l_plan[N];
+  od->m2l_plan_dev[N] = g_tab.m2l_plan_dev[N];
+  od->m2l_A1[N] = g_tab.m2l_A1[N];
+  od->m2l_A3[N] = g_tab.m2l_A3[N];
+  od->m2l_CS[N] = g_tab.m2l_CS[N];
+  od->sh_A1[N] = g_tab.sh_A1[N];
+  od->sh_A3[N] = g_tab.sh_A3[N];
+  od->sh_CSup[N] = g_tab.sh_CSup[N];
+  od->sh_CSdn[N] = g_tab.sh_CSdn[N];
+  od->f_A1[N] = g_tab.f_A1[N];
+  od->f_A3[N] = g_tab.f_A3[N];
+  od->f_CS[N] = g_tab.f_CS[N];
+  od->f_A2[N] = g_tab.f_A2[N];
+  od->f_shA1[N] = g_tab.f_shA1[N];
+  od->f_shA3[N] = g_tab.f_shA3[N];
+  od->f_CSup[N] = g_tab.f_CSup[N];
+  od->f_CSdn[N] = g_tab.f_CSdn[N];
+  od->f_A2up[N] = g_tab.f_A2up[N];
+  od->f_A2dn[N] = g_tab.f_A2dn[N];
+  od->m2l_have[N] = 1;
+  pthread_mutex_unlock(&g_tab_mu);
+}
+
+static void ensure_m2l_plan(ops_dev *od, int N) { ensure_order_tables(od, N); }
 
 /* device coaxial tiles of the MFMA passes for order N: kind 2 S|R at the
    15 list-4 lengths for box width w (M2L), kind 0 S|S and kind 1 R|R at
@@ -818,13 +859,14 @@ static const double *mfma_coax(ops_dev *od, int N, int kind, double w)
 
 static const double *m2l_coax(ops_dev *od, int N, double w) { return mfma_coax(od, N, 2, w); }
 
+static void ensure_shift_tables(ops_dev *od, int N) { ensure_order_tables(od, N); }
+
 /* rotation tiles and phases of the MFMA M2M / L2L for order N */
-static void ensure_shift_tables(ops_dev *od, int N)
+static void build_shift_tables(ops_dev *od, int N)
 {
   const wbfmm_m2l_plan_t *P;
   size_t n13, ncs;
   double *a1, *a3, *cu, *cd;
-  ensure_m2l_plan(od, N);
   if (od->sh_A1[N]) return;
   P = od->m2l_plan[N];
   n13 = (size_t)2 * P->KT * 64;
@@ -871,14 +913,16 @@ static float *host_f32(const double *h, size_t n)
   return f;
 }
 
-/* FP32 tables of order N for coefficients scaled by the box width */
-static void ensure_f32_tables(ops_dev *od, int N)
+static void ensure_f32_tables(ops_dev *od, int N) { ensure_order_tables(od, N); }
+
+/* FP32 tables of order N for coefficients scaled by the box width (after
+   the FP64 plan and shift tables) */
+static void build_f32_tables(ops_dev *od, int N)
 {
   const wbfmm_m2l_plan_t *P;
   size_t n13, ncs, na2;
   double *h;
   if (od->f_A1[N]) return;
-  ensure_shift_tables(od, N); /* also the M2L plan */
   P = od->m2l_plan[N];
   n13 = (size_t)P->NTH * P->KT * 64;
   ncs = (size_t)343 * (N + 1) * 2;
@@ -1037,7 +1081,6 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                        od->f_A2dn[N], od->f_CSdn[N], od->m2l_plan[N]));
     return 0;
   }
-  ensure_lap_coax(od);
   if (t->order_s[level] == t->order_r[level] &&
       (int)t->order_s[level] >= M2L_MFMA_MIN && (int)t->order_s[level] <= M2L_MFMA_MAX) {
     /* matrix-core form: targets grouped by Morton child digit */
@@ -1055,6 +1098,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                  od->m2l_CS[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
+    ensure_lap_coax(od);
     HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
                             t->D, t->order_s[level], t->order_r[level],
                             (double *)t->mps[level], (double *)t->mpr[level],
@@ -1536,8 +1580,11 @@ gint wbfmm_shift_operators_coaxial_SR_init(wbfmm_shift_operators_t *w, gdouble D
   wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
   h = malloc(sizeof(double) * 2 * ne * nr);
   if (!h) WBFMM_FATAL("out of host memory");
-  for (i = 0; i < nr; i++)
-    HIPOK(wbfmm_hip_hel_coaxial_table(1, (int)L, k * wb * rr[i], h + 2 * ne * i));
+  {
+    double kr[16];
+    for (i = 0; i < nr; i++) kr[i] = k * wb * rr[i];
+    HIPOK(wbfmm_hip_hel_coaxial_tables(1, (int)L, nr, kr, h));
+  }
   if (w->SR[level]) wbfmm_hip_free(w->SR[level]);
   HIPOK(wbfmm_hip_malloc(&w->SR[level], sizeof(double) * 2 * ne * nr));
   HIPOK(wbfmm_hip_memcpy(w->SR[level], h, sizeof(double) * 2 * ne * nr));
@@ -1585,24 +1632,58 @@ gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *w)
   return 0;
 }
 
-/* H tables and offset geometry of the Helmholtz passes, order Lmax */
+/* H tables and offset geometry of the Helmholtz passes.  They depend on
+   the order only, so one set per order is built for the whole process (by
+   the first operator set that needs it, on host threads) and shared by
+   every operator set of that order until the process ends. */
+typedef struct {
+  int state; /* 0 none, 1 being built, 2 ready */
+  double *Hth, *Hud, *chi;
+  int *th_idx, *rix;
+  size_t nrot;
+} hel_rot_set;
+static hel_rot_set g_hel_rot[WBFMM_HIP_MAX_ORDER + 1];
+static pthread_mutex_t g_hel_rot_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_cond_t g_hel_rot_cv = PTHREAD_COND_INITIALIZER;
+
 static void ensure_hel_rotations(ops_dev *od)
 {
-  double thetas[64], chi[343], rr[16], tud[2];
-  int th_idx[343], rix[343], nth, nr;
   int L = (int)od->op.Lmax;
+  hel_rot_set *e = &g_hel_rot[L];
   if (od->d_hel_Hth) return;
-  wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
-  HIPOK(wbfmm_hip_H_tables(L, nth, thetas, &od->d_hel_Hth, &od->hel_nrot));
-  tud[0] = acos(1.0 / sqrt(3.0));
-  tud[1] = acos(-1.0 / sqrt(3.0));
-  HIPOK(wbfmm_hip_H_tables(L, 2, tud, &od->d_hel_Hud, &od->hel_nrot));
-  HIPOK(wbfmm_hip_malloc((void **)&od->d_th_idx, sizeof(th_idx)));
-  HIPOK(wbfmm_hip_memcpy(od->d_th_idx, th_idx, sizeof(th_idx)));
-  HIPOK(wbfmm_hip_malloc((void **)&od->d_rix, sizeof(rix)));
-  HIPOK(wbfmm_hip_memcpy(od->d_rix, rix, sizeof(rix)));
-  HIPOK(wbfmm_hip_malloc((void **)&od->d_chi, sizeof(chi)));
-  HIPOK(wbfmm_hip_memcpy(od->d_chi, chi, sizeof(chi)));
+  pthread_mutex_lock(&g_hel_rot_mu);
+  while (e->state == 1) pthread_cond_wait(&g_hel_rot_cv, &g_hel_rot_mu);
+  if (e->state == 0) {
+    double thetas[64], chi[343], rr[16], tud[2];
+    int th_idx[343], rix[343], nth, nr;
+    hel_rot_set b;
+    e->state = 1;
+    pthread_mutex_unlock(&g_hel_rot_mu);
+    wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
+    HIPOK(wbfmm_hip_H_tables(L, nth, thetas, &b.Hth, &b.nrot));
+    tud[0] = acos(1.0 / sqrt(3.0));
+    tud[1] = acos(-1.0 / sqrt(3.0));
+    HIPOK(wbfmm_hip_H_tables(L, 2, tud, &b.Hud, &b.nrot));
+    /* the copies from host memory return when done: the tables are ready
+       for every stream */
+    HIPOK(wbfmm_hip_malloc((void **)&b.th_idx, sizeof(th_idx)));
+    HIPOK(wbfmm_hip_memcpy(b.th_idx, th_idx, sizeof(th_idx)));
+    HIPOK(wbfmm_hip_malloc((void **)&b.rix, sizeof(rix)));
+    HIPOK(wbfmm_hip_memcpy(b.rix, rix, sizeof(rix)));
+    HIPOK(wbfmm_hip_malloc((void **)&b.chi, sizeof(chi)));
+    HIPOK(wbfmm_hip_memcpy(b.chi, chi, sizeof(chi)));
+    pthread_mutex_lock(&g_hel_rot_mu);
+    b.state = 2;
+    *e = b;
+    pthread_cond_broadcast(&g_hel_rot_cv);
+  }
+  od->d_hel_Hth = e->Hth;
+  od->d_hel_Hud = e->Hud;
+  od->hel_nrot = e->nrot;
+  od->d_th_idx = e->th_idx;
+  od->d_rix = e->rix;
+  od->d_chi = e->chi;
+  pthread_mutex_unlock(&g_hel_rot_mu);
 }
 
 /* src/pass.c:581-632: M2M from level into level-1 */

@@ -19,7 +19,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <pthread.h>
 #include <quadmath.h>
+#include <unistd.h>
 
 #include "wbfmm_hip.h"
 
@@ -56,6 +58,48 @@ static qreal gb(int n, int m)
 
 /* H_n^{nu,m}(theta) for n<=N: m=0 column from normalised Legendre
    functions up to degree 2N, then recursion 5.55 advancing m. */
+/* n independent jobs fn(i, ctx) on up to 16 host threads (the tables of
+   different angles or distances do not share state) */
+typedef struct {
+  void (*fn)(int, void *);
+  void *ctx;
+  int n, stride, first;
+} par_job;
+
+static void *par_run(void *a)
+{
+  par_job *j = (par_job *)a;
+  int i;
+  for (i = j->first; i < j->n; i += j->stride) j->fn(i, j->ctx);
+  return NULL;
+}
+
+static void par_for(int n, void (*fn)(int, void *), void *ctx)
+{
+  long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+  int t, nt = n < 16 ? n : 16, started = 0;
+  pthread_t th[16];
+  par_job jobs[16];
+  if (ncpu > 0 && nt > ncpu) nt = (int)ncpu;
+  if (nt <= 1) {
+    for (t = 0; t < n; t++) fn(t, ctx);
+    return;
+  }
+  for (t = 0; t < nt; t++) {
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].n = n;
+    jobs[t].stride = nt;
+    jobs[t].first = t;
+  }
+  for (t = 1; t < nt; t++)
+    if (pthread_create(&th[t], NULL, par_run, &jobs[t]) == 0) started |= 1 << t;
+    else par_run(&jobs[t]);  /* no thread: run the share here */
+  par_run(&jobs[0]);
+  for (t = 1; t < nt; t++)
+    if (started & (1 << t)) pthread_join(th[t], NULL);
+}
+
 static void H_rotation(double *H, int N, double th)
 {
   int L = 2 * N, n, m, nu;
@@ -598,6 +642,19 @@ int wbfmm_hip_offset_geometry(double *thetas, int *nth, int *th_idx,
   return 0;
 }
 
+typedef struct {
+  double *host;
+  size_t nr;
+  int L;
+  const double *theta;
+} hjob;
+
+static void h_one(int i, void *c)
+{
+  hjob *j = (hjob *)c;
+  H_rotation(j->host + (size_t)i * j->nr, j->L, j->theta[i]);
+}
+
 int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
                        size_t *stride)
 {
@@ -608,7 +665,11 @@ int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
   if (L < 0 || L > 64 || nth < 1) return WBFMM_HIP_ERR_ARG;
   host = malloc(sizeof(double) * nr * nth);
   if (!host) return WBFMM_HIP_ERR_NOMEM;
-  for (i = 0; i < nth; i++) H_rotation(host + i * nr, L, theta[i]);
+  {
+    hjob hj = {host, nr, L, theta};
+    par_for(nth, h_one, &hj);
+  }
+  (void)i;
   rc = wbfmm_hip_malloc((void **)dev, sizeof(double) * nr * nth);
   if (!rc) rc = wbfmm_hip_memcpy(*dev, host, sizeof(double) * nr * nth);
   free(host);
@@ -662,6 +723,27 @@ int wbfmm_hip_hel_coaxial_table(int kind, int L, double kr, double *E)
 {
   if (fabs(kr) <= 20.0) return coaxial_ld(kind, L, kr, E);
   return coaxial_q(kind, L, kr, E);
+}
+
+typedef struct {
+  int kind, L, rc;
+  const double *kr;
+  double *E;
+  size_t stride;
+} cxjob;
+
+static void cx_one(int i, void *c)
+{
+  cxjob *j = (cxjob *)c;
+  int rc = wbfmm_hip_hel_coaxial_table(j->kind, j->L, j->kr[i], j->E + (size_t)i * j->stride);
+  if (rc) j->rc = rc;
+}
+
+int wbfmm_hip_hel_coaxial_tables(int kind, int L, int n, const double *kr, double *E)
+{
+  cxjob j = {kind, L, 0, kr, E, 2 * CXI(L + 1, 0, 0)};
+  par_for(n, cx_one, &j);
+  return j.rc;
 }
 
 /* ------------------------------------------------------------------
