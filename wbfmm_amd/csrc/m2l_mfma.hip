@@ -33,6 +33,7 @@
 // D[l/16 + 4q][l%16], q = 0..3.
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <hip/hip_runtime.h>
@@ -407,6 +408,314 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   }
 }
 
+// ---------------------------------------------------------------------
+// Two wavefronts per chunk.  The one-wavefront kernel holds ~490 registers
+// (stage outputs, the next offset's sources and the 16 local expansions),
+// so one wavefront runs per SIMD and nothing hides its LDS and memory
+// waits.  Here a 128-thread block shares one LDS buffer R for its 16
+// targets: wavefront w computes the bins b = w (mod 2) of every stage and
+// owns the units u = grp + 4 w + 8 i for loading, rotation (Zin) and
+// accumulation, so each holds about half the registers and two blocks fit
+// per SIMD pair.  Barriers separate the reads of R by a stage from the
+// writes of its results.  Every output takes the same products in the same
+// order as in the one-wavefront kernel; the two may differ by the rounding
+// of contracted (fused) expressions, and each is deterministic: every LDS
+// slot has one writer.
+// ---------------------------------------------------------------------
+
+// k-steps of the bins b = WI (mod 2), in bin order
+struct WaveSeq {
+  int n, nb;
+  int t[4 * WBFMM_M2L_MAXB];
+  int start[WBFMM_M2L_MAXB];
+};
+
+constexpr WaveSeq make_seq(const Bins &B, int WI) {
+  WaveSeq s{};
+  for (int b = WI; b < B.nb; b += 2) {
+    s.start[s.nb++] = s.n;
+    for (int j = 0; j < B.nk[b]; j++) s.t[s.n++] = B.koff[b] + j;
+  }
+  return s;
+}
+
+template <typename T, int N, int WI>
+__device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int lane, int grp,
+                                        int col, typename MF<T>::v4 *y) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, WI);
+  // a wavefront may have no bins at low orders (AD then only sizes av)
+  constexpr int AD = WS.n < 16 ? (WS.n > 0 ? WS.n : 1) : 16;
+  T av[AD];
+#pragma unroll
+  for (int s = 0; s < AD && s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + 2 * lb;
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      r4 = MF<T>::mma(av[s % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4);
+      if (s + AD < WS.n) av[s % AD] = a[WS.t[s + AD] * 64 + lane];
+    }
+    y[lb] = r4;
+  }
+}
+
+template <typename T, int N, int WI>
+__device__ __forceinline__ void m2l_w2_loop(
+    T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    typename MF<T>::t2 (*cs)[N + 1], unsigned (*vm)[7], const unsigned *anym, int lane,
+    int col, int grp, bool tvalid, uint32_t tb, int ti, int tj, int tk, int px, int py,
+    int pz, int c, unsigned k16, int split, int nsplit, const ClassArgs &ca,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A1,
+    const T *__restrict__ A3, const T *__restrict__ A2, const T *__restrict__ CS,
+    const T *__restrict__ zero, const T *__restrict__ mps, T *__restrict__ mpr,
+    T *__restrict__ part, unsigned ntot) {
+  typedef typename MF<T>::t2 T2;
+  typedef typename MF<T>::v4 V4;
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU;
+  constexpr int NAW = (NU + 7) / 8;  // units per lane
+  constexpr WaveSeq WS = make_seq(SH::BN, WI);
+  constexpr int NBW = WS.nb;
+  // register prefetch of the next offset's sources while two wavefronts
+  // per SIMD still fit in the register file
+  constexpr bool PREFETCH = true;
+  const int ubase = grp + 4 * WI;
+
+  auto next_k = [&](int k) {
+    for (k++; k < 216; k++)
+      if (((anym[k >> 5] >> (k & 31)) & 1u) && k % nsplit == split) break;
+    return k;
+  };
+  auto offset_of = [&](int k) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+  };
+  // both wavefronts write the offset's factors (the same values)
+  auto source_of = [&](int k, int p) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (lane <= N) {
+      const T *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = mk2<T>(t[0], t[1]);
+    }
+    const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
+    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
+                          : 0;
+    return v ? mps + sb * (uint64_t)C : zero;
+  };
+  T pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const T *src, const int4 *tu, int ub, int i0, int i1) {
+#pragma unroll
+    for (int i = i0; i < i1; i++) {
+      const int u = min(ub + 8 * i, NU - 1);
+      const T *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  // Lanes past the last unit (loaded as unit NU-1) write nothing: the two
+  // wavefronts run separately compiled code, whose roundings of the same
+  // expression may differ, so a slot must have a single writer.
+  auto deposit = [&](const int4 *tu, int ub, int cl, int p, int i0, int i1) {
+#pragma unroll
+    for (int i = i0; i < i1; i++) {
+      if (ub + 8 * i >= NU) continue;
+      const int4 inf = tu[ub + 8 * i];
+      const T2 e = cs[p][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
+  auto load_deposit = [&](const T *src, const int4 *tu, int ub, int cl, int p) {
+#pragma unroll
+    for (int i0 = 0; i0 < NAW; i0 += 8) {
+      const int i1 = i0 + 8 < NAW ? i0 + 8 : NAW;
+      load_sources(src, tu, ub, i0, i1);
+      deposit(tu, ub, cl, p, i0, i1);
+    }
+  };
+  auto write_out = [&](const int4 *tm, const V4 *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + 2 * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  T acc_re[NAW], acc_im[NAW];
+#pragma unroll
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = T(0);
+
+  int k = next_k(-1), p = 0;
+  if (k < 216) load_deposit(source_of(k, 0), uinf, ubase, col, 0);
+  __syncthreads();
+  while (k < 216) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+    const int o = offset_of(k);
+    const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
+    const int ri = __builtin_amdgcn_readfirstlane(P->rix[o]);
+    const int kn = next_k(k);
+
+    V4 y[NBW > 0 ? NBW : 1];
+    stage_w<T, N, WI>(A1 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t1, y, gp, cl);
+    __syncthreads();
+    if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
+
+    stage_w<T, N, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t2, y, gp, cl);
+    __syncthreads();
+
+    stage_w<T, N, WI>(A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t3, y, gp, cl);
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 8 * i, NU - 1);
+      const T2 e = cs[p][tu[u].y];
+      const T tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+      acc_re[i] += e.x * tr + e.y * tq;
+      acc_im[i] += e.x * tq - e.y * tr;
+    }
+    __syncthreads();  // every read of R done before the next deposit
+
+    if (kn < 216) {
+      if (PREFETCH)
+        deposit(tu, ub, cl, p ^ 1, 0, NAW);
+      else
+        load_deposit(source_of(kn, p ^ 1), tu, ub, cl, p ^ 1);
+    }
+    __syncthreads();
+    p ^= 1;
+    k = kn;
+  }
+
+  if (!tvalid) return;
+  if (nsplit > 1) {
+    T *dst = part + ((size_t)split * ntot + ca.start[c] + k16) * C;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = ubase + 8 * i;
+      if (u < NU) {
+        const int4 inf = uinf[u];
+        dst[inf.x] = acc_re[i];
+        if (inf.y) dst[inf.x + 1] = acc_im[i];
+      }
+    }
+    return;
+  }
+  T *out = mpr + (size_t)tb * C;
+#pragma unroll
+  for (int i = 0; i < NAW; i++) {
+    const int u = ubase + 8 * i;
+    if (u < NU) {
+      const int4 inf = uinf[u];
+      out[inf.x] += acc_re[i];
+      if (inf.y) out[inf.x + 1] += acc_im[i];
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_lap_m2l_mfma2(
+    const uint32_t *__restrict__ cls, ClassArgs ca,
+    const uint32_t *__restrict__ occ, int level,
+    const T *__restrict__ mps, T *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A1,
+    const T *__restrict__ A3, const T *__restrict__ A2,
+    const T *__restrict__ CS, const T *__restrict__ zero, int nsplit,
+    T *__restrict__ part, unsigned ntot) {
+  typedef typename MF<T>::t2 T2;
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  __shared__ T R[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ T2 cs[2][N + 1];
+  __shared__ unsigned vm[16][7];
+  __shared__ unsigned anym[7];
+
+  // wv through readfirstlane: a scalar value, so the branch into the
+  // per-wavefront code below is a scalar branch and each wavefront runs
+  // (and meets the barriers of) its own path only
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 128) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<T>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 128) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 128) R[i] = T(0);
+  for (int i = tid; i < 16 * 7; i += 128) vm[i / 7][i % 7] = 0u;
+
+  const int split = (int)(blockIdx.x % (unsigned)nsplit);
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
+  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+  int c = 0;
+  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  const unsigned k16 = (chunk - ca.chunk_prefix[c]) * 16 + col;
+  const bool tvalid = k16 < ca.count[c];
+  const uint32_t tb = tvalid ? cls[ca.start[c] + k16] : 0u;
+  const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  __syncthreads();
+
+  for (int k = grp + 4 * wv; k < 216; k += 8) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int si = ti + dx, sj = tj + dy, sk = tk + dz;
+    if (!tvalid || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox)
+      continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[col][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  if (tid < 7) {
+    unsigned a = 0;
+    for (int j = 0; j < 16; j++) a |= vm[j][tid];
+    anym[tid] = a;
+  }
+  __syncthreads();
+
+  if (wv == 0)
+    m2l_w2_loop<T, N, 0>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                         ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2, CS,
+                         zero, mps, mpr, part, ntot);
+  else
+    m2l_w2_loop<T, N, 1>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                         ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2, CS,
+                         zero, mps, mpr, part, ntot);
+}
+
 // mpr[cls[i]] += sum over the splits (in split order) of the partials of
 // target i
 template <typename T>
@@ -642,6 +951,18 @@ static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
   return true;
 }
 
+// wavefronts per 16-target chunk of the M2L kernel: 2 unless
+// WBFMM_M2L_WAVES=1 selects the one-wavefront kernel (same results; kept
+// for comparison)
+static int m2l_waves() {
+  static int w = 0;
+  if (!w) {
+    const char *e = getenv("WBFMM_M2L_WAVES");
+    w = (e && atoi(e) == 1) ? 1 : 2;
+  }
+  return w;
+}
+
 template <typename T, int N>
 static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
                       const uint32_t *occ, int level, const T *mps,
@@ -664,10 +985,18 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   if (nsplit > 1 &&
       dmalloc((void **)&part, sizeof(T) * (size_t)nsplit * ntot * C))
     return WBFMM_HIP_ERR_NOMEM;
-  WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
-               dim3(nchunks * nsplit), dim3(64), 0, cls,
-               ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
-               A1, A3, A2, CS, zero, nsplit, part, ntot);
+  // two wavefronts per chunk where that kernel fits two per SIMD without
+  // spilling (FP32 up to order 15, FP64 up to 12)
+  if (m2l_waves() == 2 && (sizeof(T) == 4 || N <= 12))
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N>),
+                 dim3(nchunks * nsplit), dim3(128), 0, cls,
+                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                 A1, A3, A2, CS, zero, nsplit, part, ntot);
+  else
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
+                 dim3(nchunks * nsplit), dim3(64), 0, cls,
+                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                 A1, A3, A2, CS, zero, nsplit, part, ntot);
   if (nsplit > 1) {
     const size_t ne = (size_t)ntot * C;
     WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<T>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
