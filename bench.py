@@ -346,6 +346,11 @@ def run_batched(args):
     st = srv.stats()
     srv.close()
     _release(probs)
+    for p, r in zip(probs, res):
+        print(f"bench: problem n={p['n']} {'helm k=%.1f' % p['k'] if p['helm'] else 'lap'} "
+              f"{'sphere' if p['sphere'] else 'cube'} fp{r.precision} depth={r.depth} "
+              f"order={r.order} tables={r.tables_ms:.1f} solve={r.solve_ms:.1f}",
+              file=sys.stderr)
     bad = [r.status for r in res if r.status]
     ep = np.array([r.err_potential for r in res])
     eg = np.array([r.err_gradient for r in res])

@@ -409,41 +409,41 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 }
 
 // ---------------------------------------------------------------------
-// Two wavefronts per chunk.  The one-wavefront kernel holds ~490 registers
-// (stage outputs, the next offset's sources and the 16 local expansions),
-// so one wavefront runs per SIMD and nothing hides its LDS and memory
-// waits.  Here a 128-thread block shares one LDS buffer R for its 16
-// targets: wavefront w computes the bins b = w (mod 2) of every stage and
-// owns the units u = grp + 4 w + 8 i for loading, rotation (Zin) and
-// accumulation, so each holds about half the registers and two blocks fit
-// per SIMD pair.  Barriers separate the reads of R by a stage from the
-// writes of its results.  Every output takes the same products in the same
-// order as in the one-wavefront kernel; the two may differ by the rounding
-// of contracted (fused) expressions, and each is deterministic: every LDS
-// slot has one writer.
+// W wavefronts per chunk (W = 2 or 4).  The one-wavefront kernel holds ~490
+// registers at order 12 (stage outputs, the next offset's sources and the
+// 16 local expansions), so one wavefront runs per SIMD and nothing hides
+// its LDS and memory waits.  Here a block of W wavefronts shares one LDS
+// buffer R for its 16 targets: wavefront w computes the bins b = w (mod W)
+// of every stage and owns the units u = grp + 4 w + 4 W i for loading,
+// rotation (Zin) and accumulation, so each holds about 1/W of the registers
+// and several wavefronts run per SIMD.  Barriers separate the reads of R by
+// a stage from the writes of its results.  Every output takes the same
+// products in the same order as in the one-wavefront kernel; the kernels
+// may differ by the rounding of contracted (fused) expressions, and each
+// is deterministic: every LDS slot has one writer.
 // ---------------------------------------------------------------------
 
-// k-steps of the bins b = WI (mod 2), in bin order
+// k-steps of the bins b = WI (mod W), in bin order
 struct WaveSeq {
   int n, nb;
   int t[4 * WBFMM_M2L_MAXB];
   int start[WBFMM_M2L_MAXB];
 };
 
-constexpr WaveSeq make_seq(const Bins &B, int WI) {
+constexpr WaveSeq make_seq(const Bins &B, int W, int WI) {
   WaveSeq s{};
-  for (int b = WI; b < B.nb; b += 2) {
+  for (int b = WI; b < B.nb; b += W) {
     s.start[s.nb++] = s.n;
     for (int j = 0; j < B.nk[b]; j++) s.t[s.n++] = B.koff[b] + j;
   }
   return s;
 }
 
-template <typename T, int N, int WI>
+template <typename T, int N, int W, int WI>
 __device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int lane, int grp,
                                         int col, typename MF<T>::v4 *y) {
   constexpr Bins BN = M2LShape<N>::BN;
-  constexpr WaveSeq WS = make_seq(BN, WI);
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
   // a wavefront may have no bins at low orders (AD then only sizes av)
   constexpr int AD = WS.n < 16 ? (WS.n > 0 ? WS.n : 1) : 16;
   T av[AD];
@@ -451,7 +451,7 @@ __device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int
   for (int s = 0; s < AD && s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
 #pragma unroll
   for (int lb = 0; lb < WS.nb; lb++) {
-    const int b = WI + 2 * lb;
+    const int b = WI + W * lb;
     typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
@@ -463,7 +463,7 @@ __device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int
   }
 }
 
-template <typename T, int N, int WI>
+template <typename T, int N, int W, int WI>
 __device__ __forceinline__ void m2l_w2_loop(
     T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
     typename MF<T>::t2 (*cs)[N + 1], unsigned (*vm)[7], const unsigned *anym, int lane,
@@ -477,8 +477,8 @@ __device__ __forceinline__ void m2l_w2_loop(
   typedef typename MF<T>::v4 V4;
   using SH = M2LShape<N>;
   constexpr int C = SH::C, NU = SH::NU;
-  constexpr int NAW = (NU + 7) / 8;  // units per lane
-  constexpr WaveSeq WS = make_seq(SH::BN, WI);
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
   constexpr int NBW = WS.nb;
   // register prefetch of the next offset's sources while two wavefronts
   // per SIMD still fit in the register file
@@ -510,7 +510,7 @@ __device__ __forceinline__ void m2l_w2_loop(
   auto load_sources = [&](const T *src, const int4 *tu, int ub, int i0, int i1) {
 #pragma unroll
     for (int i = i0; i < i1; i++) {
-      const int u = min(ub + 8 * i, NU - 1);
+      const int u = min(ub + 4 * W * i, NU - 1);
       const T *q = src + tu[u].x;
       pre_re[i] = q[0];
       pre_im[i] = q[1];
@@ -522,8 +522,8 @@ __device__ __forceinline__ void m2l_w2_loop(
   auto deposit = [&](const int4 *tu, int ub, int cl, int p, int i0, int i1) {
 #pragma unroll
     for (int i = i0; i < i1; i++) {
-      if (ub + 8 * i >= NU) continue;
-      const int4 inf = tu[ub + 8 * i];
+      if (ub + 4 * W * i >= NU) continue;
+      const int4 inf = tu[ub + 4 * W * i];
       const T2 e = cs[p][inf.y];
       R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
       R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
@@ -540,7 +540,7 @@ __device__ __forceinline__ void m2l_w2_loop(
   auto write_out = [&](const int4 *tm, const V4 *y, int gp, int cl) {
 #pragma unroll
     for (int lb = 0; lb < NBW; lb++) {
-      const int4 d = tm[4 * (WI + 2 * lb) + gp];
+      const int4 d = tm[4 * (WI + W * lb) + gp];
       R[d.x * 16 + cl] = y[lb][0];
       R[d.y * 16 + cl] = y[lb][1];
       R[d.z * 16 + cl] = y[lb][2];
@@ -552,39 +552,48 @@ __device__ __forceinline__ void m2l_w2_loop(
 #pragma unroll
   for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = T(0);
 
-  int k = next_k(-1), p = 0;
-  if (k < 216) load_deposit(source_of(k, 0), uinf, ubase, col, 0);
+  // k is the same in every lane (it comes from the shared anym): held as a
+  // scalar, and the next offset's table indices are read one offset ahead
+  int k = __builtin_amdgcn_readfirstlane(next_k(-1)), p = 0;
+  int th = 0, ri = 0;
+  if (k < 216) {
+    th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
+    ri = __builtin_amdgcn_readfirstlane(P->rix[offset_of(k)]);
+    load_deposit(source_of(k, 0), uinf, ubase, col, 0);
+  }
   __syncthreads();
   while (k < 216) {
     int z0;
     asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
     const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
     const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
-    const int o = offset_of(k);
-    const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
-    const int ri = __builtin_amdgcn_readfirstlane(P->rix[o]);
-    const int kn = next_k(k);
+    const int kn = __builtin_amdgcn_readfirstlane(next_k(k));
+    int th_n = 0, ri_n = 0;
+    if (kn < 216) {
+      th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
+      ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
+    }
 
     V4 y[NBW > 0 ? NBW : 1];
-    stage_w<T, N, WI>(A1 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+    stage_w<T, N, W, WI>(A1 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t1, y, gp, cl);
     __syncthreads();
     if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
 
-    stage_w<T, N, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
+    stage_w<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t2, y, gp, cl);
     __syncthreads();
 
-    stage_w<T, N, WI>(A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+    stage_w<T, N, W, WI>(A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t3, y, gp, cl);
     __syncthreads();
 
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
-      const int u = min(ub + 8 * i, NU - 1);
+      const int u = min(ub + 4 * W * i, NU - 1);
       const T2 e = cs[p][tu[u].y];
       const T tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
       acc_re[i] += e.x * tr + e.y * tq;
@@ -601,6 +610,8 @@ __device__ __forceinline__ void m2l_w2_loop(
     __syncthreads();
     p ^= 1;
     k = kn;
+    th = th_n;
+    ri = ri_n;
   }
 
   if (!tvalid) return;
@@ -608,7 +619,7 @@ __device__ __forceinline__ void m2l_w2_loop(
     T *dst = part + ((size_t)split * ntot + ca.start[c] + k16) * C;
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
-      const int u = ubase + 8 * i;
+      const int u = ubase + 4 * W * i;
       if (u < NU) {
         const int4 inf = uinf[u];
         dst[inf.x] = acc_re[i];
@@ -620,7 +631,7 @@ __device__ __forceinline__ void m2l_w2_loop(
   T *out = mpr + (size_t)tb * C;
 #pragma unroll
   for (int i = 0; i < NAW; i++) {
-    const int u = ubase + 8 * i;
+    const int u = ubase + 4 * W * i;
     if (u < NU) {
       const int4 inf = uinf[u];
       out[inf.x] += acc_re[i];
@@ -629,8 +640,8 @@ __device__ __forceinline__ void m2l_w2_loop(
   }
 }
 
-template <typename T, int N>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_lap_m2l_mfma2(
+template <typename T, int N, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void k_lap_m2l_mfma2(
     const uint32_t *__restrict__ cls, ClassArgs ca,
     const uint32_t *__restrict__ occ, int level,
     const T *__restrict__ mps, T *__restrict__ mpr,
@@ -653,7 +664,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   // (and meets the barriers of) its own path only
   const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int i = tid; i < 4 * NB; i += 128) {
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
     const int b = i >> 2, g = i & 3;
     int t1[4], t2[4], t3[4];
     for (int q = 0; q < 4; q++) {
@@ -673,8 +684,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
     uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
                         (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = tid; i < (S + 1) * 16; i += 128) R[i] = T(0);
-  for (int i = tid; i < 16 * 7; i += 128) vm[i / 7][i % 7] = 0u;
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = T(0);
+  for (int i = tid; i < 16 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
 
   const int split = (int)(blockIdx.x % (unsigned)nsplit);
   const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
@@ -689,7 +700,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int nbox = 1 << level;
   __syncthreads();
 
-  for (int k = grp + 4 * wv; k < 216; k += 8) {
+  for (int k = grp + 4 * wv; k < 216; k += 4 * W) {
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
     const int si = ti + dx, sj = tj + dy, sk = tk + dz;
@@ -706,14 +717,39 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
   __syncthreads();
 
-  if (wv == 0)
-    m2l_w2_loop<T, N, 0>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
-                         ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2, CS,
-                         zero, mps, mpr, part, ntot);
-  else
-    m2l_w2_loop<T, N, 1>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
-                         ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2, CS,
-                         zero, mps, mpr, part, ntot);
+  if constexpr (W == 2) {
+    if (wv == 0)
+      m2l_w2_loop<T, N, 2, 0>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+    else
+      m2l_w2_loop<T, N, 2, 1>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+  } else {
+    switch (wv) {
+      case 0:
+        m2l_w2_loop<T, N, W, 0>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+        break;
+      case 1:
+        m2l_w2_loop<T, N, W, 1>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+        break;
+      case 2:
+        m2l_w2_loop<T, N, W, 2>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+        break;
+      default:
+        m2l_w2_loop<T, N, W, 3>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
+                            ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
+                            CS, zero, mps, mpr, part, ntot);
+        break;
+    }
+  }
 }
 
 // mpr[cls[i]] += sum over the splits (in split order) of the partials of
@@ -951,14 +987,15 @@ static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
   return true;
 }
 
-// wavefronts per 16-target chunk of the M2L kernel: 2 unless
-// WBFMM_M2L_WAVES=1 selects the one-wavefront kernel (same results; kept
-// for comparison)
+// wavefronts per 16-target chunk of the M2L kernel: 4 (every order and
+// precision), or 2 / 1 when WBFMM_M2L_WAVES says so (kept for comparison;
+// the results agree to rounding)
 static int m2l_waves() {
   static int w = 0;
   if (!w) {
     const char *e = getenv("WBFMM_M2L_WAVES");
-    w = (e && atoi(e) == 1) ? 1 : 2;
+    w = e ? atoi(e) : 4;
+    if (w != 1 && w != 2) w = 4;
   }
   return w;
 }
@@ -985,10 +1022,15 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   if (nsplit > 1 &&
       dmalloc((void **)&part, sizeof(T) * (size_t)nsplit * ntot * C))
     return WBFMM_HIP_ERR_NOMEM;
-  // two wavefronts per chunk where that kernel fits two per SIMD without
-  // spilling (FP32 up to order 15, FP64 up to 12)
-  if (m2l_waves() == 2 && (sizeof(T) == 4 || N <= 12))
-    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N>),
+  // several wavefronts per chunk: four fit three per SIMD without spilling
+  // at every order; two fit two per SIMD for FP32 and for FP64 up to 12
+  if (m2l_waves() == 4)
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 4>),
+                 dim3(nchunks * nsplit), dim3(256), 0, cls,
+                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                 A1, A3, A2, CS, zero, nsplit, part, ntot);
+  else if (m2l_waves() >= 2 && (sizeof(T) == 4 || N <= 12))
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 2>),
                  dim3(nchunks * nsplit), dim3(128), 0, cls,
                  ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                  A1, A3, A2, CS, zero, nsplit, part, ntot);
