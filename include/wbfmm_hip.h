@@ -151,12 +151,17 @@ int wbfmm_hip_laplace_coaxial_tables(int L, double **dev);
      A2: NR x KT x 64, the coaxial S|R at distance rr[i] w for one box
        width w (wbfmm_hip_lap_m2l_coax);
      CS: 343 x (N+1) x 2, cos and sin of m chi for each offset. */
-#define WBFMM_M2L_MAXN 20
-#define WBFMM_M2L_MAXB 32
-#define WBFMM_M2L_MAXU 231
+#define WBFMM_M2L_MAXN 30
+#define WBFMM_M2L_MAXB 96
+#define WBFMM_M2L_MAXU 528
+/* bins: 16-row groups of the blocks of a stage.  Blocks of up to 16 rows
+   share bins (first fit, decreasing); a larger block spans consecutive
+   bins of its own.  Bin b's k-steps j read input rows kbase[b] + 4j ..
+   +3: kbase = 16b for shared bins, the block's first row for the bins of
+   a large block, whose every row tile reads all the block's columns. */
 typedef struct {
   int N, C, NU, NB, KT, NTH, S, NR;
-  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB];
+  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB], kbase[WBFMM_M2L_MAXB];
   int map1[16 * WBFMM_M2L_MAXB], map2[16 * WBFMM_M2L_MAXB],
       map3[16 * WBFMM_M2L_MAXB];
   int s1[WBFMM_M2L_MAXU];
@@ -165,8 +170,8 @@ typedef struct {
   double r[343];
   double rr[32];
 } wbfmm_m2l_plan_t;
-/* builds the plan of order N (1 <= N <= 15, so that every block fits one
-   16-row bin) and, for the pointers that are not NULL, fills the host
+/* builds the plan of order N (1 <= N <= 24) and, for the pointers that
+   are not NULL, fills the host
    tables A1, A3 (NTH*KT*64 doubles each) and CS (343*(N+1)*2); call once
    with NULL tables to learn the sizes */
 int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *plan, double *A1,

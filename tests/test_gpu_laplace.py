@@ -274,10 +274,11 @@ def test_mfma_f64_operand_layout(wb):
     assert np.array_equal(got, Dm)
 
 
-@pytest.mark.parametrize("p", [2, 4, 7, 12, 13, 15, 16])
+@pytest.mark.parametrize("p", [2, 4, 7, 12, 13, 15, 16, 18, 20, 24, 25])
 def test_m2l_mfma_orders_match_oracle(wb, oracle, p):
-    # the matrix-core M2L over the orders it serves (2..15; 13 and above
-    # without the register prefetch) and the vector M2L above them (16)
+    # the matrix-core M2L over the orders it serves (2..24; above 15 its
+    # blocks span several bins and the M2M / L2L use the vector kernels)
+    # and the vector M2L above them (25)
     rng = np.random.default_rng(20 + p)
     xs = rng.uniform(0, 1, (1800, 3))
     q = rng.normal(size=1800)

@@ -35,7 +35,7 @@ def emulate(P, A1, A3, A2, CS, M, o):
         for b in range(NB):
             for j in range(P.nk[b]):
                 a = A[P.koff[b] + j].reshape(4, 16).T        # a[r][kk] = lane kk*16 + r
-                y[16 * b:16 * b + 16] += a @ v[16 * b + 4 * j:16 * b + 4 * j + 4]
+                y[16 * b:16 * b + 16] += a @ v[P.kbase[b] + 4 * j:P.kbase[b] + 4 * j + 4]
         return y
 
     def move(y, mp):
@@ -61,13 +61,20 @@ def emulate(P, A1, A3, A2, CS, M, o):
     return L
 
 
-@pytest.mark.parametrize("N", [1, 2, 5, 8, 12, 13, 15])
+@pytest.mark.parametrize("N", [1, 2, 5, 8, 12, 13, 15, 16, 17, 20, 24])
 def test_plan_matches_oracle_m2l(wb, oracle, N):
     w = 0.37
     P, A1, A3, A2, CS = wb.m2l_tables(N, w)
     assert P.N == N and P.C == (N + 1) ** 2
-    # every bin holds at most 16 rows and the k steps cover it
-    assert sum(P.nk[b] for b in range(P.NB)) == P.KT and all(1 <= P.nk[b] <= 4 for b in range(P.NB))
+    # every bin holds at most 16 rows and its k steps cover its block(s):
+    # shared bins read their own rows, the bins of a block above 16 rows
+    # read the block's rows from its first one
+    assert sum(P.nk[b] for b in range(P.NB)) == P.KT
+    for b in range(P.NB):
+        assert 1 <= P.nk[b] and P.kbase[b] % 16 == 0 and P.kbase[b] <= 16 * b
+        if P.kbase[b] == 16 * b and (b + 1 == P.NB or P.kbase[b + 1] != 16 * b):
+            assert P.nk[b] <= 4
+        assert P.kbase[b] + 4 * P.nk[b] <= 16 * P.NB
     # the maps are injective on their real rows
     for mp in (P.map1, P.map2, P.map3):
         dest = [v >> 6 for v in mp[:16 * P.NB] if (v >> 6) != P.S]
@@ -90,10 +97,10 @@ def test_plan_matches_oracle_m2l(wb, oracle, N):
         assert np.abs(L - Lref).max() < 1e-12 * np.abs(Lref).max(), off
 
 
-def test_orders_beyond_one_bin_are_refused(wb):
+def test_orders_beyond_the_plan_are_refused(wb):
     import ctypes
     P = wb.M2LPlan()
-    assert wb.lib().wbfmm_hip_lap_m2l_tables(16, ctypes.byref(P), None, None, None) != 0
+    assert wb.lib().wbfmm_hip_lap_m2l_tables(25, ctypes.byref(P), None, None, None) != 0
 
 
 def test_theta_table_has_49_angles(wb):
@@ -125,7 +132,7 @@ def emulate_shift(P, A1, A3, A2, CS, th, M):
         for b in range(NB):
             for j in range(P.nk[b]):
                 a = A[P.koff[b] + j].reshape(4, 16).T
-                y[16 * b:16 * b + 16] += a @ v[16 * b + 4 * j:16 * b + 4 * j + 4]
+                y[16 * b:16 * b + 16] += a @ v[P.kbase[b] + 4 * j:P.kbase[b] + 4 * j + 4]
         return y
 
     def move(y, mp):

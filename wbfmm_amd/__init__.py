@@ -45,7 +45,7 @@ class WbfmmTree(ctypes.Structure):
 
 
 # name -> (restype, argtypes)
-M2L_MAXN, M2L_MAXB, M2L_MAXU = 20, 32, 231
+M2L_MAXN, M2L_MAXB, M2L_MAXU = 30, 96, 528
 
 
 class M2LPlan(ctypes.Structure):
@@ -53,7 +53,7 @@ class M2LPlan(ctypes.Structure):
     _fields_ = [
         ("N", _i), ("C", _i), ("NU", _i), ("NB", _i), ("KT", _i), ("NTH", _i), ("S", _i),
         ("NR", _i),
-        ("nk", _i * M2L_MAXB), ("koff", _i * M2L_MAXB),
+        ("nk", _i * M2L_MAXB), ("koff", _i * M2L_MAXB), ("kbase", _i * M2L_MAXB),
         ("map1", _i * (16 * M2L_MAXB)), ("map2", _i * (16 * M2L_MAXB)),
         ("map3", _i * (16 * M2L_MAXB)),
         ("s1", _i * M2L_MAXU),

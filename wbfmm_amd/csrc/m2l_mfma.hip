@@ -87,13 +87,15 @@ struct ClassArgs {
 // each bin, so that every MFMA loop has a constant trip count.
 struct Bins {
   int nb, kt;
-  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB];
+  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB], kbase[WBFMM_M2L_MAXB];
 };
 
+// blocks of more than 16 rows take consecutive bins of their own, whose
+// row tiles all read the block's full column range from its first row
 constexpr Bins make_bins(int N) {
   Bins B{};
   int sz[2 * WBFMM_M2L_MAXN + 2] = {}, order[2 * WBFMM_M2L_MAXN + 2] = {};
-  int fill[WBFMM_M2L_MAXB] = {};
+  int fill[WBFMM_M2L_MAXB] = {}, kb[WBFMM_M2L_MAXB] = {}, nk[WBFMM_M2L_MAXB] = {};
   const int nblk = 2 * N + 1;
   for (int i = 0; i < nblk; i++) {
     sz[i] = i <= N ? i + 1 : i - N;
@@ -107,14 +109,33 @@ constexpr Bins make_bins(int N) {
   int nb = 0;
   for (int i = 0; i < nblk; i++) {
     int k = order[i], b = 0;
+    if (sz[k] > 16) {
+      const int nt = (sz[k] + 15) / 16;
+      for (int t = 0; t < nt; t++) {
+        fill[nb + t] = 16;
+        kb[nb + t] = 16 * nb;
+        nk[nb + t] = (sz[k] + 3) / 4;
+      }
+      nb += nt;
+      continue;
+    }
     for (; b < nb; b++)
       if (fill[b] + sz[k] <= 16) break;
-    if (b == nb) fill[nb++] = 0;
+    if (b == nb) {
+      fill[nb] = 0;
+      kb[nb] = -1;
+      nb++;
+    }
     fill[b] += sz[k];
   }
   B.nb = nb;
   for (int b = 0; b < nb; b++) {
-    B.nk[b] = (fill[b] + 3) / 4;
+    if (kb[b] < 0) {
+      kb[b] = 16 * b;
+      nk[b] = (fill[b] + 3) / 4;
+    }
+    B.nk[b] = nk[b];
+    B.kbase[b] = kb[b];
     B.koff[b] = B.kt;
     B.kt += B.nk[b];
   }
@@ -153,7 +174,7 @@ __device__ __forceinline__ void stage(const Bins &BN, const T *__restrict__ a,
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
       const int t = BN.koff[b] + j;
-      r4 = MF<T>::mma(av[t % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4);
+      r4 = MF<T>::mma(av[t % AD], R[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
       if (t + AD < KT) av[t % AD] = a[(t + AD) * 64 + lane];
     }
     y[b] = r4;
@@ -426,7 +447,7 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 // k-steps of the bins b = WI (mod W), in bin order
 struct WaveSeq {
   int n, nb;
-  int t[4 * WBFMM_M2L_MAXB];
+  int t[8 * WBFMM_M2L_MAXB];
   int start[WBFMM_M2L_MAXB];
 };
 
@@ -456,7 +477,7 @@ __device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
       const int s = WS.start[lb] + j;
-      r4 = MF<T>::mma(av[s % AD], R[(16 * b + 4 * j + grp) * 16 + col], r4);
+      r4 = MF<T>::mma(av[s % AD], R[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
       if (s + AD < WS.n) av[s % AD] = a[WS.t[s + AD] * 64 + lane];
     }
     y[lb] = r4;
@@ -640,8 +661,12 @@ __device__ __forceinline__ void m2l_w2_loop(
   }
 }
 
+// two wavefronts per SIMD where the LDS buffer leaves room for them (FP32,
+// and FP64 up to order 15); above, LDS limits the blocks per CU anyway and
+// the registers are not capped
 template <typename T, int N, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void k_lap_m2l_mfma2(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 4 || N <= 15 ? 2 : 1))) void k_lap_m2l_mfma2(
     const uint32_t *__restrict__ cls, ClassArgs ca,
     const uint32_t *__restrict__ occ, int level,
     const T *__restrict__ mps, T *__restrict__ mpr,
@@ -983,7 +1008,8 @@ static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
   constexpr Bins B = make_bins(N);
   if (hp->N != N || hp->NB != B.nb || hp->KT != B.kt || hp->S != M2LShape<N>::S) return false;
   for (int b = 0; b < B.nb; b++)
-    if (hp->nk[b] != B.nk[b] || hp->koff[b] != B.koff[b]) return false;
+    if (hp->nk[b] != B.nk[b] || hp->koff[b] != B.koff[b] || hp->kbase[b] != B.kbase[b])
+      return false;
   return true;
 }
 
@@ -1024,21 +1050,25 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
     return WBFMM_HIP_ERR_NOMEM;
   // several wavefronts per chunk: four fit three per SIMD without spilling
   // at every order; two fit two per SIMD for FP32 and for FP64 up to 12
-  if (m2l_waves() == 4)
+  // orders above 15 (blocks spanning several bins) have the four-wavefront
+  // kernel only
+  if (N > 15 || m2l_waves() == 4) {
     WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 4>),
                  dim3(nchunks * nsplit), dim3(256), 0, cls,
                  ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                  A1, A3, A2, CS, zero, nsplit, part, ntot);
-  else if (m2l_waves() >= 2 && (sizeof(T) == 4 || N <= 12))
-    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 2>),
-                 dim3(nchunks * nsplit), dim3(128), 0, cls,
-                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
-                 A1, A3, A2, CS, zero, nsplit, part, ntot);
-  else
-    WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
-                 dim3(nchunks * nsplit), dim3(64), 0, cls,
-                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
-                 A1, A3, A2, CS, zero, nsplit, part, ntot);
+  } else if constexpr (N <= 15) {
+    if (m2l_waves() >= 2 && (sizeof(T) == 4 || N <= 12))
+      WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 2>),
+                   dim3(nchunks * nsplit), dim3(128), 0, cls,
+                   ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                   A1, A3, A2, CS, zero, nsplit, part, ntot);
+    else
+      WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
+                   dim3(nchunks * nsplit), dim3(64), 0, cls,
+                   ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                   A1, A3, A2, CS, zero, nsplit, part, ntot);
+  }
   if (nsplit > 1) {
     const size_t ne = (size_t)ntot * C;
     WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<T>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
@@ -1129,7 +1159,7 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
   if (level > 20) return WBFMM_HIP_ERR_ARG;
-  if (N < 2 || N > 15) return WBFMM_HIP_ERR_ORDER;
+  if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
   const uint64_t nbox = (uint64_t)1 << (3 * level);
   const size_t words = (size_t)((nbox + 31) / 32);
   {
@@ -1149,7 +1179,8 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
     rc = launch_m2l<T, n>(cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A1, A3, A2, CS, zero, plan_host); \
     break;
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
 #undef CASE
     default:
       break;

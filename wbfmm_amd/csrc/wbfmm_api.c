@@ -121,7 +121,9 @@ typedef struct {
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
 #define M2L_MFMA_MIN 2
-#define M2L_MFMA_MAX 15
+#define M2L_MFMA_MAX 24
+/* orders served by the MFMA M2M / L2L and by the FP32 passes */
+#define SHIFT_MFMA_MAX 15
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
 
@@ -945,7 +947,7 @@ static void build_f32_tables(ops_dev *od, int N)
   free(h);
 }
 
-static int mfma_order(int n) { return n >= M2L_MFMA_MIN && n <= M2L_MFMA_MAX; }
+static int mfma_order(int n) { return n >= M2L_MFMA_MIN && n <= SHIFT_MFMA_MAX; }
 
 gint wbfmm_laplace_coaxial_translate_init(gint N)
 {
@@ -973,7 +975,7 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if (td->prec32) {
     int N = (int)t->order_s[level];
     if (t->order_s[level - 1] != t->order_s[level] || !mfma_order(N))
-      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, M2L_MFMA_MAX);
+      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, SHIFT_MFMA_MAX);
     ensure_f32_tables(od, N);
     HIPOK(wbfmm_hip_lap_shift_mfma_f32(0, td->d_occ[level - 1], td->nocc[level - 1],
                                        td->d_start[level], td->d_end[level], NULL, N,
@@ -983,7 +985,7 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     return 0;
   }
   if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
-      (int)t->order_s[level] <= M2L_MFMA_MAX) {
+      (int)t->order_s[level] <= SHIFT_MFMA_MAX) {
     /* matrix-core form: 16 parents per wavefront */
     int N = (int)t->order_s[level];
     double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << level);
@@ -1062,7 +1064,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     uint32_t *cls;
     if (t->order_r[level] != t->order_s[level] || !mfma_order(N) ||
         (level < t->depth && t->order_r[level + 1] != t->order_r[level]))
-      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, M2L_MFMA_MAX);
+      WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, SHIFT_MFMA_MAX);
     ensure_f32_tables(od, N);
     HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
     HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
@@ -1108,7 +1110,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
   if (t->order_r[level] == t->order_r[level + 1] && (int)t->order_r[level] >= M2L_MFMA_MIN &&
-      (int)t->order_r[level] <= M2L_MFMA_MAX) {
+      (int)t->order_r[level] <= SHIFT_MFMA_MAX) {
     int N = (int)t->order_r[level];
     double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << (level + 1));
     ensure_shift_tables(od, N);

@@ -338,9 +338,13 @@ int wbfmm_hip_laplace_coaxial_tables(int L, double **dev)
    rows; ties in size keep block order.  bin[i] gets the bin of block i,
    pos[i] its first slot within the bin; returns the number of bins and
    fills fill[b] (rows used by bin b). */
-static int pack_bins(int nblk, const int *sz, int *bin, int *pos, int *fill)
+/* Blocks sorted by size (stable, decreasing).  A block of more than 16
+   rows takes ceil(size/16) consecutive bins of its own (their fill set to
+   16 so nothing else joins them); the others go first fit into shared
+   bins.  kb[b]: first input row of bin b's k-steps, nk[b]: their number. */
+static int pack_bins(int nblk, const int *sz, int *bin, int *pos, int *fill, int *kb, int *nk)
 {
-  int order[2 * WBFMM_M2L_MAXN + 2], i, j, nb = 0;
+  int order[2 * WBFMM_M2L_MAXN + 2], i, j, b, nb = 0;
   for (i = 0; i < nblk; i++) order[i] = i;
   for (i = 1; i < nblk; i++) {        /* stable insertion sort, size desc */
     int v = order[i];
@@ -348,14 +352,37 @@ static int pack_bins(int nblk, const int *sz, int *bin, int *pos, int *fill)
     order[j] = v;
   }
   for (i = 0; i < nblk; i++) {
-    int k = order[i], b;
+    int k = order[i];
+    if (sz[k] > 16) {
+      int nt = (sz[k] + 15) / 16, t;
+      if (nb + nt > WBFMM_M2L_MAXB) return -1;
+      for (t = 0; t < nt; t++) {
+        fill[nb + t] = 16;
+        kb[nb + t] = 16 * nb;
+        nk[nb + t] = (sz[k] + 3) / 4;
+      }
+      bin[k] = nb;
+      pos[k] = 0;
+      nb += nt;
+      continue;
+    }
     for (b = 0; b < nb; b++)
       if (fill[b] + sz[k] <= 16) break;
-    if (b == nb) fill[nb++] = 0;
+    if (b == nb) {
+      if (nb == WBFMM_M2L_MAXB) return -1;
+      fill[nb] = 0;
+      kb[nb] = -1;   /* shared bin */
+      nb++;
+    }
     bin[k] = b;
     pos[k] = fill[b];
     fill[b] += sz[k];
   }
+  for (b = 0; b < nb; b++)
+    if (kb[b] < 0) {
+      kb[b] = 16 * b;
+      nk[b] = (fill[b] + 3) / 4;
+    }
   return nb;
 }
 
@@ -367,7 +394,7 @@ static int blk_size2(int N, int i) { return i <= N ? N + 1 - i : N + 1 - (i - N)
 
 typedef struct {
   int nb, bin[2 * WBFMM_M2L_MAXN + 2], pos[2 * WBFMM_M2L_MAXN + 2];
-  int fill[WBFMM_M2L_MAXB];
+  int fill[WBFMM_M2L_MAXB], kb[WBFMM_M2L_MAXB], nk[WBFMM_M2L_MAXB];
 } packing;
 
 static int do_pack(int N, int stage2, packing *pk)
@@ -375,7 +402,7 @@ static int do_pack(int N, int stage2, packing *pk)
   int sz[2 * WBFMM_M2L_MAXN + 2], i, nblk = 2 * N + 1;
   for (i = 0; i < nblk; i++) sz[i] = stage2 ? blk_size2(N, i) : blk_size13(N, i);
   memset(pk->fill, 0, sizeof(pk->fill));
-  pk->nb = pack_bins(nblk, sz, pk->bin, pk->pos, pk->fill);
+  pk->nb = pack_bins(nblk, sz, pk->bin, pk->pos, pk->fill, pk->kb, pk->nk);
   return pk->nb;
 }
 
@@ -417,7 +444,7 @@ static void put_tiles(double *dst, const double *T, const wbfmm_m2l_plan_t *P)
   for (b = 0; b < P->NB; b++)
     for (j = 0; j < P->nk[b]; j++)
       for (l = 0; l < 64; l++) {
-        int r = 16 * b + l % 16, c = 16 * b + 4 * j + l / 16;
+        int r = 16 * b + l % 16, c = P->kbase[b] + 4 * j + l / 16;
         dst[((size_t)P->koff[b] + j) * 64 + l] = T[(size_t)r * S + c];
       }
 }
@@ -468,21 +495,21 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
   int b, n, nu, m, part, i, dx, dy, dz, S;
   double *T, ths[343];
   hcache hc;
-  /* every block must fit one 16-row bin: cos block of degree N has N+1 rows */
-  if (N < 1 || N > 15) return WBFMM_HIP_ERR_ORDER;
+  if (N < 1 || N > 24) return WBFMM_HIP_ERR_ORDER;
   memset(P, 0, sizeof(*P));
   P->N = N;
   P->C = (N + 1) * (N + 1);
   P->NU = (N + 1) * (N + 2) / 2;
-  do_pack(N, 0, &p13);
-  do_pack(N, 1, &p2);
-  /* the two block multisets are equal, so are the bin fills */
-  if (p13.nb != p2.nb || p13.nb > WBFMM_M2L_MAXB) return WBFMM_HIP_ERR_ORDER;
+  if (do_pack(N, 0, &p13) < 0 || do_pack(N, 1, &p2) < 0) return WBFMM_HIP_ERR_ORDER;
+  /* the two block multisets are equal, so are the bins */
+  if (p13.nb != p2.nb || 2 * (N + 1) * (N + 2) / 2 > WBFMM_M2L_MAXU * 2) return WBFMM_HIP_ERR_ORDER;
   for (b = 0; b < p13.nb; b++)
-    if (p13.fill[b] != p2.fill[b]) return WBFMM_HIP_ERR_ORDER;
+    if (p13.fill[b] != p2.fill[b] || p13.kb[b] != p2.kb[b] || p13.nk[b] != p2.nk[b])
+      return WBFMM_HIP_ERR_ORDER;
   P->NB = p13.nb;
   for (b = 0; b < P->NB; b++) {
-    P->nk[b] = (p13.fill[b] + 3) / 4;
+    P->nk[b] = p13.nk[b];
+    P->kbase[b] = p13.kb[b];
     P->koff[b] = P->KT;
     P->KT += P->nk[b];
   }
