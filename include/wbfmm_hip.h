@@ -257,6 +257,14 @@ int wbfmm_hip_lap_field_f32(const double *xt, const uint32_t *tid, unsigned n,
                             const double *dip, unsigned level, const double *x0,
                             double D, int N, const float *mpr, int field,
                             double *f, int fstr);
+/* Helmholtz M2L on matrix cores (Laplace plan of the same order, in the
+   basis P_m = x'_m + x'_{-m}, Q_m = x'_m - x'_{-m} of the phase-rotated
+   expansion): rotation tiles of the plain H(theta) (NTH x KT x 64, both
+   rotations) and coaxial tiles from nr host S|R tables E of order L >= N
+   (nr x 2 x KT x 64: real, imaginary part) */
+int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *plan, double *A);
+int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *plan, int L, int nr,
+                                 const double *E, double *A2);
 /* out[i] = (float)in[i] on the device */
 int wbfmm_hip_to_f32(const double *in, float *out, size_t n);
 /* sort a box list by Morton child digit (stable); returns the per-class
@@ -357,6 +365,20 @@ int wbfmm_hip_hel_l2l(const uint32_t *parents, unsigned npar,
                       const uint32_t *cflag, int Np, int Nc,
                       const double *mpr_parent, double *mpr_child,
                       const double *Hud, size_t nrot, const double *SS);
+/* M2L on matrix cores for Ns = Nr = N in 2..24 (src/pass.c:56-105,
+   :448-537): the targets of level `level' split by Morton child digit
+   (cls, cstart, ccount from wbfmm_hip_split_classes), sources by the
+   level's box ranges; the Laplace plan of order N (plan_dev on the device,
+   plan_host on the host, CS its cos/sin of m chi), the rotation tiles A
+   (wbfmm_hip_hel_m2l_rot_tiles) and the level's coaxial tiles A2
+   (wbfmm_hip_hel_m2l_coax_tiles), all on the device.  Accumulates into
+   mpr; the same sums as wbfmm_hip_hel_m2l to rounding. */
+int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
+                           const unsigned *ccount, const uint32_t *sstart,
+                           const uint32_t *send, unsigned level, int N,
+                           const double *mps, double *mpr, const void *plan_dev,
+                           const double *A, const double *A2, const double *CS,
+                           const wbfmm_m2l_plan_t *plan_host);
 /* M2L for the listed target boxes, accumulating into mpr; Hth: H tables
    of the offset polar angles; th_idx/chi/rix: per-slot geometry (device);
    SR: one S|R table of ne complex entries per offset length

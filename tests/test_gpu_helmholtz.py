@@ -212,3 +212,22 @@ def test_hel_target_list_matches_field_targets(wb, oracle):
     L.wbfmm_target_list_free(tl)
     S.close()
     assert np.abs(g - f).max() < 1e-12 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("p", [2, 5, 13, 16, 20, 23, 24, 25])
+def test_hel_m2l_mfma_orders_match_oracle(wb, oracle, p):
+    # the matrix-core M2L over the orders it serves (2..24: four wavefronts
+    # per block below 16, eight from 16) and the vector M2L above (25); a
+    # second solve gives the same bits
+    rng = np.random.default_rng(40 + p)
+    xs = rng.uniform(-1, 1, (1500, 3))
+    q = rng.normal(size=1500) + 1j * rng.normal(size=1500)
+    k = 3.0
+    print(f"order {p}", flush=True)
+    x0, D = fit_box(xs)
+    fg = gpu_hel(wb, xs, q, xs, x0, D, 3, p, k)
+    fo, _, _ = oracle.fmm(2, xs, xs, x0, D, 3, [p] * 17, [p] * 17, q=c2(q), k=k, field=3,
+                          nthreads=8)
+    for c in range(8):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (p, c, rel(fg[:, c], fo[:, c]))
+    assert np.array_equal(fg, gpu_hel(wb, xs, q, xs, x0, D, 3, p, k))

@@ -266,3 +266,85 @@ def test_width_scaled_tables(wb, oracle, N):
         Lw = emulate(Pm, B1, B3, B2, BCS, M, o)
         Lu = emulate(Pm, B1, B3, B2u, BCS, M / degree_scale(N, w, 0), o)
         assert np.abs(Lu / degree_scale(N, w, 1) - Lw).max() < 1e-12 * np.abs(Lw).max(), off
+
+
+def emulate_hel(P, A, A2, CS, M, o, N):
+    """Helmholtz M2L as the kernel applies the tiles: phase e^{i m chi} in,
+    P/Q basis, the rotation tiles, the complex coaxial tiles (real and
+    imaginary part), the rotation tiles again, back to x_{+-nu} and the
+    phase e^{-i nu chi} out.  M, the result: complex, index n(n+1)+m."""
+    NU, NB = P.NU, P.NB
+    S = P.S + 1
+    th, ri = P.th[o], P.rix[o]
+    x = np.zeros(S, complex)
+    for n in range(N + 1):
+        for m in range(n + 1):
+            u = n * (n + 1) // 2 + m
+            e = np.exp(1j * m * P.chi[o])
+            xp, xm = e * M[n * (n + 1) + m], np.conj(e) * M[n * (n + 1) - m]
+            if m == 0:
+                x[P.s1[u] & 0xffff] = 2 * xp
+            else:
+                x[P.s1[u] & 0xffff] = xp + xm
+                x[(P.s1[u] >> 16) & 0xffff] = xp - xm
+        assert np.allclose([CS[o, m, 0] for m in range(N + 1)],
+                           np.cos(np.arange(N + 1) * P.chi[o]))
+
+    def tiles(T, v):
+        y = np.zeros(S, complex)
+        for b in range(NB):
+            for j in range(P.nk[b]):
+                a = T[P.koff[b] + j].reshape(4, 16).T
+                y[16 * b:16 * b + 16] += a @ v[P.kbase[b] + 4 * j:P.kbase[b] + 4 * j + 4]
+        return y
+
+    def move(y, mp):
+        z = np.zeros(S, complex)
+        for r in range(16 * NB):
+            z[mp[r] >> 6] = y[r]
+        return z
+
+    y = move(tiles(A[th], x), P.map1)
+    z = move(tiles(A2[ri, 0], y) + 1j * tiles(A2[ri, 1], y), P.map2)
+    a = move(tiles(A[th], z), P.map3)
+    L = np.zeros((N + 1) ** 2, complex)
+    for n in range(N + 1):
+        for nu in range(n + 1):
+            u = n * (n + 1) // 2 + nu
+            pp, qq = a[u], a[NU + u]
+            if nu == 0:
+                L[n * n + n] = 0.5 * pp
+            else:
+                L[n * n + n + nu] = np.exp(-1j * nu * P.chi[o]) * 0.5 * (pp + qq)
+                L[n * n + n - nu] = np.exp(1j * nu * P.chi[o]) * 0.5 * (pp - qq)
+    return L
+
+
+@pytest.mark.parametrize("N,L", [(2, 2), (5, 7), (9, 9), (13, 13), (16, 18), (20, 20)])
+def test_hel_plan_matches_oracle_m2l(wb, oracle, N, L):
+    # the Helmholtz M2L tiles over the Laplace plan of the same order
+    # against the oracle's rotate / coaxial S|R / rotate back
+    # (src/pass.c:56-105) for several list-4 directions; the S|R tables
+    # are of order L >= N as when the source and receiver orders differ
+    k, w = 2.3, 0.41
+    P0, _, _, _, CS = wb.m2l_tables(N, w)
+    ne = (L + 1) * (L + 2) * (L + 3) // 6
+    E = np.stack([oracle.hel_coaxial_table(1, L, k * w * P0.rr[i])[:2 * ne] for i in range(P0.NR)])
+    P, A, A2 = wb.hel_m2l_tables(N, L, E)
+    assert P.NTH == 49 and A2.shape == (15, 2, P.KT, 64)
+    rng = np.random.default_rng(100 + N)
+    offs = [(2, 0, 0), (-3, 1, 2), (0, 0, 2), (0, 0, -3), (3, -3, 3), (2, 3, -2), (-2, -2, 0), (0, 3, 1)]
+    for off in offs:
+        o = (off[0] + 3) * 49 + (off[1] + 3) * 7 + off[2] + 3
+        M = np.zeros(2 * (N + 1) ** 2)
+        for _ in range(3):
+            s_ = rng.uniform(-0.5, 0.5, 3) * w
+            oracle.hel_p2m(k, N, np.zeros(3), s_, complex(*rng.normal(size=2)), M)
+        th, ph, ch, r = oracle.shift_angles(*off)
+        H = oracle.H_rotation(N, th)
+        Ef = oracle.hel_coaxial_table(1, L, k * w * r)
+        Lref = oracle.hel_rotate(oracle.hel_coaxial(oracle.hel_rotate(M, N, H, ph, ch), N, N, Ef, L, 1),
+                                 N, H, ch, ph)
+        Lref = Lref[0::2] + 1j * Lref[1::2]
+        Lc = emulate_hel(P, A, A2, CS, M[0::2] + 1j * M[1::2], o, N)
+        assert np.abs(Lc - Lref).max() < 1e-12 * np.abs(Lref).max(), off

@@ -82,6 +82,26 @@ def m2l_tables(N, w):
     return P, A1, A3, A2, CS
 
 
+def hel_m2l_tables(N, L, E):
+    """host copy of the Helmholtz M2L tiles of order N: (plan, A (NTH, KT,
+    64) for both rotations, A2 (NR, 2, KT, 64) from the NR host S|R tables
+    E of order L, each 2 (L+1)(L+2)(L+3)/6 doubles)"""
+    P = M2LPlan()
+    rc = lib().wbfmm_hip_lap_m2l_tables(N, ctypes.byref(P), None, None, None)
+    if rc != 0:
+        raise RuntimeError(f"wbfmm_hip_lap_m2l_tables({N}) = {rc}")
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    nr = len(E)
+    A = np.zeros((P.NTH, P.KT, 64))
+    A2 = np.zeros((nr, 2, P.KT, 64))
+    rc = lib().wbfmm_hip_hel_m2l_rot_tiles(ctypes.byref(P), ptr(A))
+    if rc == 0:
+        rc = lib().wbfmm_hip_hel_m2l_coax_tiles(ctypes.byref(P), L, nr, ptr(E), ptr(A2))
+    if rc != 0:
+        raise RuntimeError(f"Helmholtz M2L tiles of order {N}: status {rc}")
+    return P, A, A2
+
+
 _SIGS = {
     "wbfmm_tree_new": (ctypes.POINTER(WbfmmTree), [_vp, _d, _u]),
     "wbfmm_tree_add_points": (_i, [_vp, _vp, _sz, _vp, _sz, _u, _i]),
@@ -131,6 +151,8 @@ _SIGS = {
     "wbfmm_tree_set_precision": (_i, [_vp, _i]),
     "wbfmm_hip_lap_m2l_tables": (_i, [_i, _vp, _vp, _vp, _vp]),
     "wbfmm_hip_lap_m2l_coax": (_i, [_vp, _d, _vp]),
+    "wbfmm_hip_hel_m2l_rot_tiles": (_i, [_vp, _vp]),
+    "wbfmm_hip_hel_m2l_coax_tiles": (_i, [_vp, _i, _i, _vp, _vp]),
     "wbfmm_tree_coefficient_init": (_i, [_vp, _u, _u, _u]),
     "wbfmm_tree_leaf_expansions": (_i, [_vp, _d, _vp, _i, _vp, _i, _i, _vp]),
     "wbfmm_shift_operators_coaxial_SR_init": (_i, [_vp, _d, _u, _u, _d, _vp]),

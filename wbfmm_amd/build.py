@@ -17,7 +17,7 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwbfmm.so")
 
-HIP_SOURCES = ["runtime.hip", "laplace.hip", "m2l_mfma.hip", "helmholtz.hip"]
+HIP_SOURCES = ["runtime.hip", "laplace.hip", "m2l_mfma.hip", "hel_m2l_mfma.hip", "helmholtz.hip"]
 C_SOURCES = ["wbfmm_api.c", "wbfmm_tables.c"]
 CXX_SOURCES = ["serve.cpp"]
 

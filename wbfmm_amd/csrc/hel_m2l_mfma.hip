@@ -1,0 +1,446 @@
+// hel_m2l_mfma.hip -- Helmholtz M2L as FP64 matrix-core products (gfx950).
+//
+// The reference translates each source expansion x by rotating it into the
+// frame of the offset (H(theta), phases e^{i m chi} in and e^{-i nu phi}
+// out), shifting it along the axis with the complex coaxial S|R table, and
+// rotating back (phases e^{i m phi} in, e^{-i nu chi} out)
+// (src/pass.c:56-105).  The phi phases of the two rotations meet across the
+// coaxial stage, which keeps the order m, and cancel.  With the
+// phase-rotated x'_m = e^{i m chi} x_m and
+//   P_m = x'_m + x'_{-m},  Q_m = x'_m - x'_{-m}   (m >= 0, P_0 = 2 x'_0)
+// the real H(theta) of degree n, for which h(-nu,-m) = h(nu,m), splits into
+//   a cos block  A_{nu m} = h(nu,m) + h(-nu,m) (m > 0), A_{nu 0} = h(nu,0)
+//   on P (nu, m = 0..n), and a sin block B_{nu m} = h(nu,m) - h(-nu,m) on
+//   Q (nu, m = 1..n);
+// the coaxial table of order m acts alike on P and Q; and after the second
+// rotation (the same tiles) y_nu = (P + Q)/2, y_{-nu} = (P - Q)/2 and
+// L_n^nu += e^{-i nu chi} y_nu.  These are the block shapes of the Laplace
+// M2L, so the Laplace plan of the same order (bins, slot maps, per-offset
+// theta and distance indices, the cos/sin of m chi) carries over: a block
+// of W = 4 (orders below 16) or 8 wavefronts takes 8 target boxes with the
+// same Morton child digit,
+// and the 16 MFMA columns are (target t, part s) = 2t + s with s = 0 the
+// real and s = 1 the imaginary part.  The rotation tiles are real and act
+// on both parts at once; the coaxial stage takes two products per tile step,
+// the real part of the table on the column and the imaginary part on the
+// partner column (col ^ 1).  Wavefront w computes the bins b = w (mod W) of
+// every stage; its lane (col, grp) owns the units u = grp + 4 w + 4 W i: for each it loads x_{+m} (s = 0) or x_{-m} (s = 1),
+// applies its phase, exchanges the result with its partner lane and writes
+// the real (s = 0) or imaginary (s = 1) parts of P and Q; after the third
+// stage it reads both parts of P and Q and accumulates L^{+nu} (s = 0) or
+// L^{-nu} (s = 1).  Expansions are box-major, (N+1)^2 complex coefficients
+// per box at n(n+1)+m.  Every LDS slot has one writer and the sums run in
+// a fixed order: results do not depend on the launch.
+
+#include <cstdio>
+
+#include <hip/hip_runtime.h>
+
+#include "hip_common.h"
+#include "m2l_mfma.h"
+
+namespace wbfmm {
+
+// stage 2: complex coaxial tiles ar + i ai.  The imaginary product reads
+// the partner column, so that column 2t receives Ar Zr - Ai Zi and column
+// 2t+1 receives Ar Zi + Ai Zr.
+template <int N, int W, int WI>
+__device__ __forceinline__ void stage_cx(const double *__restrict__ ar,
+                                         const double *__restrict__ ai, const double *R,
+                                         int lane, int grp, int col, d4 *y) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+  constexpr int AD = WS.n < 8 ? (WS.n > 0 ? WS.n : 1) : 8;
+  double avr[AD], avi[AD];
+#pragma unroll
+  for (int s = 0; s < AD && s < WS.n; s++) {
+    avr[s] = ar[WS.t[s] * 64 + lane];
+    avi[s] = ai[WS.t[s] * 64 + lane];
+  }
+  const double sg = (col & 1) ? 1.0 : -1.0;
+  const int pc = col ^ 1;
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    d4 r4 = {0.0, 0.0, 0.0, 0.0}, i4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      const int row = (BN.kbase[b] + 4 * j + grp) * 16;
+      r4 = MF<double>::mma(avr[s % AD], R[row + col], r4);
+      i4 = MF<double>::mma(avi[s % AD], R[row + pc], i4);
+      if (s + AD < WS.n) {
+        avr[s % AD] = ar[WS.t[s + AD] * 64 + lane];
+        avi[s % AD] = ai[WS.t[s + AD] * 64 + lane];
+      }
+    }
+    y[lb] = r4 + sg * i4;
+  }
+}
+
+template <int N, int W, int WI>
+__device__ __forceinline__ void hel_m2l_loop(
+    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    double2 (*cs)[N + 1], unsigned (*vm)[7], const unsigned *anym, int lane, int col, int grp,
+    bool tvalid, uint32_t tb, int ti, int tj, int tk, int px, int py, int pz, int c,
+    unsigned k8, int split, int nsplit, const ClassArgs &ca,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
+    const double *__restrict__ A2, const double *__restrict__ CS,
+    const double2 *__restrict__ zero, const double2 *__restrict__ mps,
+    double2 *__restrict__ mpr, double2 *__restrict__ part, unsigned ntot) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
+  constexpr int NBW = WS.nb;
+  const int ubase = grp + 4 * WI, tg = col >> 1;
+
+  auto next_k = [&](int k) {
+    for (k++; k < 216; k++)
+      if (((anym[k >> 5] >> (k & 31)) & 1u) && k % nsplit == split) break;
+    return k;
+  };
+  auto offset_of = [&](int k) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+  };
+  // every wavefront writes the offset's factors (the same values)
+  auto source_of = [&](int k, int p) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (lane <= N) {
+      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = make_double2(t[0], t[1]);
+    }
+    const bool v = (vm[tg][k >> 5] >> (k & 31)) & 1u;
+    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
+                          : 0;
+    return v ? mps + sb * (uint64_t)C : zero;
+  };
+  double pre_re[NAW], pre_im[NAW];
+  // x_{+m} on the real lane, x_{-m} on the imaginary one
+  auto load_sources = [&](const double2 *src, const int4 *tu, int ub, int sp) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 q = src[inf.x + (sp ? -inf.y : inf.y)];
+      pre_re[i] = q.x;
+      pre_im[i] = q.y;
+    }
+  };
+  // x'_{+-m} = e^{+-i m chi} x_{+-m}; the partner lane holds the other
+  // sign.  Lanes past the last unit (loaded as unit NU-1) exchange but
+  // write nothing.
+  auto deposit = [&](const int4 *tu, int ub, int cl, int p) {
+    const int sp = cl & 1;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 e = cs[p][inf.y];
+      const double es = sp ? -e.y : e.y;
+      const double xr = e.x * pre_re[i] - es * pre_im[i];
+      const double xi = es * pre_re[i] + e.x * pre_im[i];
+      const double yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
+      if (ub + 4 * W * i >= NU) continue;
+      // real lane: x' = x'_{+m}, partner x'_{-m}; imaginary lane: the reverse
+      R[inf.z * 16 + cl] = sp ? yi + xi : xr + yr;
+      R[inf.w * 16 + cl] = sp ? yi - xi : xr - yr;
+    }
+  };
+  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + W * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  double acc_re[NAW], acc_im[NAW];
+#pragma unroll
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = 0.0;
+
+  // k is the same in every lane (it comes from the shared anym): held as a
+  // scalar, and the next offset's table indices are read one offset ahead
+  int k = __builtin_amdgcn_readfirstlane(next_k(-1)), p = 0;
+  int th = 0, ri = 0;
+  if (k < 216) {
+    th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
+    ri = __builtin_amdgcn_readfirstlane(P->rix[offset_of(k)]);
+    load_sources(source_of(k, 0), uinf, ubase, col & 1);
+    deposit(uinf, ubase, col, 0);
+  }
+  __syncthreads();
+  while (k < 216) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI, sp = cl & 1;
+    const int kn = __builtin_amdgcn_readfirstlane(next_k(k));
+    int th_n = 0, ri_n = 0;
+    if (kn < 216) {
+      th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
+      ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
+    }
+
+    d4 y[NBW > 0 ? NBW : 1];
+    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t1, y, gp, cl);
+    __syncthreads();
+    if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
+
+    stage_cx<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64, R,
+                       ln, gp, cl, y);
+    __syncthreads();
+    write_out(t2, y, gp, cl);
+    __syncthreads();
+
+    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t3, y, gp, cl);
+    __syncthreads();
+
+    // y_{+-nu} = (P +- Q)/2 (y_0 = P/2), then L^{+-nu} += e^{-+i nu chi} y
+    {
+      const int c0 = cl & ~1;
+#pragma unroll
+      for (int i = 0; i < NAW; i++) {
+        const int u = min(ub + 4 * W * i, NU - 1);
+        const int nu = tu[u].y;
+        const double2 e = cs[p][nu];
+        const double es = sp ? -e.y : e.y;
+        const double pr = R[u * 16 + c0], pi = R[u * 16 + c0 + 1];
+        const double qr = nu ? R[(NU + u) * 16 + c0] : 0.0;
+        const double qi = nu ? R[(NU + u) * 16 + c0 + 1] : 0.0;
+        const double yr = 0.5 * (sp ? pr - qr : pr + qr);
+        const double yi = 0.5 * (sp ? pi - qi : pi + qi);
+        acc_re[i] += e.x * yr + es * yi;
+        acc_im[i] += e.x * yi - es * yr;
+      }
+    }
+    __syncthreads();  // every read of R done before the next deposit
+
+    if (kn < 216) deposit(tu, ub, cl, p ^ 1);
+    __syncthreads();
+    p ^= 1;
+    k = kn;
+    th = th_n;
+    ri = ri_n;
+  }
+
+  if (!tvalid) return;
+  const int sp = col & 1;
+  double2 *dst = nsplit > 1 ? part + ((size_t)split * ntot + ca.start[c] + k8) * C
+                            : mpr + (size_t)tb * C;
+#pragma unroll
+  for (int i = 0; i < NAW; i++) {
+    const int u = ubase + 4 * W * i;
+    if (u >= NU) continue;
+    const int4 inf = uinf[u];
+    if (sp && inf.y == 0) continue;  // L^0 is the real lane's
+    const int r = inf.x + (sp ? -inf.y : inf.y);
+    if (nsplit > 1) {
+      dst[r] = make_double2(acc_re[i], acc_im[i]);
+    } else {
+      const double2 o = dst[r];
+      dst[r] = make_double2(o.x + acc_re[i], o.y + acc_im[i]);
+    }
+  }
+}
+
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
+    const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ, int level,
+    const double2 *__restrict__ mps, double2 *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
+    const double *__restrict__ A2, const double *__restrict__ CS,
+    const double2 *__restrict__ zero, int nsplit, double2 *__restrict__ part, unsigned ntot) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  __shared__ double R[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[2][N + 1];
+  __shared__ unsigned vm[8][7];
+  __shared__ unsigned anym[7];
+
+  // wv through readfirstlane: a scalar value, so the branch into the
+  // per-wavefront code below is a scalar branch
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  // unit u = (n, m): index of x_n^0, m, the slots of P_m and Q_m
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < 8 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
+
+  const int split = (int)(blockIdx.x % (unsigned)nsplit);
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
+  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+  int c = 0;
+  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  const int tg = col >> 1;
+  const unsigned k8 = (chunk - ca.chunk_prefix[c]) * 8 + tg;
+  const bool tvalid = k8 < ca.count[c];
+  const uint32_t tb = tvalid ? cls[ca.start[c] + k8] : 0u;
+  const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  __syncthreads();
+
+  // occupied list-4 sources per target (the real lane of each target)
+  if ((col & 1) == 0) {
+    for (int k = grp + 4 * wv; k < 216; k += 4 * W) {
+      const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+      if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+      const int si = ti + dx, sj = tj + dy, sk = tk + dz;
+      if (!tvalid || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox)
+        continue;
+      const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+      if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[tg][k >> 5], 1u << (k & 31));
+    }
+  }
+  __syncthreads();
+  if (tid < 7) {
+    unsigned a = 0;
+    for (int j = 0; j < 8; j++) a |= vm[j][tid];
+    anym[tid] = a;
+  }
+  __syncthreads();
+
+#define HEL_LOOP(WI)                                                                          \
+  hel_m2l_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid,  \
+                         tb, ti, tj, tk, px, py, pz, c, k8, split, nsplit, ca, P, A, A2, CS, \
+                         zero, mps, mpr, part, ntot)
+  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: HEL_LOOP(0); break;
+      case 1: HEL_LOOP(1); break;
+      case 2: HEL_LOOP(2); break;
+      default: HEL_LOOP(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: HEL_LOOP(0); break;
+      case 1: HEL_LOOP(1); break;
+      case 2: HEL_LOOP(2); break;
+      case 3: HEL_LOOP(3); break;
+      case 4: HEL_LOOP(4); break;
+      case 5: HEL_LOOP(5); break;
+      case 6: HEL_LOOP(6); break;
+      default: HEL_LOOP(7); break;
+    }
+  }
+#undef HEL_LOOP
+}
+
+template <int N>
+static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
+                          const uint32_t *occ, int level, const double *mps, double *mpr,
+                          const void *plan, const double *A, const double *A2,
+                          const double *CS, const double *zero,
+                          const wbfmm_m2l_plan_t *hplan) {
+  if (!plan_matches<N>(hplan)) {
+    fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
+    return WBFMM_HIP_ERR_ARG;
+  }
+  // the offsets of a chunk split over up to 8 blocks when there are few
+  // chunks (the split depends on the chunk count only: fixed summation
+  // order for a given problem)
+  constexpr int C = (N + 1) * (N + 1);
+  unsigned ntot = 0;
+  for (int c = 0; c < 8; c++) ntot += ca.count[c];
+  const int nsplit = nchunks >= 1024 ? 1 : (int)(2048 / nchunks < 8 ? 2048 / nchunks : 8);
+  double2 *part = nullptr;
+  if (nsplit > 1 && dmalloc((void **)&part, sizeof(double2) * (size_t)nsplit * ntot * C))
+    return WBFMM_HIP_ERR_NOMEM;
+  // eight wavefronts per block from order 16, where four would need more
+  // than 256 registers each (the LDS buffer allows one block per CU there)
+  constexpr int W = N >= 16 ? 8 : 4;
+  WBFMM_LAUNCH("hel_m2l", (k_hel_m2l_mfma<N, W>), dim3(nchunks * nsplit), dim3(64 * W), 0, cls, ca,
+               occ, level, (const double2 *)mps, (double2 *)mpr,
+               (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero, nsplit, part,
+               ntot);
+  if (nsplit > 1) {
+    const size_t ne = (size_t)ntot * 2 * C;
+    WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<double>, dim3((unsigned)((ne + 255) / 256)),
+                 dim3(256), 0, cls, ntot, nsplit, 2 * C, (const double *)part, mpr);
+    dfree(part);
+  }
+  return 0;
+}
+
+}  // namespace wbfmm
+
+using namespace wbfmm;
+
+extern "C" {
+
+int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
+                           const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
+                           const double *mps, double *mpr, const void *plan_dev,
+                           const double *A, const double *A2, const double *CS,
+                           const wbfmm_m2l_plan_t *plan_host) {
+  ClassArgs ca;
+  unsigned nch = 0;
+  for (int c = 0; c < 8; c++) {
+    ca.start[c] = cstart[c];
+    ca.count[c] = ccount[c];
+    ca.chunk_prefix[c] = nch;
+    nch += (ccount[c] + 7) / 8;
+  }
+  ca.chunk_prefix[8] = nch;
+  if (nch == 0) return 0;
+  if (level < 2 || level > 20) return WBFMM_HIP_ERR_ARG;
+  if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
+  if (!plan_host || plan_host->N != N) return WBFMM_HIP_ERR_ARG;
+  {
+    const int rz = ensure_zero();
+    if (rz) return rz;
+  }
+  uint32_t *occ = nullptr;
+  {
+    const int ro = occupancy_bits(sstart, send, level, &occ);
+    if (ro) return ro;
+  }
+  int rc = WBFMM_HIP_ERR_ORDER;
+  switch (N) {
+#define CASE(n)                                                                             \
+  case n:                                                                                   \
+    rc = launch_hel_m2l<n>(cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A, A2, CS,    \
+                           zero_expansion(), plan_host);                                    \
+    break;
+    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
+#undef CASE
+    default:
+      break;
+  }
+  dfree(occ);
+  return rc;
+}
+
+}  // extern "C"

@@ -42,144 +42,9 @@
 #include <mutex>
 
 #include "hip_common.h"
+#include "m2l_mfma.h"
 
 namespace wbfmm {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-// The kernels run in FP64 (v_mfma_f64_16x16x4f64) or FP32
-// (v_mfma_f32_16x16x4f32).  Both take A[l%16][l/16] and B[l/16][l%16] from
-// lane l; the four results of a lane are rows drow(l/16, q) of column l%16
-// (checked by wbfmm_hip_mfma_layout_probe and _probe_f32).
-template <typename T> struct MF;
-template <> struct MF<double> {
-  typedef d4 v4;
-  typedef double2 t2;
-  static __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static constexpr __host__ __device__ int drow(int g, int q) { return g + 4 * q; }
-};
-template <> struct MF<float> {
-  typedef f4 v4;
-  typedef float2 t2;
-  static __device__ __forceinline__ f4 mma(float a, float b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static constexpr __host__ __device__ int drow(int g, int q) { return 4 * g + q; }
-};
-template <typename T>
-__device__ __forceinline__ typename MF<T>::t2 mk2(T a, T b) {
-  typename MF<T>::t2 v;
-  v.x = a;
-  v.y = b;
-  return v;
-}
-
-struct ClassArgs {
-  unsigned start[8], count[8];
-  unsigned chunk_prefix[9];
-};
-
-// Compile-time copy of the bin packing of wbfmm_tables.c (pack_bins over
-// the cos/sin blocks, sizes n+1 and n): number of bins and the k steps of
-// each bin, so that every MFMA loop has a constant trip count.
-struct Bins {
-  int nb, kt;
-  int nk[WBFMM_M2L_MAXB], koff[WBFMM_M2L_MAXB], kbase[WBFMM_M2L_MAXB];
-};
-
-// blocks of more than 16 rows take consecutive bins of their own, whose
-// row tiles all read the block's full column range from its first row
-constexpr Bins make_bins(int N) {
-  Bins B{};
-  int sz[2 * WBFMM_M2L_MAXN + 2] = {}, order[2 * WBFMM_M2L_MAXN + 2] = {};
-  int fill[WBFMM_M2L_MAXB] = {}, kb[WBFMM_M2L_MAXB] = {}, nk[WBFMM_M2L_MAXB] = {};
-  const int nblk = 2 * N + 1;
-  for (int i = 0; i < nblk; i++) {
-    sz[i] = i <= N ? i + 1 : i - N;
-    order[i] = i;
-  }
-  for (int i = 1; i < nblk; i++) {
-    int v = order[i], j = i;
-    for (; j > 0 && sz[order[j - 1]] < sz[v]; j--) order[j] = order[j - 1];
-    order[j] = v;
-  }
-  int nb = 0;
-  for (int i = 0; i < nblk; i++) {
-    int k = order[i], b = 0;
-    if (sz[k] > 16) {
-      const int nt = (sz[k] + 15) / 16;
-      for (int t = 0; t < nt; t++) {
-        fill[nb + t] = 16;
-        kb[nb + t] = 16 * nb;
-        nk[nb + t] = (sz[k] + 3) / 4;
-      }
-      nb += nt;
-      continue;
-    }
-    for (; b < nb; b++)
-      if (fill[b] + sz[k] <= 16) break;
-    if (b == nb) {
-      fill[nb] = 0;
-      kb[nb] = -1;
-      nb++;
-    }
-    fill[b] += sz[k];
-  }
-  B.nb = nb;
-  for (int b = 0; b < nb; b++) {
-    if (kb[b] < 0) {
-      kb[b] = 16 * b;
-      nk[b] = (fill[b] + 3) / 4;
-    }
-    B.nk[b] = nk[b];
-    B.kbase[b] = kb[b];
-    B.koff[b] = B.kt;
-    B.kt += B.nk[b];
-  }
-  return B;
-}
-
-template <int N>
-struct M2LShape {
-  static constexpr Bins BN = make_bins(N);
-  static constexpr int NB = BN.nb;
-  static constexpr int KT = BN.kt;
-  static constexpr int C = (N + 1) * (N + 1);
-  static constexpr int NU = (N + 1) * (N + 2) / 2;
-  static constexpr int S = (16 * NB > 2 * NU) ? 16 * NB : 2 * NU;  // = plan S
-  static constexpr int NAU = (NU + 3) / 4;         // units per lane
-  // register prefetch of the next offset's sources while it fits
-  static constexpr bool PREFETCH = N <= 12;
-};
-
-// one stage: y[b] = tile(b) * R[bin b's k slots].  The A operands (lane
-// order) run through a ring of AD registers: step t is requested AD steps
-// before its product, so a product waits on a load issued long before.
-// `a` is wave-uniform (scalar base) and the lane index is added per load:
-// the loads use a scalar base plus a 32-bit lane offset.
-template <typename T, int NB, int KT>
-__device__ __forceinline__ void stage(const Bins &BN, const T *__restrict__ a,
-                                      const T *R, int lane, int grp, int col,
-                                      typename MF<T>::v4 *y) {
-  constexpr int AD = KT < 16 ? KT : 16;
-  T av[AD];
-#pragma unroll
-  for (int t = 0; t < AD; t++) av[t] = a[t * 64 + lane];
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int j = 0; j < BN.nk[b]; j++) {
-      const int t = BN.koff[b] + j;
-      r4 = MF<T>::mma(av[t % AD], R[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
-      if (t + AD < KT) av[t % AD] = a[(t + AD) * 64 + lane];
-    }
-    y[b] = r4;
-  }
-}
 
 template <typename T, int N>
 __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
@@ -443,46 +308,6 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 // may differ by the rounding of contracted (fused) expressions, and each
 // is deterministic: every LDS slot has one writer.
 // ---------------------------------------------------------------------
-
-// k-steps of the bins b = WI (mod W), in bin order
-struct WaveSeq {
-  int n, nb;
-  int t[8 * WBFMM_M2L_MAXB];
-  int start[WBFMM_M2L_MAXB];
-};
-
-constexpr WaveSeq make_seq(const Bins &B, int W, int WI) {
-  WaveSeq s{};
-  for (int b = WI; b < B.nb; b += W) {
-    s.start[s.nb++] = s.n;
-    for (int j = 0; j < B.nk[b]; j++) s.t[s.n++] = B.koff[b] + j;
-  }
-  return s;
-}
-
-template <typename T, int N, int W, int WI>
-__device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int lane, int grp,
-                                        int col, typename MF<T>::v4 *y) {
-  constexpr Bins BN = M2LShape<N>::BN;
-  constexpr WaveSeq WS = make_seq(BN, W, WI);
-  // a wavefront may have no bins at low orders (AD then only sizes av)
-  constexpr int AD = WS.n < 16 ? (WS.n > 0 ? WS.n : 1) : 16;
-  T av[AD];
-#pragma unroll
-  for (int s = 0; s < AD && s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
-#pragma unroll
-  for (int lb = 0; lb < WS.nb; lb++) {
-    const int b = WI + W * lb;
-    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-    for (int j = 0; j < BN.nk[b]; j++) {
-      const int s = WS.start[lb] + j;
-      r4 = MF<T>::mma(av[s % AD], R[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
-      if (s + AD < WS.n) av[s % AD] = a[WS.t[s + AD] * 64 + lane];
-    }
-    y[lb] = r4;
-  }
-}
 
 template <typename T, int N, int W, int WI>
 __device__ __forceinline__ void m2l_w2_loop(
@@ -777,18 +602,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   }
 }
 
-// mpr[cls[i]] += sum over the splits (in split order) of the partials of
-// target i
-template <typename T>
-__global__ void k_m2l_reduce(const uint32_t *__restrict__ cls, unsigned ntot, int nsplit,
-                             int C, const T *__restrict__ part, T *__restrict__ mpr) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)ntot * C) return;
-  const size_t i = e / C, c = e % C;
-  T s = part[i * C + c];
-  for (int sp = 1; sp < nsplit; sp++) s += part[((size_t)sp * ntot + i) * C + c];
-  mpr[(size_t)cls[i] * C + c] += s;
-}
 
 // ---------------------------------------------------------------------
 // M2M (MODE 0, src/shift-laplace.c:225-442) and L2L (MODE 1, :562-789) in
@@ -969,6 +782,19 @@ __global__ void k_occ_bits(const uint32_t *start, const uint32_t *end,
   bits[wd] = v;
 }
 
+// the occupancy bitmap of a level of 2^(3 level) boxes into stream-ordered
+// memory that the caller releases with dfree
+int occupancy_bits(const uint32_t *sstart, const uint32_t *send, unsigned level,
+                   uint32_t **occ) {
+  const uint64_t nbox = (uint64_t)1 << (3 * level);
+  const size_t words = (size_t)((nbox + 31) / 32);
+  *occ = nullptr;
+  if (dmalloc((void **)occ, words * sizeof(uint32_t))) return WBFMM_HIP_ERR_NOMEM;
+  WBFMM_LAUNCH("m2l_occ_bits", k_occ_bits, dim3((unsigned)((words + 255) / 256)), dim3(256), 0,
+               sstart, send, nbox, *occ);
+  return 0;
+}
+
 // one 16x16x4 product with known operands; out gets D in lane order
 __global__ void k_mfma_layout_probe(double *out) {
   const int l = threadIdx.x;
@@ -1003,15 +829,6 @@ __global__ void k_class_count(const uint32_t *list, unsigned n,
   if (threadIdx.x < 8 && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
-template <int N>
-static bool plan_matches(const wbfmm_m2l_plan_t *hp) {
-  constexpr Bins B = make_bins(N);
-  if (hp->N != N || hp->NB != B.nb || hp->KT != B.kt || hp->S != M2LShape<N>::S) return false;
-  for (int b = 0; b < B.nb; b++)
-    if (hp->nk[b] != B.nk[b] || hp->koff[b] != B.koff[b] || hp->kbase[b] != B.kbase[b])
-      return false;
-  return true;
-}
 
 // wavefronts per 16-target chunk of the M2L kernel: 4 (every order and
 // precision), or 2 / 1 when WBFMM_M2L_WAVES says so (kept for comparison;
@@ -1096,15 +913,15 @@ static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *
   return 0;
 }
 
-// a zero expansion read by the kernels in place of absent boxes (its bytes
-// are zero for either precision); made once and only read afterwards
+// the zero expansion (its bytes are zero for either precision, real or
+// complex coefficients)
 static double *g_zero = nullptr;
 static std::mutex g_zero_mu;
 
-static int ensure_zero() {
+int ensure_zero() {
   std::lock_guard<std::mutex> lk(g_zero_mu);
   if (!g_zero) {
-    const size_t bytes = sizeof(double) * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1);
+    const size_t bytes = sizeof(double) * 2 * (WBFMM_M2L_MAXN + 1) * (WBFMM_M2L_MAXN + 1);
     double *z = nullptr;
     WBFMM_CHECK(hipMalloc(&z, bytes));
     WBFMM_CHECK(hipMemsetAsync(z, 0, bytes, stream()));
@@ -1114,6 +931,8 @@ static int ensure_zero() {
   }
   return 0;
 }
+
+const double *zero_expansion() { return g_zero; }
 
 template <typename T>
 static int shift_entry(int mode, const uint32_t *parents, unsigned npar,

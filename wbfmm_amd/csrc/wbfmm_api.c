@@ -117,6 +117,16 @@ typedef struct {
   size_t hel_nrot;
   int *d_th_idx, *d_rix;
   double *d_chi;
+  /* Helmholtz M2L on matrix cores: rotation tiles per order (shared by
+     the process), the host S|R tables of each level (kept for the tiles)
+     with the offset lengths they follow, and the coaxial tiles of each
+     level with the order they were made for */
+  double *hel_A[WBFMM_HIP_MAX_ORDER + 1];
+  double *sr_host[WBFMM_TREE_MAX_DEPTH + 1];
+  double sr_rr[16];
+  int sr_nr;
+  double *hel_A2[WBFMM_TREE_MAX_DEPTH + 1];
+  int hel_A2_N[WBFMM_TREE_MAX_DEPTH + 1];
 } ops_dev;
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
@@ -766,8 +776,10 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
   for (n = 0; n <= WBFMM_TREE_MAX_DEPTH; n++) {
     wbfmm_hip_free(od->op.SR[n]);
     wbfmm_hip_free(od->op.SS[n]);
+    wbfmm_hip_free(od->hel_A2[n]);
+    free(od->sr_host[n]);
   }
-  /* d_hel_* point to the process-wide tables of the order: not freed */
+  /* d_hel_* and hel_A point to the process-wide tables: not freed */
   free(od);
   return 0;
 }
@@ -1590,7 +1602,13 @@ gint wbfmm_shift_operators_coaxial_SR_init(wbfmm_shift_operators_t *w, gdouble D
   if (w->SR[level]) wbfmm_hip_free(w->SR[level]);
   HIPOK(wbfmm_hip_malloc(&w->SR[level], sizeof(double) * 2 * ne * nr));
   HIPOK(wbfmm_hip_memcpy(w->SR[level], h, sizeof(double) * 2 * ne * nr));
-  free(h);
+  /* the host copy stays for the matrix-core tiles, made on first use */
+  free(od->sr_host[level]);
+  od->sr_host[level] = h;
+  memcpy(od->sr_rr, rr, sizeof(double) * nr);
+  od->sr_nr = nr;
+  wbfmm_hip_free(od->hel_A2[level]);
+  od->hel_A2[level] = NULL;
   w->L[level] = L;
   return 0;
 }
@@ -1629,6 +1647,10 @@ gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *w)
     w->SR[n] = w->SS[n] = NULL;
     w->L[n] = 0;
     od->ss_L[n] = 0;
+    wbfmm_hip_free(od->hel_A2[n]);
+    od->hel_A2[n] = NULL;
+    free(od->sr_host[n]);
+    od->sr_host[n] = NULL;
   }
   od->hel_k_set = 0;
   return 0;
@@ -1713,6 +1735,70 @@ gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level
   return 0;
 }
 
+/* the matrix-core Helmholtz M2L is used for equal orders in
+   M2L_MFMA_MIN..M2L_MFMA_MAX unless WBFMM_HEL_M2L=vector asks for the
+   vector kernel (kept for comparison; the results agree to rounding) */
+static int hel_mfma_enabled(void)
+{
+  static int on = -1;
+  if (on < 0) {
+    const char *e = getenv("WBFMM_HEL_M2L");
+    on = !(e && strcmp(e, "vector") == 0);
+  }
+  return on;
+}
+
+/* rotation tiles of the Helmholtz matrix-core M2L of order N (both
+   rotations), built once per process */
+static void ensure_hel_rot_tiles(ops_dev *od, int N)
+{
+  if (od->hel_A[N]) return;
+  pthread_mutex_lock(&g_tab_mu);
+  if (!g_tab.hel_A[N]) {
+    const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+    size_t n = (size_t)P->NTH * P->KT * 64;
+    double *h = malloc(sizeof(double) * n), *d;
+    if (!h) WBFMM_FATAL("out of host memory");
+    HIPOK(wbfmm_hip_hel_m2l_rot_tiles(P, h));
+    HIPOK(wbfmm_hip_malloc((void **)&d, sizeof(double) * n));
+    HIPOK(wbfmm_hip_memcpy(d, h, sizeof(double) * n));
+    free(h);
+    g_tab.hel_A[N] = d;
+  }
+  od->hel_A[N] = g_tab.hel_A[N];
+  pthread_mutex_unlock(&g_tab_mu);
+}
+
+/* coaxial tiles of the Helmholtz matrix-core M2L at `level' for order N,
+   from the level's host S|R tables (ordered as the plan's offset lengths) */
+static void ensure_hel_coax_tiles(ops_dev *od, guint level, int N)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  size_t ne = coax_entries(od->op.L[level]), n = (size_t)P->NR * 2 * P->KT * 64;
+  double *E, *h;
+  int i, j;
+  if (od->hel_A2[level] && od->hel_A2_N[level] == N) return;
+  if (!od->sr_host[level]) WBFMM_FATAL("S|R operators not initialised at level %u", level);
+  if (od->sr_nr != P->NR) WBFMM_FATAL("%d offset lengths, the plan has %d", od->sr_nr, P->NR);
+  E = malloc(sizeof(double) * 2 * ne * P->NR);
+  h = malloc(sizeof(double) * n);
+  if (!E || !h) WBFMM_FATAL("out of host memory");
+  for (i = 0; i < P->NR; i++) {
+    for (j = 0; j < od->sr_nr; j++)
+      if (od->sr_rr[j] == P->rr[i]) break;
+    if (j == od->sr_nr) WBFMM_FATAL("offset length %g not in the S|R tables", P->rr[i]);
+    memcpy(E + (size_t)i * 2 * ne, od->sr_host[level] + (size_t)j * 2 * ne,
+           sizeof(double) * 2 * ne);
+  }
+  HIPOK(wbfmm_hip_hel_m2l_coax_tiles(P, (int)od->op.L[level], P->NR, E, h));
+  wbfmm_hip_free(od->hel_A2[level]);
+  HIPOK(wbfmm_hip_malloc((void **)&od->hel_A2[level], sizeof(double) * n));
+  HIPOK(wbfmm_hip_memcpy(od->hel_A2[level], h, sizeof(double) * n));
+  od->hel_A2_N[level] = N;
+  free(E);
+  free(h);
+}
+
 static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level)
 {
   tree_dev *td = TD(t);
@@ -1737,10 +1823,27 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
   } else {
     tl = all_boxes(level, &nt);
   }
-  HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
-                          (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
-                          od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
-                          (double *)op->SR[level], coax_entries(op->L[level])));
+  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= M2L_MFMA_MAX && hel_mfma_enabled()) {
+    /* matrix-core form: targets grouped by Morton child digit */
+    int N = (int)Ns;
+    unsigned cstart[8], ccount[8];
+    uint32_t *cls;
+    ensure_m2l_plan(od, N);
+    ensure_hel_rot_tiles(od, N);
+    ensure_hel_coax_tiles(od, level, N);
+    HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
+    HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
+    HIPOK(wbfmm_hip_hel_m2l_mfma(cls, cstart, ccount, td->d_start[level], td->d_end[level],
+                                 level, N, (double *)t->mps[level], (double *)t->mpr[level],
+                                 od->m2l_plan_dev[N], od->hel_A[N], od->hel_A2[level],
+                                 od->m2l_CS[N], od->m2l_plan[N]));
+    wbfmm_hip_free(cls);
+  } else {
+    HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
+                            (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
+                            od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
+                            (double *)op->SR[level], coax_entries(op->L[level])));
+  }
   if (level == t->depth) return 0;
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if (!op->SS[level + 1]) WBFMM_FATAL("S|S operators not initialised at level %u", level + 1);

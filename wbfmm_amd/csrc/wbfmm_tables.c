@@ -706,9 +706,6 @@ int wbfmm_hip_H_tables(int L, int nth, const double *theta, double **dev,
 /* index of (E|F)_{l n}^m, l >= n >= m >= 0 (src/wbfmm-private.h:149-157) */
 #define CXI(l, m, n) ((size_t)(l) * ((l) + 1) * ((l) + 2) / 6 + (size_t)(n) * ((n) + 1) / 2 + (m))
 
-/* index of (E|F)_{l n}^m, l >= n >= m >= 0 (src/wbfmm-private.h:149-157) */
-#define CXI(l, m, n) ((size_t)(l) * ((l) + 1) * ((l) + 2) / 6 + (size_t)(n) * ((n) + 1) / 2 + (m))
-
 /* The recursion loses about log10(amplification) digits to cancellation,
    which grows with |kr|: in x87 extended precision (64-bit mantissa) the
    tables stay within 2e-13 of the quad-precision ones for |kr| <= 20 at
@@ -863,6 +860,132 @@ static int shift_coax(const wbfmm_m2l_plan_t *P, int kind, double t, double a, d
           T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, nu, m, part)] = v;
         }
   put_tiles(A2, T, P);
+  free(T);
+  return 0;
+}
+
+/* ------------------------------------------------------------------
+ * Helmholtz M2L on matrix cores, with the Laplace plan of the same order.
+ *
+ * The reference translates a complex expansion x by rotate (H(theta) with
+ * the phases e^{i m chi} in and i^nu out), coaxial S|R, rotate back (phases
+ * (-i)^m in and e^{-i nu chi} out) (src/pass.c:56-105).  The i^nu of the
+ * first rotation and the (-i)^m of the second meet across the coaxial
+ * stage, which keeps the order m, and cancel.  In the basis
+ *   P_m = x'_m + x'_{-m},  Q_m = x'_m - x'_{-m}   (m >= 0, P_0 = 2 x'_0)
+ * of the phase-rotated x' the real H(theta) of degree n splits, by its
+ * symmetry h(-nu,-m) = h(nu,m), into a cos block
+ *   A_{nu m} = h(nu,m) + h(-nu,m)   (m > 0),   A_{nu 0} = h(nu,0)
+ * on P (nu, m = 0..n) and a sin block B_{nu m} = h(nu,m) - h(-nu,m) on Q
+ * (nu, m = 1..n): the blocks of the Laplace plan.  The coaxial table of
+ * order m acts alike on P and Q.  Both rotations use the same tiles; the
+ * output is y_nu = (P + Q)/2, y_{-nu} = (P - Q)/2, times e^{-i nu chi}.
+ * ------------------------------------------------------------------ */
+
+/* the distinct polar angles of the list-4 offsets in the plan's order */
+static int plan_thetas(double *ths)
+{
+  int dx, dy, dz, n = 0;
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        double th, ph, ch;
+        int k;
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        dir_angles(-dx, -dy, -dz, &th, &ph, &ch);
+        for (k = 0; k < n; k++)
+          if (fabs(ths[k] - th) < 1e-12) break;
+        if (k == n) ths[n++] = th;
+      }
+  return n;
+}
+
+typedef struct {
+  const wbfmm_m2l_plan_t *P;
+  const packing *p13;
+  const double *ths;
+  double *A;
+  int rc;
+} hel_rot_job;
+
+static void hel_rot_one(int k, void *c)
+{
+  hel_rot_job *j = (hel_rot_job *)c;
+  const wbfmm_m2l_plan_t *P = j->P;
+  int N = P->N, S = 16 * P->NB, n, nu, m;
+  double *H = malloc(sizeof(double) * rot_size(N));
+  double *T = calloc((size_t)S * S, sizeof(double));
+  if (!H || !T) {
+    j->rc = WBFMM_HIP_ERR_NOMEM;
+    free(H);
+    free(T);
+    return;
+  }
+  H_rotation(H, N, j->ths[k]);
+  for (n = 0; n <= N; n++) {
+    for (nu = 0; nu <= n; nu++)
+      for (m = 0; m <= n; m++)
+        T[(size_t)slot13(j->p13, N, n, nu, 0) * S + slot13(j->p13, N, n, m, 0)] =
+            m == 0 ? H[RIDX(nu, 0, n)] : H[RIDX(nu, m, n)] + H[RIDX(-nu, m, n)];
+    for (nu = 1; nu <= n; nu++)
+      for (m = 1; m <= n; m++)
+        T[(size_t)slot13(j->p13, N, n, nu, 1) * S + slot13(j->p13, N, n, m, 1)] =
+            H[RIDX(nu, m, n)] - H[RIDX(-nu, m, n)];
+  }
+  put_tiles(j->A + (size_t)k * P->KT * 64, T, P);
+  free(H);
+  free(T);
+}
+
+/* rotation tiles of the Helmholtz M2L for plan P: NTH x KT x 64, the same
+   for both rotations */
+int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *P, double *A)
+{
+  packing p13;
+  double ths[64];
+  hel_rot_job j;
+  if (plan_thetas(ths) != P->NTH) return WBFMM_HIP_ERR_ARG;
+  if (do_pack(P->N, 0, &p13) != P->NB) return WBFMM_HIP_ERR_ARG;
+  j.P = P;
+  j.p13 = &p13;
+  j.ths = ths;
+  j.A = A;
+  j.rc = 0;
+  par_for(P->NTH, hel_rot_one, &j);
+  return j.rc;
+}
+
+/* coaxial tiles of the Helmholtz M2L for plan P from nr host S|R tables E
+   of order L >= P->N (2 doubles per entry CXI(l, m, n), table i at
+   E + i * 2 CXI(L+1, 0, 0)): A2 + (2 i + c) KT 64 holds the real (c = 0)
+   or imaginary (c = 1) part, on the blocks of order m for P and Q, with
+   the reference's (E|F) indexing: input degree l, output n, entry
+   CXI(max(l,n), m, min(l,n)), negated when l < n and n + l is odd */
+int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *P, int L, int nr, const double *E,
+                                 double *A2)
+{
+  packing p2;
+  int N = P->N, S = 16 * P->NB, i, c, m, part, n, l;
+  size_t ne = CXI(L + 1, 0, 0), per = (size_t)P->KT * 64;
+  double *T;
+  if (L < N) return WBFMM_HIP_ERR_ORDER;
+  if (do_pack(N, 1, &p2) != P->NB) return WBFMM_HIP_ERR_ARG;
+  T = calloc((size_t)S * S, sizeof(double));
+  if (!T) return WBFMM_HIP_ERR_NOMEM;
+  for (i = 0; i < nr; i++)
+    for (c = 0; c < 2; c++) {
+      memset(T, 0, sizeof(double) * (size_t)S * S);
+      for (m = 0; m <= N; m++)
+        for (part = 0; part < (m ? 2 : 1); part++)
+          for (n = m; n <= N; n++)
+            for (l = m; l <= N; l++) {
+              size_t idx = l >= n ? CXI(l, m, n) : CXI(n, m, l);
+              double v = E[(size_t)i * 2 * ne + 2 * idx + c];
+              if (l < n && ((n + l) & 1)) v = -v;
+              T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, l, m, part)] = v;
+            }
+      put_tiles(A2 + (size_t)(2 * i + c) * per, T, P);
+    }
   free(T);
   return 0;
 }
