@@ -91,6 +91,25 @@ __device__ inline void bessel_j(int N, double x, double *j) {
     j[n + 1] = (x > n + 1) ? j[n] * (2 * n + 1) / x - j[n - 1] : bessel_j_series(n + 1, x);
 }
 
+// j_n(x) of one order by the rule of bessel_j (upward recursion from j_0,
+// j_1 for x > n, the power series otherwise), so that the lanes of a
+// wavefront compute the orders side by side
+__device__ inline double bessel_j_one(int n, double x) {
+  if (!(x > 0.0)) return n == 0 ? 1.0 : 0.0;
+  const double j0 = sin(x) / x;
+  if (n == 0) return j0;
+  const double j1 = j0 / x - cos(x) / x;
+  if (n == 1) return j1;
+  if (!(x > n)) return bessel_j_series(n, x);
+  double a = j0, b = j1;
+  for (int q = 1; q < n; q++) {
+    const double c = b * (2 * q + 1) / x - a;
+    a = b;
+    b = c;
+  }
+  return b;
+}
+
 // spherical coordinates of x - x0: r, cos/sin theta, cos/sin phi
 __device__ __forceinline__ void sph(double dx, double dy, double dz, double &r,
                                     double &C, double &S, double &cp, double &sp) {
@@ -195,10 +214,13 @@ __global__ __launch_bounds__(64) void k_hel_p2m(
     unsigned level, double x0, double y0, double z0, double D, int N,
     double k, double2 *__restrict__ mps) {
   extern __shared__ double2 smem[];
-  const int N1 = N + 1, lane = threadIdx.x;
+  const int N1 = N + 1, lane = threadIdx.x, WA = 2 * N + 3;
   double2 *R = smem;                      // (N+2)^2 (n <= N+1, m >= 0 used)
   double2 *acc = R + (N1 + 1) * (N1 + 1); // (N+1)^2
   double *jn = (double *)(acc + N1 * N1); // N+2
+  // with dipoles: a_n^m (n = 0..N) and b_n^m (n = 0..N+1), |m| <= N+1,
+  // at [n WA + m + N + 1]
+  double *ta = jn + N1 + 1, *tb = ta + (size_t)N1 * WA;
   const unsigned bi = blockIdx.x;
   if (bi >= nbox) return;
   const uint32_t b = boxes[bi];
@@ -206,11 +228,17 @@ __global__ __launch_bounds__(64) void k_hel_p2m(
   const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
                zc = z0 + (compact3(b >> 2) + 0.5) * w;
   for (int r = lane; r < N1 * N1; r += kWave) acc[r] = make_double2(0.0, 0.0);
+  if (dip) {
+    for (int i = lane; i < N1 * WA; i += kWave) ta[i] = rec_a(i / WA, i % WA - N1);
+    for (int i = lane; i < (N1 + 1) * WA; i += kWave) tb[i] = rec_b(i / WA, i % WA - N1);
+  }
+  auto A = [&](int n, int m) { return n < 0 ? 0.0 : ta[n * WA + m + N1]; };
+  auto B = [&](int n, int m) { return tb[n * WA + m + N1]; };
   __syncthreads();
   for (uint32_t s = start[b]; s < end[b]; s++) {
     double r, C, S, cp, sp;
     sph(xyz[4 * s] - xc, xyz[4 * s + 1] - yc, xyz[4 * s + 2] - zc, r, C, S, cp, sp);
-    if (lane == 0) bessel_j(N1, k * r, jn);
+    if (lane <= N1) jn[lane] = bessel_j_one(lane, k * r);
     __syncthreads();
     // column m of R for n = m..N+1
     if (lane <= N1) {
@@ -226,41 +254,38 @@ __global__ __launch_bounds__(64) void k_hel_p2m(
       }
     }
     __syncthreads();
-    if (lane <= N) {
-      const int m = lane;
+    // one lane per signed order mm = l (l <= N) or N - l
+    for (int l = lane; l < 2 * N + 1; l += kWave) {
+      const int mm = l <= N ? l : N - l, m = mm < 0 ? -mm : mm;
       const double2 qq = q ? q[s] : make_double2(0.0, 0.0);
       const double2 dd = dip ? dip[s] : make_double2(0.0, 0.0);
       const double nx = dip ? nrm[4 * s] : 0.0, ny = dip ? nrm[4 * s + 1] : 0.0,
                    nz = dip ? nrm[4 * s + 2] : 0.0;
-      auto Rg = [&](int n, int mm) -> double2 {
-        if (n < 0 || n > N1 || mm > n || -mm > n) return make_double2(0.0, 0.0);
-        const double2 v = R[hidx(n, mm < 0 ? -mm : mm)];
-        return mm >= 0 ? v : make_double2(v.x, -v.y);
+      auto Rg = [&](int n, int mq) -> double2 {
+        if (n < 0 || n > N1 || mq > n || -mq > n) return make_double2(0.0, 0.0);
+        const double2 v = R[hidx(n, mq < 0 ? -mq : mq)];
+        return mq >= 0 ? v : make_double2(v.x, -v.y);
       };
-      for (int sgn = 1; sgn >= -1; sgn -= 2) {
-        if (m == 0 && sgn < 0) break;
-        const int mm = sgn * m;
-        for (int n = m; n <= N; n++) {
-          double2 v = make_double2(0.0, 0.0);
-          if (q) v = cmulc(qq, Rg(n, mm));
-          if (dip) {
-            // G&D 2004 eq 3.7
-            const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), rec_a(n - 1, mm)),
-                                           cscale(Rg(n + 1, mm), -rec_a(n, mm))), k);
-            const double2 dp = cscale(cadd(cscale(Rg(n + 1, mm + 1), rec_b(n + 1, -mm - 1)),
-                                           cscale(Rg(n - 1, mm + 1), -rec_b(n, mm))), k);
-            const double2 dm = cscale(cadd(cscale(Rg(n + 1, mm - 1), rec_b(n + 1, mm - 1)),
-                                           cscale(Rg(n - 1, mm - 1), -rec_b(n, -mm))), k);
-            // gx = (dp + dm)/2, gy = -i (dp - dm)/2
-            const double2 gx = cscale(cadd(dp, dm), 0.5);
-            const double2 dd2 = cscale(cadd(dp, cscale(dm, -1.0)), 0.5);
-            const double2 gy = make_double2(dd2.y, -dd2.x);
-            const double2 g = cadd(cadd(cscale(gx, nx), cscale(gy, ny)), cscale(dz, nz));
-            const double2 t = cmulc(dd, g);
-            v = cadd(v, make_double2(-t.x, -t.y));
-          }
-          acc[hidx(n, mm)] = cadd(acc[hidx(n, mm)], v);
+      for (int n = m; n <= N; n++) {
+        double2 v = make_double2(0.0, 0.0);
+        if (q) v = cmulc(qq, Rg(n, mm));
+        if (dip) {
+          // G&D 2004 eq 3.7
+          const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), A(n - 1, mm)),
+                                         cscale(Rg(n + 1, mm), -A(n, mm))), k);
+          const double2 dp = cscale(cadd(cscale(Rg(n + 1, mm + 1), B(n + 1, -mm - 1)),
+                                         cscale(Rg(n - 1, mm + 1), -B(n, mm))), k);
+          const double2 dm = cscale(cadd(cscale(Rg(n + 1, mm - 1), B(n + 1, mm - 1)),
+                                         cscale(Rg(n - 1, mm - 1), -B(n, -mm))), k);
+          // gx = (dp + dm)/2, gy = -i (dp - dm)/2
+          const double2 gx = cscale(cadd(dp, dm), 0.5);
+          const double2 dd2 = cscale(cadd(dp, cscale(dm, -1.0)), 0.5);
+          const double2 gy = make_double2(dd2.y, -dd2.x);
+          const double2 g = cadd(cadd(cscale(gx, nx), cscale(gy, ny)), cscale(dz, nz));
+          const double2 t = cmulc(dd, g);
+          v = cadd(v, make_double2(-t.x, -t.y));
         }
+        acc[hidx(n, mm)] = cadd(acc[hidx(n, mm)], v);
       }
     }
     __syncthreads();
@@ -623,7 +648,7 @@ int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *star
   if (N < 0 || N > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
   if (nbox == 0) return 0;
   const size_t sh = sizeof(double2) * ((size_t)(N + 2) * (N + 2) + (size_t)(N + 1) * (N + 1)) +
-                    sizeof(double) * (N + 3);
+                    sizeof(double) * ((N + 2) + (dip ? (size_t)(2 * N + 3) * (2 * N + 3) : 0));
   WBFMM_LAUNCH("hel_p2m", k_hel_p2m, dim3(nbox), dim3(64), sh, boxes, nbox, start, end,
                xyz, (const double2 *)q, nrm, (const double2 *)dip, level, x0[0], x0[1],
                x0[2], D, N, k, (double2 *)mps);
