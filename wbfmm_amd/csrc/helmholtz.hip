@@ -206,77 +206,99 @@ __device__ void coax_wave(const double2 *in, int Ni, double2 *out, int No,
 // monopole:  M_n^m += q conj(R_n^m(y - c))
 // dipole:    M_n^m += -d conj(nrm . grad R_n^m(y - c))  (the source
 //            d nrm.grad_x G, src/local.c:155-166)
-__global__ __launch_bounds__(64) void k_hel_p2m(
+// A block of 256 threads per box takes the box's points in batches of pb
+// (<= 8, set by the LDS the batch needs):
+//   (1) j_n(k r_p) for every point p of the batch and order n, one thread
+//       each;
+//   (2) column m of R_n^m(p) (n = m..N+1, m >= 0) for every (p, m), one
+//       thread each, into LDS;
+//   (3) every thread adds the batch's contributions, point by point in
+//       source order, to the coefficients n(n+1)+m it owns (tid + 256 i),
+//       held in registers until the box is done.
+// The sums run in source order per coefficient: results do not depend on
+// the launch.
+constexpr int kP2MThreads = 256, kP2MBatch = 8;
+constexpr int kP2MCoef =
+    ((WBFMM_HIP_MAX_ORDER + 1) * (WBFMM_HIP_MAX_ORDER + 1) + kP2MThreads - 1) / kP2MThreads;
+
+__global__ __launch_bounds__(256) void k_hel_p2m(
     const uint32_t *__restrict__ boxes, unsigned nbox,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
     const double *__restrict__ xyz, const double2 *__restrict__ q,
     const double *__restrict__ nrm, const double2 *__restrict__ dip,
     unsigned level, double x0, double y0, double z0, double D, int N,
-    double k, double2 *__restrict__ mps) {
+    double k, int pb, double2 *__restrict__ mps) {
   extern __shared__ double2 smem[];
-  const int N1 = N + 1, lane = threadIdx.x, WA = 2 * N + 3;
-  double2 *R = smem;                      // (N+2)^2 (n <= N+1, m >= 0 used)
-  double2 *acc = R + (N1 + 1) * (N1 + 1); // (N+1)^2
-  double *jn = (double *)(acc + N1 * N1); // N+2
-  // with dipoles: a_n^m (n = 0..N) and b_n^m (n = 0..N+1), |m| <= N+1,
-  // at [n WA + m + N + 1]
-  double *ta = jn + N1 + 1, *tb = ta + (size_t)N1 * WA;
+  const int N1 = N + 1, tid = threadIdx.x, NT = (N1 + 1) * (N1 + 2) / 2;
+  double2 *R = smem;                        // pb x NT: R_n^m, n <= N+1, m >= 0
+  double *jn = (double *)(R + (size_t)pb * NT);  // pb x (N+2)
   const unsigned bi = blockIdx.x;
   if (bi >= nbox) return;
   const uint32_t b = boxes[bi];
   const double w = D / (double)(1u << level);
   const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
                zc = z0 + (compact3(b >> 2) + 0.5) * w;
-  for (int r = lane; r < N1 * N1; r += kWave) acc[r] = make_double2(0.0, 0.0);
-  if (dip) {
-    for (int i = lane; i < N1 * WA; i += kWave) ta[i] = rec_a(i / WA, i % WA - N1);
-    for (int i = lane; i < (N1 + 1) * WA; i += kWave) tb[i] = rec_b(i / WA, i % WA - N1);
-  }
-  auto A = [&](int n, int m) { return n < 0 ? 0.0 : ta[n * WA + m + N1]; };
-  auto B = [&](int n, int m) { return tb[n * WA + m + N1]; };
-  __syncthreads();
-  for (uint32_t s = start[b]; s < end[b]; s++) {
-    double r, C, S, cp, sp;
-    sph(xyz[4 * s] - xc, xyz[4 * s + 1] - yc, xyz[4 * s + 2] - zc, r, C, S, cp, sp);
-    if (lane <= N1) jn[lane] = bessel_j_one(lane, k * r);
+  const uint32_t s0 = start[b], s1 = end[b];
+  double2 acc[kP2MCoef];
+#pragma unroll
+  for (int i = 0; i < kP2MCoef; i++) acc[i] = make_double2(0.0, 0.0);
+
+  for (uint32_t sb = s0; sb < s1; sb += (uint32_t)pb) {
+    const int np = (int)min((uint32_t)pb, s1 - sb);
+    for (int it = tid; it < np * (N1 + 1); it += kP2MThreads) {
+      const int p = it / (N1 + 1), n = it - p * (N1 + 1);
+      const uint32_t s = sb + p;
+      double r, C, S, cp, sp;
+      sph(xyz[4 * s] - xc, xyz[4 * s + 1] - yc, xyz[4 * s + 2] - zc, r, C, S, cp, sp);
+      jn[p * (N1 + 1) + n] = bessel_j_one(n, k * r);
+    }
     __syncthreads();
-    // column m of R for n = m..N+1
-    if (lane <= N1) {
-      const int m = lane;
+    for (int it = tid; it < np * (N1 + 1); it += kP2MThreads) {
+      const int p = it / (N1 + 1), m = it - p * (N1 + 1);
+      const uint32_t s = sb + p;
+      double r, C, S, cp, sp;
+      sph(xyz[4 * s] - xc, xyz[4 * s + 1] - yc, xyz[4 * s + 2] - zc, r, C, S, cp, sp);
       double sn, cs;
       sincos((double)m * atan2(sp, cp), &sn, &cs);
-      double pmm = pbar_mm(m, S), pprev = 0.0, p = pmm;
+      const double *j = jn + p * (N1 + 1);
+      double2 *Rp = R + (size_t)p * NT;
+      double pv = pbar_mm(m, S), pprev = 0.0;
       for (int n = m; n <= N1; n++) {
-        R[hidx(n, m)] = make_double2(jn[n] * p * cs, jn[n] * p * sn);
-        const double pn = (n + 1 <= N1) ? pbar_up(n, m, C, p, pprev) : 0.0;
-        pprev = p;
-        p = pn;
+        Rp[n * (n + 1) / 2 + m] = make_double2(j[n] * pv * cs, j[n] * pv * sn);
+        const double pn = (n + 1 <= N1) ? pbar_up(n, m, C, pv, pprev) : 0.0;
+        pprev = pv;
+        pv = pn;
       }
     }
     __syncthreads();
-    // one lane per signed order mm = l (l <= N) or N - l
-    for (int l = lane; l < 2 * N + 1; l += kWave) {
-      const int mm = l <= N ? l : N - l, m = mm < 0 ? -mm : mm;
-      const double2 qq = q ? q[s] : make_double2(0.0, 0.0);
-      const double2 dd = dip ? dip[s] : make_double2(0.0, 0.0);
-      const double nx = dip ? nrm[4 * s] : 0.0, ny = dip ? nrm[4 * s + 1] : 0.0,
-                   nz = dip ? nrm[4 * s + 2] : 0.0;
-      auto Rg = [&](int n, int mq) -> double2 {
-        if (n < 0 || n > N1 || mq > n || -mq > n) return make_double2(0.0, 0.0);
-        const double2 v = R[hidx(n, mq < 0 ? -mq : mq)];
-        return mq >= 0 ? v : make_double2(v.x, -v.y);
-      };
-      for (int n = m; n <= N; n++) {
+#pragma unroll
+    for (int i = 0; i < kP2MCoef; i++) {
+      const int c = tid + kP2MThreads * i;
+      if (c >= N1 * N1) break;
+      const int n = isqrt_deg(c), mm = c - n * (n + 1);
+      // G&D 2004 eq 3.7 coefficients of this (n, mm)
+      const double a0 = rec_a(n - 1, mm), a1 = -rec_a(n, mm);
+      const double bp0 = rec_b(n + 1, -mm - 1), bp1 = -rec_b(n, mm);
+      const double bm0 = rec_b(n + 1, mm - 1), bm1 = -rec_b(n, -mm);
+      double2 ac = acc[i];
+      for (int p = 0; p < np; p++) {
+        const uint32_t s = sb + p;
+        const double2 *Rp = R + (size_t)p * NT;
+        auto Rg = [&](int nq, int mq) -> double2 {
+          if (nq < 0 || nq > N1 || mq > nq || -mq > nq) return make_double2(0.0, 0.0);
+          const double2 v = Rp[nq * (nq + 1) / 2 + (mq < 0 ? -mq : mq)];
+          return mq >= 0 ? v : make_double2(v.x, -v.y);
+        };
         double2 v = make_double2(0.0, 0.0);
-        if (q) v = cmulc(qq, Rg(n, mm));
+        if (q) v = cmulc(q[s], Rg(n, mm));
         if (dip) {
-          // G&D 2004 eq 3.7
-          const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), A(n - 1, mm)),
-                                         cscale(Rg(n + 1, mm), -A(n, mm))), k);
-          const double2 dp = cscale(cadd(cscale(Rg(n + 1, mm + 1), B(n + 1, -mm - 1)),
-                                         cscale(Rg(n - 1, mm + 1), -B(n, mm))), k);
-          const double2 dm = cscale(cadd(cscale(Rg(n + 1, mm - 1), B(n + 1, mm - 1)),
-                                         cscale(Rg(n - 1, mm - 1), -B(n, -mm))), k);
+          const double2 dd = dip[s];
+          const double nx = nrm[4 * s], ny = nrm[4 * s + 1], nz = nrm[4 * s + 2];
+          const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), a0), cscale(Rg(n + 1, mm), a1)), k);
+          const double2 dp =
+              cscale(cadd(cscale(Rg(n + 1, mm + 1), bp0), cscale(Rg(n - 1, mm + 1), bp1)), k);
+          const double2 dm =
+              cscale(cadd(cscale(Rg(n + 1, mm - 1), bm0), cscale(Rg(n - 1, mm - 1), bm1)), k);
           // gx = (dp + dm)/2, gy = -i (dp - dm)/2
           const double2 gx = cscale(cadd(dp, dm), 0.5);
           const double2 dd2 = cscale(cadd(dp, cscale(dm, -1.0)), 0.5);
@@ -285,13 +307,18 @@ __global__ __launch_bounds__(64) void k_hel_p2m(
           const double2 t = cmulc(dd, g);
           v = cadd(v, make_double2(-t.x, -t.y));
         }
-        acc[hidx(n, mm)] = cadd(acc[hidx(n, mm)], v);
+        ac = cadd(ac, v);
       }
+      acc[i] = ac;
     }
-    __syncthreads();
+    __syncthreads();  // R and jn are rewritten by the next batch
   }
   double2 *out = mps + (size_t)b * N1 * N1;
-  for (int r = lane; r < N1 * N1; r += kWave) out[r] = cadd(out[r], acc[r]);
+#pragma unroll
+  for (int i = 0; i < kP2MCoef; i++) {
+    const int c = tid + kP2MThreads * i;
+    if (c < N1 * N1) out[c] = cadd(out[c], acc[i]);
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -647,11 +674,13 @@ int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *star
                       const double *x0, double D, int N, double k, double *mps) {
   if (N < 0 || N > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
   if (nbox == 0) return 0;
-  const size_t sh = sizeof(double2) * ((size_t)(N + 2) * (N + 2) + (size_t)(N + 1) * (N + 1)) +
-                    sizeof(double) * ((N + 2) + (dip ? (size_t)(2 * N + 3) * (2 * N + 3) : 0));
-  WBFMM_LAUNCH("hel_p2m", k_hel_p2m, dim3(nbox), dim3(64), sh, boxes, nbox, start, end,
-               xyz, (const double2 *)q, nrm, (const double2 *)dip, level, x0[0], x0[1],
-               x0[2], D, N, k, (double2 *)mps);
+  // points per batch: as many as 60 KiB of LDS hold, at most 8
+  const size_t per = sizeof(double2) * (size_t)(N + 2) * (N + 3) / 2 + sizeof(double) * (N + 2);
+  int pb = (int)((60 * 1024) / per);
+  pb = pb < 1 ? 1 : (pb > kP2MBatch ? kP2MBatch : pb);
+  WBFMM_LAUNCH("hel_p2m", k_hel_p2m, dim3(nbox), dim3(kP2MThreads), per * pb, boxes, nbox,
+               start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level, x0[0],
+               x0[1], x0[2], D, N, k, pb, (double2 *)mps);
   return 0;
 }
 
