@@ -263,6 +263,10 @@ int wbfmm_hip_lap_field_f32(const double *xt, const uint32_t *tid, unsigned n,
    rotations) and coaxial tiles from nr host S|R tables E of order L >= N
    (nr x 2 x KT x 64: real, imaginary part) */
 int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *plan, double *A);
+/* the same tiles at nth given polar angles (nth x KT x 64); the M2M / L2L
+   use the child directions' acos(1/sqrt 3) (index 0) and acos(-1/sqrt 3) */
+int wbfmm_hip_hel_rot_tiles(const wbfmm_m2l_plan_t *plan, int nth, const double *theta,
+                            double *A);
 int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *plan, int L, int nr,
                                  const double *E, double *A2);
 /* out[i] = (float)in[i] on the device */
@@ -379,6 +383,21 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const double *mps, double *mpr, const void *plan_dev,
                            const double *A, const double *A2, const double *CS,
                            const wbfmm_m2l_plan_t *plan_host);
+/* M2M (mode 0: the occupied children of each listed parent, by
+   cstart/cend of the child level, into the parent, overwriting it;
+   src/pass.c:581-632) and L2L (mode 1: each listed parent into its
+   children flagged in cflag, or all if cflag is NULL, accumulating;
+   src/pass.c:550-576) on matrix cores for Nc = Np = N in 2..24: the
+   Laplace plan of order N, the rotation tiles A of the child polar angles
+   (wbfmm_hip_hel_rot_tiles), the real part A2 of the coaxial tiles of the
+   R|R table (wbfmm_hip_hel_m2l_coax_tiles with one table) and the
+   child-direction phases CS (CSup for M2M, CSdn for L2L of
+   wbfmm_hip_lap_shift_tables), all on the device */
+int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
+                             const uint32_t *cstart, const uint32_t *cend,
+                             const uint32_t *cflag, int N, const double *in, double *out,
+                             const void *plan_dev, const double *A, const double *A2,
+                             const double *CS, const wbfmm_m2l_plan_t *plan_host);
 /* M2L for the listed target boxes, accumulating into mpr; Hth: H tables
    of the offset polar angles; th_idx/chi/rix: per-slot geometry (device);
    SR: one S|R table of ne complex entries per offset length

@@ -268,27 +268,25 @@ def test_width_scaled_tables(wb, oracle, N):
         assert np.abs(Lu / degree_scale(N, w, 1) - Lw).max() < 1e-12 * np.abs(Lw).max(), off
 
 
-def emulate_hel(P, A, A2, CS, M, o, N):
-    """Helmholtz M2L as the kernel applies the tiles: phase e^{i m chi} in,
-    P/Q basis, the rotation tiles, the complex coaxial tiles (real and
-    imaginary part), the rotation tiles again, back to x_{+-nu} and the
-    phase e^{-i nu chi} out.  M, the result: complex, index n(n+1)+m."""
+def emulate_hel_core(P, T1, Ar, Ai, chi, M, N):
+    """one Helmholtz translation as the kernels apply the tiles: phase
+    e^{i m chi} in, P/Q basis, the rotation tiles T1, the coaxial tiles
+    (real part Ar, imaginary part Ai or None), T1 again, back to x_{+-nu}
+    and the phase e^{-i nu chi} out.  M and the result: complex, index
+    n(n+1)+m."""
     NU, NB = P.NU, P.NB
     S = P.S + 1
-    th, ri = P.th[o], P.rix[o]
     x = np.zeros(S, complex)
     for n in range(N + 1):
         for m in range(n + 1):
             u = n * (n + 1) // 2 + m
-            e = np.exp(1j * m * P.chi[o])
+            e = np.exp(1j * m * chi)
             xp, xm = e * M[n * (n + 1) + m], np.conj(e) * M[n * (n + 1) - m]
             if m == 0:
                 x[P.s1[u] & 0xffff] = 2 * xp
             else:
                 x[P.s1[u] & 0xffff] = xp + xm
                 x[(P.s1[u] >> 16) & 0xffff] = xp - xm
-        assert np.allclose([CS[o, m, 0] for m in range(N + 1)],
-                           np.cos(np.arange(N + 1) * P.chi[o]))
 
     def tiles(T, v):
         y = np.zeros(S, complex)
@@ -304,9 +302,12 @@ def emulate_hel(P, A, A2, CS, M, o, N):
             z[mp[r] >> 6] = y[r]
         return z
 
-    y = move(tiles(A[th], x), P.map1)
-    z = move(tiles(A2[ri, 0], y) + 1j * tiles(A2[ri, 1], y), P.map2)
-    a = move(tiles(A[th], z), P.map3)
+    y = move(tiles(T1, x), P.map1)
+    z = tiles(Ar, y)
+    if Ai is not None:
+        z = z + 1j * tiles(Ai, y)
+    z = move(z, P.map2)
+    a = move(tiles(T1, z), P.map3)
     L = np.zeros((N + 1) ** 2, complex)
     for n in range(N + 1):
         for nu in range(n + 1):
@@ -315,9 +316,15 @@ def emulate_hel(P, A, A2, CS, M, o, N):
             if nu == 0:
                 L[n * n + n] = 0.5 * pp
             else:
-                L[n * n + n + nu] = np.exp(-1j * nu * P.chi[o]) * 0.5 * (pp + qq)
-                L[n * n + n - nu] = np.exp(1j * nu * P.chi[o]) * 0.5 * (pp - qq)
+                L[n * n + n + nu] = np.exp(-1j * nu * chi) * 0.5 * (pp + qq)
+                L[n * n + n - nu] = np.exp(1j * nu * chi) * 0.5 * (pp - qq)
     return L
+
+
+def emulate_hel(P, A, A2, CS, M, o, N):
+    """the Helmholtz M2L of list-4 slot o"""
+    assert np.allclose(CS[o, :, 0], np.cos(np.arange(N + 1) * P.chi[o]))
+    return emulate_hel_core(P, A[P.th[o]], A2[P.rix[o], 0], A2[P.rix[o], 1], P.chi[o], M, N)
 
 
 @pytest.mark.parametrize("N,L", [(2, 2), (5, 7), (9, 9), (13, 13), (16, 18), (20, 20)])
@@ -348,3 +355,44 @@ def test_hel_plan_matches_oracle_m2l(wb, oracle, N, L):
         Lref = Lref[0::2] + 1j * Lref[1::2]
         Lc = emulate_hel(P, A, A2, CS, M[0::2] + 1j * M[1::2], o, N)
         assert np.abs(Lc - Lref).max() < 1e-12 * np.abs(Lref).max(), off
+
+
+def child_angles(c, sgn):
+    ex, ey, ez = (sgn * (0.5 - ((c >> i) & 1)) for i in range(3))
+    return math.acos(ez / math.sqrt(ex * ex + ey * ey + ez * ez)), -0.5 * math.pi, math.atan2(ey, ex)
+
+
+@pytest.mark.parametrize("N", [2, 7, 12, 16, 23])
+def test_hel_shift_tiles_match_oracle(wb, oracle, N):
+    # Helmholtz M2M (child -> parent) and L2L (parent -> child): the rotation
+    # tiles at the two child polar angles, the real R|R coaxial tiles at
+    # sqrt(3)/2 of the child width and the child-direction phases of the
+    # Laplace shift tables, against the oracle's rotate / coaxial R|R /
+    # rotate back (src/pass.c:550-632)
+    import ctypes
+    k, wc = 3.7, 0.23
+    P, _, _, CSu, CSd, _, _ = shift_tables(wb, N, 1.0)
+    ths = np.array([math.acos(1 / math.sqrt(3)), math.acos(-1 / math.sqrt(3))])
+    A = np.zeros((2, P.KT, 64))
+    assert wb.lib().wbfmm_hip_hel_rot_tiles(ctypes.byref(P), 2, wb.ptr(ths), wb.ptr(A)) == 0
+    ne = (N + 1) * (N + 2) * (N + 3) // 6
+    E = oracle.hel_coaxial_table(0, N, k * wc * 0.5 * math.sqrt(3.0))
+    _, _, A2 = wb.hel_m2l_tables(N, N, E[None, :2 * ne])
+    rng = np.random.default_rng(200 + N)
+    for mode, CS, sgn in ((0, CSu, 1), (1, CSd, -1)):
+        M = np.zeros(2 * (N + 1) ** 2)
+        for _ in range(3):
+            s_ = rng.uniform(-0.5, 0.5, 3) * wc
+            oracle.hel_p2m(k, N, np.zeros(3), s_, complex(*rng.normal(size=2)), M)
+        for c in range(8):
+            th, ph, ch = child_angles(c, sgn)
+            assert np.allclose(CS[c, :, 0], np.cos(np.arange(N + 1) * ch))
+            assert np.allclose(CS[c, :, 1], np.sin(np.arange(N + 1) * ch))
+            ti = (c >> 2) & 1 if mode == 0 else 1 - ((c >> 2) & 1)
+            assert abs(ths[ti] - th) < 1e-14
+            H = oracle.H_rotation(N, th)
+            Lref = oracle.hel_rotate(oracle.hel_coaxial(oracle.hel_rotate(M, N, H, ph, ch), N, N, E,
+                                                        N, 0), N, H, ch, ph)
+            Lref = Lref[0::2] + 1j * Lref[1::2]
+            Lc = emulate_hel_core(P, A[ti], A2[0, 0], None, ch, M[0::2] + 1j * M[1::2], N)
+            assert np.abs(Lc - Lref).max() < 1e-12 * np.abs(Lref).max(), (mode, c)

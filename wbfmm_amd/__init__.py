@@ -152,6 +152,7 @@ _SIGS = {
     "wbfmm_hip_lap_m2l_tables": (_i, [_i, _vp, _vp, _vp, _vp]),
     "wbfmm_hip_lap_m2l_coax": (_i, [_vp, _d, _vp]),
     "wbfmm_hip_hel_m2l_rot_tiles": (_i, [_vp, _vp]),
+    "wbfmm_hip_hel_rot_tiles": (_i, [_vp, _i, _vp, _vp]),
     "wbfmm_hip_hel_m2l_coax_tiles": (_i, [_vp, _i, _i, _vp, _vp]),
     "wbfmm_tree_coefficient_init": (_i, [_vp, _u, _u, _u]),
     "wbfmm_tree_leaf_expansions": (_i, [_vp, _d, _vp, _i, _vp, _i, _i, _vp]),

@@ -392,6 +392,257 @@ static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nch
   return 0;
 }
 
+
+// ---------------------------------------------------------------------
+// M2M (MODE 0, src/pass.c:581-632, src/shift.c:244-401) and L2L (MODE 1,
+// src/pass.c:550-576, src/shift.c:403-585) in the same form: the rotate /
+// coaxial R|R / rotate-back of each child direction c, whose polar angle
+// is acos(1/sqrt 3) (tile 0) or acos(-1/sqrt 3) (tile 1) and whose phases
+// are CS[c] (cos, sin of m chi_c).  The R|R table is real: stage 2 takes
+// one product per tile step.  A block of W wavefronts takes 8 parents
+// (columns 2t + s as in the M2L).  M2M adds the 8 children of each parent
+// in registers, in child order, and writes the parent; L2L translates each
+// parent's local expansion to each flagged child and adds it there.
+// ---------------------------------------------------------------------
+template <int N, int W, int WI, int MODE>
+__device__ __forceinline__ void hel_shift_loop(
+    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    const double2 (*cs)[N + 1], const unsigned *cmask, unsigned anyc, int lane, int col, int grp,
+    bool pvalid, uint64_t p, const double2 *__restrict__ in, double2 *__restrict__ out,
+    const double *__restrict__ A, const double *__restrict__ A2, const double *__restrict__ CS,
+    const double2 *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
+  constexpr int NBW = WS.nb;
+  const int ubase = grp + 4 * WI, tg = col >> 1;
+
+  double pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const double2 *src, const int4 *tu, int ub, int sp) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 q = src[inf.x + (sp ? -inf.y : inf.y)];
+      pre_re[i] = q.x;
+      pre_im[i] = q.y;
+    }
+  };
+  auto deposit = [&](const int4 *tu, int ub, int cl, int c) {
+    const int sp = cl & 1;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 e = cs[c][inf.y];
+      const double es = sp ? -e.y : e.y;
+      const double xr = e.x * pre_re[i] - es * pre_im[i];
+      const double xi = es * pre_re[i] + e.x * pre_im[i];
+      const double yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
+      if (ub + 4 * W * i >= NU) continue;
+      R[inf.z * 16 + cl] = sp ? yi + xi : xr + yr;
+      R[inf.w * 16 + cl] = sp ? yi - xi : xr - yr;
+    }
+  };
+  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + W * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  double acc_re[NAW], acc_im[NAW];
+#pragma unroll
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = 0.0;
+  if (MODE == 1) load_sources(pvalid ? in + p * (uint64_t)C : zero, uinf, ubase, col & 1);
+
+  for (int c = 0; c < 8; c++) {
+    if (!((anyc >> c) & 1u)) continue;  // the same in every lane of the block
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI, sp = cl & 1;
+    const bool v = (cmask[tg] >> c) & 1u;
+    const int th = MODE == 0 ? (c >> 2) & 1 : 1 - ((c >> 2) & 1);
+    if (MODE == 0) load_sources(v ? in + (8 * p + c) * (uint64_t)C : zero, tu, ub, sp);
+    __syncthreads();  // every read of R by the last direction done
+    deposit(tu, ub, cl, c);
+    __syncthreads();
+
+    d4 y[NBW > 0 ? NBW : 1];
+    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t1, y, gp, cl);
+    __syncthreads();
+    stage_w<double, N, W, WI>(A2, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t2, y, gp, cl);
+    __syncthreads();
+    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t3, y, gp, cl);
+    __syncthreads();
+
+    const int c0 = cl & ~1;
+    double2 *dst = out + (8 * p + c) * (uint64_t)C;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const int nu = inf.y;
+      const double2 e = cs[c][nu];
+      const double es = sp ? -e.y : e.y;
+      const double pr = R[u * 16 + c0], pi = R[u * 16 + c0 + 1];
+      const double qr = nu ? R[(NU + u) * 16 + c0] : 0.0;
+      const double qi = nu ? R[(NU + u) * 16 + c0 + 1] : 0.0;
+      const double yr = 0.5 * (sp ? pr - qr : pr + qr);
+      const double yi = 0.5 * (sp ? pi - qi : pi + qi);
+      const double orr = e.x * yr + es * yi, oi = e.x * yi - es * yr;
+      if (MODE == 0) {
+        acc_re[i] += orr;
+        acc_im[i] += oi;
+      } else if (v && ub + 4 * W * i < NU && !(sp && nu == 0)) {
+        const int r = inf.x + (sp ? -nu : nu);
+        const double2 o = dst[r];
+        dst[r] = make_double2(o.x + orr, o.y + oi);
+      }
+    }
+  }
+  if (MODE == 0 && pvalid) {
+    double2 *dst = out + p * (uint64_t)C;
+    const int sp = col & 1;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = ubase + 4 * W * i;
+      if (u >= NU) continue;
+      const int4 inf = uinf[u];
+      if (sp && inf.y == 0) continue;
+      dst[inf.x + (sp ? -inf.y : inf.y)] = make_double2(acc_re[i], acc_im[i]);
+    }
+  }
+}
+
+template <int N, int W, int MODE>
+__global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
+    const uint32_t *__restrict__ parents, unsigned npar, const uint32_t *__restrict__ cstart,
+    const uint32_t *__restrict__ cend, const uint32_t *__restrict__ cflag,
+    const double2 *__restrict__ in, double2 *__restrict__ out,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
+    const double *__restrict__ A2, const double *__restrict__ CS,
+    const double2 *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  __shared__ double R[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[8][N + 1];
+  __shared__ unsigned cmask[8];
+  __shared__ unsigned anyc;
+
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < 8 * (N + 1); i += 64 * W)
+    cs[i / (N + 1)][i % (N + 1)] = make_double2(CS[2 * i], CS[2 * i + 1]);
+  // children to translate, per parent column: occupied children (M2M) or
+  // flagged ones (L2L)
+  const unsigned pi = blockIdx.x * 8 + (col >> 1);
+  const bool pvalid = pi < npar;
+  const uint64_t p = pvalid ? parents[pi] : 0;
+  if (tid < 8) {
+    const unsigned pj = blockIdx.x * 8 + tid;
+    unsigned m = 0;
+    if (pj < npar) {
+      const uint64_t pp = parents[pj];
+      for (int c = 0; c < 8; c++) {
+        const uint64_t cb = 8 * pp + c;
+        const bool v = MODE == 0 ? cend[cb] > cstart[cb] : (!cflag || cflag[cb] != 0);
+        if (v) m |= 1u << c;
+      }
+    }
+    cmask[tid] = m;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned a = 0;
+    for (int j = 0; j < 8; j++) a |= cmask[j];
+    anyc = a;
+  }
+  __syncthreads();
+  const unsigned any = __builtin_amdgcn_readfirstlane(anyc);
+
+#define HEL_SHIFT(WI)                                                                        \
+  hel_shift_loop<N, W, WI, MODE>(R, tmap1, tmap2, tmap3, uinf, cs, cmask, any, lane, col,   \
+                                 grp, pvalid, p, in, out, A, A2, CS, zero)
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: HEL_SHIFT(0); break;
+      case 1: HEL_SHIFT(1); break;
+      case 2: HEL_SHIFT(2); break;
+      default: HEL_SHIFT(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: HEL_SHIFT(0); break;
+      case 1: HEL_SHIFT(1); break;
+      case 2: HEL_SHIFT(2); break;
+      case 3: HEL_SHIFT(3); break;
+      case 4: HEL_SHIFT(4); break;
+      case 5: HEL_SHIFT(5); break;
+      case 6: HEL_SHIFT(6); break;
+      default: HEL_SHIFT(7); break;
+    }
+  }
+#undef HEL_SHIFT
+}
+
+template <int N>
+static int launch_hel_shift(int mode, const uint32_t *parents, unsigned npar,
+                            const uint32_t *cstart, const uint32_t *cend, const uint32_t *cflag,
+                            const double *in, double *out, const void *plan, const double *A,
+                            const double *A2, const double *CS, const double *zero,
+                            const wbfmm_m2l_plan_t *hplan) {
+  if (!plan_matches<N>(hplan)) {
+    fprintf(stderr, "wbfmm (HIP): shift plan for order %d does not match the kernel's bins\n", N);
+    return WBFMM_HIP_ERR_ARG;
+  }
+  constexpr int W = N >= 16 ? 8 : 4;
+  const dim3 grid((npar + 7) / 8), block(64 * W);
+  if (mode == 0)
+    WBFMM_LAUNCH("hel_m2m", (k_hel_shift_mfma<N, W, 0>), grid, block, 0, parents, npar, cstart,
+                 cend, cflag, (const double2 *)in, (double2 *)out,
+                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero);
+  else
+    WBFMM_LAUNCH("hel_l2l", (k_hel_shift_mfma<N, W, 1>), grid, block, 0, parents, npar, cstart,
+                 cend, cflag, (const double2 *)in, (double2 *)out,
+                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero);
+  return 0;
+}
+
 }  // namespace wbfmm
 
 using namespace wbfmm;
@@ -441,6 +692,34 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const un
   }
   dfree(occ);
   return rc;
+}
+
+int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
+                             const uint32_t *cstart, const uint32_t *cend,
+                             const uint32_t *cflag, int N, const double *in, double *out,
+                             const void *plan_dev, const double *A, const double *A2,
+                             const double *CS, const wbfmm_m2l_plan_t *plan_host) {
+  if (npar == 0) return 0;
+  if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
+  if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
+  if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
+  if (!plan_host || plan_host->N != N) return WBFMM_HIP_ERR_ARG;
+  {
+    const int rz = ensure_zero();
+    if (rz) return rz;
+  }
+  switch (N) {
+#define CASE(n)                                                                            \
+  case n:                                                                                  \
+    return launch_hel_shift<n>(mode, parents, npar, cstart, cend, cflag, in, out, plan_dev, \
+                               A, A2, CS, zero_expansion(), plan_host);
+    CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
+#undef CASE
+    default:
+      return WBFMM_HIP_ERR_ORDER;
+  }
 }
 
 }  // extern "C"

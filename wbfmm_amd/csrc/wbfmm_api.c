@@ -127,6 +127,14 @@ typedef struct {
   int sr_nr;
   double *hel_A2[WBFMM_TREE_MAX_DEPTH + 1];
   int hel_A2_N[WBFMM_TREE_MAX_DEPTH + 1];
+  /* Helmholtz M2M / L2L on matrix cores: rotation tiles of the two child
+     polar angles per order (shared by the process), the host R|R table of
+     each level and its real coaxial tiles with the order they were made
+     for */
+  double *hel_shA[WBFMM_HIP_MAX_ORDER + 1];
+  double *ss_host[WBFMM_TREE_MAX_DEPTH + 1];
+  double *hel_SS2[WBFMM_TREE_MAX_DEPTH + 1];
+  int hel_SS2_N[WBFMM_TREE_MAX_DEPTH + 1];
 } ops_dev;
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
@@ -778,8 +786,10 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     wbfmm_hip_free(od->op.SS[n]);
     wbfmm_hip_free(od->hel_A2[n]);
     free(od->sr_host[n]);
+    wbfmm_hip_free(od->hel_SS2[n]);
+    free(od->ss_host[n]);
   }
-  /* d_hel_* and hel_A point to the process-wide tables: not freed */
+  /* d_hel_*, hel_A and hel_shA point to the process-wide tables: not freed */
   free(od);
   return 0;
 }
@@ -1632,7 +1642,11 @@ gint wbfmm_shift_operators_coaxial_SS_init(wbfmm_shift_operators_t *w, gdouble D
   if (w->SS[level]) wbfmm_hip_free(w->SS[level]);
   HIPOK(wbfmm_hip_malloc(&w->SS[level], sizeof(double) * 2 * ne));
   HIPOK(wbfmm_hip_memcpy(w->SS[level], h, sizeof(double) * 2 * ne));
-  free(h);
+  /* the host copy stays for the matrix-core tiles, made on first use */
+  free(od->ss_host[level]);
+  od->ss_host[level] = h;
+  wbfmm_hip_free(od->hel_SS2[level]);
+  od->hel_SS2[level] = NULL;
   od->ss_L[level] = L;
   return 0;
 }
@@ -1651,6 +1665,10 @@ gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *w)
     od->hel_A2[n] = NULL;
     free(od->sr_host[n]);
     od->sr_host[n] = NULL;
+    wbfmm_hip_free(od->hel_SS2[n]);
+    od->hel_SS2[n] = NULL;
+    free(od->ss_host[n]);
+    od->ss_host[n] = NULL;
   }
   od->hel_k_set = 0;
   return 0;
@@ -1710,6 +1728,10 @@ static void ensure_hel_rotations(ops_dev *od)
   pthread_mutex_unlock(&g_hel_rot_mu);
 }
 
+static int hel_shift_mfma(ops_dev *od, guint Nc, guint Np);
+static void ensure_hel_shift_rot_tiles(ops_dev *od, int N);
+static void ensure_hel_shift_coax_tiles(ops_dev *od, guint level, int N);
+
 /* src/pass.c:581-632: M2M from level into level-1 */
 gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
                        gdouble *work)
@@ -1727,6 +1749,18 @@ gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level
   if (!op->SS[level]) WBFMM_FATAL("S|S operators not initialised at level %u", level);
   if ((Nc > Np ? Nc : Np) > od->ss_L[level] || (Nc > Np ? Nc : Np) > op->Lmax)
     WBFMM_FATAL("operators at level %u built to order %u", level, od->ss_L[level]);
+  if (hel_shift_mfma(od, Nc, Np)) {
+    int N = (int)Nc;
+    ensure_shift_tables(od, N);
+    ensure_hel_shift_rot_tiles(od, N);
+    ensure_hel_shift_coax_tiles(od, level, N);
+    HIPOK(wbfmm_hip_hel_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                   td->d_start[level], td->d_end[level], NULL, N,
+                                   (double *)t->mps[level], (double *)t->mps[level - 1],
+                                   od->m2l_plan_dev[N], od->hel_shA[N], od->hel_SS2[level],
+                                   od->sh_CSup[N], od->m2l_plan[N]));
+    return 0;
+  }
   ensure_hel_rotations(od);
   HIPOK(wbfmm_hip_hel_m2m(td->d_occ[level - 1], td->nocc[level - 1], td->d_start[level],
                           td->d_end[level], (int)Nc, (int)Np, (double *)t->mps[level],
@@ -1799,6 +1833,55 @@ static void ensure_hel_coax_tiles(ops_dev *od, guint level, int N)
   free(h);
 }
 
+/* rotation tiles of the Helmholtz matrix-core M2M / L2L of order N at the
+   child polar angles acos(1/sqrt 3), acos(-1/sqrt 3), built once per
+   process */
+static void ensure_hel_shift_rot_tiles(ops_dev *od, int N)
+{
+  if (od->hel_shA[N]) return;
+  pthread_mutex_lock(&g_tab_mu);
+  if (!g_tab.hel_shA[N]) {
+    const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+    size_t n = (size_t)2 * P->KT * 64;
+    double ths[2] = {acos(1.0 / sqrt(3.0)), acos(-1.0 / sqrt(3.0))}, *d;
+    double *h = malloc(sizeof(double) * n);
+    if (!h) WBFMM_FATAL("out of host memory");
+    HIPOK(wbfmm_hip_hel_rot_tiles(P, 2, ths, h));
+    HIPOK(wbfmm_hip_malloc((void **)&d, sizeof(double) * n));
+    HIPOK(wbfmm_hip_memcpy(d, h, sizeof(double) * n));
+    free(h);
+    g_tab.hel_shA[N] = d;
+  }
+  od->hel_shA[N] = g_tab.hel_shA[N];
+  pthread_mutex_unlock(&g_tab_mu);
+}
+
+/* real coaxial tiles of the Helmholtz matrix-core M2M / L2L between
+   `level' and its parents, order N, from the level's host R|R table */
+static void ensure_hel_shift_coax_tiles(ops_dev *od, guint level, int N)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  size_t n = (size_t)P->KT * 64;
+  double *h;
+  if (od->hel_SS2[level] && od->hel_SS2_N[level] == N) return;
+  if (!od->ss_host[level]) WBFMM_FATAL("S|S operators not initialised at level %u", level);
+  h = malloc(sizeof(double) * 2 * n);
+  if (!h) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_hel_m2l_coax_tiles(P, (int)od->ss_L[level], 1, od->ss_host[level], h));
+  wbfmm_hip_free(od->hel_SS2[level]);
+  HIPOK(wbfmm_hip_malloc((void **)&od->hel_SS2[level], sizeof(double) * n));
+  HIPOK(wbfmm_hip_memcpy(od->hel_SS2[level], h, sizeof(double) * n));  /* real part */
+  od->hel_SS2_N[level] = N;
+  free(h);
+}
+
+/* the matrix-core M2M / L2L serves equal orders in M2L_MFMA_MIN..M2L_MFMA_MAX */
+static int hel_shift_mfma(ops_dev *od, guint Nc, guint Np)
+{
+  (void)od;
+  return Nc == Np && (int)Nc >= M2L_MFMA_MIN && (int)Nc <= M2L_MFMA_MAX && hel_mfma_enabled();
+}
+
 static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level)
 {
   tree_dev *td = TD(t);
@@ -1851,10 +1934,22 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
     guint Nc = t->order_r[level + 1];
     if ((Nr > Nc ? Nr : Nc) > od->ss_L[level + 1] || (Nr > Nc ? Nr : Nc) > op->Lmax)
       WBFMM_FATAL("S|S operators at level %u built to order %u", level + 1, od->ss_L[level + 1]);
-    HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
-                            (int)Nr, (int)Nc, (double *)t->mpr[level],
-                            (double *)t->mpr[level + 1], od->d_hel_Hud, od->hel_nrot,
-                            (double *)op->SS[level + 1]));
+    if (hel_shift_mfma(od, Nc, Nr)) {
+      int N = (int)Nc;
+      ensure_shift_tables(od, N);
+      ensure_hel_shift_rot_tiles(od, N);
+      ensure_hel_shift_coax_tiles(od, level + 1, N);
+      HIPOK(wbfmm_hip_hel_shift_mfma(1, tl, nt, NULL, NULL,
+                                     td->have_targets ? td->d_flag[level + 1] : NULL, N,
+                                     (double *)t->mpr[level], (double *)t->mpr[level + 1],
+                                     od->m2l_plan_dev[N], od->hel_shA[N],
+                                     od->hel_SS2[level + 1], od->sh_CSdn[N], od->m2l_plan[N]));
+    } else {
+      HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
+                              (int)Nr, (int)Nc, (double *)t->mpr[level],
+                              (double *)t->mpr[level + 1], od->d_hel_Hud, od->hel_nrot,
+                              (double *)op->SS[level + 1]));
+    }
   }
   return 0;
 }

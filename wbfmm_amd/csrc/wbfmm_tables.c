@@ -937,22 +937,30 @@ static void hel_rot_one(int k, void *c)
   free(T);
 }
 
-/* rotation tiles of the Helmholtz M2L for plan P: NTH x KT x 64, the same
-   for both rotations */
-int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *P, double *A)
+/* rotation tiles of the Helmholtz translations for plan P at the nth
+   polar angles ths: nth x KT x 64, the same for both rotations */
+int wbfmm_hip_hel_rot_tiles(const wbfmm_m2l_plan_t *P, int nth, const double *ths, double *A)
 {
   packing p13;
-  double ths[64];
   hel_rot_job j;
-  if (plan_thetas(ths) != P->NTH) return WBFMM_HIP_ERR_ARG;
+  if (nth < 1) return WBFMM_HIP_ERR_ARG;
   if (do_pack(P->N, 0, &p13) != P->NB) return WBFMM_HIP_ERR_ARG;
   j.P = P;
   j.p13 = &p13;
   j.ths = ths;
   j.A = A;
   j.rc = 0;
-  par_for(P->NTH, hel_rot_one, &j);
+  par_for(nth, hel_rot_one, &j);
   return j.rc;
+}
+
+/* rotation tiles of the Helmholtz M2L for plan P: NTH x KT x 64 at the
+   plan's polar angles */
+int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *P, double *A)
+{
+  double ths[64];
+  if (plan_thetas(ths) != P->NTH) return WBFMM_HIP_ERR_ARG;
+  return wbfmm_hip_hel_rot_tiles(P, P->NTH, ths, A);
 }
 
 /* coaxial tiles of the Helmholtz M2L for plan P from nr host S|R tables E
