@@ -797,30 +797,78 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
 /* the shared per-order tables live in this set's fields */
 static ops_dev g_tab;
 
+/* one lock per order: a table build of one order holds up no other */
+static pthread_mutex_t g_ord_mu[WBFMM_HIP_MAX_ORDER + 1] = {
+    [0 ... WBFMM_HIP_MAX_ORDER] = PTHREAD_MUTEX_INITIALIZER};
+
+/* the M2L plan of order N and its phases (no tiles) */
 static void build_m2l_plan(ops_dev *od, int N)
 {
-  if (od->m2l_have[N]) return;
-  od->m2l_plan[N] = calloc(1, sizeof(wbfmm_m2l_plan_t));
-  HIPOK(wbfmm_hip_lap_m2l_plan(N, od->m2l_plan[N], &od->m2l_plan_dev[N],
-                               &od->m2l_A1[N], &od->m2l_A3[N], &od->m2l_CS[N]));
-  od->m2l_have[N] = 1;
+  wbfmm_m2l_plan_t *P;
+  if (od->m2l_plan[N]) return;
+  P = calloc(1, sizeof(wbfmm_m2l_plan_t));
+  if (!P) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_lap_m2l_plan(N, P, &od->m2l_plan_dev[N], NULL, NULL, &od->m2l_CS[N]));
+  od->m2l_plan[N] = P;
+}
+
+/* the Laplace M2L rotation tiles of order N */
+static void build_lap_m2l_tiles(ops_dev *od, int N)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  wbfmm_m2l_plan_t Q;
+  size_t n13;
+  double *h1, *h3;
+  if (od->m2l_A1[N]) return;
+  n13 = (size_t)P->NTH * P->KT * 64;
+  h1 = malloc(sizeof(double) * n13);
+  h3 = malloc(sizeof(double) * n13);
+  if (!h1 || !h3) WBFMM_FATAL("out of host memory");
+  HIPOK(wbfmm_hip_lap_m2l_tables(N, &Q, h1, h3, NULL));
+  HIPOK(wbfmm_hip_malloc((void **)&od->m2l_A3[N], sizeof(double) * n13));
+  HIPOK(wbfmm_hip_memcpy(od->m2l_A3[N], h3, sizeof(double) * n13));
+  HIPOK(wbfmm_hip_malloc((void **)&od->m2l_A1[N], sizeof(double) * n13));
+  HIPOK(wbfmm_hip_memcpy(od->m2l_A1[N], h1, sizeof(double) * n13));
+  free(h1);
+  free(h3);
 }
 
 static void build_shift_tables(ops_dev *od, int N);
 static void build_f32_tables(ops_dev *od, int N);
+static void copy_order_tables(ops_dev *od, int N);
+
+/* the plan of order N with its phases and the M2M / L2L tiles and phases
+   (what the Helmholtz matrix-core passes take from it) into od */
+static void ensure_plan(ops_dev *od, int N)
+{
+  if (od->m2l_plan[N] && od->sh_CSup[N]) return;
+  pthread_mutex_lock(&g_ord_mu[N]);
+  build_m2l_plan(&g_tab, N);
+  build_shift_tables(&g_tab, N);
+  copy_order_tables(od, N);
+  pthread_mutex_unlock(&g_ord_mu[N]);
+}
 
 /* every matrix-core table of order N (FP64 M2L plan and tiles, M2M / L2L
    tiles, FP32 copies) into od, from the shared set */
 static void ensure_order_tables(ops_dev *od, int N)
 {
   if (od->m2l_have[N]) return;
-  pthread_mutex_lock(&g_tab_mu);
+  pthread_mutex_lock(&g_ord_mu[N]);
   if (!g_tab.f_A1[N]) {
     build_m2l_plan(&g_tab, N);
+    build_lap_m2l_tiles(&g_tab, N);
     build_shift_tables(&g_tab, N);
     build_f32_tables(&g_tab, N);
     HIPOK(wbfmm_hip_sync()); /* conversions queued on this thread's stream */
   }
+  copy_order_tables(od, N);
+  od->m2l_have[N] = 1;
+  pthread_mutex_unlock(&g_ord_mu[N]);
+}
+
+static void copy_order_tables(ops_dev *od, int N)
+{
   od->m2l_plan[N] = g_tab.m2l_plan[N];
   od->m2l_plan_dev[N] = g_tab.m2l_plan_dev[N];
   od->m2l_A1[N] = g_tab.m2l_A1[N];
@@ -840,8 +888,6 @@ static void ensure_order_tables(ops_dev *od, int N)
   od->f_CSdn[N] = g_tab.f_CSdn[N];
   od->f_A2up[N] = g_tab.f_A2up[N];
   od->f_A2dn[N] = g_tab.f_A2dn[N];
-  od->m2l_have[N] = 1;
-  pthread_mutex_unlock(&g_tab_mu);
 }
 
 static void ensure_m2l_plan(ops_dev *od, int N) { ensure_order_tables(od, N); }
@@ -1751,7 +1797,7 @@ gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level
     WBFMM_FATAL("operators at level %u built to order %u", level, od->ss_L[level]);
   if (hel_shift_mfma(od, Nc, Np)) {
     int N = (int)Nc;
-    ensure_shift_tables(od, N);
+    ensure_plan(od, N);
     ensure_hel_shift_rot_tiles(od, N);
     ensure_hel_shift_coax_tiles(od, level, N);
     HIPOK(wbfmm_hip_hel_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
@@ -1787,7 +1833,7 @@ static int hel_mfma_enabled(void)
 static void ensure_hel_rot_tiles(ops_dev *od, int N)
 {
   if (od->hel_A[N]) return;
-  pthread_mutex_lock(&g_tab_mu);
+  pthread_mutex_lock(&g_ord_mu[N]);
   if (!g_tab.hel_A[N]) {
     const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
     size_t n = (size_t)P->NTH * P->KT * 64;
@@ -1800,7 +1846,7 @@ static void ensure_hel_rot_tiles(ops_dev *od, int N)
     g_tab.hel_A[N] = d;
   }
   od->hel_A[N] = g_tab.hel_A[N];
-  pthread_mutex_unlock(&g_tab_mu);
+  pthread_mutex_unlock(&g_ord_mu[N]);
 }
 
 /* coaxial tiles of the Helmholtz matrix-core M2L at `level' for order N,
@@ -1839,7 +1885,7 @@ static void ensure_hel_coax_tiles(ops_dev *od, guint level, int N)
 static void ensure_hel_shift_rot_tiles(ops_dev *od, int N)
 {
   if (od->hel_shA[N]) return;
-  pthread_mutex_lock(&g_tab_mu);
+  pthread_mutex_lock(&g_ord_mu[N]);
   if (!g_tab.hel_shA[N]) {
     const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
     size_t n = (size_t)2 * P->KT * 64;
@@ -1853,7 +1899,7 @@ static void ensure_hel_shift_rot_tiles(ops_dev *od, int N)
     g_tab.hel_shA[N] = d;
   }
   od->hel_shA[N] = g_tab.hel_shA[N];
-  pthread_mutex_unlock(&g_tab_mu);
+  pthread_mutex_unlock(&g_ord_mu[N]);
 }
 
 /* real coaxial tiles of the Helmholtz matrix-core M2M / L2L between
@@ -1899,7 +1945,6 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
   if ((Ns > Nr ? Ns : Nr) > op->L[level])
     WBFMM_FATAL("S|R operators at level %u built to order %u", level, op->L[level]);
   if ((Ns > Nr ? Ns : Nr) > op->Lmax) WBFMM_FATAL("rotations built to order %u", op->Lmax);
-  ensure_hel_rotations(od);
   if (td->have_targets) {
     tl = td->d_act[level];
     nt = td->nact[level];
@@ -1911,7 +1956,7 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
     int N = (int)Ns;
     unsigned cstart[8], ccount[8];
     uint32_t *cls;
-    ensure_m2l_plan(od, N);
+    ensure_plan(od, N);
     ensure_hel_rot_tiles(od, N);
     ensure_hel_coax_tiles(od, level, N);
     HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
@@ -1922,6 +1967,7 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
                                  od->m2l_CS[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
+    ensure_hel_rotations(od); /* the vector kernels' H tables, on first use */
     HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
                             (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
                             od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
@@ -1936,7 +1982,7 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
       WBFMM_FATAL("S|S operators at level %u built to order %u", level + 1, od->ss_L[level + 1]);
     if (hel_shift_mfma(od, Nc, Nr)) {
       int N = (int)Nc;
-      ensure_shift_tables(od, N);
+      ensure_plan(od, N);
       ensure_hel_shift_rot_tiles(od, N);
       ensure_hel_shift_coax_tiles(od, level + 1, N);
       HIPOK(wbfmm_hip_hel_shift_mfma(1, tl, nt, NULL, NULL,
@@ -1945,6 +1991,7 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
                                      od->m2l_plan_dev[N], od->hel_shA[N],
                                      od->hel_SS2[level + 1], od->sh_CSdn[N], od->m2l_plan[N]));
     } else {
+      ensure_hel_rotations(od);
       HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
                               (int)Nr, (int)Nc, (double *)t->mpr[level],
                               (double *)t->mpr[level + 1], od->d_hel_Hud, od->hel_nrot,

@@ -449,6 +449,35 @@ static void put_tiles(double *dst, const double *T, const wbfmm_m2l_plan_t *P)
       }
 }
 
+static void theta_tiles(const wbfmm_m2l_plan_t *P, const packing *p13,
+                        const double *H, double *A1, double *A3, double *T);
+
+/* the rotation tiles of one polar angle (a host thread each) */
+typedef struct {
+  const wbfmm_m2l_plan_t *P;
+  const packing *p13;
+  const double *ths;
+  double *A1, *A3;
+  int rc;
+} lap_tile_job;
+
+static void lap_tile_one(int k, void *c)
+{
+  lap_tile_job *j = (lap_tile_job *)c;
+  const wbfmm_m2l_plan_t *P = j->P;
+  size_t per = (size_t)P->KT * 64, S = (size_t)16 * P->NB;
+  double *H = malloc(sizeof(double) * rot_size(P->N)), *T = calloc(S * S, sizeof(double));
+  if (!H || !T) {
+    j->rc = WBFMM_HIP_ERR_NOMEM;
+  } else {
+    H_rotation(H, P->N, j->ths[k]);
+    theta_tiles(P, j->p13, H, j->A1 ? j->A1 + k * per : NULL, j->A3 ? j->A3 + k * per : NULL,
+                T);
+  }
+  free(H);
+  free(T);
+}
+
 /* stage-1 and stage-3 tiles of one polar angle (H: its H table to order
    N); T: scratch of (16 NB)^2 doubles */
 static void theta_tiles(const wbfmm_m2l_plan_t *P, const packing *p13,
@@ -562,20 +591,14 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
       }
   if (!A1 && !A3) return 0;
 
-  T = calloc((size_t)S * S, sizeof(double));
-  if (!T) return WBFMM_HIP_ERR_NOMEM;
-  if (A1 || A3) {
-    size_t per = (size_t)P->KT * 64;
-    int k;
-    hc.n = 0;
-    hc.L = N;
-    for (k = 0; k < P->NTH; k++)
-      theta_tiles(P, &p13, cached_H(&hc, ths[k]), A1 ? A1 + k * per : NULL,
-                  A3 ? A3 + k * per : NULL, T);
-    for (i = 0; i < hc.n; i++) free(hc.H[i]);
+  {
+    lap_tile_job j = {P, &p13, ths, A1, A3, 0};
+    (void)hc;
+    (void)T;
+    (void)S;
+    par_for(P->NTH, lap_tile_one, &j);
+    return j.rc;
   }
-  free(T);
-  return 0;
 }
 
 /* stage 2 at box width w: S|R(n, nu, m) / t^(n+nu+1), t = rr[i] w, the
@@ -607,21 +630,24 @@ int wbfmm_hip_lap_m2l_coax(const wbfmm_m2l_plan_t *P, double w, double *A2)
 int wbfmm_hip_lap_m2l_plan(int N, wbfmm_m2l_plan_t *P, void **plan_dev,
                            double **A1, double **A3, double **CS)
 {
-  double *h1, *h3, *hc;
+  double *h1 = NULL, *h3 = NULL, *hc;
   size_t n13, ncs;
+  const int tiles = A1 != NULL && A3 != NULL;
   int rc = wbfmm_hip_lap_m2l_tables(N, P, NULL, NULL, NULL);
   if (rc) return rc;
   n13 = (size_t)P->NTH * P->KT * 64;
   ncs = (size_t)343 * (N + 1) * 2;
-  h1 = malloc(sizeof(double) * n13);
-  h3 = malloc(sizeof(double) * n13);
+  if (tiles) {
+    h1 = malloc(sizeof(double) * n13);
+    h3 = malloc(sizeof(double) * n13);
+  }
   hc = malloc(sizeof(double) * ncs);
-  if (!h1 || !h3 || !hc) rc = WBFMM_HIP_ERR_NOMEM;
+  if ((tiles && (!h1 || !h3)) || !hc) rc = WBFMM_HIP_ERR_NOMEM;
   if (!rc) rc = wbfmm_hip_lap_m2l_tables(N, P, h1, h3, hc);
-  if (!rc) rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * n13);
-  if (!rc) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * n13);
-  if (!rc) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * n13);
-  if (!rc) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * n13);
+  if (!rc && tiles) rc = wbfmm_hip_malloc((void **)A1, sizeof(double) * n13);
+  if (!rc && tiles) rc = wbfmm_hip_memcpy(*A1, h1, sizeof(double) * n13);
+  if (!rc && tiles) rc = wbfmm_hip_malloc((void **)A3, sizeof(double) * n13);
+  if (!rc && tiles) rc = wbfmm_hip_memcpy(*A3, h3, sizeof(double) * n13);
   if (!rc) rc = wbfmm_hip_malloc((void **)CS, sizeof(double) * ncs);
   if (!rc) rc = wbfmm_hip_memcpy(*CS, hc, sizeof(double) * ncs);
   if (!rc) rc = wbfmm_hip_malloc(plan_dev, sizeof(*P));
