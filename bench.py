@@ -65,19 +65,23 @@ def workload(n=1_000_000, seed=3):
     return xs, q, x0, D
 
 
-def m2l_pairs(xs, x0, D, depth):
-    """Occupied-target x occupied-source list-4 pairs per level (targets =
-    sources).  A target at grid (i,j,k) interacts with the sources in the
-    6x6x6 block of its parent's neighbours' children, near neighbours
-    excluded."""
+def m2l_pairs(xs, x0, D, depth, xt=None):
+    """Occupied-target x occupied-source list-4 pairs per level (targets xt,
+    the sources when None).  A target at grid (i,j,k) interacts with the
+    sources in the 6x6x6 block of its parent's neighbours' children, near
+    neighbours excluded."""
     pairs = {}
     for l in range(2, depth + 1):
         nb = 1 << l
-        g = np.floor((xs - x0) / D * nb).astype(np.int64)
-        g = np.clip(g, 0, nb - 1)
-        occ = np.zeros((nb + 6, nb + 6, nb + 6), dtype=np.int32)
-        occ[g[:, 0] + 3, g[:, 1] + 3, g[:, 2] + 3] = 1
-        core = occ[3:nb + 3, 3:nb + 3, 3:nb + 3]
+
+        def grid(x):
+            g = np.floor((x - x0) / D * nb).astype(np.int64)
+            g = np.clip(g, 0, nb - 1)
+            o = np.zeros((nb + 6, nb + 6, nb + 6), dtype=np.int32)
+            o[g[:, 0] + 3, g[:, 1] + 3, g[:, 2] + 3] = 1
+            return o
+        occ = grid(xs)
+        core = (occ if xt is None else grid(xt))[3:nb + 3, 3:nb + 3, 3:nb + 3]
         tot = 0
         for px in range(2):
             for py in range(2):
@@ -226,6 +230,25 @@ def run_helmholtz(args):
     fdc = fd[:, 0::2] + 1j * fd[:, 1::2]
     err_pot = float(np.linalg.norm(fc[:, 0] - fdc[:, 0]) / np.linalg.norm(fdc[:, 0]))
     err_grad = float(np.linalg.norm(fc[:, 1:] - fdc[:, 1:]) / np.linalg.norm(fdc[:, 1:]))
+    # roofline of the M2L: the reference's operation count per translation
+    # of order N (two rotations, real H on complex coefficients: 4 flops per
+    # entry of the (2n+1)^2 blocks; the complex coaxial product: 8 flops per
+    # entry of the order-m blocks), times the occupied list-4 pairs of each
+    # level
+    pairs = m2l_pairs(xs, x0, D, depth, None if xt is xs else xt)
+
+    def hel_flops_pair(N):
+        Rn = (N + 1) * (2 * N + 1) * (2 * N + 3) // 3
+        Cn = (N + 1) ** 2 + 2 * sum((N + 1 - m) ** 2 for m in range(1, N + 1))
+        return 2 * 4 * Rn + 8 * Cn
+    m2l_flops_step = sum(pairs[l] * hel_flops_pair(orders[l]) for l in pairs)
+    m2l = kern.get("hel_m2l", {"ms_per_step": float("nan"), "launches_per_step": 1})
+    achieved = m2l_flops_step / (m2l["ms_per_step"] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "hel_m2l", "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
+                "pairs_per_step": int(sum(pairs.values())),
+                "flops_per_pair": {str(l): hel_flops_pair(orders[l]) for l in pairs}}
     lat_ms = np.array(lat) * 1e3
     out = {
         "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
@@ -238,6 +261,7 @@ def run_helmholtz(args):
                    "k": k, "orders": orders[2:depth + 1], "parallelism": "single GPU"},
         "error_vs_fp64_direct": {"targets_checked": int(len(idx)), "rel_l2_potential": err_pot,
                                  "rel_l2_gradient": err_grad},
+        "roofline": roofline,
         "tables_s": t_tables,
         "kernels": kern,
     }
