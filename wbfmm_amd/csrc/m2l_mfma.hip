@@ -309,6 +309,44 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
 // is deterministic: every LDS slot has one writer.
 // ---------------------------------------------------------------------
 
+// one stage with its results stored at once into the other LDS buffer
+// (the double-buffered form: the stage reads Rin and writes Rout, so no
+// barrier is needed between its reads and its writes)
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_wo(const T *__restrict__ a, const T *Rin, T *Rout,
+                                         const int4 *tm, int lane, int grp, int col) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+  constexpr int AD = WS.n < 16 ? (WS.n > 0 ? WS.n : 1) : 16;
+  T av[AD];
+#pragma unroll
+  for (int s = 0; s < AD && s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      r4 = MF<T>::mma(av[s % AD], Rin[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
+      if (s + AD < WS.n) av[s % AD] = a[WS.t[s + AD] * 64 + lane];
+    }
+    const int4 d = tm[4 * b + grp];
+    Rout[d.x * 16 + col] = r4[0];
+    Rout[d.y * 16 + col] = r4[1];
+    Rout[d.z * 16 + col] = r4[2];
+    Rout[d.w * 16 + col] = r4[3];
+  }
+}
+
+// the exchange buffer is doubled where the two copies take at most 48 KB,
+// which leaves room for three blocks per CU (FP64 up to order 12, FP32 up
+// to order 17)
+template <typename T, int N>
+struct M2LDouble {
+  static constexpr bool value = 2 * (M2LShape<N>::S + 1) * 16 * sizeof(T) <= 48 * 1024;
+};
+
 template <typename T, int N, int W, int WI>
 __device__ __forceinline__ void m2l_w2_loop(
     T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
@@ -323,6 +361,8 @@ __device__ __forceinline__ void m2l_w2_loop(
   typedef typename MF<T>::v4 V4;
   using SH = M2LShape<N>;
   constexpr int C = SH::C, NU = SH::NU;
+  constexpr bool DB = M2LDouble<T, N>::value;
+  T *R1 = R + (SH::S + 1) * 16;  // the second buffer (DB only)
   constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane
   constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
   constexpr int NBW = WS.nb;
@@ -420,32 +460,46 @@ __device__ __forceinline__ void m2l_w2_loop(
       ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
     }
 
-    V4 y[NBW > 0 ? NBW : 1];
-    stage_w<T, N, W, WI>(A1 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
-    __syncthreads();
-    write_out(t1, y, gp, cl);
-    __syncthreads();
-    if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
+    // double-buffered: R -> R1 -> R -> R1, one barrier per stage; the
+    // deposit of the next offset goes into R while R1 is read below
+    T *RZ = R;
+    if constexpr (DB) {
+      stage_wo<T, N, W, WI>(A1 + (size_t)th * SH::KT * 64, R, R1, t1, ln, gp, cl);
+      __syncthreads();
+      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
+      stage_wo<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R1, R, t2, ln, gp, cl);
+      __syncthreads();
+      stage_wo<T, N, W, WI>(A3 + (size_t)th * SH::KT * 64, R, R1, t3, ln, gp, cl);
+      __syncthreads();
+      RZ = R1;
+    } else {
+      V4 y[NBW > 0 ? NBW : 1];
+      stage_w<T, N, W, WI>(A1 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t1, y, gp, cl);
+      __syncthreads();
+      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
 
-    stage_w<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
-    __syncthreads();
-    write_out(t2, y, gp, cl);
-    __syncthreads();
+      stage_w<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t2, y, gp, cl);
+      __syncthreads();
 
-    stage_w<T, N, W, WI>(A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
-    __syncthreads();
-    write_out(t3, y, gp, cl);
-    __syncthreads();
+      stage_w<T, N, W, WI>(A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t3, y, gp, cl);
+      __syncthreads();
+    }
 
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const T2 e = cs[p][tu[u].y];
-      const T tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+      const T tr = RZ[u * 16 + cl], tq = RZ[(NU + u) * 16 + cl];
       acc_re[i] += e.x * tr + e.y * tq;
       acc_im[i] += e.x * tq - e.y * tr;
     }
-    __syncthreads();  // every read of R done before the next deposit
+    if constexpr (!DB) __syncthreads();  // every read of R done before the next deposit
 
     if (kn < 216) {
       if (PREFETCH)
@@ -502,7 +556,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   typedef typename MF<T>::t2 T2;
   using SH = M2LShape<N>;
   constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
-  __shared__ T R[(S + 1) * 16];
+  __shared__ T R[(M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
   __shared__ T2 cs[2][N + 1];
@@ -534,7 +588,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
     uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
                         (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = T(0);
+  for (int i = tid; i < (M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16; i += 64 * W) R[i] = T(0);
   for (int i = tid; i < 16 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
 
   const int split = (int)(blockIdx.x % (unsigned)nsplit);
