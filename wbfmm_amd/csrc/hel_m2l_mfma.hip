@@ -78,6 +78,48 @@ __device__ __forceinline__ void stage_cx(const double *__restrict__ ar,
   }
 }
 
+// stage 2 of the double-buffered form: as stage_cx, results stored per bin
+// into the other buffer
+template <int N, int W, int WI>
+__device__ __forceinline__ void stage_cx_wo(const double *__restrict__ ar,
+                                            const double *__restrict__ ai, const double *Rin,
+                                            double *Rout, const int4 *tm, int lane, int grp,
+                                            int col) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+  constexpr int AD = WS.n < 8 ? (WS.n > 0 ? WS.n : 1) : 8;
+  double avr[AD], avi[AD];
+#pragma unroll
+  for (int s = 0; s < AD && s < WS.n; s++) {
+    avr[s] = ar[WS.t[s] * 64 + lane];
+    avi[s] = ai[WS.t[s] * 64 + lane];
+  }
+  const double sg = (col & 1) ? 1.0 : -1.0;
+  const int pc = col ^ 1;
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    d4 r4 = {0.0, 0.0, 0.0, 0.0}, i4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      const int row = (BN.kbase[b] + 4 * j + grp) * 16;
+      r4 = MF<double>::mma(avr[s % AD], Rin[row + col], r4);
+      i4 = MF<double>::mma(avi[s % AD], Rin[row + pc], i4);
+      if (s + AD < WS.n) {
+        avr[s % AD] = ar[WS.t[s + AD] * 64 + lane];
+        avi[s % AD] = ai[WS.t[s + AD] * 64 + lane];
+      }
+    }
+    const d4 y = r4 + sg * i4;
+    const int4 d = tm[4 * b + grp];
+    Rout[d.x * 16 + col] = y[0];
+    Rout[d.y * 16 + col] = y[1];
+    Rout[d.z * 16 + col] = y[2];
+    Rout[d.w * 16 + col] = y[3];
+  }
+}
+
 template <int N, int W, int WI>
 __device__ __forceinline__ void hel_m2l_loop(
     double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
@@ -93,6 +135,8 @@ __device__ __forceinline__ void hel_m2l_loop(
   constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane
   constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
   constexpr int NBW = WS.nb;
+  constexpr bool DB = M2LDouble<double, N>::value;
+  double *R1 = R + (SH::S + 1) * 16;  // the second buffer (DB only)
   const int ubase = grp + 4 * WI, tg = col >> 1;
 
   auto next_k = [&](int k) {
@@ -186,23 +230,38 @@ __device__ __forceinline__ void hel_m2l_loop(
       ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
     }
 
-    d4 y[NBW > 0 ? NBW : 1];
-    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
-    __syncthreads();
-    write_out(t1, y, gp, cl);
-    __syncthreads();
-    if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
+    // double-buffered where it fits: R -> R1 -> R -> R1, one barrier per
+    // stage, and the next deposit goes into R while R1 is read below
+    double *RZ = R;
+    if constexpr (DB) {
+      stage_wo<double, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t1, ln, gp, cl);
+      __syncthreads();
+      if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
+      stage_cx_wo<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+                            R1, R, t2, ln, gp, cl);
+      __syncthreads();
+      stage_wo<double, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t3, ln, gp, cl);
+      __syncthreads();
+      RZ = R1;
+    } else {
+      d4 y[NBW > 0 ? NBW : 1];
+      stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t1, y, gp, cl);
+      __syncthreads();
+      if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
 
-    stage_cx<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64, R,
-                       ln, gp, cl, y);
-    __syncthreads();
-    write_out(t2, y, gp, cl);
-    __syncthreads();
+      stage_cx<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+                         R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t2, y, gp, cl);
+      __syncthreads();
 
-    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
-    __syncthreads();
-    write_out(t3, y, gp, cl);
-    __syncthreads();
+      stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t3, y, gp, cl);
+      __syncthreads();
+    }
 
     // y_{+-nu} = (P +- Q)/2 (y_0 = P/2), then L^{+-nu} += e^{-+i nu chi} y
     {
@@ -213,16 +272,16 @@ __device__ __forceinline__ void hel_m2l_loop(
         const int nu = tu[u].y;
         const double2 e = cs[p][nu];
         const double es = sp ? -e.y : e.y;
-        const double pr = R[u * 16 + c0], pi = R[u * 16 + c0 + 1];
-        const double qr = nu ? R[(NU + u) * 16 + c0] : 0.0;
-        const double qi = nu ? R[(NU + u) * 16 + c0 + 1] : 0.0;
+        const double pr = RZ[u * 16 + c0], pi = RZ[u * 16 + c0 + 1];
+        const double qr = nu ? RZ[(NU + u) * 16 + c0] : 0.0;
+        const double qi = nu ? RZ[(NU + u) * 16 + c0 + 1] : 0.0;
         const double yr = 0.5 * (sp ? pr - qr : pr + qr);
         const double yi = 0.5 * (sp ? pi - qi : pi + qi);
         acc_re[i] += e.x * yr + es * yi;
         acc_im[i] += e.x * yi - es * yr;
       }
     }
-    __syncthreads();  // every read of R done before the next deposit
+    if constexpr (!DB) __syncthreads();  // every read of R done before the next deposit
 
     if (kn < 216) deposit(tu, ub, cl, p ^ 1);
     __syncthreads();
@@ -261,7 +320,7 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
     const double2 *__restrict__ zero, int nsplit, double2 *__restrict__ part, unsigned ntot) {
   using SH = M2LShape<N>;
   constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
-  __shared__ double R[(S + 1) * 16];
+  __shared__ double R[(M2LDouble<double, N>::value ? 2 : 1) * (S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
   __shared__ double2 cs[2][N + 1];
@@ -292,7 +351,8 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
     const int m = u - n * (n + 1) / 2;
     uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < (M2LDouble<double, N>::value ? 2 : 1) * (S + 1) * 16; i += 64 * W)
+    R[i] = 0.0;
   for (int i = tid; i < 8 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
 
   const int split = (int)(blockIdx.x % (unsigned)nsplit);

@@ -188,6 +188,44 @@ __device__ __forceinline__ void stage_w(const T *__restrict__ a, const T *R, int
   }
 }
 
+// one stage with its results stored at once into the other LDS buffer
+// (the double-buffered form: the stage reads Rin and writes Rout, so no
+// barrier is needed between its reads and its writes)
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_wo(const T *__restrict__ a, const T *Rin, T *Rout,
+                                         const int4 *tm, int lane, int grp, int col) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+  constexpr int AD = WS.n < 16 ? (WS.n > 0 ? WS.n : 1) : 16;
+  T av[AD];
+#pragma unroll
+  for (int s = 0; s < AD && s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      r4 = MF<T>::mma(av[s % AD], Rin[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
+      if (s + AD < WS.n) av[s % AD] = a[WS.t[s + AD] * 64 + lane];
+    }
+    const int4 d = tm[4 * b + grp];
+    Rout[d.x * 16 + col] = r4[0];
+    Rout[d.y * 16 + col] = r4[1];
+    Rout[d.z * 16 + col] = r4[2];
+    Rout[d.w * 16 + col] = r4[3];
+  }
+}
+
+// the exchange buffer is doubled where the two copies take at most 48 KB,
+// which leaves room for three blocks per CU (FP64 up to order 12, FP32 up
+// to order 17)
+template <typename T, int N>
+struct M2LDouble {
+  static constexpr bool value = 2 * (M2LShape<N>::S + 1) * 16 * sizeof(T) <= 48 * 1024;
+};
+
 // mpr[cls[i]] += sum over the splits (in split order) of the partials of
 // target i
 template <typename T>
