@@ -785,6 +785,206 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
   }
 }
 
+// ---------------------------------------------------------------------
+// M2M / L2L for orders 16..24 (FP64): the blocks of more than 16 rows need
+// more registers than one wavefront holds, so W = 8 wavefronts share the
+// LDS buffer of 16 parents as in the M2L (wavefront w computes the bins
+// b = w (mod W) and owns the units u = grp + 4 w + 4 W i).  The
+// directions, tiles and sums are those of k_lap_shift_mfma; the children
+// of a parent are added in child order.
+// ---------------------------------------------------------------------
+template <int N, int W, int WI, int MODE>
+__device__ __forceinline__ void lap_shift_loop(
+    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    const double2 (*cs)[N + 1], const unsigned *cmask, unsigned anyc, int lane, int col,
+    int grp, bool pvalid, uint64_t p, const double *__restrict__ in, double *__restrict__ out,
+    const double *__restrict__ A1, const double *__restrict__ A3,
+    const double *__restrict__ A2, const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
+  constexpr int NBW = WS.nb;
+  const int ubase = grp + 4 * WI;
+
+  double pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const double *src, const int4 *tu, int ub) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const double *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  // lanes past the last unit write nothing (one writer per LDS slot)
+  auto deposit = [&](const int4 *tu, int ub, int cl, int c) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      if (ub + 4 * W * i >= NU) continue;
+      const int4 inf = tu[ub + 4 * W * i];
+      const double2 e = cs[c][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
+  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + W * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  double acc_re[NAW], acc_im[NAW];
+#pragma unroll
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = 0.0;
+  if (MODE == 1) load_sources(pvalid ? in + p * (uint64_t)C : zero, uinf, ubase);
+
+  for (int c = 0; c < 8; c++) {
+    if (!((anyc >> c) & 1u)) continue;  // the same in every lane of the block
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+    const bool v = (cmask[cl] >> c) & 1u;
+    const int th = MODE == 0 ? (c >> 2) & 1 : 1 - ((c >> 2) & 1);
+    if (MODE == 0) load_sources(v ? in + (8 * p + c) * (uint64_t)C : zero, tu, ub);
+    __syncthreads();  // every read of R by the last direction done
+    deposit(tu, ub, cl, c);
+    __syncthreads();
+
+    d4 y[NBW > 0 ? NBW : 1];
+    stage_w<double, N, W, WI>(A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t1, y, gp, cl);
+    __syncthreads();
+    stage_w<double, N, W, WI>(A2, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t2, y, gp, cl);
+    __syncthreads();
+    stage_w<double, N, W, WI>(A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t3, y, gp, cl);
+    __syncthreads();
+
+    double *dst = out + (8 * p + c) * (uint64_t)C;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const double2 e = cs[c][inf.y];
+      const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+      const double orr = e.x * tr + e.y * tq, oi = e.x * tq - e.y * tr;
+      if (MODE == 0) {
+        acc_re[i] += orr;
+        acc_im[i] += oi;
+      } else if (v && ub + 4 * W * i < NU) {
+        dst[inf.x] += orr;
+        if (inf.y) dst[inf.x + 1] += oi;
+      }
+    }
+  }
+  if (MODE == 0 && pvalid) {
+    double *dst = out + p * (uint64_t)C;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = ubase + 4 * W * i;
+      if (u >= NU) continue;
+      const int4 inf = uinf[u];
+      dst[inf.x] = acc_re[i];
+      if (inf.y) dst[inf.x + 1] = acc_im[i];
+    }
+  }
+}
+
+template <int N, int W, int MODE>
+__global__ __launch_bounds__(64 * W) void k_lap_shift_mfma2(
+    const uint32_t *__restrict__ parents, unsigned npar, const uint32_t *__restrict__ cstart,
+    const uint32_t *__restrict__ cend, const uint32_t *__restrict__ cflag,
+    const double *__restrict__ in, double *__restrict__ out,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  __shared__ double R[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[8][N + 1];
+  __shared__ unsigned cmask[16];
+  __shared__ unsigned anyc;
+
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < 8 * (N + 1); i += 64 * W)
+    cs[i / (N + 1)][i % (N + 1)] = make_double2(CS[2 * i], CS[2 * i + 1]);
+  const unsigned pi = blockIdx.x * 16 + col;
+  const bool pvalid = pi < npar;
+  const uint64_t p = pvalid ? parents[pi] : 0;
+  if (tid < 16) {
+    const unsigned pj = blockIdx.x * 16 + tid;
+    unsigned m = 0;
+    if (pj < npar) {
+      const uint64_t pp = parents[pj];
+      for (int c = 0; c < 8; c++) {
+        const uint64_t cb = 8 * pp + c;
+        const bool v = MODE == 0 ? cend[cb] > cstart[cb] : (!cflag || cflag[cb] != 0);
+        if (v) m |= 1u << c;
+      }
+    }
+    cmask[tid] = m;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned a = 0;
+    for (int j = 0; j < 16; j++) a |= cmask[j];
+    anyc = a;
+  }
+  __syncthreads();
+  const unsigned any = __builtin_amdgcn_readfirstlane(anyc);
+
+#define LAP_SHIFT(WI)                                                                       \
+  lap_shift_loop<N, W, WI, MODE>(R, tmap1, tmap2, tmap3, uinf, cs, cmask, any, lane, col,  \
+                                 grp, pvalid, p, in, out, A1, A3, A2, zero)
+  static_assert(W == 8, "eight wavefronts per block");
+  switch (wv) {
+    case 0: LAP_SHIFT(0); break;
+    case 1: LAP_SHIFT(1); break;
+    case 2: LAP_SHIFT(2); break;
+    case 3: LAP_SHIFT(3); break;
+    case 4: LAP_SHIFT(4); break;
+    case 5: LAP_SHIFT(5); break;
+    case 6: LAP_SHIFT(6); break;
+    default: LAP_SHIFT(7); break;
+  }
+#undef LAP_SHIFT
+}
+
 // occupancy bitmap of a level: bit b set when box b holds points
 __global__ void k_occ_bits(const uint32_t *start, const uint32_t *end,
                            uint64_t nbox, uint32_t *bits) {
@@ -921,11 +1121,19 @@ static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *
     fprintf(stderr, "wbfmm (HIP): plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
-  WBFMM_LAUNCH(sizeof(T) == 4 ? (MODE == 0 ? "lap_m2m_f32" : "lap_l2l_f32")
-                              : (MODE == 0 ? "lap_m2m" : "lap_l2l"),
-               (k_lap_shift_mfma<T, N, MODE>),
-               dim3((npar + 15) / 16), dim3(64), 0, parents, npar, cstart, cend, cflag, in,
-               out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+  if constexpr (N <= 15) {
+    WBFMM_LAUNCH(sizeof(T) == 4 ? (MODE == 0 ? "lap_m2m_f32" : "lap_l2l_f32")
+                                : (MODE == 0 ? "lap_m2m" : "lap_l2l"),
+                 (k_lap_shift_mfma<T, N, MODE>),
+                 dim3((npar + 15) / 16), dim3(64), 0, parents, npar, cstart, cend, cflag, in,
+                 out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+  } else if constexpr (sizeof(T) == 8) {
+    WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma2<N, 8, MODE>),
+                 dim3((npar + 15) / 16), dim3(512), 0, parents, npar, cstart, cend, cflag, in,
+                 out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+  } else {
+    return WBFMM_HIP_ERR_ORDER;  // FP32 passes: orders up to 15
+  }
   return 0;
 }
 
@@ -971,7 +1179,8 @@ static int shift_entry(int mode, const uint32_t *parents, unsigned npar,
                      : launch_shift<T, n, 1>(parents, npar, cstart, cend, cflag, in, out, \
                                              plan_dev, A1, A3, A2, CS, zero, plan_host);
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
-    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15)
+    CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
 #undef CASE
     default:
       return WBFMM_HIP_ERR_ORDER;

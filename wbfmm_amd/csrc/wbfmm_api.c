@@ -140,7 +140,8 @@ typedef struct {
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
 #define M2L_MFMA_MIN 2
 #define M2L_MFMA_MAX 24
-/* orders served by the MFMA M2M / L2L and by the FP32 passes */
+/* orders served by the FP32 passes (the FP64 matrix-core M2M / L2L serve
+   M2L_MFMA_MIN..M2L_MFMA_MAX) */
 #define SHIFT_MFMA_MAX 15
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
@@ -1053,8 +1054,8 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     return 0;
   }
   if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
-      (int)t->order_s[level] <= SHIFT_MFMA_MAX) {
-    /* matrix-core form: 16 parents per wavefront */
+      (int)t->order_s[level] <= M2L_MFMA_MAX) {
+    /* matrix-core form: 16 parents per wavefront (8 wavefronts above 15) */
     int N = (int)t->order_s[level];
     double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << level);
     ensure_shift_tables(od, N);
@@ -1178,7 +1179,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
   if (t->order_r[level] == t->order_r[level + 1] && (int)t->order_r[level] >= M2L_MFMA_MIN &&
-      (int)t->order_r[level] <= SHIFT_MFMA_MAX) {
+      (int)t->order_r[level] <= M2L_MFMA_MAX) {
     int N = (int)t->order_r[level];
     double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << (level + 1));
     ensure_shift_tables(od, N);

@@ -217,6 +217,60 @@ static const double *cached_H(hcache *hc, double th)
   return hc->H[hc->n++];
 }
 
+/* the H tables of every polar angle the list-4 offsets and the 8 child
+   directions use, computed on host threads into the cache */
+typedef struct {
+  hcache *hc;
+} hfill_job;
+
+static void hfill_one(int i, void *c)
+{
+  hcache *hc = ((hfill_job *)c)->hc;
+  H_rotation(hc->H[i], hc->L, hc->th[i]);
+}
+
+static void prefill_H(hcache *hc)
+{
+  int dx, dy, dz, c, i, k;
+  double e[8 + 343][3];
+  hfill_job j;
+  k = 0;
+  for (dx = -3; dx <= 3; dx++)
+    for (dy = -3; dy <= 3; dy++)
+      for (dz = -3; dz <= 3; dz++) {
+        if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+        e[k][0] = -dx;
+        e[k][1] = -dy;
+        e[k][2] = -dz;
+        k++;
+      }
+  for (c = 0; c < 8; c++) {
+    e[k][0] = 0.5 - (c & 1);
+    e[k][1] = 0.5 - ((c >> 1) & 1);
+    e[k][2] = 0.5 - ((c >> 2) & 1);
+    k++;
+  }
+  hc->n = 0;
+  for (i = 0; i < k; i++) {
+    double th, ph, ch;
+    int q, s;
+    for (s = 1; s >= -1; s -= 2) {
+      dir_angles(s * e[i][0], s * e[i][1], s * e[i][2], &th, &ph, &ch);
+      for (q = 0; q < hc->n; q++)
+        if (hc->th[q] == th) break;
+      if (q == hc->n && hc->n < 400) {
+        double *H = malloc(sizeof(double) * rot_size(hc->L));
+        if (!H) continue; /* left to cached_H */
+        hc->th[hc->n] = th;
+        hc->H[hc->n] = H;
+        hc->n++;
+      }
+    }
+  }
+  j.hc = hc;
+  par_for(hc->n, hfill_one, &j);
+}
+
 static void add_direction(double *host, size_t fwd, size_t back, hcache *hc,
                           double ex, double ey, double ez)
 {
@@ -251,6 +305,7 @@ int wbfmm_hip_rotation_tables(int L, double **dev, wbfmm_rot_layout_t *lay)
   if (!host) return WBFMM_HIP_ERR_NOMEM;
   hc.n = 0;
   hc.L = L;
+  prefill_H(&hc);
   for (dx = -3; dx <= 3; dx++)
     for (dy = -3; dy <= 3; dy++)
       for (dz = -3; dz <= 3; dz++) {
