@@ -382,6 +382,54 @@ gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *op);
    download returns the coefficients in the reference's scaling. */
 gint wbfmm_tree_set_precision(wbfmm_tree_t *t, gint bits);
 
+/* ---- single precision (src/wbfmm.h:432-440, :690-740, :996-1112) ------
+   The Laplace tree pipeline, its target lists and the direct sum with
+   float arguments.  A tree made by wbfmm_tree_new_f is an FP32 tree, as
+   after wbfmm_tree_set_precision(t, 32): orders 2..15, the same order at
+   every level.  Float arrays are host arrays; float outputs are
+   accumulated in float.  The tree origin is held in double. */
+wbfmm_tree_t *wbfmm_tree_new_f(gfloat *x, gfloat D, guint maxpoints);
+gint wbfmm_tree_add_points_f(wbfmm_tree_t *t, gpointer pts, gsize pstr,
+                             gpointer normals, gsize nstr, guint npts,
+                             gboolean sorted);
+gint wbfmm_tree_refine_f(wbfmm_tree_t *t);
+gint wbfmm_tree_laplace_coefficient_init_f(wbfmm_tree_t *t, guint l, guint nr,
+                                           guint ns);
+gint wbfmm_tree_laplace_leaf_expansions_f(wbfmm_tree_t *t, gfloat *src,
+                                          gint sstr, gfloat *dipoles,
+                                          gint dstr, gboolean zero_expansions,
+                                          gfloat *work);
+gint wbfmm_shift_angle_table_init_f(void);
+wbfmm_shift_operators_t *wbfmm_shift_operators_new_f(guint L, gboolean bw,
+                                                     gfloat *work);
+gint wbfmm_laplace_coaxial_translate_init_f(gint N);
+gint wbfmm_laplace_upward_pass_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                                 guint level, gfloat *work);
+gint wbfmm_laplace_downward_pass_ref_f(wbfmm_tree_t *t,
+                                       wbfmm_shift_operators_t *op,
+                                       guint level, gfloat *work,
+                                       gint nthreads);
+#define wbfmm_laplace_downward_pass_f(_t, _op, _level, _work, _nth) \
+  wbfmm_laplace_downward_pass_ref_f(_t, _op, _level, _work, _nth)
+gint wbfmm_laplace_box_field_f(wbfmm_tree_t *t, guint level, guint b,
+                               gfloat *src, gint sstr, gfloat *d, gint dstr,
+                               guint field, gboolean eval_neighbours,
+                               gfloat *x, gfloat *f, gint fstr, gfloat *work);
+gint wbfmm_laplace_field_direct_f(gfloat *xs, gint xstride, gfloat *n,
+                                  gint nstr, gint nsrc, gfloat *src,
+                                  gint sstr, gfloat *d, gint dstr, gint nq,
+                                  guint field, gfloat *xf, gfloat *f,
+                                  gint fstr);
+wbfmm_target_list_t *wbfmm_target_list_new_f(wbfmm_tree_t *t, guint npts);
+gint wbfmm_target_list_add_points_f(wbfmm_target_list_t *l, gpointer pts,
+                                    gsize pstr, guint npts);
+gint wbfmm_laplace_target_list_local_coefficients_f(wbfmm_target_list_t *l,
+                                                    guint source,
+                                                    gfloat *work);
+gint wbfmm_target_list_local_field_f(wbfmm_target_list_t *l, gfloat *src,
+                                     gint sstr, gfloat *d, gint dstr,
+                                     gfloat *f, gint fstr);
+
 #ifdef __cplusplus
 }
 #endif

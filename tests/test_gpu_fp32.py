@@ -120,3 +120,99 @@ def test_fp32_error_against_direct(wb, oracle):
     e32 = np.linalg.norm(f32[idx, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0])
     assert e64 < 1e-4 and e32 < 1e-4
     assert abs(e32 - e64) < 0.1 * e64 + 1e-6, (e32, e64)
+
+
+def _f_api(wb):
+    """ctypes signatures of the single-precision entry points"""
+    import ctypes
+    L = wb.lib()
+    vp, i, u, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_float
+    sz = ctypes.c_size_t
+    sigs = {
+        "wbfmm_tree_new_f": (ctypes.POINTER(wb.WbfmmTree), [vp, f, u]),
+        "wbfmm_tree_add_points_f": (i, [vp, vp, sz, vp, sz, u, i]),
+        "wbfmm_tree_refine_f": (i, [vp]),
+        "wbfmm_tree_laplace_coefficient_init_f": (i, [vp, u, u, u]),
+        "wbfmm_tree_laplace_leaf_expansions_f": (i, [vp, vp, i, vp, i, i, vp]),
+        "wbfmm_shift_angle_table_init_f": (i, []),
+        "wbfmm_shift_operators_new_f": (vp, [u, i, vp]),
+        "wbfmm_laplace_coaxial_translate_init_f": (i, [i]),
+        "wbfmm_laplace_upward_pass_f": (i, [vp, vp, u, vp]),
+        "wbfmm_laplace_downward_pass_ref_f": (i, [vp, vp, u, vp, i]),
+        "wbfmm_laplace_box_field_f": (i, [vp, u, u, vp, i, vp, i, u, i, vp, vp, i, vp]),
+        "wbfmm_laplace_field_direct_f": (i, [vp, i, vp, i, i, vp, i, vp, i, i, u, vp, vp, i]),
+        "wbfmm_target_list_new_f": (vp, [vp, u]),
+        "wbfmm_target_list_add_points_f": (i, [vp, vp, sz, u]),
+        "wbfmm_laplace_target_list_local_coefficients_f": (i, [vp, u, vp]),
+        "wbfmm_target_list_local_field_f": (i, [vp, vp, i, vp, i, vp, i]),
+        "wbfmm_target_list_coefficients_init": (i, [vp, u]),
+        "wbfmm_target_list_free": (i, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def test_single_precision_api(wb, oracle):
+    # the float entry points (src/wbfmm.h:432-440, :690-740, :996-1112) run
+    # the FP32 tree: with the same float inputs they give the bits of the
+    # double entry points on an FP32 tree, rounded to float
+    import ctypes
+    L = _f_api(wb)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rng = np.random.default_rng(91)
+    n, nt, depth, p = 4000, 700, 4, 9
+    xs = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    xt = rng.uniform(0, 1, (nt, 3)).astype(np.float32)
+    q = rng.normal(size=n).astype(np.float32)
+    x0 = np.full(3, -0.01, np.float32)
+    D = np.float32(1.02)
+
+    tp = L.wbfmm_tree_new_f(P(x0), D, n)
+    t = ctypes.cast(tp, ctypes.c_void_p).value
+    assert L.wbfmm_tree_add_points_f(t, P(xs), 12, None, 0, n, 0) == 0
+    for _ in range(depth):
+        L.wbfmm_tree_refine_f(t)
+    tp.contents.problem = 1
+    tp.contents.nq = 1
+    for lv in range(1, depth + 1):
+        L.wbfmm_tree_laplace_coefficient_init_f(t, lv, p, p)
+    L.wbfmm_shift_angle_table_init_f()
+    op = L.wbfmm_shift_operators_new_f(p, 0, None)
+    L.wbfmm_laplace_coaxial_translate_init_f(p + 2)
+    L.wbfmm_tree_laplace_leaf_expansions_f(t, P(q), 1, None, 1, 1, None)
+    for lv in range(depth, 2, -1):
+        L.wbfmm_laplace_upward_pass_f(t, op, lv, None)
+    for lv in range(2, depth + 1):
+        L.wbfmm_laplace_downward_pass_ref_f(t, op, lv, None, 0)
+    tl = L.wbfmm_target_list_new_f(t, nt)
+    L.wbfmm_target_list_add_points_f(tl, P(xt), 12, nt)
+    L.wbfmm_target_list_coefficients_init(tl, 3)
+    L.wbfmm_laplace_target_list_local_coefficients_f(tl, 1, None)
+    ff = np.zeros((nt, 4), np.float32)
+    L.wbfmm_target_list_local_field_f(tl, P(q), 1, None, 1, P(ff), 4)
+    # one target through the box field, potential only
+    fb = np.zeros(1, np.float32)
+    x5 = np.ascontiguousarray(xt[5])
+    L.wbfmm_laplace_box_field_f(t, depth, 0, P(q), 1, None, 1, 1, 1, P(x5), P(fb), 1, None)
+    # direct sum at the same target
+    fdr = np.zeros(3, np.float32)
+    L.wbfmm_laplace_field_direct_f(P(xs), 3, None, 3, n, P(q), 1, None, 1, 1, 2, P(x5),
+                                   P(fdr), 3)
+    wb.sync()
+    L.wbfmm_target_list_free(tl)
+    lib = wb.lib()
+    lib.wbfmm_shift_operators_free(op)
+    lib.wbfmm_tree_free(t)
+
+    f64 = solve(wb, xs.astype(np.float64), q.astype(np.float64), xt.astype(np.float64),
+                x0.astype(np.float64), float(D), depth, p, 32)
+    assert np.array_equal(ff, f64.astype(np.float32))
+    f1 = solve(wb, xs.astype(np.float64), q.astype(np.float64), xt[5:6].astype(np.float64),
+               x0.astype(np.float64), float(D), depth, p, 32)
+    assert abs(fb[0] - f1[0, 0]) <= 1e-6 * abs(f1[0, 0])
+    fd = oracle.lap_direct(xs.astype(np.float64), q.astype(np.float64), x5[None].astype(np.float64),
+                           2)
+    assert np.allclose(fdr, fd[0, 1:], rtol=1e-6, atol=0)

@@ -463,15 +463,25 @@ static void mirror_level(tree_dev *td, unsigned l)
   free(e);
 }
 
+static gint add_points_dbl(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer normals,
+                           gsize nstr, guint npts, gboolean sorted);
+
 gint wbfmm_tree_add_points(wbfmm_tree_t *t, gpointer pts, gsize pstr,
                            gpointer normals, gsize nstr, guint npts,
                            gboolean sorted)
 {
+  if (t->size != sizeof(double))
+    WBFMM_FATAL("mixed precision not implemented");
+  return add_points_dbl(t, pts, pstr, normals, nstr, npts, sorted);
+}
+
+/* double positions (host or device, byte strides) into the device tree */
+static gint add_points_dbl(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer normals,
+                           gsize nstr, guint npts, gboolean sorted)
+{
   tree_dev *td = TD(t);
   unsigned nbad = 0;
   double *x0 = (double *)wbfmm_tree_origin(t);
-  if (t->size != sizeof(double))
-    WBFMM_FATAL("mixed precision not implemented");
   if (npts > t->maxpoints)
     WBFMM_FATAL("too many points (%u) for tree (%u)", npts, t->maxpoints);
   /* a new point set starts a new tree: levels are rebuilt by refine and
@@ -2177,4 +2187,223 @@ gint wbfmm_total_field_grad(gdouble k, gdouble *xs, gint xstride, gdouble *src,
   (void)fstr;
   return hel_total(k, xs, xstride, src, sstride, normals, nstr, dipoles, dstr, nq, nsrc,
                    xf, field, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Single-precision entry points of the Laplace tree pipeline, the      */
+/* target lists and the direct sum (src/wbfmm.h:432-440, :690-740,      */
+/* :996-1112).  A tree made by wbfmm_tree_new_f is an FP32 tree (as     */
+/* wbfmm_tree_set_precision(t, 32)): expansions in FP32 scaled by the   */
+/* box width and FP32 matrix-core translations, orders 2..15 and the    */
+/* same order at every level, with FP64 recurrences and near field.     */
+/* Float arguments are host arrays, widened to double on the host;      */
+/* float outputs are accumulated in float.  The tree's origin is held   */
+/* in double.                                                           */
+/* ------------------------------------------------------------------ */
+
+/* n items of ncomp floats, item i at x + i * stride floats, into n *
+   ncomp contiguous doubles (NULL for NULL) */
+static double *widen(const float *x, size_t stride, int ncomp, size_t n)
+{
+  double *y;
+  size_t i;
+  int c;
+  if (!x) return NULL;
+  if (wbfmm_hip_is_device(x)) WBFMM_FATAL("the single-precision entry points take host arrays");
+  y = malloc(sizeof(double) * ncomp * (n ? n : 1));
+  if (!y) WBFMM_FATAL("out of host memory");
+  for (i = 0; i < n; i++)
+    for (c = 0; c < ncomp; c++) y[i * ncomp + c] = x[i * stride + c];
+  return y;
+}
+
+static void require_f(wbfmm_tree_t *t)
+{
+  if (t->size != sizeof(gfloat)) WBFMM_FATAL("not a single-precision tree (wbfmm_tree_new_f)");
+}
+
+wbfmm_tree_t *wbfmm_tree_new_f(gfloat *x, gfloat D, guint maxpoints)
+{
+  double x0[3] = {x[0], x[1], x[2]};
+  wbfmm_tree_t *t = wbfmm_tree_new(x0, (double)D, maxpoints);
+  TD(t)->prec32 = 1;
+  TD(t)->host_mirror = 0;
+  t->size = sizeof(gfloat);
+  return t;
+}
+
+gint wbfmm_tree_add_points_f(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer normals,
+                             gsize nstr, guint npts, gboolean sorted)
+{
+  double *xd, *nd;
+  require_f(t);
+  if (pstr % sizeof(gfloat) || nstr % sizeof(gfloat))
+    WBFMM_FATAL("point strides must be whole floats");
+  xd = widen((const float *)pts, pstr / sizeof(gfloat), 3, npts);
+  nd = widen((const float *)normals, nstr / sizeof(gfloat), 3, npts);
+  add_points_dbl(t, xd, 3 * sizeof(double), nd, 3 * sizeof(double), npts, sorted);
+  HIPOK(wbfmm_hip_sync());
+  free(xd);
+  free(nd);
+  t->points = (char *)pts;
+  t->pstr = pstr;
+  t->normals = (char *)normals;
+  t->nstr = nstr;
+  return 0;
+}
+
+gint wbfmm_tree_refine_f(wbfmm_tree_t *t)
+{
+  require_f(t);
+  return wbfmm_tree_refine(t);
+}
+
+gint wbfmm_tree_laplace_coefficient_init_f(wbfmm_tree_t *t, guint l, guint nr, guint ns)
+{
+  require_f(t);
+  return wbfmm_tree_laplace_coefficient_init(t, l, nr, ns);
+}
+
+gint wbfmm_tree_laplace_leaf_expansions_f(wbfmm_tree_t *t, gfloat *src, gint sstr,
+                                          gfloat *dipoles, gint dstr,
+                                          gboolean zero_expansions, gfloat *work)
+{
+  double *q, *d;
+  (void)work;
+  require_f(t);
+  q = widen(src, sstr, 1, t->npoints);
+  d = widen(dipoles, dstr, 1, t->npoints);
+  wbfmm_tree_laplace_leaf_expansions(t, q, 1, d, 1, zero_expansions, NULL);
+  HIPOK(wbfmm_hip_sync());
+  free(q);
+  free(d);
+  return 0;
+}
+
+gint wbfmm_shift_angle_table_init_f(void) { return 0; }
+
+wbfmm_shift_operators_t *wbfmm_shift_operators_new_f(guint L, gboolean bw, gfloat *work)
+{
+  (void)work;
+  return wbfmm_shift_operators_new(L, bw, NULL);
+}
+
+gint wbfmm_laplace_coaxial_translate_init_f(gint N)
+{
+  return wbfmm_laplace_coaxial_translate_init(N);
+}
+
+gint wbfmm_laplace_upward_pass_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                                 gfloat *work)
+{
+  (void)work;
+  require_f(t);
+  return wbfmm_laplace_upward_pass(t, op, level, NULL);
+}
+
+gint wbfmm_laplace_downward_pass_ref_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
+                                       guint level, gfloat *work, gint nthreads)
+{
+  (void)work;
+  require_f(t);
+  return wbfmm_laplace_downward_pass_ref(t, op, level, NULL, nthreads);
+}
+
+/* values box_field writes at unit stride: 1 (potential) or 3 (gradient) */
+static int lap_field_count(guint field)
+{
+  return (field & WBFMM_FIELD_GRADIENT) ? 3 : 1;
+}
+
+gint wbfmm_laplace_box_field_f(wbfmm_tree_t *t, guint level, guint b, gfloat *src, gint sstr,
+                               gfloat *d, gint dstr, guint field, gboolean eval_neighbours,
+                               gfloat *x, gfloat *f, gint fstr, gfloat *work)
+{
+  double *q, *dd, xd[3] = {x[0], x[1], x[2]}, tmp[4] = {0, 0, 0, 0};
+  int k;
+  (void)work;
+  require_f(t);
+  q = eval_neighbours ? widen(src, sstr, 1, t->npoints) : NULL;
+  dd = eval_neighbours ? widen(d, dstr, 1, t->npoints) : NULL;
+  wbfmm_laplace_box_field(t, level, b, q, 1, dd, 1, field, eval_neighbours, xd, tmp, fstr,
+                          NULL);
+  HIPOK(wbfmm_hip_sync());
+  free(q);
+  free(dd);
+  for (k = 0; k < lap_field_count(field); k++) f[k] += (gfloat)tmp[k];
+  return 0;
+}
+
+gint wbfmm_laplace_field_direct_f(gfloat *xs, gint xstride, gfloat *n, gint nstr, gint nsrc,
+                                  gfloat *src, gint sstr, gfloat *d, gint dstr, gint nq,
+                                  guint field, gfloat *xf, gfloat *f, gint fstr)
+{
+  double *xd, *nd, *q, *dd, xfd[3] = {xf[0], xf[1], xf[2]}, tmp[4] = {0, 0, 0, 0};
+  int k;
+  xd = widen(xs, xstride, 3, nsrc);
+  nd = widen(n, nstr, 3, nsrc);
+  q = widen(src, sstr, 1, nsrc);
+  dd = widen(d, dstr, 1, nsrc);
+  wbfmm_laplace_field_direct(xd, 3, nd, 3, nsrc, q, 1, dd, 1, nq, field, xfd, tmp, fstr);
+  HIPOK(wbfmm_hip_sync());
+  free(xd);
+  free(nd);
+  free(q);
+  free(dd);
+  for (k = 0; k < lap_field_count(field); k++) f[k] += (gfloat)tmp[k];
+  return 0;
+}
+
+wbfmm_target_list_t *wbfmm_target_list_new_f(wbfmm_tree_t *t, guint npts)
+{
+  wbfmm_target_list_t *l;
+  require_f(t);
+  l = wbfmm_target_list_new(t, npts);
+  l->size = sizeof(gfloat);
+  return l;
+}
+
+gint wbfmm_target_list_add_points_f(wbfmm_target_list_t *l, gpointer pts, gsize pstr,
+                                    guint npts)
+{
+  double *xd;
+  if (pstr % sizeof(gfloat)) WBFMM_FATAL("point stride must be whole floats");
+  xd = widen((const float *)pts, pstr / sizeof(gfloat), 3, npts);
+  wbfmm_target_list_add_points(l, xd, 3 * sizeof(double), npts);
+  HIPOK(wbfmm_hip_sync());
+  free(xd);
+  l->points = (char *)pts;
+  l->pstr = pstr;
+  return 0;
+}
+
+gint wbfmm_laplace_target_list_local_coefficients_f(wbfmm_target_list_t *l, guint source,
+                                                    gfloat *work)
+{
+  (void)work;
+  return wbfmm_laplace_target_list_local_coefficients(l, source, NULL);
+}
+
+gint wbfmm_target_list_local_field_f(wbfmm_target_list_t *l, gfloat *src, gint sstr,
+                                     gfloat *d, gint dstr, gfloat *f, gint fstr)
+{
+  size_t nt = l->npoints, i;
+  int fl = (int)(l->field & 3), nv, k;
+  double *q, *dd, *fd;
+  if (l->t->problem != WBFMM_PROBLEM_LAPLACE)
+    WBFMM_FATAL("single-precision target lists serve Laplace trees");
+  nv = ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0);
+  if (fstr < nv) WBFMM_FATAL("field stride %d too small", fstr);
+  q = widen(src, sstr, 1, l->t->npoints);
+  dd = widen(d, dstr, 1, l->t->npoints);
+  fd = calloc((nt ? nt : 1) * nv, sizeof(double));
+  if (!fd) WBFMM_FATAL("out of host memory");
+  wbfmm_target_list_local_field(l, q, 1, dd, 1, fd, nv);
+  HIPOK(wbfmm_hip_sync());
+  for (i = 0; i < nt; i++)
+    for (k = 0; k < nv; k++) f[i * fstr + k] += (gfloat)fd[i * nv + k];
+  free(q);
+  free(dd);
+  free(fd);
+  return 0;
 }
