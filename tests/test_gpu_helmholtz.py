@@ -214,11 +214,11 @@ def test_hel_target_list_matches_field_targets(wb, oracle):
     assert np.abs(g - f).max() < 1e-12 * np.abs(f).max()
 
 
-@pytest.mark.parametrize("p", [2, 5, 13, 16, 20, 23, 24, 25])
+@pytest.mark.parametrize("p", [2, 5, 13, 16, 20, 23, 24, 25, 27, 28])
 def test_hel_m2l_mfma_orders_match_oracle(wb, oracle, p):
-    # the matrix-core M2L over the orders it serves (2..24: four wavefronts
-    # per block below 16, eight from 16) and the vector M2L above (25); a
-    # second solve gives the same bits
+    # the matrix-core M2L, M2M and L2L over the orders they serve (2..27:
+    # four wavefronts per block below 16, eight from 16) and the vector
+    # kernels above (28); a second solve gives the same bits
     rng = np.random.default_rng(40 + p)
     xs = rng.uniform(-1, 1, (1500, 3))
     q = rng.normal(size=1500) + 1j * rng.normal(size=1500)

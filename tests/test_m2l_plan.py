@@ -100,7 +100,7 @@ def test_plan_matches_oracle_m2l(wb, oracle, N):
 def test_orders_beyond_the_plan_are_refused(wb):
     import ctypes
     P = wb.M2LPlan()
-    assert wb.lib().wbfmm_hip_lap_m2l_tables(25, ctypes.byref(P), None, None, None) != 0
+    assert wb.lib().wbfmm_hip_lap_m2l_tables(28, ctypes.byref(P), None, None, None) != 0
 
 
 def test_theta_table_has_49_angles(wb):
@@ -327,7 +327,7 @@ def emulate_hel(P, A, A2, CS, M, o, N):
     return emulate_hel_core(P, A[P.th[o]], A2[P.rix[o], 0], A2[P.rix[o], 1], P.chi[o], M, N)
 
 
-@pytest.mark.parametrize("N,L", [(2, 2), (5, 7), (9, 9), (13, 13), (16, 18), (20, 20)])
+@pytest.mark.parametrize("N,L", [(2, 2), (5, 7), (9, 9), (13, 13), (16, 18), (20, 20), (27, 27)])
 def test_hel_plan_matches_oracle_m2l(wb, oracle, N, L):
     # the Helmholtz M2L tiles over the Laplace plan of the same order
     # against the oracle's rotate / coaxial S|R / rotate back
@@ -362,7 +362,7 @@ def child_angles(c, sgn):
     return math.acos(ez / math.sqrt(ex * ex + ey * ey + ez * ez)), -0.5 * math.pi, math.atan2(ey, ex)
 
 
-@pytest.mark.parametrize("N", [2, 7, 12, 16, 23])
+@pytest.mark.parametrize("N", [2, 7, 12, 16, 23, 26])
 def test_hel_shift_tiles_match_oracle(wb, oracle, N):
     # Helmholtz M2M (child -> parent) and L2L (parent -> child): the rotation
     # tiles at the two child polar angles, the real R|R coaxial tiles at

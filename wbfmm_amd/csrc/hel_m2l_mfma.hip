@@ -725,7 +725,7 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const un
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
   if (level < 2 || level > 20) return WBFMM_HIP_ERR_ARG;
-  if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
+  if (N < 2 || N > 27) return WBFMM_HIP_ERR_ORDER;
   if (!plan_host || plan_host->N != N) return WBFMM_HIP_ERR_ARG;
   {
     const int rz = ensure_zero();
@@ -745,7 +745,7 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const un
     break;
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
     CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
-    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24) CASE(25) CASE(26) CASE(27)
 #undef CASE
     default:
       break;
@@ -762,7 +762,7 @@ int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
   if (npar == 0) return 0;
   if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
   if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
-  if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
+  if (N < 2 || N > 27) return WBFMM_HIP_ERR_ORDER;
   if (!plan_host || plan_host->N != N) return WBFMM_HIP_ERR_ARG;
   {
     const int rz = ensure_zero();
@@ -775,7 +775,7 @@ int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
                                A, A2, CS, zero_expansion(), plan_host);
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
     CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
-    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24)
+    CASE(20) CASE(21) CASE(22) CASE(23) CASE(24) CASE(25) CASE(26) CASE(27)
 #undef CASE
     default:
       return WBFMM_HIP_ERR_ORDER;

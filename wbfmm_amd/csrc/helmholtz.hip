@@ -732,6 +732,7 @@ int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
                tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,  \
                level, x0[0], x0[1], x0[2], D, N, k, (const double2 *)mpr, field, f, fstr)
   if (N <= 16) HF(16);
+  else if (N <= 24) HF(24);
   else if (N <= 32) HF(32);
   else if (N <= WBFMM_HIP_MAX_ORDER) HF(WBFMM_HIP_MAX_ORDER);
   else return WBFMM_HIP_ERR_ORDER;

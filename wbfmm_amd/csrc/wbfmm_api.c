@@ -140,6 +140,8 @@ typedef struct {
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
 #define M2L_MFMA_MIN 2
 #define M2L_MFMA_MAX 24
+/* orders served by the Helmholtz matrix-core M2L, M2M and L2L */
+#define HEL_MFMA_MAX 27
 /* orders served by the FP32 passes (the FP64 matrix-core M2M / L2L serve
    M2L_MFMA_MIN..M2L_MFMA_MAX) */
 #define SHIFT_MFMA_MAX 15
@@ -1827,7 +1829,7 @@ gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level
 }
 
 /* the matrix-core Helmholtz M2L is used for equal orders in
-   M2L_MFMA_MIN..M2L_MFMA_MAX unless WBFMM_HEL_M2L=vector asks for the
+   M2L_MFMA_MIN..HEL_MFMA_MAX unless WBFMM_HEL_M2L=vector asks for the
    vector kernel (kept for comparison; the results agree to rounding) */
 static int hel_mfma_enabled(void)
 {
@@ -1932,11 +1934,11 @@ static void ensure_hel_shift_coax_tiles(ops_dev *od, guint level, int N)
   free(h);
 }
 
-/* the matrix-core M2M / L2L serves equal orders in M2L_MFMA_MIN..M2L_MFMA_MAX */
+/* the matrix-core M2M / L2L serves equal orders in M2L_MFMA_MIN..HEL_MFMA_MAX */
 static int hel_shift_mfma(ops_dev *od, guint Nc, guint Np)
 {
   (void)od;
-  return Nc == Np && (int)Nc >= M2L_MFMA_MIN && (int)Nc <= M2L_MFMA_MAX && hel_mfma_enabled();
+  return Nc == Np && (int)Nc >= M2L_MFMA_MIN && (int)Nc <= HEL_MFMA_MAX && hel_mfma_enabled();
 }
 
 static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level)
@@ -1962,7 +1964,7 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
   } else {
     tl = all_boxes(level, &nt);
   }
-  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= M2L_MFMA_MAX && hel_mfma_enabled()) {
+  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= HEL_MFMA_MAX && hel_mfma_enabled()) {
     /* matrix-core form: targets grouped by Morton child digit */
     int N = (int)Ns;
     unsigned cstart[8], ccount[8];

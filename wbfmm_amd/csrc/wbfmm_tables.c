@@ -579,7 +579,7 @@ int wbfmm_hip_lap_m2l_tables(int N, wbfmm_m2l_plan_t *P, double *A1,
   int b, n, nu, m, part, i, dx, dy, dz, S;
   double *T, ths[343];
   hcache hc;
-  if (N < 1 || N > 24) return WBFMM_HIP_ERR_ORDER;
+  if (N < 1 || N > 27) return WBFMM_HIP_ERR_ORDER;
   memset(P, 0, sizeof(*P));
   P->N = N;
   P->C = (N + 1) * (N + 1);
