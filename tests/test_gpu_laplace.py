@@ -287,3 +287,24 @@ def test_m2l_mfma_orders_match_oracle(wb, oracle, p):
     fg = gpu_fmm(wb, xs, q, xs, x0, D, 3, p)
     for c in range(4):
         assert rel(fg[:, c], fo[:, c]) < PARITY, (p, c, rel(fg[:, c], fo[:, c]))
+
+
+def test_packed_sparse_level_m2l(wb, oracle, monkeypatch):
+    # at a level where about 40 % of the boxes are occupied the M2L packs,
+    # per offset, the targets whose source exists into 16-column tiles;
+    # forced off and on, the two kernels agree to rounding
+    rng = np.random.default_rng(77)
+    n = 200_000
+    xs = rng.uniform(0, 1, (n, 3))
+    q = rng.normal(size=n)
+    x0, D = fit_box(xs)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("WBFMM_M2L_PACKED", mode)
+        out[mode] = gpu_fmm(wb, xs, q, xs, x0, D, 6, 10)
+    monkeypatch.delenv("WBFMM_M2L_PACKED")
+    for c in range(4):
+        assert rel(out["1"][:, c], out["0"][:, c]) < 1e-12, c
+    idx = np.arange(0, n, n // 150)
+    fd = oracle.lap_direct(xs, q, xs[idx], 3, cutoff=1e-6)
+    assert np.linalg.norm(out["1"][idx, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0]) < 1e-5

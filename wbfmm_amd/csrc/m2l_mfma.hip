@@ -1059,6 +1059,245 @@ static int m2l_waves() {
   return w;
 }
 
+// ---------------------------------------------------------------------
+// Sparse levels (FP64, orders up to 12).  A 16-target chunk runs every
+// offset that any of its targets uses; at 40 % box occupancy nearly every
+// offset is used by some target of a chunk while each target has a source
+// at only 40 % of them, so most columns of each product hold no source.
+// Here a block keeps the local expansions of 32 targets in LDS and, for
+// each offset, packs the targets whose source box is occupied into
+// 16-column tiles (usually one): the products run on occupied pairs only.
+// Each target's expansion is the sum of its translations in increasing
+// offset order, as in the chunked kernel; the results differ from it by
+// rounding only, and do not depend on the launch.
+// ---------------------------------------------------------------------
+constexpr int kPackT = 32;
+
+template <int N>
+struct PackedFits {
+  static constexpr bool value =
+      N <= 12 && (M2LShape<N>::S + 1) * 16 * 8 + kPackT * M2LShape<N>::C * 8 <= 80 * 1024;
+};
+
+template <int N, int W, int WI>
+__device__ __forceinline__ void m2l_packed_loop(
+    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    double2 *cs, unsigned (*vm)[7], const unsigned *anym, int *list, int *ncolp,
+    const int4 *tco, double *acc, int lane, int px, int py, int pz,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero,
+    const double *__restrict__ mps) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
+  constexpr int NBW = WS.nb;
+
+  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + W * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  for (int k = 0; k < 216; k++) {
+    if (!((anym[k >> 5] >> (k & 31)) & 1u)) continue;  // the same in every lane
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
+    const int ri = __builtin_amdgcn_readfirstlane(P->rix[o]);
+    // wavefront 0: the offset's phases and its packed target list
+    if (WI == 0) {
+      if (lane <= N) {
+        const double *t = CS + ((size_t)o * (N + 1) + lane) * 2;
+        cs[lane] = make_double2(t[0], t[1]);
+      }
+      const bool v = lane < kPackT && ((vm[lane][k >> 5] >> (k & 31)) & 1u);
+      const uint64_t m = __ballot(v);
+      if (v) list[__popcll(m & ((1ull << lane) - 1ull))] = lane;
+      if (lane == 0) *ncolp = __popcll(m);
+    }
+    __syncthreads();
+    const int ncol = __builtin_amdgcn_readfirstlane(*ncolp);
+    for (int t0 = 0; t0 < ncol; t0 += 16) {
+      int z0;
+      asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+      const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+      const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+      const int sl = t0 + cl;
+      const bool cv = sl < ncol;
+      const int j = cv ? list[sl] : 0;
+      const double *src = zero;
+      if (cv) {
+        const int4 tc = tco[j];
+        const uint64_t sb =
+            spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
+        src = mps + sb * (uint64_t)C;
+      }
+      // Zin: one writer per LDS slot (lanes past the last unit write nothing)
+#pragma unroll
+      for (int i0 = 0; i0 < NAW; i0 += 8) {
+        double pre_re[8], pre_im[8];
+#pragma unroll
+        for (int i = i0; i < i0 + 8 && i < NAW; i++) {
+          const int u = min(ub + 4 * W * i, NU - 1);
+          const double *q = src + tu[u].x;
+          pre_re[i - i0] = q[0];
+          pre_im[i - i0] = q[1];
+        }
+#pragma unroll
+        for (int i = i0; i < i0 + 8 && i < NAW; i++) {
+          if (ub + 4 * W * i >= NU) continue;
+          const int4 inf = tu[ub + 4 * W * i];
+          const double2 e = cs[inf.y];
+          R[inf.z * 16 + cl] = e.x * pre_re[i - i0] - e.y * pre_im[i - i0];
+          R[inf.w * 16 + cl] = e.y * pre_re[i - i0] + e.x * pre_im[i - i0];
+        }
+      }
+      __syncthreads();
+      d4 y[NBW > 0 ? NBW : 1];
+      stage_w<double, N, W, WI>(A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t1, y, gp, cl);
+      __syncthreads();
+      stage_w<double, N, W, WI>(A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t2, y, gp, cl);
+      __syncthreads();
+      stage_w<double, N, W, WI>(A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t3, y, gp, cl);
+      __syncthreads();
+      // Zout into the target's expansion in LDS (each (target, unit) has one
+      // writer per offset; offsets are separated by barriers)
+      if (cv) {
+        double *a = acc + j * C;
+#pragma unroll
+        for (int i = 0; i < NAW; i++) {
+          const int u = ub + 4 * W * i;
+          if (u >= NU) continue;
+          const int4 inf = tu[u];
+          const double2 e = cs[inf.y];
+          const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+          a[inf.x] += e.x * tr + e.y * tq;
+          if (inf.y) a[inf.x + 1] += e.x * tq - e.y * tr;
+        }
+      }
+      __syncthreads();  // R, cs and the list are rewritten next
+    }
+  }
+}
+
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
+    const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ,
+    int level, const double *__restrict__ mps, double *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S, C = SH::C;
+  __shared__ double R[(S + 1) * 16];
+  __shared__ double acc[kPackT * C];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[N + 1];
+  __shared__ unsigned vm[kPackT][7];
+  __shared__ unsigned anym[7];
+  __shared__ int list[kPackT];
+  __shared__ int ncol;
+  __shared__ int4 tco[kPackT];
+  __shared__ uint32_t tbox[kPackT];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < kPackT * C; i += 64 * W) acc[i] = 0.0;
+  for (int i = tid; i < kPackT * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
+
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x, q8 = nch / 8;
+  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+  int c = 0;
+  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  if (tid < kPackT) {
+    const unsigned kk = (chunk - ca.chunk_prefix[c]) * kPackT + tid;
+    const bool tv = kk < ca.count[c];
+    const uint32_t tb = tv ? cls[ca.start[c] + kk] : 0u;
+    tbox[tid] = tb;
+    tco[tid] = make_int4(compact3(tb), compact3(tb >> 1), compact3(tb >> 2), tv ? 1 : 0);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kPackT * 216; idx += 64 * W) {
+    const int j = idx % kPackT, k = idx / kPackT;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int4 tc = tco[j];
+    const int si = tc.x + dx, sj = tc.y + dy, sk = tc.z + dz;
+    if (!tc.w || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox) continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[j][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  if (tid < 7) {
+    unsigned a = 0;
+    for (int j = 0; j < kPackT; j++) a |= vm[j][tid];
+    anym[tid] = a;
+  }
+  __syncthreads();
+
+#define PACKED(WI)                                                                          \
+  m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, list, &ncol, tco,   \
+                            acc, lane, px, py, pz, P, A1, A3, A2, CS, zero, mps)
+  static_assert(W == 4, "four wavefronts per block");
+  switch (wv) {
+    case 0: PACKED(0); break;
+    case 1: PACKED(1); break;
+    case 2: PACKED(2); break;
+    default: PACKED(3); break;
+  }
+#undef PACKED
+  __syncthreads();
+  for (int i = tid; i < kPackT * C; i += 64 * W) {
+    const int j = i / C;
+    if (tco[j].w) mpr[(size_t)tbox[j] * C + (i - j * C)] += acc[i];
+  }
+}
+
+// the packed kernel for levels where fewer than 60 % of the boxes hold
+// targets; WBFMM_M2L_PACKED=0 or 1 forces the choice (read at each call,
+// for comparison)
+static int m2l_packed_choice() {
+  const char *e = getenv("WBFMM_M2L_PACKED");
+  return e ? atoi(e) : -1;
+}
+
 template <typename T, int N>
 static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
                       const uint32_t *occ, int level, const T *mps,
@@ -1076,6 +1315,24 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   constexpr int C = (N + 1) * (N + 1);
   unsigned ntot = 0;
   for (int c = 0; c < 8; c++) ntot += ca.count[c];
+  if constexpr (sizeof(T) == 8 && PackedFits<N>::value) {
+    const int pc = m2l_packed_choice();
+    const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
+    if (nchunks >= 1024 && (pc == 1 || (pc == -1 && fill < 0.6))) {
+      ClassArgs cp = ca;
+      unsigned nch = 0;
+      for (int c = 0; c < 8; c++) {
+        cp.chunk_prefix[c] = nch;
+        nch += (ca.count[c] + kPackT - 1) / kPackT;
+      }
+      cp.chunk_prefix[8] = nch;
+      WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 4>), dim3(nch), dim3(256), 0, cls, cp, occ,
+                   level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
+                   (const double *)A1, (const double *)A3, (const double *)A2,
+                   (const double *)CS, (const double *)zero);
+      return 0;
+    }
+  }
   const int nsplit = nchunks >= 1024 ? 1 : (int)(2048 / nchunks < 8 ? 2048 / nchunks : 8);
   T *part = nullptr;
   if (nsplit > 1 &&
