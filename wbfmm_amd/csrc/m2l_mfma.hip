@@ -1082,8 +1082,8 @@ struct PackedFits {
 template <int N, int W, int WI>
 __device__ __forceinline__ void m2l_packed_loop(
     double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
-    double2 *cs, unsigned (*vm)[7], const unsigned *anym, int *list, int *ncolp,
-    const int4 *tco, double *acc, int lane, int px, int py, int pz,
+    double2 (*cs)[N + 1], const unsigned *anym, const uint8_t (*lists)[kPackT],
+    const uint8_t *ncols, const int4 *tco, double *acc, int lane, int px, int py, int pz,
     const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
     const double *__restrict__ A3, const double *__restrict__ A2,
     const double *__restrict__ CS, const double *__restrict__ zero,
@@ -1093,7 +1093,60 @@ __device__ __forceinline__ void m2l_packed_loop(
   constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
   constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
   constexpr int NBW = WS.nb;
+  const int col = lane & 15, ubase = (lane >> 4) + 4 * WI;
 
+  auto offset_of = [&](int k) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+  };
+  // the tiles run in offset order, and within an offset in list order
+  auto next_tile = [&](int k, int t0, int &kn, int &tn) {
+    if (k >= 0 && k < 216 && t0 + 16 < ncols[k]) {
+      kn = k;
+      tn = t0 + 16;
+      return;
+    }
+    for (k++; k < 216; k++)
+      if ((anym[k >> 5] >> (k & 31)) & 1u) break;
+    kn = k;
+    tn = 0;
+  };
+  // every wavefront writes the offset's phases (the same values)
+  auto source_of = [&](int k, int t0, int p, int &j) -> const double * {
+    if (lane <= N) {
+      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = make_double2(t[0], t[1]);
+    }
+    const int sl = t0 + col;
+    j = sl < ncols[k] ? lists[k][sl] : -1;
+    if (j < 0) return zero;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int4 tc = tco[j];
+    const uint64_t sb =
+        spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
+    return mps + sb * (uint64_t)C;
+  };
+  double pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const double *src, const int4 *tu, int ub) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const double *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  // one writer per LDS slot (lanes past the last unit write nothing)
+  auto deposit = [&](const int4 *tu, int ub, int cl, int p) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      if (ub + 4 * W * i >= NU) continue;
+      const int4 inf = tu[ub + 4 * W * i];
+      const double2 e = cs[p][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
   auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
 #pragma unroll
     for (int lb = 0; lb < NBW; lb++) {
@@ -1105,91 +1158,70 @@ __device__ __forceinline__ void m2l_packed_loop(
     }
   };
 
-  for (int k = 0; k < 216; k++) {
-    if (!((anym[k >> 5] >> (k & 31)) & 1u)) continue;  // the same in every lane
-    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
-    const int o = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
-    const int th = __builtin_amdgcn_readfirstlane(P->th[o]);
-    const int ri = __builtin_amdgcn_readfirstlane(P->rix[o]);
-    // wavefront 0: the offset's phases and its packed target list
-    if (WI == 0) {
-      if (lane <= N) {
-        const double *t = CS + ((size_t)o * (N + 1) + lane) * 2;
-        cs[lane] = make_double2(t[0], t[1]);
-      }
-      const bool v = lane < kPackT && ((vm[lane][k >> 5] >> (k & 31)) & 1u);
-      const uint64_t m = __ballot(v);
-      if (v) list[__popcll(m & ((1ull << lane) - 1ull))] = lane;
-      if (lane == 0) *ncolp = __popcll(m);
+  int k, t0, p = 0, j = -1;
+  next_tile(-1, 0, k, t0);
+  k = __builtin_amdgcn_readfirstlane(k);
+  int th = 0, ri = 0;
+  if (k < 216) {
+    th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
+    ri = __builtin_amdgcn_readfirstlane(P->rix[offset_of(k)]);
+    load_sources(source_of(k, t0, 0, j), uinf, ubase);
+    deposit(uinf, ubase, col, 0);
+  }
+  __syncthreads();
+  while (k < 216) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+    int kn, tn;
+    next_tile(k, t0, kn, tn);
+    kn = __builtin_amdgcn_readfirstlane(kn);
+    tn = __builtin_amdgcn_readfirstlane(tn);
+    int th_n = 0, ri_n = 0, jn = -1;
+    if (kn < 216) {
+      th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
+      ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
     }
+
+    d4 y[NBW > 0 ? NBW : 1];
+    stage_w<double, N, W, WI>(A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
     __syncthreads();
-    const int ncol = __builtin_amdgcn_readfirstlane(*ncolp);
-    for (int t0 = 0; t0 < ncol; t0 += 16) {
-      int z0;
-      asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
-      const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
-      const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
-      const int sl = t0 + cl;
-      const bool cv = sl < ncol;
-      const int j = cv ? list[sl] : 0;
-      const double *src = zero;
-      if (cv) {
-        const int4 tc = tco[j];
-        const uint64_t sb =
-            spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
-        src = mps + sb * (uint64_t)C;
+    write_out(t1, y, gp, cl);
+    __syncthreads();
+    if (kn < 216) load_sources(source_of(kn, tn, p ^ 1, jn), tu, ub);
+    stage_w<double, N, W, WI>(A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t2, y, gp, cl);
+    __syncthreads();
+    stage_w<double, N, W, WI>(A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    __syncthreads();
+    write_out(t3, y, gp, cl);
+    __syncthreads();
+    // Zout into the target's expansion in LDS: each (target, unit) has one
+    // writer per tile, and a target appears once per offset
+    if (j >= 0) {
+      double *a = acc + j * C;
+#pragma unroll
+      for (int i = 0; i < NAW; i++) {
+        const int u = ub + 4 * W * i;
+        if (u >= NU) continue;
+        const int4 inf = tu[u];
+        const double2 e = cs[p][inf.y];
+        const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
+        a[inf.x] += e.x * tr + e.y * tq;
+        if (inf.y) a[inf.x + 1] += e.x * tq - e.y * tr;
       }
-      // Zin: one writer per LDS slot (lanes past the last unit write nothing)
-#pragma unroll
-      for (int i0 = 0; i0 < NAW; i0 += 8) {
-        double pre_re[8], pre_im[8];
-#pragma unroll
-        for (int i = i0; i < i0 + 8 && i < NAW; i++) {
-          const int u = min(ub + 4 * W * i, NU - 1);
-          const double *q = src + tu[u].x;
-          pre_re[i - i0] = q[0];
-          pre_im[i - i0] = q[1];
-        }
-#pragma unroll
-        for (int i = i0; i < i0 + 8 && i < NAW; i++) {
-          if (ub + 4 * W * i >= NU) continue;
-          const int4 inf = tu[ub + 4 * W * i];
-          const double2 e = cs[inf.y];
-          R[inf.z * 16 + cl] = e.x * pre_re[i - i0] - e.y * pre_im[i - i0];
-          R[inf.w * 16 + cl] = e.y * pre_re[i - i0] + e.x * pre_im[i - i0];
-        }
-      }
-      __syncthreads();
-      d4 y[NBW > 0 ? NBW : 1];
-      stage_w<double, N, W, WI>(A1 + (size_t)th * KT * 64, R, ln, gp, cl, y);
-      __syncthreads();
-      write_out(t1, y, gp, cl);
-      __syncthreads();
-      stage_w<double, N, W, WI>(A2 + (size_t)ri * KT * 64, R, ln, gp, cl, y);
-      __syncthreads();
-      write_out(t2, y, gp, cl);
-      __syncthreads();
-      stage_w<double, N, W, WI>(A3 + (size_t)th * KT * 64, R, ln, gp, cl, y);
-      __syncthreads();
-      write_out(t3, y, gp, cl);
-      __syncthreads();
-      // Zout into the target's expansion in LDS (each (target, unit) has one
-      // writer per offset; offsets are separated by barriers)
-      if (cv) {
-        double *a = acc + j * C;
-#pragma unroll
-        for (int i = 0; i < NAW; i++) {
-          const int u = ub + 4 * W * i;
-          if (u >= NU) continue;
-          const int4 inf = tu[u];
-          const double2 e = cs[inf.y];
-          const double tr = R[u * 16 + cl], tq = R[(NU + u) * 16 + cl];
-          a[inf.x] += e.x * tr + e.y * tq;
-          if (inf.y) a[inf.x + 1] += e.x * tq - e.y * tr;
-        }
-      }
-      __syncthreads();  // R, cs and the list are rewritten next
     }
+    __syncthreads();  // every read of R done before the next deposit
+    if (kn < 216) deposit(tu, ub, cl, p ^ 1);
+    __syncthreads();
+    p ^= 1;
+    k = kn;
+    t0 = tn;
+    j = jn;
+    th = th_n;
+    ri = ri_n;
   }
 }
 
@@ -1206,11 +1238,11 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
   __shared__ double acc[kPackT * C];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
-  __shared__ double2 cs[N + 1];
+  __shared__ double2 cs[2][N + 1];
   __shared__ unsigned vm[kPackT][7];
   __shared__ unsigned anym[7];
-  __shared__ int list[kPackT];
-  __shared__ int ncol;
+  __shared__ uint8_t lists[216][kPackT];
+  __shared__ uint8_t ncols[216];
   __shared__ int4 tco[kPackT];
   __shared__ uint32_t tbox[kPackT];
 
@@ -1270,11 +1302,18 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
     for (int j = 0; j < kPackT; j++) a |= vm[j][tid];
     anym[tid] = a;
   }
+  // per offset, the targets with an occupied source, in target order
+  for (int k = tid; k < 216; k += 64 * W) {
+    int n = 0;
+    for (int j = 0; j < kPackT; j++)
+      if ((vm[j][k >> 5] >> (k & 31)) & 1u) lists[k][n++] = (uint8_t)j;
+    ncols[k] = (uint8_t)n;
+  }
   __syncthreads();
 
 #define PACKED(WI)                                                                          \
-  m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, list, &ncol, tco,   \
-                            acc, lane, px, py, pz, P, A1, A3, A2, CS, zero, mps)
+  m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, anym, lists, ncols, tco, acc, \
+                            lane, px, py, pz, P, A1, A3, A2, CS, zero, mps)
   static_assert(W == 4, "four wavefronts per block");
   switch (wv) {
     case 0: PACKED(0); break;
