@@ -786,9 +786,10 @@ __global__ __launch_bounds__(64) void k_lap_shift_mfma(
 }
 
 // ---------------------------------------------------------------------
-// M2M / L2L for orders 16..24 (FP64): the blocks of more than 16 rows need
-// more registers than one wavefront holds, so W = 8 wavefronts share the
-// LDS buffer of 16 parents as in the M2L (wavefront w computes the bins
+// M2M / L2L for FP64 orders 8..24: the blocks of more than 16 rows need
+// more registers than one wavefront holds, and below that several
+// wavefronts hide each other's waits, so W wavefronts (4 up to order 15,
+// 8 above) share the LDS buffer of 16 parents as in the M2L (wavefront w computes the bins
 // b = w (mod W) and owns the units u = grp + 4 w + 4 W i).  The
 // directions, tiles and sums are those of k_lap_shift_mfma; the children
 // of a parent are added in child order.
@@ -971,16 +972,25 @@ __global__ __launch_bounds__(64 * W) void k_lap_shift_mfma2(
 #define LAP_SHIFT(WI)                                                                       \
   lap_shift_loop<N, W, WI, MODE>(R, tmap1, tmap2, tmap3, uinf, cs, cmask, any, lane, col,  \
                                  grp, pvalid, p, in, out, A1, A3, A2, zero)
-  static_assert(W == 8, "eight wavefronts per block");
-  switch (wv) {
-    case 0: LAP_SHIFT(0); break;
-    case 1: LAP_SHIFT(1); break;
-    case 2: LAP_SHIFT(2); break;
-    case 3: LAP_SHIFT(3); break;
-    case 4: LAP_SHIFT(4); break;
-    case 5: LAP_SHIFT(5); break;
-    case 6: LAP_SHIFT(6); break;
-    default: LAP_SHIFT(7); break;
+  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: LAP_SHIFT(0); break;
+      case 1: LAP_SHIFT(1); break;
+      case 2: LAP_SHIFT(2); break;
+      default: LAP_SHIFT(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: LAP_SHIFT(0); break;
+      case 1: LAP_SHIFT(1); break;
+      case 2: LAP_SHIFT(2); break;
+      case 3: LAP_SHIFT(3); break;
+      case 4: LAP_SHIFT(4); break;
+      case 5: LAP_SHIFT(5); break;
+      case 6: LAP_SHIFT(6); break;
+      default: LAP_SHIFT(7); break;
+    }
   }
 #undef LAP_SHIFT
 }
@@ -1416,6 +1426,18 @@ static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
+  }
+  // FP64 orders 8..15: four wavefronts per 16 parents (WBFMM_SHIFT_WAVES=1
+  // keeps the one-wavefront kernel, for comparison; the results agree to
+  // rounding)
+  if constexpr (sizeof(T) == 8 && N >= 8 && N <= 15) {
+    const char *e = getenv("WBFMM_SHIFT_WAVES");
+    if (!(e && atoi(e) == 1)) {
+      WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma2<N, 4, MODE>),
+                   dim3((npar + 15) / 16), dim3(256), 0, parents, npar, cstart, cend, cflag, in,
+                   out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+      return 0;
+    }
   }
   if constexpr (N <= 15) {
     WBFMM_LAUNCH(sizeof(T) == 4 ? (MODE == 0 ? "lap_m2m_f32" : "lap_l2l_f32")
