@@ -543,7 +543,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
     tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
     tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
   }
-  for (int u = tid; u < NU; u += 128) {
+  for (int u = tid; u < NU; u += 64 * W) {
     int n = 0;
     while ((n + 1) * (n + 2) / 2 <= u) n++;
     const int m = u - n * (n + 1) / 2;
@@ -592,6 +592,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
       m2l_w2_loop<T, N, 2, 1>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, tb,
                             ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3, A2,
                             CS, zero, mps, mpr, part, ntot);
+  } else if constexpr (W == 8) {
+#define CHUNK8(WI)                                                                            \
+  m2l_w2_loop<T, N, 8, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid, \
+                           tb, ti, tj, tk, px, py, pz, c, k16, split, nsplit, ca, P, A1, A3,  \
+                           A2, CS, zero, mps, mpr, part, ntot)
+    switch (wv) {
+      case 0: CHUNK8(0); break;
+      case 1: CHUNK8(1); break;
+      case 2: CHUNK8(2); break;
+      case 3: CHUNK8(3); break;
+      case 4: CHUNK8(4); break;
+      case 5: CHUNK8(5); break;
+      case 6: CHUNK8(6); break;
+      default: CHUNK8(7); break;
+    }
+#undef CHUNK8
   } else {
     switch (wv) {
       case 0:
@@ -1056,15 +1072,16 @@ __global__ void k_class_count(const uint32_t *list, unsigned n,
 }
 
 
-// wavefronts per 16-target chunk of the M2L kernel: 4 (every order and
-// precision), or 2 / 1 when WBFMM_M2L_WAVES says so (kept for comparison;
-// the results agree to rounding)
+// wavefronts per 16-target chunk of the M2L kernel: 8 for FP64 orders
+// 9..15 and 4 otherwise, or 4 / 2 / 1 when WBFMM_M2L_WAVES says so (kept
+// for comparison; the results agree to rounding).  The sparse-level
+// kernel takes 8 wavefronts per 32 targets unless WBFMM_PACKED_WAVES=4.
 static int m2l_waves() {
   static int w = 0;
   if (!w) {
     const char *e = getenv("WBFMM_M2L_WAVES");
-    w = e ? atoi(e) : 4;
-    if (w != 1 && w != 2) w = 4;
+    w = e ? atoi(e) : 8;
+    if (w != 1 && w != 2 && w != 4) w = 8;
   }
   return w;
 }
@@ -1324,12 +1341,25 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
 #define PACKED(WI)                                                                          \
   m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, anym, lists, ncols, tco, acc, \
                             lane, px, py, pz, P, A1, A3, A2, CS, zero, mps)
-  static_assert(W == 4, "four wavefronts per block");
-  switch (wv) {
-    case 0: PACKED(0); break;
-    case 1: PACKED(1); break;
-    case 2: PACKED(2); break;
-    default: PACKED(3); break;
+  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: PACKED(0); break;
+      case 1: PACKED(1); break;
+      case 2: PACKED(2); break;
+      default: PACKED(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: PACKED(0); break;
+      case 1: PACKED(1); break;
+      case 2: PACKED(2); break;
+      case 3: PACKED(3); break;
+      case 4: PACKED(4); break;
+      case 5: PACKED(5); break;
+      case 6: PACKED(6); break;
+      default: PACKED(7); break;
+    }
   }
 #undef PACKED
   __syncthreads();
@@ -1375,10 +1405,17 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
         nch += (ca.count[c] + kPackT - 1) / kPackT;
       }
       cp.chunk_prefix[8] = nch;
-      WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 4>), dim3(nch), dim3(256), 0, cls, cp, occ,
-                   level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
-                   (const double *)A1, (const double *)A3, (const double *)A2,
-                   (const double *)CS, (const double *)zero);
+      const char *pw = getenv("WBFMM_PACKED_WAVES");
+      if (!(pw && atoi(pw) == 4))
+        WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 8>), dim3(nch), dim3(512), 0, cls, cp, occ,
+                     level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
+                     (const double *)A1, (const double *)A3, (const double *)A2,
+                     (const double *)CS, (const double *)zero);
+      else
+        WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 4>), dim3(nch), dim3(256), 0, cls, cp, occ,
+                     level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
+                     (const double *)A1, (const double *)A3, (const double *)A2,
+                     (const double *)CS, (const double *)zero);
       return 0;
     }
   }
@@ -1387,11 +1424,20 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   if (nsplit > 1 &&
       dmalloc((void **)&part, sizeof(T) * (size_t)nsplit * ntot * C))
     return WBFMM_HIP_ERR_NOMEM;
-  // several wavefronts per chunk: four fit three per SIMD without spilling
-  // at every order; two fit two per SIMD for FP32 and for FP64 up to 12
-  // orders above 15 (blocks spanning several bins) have the four-wavefront
-  // kernel only
-  if (N > 15 || m2l_waves() == 4) {
+  // several wavefronts per chunk: eight for FP64 orders 9..15 (at order 12
+  // the 11 bins of a stage take 2 per wavefront instead of 3); four fit
+  // three per SIMD without spilling at every order; two fit two per SIMD
+  // for FP32 and for FP64 up to 12; orders above 15 (blocks spanning
+  // several bins) have the four-wavefront kernel only
+  if constexpr (sizeof(T) == 8 && N >= 9 && N <= 15) {
+    if (m2l_waves() == 8) {
+      WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_mfma2<T, N, 8>), dim3(nchunks * nsplit), dim3(512), 0,
+                   cls, ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS,
+                   zero, nsplit, part, ntot);
+      goto reduce;
+    }
+  }
+  if (N > 15 || m2l_waves() == 4 || m2l_waves() == 8) {
     WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 4>),
                  dim3(nchunks * nsplit), dim3(256), 0, cls,
                  ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
@@ -1408,6 +1454,7 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
                    ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                    A1, A3, A2, CS, zero, nsplit, part, ntot);
   }
+reduce:
   if (nsplit > 1) {
     const size_t ne = (size_t)ntot * C;
     WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<T>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
