@@ -1474,12 +1474,18 @@ static int launch_shift(const uint32_t *parents, unsigned npar, const uint32_t *
     fprintf(stderr, "wbfmm (HIP): plan for order %d does not match the kernel's bins\n", N);
     return WBFMM_HIP_ERR_ARG;
   }
-  // FP64 orders 8..15: four wavefronts per 16 parents (WBFMM_SHIFT_WAVES=1
-  // keeps the one-wavefront kernel, for comparison; the results agree to
-  // rounding)
+  // FP64 orders 8..15: eight wavefronts per 16 parents (WBFMM_SHIFT_WAVES=4
+  // or 1 selects the four- or one-wavefront kernel, for comparison; the
+  // results agree to rounding)
   if constexpr (sizeof(T) == 8 && N >= 8 && N <= 15) {
     const char *e = getenv("WBFMM_SHIFT_WAVES");
-    if (!(e && atoi(e) == 1)) {
+    if (!(e && (atoi(e) == 4 || atoi(e) == 1))) {
+      WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma2<N, 8, MODE>),
+                   dim3((npar + 15) / 16), dim3(512), 0, parents, npar, cstart, cend, cflag, in,
+                   out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
+      return 0;
+    }
+    if (atoi(e) == 4) {
       WBFMM_LAUNCH(MODE == 0 ? "lap_m2m" : "lap_l2l", (k_lap_shift_mfma2<N, 4, MODE>),
                    dim3((npar + 15) / 16), dim3(256), 0, parents, npar, cstart, cend, cflag, in,
                    out, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS, zero);
