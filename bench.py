@@ -509,8 +509,8 @@ def main():
                          "192 MiB read, 256 MiB written) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not launched)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding a tree over several GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
         run_batched(args)
