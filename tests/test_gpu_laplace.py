@@ -308,3 +308,25 @@ def test_packed_sparse_level_m2l(wb, oracle, monkeypatch):
     idx = np.arange(0, n, n // 150)
     fd = oracle.lap_direct(xs, q, xs[idx], 3, cutoff=1e-6)
     assert np.linalg.norm(out["1"][idx, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0]) < 1e-5
+
+
+def test_packed_sparse_level_separate_targets_dipoles(wb, monkeypatch):
+    # the sparse-level kernel over a target list (boxes holding sources or
+    # targets) with normal dipoles: forced off and on, the same field to
+    # rounding
+    rng = np.random.default_rng(78)
+    n, nt = 150_000, 60_000
+    xs = rng.uniform(0, 1, (n, 3))
+    xt = rng.uniform(0, 1, (nt, 3))
+    q = rng.normal(size=n)
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    d = rng.normal(size=n)
+    x0, D = fit_box(xs, xt)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("WBFMM_M2L_PACKED", mode)
+        out[mode] = gpu_fmm(wb, xs, q, xt, x0, D, 6, 9, nrm=nrm, d=d)
+    monkeypatch.delenv("WBFMM_M2L_PACKED")
+    for c in range(4):
+        assert rel(out["1"][:, c], out["0"][:, c]) < 1e-12, c
