@@ -100,21 +100,23 @@ def m2l_pairs(xs, x0, D, depth, xt=None):
     return pairs
 
 
-def cpu_baseline(p, seconds_hint=20.0):
-    """Oracle (CPU restatement of the reference path) on a bounded sample
-    with the same leaf density: 1e6/64 sources, depth 5, same order, M2L
-    threaded over the host cores like the reference's downward pass."""
+def cpu_baseline(p):
+    """Oracle (CPU restatement of the reference path, not the reference's
+    own AVX build, which cannot be built here) on a bounded sample with the
+    same leaf density: 1e6/8 sources, depth 6, same order, M2L threaded
+    over the host cores like the reference's downward pass."""
     import oracle
-    n = 1_000_000 // 64
+    n = 1_000_000 // 8
     xs, q, x0, D = workload(n, seed=3)
     nth = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    f, ph, _ = oracle.fmm(1, xs, xs, x0, D, 5, [p] * 17, [p] * 17, q=q, field=3, nthreads=nth)
+    f, ph, _ = oracle.fmm(1, xs, xs, x0, D, 6, [p] * 17, [p] * 17, q=q, field=3, nthreads=nth)
     el = time.perf_counter() - t0
     return {"value": n / el, "unit": "targets/s", "cores": nth, "kind": "port",
-            "sample": f"oracle FMM, {n} uniform sources = targets, depth 5 (same leaf "
-                      f"density as 1e6 at depth 7), order {p}, potential+gradient, "
-                      f"{el:.1f} s wall, M2L on {nth} threads"}
+            "sample": f"estimate from the oracle port (C restatement of the reference FMM; "
+                      f"the reference's -O3 AVX build is not buildable here): {n} uniform "
+                      f"sources = targets, depth 6 (same leaf density as 1e6 at depth 7), "
+                      f"order {p}, potential+gradient, {el:.1f} s wall, M2L on {nth} threads"}
 
 
 def sphere_points(seed, n):
@@ -482,6 +484,102 @@ def run_serving(args):
     print(json.dumps(out))
 
 
+def lap_flops_pair(p):
+    """flops of one Laplace M2L translation (two rotations, one coaxial
+    shift) at order p: 2 x 2R(p) + 2C(p)"""
+    R = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
+    C = (p + 1) ** 2 + 2 * sum((p + 1 - m) ** 2 for m in range(1, p + 1))
+    return 2 * (2 * R) + 2 * C
+
+
+def direct_reference(W, L, xs, d_xs, d_q, n, idx):
+    """FP64 direct sum (GPU) at the sampled targets, self-cutoff 1e-6"""
+    d_src4 = W.DeviceBuffer(n * 32)
+    L.wbfmm_hip_stage4(d_xs.ptr, 24, n, d_src4.ptr)
+    d_t4 = W.DeviceBuffer(len(idx) * 32)
+    L.wbfmm_hip_stage4(W.DeviceBuffer.from_array(np.ascontiguousarray(xs[idx])).ptr, 24,
+                       len(idx), d_t4.ptr)
+    d_fd = W.DeviceBuffer(len(idx) * 32)
+    d_fd.zero()
+    L.wbfmm_hip_lap_direct(d_src4.ptr, d_q.ptr, None, None, n, d_t4.ptr, len(idx), 3, 1e-6,
+                           d_fd.ptr, 4)
+    fd = d_fd.download(np.zeros((len(idx), 4)))
+    d_src4.free()
+    d_t4.free()
+    d_fd.free()
+    return fd
+
+
+def field_errors(f, fd):
+    """relative L2 errors of potential and gradient, and max|diff|/max|f|
+    per component (tests/maxdiff of the reference)"""
+    err_pot = float(np.linalg.norm(f[:, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0]))
+    err_grad = float(np.linalg.norm(f[:, 1:] - fd[:, 1:]) / np.linalg.norm(fd[:, 1:]))
+    maxd = [float(np.abs(f[:, c] - fd[:, c]).max() / np.abs(fd[:, c]).max()) for c in range(4)]
+    return err_pot, err_grad, maxd
+
+
+KERNEL_NAMES = ("lap_m2l", "lap_m2m", "lap_l2l", "lap_p2m", "lap_l2p_p2p", "tree_sort",
+                "tree_morton", "tree_box_ranges", "tree_box_flags", "tree_box_scatter",
+                "tree_mark_boxes", "gather4", "gather_strength", "stage4")
+
+
+def laplace_timed(W, L, d_xs, d_q, d_f, x0, D, n, depth, p, steps, warmup):
+    """warmup + timed solves of the Laplace drop-in: a step is tree build,
+    P2M, upward pass, downward pass and field at every target, inputs in
+    HBM; the first solve's extra time is the table precompute"""
+    S = W.LaplaceSolver(d_xs, x0, D, depth, [p] * 17, [p] * 17, targets=d_xs,
+                        ntgt=n, nsrc=n, host_mirror=False)
+
+    def step():
+        d_f.zero()
+        S.build_tree()
+        S.passes(d_q)
+        S.field(d_q, d_f, field=3, fstr=4)
+
+    W.sync()
+    t0 = time.perf_counter()
+    step()
+    W.sync()
+    first = time.perf_counter() - t0
+    for _ in range(max(warmup - 1, 0)):
+        step()
+    W.sync()
+    print(f"bench: order {p} warmup done", file=sys.stderr, flush=True)
+    try:
+        import torch
+        torch_sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        torch_sync = lambda: None
+
+    W.Timing.reset()
+    W.Timing.enable(True)
+    lat = []
+    torch_sync()
+    W.sync()
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        step()
+        W.sync()
+        lat.append(time.perf_counter() - t0)
+    W.sync()
+    torch_sync()
+    total = time.perf_counter() - t_all
+    W.Timing.enable(False)
+    kern = {}
+    for name in KERNEL_NAMES:
+        cnt, ms = W.Timing.query(name)
+        if cnt:
+            kern[name] = {"launches_per_step": cnt / steps, "ms_per_step": ms / steps,
+                          "avg_ms": ms / cnt}
+    S.close()
+    print(f"bench: order {p}: timed {steps} steps, {1e3 * total / steps:.2f} ms/step",
+          file=sys.stderr, flush=True)
+    return {"total": total, "lat": lat, "kernels": kern,
+            "precompute_ms": 1e3 * max(first - total / steps, 0.0)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,6 +590,8 @@ def main():
     ap.add_argument("--order", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-targets", type=int, default=2000)
+    ap.add_argument("--no-accuracy-line", action="store_true",
+                    help="skip the extra run at the order that meets 1e-6 on every component")
     ap.add_argument("--workload", default="laplace",
                     choices=["laplace", "helmholtz-cube", "helmholtz-sphere", "batched",
                              "serving"],
@@ -532,74 +632,18 @@ def main():
     d_xs = W.DeviceBuffer.from_array(xs)
     d_q = W.DeviceBuffer.from_array(q)
     d_f = W.DeviceBuffer(n * 4 * 8)
-    S = W.LaplaceSolver(d_xs, x0, D, depth, [p] * 17, [p] * 17, targets=d_xs,
-                        ntgt=n, nsrc=n, host_mirror=False)
-
-    def step():
-        d_f.zero()
-        S.build_tree()
-        S.passes(d_q)
-        S.field(d_q, d_f, field=3, fstr=4)
-
-    for _ in range(args.warmup):
-        step()
-    W.sync()
-    print("bench: warmup done", file=sys.stderr, flush=True)
-    try:
-        import torch
-        torch_sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        torch_sync = lambda: None
-
-    W.Timing.reset()
-    W.Timing.enable(True)
-    lat = []
-    torch_sync()
-    W.sync()
-    t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        step()
-        W.sync()
-        lat.append(time.perf_counter() - t0)
-    W.sync()
-    torch_sync()
-    total = time.perf_counter() - t_all
-    W.Timing.enable(False)
-    kern = {}
-    for name in ("lap_m2l", "lap_m2m", "lap_l2l", "lap_p2m", "lap_l2p_p2p", "tree_sort",
-                 "tree_morton", "tree_box_ranges", "tree_box_flags", "tree_box_scatter",
-                 "tree_mark_boxes", "gather4", "gather_strength", "stage4"):
-        cnt, ms = W.Timing.query(name)
-        if cnt:
-            kern[name] = {"launches_per_step": cnt / args.steps, "ms_per_step": ms / args.steps,
-                          "avg_ms": ms / cnt}
-    print(f"bench: timed {args.steps} steps, {1e3 * total / args.steps:.2f} ms/step",
-          file=sys.stderr, flush=True)
-
-    # accuracy against the FP64 direct sum on sampled targets
-    f = d_f.download(np.zeros((n, 4)))
     rng = np.random.default_rng(99)
     idx = np.sort(rng.choice(n, args.check_targets, replace=False))
-    d_src4 = W.DeviceBuffer(n * 32)
-    L.wbfmm_hip_stage4(d_xs.ptr, 24, n, d_src4.ptr)
-    d_t4 = W.DeviceBuffer(len(idx) * 32)
-    L.wbfmm_hip_stage4(W.DeviceBuffer.from_array(np.ascontiguousarray(xs[idx])).ptr, 24,
-                       len(idx), d_t4.ptr)
-    d_fd = W.DeviceBuffer(len(idx) * 32)
-    d_fd.zero()
-    L.wbfmm_hip_lap_direct(d_src4.ptr, d_q.ptr, None, None, n, d_t4.ptr, len(idx), 3, 1e-6,
-                           d_fd.ptr, 4)
-    fd = d_fd.download(np.zeros((len(idx), 4)))
-    err_pot = float(np.linalg.norm(f[idx, 0] - fd[:, 0]) / np.linalg.norm(fd[:, 0]))
-    err_grad = float(np.linalg.norm(f[idx, 1:] - fd[:, 1:]) / np.linalg.norm(fd[:, 1:]))
-    maxd = [float(np.abs(f[idx, c] - fd[:, c]).max() / np.abs(fd[:, c]).max()) for c in range(4)]
+    fd = direct_reference(W, L, xs, d_xs, d_q, n, idx)
+
+    run = laplace_timed(W, L, d_xs, d_q, d_f, x0, D, n, depth, p, args.steps, args.warmup)
+    kern = run["kernels"]
+    f = d_f.download(np.zeros((n, 4)))
+    err_pot, err_grad, maxd = field_errors(f[idx], fd)
 
     # roofline of the dominant kernel
     pairs = m2l_pairs(xs, x0, D, depth)
-    R = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
-    C = (p + 1) ** 2 + 2 * sum((p + 1 - m) ** 2 for m in range(1, p + 1))
-    flops_pair = 2 * (2 * R) + 2 * C
+    flops_pair = lap_flops_pair(p)
     m2l_flops_step = flops_pair * sum(pairs.values())
     m2l = kern.get("lap_m2l", {"ms_per_step": float("nan"), "launches_per_step": 1})
     achieved = m2l_flops_step / (m2l["ms_per_step"] * 1e-3) / 1e12
@@ -609,15 +653,16 @@ def main():
                 "kernel": "lap_m2l", "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
                 "pairs_per_step": int(sum(pairs.values())), "flops_per_pair": flops_pair}
 
-    lat_ms = np.array(lat) * 1e3
+    lat_ms = np.array(run["lat"]) * 1e3
+    tol = 1e-6
     out = {
         "metric": "targets/sec at 1e-6 rel err vs FP64 direct; p50/p99 solve latency",
-        "value": n * args.steps / total,
+        "value": n * args.steps / run["total"],
         "unit": "targets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
+        "ms_per_step": 1e3 * run["total"] / args.steps,
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "higher_is_better": True,
@@ -625,19 +670,37 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (SplitMix64 seed 3, uniform cube, N(0,1) charges)",
-        "config": {"workload": "laplace_fp64_dropin: 1e6 sources = targets, unit cube, "
-                               "depth 7, order 12, potential+gradient",
+        "config": {"workload": f"laplace_fp64_dropin: 1e6 sources = targets, unit cube, "
+                               f"depth {depth}, order {p}, potential+gradient",
                    "n_sources": n, "n_targets": n, "depth": depth, "order": p,
                    "parallelism": "single GPU"},
         "error_vs_fp64_direct": {"targets_checked": int(len(idx)), "rel_l2_potential": err_pot,
                                  "rel_l2_gradient": err_grad, "max_rel_per_component": maxd},
+        "meets_1e-6": {"potential": err_pot <= tol, "gradient": err_grad <= tol},
         "roofline": roofline,
         "kernels": kern,
+        "precompute_ms": run["precompute_ms"],
     }
+    # the same workload at the lowest order whose potential and gradient
+    # both reach the metric's accuracy (the configured order is kept as the
+    # headline; this line says what the bar costs)
+    if not (err_pot <= tol and err_grad <= tol) and not args.no_accuracy_line:
+        for pa in range(p + 1, min(p + 5, 25)):
+            ra = laplace_timed(W, L, d_xs, d_q, d_f, x0, D, n, depth, pa, args.steps,
+                               args.warmup)
+            fa = d_f.download(np.zeros((n, 4)))
+            ea_p, ea_g, _ = field_errors(fa[idx], fd)
+            print(f"bench: order {pa}: {1e3 * ra['total'] / args.steps:.2f} ms/step, "
+                  f"errors {ea_p:.2e} {ea_g:.2e}", file=sys.stderr, flush=True)
+            if ea_p <= tol and ea_g <= tol:
+                out["at_1e-6_all_components"] = {
+                    "order": pa, "value": n * args.steps / ra["total"], "unit": "targets/s",
+                    "ms_per_step": 1e3 * ra["total"] / args.steps,
+                    "rel_l2_potential": ea_p, "rel_l2_gradient": ea_g}
+                break
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)
-    S.close()
     if args.calibrate:
         rows = 8 << 20
         d_a = W.DeviceBuffer(rows * 24)

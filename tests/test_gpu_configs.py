@@ -1,0 +1,262 @@
+"""The Helmholtz sphere, batched and serving workloads on the GPU.
+
+- Sphere (10^6 points on the unit sphere, half monopoles and half radial
+  normal dipoles with complex N(0,1) strengths, k = 16, depth 7, orders from
+  the truncation number at tol 1e-6): at 12,000 points and depth 4 the
+  solve is compared with the oracle FMM on the same tree (rounding level);
+  at full size through properties: identical bits on a rerun, linearity in
+  the strengths, and the error against the FP64 direct sum on sampled
+  source points and on interior points (radius 0.5), whose field is all
+  far field, so the near-field sum cannot dominate the norm.
+- Batched: 256 problems (N = 10^U(3,5), Laplace / Helmholtz alternating,
+  cube or sphere, tol 1e-6) through the engine at once; every problem's
+  error against the FP64 direct sum is checked and a sample of problems is
+  solved again alone, giving the same bits.
+- Serving: a Poisson stream of mixed requests, with Helmholtz at k = 32
+  and tol 1e-8 among them, each within its requested accuracy; requests
+  rerun alone give the same bits.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import fit_box
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-10
+
+
+def c2(z):
+    return np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], 1))
+
+
+def sphere_problem(seed, n):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    xs = v / np.linalg.norm(v, axis=1)[:, None]
+    half = np.arange(n) % 2 == 0
+    q = np.where(half, rng.normal(size=n) + 1j * rng.normal(size=n), 0.0)
+    d = np.where(~half, rng.normal(size=n) + 1j * rng.normal(size=n), 0.0)
+    return np.ascontiguousarray(xs), q, d
+
+
+def truncation_orders(W, D, k, tol, depth):
+    t = W.WbfmmTree()
+    t.D = D
+    o = [W.lib().wbfmm_truncation_number(ctypes.byref(t), k, max(l, 2), tol)
+         for l in range(depth + 1)]
+    return o + [o[-1]] * (17 - len(o))
+
+
+def hel_solve(W, xs, q, d, xt, x0, D, depth, orders, k, keep=False):
+    # the targets are registered, so the downward pass runs over the boxes
+    # holding sources or targets only
+    xt = np.ascontiguousarray(xt)
+    S = W.HelmholtzSolver(xs, x0, D, depth, orders, orders, k, normals=xs, nsrc=len(xs),
+                          targets=xt)
+    S.build_tree()
+    S.passes(c2(q), d=c2(d))
+    f = np.zeros((len(xt), 8))
+    S.field(np.ascontiguousarray(xt), c2(q), f, field=3, fstr=8, d=c2(d))
+    W.sync()
+    if keep:
+        return f, S
+    S.close()
+    return f
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_sphere_small_matches_oracle(wb, oracle):
+    n, k, depth = 12_000, 16.0, 4
+    xs, q, d = sphere_problem(401, n)
+    x0, D = fit_box(xs)
+    orders = truncation_orders(wb, D, k, 1e-6, depth)
+    print(f"sphere n={n} k={k} depth={depth} orders {orders[2:depth + 1]}", flush=True)
+    assert orders[2] >= 23
+    fg = hel_solve(wb, xs, q, d, xs, x0, D, depth, orders, k)
+    fo, _, _ = oracle.fmm(2, xs, xs, x0, D, depth, orders, orders, q=c2(q), nrm=xs, d=c2(d),
+                          k=k, field=3, nthreads=8)
+    for c in range(8):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (c, rel(fg[:, c], fo[:, c]))
+
+
+def hel_direct_gpu(W, xs, q, d, xt, k):
+    L = W.lib()
+    n, m = len(xs), len(xt)
+    s4 = W.DeviceBuffer.from_array(np.concatenate([xs, np.zeros((n, 1))], 1))
+    t4 = W.DeviceBuffer.from_array(np.concatenate([xt, np.zeros((m, 1))], 1))
+    dq = W.DeviceBuffer.from_array(c2(q))
+    dd = W.DeviceBuffer.from_array(c2(d))
+    fd = W.DeviceBuffer(m * 64)
+    fd.zero()
+    rc = L.wbfmm_hip_hel_direct(s4.ptr, dq.ptr, s4.ptr, dd.ptr, n, t4.ptr, m, k, 3, 1e-6,
+                                fd.ptr, 8)
+    assert rc == 0
+    out = fd.download(np.zeros((m, 8)))
+    for b in (s4, t4, dq, dd, fd):
+        b.free()
+    return out[:, 0::2] + 1j * out[:, 1::2]
+
+
+def test_sphere_full_size_properties(wb):
+    n, k, depth = 1_000_000, 16.0, 7
+    xs, q1, d1 = sphere_problem(402, n)
+    _, q2, d2 = sphere_problem(403, n)
+    x0, D = fit_box(xs)
+    orders = truncation_orders(wb, D, k, 1e-6, depth)
+    print(f"sphere full size: orders {orders[2:depth + 1]}", flush=True)
+    rng = np.random.default_rng(404)
+    idx = np.sort(rng.choice(n, 4000, replace=False))
+    v = rng.normal(size=(2000, 3))
+    xin = 0.5 * v / np.linalg.norm(v, axis=1)[:, None]
+    xt = np.concatenate([xs, xin])
+    f1 = hel_solve(wb, xs, q1, d1, xt, x0, D, depth, orders, k)
+    print("sphere full size: first solve done", flush=True)
+    f1b = hel_solve(wb, xs, q1, d1, xt, x0, D, depth, orders, k)
+    assert np.array_equal(f1, f1b)
+    f2 = hel_solve(wb, xs, q2, d2, xt, x0, D, depth, orders, k)
+    f12 = hel_solve(wb, xs, q1 + 2.0 * q2, d1 + 2.0 * d2, xt, x0, D, depth, orders, k)
+    assert rel(f12, f1 + 2.0 * f2) < 1e-12
+    print("sphere full size: rerun and linearity checked", flush=True)
+    sel = np.concatenate([idx, n + np.arange(len(xin))])
+    fd = hel_direct_gpu(wb, xs, q1, d1, xt[sel], k)
+    fc = f1[sel][:, 0::2] + 1j * f1[sel][:, 1::2]
+    for name, rows in (("sources", slice(0, len(idx))), ("interior", slice(len(idx), None))):
+        ep = np.linalg.norm(fc[rows, 0] - fd[rows, 0]) / np.linalg.norm(fd[rows, 0])
+        eg = np.linalg.norm(fc[rows, 1:] - fd[rows, 1:]) / np.linalg.norm(fd[rows, 1:])
+        print(f"sphere full size: {name} rel L2 error potential {ep:.3e} gradient {eg:.3e}",
+              flush=True)
+        assert ep < 1e-6, (name, ep)
+        assert eg < 1e-6, (name, eg)
+
+
+@pytest.fixture(scope="module")
+def S(wb):
+    import wbfmm_amd.serve as S
+    return S
+
+
+def _upload(W, probs):
+    for p in probs:
+        p["d_xs"] = W.DeviceBuffer.from_array(p["xs"])
+        p["d_q"] = W.DeviceBuffer.from_array(p["q"])
+        p["d_f"] = W.DeviceBuffer(p["n"] * 8 * 8)
+        p["d_f"].zero()
+    W.sync()
+
+
+def _release(probs):
+    for p in probs:
+        for key in ("d_xs", "d_q", "d_f"):
+            p[key].free()
+
+
+def _request(S, p, check=0, depth=0):
+    fw = 8 if p["helm"] else 4
+    return S.make_request(2 if p["helm"] else 1, p["d_xs"], q=p["d_q"], f=p["d_f"], fstr=fw,
+                          k=p["k"], tol=p["tol"], field=3, depth=depth, check=check,
+                          nsrc=p["n"])
+
+
+def _field(p):
+    fw = 8 if p["helm"] else 4
+    return p["d_f"].download(np.zeros((p["n"], fw)))
+
+
+def test_batched_256_problems(wb, S):
+    import bench
+    probs = bench.batched_problems(256)
+    _upload(wb, probs)
+    try:
+        with S.Server(8) as srv:
+            tk = [srv.submit(_request(S, p, check=100, depth=p["depth"])) for p in probs]
+            res = [srv.wait(t) for t in tk]
+            assert all(r.status == 0 for r in res), [r.status for r in res if r.status]
+            ep = np.array([r.err_potential for r in res])
+            eg = np.array([r.err_gradient for r in res])
+            print(f"batched: potential error max {ep.max():.2e} median {np.median(ep):.2e}; "
+                  f"gradient max {eg.max():.2e} median {np.median(eg):.2e}", flush=True)
+            assert (ep > 0).all() and ep.max() <= 1e-6
+            assert eg.max() <= 1e-5
+            # problems solved again alone give the same bits
+            pick = [0, 1, 2, 3, 100, 101, 254, 255]
+            batch = {i: _field(probs[i]) for i in pick}
+            for i in pick:
+                probs[i]["d_f"].zero()
+                r = srv.wait(srv.submit(_request(S, probs[i], depth=probs[i]["depth"])))
+                assert r.status == 0
+                assert np.array_equal(_field(probs[i]), batch[i]), i
+    finally:
+        _release(probs)
+
+
+def serving_stream(count, qps, seed):
+    """Poisson arrivals: N = 10^U(3, 5.3), Laplace or Helmholtz (k ~
+    U(1, 32)), cube or sphere, tol 1e-4 or 1e-8; the first request is a
+    Helmholtz solve at k = 32 and tol 1e-8, the second a Laplace one at
+    tol 1e-8"""
+    import bench
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(count):
+        n = int(10.0 ** rng.uniform(3.0, 5.3))
+        helm = rng.uniform() < 0.5
+        k = rng.uniform(1.0, 32.0)
+        tol = 1e-4 if rng.uniform() < 0.5 else 1e-8
+        sphere = rng.uniform() < 0.5
+        if i == 0:
+            n, helm, k, tol = 30_000, True, 32.0, 1e-8
+        if i == 1:
+            n, helm, tol = 60_000, False, 1e-8
+        t += rng.exponential(1.0 / qps)
+        xs, q = bench.mixed_problem(seed * 7919 + i, n, helm, sphere, k)
+        out.append(dict(n=n, helm=helm, sphere=sphere, k=k if helm else 0.0, tol=tol, t=t,
+                        xs=xs, q=q))
+    return out
+
+
+def test_serving_poisson_stream(wb, S):
+    import threading
+    import time
+    reqs = serving_stream(14, 8.0, 501)
+    _upload(wb, reqs)
+    try:
+        with S.Server(4) as srv:
+            tickets = [None] * len(reqs)
+            t0 = time.perf_counter()
+
+            def feeder():
+                for i, p in enumerate(reqs):
+                    delay = t0 + p["t"] - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    tickets[i] = srv.submit(_request(S, p, check=150))
+
+            th = threading.Thread(target=feeder)
+            th.start()
+            th.join()
+            res = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+            for p, r in zip(reqs, res):
+                print(f"serving: n={p['n']} {'helm k=%.1f' % p['k'] if p['helm'] else 'lap'} "
+                      f"tol={p['tol']:g} fp{r.precision} order={r.order} depth={r.depth} "
+                      f"latency={r.latency_ms:.1f} ms err={r.err_potential:.2e} "
+                      f"{r.err_gradient:.2e}", flush=True)
+                assert r.status == 0
+                assert 0 < r.err_potential <= p["tol"], (p["n"], p["helm"], p["tol"])
+            assert st.completed == len(reqs)
+            assert 0 < st.p50_ms <= st.p99_ms
+            mixed = {i: _field(reqs[i]) for i in (0, 1, len(reqs) - 1)}
+            for i in mixed:
+                reqs[i]["d_f"].zero()
+                r = srv.wait(srv.submit(_request(S, reqs[i])))
+                assert r.status == 0
+                assert np.array_equal(_field(reqs[i]), mixed[i]), i
+    finally:
+        _release(reqs)

@@ -188,8 +188,9 @@ def test_hel_config1_against_direct(wb, oracle):
         assert e < 1e-4, (c, e)
 
 
-def test_hel_target_list_matches_field_targets(wb, oracle):
-    # the target-list path (src/targets.c) gives the batched field
+def test_hel_target_list_matches_oracle(wb, oracle):
+    # the target-list path (src/targets.c) against the oracle FMM on the
+    # same tree, and against the batched field
     import ctypes
     rng = np.random.default_rng(12)
     xs = rng.uniform(-1, 1, (1500, 3))
@@ -211,6 +212,10 @@ def test_hel_target_list_matches_field_targets(wb, oracle):
     wb.sync()
     L.wbfmm_target_list_free(tl)
     S.close()
+    fo, _, _ = oracle.fmm(2, xs, xt, x0, D, 3, [9] * 17, [9] * 17, q=qq, k=k, field=3,
+                          nthreads=8)
+    for c in range(8):
+        assert rel(g[:, c], fo[:, c]) < PARITY, (c, rel(g[:, c], fo[:, c]))
     assert np.abs(g - f).max() < 1e-12 * np.abs(f).max()
 
 

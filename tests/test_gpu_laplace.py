@@ -330,3 +330,67 @@ def test_packed_sparse_level_separate_targets_dipoles(wb, monkeypatch):
     monkeypatch.delenv("WBFMM_M2L_PACKED")
     for c in range(4):
         assert rel(out["1"][:, c], out["0"][:, c]) < 1e-12, c
+
+
+def packed_cases():
+    rng = np.random.default_rng(79)
+    out = {}
+    # about 60 % of the level-4 boxes occupied
+    xs = rng.uniform(-1, 1, (4000, 3))
+    out["cube_d4"] = (xs, rng.normal(size=4000), xs, 4, None, None)
+    # sparse: points on a sphere, most boxes empty
+    v = rng.normal(size=(6000, 3))
+    xs = v / np.linalg.norm(v, axis=1)[:, None]
+    out["sphere_d5"] = (xs, rng.normal(size=6000), xs, 5, None, None)
+    # separate targets (a target list) with normal dipoles
+    xs = rng.uniform(0, 1, (2500, 3))
+    xt = rng.uniform(0, 1, (1100, 3))
+    nrm = rng.normal(size=(2500, 3))
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    out["targets_dipoles_d4"] = (xs, rng.normal(size=2500), xt, 4, nrm, rng.normal(size=2500))
+    return out
+
+
+PACKED_CASES = packed_cases()
+
+
+@pytest.mark.parametrize("waves", ["4", "8"])
+@pytest.mark.parametrize("p", [2, 3, 5, 8, 10, 11, 12])
+@pytest.mark.parametrize("name", sorted(PACKED_CASES))
+def test_packed_m2l_matches_oracle(wb, oracle, monkeypatch, name, p, waves):
+    # the sparse-level M2L (forced at every level, four or eight wavefronts
+    # per 32 targets) against the oracle FMM on the same tree
+    xs, q, xt, depth, nrm, d = PACKED_CASES[name]
+    x0, D = fit_box(xs, xt)
+    print(f"packed case {name} p={p} waves={waves}", flush=True)
+    fo, _, _ = oracle.fmm(1, xs, xt, x0, D, depth, [p] * 17, [p] * 17, q=q, nrm=nrm, d=d,
+                          field=3, nthreads=8)
+    monkeypatch.setenv("WBFMM_M2L_PACKED", "1")
+    monkeypatch.setenv("WBFMM_PACKED_WAVES", waves)
+    fg = gpu_fmm(wb, xs, q, xt, x0, D, depth, p, field=3, nrm=nrm, d=d)
+    for c in range(4):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (name, p, waves, c, rel(fg[:, c], fo[:, c]))
+
+
+def test_packed_m2l_chosen_by_size_matches_oracle(wb, oracle):
+    # 25,000 uniform points at depth 5: level 5 has about 17,500 occupied
+    # boxes (53 %) in more than 1024 chunks, so the sparse-level kernel is
+    # chosen there without forcing; compared with the oracle on the same
+    # tree, at the headline order
+    rng = np.random.default_rng(80)
+    n = 25_000
+    xs = rng.uniform(0, 1, (n, 3))
+    q = rng.normal(size=n)
+    x0, D = fit_box(xs)
+    # the kernel's selection rule, restated: fewer than 60 % of the level's
+    # boxes hold targets and the 16-target chunks of the eight Morton child
+    # classes number at least 1024
+    g = np.floor((xs - x0) / D * 32).astype(np.int64)
+    boxes = np.unique(g[:, 0] + 32 * g[:, 1] + 1024 * g[:, 2])
+    cls = (boxes & 1) | (((boxes >> 5) & 1) << 1) | (((boxes >> 10) & 1) << 2)
+    chunks = sum((np.count_nonzero(cls == c) + 15) // 16 for c in range(8))
+    assert len(boxes) / 32 ** 3 < 0.6 and chunks >= 1024, (len(boxes), chunks)
+    fg = gpu_fmm(wb, xs, q, xs, x0, D, 5, 12)
+    fo, _, _ = oracle.fmm(1, xs, xs, x0, D, 5, [12] * 17, [12] * 17, q=q, field=3, nthreads=8)
+    for c in range(4):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (c, rel(fg[:, c], fo[:, c]))

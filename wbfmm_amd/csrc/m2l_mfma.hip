@@ -1370,8 +1370,9 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
 }
 
 // the packed kernel for levels where fewer than 60 % of the boxes hold
-// targets; WBFMM_M2L_PACKED=0 or 1 forces the choice (read at each call,
-// for comparison)
+// targets and there are at least 1024 chunks; WBFMM_M2L_PACKED=0 or 1
+// forces the choice at every level and size (read at each call, for
+// comparison)
 static int m2l_packed_choice() {
   const char *e = getenv("WBFMM_M2L_PACKED");
   return e ? atoi(e) : -1;
@@ -1397,7 +1398,9 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   if constexpr (sizeof(T) == 8 && PackedFits<N>::value) {
     const int pc = m2l_packed_choice();
     const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
-    if (nchunks >= 1024 && (pc == 1 || (pc == -1 && fill < 0.6))) {
+    // forced on, the kernel runs at any size (it needs no minimum chunk
+    // count); chosen automatically, only where it pays
+    if (pc == 1 || (pc == -1 && nchunks >= 1024 && fill < 0.6)) {
       ClassArgs cp = ca;
       unsigned nch = 0;
       for (int c = 0; c < 8; c++) {
