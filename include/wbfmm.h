@@ -176,6 +176,12 @@ wbfmm_tree_t *wbfmm_tree_new(gdouble *x, gdouble D, guint maxpoints);
 gint wbfmm_tree_add_points(wbfmm_tree_t *t, gpointer pts, gsize pstr,
                            gpointer normals, gsize nstr, guint npts,
                            gboolean sorted);
+/* src/wbfmm.h:826-828 (src/boxes.c:190-206): reorders the caller's npts
+   records of psize bytes (coordinates first) in place by Morton index,
+   equal indices keeping their order; Morton codes and the stable sort run
+   on the GPU, the records move on the GPU for device arrays and on the
+   host for host arrays.  Aborts on a point outside the domain. */
+gint wbfmm_tree_sort_points(wbfmm_tree_t *t, gpointer pts, gsize psize, guint npts);
 /* src/wbfmm.h:808: adds one level; box ranges computed on the GPU */
 gint wbfmm_tree_refine(wbfmm_tree_t *t);
 /* src/wbfmm.h:807 */
@@ -387,11 +393,15 @@ gint wbfmm_tree_set_precision(wbfmm_tree_t *t, gint bits);
    float arguments.  A tree made by wbfmm_tree_new_f is an FP32 tree, as
    after wbfmm_tree_set_precision(t, 32): orders 2..15, the same order at
    every level.  Float arrays are host arrays; float outputs are
-   accumulated in float.  The tree origin is held in double. */
+   accumulated in float.  The public origin (wbfmm_tree_origin) holds
+   three gfloat, as in the reference; the passes use a double copy. */
 wbfmm_tree_t *wbfmm_tree_new_f(gfloat *x, gfloat D, guint maxpoints);
 gint wbfmm_tree_add_points_f(wbfmm_tree_t *t, gpointer pts, gsize pstr,
                              gpointer normals, gsize nstr, guint npts,
                              gboolean sorted);
+/* src/wbfmm.h:1011-1013: float records (host arrays), codes from the
+   coordinates widened to double */
+gint wbfmm_tree_sort_points_f(wbfmm_tree_t *t, gpointer pts, gsize psize, guint npts);
 gint wbfmm_tree_refine_f(wbfmm_tree_t *t);
 gint wbfmm_tree_laplace_coefficient_init_f(wbfmm_tree_t *t, guint l, guint nr,
                                            guint ns);

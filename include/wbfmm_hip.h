@@ -79,6 +79,10 @@ int wbfmm_hip_sort(uint64_t *codes, uint32_t *idx, unsigned n);
 /* out[4i..4i+3] = in[4 idx[i] ..] */
 int wbfmm_hip_gather4(const double *in, const uint32_t *idx, unsigned n,
                       double *out);
+/* record i of out = record idx[i] of in, records of rsize bytes (device
+   arrays) */
+int wbfmm_hip_gather_records(const void *in, size_t rsize, const uint32_t *idx,
+                             unsigned n, void *out);
 /* out[i] = src[idx[i]*sstr] (src host or device) */
 int wbfmm_hip_gather_strided(const double *src, int sstr, int ncomp,
                              const uint32_t *idx, unsigned n, double *out);

@@ -44,19 +44,41 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
+def _deps(obj):
+    """files the object was built from, per the compiler's -MMD list"""
+    dep = obj + ".d"
+    if not os.path.exists(dep):
+        return None
+    with open(dep) as fh:
+        text = fh.read().replace("\\\n", " ")
+    if ":" not in text:
+        return None
+    return [f for f in text.split(":", 1)[1].split() if f]
+
+
+def _stale(src, obj):
+    """object missing or older than a file it was built from"""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = _deps(obj) or [src]
+    return any(not os.path.exists(f) or os.path.getmtime(f) > t for f in deps + [src])
+
+
+def build(verbose=False, incremental=False):
+    """compile every source (incremental: only those whose object is out
+    of date) and link the library"""
     os.makedirs(OBJ, exist_ok=True)
     jobs = []
-    for s in HIP_SOURCES:
-        o = os.path.join(OBJ, s + ".o")
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", "-o", o, os.path.join(CSRC, s)])
-    for s in C_SOURCES:
-        o = os.path.join(OBJ, s + ".o")
-        jobs.append([CC, *C_FLAGS, "-c", "-o", o, os.path.join(CSRC, s)])
-    for s in CXX_SOURCES:
-        o = os.path.join(OBJ, s + ".o")
-        jobs.append([CXX, *CXX_FLAGS, "-c", "-o", o, os.path.join(CSRC, s)])
-    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+    for srcs, cmd in ((HIP_SOURCES, [HIPCC, *HIP_FLAGS]), (C_SOURCES, [CC, *C_FLAGS]),
+                      (CXX_SOURCES, [CXX, *CXX_FLAGS])):
+        for s in srcs:
+            o = os.path.join(OBJ, s + ".o")
+            src = os.path.join(CSRC, s)
+            if incremental and not _stale(src, o):
+                continue
+            jobs.append([*cmd, "-MMD", "-MF", o + ".d", "-c", "-o", o, src])
+    with ThreadPoolExecutor(max_workers=max(1, min(8, len(jobs)))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
@@ -65,8 +87,24 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp, *objs,
           "-lquadmath", "-lm", "-lpthread"])
     os.replace(tmp, LIB)
+    build_drivers()
     return LIB
 
 
+DRIVERS = ["wbfmm-tree-calc", "wbfmm-laplace-tree-calc"]
+BIN = os.path.join(HERE, "bin")
+
+
+def build_drivers():
+    """the replacement driver programs, plain C against include/wbfmm.h,
+    linked to the library (found next to bin/ at run time)"""
+    os.makedirs(BIN, exist_ok=True)
+    src_dir = os.path.join(HERE, "drivers")
+    for d in DRIVERS:
+        _run([CC, "-O2", "-g", "-std=gnu11", "-Wall", "-I", INC, "-I", src_dir, "-o",
+              os.path.join(BIN, d), os.path.join(src_dir, d + ".c"), "-L", HERE, "-l:libwbfmm.so",
+              "-Wl,-rpath,$ORIGIN/..", "-lm"])
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, incremental="--incremental" in sys.argv))

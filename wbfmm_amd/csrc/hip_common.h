@@ -18,6 +18,8 @@ int dmalloc(void **p, size_t bytes);
 void dfree(void *p);
 // record an error message; returns WBFMM_HIP_ERR_RUNTIME
 int fail(const char *what, hipError_t e);
+// record an error message of the library's own; returns code
+int fail_msg(const char *msg, int code);
 // begin/end a timed launch of kernel `name` (no-ops unless timing is on)
 void timing_begin(const char *name);
 void timing_end(const char *name);

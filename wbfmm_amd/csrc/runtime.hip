@@ -51,6 +51,12 @@ int fail(const char *what, hipError_t e) {
   return WBFMM_HIP_ERR_RUNTIME;
 }
 
+int fail_msg(const char *msg, int code) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  fprintf(stderr, "wbfmm (HIP): %s\n", g_err);
+  return code;
+}
+
 // Timing covers launches made on the library stream only; launches on a
 // thread's own stream are not timed.
 void timing_begin(const char *name) {
@@ -130,6 +136,24 @@ __global__ void k_gather4(const double *in, const uint32_t *idx, unsigned n,
   d[1] = s[1];
   d[2] = s[2];
   d[3] = s[3];
+}
+
+// out record i = in record idx[i], records of rsize bytes moved as
+// 4-byte words when rsize allows, else bytes
+__global__ void k_gather_records4(const uint32_t *in, size_t rw, const uint32_t *idx,
+                                  unsigned n, uint32_t *out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * rw) return;
+  const size_t i = e / rw, w = e % rw;
+  out[e] = in[(size_t)idx[i] * rw + w];
+}
+
+__global__ void k_gather_records1(const uint8_t *in, size_t rsize, const uint32_t *idx,
+                                  unsigned n, uint8_t *out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * rsize) return;
+  const size_t i = e / rsize, w = e % rsize;
+  out[e] = in[(size_t)idx[i] * rsize + w];
 }
 
 __global__ void k_gather_strided(const double *src, int sstr, int ncomp,
@@ -465,6 +489,22 @@ int wbfmm_hip_gather_strided(const double *src, int sstr, int ncomp,
   if (staged) {
     dfree(staged);
     WBFMM_CHECK(hipStreamSynchronize(stream()));
+  }
+  return 0;
+}
+
+int wbfmm_hip_gather_records(const void *in, size_t rsize, const uint32_t *idx, unsigned n,
+                             void *out) {
+  if (n == 0) return 0;
+  if (rsize == 0) return WBFMM_HIP_ERR_ARG;
+  if (rsize % 4 == 0 && (uintptr_t)in % 4 == 0 && (uintptr_t)out % 4 == 0) {
+    const size_t rw = rsize / 4, tot = (size_t)n * rw;
+    WBFMM_LAUNCH("gather_records", k_gather_records4, dim3((unsigned)((tot + 255) / 256)),
+                 dim3(256), 0, (const uint32_t *)in, rw, idx, n, (uint32_t *)out);
+  } else {
+    const size_t tot = (size_t)n * rsize;
+    WBFMM_LAUNCH("gather_records", k_gather_records1, dim3((unsigned)((tot + 255) / 256)),
+                 dim3(256), 0, (const uint8_t *)in, rsize, idx, n, (uint8_t *)out);
   }
   return 0;
 }

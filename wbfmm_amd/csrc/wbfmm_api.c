@@ -67,6 +67,10 @@ typedef struct {
   double *d_q, *d_dip; /* strengths in sorted order */
   unsigned cap_q, cap_dip;
   int prec32; /* FP32 coefficients scaled by the box width (Laplace) */
+  /* the origin in double for the device passes; the public x[] holds it
+     in the tree's own precision (gfloat for wbfmm_tree_new_f), as the
+     reference does */
+  double x0[3];
 } tree_dev;
 
 typedef struct {
@@ -147,6 +151,8 @@ typedef struct {
 #define SHIFT_MFMA_MAX 15
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
+/* origin of the domain cube in double, whatever the tree's precision */
+#define ORIGIN(_t) (TD(_t)->x0)
 
 static int g_lap_coax_N = -1; /* wbfmm_laplace_coaxial_translate_init */
 
@@ -330,7 +336,7 @@ guint64 wbfmm_point_index_3d(gdouble *x, gdouble *c, gdouble D)
 
 guint64 wbfmm_point_box(wbfmm_tree_t *t, guint level, gdouble *x)
 {
-  guint64 i = wbfmm_point_index_3d(x, (gdouble *)wbfmm_tree_origin(t), t->D);
+  guint64 i = wbfmm_point_index_3d(x, ORIGIN(t), t->D);
   return wbfmm_point_locate_box(i, level);
 }
 
@@ -351,7 +357,7 @@ gint wbfmm_box_location_from_index(guint64 idx, guint32 level, gdouble *x0,
 gint wbfmm_tree_box_centre(wbfmm_tree_t *t, guint level, guint64 b,
                            gdouble *xb, gdouble *wb)
 {
-  wbfmm_box_location_from_index(b, level, (gdouble *)wbfmm_tree_origin(t),
+  wbfmm_box_location_from_index(b, level, ORIGIN(t),
                                 t->D, xb, wb);
   xb[0] += 0.5 * (*wb);
   xb[1] += 0.5 * (*wb);
@@ -406,6 +412,9 @@ wbfmm_tree_t *wbfmm_tree_new(gdouble *x, gdouble D, guint maxpoints)
   wbfmm_tree_t *t = &td->t;
   double *xt = (double *)wbfmm_tree_origin(t);
   HIPOK(wbfmm_hip_init());
+  td->x0[0] = x[0];
+  td->x0[1] = x[1];
+  td->x0[2] = x[2];
   t->problem = 0;
   t->sorted = FALSE;
   t->maxpoints = maxpoints;
@@ -483,7 +492,7 @@ static gint add_points_dbl(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer n
 {
   tree_dev *td = TD(t);
   unsigned nbad = 0;
-  double *x0 = (double *)wbfmm_tree_origin(t);
+  double *x0 = ORIGIN(t);
   if (npts > t->maxpoints)
     WBFMM_FATAL("too many points (%u) for tree (%u)", npts, t->maxpoints);
   /* a new point set starts a new tree: levels are rebuilt by refine and
@@ -515,6 +524,61 @@ static gint add_points_dbl(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer n
   t->boxes[0][0].n = npts;
   level_ranges(td, 0);
   return 0;
+}
+
+/* Reorders the caller's npts records of psize bytes in place by the
+   Morton index of their first three coordinates (xyz read at the start of
+   each record, xstr bytes apart in the staging copy), keeping the order of
+   equal indices (src/boxes.c:190-206 sorts with GLib's stable merge sort).
+   Codes and the stable sort run on the GPU; the records move on the GPU
+   when they are device memory and on the host otherwise. */
+static gint sort_records(wbfmm_tree_t *t, const void *xyz, gsize xstr, gpointer pts,
+                         gsize psize, guint npts)
+{
+  double *dx = NULL;
+  uint64_t *dc = NULL;
+  uint32_t *di = NULL;
+  unsigned nbad = 0;
+  if (npts == 0) return 0;
+  if (psize < 3 * t->size) WBFMM_FATAL("point size %zu below three coordinates", (size_t)psize);
+  HIPOK(wbfmm_hip_malloc((void **)&dx, sizeof(double) * 4 * npts));
+  HIPOK(wbfmm_hip_malloc((void **)&dc, sizeof(uint64_t) * npts));
+  HIPOK(wbfmm_hip_malloc((void **)&di, sizeof(uint32_t) * npts));
+  HIPOK(wbfmm_hip_morton(xyz, xstr, npts, ORIGIN(t), t->D, dx, dc, di, &nbad));
+  if (nbad)
+    WBFMM_FATAL("%u points do not lie in box of width %g at (%g,%g,%g)", nbad, t->D,
+                ORIGIN(t)[0], ORIGIN(t)[1], ORIGIN(t)[2]);
+  HIPOK(wbfmm_hip_sort(dc, di, npts));
+  if (wbfmm_hip_is_device(pts)) {
+    void *tmp = NULL;
+    HIPOK(wbfmm_hip_malloc(&tmp, psize * npts));
+    HIPOK(wbfmm_hip_gather_records(pts, psize, di, npts, tmp));
+    HIPOK(wbfmm_hip_memcpy(pts, tmp, psize * npts));
+    HIPOK(wbfmm_hip_free(tmp));
+  } else {
+    uint32_t *hi = malloc(sizeof(uint32_t) * npts);
+    char *tmp = malloc(psize * npts);
+    guint i;
+    if (!hi || !tmp) WBFMM_FATAL("out of host memory");
+    HIPOK(wbfmm_hip_memcpy(hi, di, sizeof(uint32_t) * npts));
+    for (i = 0; i < npts; i++) memcpy(tmp + (size_t)i * psize, (char *)pts + (size_t)hi[i] * psize, psize);
+    memcpy(pts, tmp, psize * npts);
+    free(hi);
+    free(tmp);
+  }
+  HIPOK(wbfmm_hip_free(dx));
+  HIPOK(wbfmm_hip_free(dc));
+  HIPOK(wbfmm_hip_free(di));
+  HIPOK(wbfmm_hip_sync());
+  return 0;
+}
+
+gint wbfmm_tree_sort_points(wbfmm_tree_t *t, gpointer pts, gsize psize, guint npts)
+{
+  if (t->size != sizeof(double))
+    WBFMM_FATAL("mixed precision not implemented (size of tree data type (%zu) not equal "
+                "to size of requested target type (%zu))", (size_t)t->size, sizeof(double));
+  return sort_records(t, pts, psize, pts, psize, npts);
 }
 
 gint wbfmm_tree_refine(wbfmm_tree_t *t)
@@ -711,12 +775,12 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
   if (td->prec32)
     HIPOK(wbfmm_hip_lap_p2m_f32(td->d_occ[d], td->nocc[d], td->d_start[d],
                                 td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                                dp, d, (double *)wbfmm_tree_origin(t), t->D,
+                                dp, d, ORIGIN(t), t->D,
                                 t->order_s[d], (float *)t->mps[d]));
   else
     HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
                             td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                            dp, d, (double *)wbfmm_tree_origin(t), t->D,
+                            dp, d, ORIGIN(t), t->D,
                             t->order_s[d], (double *)t->mps[d]));
   return 0;
 }
@@ -1291,12 +1355,12 @@ static void lap_field_run(tree_dev *td, const double *dxt,
   if (td->prec32)
     HIPOK(wbfmm_hip_lap_field_f32(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
                                   td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
-                                  (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
+                                  ORIGIN(t), t->D, t->order_r[lv],
                                   (const float *)t->mpr[lv], field, df, fstr));
   else
     HIPOK(wbfmm_hip_lap_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv],
                               td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv,
-                              (double *)wbfmm_tree_origin(t), t->D, t->order_r[lv],
+                              ORIGIN(t), t->D, t->order_r[lv],
                               (double *)t->mpr[lv], field, df, fstr));
   field_finish(f, df, nelem, staged);
 }
@@ -1315,7 +1379,7 @@ gint wbfmm_laplace_field_targets(wbfmm_tree_t *t, gdouble *xt, gsize tstr,
   if (fl == 0 || (field & ~3u)) return -2;
   if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0)) return -2;
   if (ntgt == 0) return 0;
-  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, (double *)wbfmm_tree_origin(t),
+  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, ORIGIN(t),
                 t->D, 1);
   lap_field_run(td, dx, did, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
   HIPOK(wbfmm_hip_sync());
@@ -1348,7 +1412,7 @@ gint wbfmm_laplace_box_field(wbfmm_tree_t *t, guint level, guint b,
   }
   if (fl == 2 && fstr < 3) WBFMM_FATAL("field data stride (%d) must be greater than two", fstr);
   stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc,
-                (double *)wbfmm_tree_origin(t), t->D, 0);
+                ORIGIN(t), t->D, 0);
   if (!eval_neighbours) { src = NULL; d = NULL; }
   {
     /* f holds 1 (potential) or 3 (gradient) values at unit stride */
@@ -1474,7 +1538,7 @@ gint wbfmm_target_list_add_points(wbfmm_target_list_t *l, gpointer pts,
   l->pstr = pstr;
   l->npoints = npts;
   stage_targets(pts, pstr, npts, &ld->d_xt, &ld->d_tid, &ld->d_codes,
-                (double *)wbfmm_tree_origin(l->t), l->t->D, 1);
+                ORIGIN(l->t), l->t->D, 1);
   register_targets(td, ld->d_codes, npts);
   return 0;
 }
@@ -1631,7 +1695,7 @@ gint wbfmm_tree_leaf_expansions(wbfmm_tree_t *t, gdouble k, gdouble *src,
   dp = gather_strengths_n(td, dipoles, dstr, 1, 2);
   HIPOK(wbfmm_hip_hel_p2m(td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d],
                           td->d_xyz, q, dp ? td->d_nrm : NULL, dp, d,
-                          (double *)wbfmm_tree_origin(t), t->D, t->order_s[d], k,
+                          ORIGIN(t), t->D, t->order_s[d], k,
                           (double *)t->mps[d]));
   return 0;
 }
@@ -2052,7 +2116,7 @@ static void hel_field_run(tree_dev *td, double k, const double *dxt,
   nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0) : 0;
   df = field_begin(f, nelem, &staged);
   HIPOK(wbfmm_hip_hel_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
-                            dp ? td->d_nrm : NULL, dp, lv, (double *)wbfmm_tree_origin(t),
+                            dp ? td->d_nrm : NULL, dp, lv, ORIGIN(t),
                             t->D, t->order_r[lv], k, (double *)t->mpr[lv], fl, df, fstr));
   field_finish(f, df, nelem, staged);
 }
@@ -2088,7 +2152,7 @@ gint wbfmm_tree_box_local_field(wbfmm_tree_t *t, guint level, guint b, gdouble k
     default:
       WBFMM_FATAL("unrecognized field identifier (%u)", field);
   }
-  stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc, (double *)wbfmm_tree_origin(t),
+  stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc, ORIGIN(t),
                 t->D, 0);
   if (!eval_neighbours) { src = NULL; d = NULL; }
   hel_field_run(td, k, dx, did, 1, src, sstr, d, dstr, fl, tmp, 8, 1);
@@ -2116,7 +2180,7 @@ gint wbfmm_helmholtz_field_targets(wbfmm_tree_t *t, gdouble k, gdouble *xt,
   if (fl == 0 || (field & ~3u)) return -2;
   if (fstr < ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0)) return -2;
   if (ntgt == 0) return 0;
-  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, (double *)wbfmm_tree_origin(t), t->D, 1);
+  stage_targets(xt, tstr, ntgt, &dx, &did, &dc, ORIGIN(t), t->D, 1);
   hel_field_run(td, k, dx, did, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
   HIPOK(wbfmm_hip_sync());
   wbfmm_hip_free(dx);
@@ -2199,8 +2263,8 @@ gint wbfmm_total_field_grad(gdouble k, gdouble *xs, gint xstride, gdouble *src,
 /* box width and FP32 matrix-core translations, orders 2..15 and the    */
 /* same order at every level, with FP64 recurrences and near field.     */
 /* Float arguments are host arrays, widened to double on the host;      */
-/* float outputs are accumulated in float.  The tree's origin is held   */
-/* in double.                                                           */
+/* float outputs are accumulated in float.  The public origin holds     */
+/* gfloat like the reference's; the passes use a double copy.           */
 /* ------------------------------------------------------------------ */
 
 /* n items of ncomp floats, item i at x + i * stride floats, into n *
@@ -2228,9 +2292,15 @@ wbfmm_tree_t *wbfmm_tree_new_f(gfloat *x, gfloat D, guint maxpoints)
 {
   double x0[3] = {x[0], x[1], x[2]};
   wbfmm_tree_t *t = wbfmm_tree_new(x0, (double)D, maxpoints);
+  gfloat *xf = (gfloat *)wbfmm_tree_origin(t);
   TD(t)->prec32 = 1;
   TD(t)->host_mirror = 0;
   t->size = sizeof(gfloat);
+  /* the public origin in the tree's precision (src/boxes.c:58-88) */
+  memset(t->x, 0, sizeof(t->x));
+  xf[0] = x[0];
+  xf[1] = x[1];
+  xf[2] = x[2];
   return t;
 }
 
@@ -2252,6 +2322,20 @@ gint wbfmm_tree_add_points_f(wbfmm_tree_t *t, gpointer pts, gsize pstr, gpointer
   t->normals = (char *)normals;
   t->nstr = nstr;
   return 0;
+}
+
+gint wbfmm_tree_sort_points_f(wbfmm_tree_t *t, gpointer pts, gsize psize, guint npts)
+{
+  double *xd;
+  gint rc;
+  require_f(t);
+  if (psize % sizeof(gfloat)) WBFMM_FATAL("point size must be whole floats");
+  /* the Morton codes from the coordinates widened to double, as in
+     wbfmm_tree_add_points_f */
+  xd = widen((const float *)pts, psize / sizeof(gfloat), 3, npts);
+  rc = sort_records(t, xd, 3 * sizeof(double), pts, psize, npts);
+  free(xd);
+  return rc;
 }
 
 gint wbfmm_tree_refine_f(wbfmm_tree_t *t)
