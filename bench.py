@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per lap_m2l launch (mean over the 6 launches of one solve) from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size;
@@ -153,7 +154,8 @@ def hel_workload(kind):
         xt = xs
         k, depth = 16.0, 7
         orders = [25, 25, 25, 23, 23, 23, 23, 23] + [23] * 9
-        desc = ("helmholtz_fp64_sphere: 1e6 points on the unit sphere, 50% monopoles / 50% radial "
+        desc = ("helmholtz_sphere (mixed precision: FP32 at levels with k a <= 2, FP64 above; "
+                "--precision 64 for all FP64): 1e6 points on the unit sphere, 50% monopoles / 50% radial "
                 "dipoles, complex N(0,1), k=16, depth 7, orders 25,23,23,23,23,23 (levels 2..7), "
                 "targets = sources, potential+gradient")
     lo = np.minimum(xs.min(0), xt.min(0))
@@ -176,9 +178,13 @@ def run_helmholtz(args):
     d_d = None if d is None else W.DeviceBuffer.from_array(c2(d))
     d_n = None if nrm is None else W.DeviceBuffer.from_array(np.ascontiguousarray(nrm))
     d_f = W.DeviceBuffer(nt * 64)
+    # the sphere configuration is defined in mixed precision (FP32 at the
+    # low-frequency levels); --precision 64 runs it all in FP64
+    prec = 32 if (args.workload == "helmholtz-sphere" and args.precision != "64") or \
+        args.precision == "32" else 64
     t0 = time.perf_counter()
     S = W.HelmholtzSolver(d_xs, x0, D, depth, orders, orders, k, normals=d_n, nsrc=n,
-                          targets=d_xt, ntgt=nt)
+                          targets=d_xt, ntgt=nt, precision=prec)
     W.sync()
     t_tables = time.perf_counter() - t0
 
@@ -203,7 +209,9 @@ def run_helmholtz(args):
     total = time.perf_counter() - t_all
     W.Timing.enable(False)
     kern = {}
-    for name in ("hel_m2l", "hel_m2m", "hel_l2l", "hel_p2m", "hel_l2p_p2p", "tree_sort"):
+    for name in ("hel_m2l", "hel_m2m", "hel_l2l", "hel_p2m", "hel_l2p_p2p", "tree_sort",
+                 "hel_m2l_f32", "hel_m2m_f32", "hel_l2l_f32", "hel_p2m_f32", "hel_l2p_p2p_f32",
+                 "hel_to_f32", "hel_to_f64"):
         cnt, ms = W.Timing.query(name)
         if cnt:
             kern[name] = {"launches_per_step": cnt / args.steps, "ms_per_step": ms / args.steps,
@@ -244,11 +252,21 @@ def run_helmholtz(args):
         Cn = (N + 1) ** 2 + 2 * sum((N + 1 - m) ** 2 for m in range(1, N + 1))
         return 2 * 4 * Rn + 8 * Cn
     m2l_flops_step = sum(pairs[l] * hel_flops_pair(orders[l]) for l in pairs)
-    m2l = kern.get("hel_m2l", {"ms_per_step": float("nan"), "launches_per_step": 1})
+    # mixed precision: the M2L time is that of both kernels; the peak is
+    # weighted by the flops each precision does (FP32 matrix peak 157.3)
+    m64 = kern.get("hel_m2l", {"ms_per_step": 0.0, "launches_per_step": 0})
+    m32 = kern.get("hel_m2l_f32", {"ms_per_step": 0.0, "launches_per_step": 0})
+    m2l = {"ms_per_step": m64["ms_per_step"] + m32["ms_per_step"],
+           "launches_per_step": max(m64["launches_per_step"] + m32["launches_per_step"], 1)}
+    ka_thr = float(os.environ.get("WBFMM_HEL_FP64_KA", "2.0"))
+    f32_flops = sum(pairs[l] * hel_flops_pair(orders[l]) for l in pairs
+                    if prec == 32 and k * 0.5 * np.sqrt(3.0) * D / 2 ** l <= ka_thr)
+    peak = FP64_PEAK_TFLOPS + (FP32_PEAK_TFLOPS - FP64_PEAK_TFLOPS) * f32_flops / m2l_flops_step
     achieved = m2l_flops_step / (m2l["ms_per_step"] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "kernel": "hel_m2l", "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "kernel": "hel_m2l" + ("+hel_m2l_f32" if prec == 32 else ""),
+                "flops_per_launch": m2l_flops_step / m2l["launches_per_step"],
                 "pairs_per_step": int(sum(pairs.values())),
                 "flops_per_pair": {str(l): hel_flops_pair(orders[l]) for l in pairs}}
     lat_ms = np.array(lat) * 1e3
@@ -257,7 +275,8 @@ def run_helmholtz(args):
         "value": nt * args.steps / total, "unit": "targets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * total / args.steps,
         "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32/f64 mixed" if prec == 32 else "f64",
         "data": "synthetic (SplitMix64 streams)",
         "config": {"workload": desc, "n_sources": n, "n_targets": nt, "depth": depth,
                    "k": k, "orders": orders[2:depth + 1], "parallelism": "single GPU"},

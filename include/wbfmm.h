@@ -380,12 +380,21 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op);
    set, keeping its rotation tables, so the set serves another wavenumber
    after new coaxial_SR_init / _SS_init calls */
 gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *op);
-/* precision of a Laplace tree's expansions, 64 (default) or 32, set
-   before the coefficients are initialised.  32: FP32 coefficients scaled
-   by each level's box width and FP32 matrix-core translations (orders
-   2..15, the same order at every level); Legendre and power recurrences
-   and the near field stay FP64, and inputs and outputs are FP64.  The
-   download returns the coefficients in the reference's scaling. */
+/* precision of a tree's expansions, 64 (default) or 32, set before the
+   coefficients are initialised; inputs and outputs stay FP64 and the
+   download returns the coefficients in the reference's scaling.
+   Laplace, 32: FP32 coefficients scaled by each level's box width and FP32
+   matrix-core translations (orders 2..15, the same order at every level);
+   Legendre and power recurrences and the near field stay FP64.
+   Helmholtz, 32 (mixed precision): levels with k a_l <= 2 (a_l the box
+   circumradius; WBFMM_HEL_FP64_KA sets the threshold) and orders 2..27
+   hold FP32 coefficients scaled per degree by s_n = min(1, (k a_l)^n /
+   (2n+1)!!) and run FP32 matrix-core translations and an FP32 near field
+   with compensated sums; the other levels (large k a_l, near the root)
+   stay FP64, and passes between the two run in FP64 on converted copies.
+   Operator tables and the Bessel/Legendre recurrences stay FP64.  The
+   precision of each level is fixed by the wavenumber of the leaf
+   expansions. */
 gint wbfmm_tree_set_precision(wbfmm_tree_t *t, gint bits);
 
 /* ---- single precision (src/wbfmm.h:432-440, :690-740, :996-1112) ------

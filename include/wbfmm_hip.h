@@ -56,6 +56,9 @@ int wbfmm_hip_memset(void *p, int v, size_t bytes);
 int wbfmm_hip_is_device(const void *p);
 int wbfmm_hip_sync(void);
 const char *wbfmm_hip_last_error(void);
+/* failure-handling tests: the calling thread's next n device allocations
+   (wbfmm_hip_malloc and the library's own) return WBFMM_HIP_ERR_NOMEM */
+int wbfmm_hip_debug_fail_allocs(int n);
 
 /* per-kernel timing with HIP events on the library stream:
    enable(1) starts recording every launch; query returns, for kernel
@@ -273,6 +276,12 @@ int wbfmm_hip_hel_rot_tiles(const wbfmm_m2l_plan_t *plan, int nth, const double 
                             double *A);
 int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *plan, int L, int nr,
                                  const double *E, double *A2);
+/* the same tiles with each output degree n scaled by sout[n] and each
+   input degree l by sin[l] (NULL: 1): the operators of coefficients held
+   as M_n / s_n (multipoles) or L_n s_n (local expansions) */
+int wbfmm_hip_hel_coax_tiles_scaled(const wbfmm_m2l_plan_t *plan, int L, int nr,
+                                    const double *E, const double *sout, const double *sin,
+                                    double *A2);
 /* out[i] = (float)in[i] on the device */
 int wbfmm_hip_to_f32(const double *in, float *out, size_t n);
 /* sort a box list by Morton child digit (stable); returns the per-class
@@ -421,6 +430,46 @@ int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
                         const double *dip, unsigned level, const double *x0,
                         double D, int N, double k, const double *mpr, int field,
                         double *f, int fstr);
+/* ---- mixed-precision Helmholtz (FP32 levels) -----------------------------
+   An FP32 level holds M_n / s_n (multipoles) and L_n s_n (local
+   expansions), s_n the level's per-degree scale (device array of N+2
+   doubles), as complex float pairs, box-major.  P2M scales and rounds on
+   store; the matrix-core passes are those above on FP32 tiles (operators
+   scaled per degree, wbfmm_hip_hel_coax_tiles_scaled); the field kernel
+   divides the scale out in FP64. */
+int wbfmm_hip_hel_p2m_f32(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                          const uint32_t *end, const double *xyz, const double *q,
+                          const double *nrm, const double *dip, unsigned level,
+                          const double *x0, double D, int N, double k, const double *scale,
+                          float *mps);
+int wbfmm_hip_hel_m2l_mfma_f32(const uint32_t *cls, const unsigned *cstart,
+                               const unsigned *ccount, const uint32_t *sstart,
+                               const uint32_t *send, unsigned level, int N, const float *mps,
+                               float *mpr, const void *plan_dev, const float *A, const float *A2,
+                               const float *CS, const wbfmm_m2l_plan_t *plan_host);
+int wbfmm_hip_hel_shift_mfma_f32(int mode, const uint32_t *parents, unsigned npar,
+                                 const uint32_t *cstart, const uint32_t *cend,
+                                 const uint32_t *cflag, int N, const float *in, float *out,
+                                 const void *plan_dev, const float *A, const float *A2,
+                                 const float *CS, const wbfmm_m2l_plan_t *plan_host);
+/* L2P + near field with one wavefront per target box (src/local.c:32-310):
+   tboxes lists the nbox leaf boxes holding targets, whose sorted targets
+   are [tstart[b], tend[b]) of xt (4 doubles each, caller index tid); the
+   27 neighbour boxes' sources pass through LDS.  prec 64: FP64 mpr and
+   near field; prec 32: FP32 mpr scaled by scale[n], FP32 near field with
+   compensated sums.  Output as wbfmm_hip_hel_field. */
+int wbfmm_hip_hel_field_box(const uint32_t *tboxes, unsigned nbox, const uint32_t *tstart,
+                            const uint32_t *tend, const double *xt, const uint32_t *tid,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip,
+                            unsigned level, const double *x0, double D, int N, double k,
+                            int prec, const void *mpr, const double *scale, int field,
+                            double *f, int fstr);
+/* level conversion for the listed boxes: to32 1: FP64 in -> FP32 out
+   (overwritten, or added to when add is 1), to32 0: FP32 in -> FP64 out;
+   degree n multiplied by scale[n] */
+int wbfmm_hip_hel_convert(int to32, int add, const uint32_t *boxes, unsigned nbox, int N,
+                          const void *in, const double *scale, void *out);
 /* FP64 Helmholtz direct sum (src/direct.c:35-294); pairs with r <= cutoff
    skipped; same output layout as wbfmm_hip_hel_field */
 int wbfmm_hip_hel_direct(const double *xyz, const double *q, const double *nrm,

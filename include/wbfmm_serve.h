@@ -15,8 +15,16 @@
  * stream (continuous batching: a finished solve frees its stream and its
  * device memory for the next request at once).  A request is admitted only
  * while the estimated device memory of all running requests, its own
- * included, stays within the budget; a request larger than the whole
- * budget runs alone.
+ * included, stays within the budget.  The estimate counts the coefficient
+ * arrays, points, targets and field, the largest transient buffer of a
+ * pass and the operator tables of the request's order.  A request whose
+ * estimate exceeds the whole budget is refused at submission
+ * (WBFMM_HIP_ERR_NOMEM).
+ *
+ * Failures: a solve that fails on the device (an allocation refused, a
+ * launch error) or that the drop-in API would abort on ends with a
+ * negative status in its result; its device objects are released and the
+ * engine goes on with the other requests.
  *
  * Results are deterministic: a request gives the same bits whether it
  * runs alone or among others.  All calls return 0 (or a ticket >= 0) on
@@ -111,6 +119,10 @@ gint wbfmm_serve_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t *r,
    (problem, precision, order, log2 of the point count) and return it */
 gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r,
                       const gint *depths, gint ndepth, gint *best);
+/* for failure-handling tests: the next request submitted has its first
+   count device allocations refused (as when memory runs out), so that it
+   ends with WBFMM_HIP_ERR_NOMEM in its result */
+gint wbfmm_serve_debug_fail_allocations(wbfmm_serve_t *s, gint count);
 /* finish the queued work and release the engine */
 gint wbfmm_serve_free(wbfmm_serve_t *s);
 

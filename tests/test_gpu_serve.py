@@ -268,10 +268,49 @@ def test_serve_fp32_path(S, oracle):
     with S.Server(2) as srv:
         f32, r32 = solve(S, srv, case, tol=1e-4, precision=0, check=300)
         f64, r64 = solve(S, srv, case, tol=1e-4, precision=64, check=300)
-        with pytest.raises(RuntimeError):   # FP32 is not offered for Helmholtz
-            solve(S, srv, hel_case(82, 500, 2.0), tol=1e-4, precision=32)
+        with pytest.raises(RuntimeError):   # no FP32 Laplace passes above order 15
+            solve(S, srv, lap_case(82, 500), tol=1e-10, precision=32)
     assert r32.precision == 32 and r64.precision == 64 and r32.order == r64.order
     for c in range(4):
         assert np.abs(f32[:, c] - f64[:, c]).max() <= 3e-5 * np.abs(f64[:, c]).max(), c
     assert 0 < r32.err_potential < 1e-4 and 0 < r64.err_potential < 1e-4
     assert abs(r32.err_potential - r64.err_potential) < 0.1 * r64.err_potential
+
+
+def test_serve_failed_request_reports_status_and_engine_survives(S):
+    # the first device allocations of a request refused (forced from the
+    # host, as when memory runs out): its result carries
+    # WBFMM_HIP_ERR_NOMEM, its objects are released and the engine serves
+    # the next requests, which give the same bits as a fresh engine
+    cases = [lap_case(91, 6000), hel_case(92, 3000, 3.0)]
+    kws = [dict(tol=1e-6), dict(tol=1e-4)]
+    with S.Server(2) as srv:
+        ref, _ = run_mix(S, srv, cases, kws)
+    with S.Server(2) as srv:
+        for nfail in (1, 3, 12):
+            for c, kw in zip(cases, kws):
+                f = out_array(S, c)
+                srv.debug_fail_allocations(nfail)
+                res = srv.wait(srv.submit(request(S, c, f, **kw), keep=f))
+                assert res.status == -5, (nfail, res.as_dict())
+        got, res = run_mix(S, srv, cases, kws)
+        st = srv.stats()
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+    assert st.completed == len(cases)
+
+
+def test_serve_refuses_request_above_budget(S):
+    # a request whose estimated device memory exceeds the whole budget is
+    # refused at submission; requests that fit still run
+    small, big = lap_case(93, 2000), lap_case(94, 60000)
+    with S.Server(2) as srv:
+        fp_small = srv.plan(request(S, small, out_array(S, small), tol=1e-6))[2]
+        fp_big = srv.plan(request(S, big, out_array(S, big), tol=1e-6))[2]
+    assert fp_big > fp_small
+    with S.Server(2, fp_small + 1) as srv:
+        f = out_array(S, big)
+        with pytest.raises(RuntimeError, match="-5"):
+            srv.submit(request(S, big, f, tol=1e-6), keep=f)
+        f, res = solve(S, srv, small, tol=1e-6)
+        assert res.status == 0

@@ -381,7 +381,8 @@ class HelmholtzSolver:
     targets.  Strengths are complex: (n, 2) float64 arrays (re, im)."""
 
     def __init__(self, xs, x0, D, depth, order_s, order_r, k, pstr=24,
-                 normals=None, nstr=24, nsrc=None, targets=None, ntgt=None, tstr=24):
+                 normals=None, nstr=24, nsrc=None, targets=None, ntgt=None, tstr=24,
+                 precision=64):
         L = lib()
         self.targets = targets
         self.ntgt = ntgt if ntgt is not None else (None if targets is None else len(targets))
@@ -402,6 +403,8 @@ class HelmholtzSolver:
             raise RuntimeError("wbfmm_tree_new failed")
         self.t = ctypes.cast(self.tree, _vp).value
         L.wbfmm_tree_host_mirror(self.t, 0)
+        if precision != 64:
+            L.wbfmm_tree_set_precision(self.t, precision)
         lmax = max(max(order_s[2:depth + 1]), max(order_r[2:depth + 1]))
         L.wbfmm_shift_angle_table_init()
         self.op = L.wbfmm_shift_operators_new(lmax, 0, None)

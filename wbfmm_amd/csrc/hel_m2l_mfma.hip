@@ -41,34 +41,40 @@
 
 namespace wbfmm {
 
+// coefficient types of a precision: complex pairs and MFMA result vectors
+template <typename T>
+using T2t = typename MF<T>::t2;
+template <typename T>
+using V4t = typename MF<T>::v4;
+
 // stage 2: complex coaxial tiles ar + i ai.  The imaginary product reads
 // the partner column, so that column 2t receives Ar Zr - Ai Zi and column
 // 2t+1 receives Ar Zi + Ai Zr.
-template <int N, int W, int WI>
-__device__ __forceinline__ void stage_cx(const double *__restrict__ ar,
-                                         const double *__restrict__ ai, const double *R,
-                                         int lane, int grp, int col, d4 *y) {
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_cx(const T *__restrict__ ar,
+                                         const T *__restrict__ ai, const T *R,
+                                         int lane, int grp, int col, V4t<T> *y) {
   constexpr Bins BN = M2LShape<N>::BN;
   constexpr WaveSeq WS = make_seq(BN, W, WI);
   constexpr int AD = WS.n < 8 ? (WS.n > 0 ? WS.n : 1) : 8;
-  double avr[AD], avi[AD];
+  T avr[AD], avi[AD];
 #pragma unroll
   for (int s = 0; s < AD && s < WS.n; s++) {
     avr[s] = ar[WS.t[s] * 64 + lane];
     avi[s] = ai[WS.t[s] * 64 + lane];
   }
-  const double sg = (col & 1) ? 1.0 : -1.0;
+  const T sg = (col & 1) ? T(1) : T(-1);
   const int pc = col ^ 1;
 #pragma unroll
   for (int lb = 0; lb < WS.nb; lb++) {
     const int b = WI + W * lb;
-    d4 r4 = {0.0, 0.0, 0.0, 0.0}, i4 = {0.0, 0.0, 0.0, 0.0};
+    V4t<T> r4 = {T(0), T(0), T(0), T(0)}, i4 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
       const int s = WS.start[lb] + j;
       const int row = (BN.kbase[b] + 4 * j + grp) * 16;
-      r4 = MF<double>::mma(avr[s % AD], R[row + col], r4);
-      i4 = MF<double>::mma(avi[s % AD], R[row + pc], i4);
+      r4 = MF<T>::mma(avr[s % AD], R[row + col], r4);
+      i4 = MF<T>::mma(avi[s % AD], R[row + pc], i4);
       if (s + AD < WS.n) {
         avr[s % AD] = ar[WS.t[s + AD] * 64 + lane];
         avi[s % AD] = ai[WS.t[s + AD] * 64 + lane];
@@ -78,40 +84,40 @@ __device__ __forceinline__ void stage_cx(const double *__restrict__ ar,
   }
 }
 
-// stage 2 of the double-buffered form: as stage_cx, results stored per bin
+// stage 2 of the T-buffered form: as stage_cx, results stored per bin
 // into the other buffer
-template <int N, int W, int WI>
-__device__ __forceinline__ void stage_cx_wo(const double *__restrict__ ar,
-                                            const double *__restrict__ ai, const double *Rin,
-                                            double *Rout, const int4 *tm, int lane, int grp,
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_cx_wo(const T *__restrict__ ar,
+                                            const T *__restrict__ ai, const T *Rin,
+                                            T *Rout, const int4 *tm, int lane, int grp,
                                             int col) {
   constexpr Bins BN = M2LShape<N>::BN;
   constexpr WaveSeq WS = make_seq(BN, W, WI);
   constexpr int AD = WS.n < 8 ? (WS.n > 0 ? WS.n : 1) : 8;
-  double avr[AD], avi[AD];
+  T avr[AD], avi[AD];
 #pragma unroll
   for (int s = 0; s < AD && s < WS.n; s++) {
     avr[s] = ar[WS.t[s] * 64 + lane];
     avi[s] = ai[WS.t[s] * 64 + lane];
   }
-  const double sg = (col & 1) ? 1.0 : -1.0;
+  const T sg = (col & 1) ? T(1) : T(-1);
   const int pc = col ^ 1;
 #pragma unroll
   for (int lb = 0; lb < WS.nb; lb++) {
     const int b = WI + W * lb;
-    d4 r4 = {0.0, 0.0, 0.0, 0.0}, i4 = {0.0, 0.0, 0.0, 0.0};
+    V4t<T> r4 = {T(0), T(0), T(0), T(0)}, i4 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
     for (int j = 0; j < BN.nk[b]; j++) {
       const int s = WS.start[lb] + j;
       const int row = (BN.kbase[b] + 4 * j + grp) * 16;
-      r4 = MF<double>::mma(avr[s % AD], Rin[row + col], r4);
-      i4 = MF<double>::mma(avi[s % AD], Rin[row + pc], i4);
+      r4 = MF<T>::mma(avr[s % AD], Rin[row + col], r4);
+      i4 = MF<T>::mma(avi[s % AD], Rin[row + pc], i4);
       if (s + AD < WS.n) {
         avr[s % AD] = ar[WS.t[s + AD] * 64 + lane];
         avi[s % AD] = ai[WS.t[s + AD] * 64 + lane];
       }
     }
-    const d4 y = r4 + sg * i4;
+    const V4t<T> y = r4 + sg * i4;
     const int4 d = tm[4 * b + grp];
     Rout[d.x * 16 + col] = y[0];
     Rout[d.y * 16 + col] = y[1];
@@ -120,23 +126,23 @@ __device__ __forceinline__ void stage_cx_wo(const double *__restrict__ ar,
   }
 }
 
-template <int N, int W, int WI>
+template <typename T, int N, int W, int WI>
 __device__ __forceinline__ void hel_m2l_loop(
-    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
-    double2 (*cs)[N + 1], unsigned (*vm)[7], const unsigned *anym, int lane, int col, int grp,
+    T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    T2t<T> (*cs)[N + 1], unsigned (*vm)[7], const unsigned *anym, int lane, int col, int grp,
     bool tvalid, uint32_t tb, int ti, int tj, int tk, int px, int py, int pz, int c,
     unsigned k8, int split, int nsplit, const ClassArgs &ca,
-    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
-    const double *__restrict__ A2, const double *__restrict__ CS,
-    const double2 *__restrict__ zero, const double2 *__restrict__ mps,
-    double2 *__restrict__ mpr, double2 *__restrict__ part, unsigned ntot) {
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A,
+    const T *__restrict__ A2, const T *__restrict__ CS,
+    const T2t<T> *__restrict__ zero, const T2t<T> *__restrict__ mps,
+    T2t<T> *__restrict__ mpr, T2t<T> *__restrict__ part, unsigned ntot) {
   using SH = M2LShape<N>;
   constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
   constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane
   constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
   constexpr int NBW = WS.nb;
-  constexpr bool DB = M2LDouble<double, N>::value;
-  double *R1 = R + (SH::S + 1) * 16;  // the second buffer (DB only)
+  constexpr bool DB = M2LDouble<T, N>::value;
+  T *R1 = R + (SH::S + 1) * 16;  // the second buffer (DB only)
   const int ubase = grp + 4 * WI, tg = col >> 1;
 
   auto next_k = [&](int k) {
@@ -152,22 +158,22 @@ __device__ __forceinline__ void hel_m2l_loop(
   auto source_of = [&](int k, int p) {
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     if (lane <= N) {
-      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
-      cs[p][lane] = make_double2(t[0], t[1]);
+      const T *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = mk2<T>(t[0], t[1]);
     }
     const bool v = (vm[tg][k >> 5] >> (k & 31)) & 1u;
     const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
                           : 0;
     return v ? mps + sb * (uint64_t)C : zero;
   };
-  double pre_re[NAW], pre_im[NAW];
+  T pre_re[NAW], pre_im[NAW];
   // x_{+m} on the real lane, x_{-m} on the imaginary one
-  auto load_sources = [&](const double2 *src, const int4 *tu, int ub, int sp) {
+  auto load_sources = [&](const T2t<T> *src, const int4 *tu, int ub, int sp) {
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const int4 inf = tu[u];
-      const double2 q = src[inf.x + (sp ? -inf.y : inf.y)];
+      const T2t<T> q = src[inf.x + (sp ? -inf.y : inf.y)];
       pre_re[i] = q.x;
       pre_im[i] = q.y;
     }
@@ -181,18 +187,18 @@ __device__ __forceinline__ void hel_m2l_loop(
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const int4 inf = tu[u];
-      const double2 e = cs[p][inf.y];
-      const double es = sp ? -e.y : e.y;
-      const double xr = e.x * pre_re[i] - es * pre_im[i];
-      const double xi = es * pre_re[i] + e.x * pre_im[i];
-      const double yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
+      const T2t<T> e = cs[p][inf.y];
+      const T es = sp ? -e.y : e.y;
+      const T xr = e.x * pre_re[i] - es * pre_im[i];
+      const T xi = es * pre_re[i] + e.x * pre_im[i];
+      const T yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
       if (ub + 4 * W * i >= NU) continue;
       // real lane: x' = x'_{+m}, partner x'_{-m}; imaginary lane: the reverse
       R[inf.z * 16 + cl] = sp ? yi + xi : xr + yr;
       R[inf.w * 16 + cl] = sp ? yi - xi : xr - yr;
     }
   };
-  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+  auto write_out = [&](const int4 *tm, const V4t<T> *y, int gp, int cl) {
 #pragma unroll
     for (int lb = 0; lb < NBW; lb++) {
       const int4 d = tm[4 * (WI + W * lb) + gp];
@@ -203,9 +209,9 @@ __device__ __forceinline__ void hel_m2l_loop(
     }
   };
 
-  double acc_re[NAW], acc_im[NAW];
+  T acc_re[NAW], acc_im[NAW];
 #pragma unroll
-  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = 0.0;
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = T(0);
 
   // k is the same in every lane (it comes from the shared anym): held as a
   // scalar, and the next offset's table indices are read one offset ahead
@@ -230,34 +236,34 @@ __device__ __forceinline__ void hel_m2l_loop(
       ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
     }
 
-    // double-buffered where it fits: R -> R1 -> R -> R1, one barrier per
+    // T-buffered where it fits: R -> R1 -> R -> R1, one barrier per
     // stage, and the next deposit goes into R while R1 is read below
-    double *RZ = R;
+    T *RZ = R;
     if constexpr (DB) {
-      stage_wo<double, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t1, ln, gp, cl);
+      stage_wo<T, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t1, ln, gp, cl);
       __syncthreads();
       if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
-      stage_cx_wo<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+      stage_cx_wo<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
                             R1, R, t2, ln, gp, cl);
       __syncthreads();
-      stage_wo<double, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t3, ln, gp, cl);
+      stage_wo<T, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t3, ln, gp, cl);
       __syncthreads();
       RZ = R1;
     } else {
-      d4 y[NBW > 0 ? NBW : 1];
-      stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      V4t<T> y[NBW > 0 ? NBW : 1];
+      stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
       __syncthreads();
       write_out(t1, y, gp, cl);
       __syncthreads();
       if (kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, sp);
 
-      stage_cx<N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+      stage_cx<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
                          R, ln, gp, cl, y);
       __syncthreads();
       write_out(t2, y, gp, cl);
       __syncthreads();
 
-      stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
       __syncthreads();
       write_out(t3, y, gp, cl);
       __syncthreads();
@@ -270,13 +276,13 @@ __device__ __forceinline__ void hel_m2l_loop(
       for (int i = 0; i < NAW; i++) {
         const int u = min(ub + 4 * W * i, NU - 1);
         const int nu = tu[u].y;
-        const double2 e = cs[p][nu];
-        const double es = sp ? -e.y : e.y;
-        const double pr = RZ[u * 16 + c0], pi = RZ[u * 16 + c0 + 1];
-        const double qr = nu ? RZ[(NU + u) * 16 + c0] : 0.0;
-        const double qi = nu ? RZ[(NU + u) * 16 + c0 + 1] : 0.0;
-        const double yr = 0.5 * (sp ? pr - qr : pr + qr);
-        const double yi = 0.5 * (sp ? pi - qi : pi + qi);
+        const T2t<T> e = cs[p][nu];
+        const T es = sp ? -e.y : e.y;
+        const T pr = RZ[u * 16 + c0], pi = RZ[u * 16 + c0 + 1];
+        const T qr = nu ? RZ[(NU + u) * 16 + c0] : T(0);
+        const T qi = nu ? RZ[(NU + u) * 16 + c0 + 1] : T(0);
+        const T yr = T(0.5) * (sp ? pr - qr : pr + qr);
+        const T yi = T(0.5) * (sp ? pi - qi : pi + qi);
         acc_re[i] += e.x * yr + es * yi;
         acc_im[i] += e.x * yi - es * yr;
       }
@@ -293,7 +299,7 @@ __device__ __forceinline__ void hel_m2l_loop(
 
   if (!tvalid) return;
   const int sp = col & 1;
-  double2 *dst = nsplit > 1 ? part + ((size_t)split * ntot + ca.start[c] + k8) * C
+  T2t<T> *dst = nsplit > 1 ? part + ((size_t)split * ntot + ca.start[c] + k8) * C
                             : mpr + (size_t)tb * C;
 #pragma unroll
   for (int i = 0; i < NAW; i++) {
@@ -303,27 +309,27 @@ __device__ __forceinline__ void hel_m2l_loop(
     if (sp && inf.y == 0) continue;  // L^0 is the real lane's
     const int r = inf.x + (sp ? -inf.y : inf.y);
     if (nsplit > 1) {
-      dst[r] = make_double2(acc_re[i], acc_im[i]);
+      dst[r] = mk2<T>(acc_re[i], acc_im[i]);
     } else {
-      const double2 o = dst[r];
-      dst[r] = make_double2(o.x + acc_re[i], o.y + acc_im[i]);
+      const T2t<T> o = dst[r];
+      dst[r] = mk2<T>(o.x + acc_re[i], o.y + acc_im[i]);
     }
   }
 }
 
-template <int N, int W>
+template <typename T, int N, int W>
 __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
     const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ, int level,
-    const double2 *__restrict__ mps, double2 *__restrict__ mpr,
-    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
-    const double *__restrict__ A2, const double *__restrict__ CS,
-    const double2 *__restrict__ zero, int nsplit, double2 *__restrict__ part, unsigned ntot) {
+    const T2t<T> *__restrict__ mps, T2t<T> *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A,
+    const T *__restrict__ A2, const T *__restrict__ CS,
+    const T2t<T> *__restrict__ zero, int nsplit, T2t<T> *__restrict__ part, unsigned ntot) {
   using SH = M2LShape<N>;
   constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
-  __shared__ double R[(M2LDouble<double, N>::value ? 2 : 1) * (S + 1) * 16];
+  __shared__ T R[(M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
-  __shared__ double2 cs[2][N + 1];
+  __shared__ T2t<T> cs[2][N + 1];
   __shared__ unsigned vm[8][7];
   __shared__ unsigned anym[7];
 
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
     const int b = i >> 2, g = i & 3;
     int t1[4], t2[4], t3[4];
     for (int q = 0; q < 4; q++) {
-      const int row = 16 * b + MF<double>::drow(g, q);
+      const int row = 16 * b + MF<T>::drow(g, q);
       t1[q] = P->map1[row] >> 6;
       t2[q] = P->map2[row] >> 6;
       t3[q] = P->map3[row] >> 6;
@@ -351,8 +357,8 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
     const int m = u - n * (n + 1) / 2;
     uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = tid; i < (M2LDouble<double, N>::value ? 2 : 1) * (S + 1) * 16; i += 64 * W)
-    R[i] = 0.0;
+  for (int i = tid; i < (M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16; i += 64 * W)
+    R[i] = T(0);
   for (int i = tid; i < 8 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
 
   const int split = (int)(blockIdx.x % (unsigned)nsplit);
@@ -390,7 +396,7 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
   __syncthreads();
 
 #define HEL_LOOP(WI)                                                                          \
-  hel_m2l_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid,  \
+  hel_m2l_loop<T, N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, vm, anym, lane, col, grp, tvalid,  \
                          tb, ti, tj, tk, px, py, pz, c, k8, split, nsplit, ca, P, A, A2, CS, \
                          zero, mps, mpr, part, ntot)
   static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
@@ -416,11 +422,11 @@ __global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
 #undef HEL_LOOP
 }
 
-template <int N>
+template <typename T, int N>
 static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
-                          const uint32_t *occ, int level, const double *mps, double *mpr,
-                          const void *plan, const double *A, const double *A2,
-                          const double *CS, const double *zero,
+                          const uint32_t *occ, int level, const T *mps, T *mpr,
+                          const void *plan, const T *A, const T *A2,
+                          const T *CS, const T *zero,
                           const wbfmm_m2l_plan_t *hplan) {
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): M2L plan for order %d does not match the kernel's bins\n", N);
@@ -433,20 +439,21 @@ static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nch
   unsigned ntot = 0;
   for (int c = 0; c < 8; c++) ntot += ca.count[c];
   const int nsplit = nchunks >= 1024 ? 1 : (int)(2048 / nchunks < 8 ? 2048 / nchunks : 8);
-  double2 *part = nullptr;
-  if (nsplit > 1 && dmalloc((void **)&part, sizeof(double2) * (size_t)nsplit * ntot * C))
+  T2t<T> *part = nullptr;
+  if (nsplit > 1 && dmalloc((void **)&part, sizeof(T2t<T>) * (size_t)nsplit * ntot * C))
     return WBFMM_HIP_ERR_NOMEM;
   // eight wavefronts per block from order 16, where four would need more
   // than 256 registers each (the LDS buffer allows one block per CU there)
   constexpr int W = N >= 16 ? 8 : 4;
-  WBFMM_LAUNCH("hel_m2l", (k_hel_m2l_mfma<N, W>), dim3(nchunks * nsplit), dim3(64 * W), 0, cls, ca,
-               occ, level, (const double2 *)mps, (double2 *)mpr,
-               (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero, nsplit, part,
+  WBFMM_LAUNCH(sizeof(T) == 4 ? "hel_m2l_f32" : "hel_m2l", (k_hel_m2l_mfma<T, N, W>),
+               dim3(nchunks * nsplit), dim3(64 * W), 0, cls, ca,
+               occ, level, (const T2t<T> *)mps, (T2t<T> *)mpr,
+               (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const T2t<T> *)zero, nsplit, part,
                ntot);
   if (nsplit > 1) {
     const size_t ne = (size_t)ntot * 2 * C;
-    WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<double>, dim3((unsigned)((ne + 255) / 256)),
-                 dim3(256), 0, cls, ntot, nsplit, 2 * C, (const double *)part, mpr);
+    WBFMM_LAUNCH("m2l_reduce", k_m2l_reduce<T>, dim3((unsigned)((ne + 255) / 256)),
+                 dim3(256), 0, cls, ntot, nsplit, 2 * C, (const T *)part, mpr);
     dfree(part);
   }
   return 0;
@@ -464,13 +471,13 @@ static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nch
 // in registers, in child order, and writes the parent; L2L translates each
 // parent's local expansion to each flagged child and adds it there.
 // ---------------------------------------------------------------------
-template <int N, int W, int WI, int MODE>
+template <typename T, int N, int W, int WI, int MODE>
 __device__ __forceinline__ void hel_shift_loop(
-    double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
-    const double2 (*cs)[N + 1], const unsigned *cmask, unsigned anyc, int lane, int col, int grp,
-    bool pvalid, uint64_t p, const double2 *__restrict__ in, double2 *__restrict__ out,
-    const double *__restrict__ A, const double *__restrict__ A2, const double *__restrict__ CS,
-    const double2 *__restrict__ zero) {
+    T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    const T2t<T> (*cs)[N + 1], const unsigned *cmask, unsigned anyc, int lane, int col, int grp,
+    bool pvalid, uint64_t p, const T2t<T> *__restrict__ in, T2t<T> *__restrict__ out,
+    const T *__restrict__ A, const T *__restrict__ A2, const T *__restrict__ CS,
+    const T2t<T> *__restrict__ zero) {
   using SH = M2LShape<N>;
   constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
   constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
@@ -478,13 +485,13 @@ __device__ __forceinline__ void hel_shift_loop(
   constexpr int NBW = WS.nb;
   const int ubase = grp + 4 * WI, tg = col >> 1;
 
-  double pre_re[NAW], pre_im[NAW];
-  auto load_sources = [&](const double2 *src, const int4 *tu, int ub, int sp) {
+  T pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const T2t<T> *src, const int4 *tu, int ub, int sp) {
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const int4 inf = tu[u];
-      const double2 q = src[inf.x + (sp ? -inf.y : inf.y)];
+      const T2t<T> q = src[inf.x + (sp ? -inf.y : inf.y)];
       pre_re[i] = q.x;
       pre_im[i] = q.y;
     }
@@ -495,17 +502,17 @@ __device__ __forceinline__ void hel_shift_loop(
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const int4 inf = tu[u];
-      const double2 e = cs[c][inf.y];
-      const double es = sp ? -e.y : e.y;
-      const double xr = e.x * pre_re[i] - es * pre_im[i];
-      const double xi = es * pre_re[i] + e.x * pre_im[i];
-      const double yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
+      const T2t<T> e = cs[c][inf.y];
+      const T es = sp ? -e.y : e.y;
+      const T xr = e.x * pre_re[i] - es * pre_im[i];
+      const T xi = es * pre_re[i] + e.x * pre_im[i];
+      const T yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
       if (ub + 4 * W * i >= NU) continue;
       R[inf.z * 16 + cl] = sp ? yi + xi : xr + yr;
       R[inf.w * 16 + cl] = sp ? yi - xi : xr - yr;
     }
   };
-  auto write_out = [&](const int4 *tm, const d4 *y, int gp, int cl) {
+  auto write_out = [&](const int4 *tm, const V4t<T> *y, int gp, int cl) {
 #pragma unroll
     for (int lb = 0; lb < NBW; lb++) {
       const int4 d = tm[4 * (WI + W * lb) + gp];
@@ -516,9 +523,9 @@ __device__ __forceinline__ void hel_shift_loop(
     }
   };
 
-  double acc_re[NAW], acc_im[NAW];
+  T acc_re[NAW], acc_im[NAW];
 #pragma unroll
-  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = 0.0;
+  for (int i = 0; i < NAW; i++) acc_re[i] = acc_im[i] = T(0);
   if (MODE == 1) load_sources(pvalid ? in + p * (uint64_t)C : zero, uinf, ubase, col & 1);
 
   for (int c = 0; c < 8; c++) {
@@ -534,47 +541,47 @@ __device__ __forceinline__ void hel_shift_loop(
     deposit(tu, ub, cl, c);
     __syncthreads();
 
-    d4 y[NBW > 0 ? NBW : 1];
-    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    V4t<T> y[NBW > 0 ? NBW : 1];
+    stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t1, y, gp, cl);
     __syncthreads();
-    stage_w<double, N, W, WI>(A2, R, ln, gp, cl, y);
+    stage_w<T, N, W, WI>(A2, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t2, y, gp, cl);
     __syncthreads();
-    stage_w<double, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+    stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
     __syncthreads();
     write_out(t3, y, gp, cl);
     __syncthreads();
 
     const int c0 = cl & ~1;
-    double2 *dst = out + (8 * p + c) * (uint64_t)C;
+    T2t<T> *dst = out + (8 * p + c) * (uint64_t)C;
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
       const int u = min(ub + 4 * W * i, NU - 1);
       const int4 inf = tu[u];
       const int nu = inf.y;
-      const double2 e = cs[c][nu];
-      const double es = sp ? -e.y : e.y;
-      const double pr = R[u * 16 + c0], pi = R[u * 16 + c0 + 1];
-      const double qr = nu ? R[(NU + u) * 16 + c0] : 0.0;
-      const double qi = nu ? R[(NU + u) * 16 + c0 + 1] : 0.0;
-      const double yr = 0.5 * (sp ? pr - qr : pr + qr);
-      const double yi = 0.5 * (sp ? pi - qi : pi + qi);
-      const double orr = e.x * yr + es * yi, oi = e.x * yi - es * yr;
+      const T2t<T> e = cs[c][nu];
+      const T es = sp ? -e.y : e.y;
+      const T pr = R[u * 16 + c0], pi = R[u * 16 + c0 + 1];
+      const T qr = nu ? R[(NU + u) * 16 + c0] : T(0);
+      const T qi = nu ? R[(NU + u) * 16 + c0 + 1] : T(0);
+      const T yr = T(0.5) * (sp ? pr - qr : pr + qr);
+      const T yi = T(0.5) * (sp ? pi - qi : pi + qi);
+      const T orr = e.x * yr + es * yi, oi = e.x * yi - es * yr;
       if (MODE == 0) {
         acc_re[i] += orr;
         acc_im[i] += oi;
       } else if (v && ub + 4 * W * i < NU && !(sp && nu == 0)) {
         const int r = inf.x + (sp ? -nu : nu);
-        const double2 o = dst[r];
-        dst[r] = make_double2(o.x + orr, o.y + oi);
+        const T2t<T> o = dst[r];
+        dst[r] = mk2<T>(o.x + orr, o.y + oi);
       }
     }
   }
   if (MODE == 0 && pvalid) {
-    double2 *dst = out + p * (uint64_t)C;
+    T2t<T> *dst = out + p * (uint64_t)C;
     const int sp = col & 1;
 #pragma unroll
     for (int i = 0; i < NAW; i++) {
@@ -582,25 +589,25 @@ __device__ __forceinline__ void hel_shift_loop(
       if (u >= NU) continue;
       const int4 inf = uinf[u];
       if (sp && inf.y == 0) continue;
-      dst[inf.x + (sp ? -inf.y : inf.y)] = make_double2(acc_re[i], acc_im[i]);
+      dst[inf.x + (sp ? -inf.y : inf.y)] = mk2<T>(acc_re[i], acc_im[i]);
     }
   }
 }
 
-template <int N, int W, int MODE>
+template <typename T, int N, int W, int MODE>
 __global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
     const uint32_t *__restrict__ parents, unsigned npar, const uint32_t *__restrict__ cstart,
     const uint32_t *__restrict__ cend, const uint32_t *__restrict__ cflag,
-    const double2 *__restrict__ in, double2 *__restrict__ out,
-    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A,
-    const double *__restrict__ A2, const double *__restrict__ CS,
-    const double2 *__restrict__ zero) {
+    const T2t<T> *__restrict__ in, T2t<T> *__restrict__ out,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A,
+    const T *__restrict__ A2, const T *__restrict__ CS,
+    const T2t<T> *__restrict__ zero) {
   using SH = M2LShape<N>;
   constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
-  __shared__ double R[(S + 1) * 16];
+  __shared__ T R[(S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
-  __shared__ double2 cs[8][N + 1];
+  __shared__ T2t<T> cs[8][N + 1];
   __shared__ unsigned cmask[8];
   __shared__ unsigned anyc;
 
@@ -610,7 +617,7 @@ __global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
     const int b = i >> 2, g = i & 3;
     int t1[4], t2[4], t3[4];
     for (int q = 0; q < 4; q++) {
-      const int row = 16 * b + MF<double>::drow(g, q);
+      const int row = 16 * b + MF<T>::drow(g, q);
       t1[q] = P->map1[row] >> 6;
       t2[q] = P->map2[row] >> 6;
       t3[q] = P->map3[row] >> 6;
@@ -625,9 +632,9 @@ __global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
     const int m = u - n * (n + 1) / 2;
     uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
   }
-  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = 0.0;
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R[i] = T(0);
   for (int i = tid; i < 8 * (N + 1); i += 64 * W)
-    cs[i / (N + 1)][i % (N + 1)] = make_double2(CS[2 * i], CS[2 * i + 1]);
+    cs[i / (N + 1)][i % (N + 1)] = mk2<T>(CS[2 * i], CS[2 * i + 1]);
   // children to translate, per parent column: occupied children (M2M) or
   // flagged ones (L2L)
   const unsigned pi = blockIdx.x * 8 + (col >> 1);
@@ -656,7 +663,7 @@ __global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
   const unsigned any = __builtin_amdgcn_readfirstlane(anyc);
 
 #define HEL_SHIFT(WI)                                                                        \
-  hel_shift_loop<N, W, WI, MODE>(R, tmap1, tmap2, tmap3, uinf, cs, cmask, any, lane, col,   \
+  hel_shift_loop<T, N, W, WI, MODE>(R, tmap1, tmap2, tmap3, uinf, cs, cmask, any, lane, col,   \
                                  grp, pvalid, p, in, out, A, A2, CS, zero)
   if constexpr (W == 4) {
     switch (wv) {
@@ -680,11 +687,11 @@ __global__ __launch_bounds__(64 * W) void k_hel_shift_mfma(
 #undef HEL_SHIFT
 }
 
-template <int N>
+template <typename T, int N>
 static int launch_hel_shift(int mode, const uint32_t *parents, unsigned npar,
                             const uint32_t *cstart, const uint32_t *cend, const uint32_t *cflag,
-                            const double *in, double *out, const void *plan, const double *A,
-                            const double *A2, const double *CS, const double *zero,
+                            const T *in, T *out, const void *plan, const T *A,
+                            const T *A2, const T *CS, const T *zero,
                             const wbfmm_m2l_plan_t *hplan) {
   if (!plan_matches<N>(hplan)) {
     fprintf(stderr, "wbfmm (HIP): shift plan for order %d does not match the kernel's bins\n", N);
@@ -693,27 +700,23 @@ static int launch_hel_shift(int mode, const uint32_t *parents, unsigned npar,
   constexpr int W = N >= 16 ? 8 : 4;
   const dim3 grid((npar + 7) / 8), block(64 * W);
   if (mode == 0)
-    WBFMM_LAUNCH("hel_m2m", (k_hel_shift_mfma<N, W, 0>), grid, block, 0, parents, npar, cstart,
-                 cend, cflag, (const double2 *)in, (double2 *)out,
-                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero);
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "hel_m2m_f32" : "hel_m2m", (k_hel_shift_mfma<T, N, W, 0>), grid,
+                 block, 0, parents, npar, cstart,
+                 cend, cflag, (const T2t<T> *)in, (T2t<T> *)out,
+                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const T2t<T> *)zero);
   else
-    WBFMM_LAUNCH("hel_l2l", (k_hel_shift_mfma<N, W, 1>), grid, block, 0, parents, npar, cstart,
-                 cend, cflag, (const double2 *)in, (double2 *)out,
-                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const double2 *)zero);
+    WBFMM_LAUNCH(sizeof(T) == 4 ? "hel_l2l_f32" : "hel_l2l", (k_hel_shift_mfma<T, N, W, 1>), grid,
+                 block, 0, parents, npar, cstart,
+                 cend, cflag, (const T2t<T> *)in, (T2t<T> *)out,
+                 (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const T2t<T> *)zero);
   return 0;
 }
 
-}  // namespace wbfmm
-
-using namespace wbfmm;
-
-extern "C" {
-
-int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
-                           const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
-                           const double *mps, double *mpr, const void *plan_dev,
-                           const double *A, const double *A2, const double *CS,
-                           const wbfmm_m2l_plan_t *plan_host) {
+template <typename T>
+static int hel_m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
+                         const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
+                         const T *mps, T *mpr, const void *plan_dev, const T *A, const T *A2,
+                         const T *CS, const wbfmm_m2l_plan_t *plan_host) {
   ClassArgs ca;
   unsigned nch = 0;
   for (int c = 0; c < 8; c++) {
@@ -736,12 +739,13 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const un
     const int ro = occupancy_bits(sstart, send, level, &occ);
     if (ro) return ro;
   }
+  const T *zero = (const T *)zero_expansion();
   int rc = WBFMM_HIP_ERR_ORDER;
   switch (N) {
 #define CASE(n)                                                                             \
   case n:                                                                                   \
-    rc = launch_hel_m2l<n>(cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A, A2, CS,    \
-                           zero_expansion(), plan_host);                                    \
+    rc = launch_hel_m2l<T, n>(cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A, A2, CS, \
+                              zero, plan_host);                                             \
     break;
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
     CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
@@ -754,11 +758,11 @@ int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const un
   return rc;
 }
 
-int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
-                             const uint32_t *cstart, const uint32_t *cend,
-                             const uint32_t *cflag, int N, const double *in, double *out,
-                             const void *plan_dev, const double *A, const double *A2,
-                             const double *CS, const wbfmm_m2l_plan_t *plan_host) {
+template <typename T>
+static int hel_shift_entry(int mode, const uint32_t *parents, unsigned npar,
+                           const uint32_t *cstart, const uint32_t *cend, const uint32_t *cflag,
+                           int N, const T *in, T *out, const void *plan_dev, const T *A,
+                           const T *A2, const T *CS, const wbfmm_m2l_plan_t *plan_host) {
   if (npar == 0) return 0;
   if (mode != 0 && mode != 1) return WBFMM_HIP_ERR_ARG;
   if (mode == 0 && (!cstart || !cend)) return WBFMM_HIP_ERR_ARG;
@@ -768,11 +772,12 @@ int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
     const int rz = ensure_zero();
     if (rz) return rz;
   }
+  const T *zero = (const T *)zero_expansion();
   switch (N) {
-#define CASE(n)                                                                            \
-  case n:                                                                                  \
-    return launch_hel_shift<n>(mode, parents, npar, cstart, cend, cflag, in, out, plan_dev, \
-                               A, A2, CS, zero_expansion(), plan_host);
+#define CASE(n)                                                                               \
+  case n:                                                                                     \
+    return launch_hel_shift<T, n>(mode, parents, npar, cstart, cend, cflag, in, out, plan_dev, \
+                                  A, A2, CS, zero, plan_host);
     CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10)
     CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17) CASE(18) CASE(19)
     CASE(20) CASE(21) CASE(22) CASE(23) CASE(24) CASE(25) CASE(26) CASE(27)
@@ -780,6 +785,48 @@ int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
     default:
       return WBFMM_HIP_ERR_ORDER;
   }
+}
+
+}  // namespace wbfmm
+
+using namespace wbfmm;
+
+extern "C" {
+
+int wbfmm_hip_hel_m2l_mfma(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
+                           const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
+                           const double *mps, double *mpr, const void *plan_dev,
+                           const double *A, const double *A2, const double *CS,
+                           const wbfmm_m2l_plan_t *plan_host) {
+  return hel_m2l_entry<double>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev,
+                               A, A2, CS, plan_host);
+}
+
+int wbfmm_hip_hel_m2l_mfma_f32(const uint32_t *cls, const unsigned *cstart,
+                               const unsigned *ccount, const uint32_t *sstart,
+                               const uint32_t *send, unsigned level, int N, const float *mps,
+                               float *mpr, const void *plan_dev, const float *A, const float *A2,
+                               const float *CS, const wbfmm_m2l_plan_t *plan_host) {
+  return hel_m2l_entry<float>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev,
+                              A, A2, CS, plan_host);
+}
+
+int wbfmm_hip_hel_shift_mfma(int mode, const uint32_t *parents, unsigned npar,
+                             const uint32_t *cstart, const uint32_t *cend,
+                             const uint32_t *cflag, int N, const double *in, double *out,
+                             const void *plan_dev, const double *A, const double *A2,
+                             const double *CS, const wbfmm_m2l_plan_t *plan_host) {
+  return hel_shift_entry<double>(mode, parents, npar, cstart, cend, cflag, N, in, out, plan_dev,
+                                 A, A2, CS, plan_host);
+}
+
+int wbfmm_hip_hel_shift_mfma_f32(int mode, const uint32_t *parents, unsigned npar,
+                                 const uint32_t *cstart, const uint32_t *cend,
+                                 const uint32_t *cflag, int N, const float *in, float *out,
+                                 const void *plan_dev, const float *A, const float *A2,
+                                 const float *CS, const wbfmm_m2l_plan_t *plan_host) {
+  return hel_shift_entry<float>(mode, parents, npar, cstart, cend, cflag, N, in, out, plan_dev,
+                                A, A2, CS, plan_host);
 }
 
 }  // extern "C"

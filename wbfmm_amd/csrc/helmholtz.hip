@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hip_common.h"
+#include "m2l_mfma.h"
 
 namespace wbfmm {
 
@@ -221,13 +222,16 @@ constexpr int kP2MThreads = 256, kP2MBatch = 8;
 constexpr int kP2MCoef =
     ((WBFMM_HIP_MAX_ORDER + 1) * (WBFMM_HIP_MAX_ORDER + 1) + kP2MThreads - 1) / kP2MThreads;
 
+// O: double2 (FP64 coefficients) or float2 (FP32 coefficients, each degree
+// n divided by sc[n] in FP64 before rounding; sc NULL: no scaling)
+template <typename O>
 __global__ __launch_bounds__(256) void k_hel_p2m(
     const uint32_t *__restrict__ boxes, unsigned nbox,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
     const double *__restrict__ xyz, const double2 *__restrict__ q,
     const double *__restrict__ nrm, const double2 *__restrict__ dip,
     unsigned level, double x0, double y0, double z0, double D, int N,
-    double k, int pb, double2 *__restrict__ mps) {
+    double k, int pb, const double *__restrict__ sc, O *__restrict__ mps) {
   extern __shared__ double2 smem[];
   const int N1 = N + 1, tid = threadIdx.x, NT = (N1 + 1) * (N1 + 2) / 2;
   double2 *R = smem;                        // pb x NT: R_n^m, n <= N+1, m >= 0
@@ -313,11 +317,17 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
     }
     __syncthreads();  // R and jn are rewritten by the next batch
   }
-  double2 *out = mps + (size_t)b * N1 * N1;
+  O *out = mps + (size_t)b * N1 * N1;
 #pragma unroll
   for (int i = 0; i < kP2MCoef; i++) {
     const int c = tid + kP2MThreads * i;
-    if (c < N1 * N1) out[c] = cadd(out[c], acc[i]);
+    if (c < N1 * N1) {
+      const double s = sc ? 1.0 / sc[isqrt_deg(c)] : 1.0;
+      O o = out[c];
+      o.x += acc[i].x * s;
+      o.y += acc[i].y * s;
+      out[c] = o;
+    }
   }
 }
 
@@ -581,6 +591,242 @@ __global__ __launch_bounds__(128) void k_hel_field(
 }
 
 // ---------------------------------------------------------------------
+// L2P + near field with one wavefront per target box (the same sums as
+// k_hel_field).  Lane i takes target t0 + i of the box (64 at a time);
+// the sources of the 27 neighbour boxes pass through LDS 64 at a time and
+// every lane runs the same source loop, so the loads are shared and the
+// lanes do not diverge.  T = double: FP64 throughout.  T = float: the
+// local expansion holds L_n^m s_n in FP32 (sc: the level's scales s_n,
+// divided out in FP64), the regular functions and the L2P sums are FP64,
+// and the near-field kernel runs in FP32 (offsets x - y formed in FP64,
+// the 1e-6 cutoff tested in FP64) with compensated (Neumaier) sums.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ void sincos_t(double a, double *s, double *c) { sincos(a, s, c); }
+__device__ __forceinline__ void sincos_t(float a, float *s, float *c) { sincosf(a, s, c); }
+
+template <typename T>
+struct CSum {  // compensated complex accumulator
+  T sx, sy, cx, cy;
+  __device__ __forceinline__ void zero() { sx = sy = cx = cy = T(0); }
+  __device__ __forceinline__ static void add1(T &s, T &c, T v) {
+    const T t = s + v;
+    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ void add(T a, T b) {
+    add1(sx, cx, a);
+    add1(sy, cy, b);
+  }
+  __device__ __forceinline__ double2 value() const {
+    return make_double2((double)sx + (double)cx, (double)sy + (double)cy);
+  }
+};
+
+template <typename T, int NMAX>
+__global__ __launch_bounds__(64) void k_hel_field_box(
+    const uint32_t *__restrict__ tboxes, unsigned nbox, const uint32_t *__restrict__ tstart,
+    const uint32_t *__restrict__ tend, const double *__restrict__ xt,
+    const uint32_t *__restrict__ tid, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ end, const double *__restrict__ xyz,
+    const double2 *__restrict__ q, const double *__restrict__ nrm,
+    const double2 *__restrict__ dip, unsigned level, double x0, double y0, double z0,
+    double D, int N, double k, const typename MF<T>::t2 *__restrict__ mpr,
+    const double *__restrict__ sc, int field, double *__restrict__ f, int fstr) {
+  typedef typename MF<T>::t2 T2;
+  __shared__ double sxs[64][3];
+  __shared__ double2 sqs[64], sds[64];
+  __shared__ double sns[64][3];
+  __shared__ double sinv[NMAX + 2];
+  const int lane = threadIdx.x;
+  const unsigned bi = blockIdx.x;
+  if (bi >= nbox) return;  // uniform over the block
+  for (int n = lane; n <= N + 1; n += 64) sinv[n] = sc ? 1.0 / sc[n] : 1.0;
+  const uint32_t b = tboxes[bi], t0 = tstart[b], t1 = tend[b];
+  const double w = D / (double)(1u << level);
+  const int ci = compact3(b), cj = compact3((uint64_t)b >> 1), ck = compact3((uint64_t)b >> 2);
+  const double xc = x0 + (ci + 0.5) * w, yc = y0 + (cj + 0.5) * w, zc = z0 + (ck + 0.5) * w;
+  const T2 *Lc = mpr + (size_t)b * (N + 1) * (N + 1);
+  const int nb = 1 << level;
+  __syncthreads();
+  for (uint32_t tb0 = t0; tb0 < t1; tb0 += 64) {
+    const uint32_t it = tb0 + lane;
+    const bool act = it < t1;
+    const uint32_t ii = act ? it : t0;
+    const double x = xt[4 * (size_t)ii], y = xt[4 * (size_t)ii + 1], z = xt[4 * (size_t)ii + 2];
+    double2 pot = make_double2(0.0, 0.0), gz = pot, dpl = pot, dmi = pot;
+    auto Lg = [&](int nn, int mm) -> double2 {
+      if (nn < 0 || nn > N || mm > nn || -mm > nn) return make_double2(0.0, 0.0);
+      const T2 v = Lc[hidx(nn, mm)];
+      return make_double2((double)v.x * sinv[nn], (double)v.y * sinv[nn]);
+    };
+    {
+      double r, C, S, cp, sp, jn[NMAX + 2];
+      sph(x - xc, y - yc, z - zc, r, C, S, cp, sp);
+      bessel_j(N + 1, k * r, jn);
+      const double phi = atan2(sp, cp);
+      double pmm = 1.0 / sqrt(4.0 * kPi);
+      for (int m = 0; m <= N + 1; m++) {
+        if (m > 0) pmm *= sqrt((2.0 * m + 1) / (2.0 * m)) * S;
+        double sn, cs;
+        sincos((double)m * phi, &sn, &cs);
+        double p = pmm, pprev = 0.0;
+        for (int nn = m; nn <= N + 1; nn++) {
+          const double a = jn[nn] * p;
+          for (int sg = 1; sg >= -1; sg -= 2) {
+            if (m == 0 && sg < 0) break;
+            const int mm = sg * m;
+            const double2 R = make_double2(a * cs, sg * a * sn);
+            if ((field & 1) && nn <= N) pot = cadd(pot, cmul(Lg(nn, mm), R));
+            if (field & 2) {
+              const double2 cz = cadd(cscale(Lg(nn + 1, mm), rec_a(nn, mm)),
+                                      cscale(Lg(nn - 1, mm), -rec_a(nn - 1, mm)));
+              const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), rec_b(nn, -mm)),
+                                       cscale(Lg(nn + 1, mm - 1), -rec_b(nn + 1, mm - 1)));
+              const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), rec_b(nn, mm)),
+                                       cscale(Lg(nn + 1, mm + 1), -rec_b(nn + 1, -mm - 1)));
+              gz = cadd(gz, cmul(cz, R));
+              dpl = cadd(dpl, cmul(cpz, R));
+              dmi = cadd(dmi, cmul(cmz, R));
+            }
+          }
+          const double pn = pbar_up(nn, m, C, p, pprev);
+          pprev = p;
+          p = pn;
+        }
+      }
+    }
+    double2 gx = cscale(cadd(dpl, dmi), 0.5 * k);
+    const double2 dd = cscale(cadd(dpl, cscale(dmi, -1.0)), 0.5 * k);
+    double2 gy = make_double2(dd.y, -dd.x);
+    gz = cscale(gz, k);
+    // near field in precision T
+    CSum<T> apot, agx, agy, agz;
+    apot.zero();
+    agx.zero();
+    agy.zero();
+    agz.zero();
+    const T kk = (T)k, c4 = (T)(1.0 / (4.0 * kPi * k));
+    for (int a = -1; a <= 1; a++)
+      for (int bb = -1; bb <= 1; bb++)
+        for (int c = -1; c <= 1; c++) {
+          const int si = ci + a, sj = cj + bb, sk = ck + c;
+          if (si < 0 || si >= nb || sj < 0 || sj >= nb || sk < 0 || sk >= nb) continue;
+          const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+          const uint32_t s0 = start[sb], s1 = end[sb];
+          for (uint32_t sc0 = s0; sc0 < s1; sc0 += 64) {
+            const uint32_t sl = sc0 + lane;
+            __syncthreads();  // the previous tile's reads are done
+            if (sl < s1) {
+              sxs[lane][0] = xyz[4 * (size_t)sl];
+              sxs[lane][1] = xyz[4 * (size_t)sl + 1];
+              sxs[lane][2] = xyz[4 * (size_t)sl + 2];
+              sqs[lane] = q ? q[sl] : make_double2(0.0, 0.0);
+              if (dip) {
+                sds[lane] = dip[sl];
+                sns[lane][0] = nrm[4 * (size_t)sl];
+                sns[lane][1] = nrm[4 * (size_t)sl + 1];
+                sns[lane][2] = nrm[4 * (size_t)sl + 2];
+              }
+            }
+            __syncthreads();
+            const int cnt = (int)min(64u, s1 - sc0);
+            for (int j = 0; j < cnt; j++) {
+              const double rxd = x - sxs[j][0], ryd = y - sxs[j][1], rzd = z - sxs[j][2];
+              const double r2 = rxd * rxd + ryd * ryd + rzd * rzd;
+              if (r2 <= 1e-12) continue;
+              const T rx = (T)rxd, ry = (T)ryd, rz = (T)rzd;
+              const T r = sqrt((T)r2), ir = T(1) / r;
+              T sn, cs;
+              sincos_t(kk * r, &sn, &cs);
+              // G = (sin - i cos) / (4 pi k r), G (ik - 1/r), G ((ik - 1/r)^2 + 1/r^2)
+              const T g0 = c4 * ir;
+              const T Gr = sn * g0, Gi = -cs * g0;
+              const T dGr = -Gr * ir - Gi * kk, dGi = Gr * kk - Gi * ir;
+              const T ux = rx * ir, uy = ry * ir, uz = rz * ir;
+              if (q) {
+                const T qr = (T)sqs[j].x, qi = (T)sqs[j].y;
+                apot.add(qr * Gr - qi * Gi, qr * Gi + qi * Gr);
+                const T qdr = qr * dGr - qi * dGi, qdi = qr * dGi + qi * dGr;
+                agx.add(qdr * ux, qdi * ux);
+                agy.add(qdr * uy, qdi * uy);
+                agz.add(qdr * uz, qdi * uz);
+              }
+              if (dip) {
+                const T dr = (T)sds[j].x, di = (T)sds[j].y;
+                const T nx = (T)sns[j][0], ny = (T)sns[j][1], nz = (T)sns[j][2];
+                const T nr = nx * ux + ny * uy + nz * uz;
+                const T ddr = dr * dGr - di * dGi, ddi = dr * dGi + di * dGr;
+                apot.add(ddr * nr, ddi * nr);
+                // (ik - 1/r)^2 + 1/r^2 = 2/r^2 - k^2 - 2ik/r
+                const T e2r = T(2) * ir * ir - kk * kk, e2i = -T(2) * kk * ir;
+                const T g2r = Gr * e2r - Gi * e2i, g2i = Gr * e2i + Gi * e2r;
+                const T t1r = (dr * g2r - di * g2i) * nr, t1i = (dr * g2i + di * g2r) * nr;
+                const T t2r = ddr * ir, t2i = ddi * ir;
+                agx.add(t1r * ux + t2r * (nx - nr * ux), t1i * ux + t2i * (nx - nr * ux));
+                agy.add(t1r * uy + t2r * (ny - nr * uy), t1i * uy + t2i * (ny - nr * uy));
+                agz.add(t1r * uz + t2r * (nz - nr * uz), t1i * uz + t2i * (nz - nr * uz));
+              }
+            }
+          }
+        }
+    pot = cadd(pot, apot.value());
+    gx = cadd(gx, agx.value());
+    gy = cadd(gy, agy.value());
+    gz = cadd(gz, agz.value());
+    if (act) {
+      double *o = f + (size_t)tid[it] * fstr;
+      int at = 0;
+      if (field & 1) {
+        o[0] += pot.x;
+        o[1] += pot.y;
+        at = 2;
+      }
+      if (field & 2) {
+        o[at + 0] += gx.x;
+        o[at + 1] += gx.y;
+        o[at + 2] += gy.x;
+        o[at + 3] += gy.y;
+        o[at + 4] += gz.x;
+        o[at + 5] += gz.y;
+      }
+    }
+  }
+}
+
+// out (FP32, box-major, complex) = in (FP64) with degree n multiplied by
+// sc[n], for the listed boxes; and the reverse (FP32 -> FP64, degree n
+// multiplied by sc[n]: pass 1/s to unscale)
+__global__ void k_hel_to_f32(const uint32_t *__restrict__ boxes, unsigned nbox, int N,
+                             const double2 *__restrict__ in, const double *__restrict__ sc,
+                             int add, float2 *__restrict__ out) {
+  const int C = (N + 1) * (N + 1);
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)nbox * C) return;
+  const size_t bx = e / C;
+  const int c = (int)(e % C);
+  const size_t at = (size_t)boxes[bx] * C + c;
+  const double s = sc[isqrt_deg(c)];
+  if (add)
+    out[at] = make_float2((float)((double)out[at].x + in[at].x * s),
+                          (float)((double)out[at].y + in[at].y * s));
+  else
+    out[at] = make_float2((float)(in[at].x * s), (float)(in[at].y * s));
+}
+
+__global__ void k_hel_to_f64(const uint32_t *__restrict__ boxes, unsigned nbox, int N,
+                             const float2 *__restrict__ in, const double *__restrict__ sc,
+                             double2 *__restrict__ out) {
+  const int C = (N + 1) * (N + 1);
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)nbox * C) return;
+  const size_t bx = e / C;
+  const int c = (int)(e % C);
+  const size_t at = (size_t)boxes[bx] * C + c;
+  const double s = sc[isqrt_deg(c)];
+  out[at] = make_double2((double)in[at].x * s, (double)in[at].y * s);
+}
+
+// ---------------------------------------------------------------------
 // FP64 direct sum (src/direct.c:35-294): thread per target, sources
 // streamed through LDS; same output layout as k_hel_field
 // ---------------------------------------------------------------------
@@ -678,9 +924,25 @@ int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *star
   const size_t per = sizeof(double2) * (size_t)(N + 2) * (N + 3) / 2 + sizeof(double) * (N + 2);
   int pb = (int)((60 * 1024) / per);
   pb = pb < 1 ? 1 : (pb > kP2MBatch ? kP2MBatch : pb);
-  WBFMM_LAUNCH("hel_p2m", k_hel_p2m, dim3(nbox), dim3(kP2MThreads), per * pb, boxes, nbox,
-               start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level, x0[0],
-               x0[1], x0[2], D, N, k, pb, (double2 *)mps);
+  WBFMM_LAUNCH("hel_p2m", k_hel_p2m<double2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
+               nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
+               x0[0], x0[1], x0[2], D, N, k, pb, (const double *)nullptr, (double2 *)mps);
+  return 0;
+}
+
+int wbfmm_hip_hel_p2m_f32(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                          const uint32_t *end, const double *xyz, const double *q,
+                          const double *nrm, const double *dip, unsigned level,
+                          const double *x0, double D, int N, double k, const double *scale,
+                          float *mps) {
+  if (N < 0 || N > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
+  if (nbox == 0) return 0;
+  const size_t per = sizeof(double2) * (size_t)(N + 2) * (N + 3) / 2 + sizeof(double) * (N + 2);
+  int pb = (int)((60 * 1024) / per);
+  pb = pb < 1 ? 1 : (pb > kP2MBatch ? kP2MBatch : pb);
+  WBFMM_LAUNCH("hel_p2m_f32", k_hel_p2m<float2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
+               nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
+               x0[0], x0[1], x0[2], D, N, k, pb, scale, (float2 *)mps);
   return 0;
 }
 
@@ -717,6 +979,51 @@ int wbfmm_hip_hel_m2l(const uint32_t *targets, unsigned ntgt, const uint32_t *ss
   WBFMM_LAUNCH("hel_m2l", k_hel_m2l, dim3(ntgt), dim3(64), lds_vectors(L, 5), targets,
                ntgt, sstart, send, (int)level, Ns, Nr, (const double2 *)mps,
                (double2 *)mpr, Hth, nrot, th_idx, chi, rix, (const double2 *)SR, ne);
+  return 0;
+}
+
+int wbfmm_hip_hel_field_box(const uint32_t *tboxes, unsigned nbox, const uint32_t *tstart,
+                            const uint32_t *tend, const double *xt, const uint32_t *tid,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip,
+                            unsigned level, const double *x0, double D, int N, double k,
+                            int prec, const void *mpr, const double *scale, int field,
+                            double *f, int fstr) {
+  if (nbox == 0) return 0;
+  if (N < 0 || N > WBFMM_HIP_MAX_ORDER) return WBFMM_HIP_ERR_ORDER;
+  if (prec != 32 && prec != 64) return WBFMM_HIP_ERR_ARG;
+  if (prec == 32 && !scale) return WBFMM_HIP_ERR_ARG;
+#define HB(TT, NM)                                                                             \
+  WBFMM_LAUNCH(prec == 32 ? "hel_l2p_p2p_f32" : "hel_l2p_p2p", (k_hel_field_box<TT, NM>),     \
+               dim3(nbox), dim3(64), 0, tboxes, nbox, tstart, tend, xt, tid, start, end, xyz, \
+               (const double2 *)q, nrm, (const double2 *)dip, level, x0[0], x0[1], x0[2], D, \
+               N, k, (const typename MF<TT>::t2 *)mpr, scale, field, f, fstr)
+  if (prec == 32) {
+    if (N <= 16) HB(float, 16);
+    else if (N <= 24) HB(float, 24);
+    else if (N <= 32) HB(float, 32);
+    else HB(float, WBFMM_HIP_MAX_ORDER);
+  } else {
+    if (N <= 16) HB(double, 16);
+    else if (N <= 24) HB(double, 24);
+    else if (N <= 32) HB(double, 32);
+    else HB(double, WBFMM_HIP_MAX_ORDER);
+  }
+#undef HB
+  return 0;
+}
+
+int wbfmm_hip_hel_convert(int to32, int add, const uint32_t *boxes, unsigned nbox, int N,
+                          const void *in, const double *scale, void *out) {
+  if (nbox == 0) return 0;
+  if (N < 0 || N > WBFMM_HIP_MAX_ORDER || !scale) return WBFMM_HIP_ERR_ARG;
+  const size_t ne = (size_t)nbox * (N + 1) * (N + 1);
+  if (to32)
+    WBFMM_LAUNCH("hel_to_f32", k_hel_to_f32, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
+                 boxes, nbox, N, (const double2 *)in, scale, add, (float2 *)out);
+  else
+    WBFMM_LAUNCH("hel_to_f64", k_hel_to_f64, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0,
+                 boxes, nbox, N, (const float2 *)in, scale, (double2 *)out);
   return 0;
 }
 

@@ -241,8 +241,16 @@ static inline unsigned nblk(size_t n, unsigned b) {
 }
 
 
+// allocations of this thread still to be refused (failure-handling tests)
+static thread_local int t_fail_allocs = 0;
+
 int dmalloc(void **p, size_t bytes) {
   *p = nullptr;
+  if (t_fail_allocs > 0) {
+    t_fail_allocs--;
+    return fail_msg("device allocation refused (forced by wbfmm_hip_debug_fail_allocs)",
+                    WBFMM_HIP_ERR_NOMEM);
+  }
   if (bytes == 0) bytes = 256;
   hipError_t e = hipMallocAsync(p, bytes, stream());
   if (e != hipSuccess) {
@@ -317,6 +325,11 @@ int wbfmm_hip_set_stream(void *s, void **prev) {
 void *wbfmm_hip_get_stream(void) { return (void *)stream(); }
 
 const char *wbfmm_hip_last_error(void) { return g_err; }
+
+int wbfmm_hip_debug_fail_allocs(int n) {
+  t_fail_allocs = n > 0 ? n : 0;
+  return 0;
+}
 
 int wbfmm_hip_malloc(void **p, size_t bytes) {
   if (wbfmm_hip_init()) return WBFMM_HIP_ERR_RUNTIME;

@@ -16,6 +16,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <pthread.h>
+#include <setjmp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -26,21 +27,83 @@
 #define M_PI 3.14159265358979323846
 #endif
 
-/* abort on misuse, like the reference's g_error() */
-#define WBFMM_FATAL(...)                              \
-  do {                                                \
-    fprintf(stderr, "wbfmm: %s: ", __func__);         \
-    fprintf(stderr, __VA_ARGS__);                     \
-    fprintf(stderr, "\n");                            \
-    abort();                                          \
+/* Abort on misuse, like the reference's g_error(), unless the calling
+   thread runs under wbfmm_guarded_call (the serving engine's workers):
+   then the locks this library holds in the thread are released and the
+   guarded call returns the failure as a status instead. */
+static __thread jmp_buf *t_recover = NULL;
+static __thread int t_recover_rc = 0;
+static __thread char t_fatal_msg[512];
+#define MAX_HELD 8
+static __thread pthread_mutex_t *t_held[MAX_HELD];
+static __thread int t_nheld = 0;
+
+static void mu_lock(pthread_mutex_t *m)
+{
+  pthread_mutex_lock(m);
+  if (t_nheld < MAX_HELD) t_held[t_nheld] = m;
+  t_nheld++;
+}
+
+static void mu_unlock(pthread_mutex_t *m)
+{
+  int i;
+  for (i = (t_nheld < MAX_HELD ? t_nheld : MAX_HELD) - 1; i >= 0; i--)
+    if (t_held[i] == m) {
+      for (; i + 1 < t_nheld && i + 1 < MAX_HELD; i++) t_held[i] = t_held[i + 1];
+      break;
+    }
+  t_nheld--;
+  pthread_mutex_unlock(m);
+}
+
+static void fatal_exit(int rc) __attribute__((noreturn));
+static void fatal_exit(int rc)
+{
+  if (t_recover) {
+    jmp_buf *jb = t_recover;
+    while (t_nheld > 0) {
+      t_nheld--;
+      if (t_nheld < MAX_HELD) pthread_mutex_unlock(t_held[t_nheld]);
+    }
+    t_recover_rc = rc ? rc : WBFMM_HIP_ERR_ARG;
+    longjmp(*jb, 1);
+  }
+  abort();
+}
+
+#define WBFMM_FATAL_RC(rc, ...)                                                   \
+  do {                                                                            \
+    int _n = snprintf(t_fatal_msg, sizeof(t_fatal_msg), "wbfmm: %s: ", __func__); \
+    if (_n > 0 && _n < (int)sizeof(t_fatal_msg))                                  \
+      snprintf(t_fatal_msg + _n, sizeof(t_fatal_msg) - _n, __VA_ARGS__);          \
+    fprintf(stderr, "%s\n", t_fatal_msg);                                         \
+    fatal_exit(rc);                                                               \
+  } while (0)
+#define WBFMM_FATAL(...) WBFMM_FATAL_RC(WBFMM_HIP_ERR_ARG, __VA_ARGS__)
+
+#define HIPOK(call)                                                              \
+  do {                                                                           \
+    int _rc = (call);                                                            \
+    if (_rc != 0)                                                                \
+      WBFMM_FATAL_RC(_rc, "%s failed (%d): %s", #call, _rc, wbfmm_hip_last_error()); \
   } while (0)
 
-#define HIPOK(call)                                                       \
-  do {                                                                    \
-    int _rc = (call);                                                     \
-    if (_rc != 0)                                                         \
-      WBFMM_FATAL("%s failed (%d): %s", #call, _rc, wbfmm_hip_last_error()); \
-  } while (0)
+int wbfmm_guarded_call(int (*fn)(void *), void *arg, char *msg, size_t msglen)
+{
+  jmp_buf jb;
+  jmp_buf *volatile prev = t_recover;
+  volatile int rc = 0;
+  if (setjmp(jb) == 0) {
+    t_recover = &jb;
+    rc = fn(arg);
+  } else {
+    rc = t_recover_rc;
+    if (msg && msglen) snprintf(msg, msglen, "%s", t_fatal_msg);
+  }
+  t_recover = prev;
+  return rc;
+}
 
 /* ------------------------------------------------------------------ */
 /* internal state behind the public structs                           */
@@ -71,6 +134,15 @@ typedef struct {
      in the tree's own precision (gfloat for wbfmm_tree_new_f), as the
      reference does */
   double x0[3];
+  /* Helmholtz with prec32 (mixed precision): the wavenumber the levels
+     were set up for, the levels kept in FP64 (large k a_l, or orders the
+     FP32 passes do not serve), and the scale s_n of every FP32 level (host
+     and device copies, orders 0..N+1): FP32 levels hold M_n / s_n and
+     L_n s_n */
+  double hk;
+  int hk_set;
+  int h64[WBFMM_TREE_MAX_DEPTH + 1];
+  double *hsc[WBFMM_TREE_MAX_DEPTH + 1], *d_hsc[WBFMM_TREE_MAX_DEPTH + 1];
 } tree_dev;
 
 typedef struct {
@@ -139,6 +211,20 @@ typedef struct {
   double *ss_host[WBFMM_TREE_MAX_DEPTH + 1];
   double *hel_SS2[WBFMM_TREE_MAX_DEPTH + 1];
   int hel_SS2_N[WBFMM_TREE_MAX_DEPTH + 1];
+  /* mixed-precision Helmholtz: FP32 rotation tiles and phases per order,
+     and per level the scaled FP32 coaxial tiles of the M2L and of the
+     M2M into / L2L from the level above, with the order and wavenumber
+     they were made for */
+  float *f_hA[WBFMM_HIP_MAX_ORDER + 1], *f_hshA[WBFMM_HIP_MAX_ORDER + 1];
+  float *f_hCS[WBFMM_HIP_MAX_ORDER + 1], *f_hCSup[WBFMM_HIP_MAX_ORDER + 1],
+      *f_hCSdn[WBFMM_HIP_MAX_ORDER + 1];
+  float *f_hA2[WBFMM_TREE_MAX_DEPTH + 1], *f_hUp[WBFMM_TREE_MAX_DEPTH + 1],
+      *f_hDn[WBFMM_TREE_MAX_DEPTH + 1];
+  int f_hA2_N[WBFMM_TREE_MAX_DEPTH + 1], f_hUp_N[WBFMM_TREE_MAX_DEPTH + 1],
+      f_hDn_N[WBFMM_TREE_MAX_DEPTH + 1];
+  double f_hA2_k[WBFMM_TREE_MAX_DEPTH + 1], f_hUp_k[WBFMM_TREE_MAX_DEPTH + 1],
+      f_hDn_k[WBFMM_TREE_MAX_DEPTH + 1];
+  double f_hD; /* box width of the domain the scaled tiles follow */
 } ops_dev;
 
 /* orders served by the MFMA M2L kernel (wbfmm_hip_lap_m2l_mfma) */
@@ -151,6 +237,8 @@ typedef struct {
 #define SHIFT_MFMA_MAX 15
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
+static int level_f32(tree_dev *td, guint level);
+static void hel_prepare(wbfmm_tree_t *t, double k);
 /* origin of the domain cube in double, whatever the tree's precision */
 #define ORIGIN(_t) (TD(_t)->x0)
 
@@ -609,13 +697,30 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
   tree_dev *td = TD(t);
   void *src = which ? t->mpr[level] : t->mps[level];
   size_t bytes = which ? td->mpr_bytes[level] : td->mps_bytes[level];
-  size_t nb = (size_t)1 << (3 * level), C, g, c, k, es = td->prec32 ? sizeof(float) : sizeof(double);
+  const int f32 = level_f32(td, level);
+  size_t nb = (size_t)1 << (3 * level), C, g, c, k, es = f32 ? sizeof(float) : sizeof(double);
   double *tmp;
   if (!src) return -1;
   C = bytes / es / nb;
   tmp = malloc(nb * C * sizeof(double));
   if (!tmp) WBFMM_FATAL("out of host memory");
-  if (td->prec32) {
+  if (f32 && t->problem == WBFMM_PROBLEM_HELMHOLTZ) {
+    /* M_n = s_n M~_n, L_n = L~_n / s_n */
+    float *ft = malloc(bytes);
+    int n;
+    if (!ft) WBFMM_FATAL("out of host memory");
+    if (!td->hsc[level]) WBFMM_FATAL("level %u has no scales yet (no leaf expansions)", level);
+    HIPOK(wbfmm_hip_memcpy(ft, src, bytes));
+    for (g = 0; g < nb; g++)
+      for (c = 0; c < C; c++) {
+        const size_t cc = c / 2;
+        n = (int)sqrt((double)cc);
+        while ((size_t)(n + 1) * (n + 1) <= cc) n++;
+        while ((size_t)n * n > cc) n--;
+        tmp[g * C + c] = (double)ft[g * C + c] * (which ? 1.0 / td->hsc[level][n] : td->hsc[level][n]);
+      }
+    free(ft);
+  } else if (f32) {
     /* FP32 coefficients scaled by the box width w: M_n = w^n M~_n,
        L_n = L~_n / w^(n+1) */
     float *ft = malloc(bytes);
@@ -839,7 +944,7 @@ static void ensure_lap_coax(ops_dev *od)
 {
   int L = od->lay.L;
   if (od->d_rot) return;
-  pthread_mutex_lock(&g_tab_mu);
+  mu_lock(&g_tab_mu);
   if (!g_vec_rot[L]) {
     HIPOK(wbfmm_hip_rotation_tables(L, &g_vec_rot[L], &g_vec_lay[L]));
     HIPOK(wbfmm_hip_laplace_coaxial_tables(L, &g_vec_coax[L]));
@@ -848,7 +953,7 @@ static void ensure_lap_coax(ops_dev *od)
   od->lay = g_vec_lay[L];
   od->d_lap_coax = g_vec_coax[L];
   od->op.rotations = od->d_rot;
-  pthread_mutex_unlock(&g_tab_mu);
+  mu_unlock(&g_tab_mu);
 }
 
 gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
@@ -865,6 +970,16 @@ gint wbfmm_shift_operators_free(wbfmm_shift_operators_t *op)
     free(od->sr_host[n]);
     wbfmm_hip_free(od->hel_SS2[n]);
     free(od->ss_host[n]);
+    wbfmm_hip_free(od->f_hA2[n]);
+    wbfmm_hip_free(od->f_hUp[n]);
+    wbfmm_hip_free(od->f_hDn[n]);
+  }
+  for (n = 0; n <= WBFMM_HIP_MAX_ORDER; n++) {
+    wbfmm_hip_free(od->f_hA[n]);
+    wbfmm_hip_free(od->f_hshA[n]);
+    wbfmm_hip_free(od->f_hCS[n]);
+    wbfmm_hip_free(od->f_hCSup[n]);
+    wbfmm_hip_free(od->f_hCSdn[n]);
   }
   /* d_hel_*, hel_A and hel_shA point to the process-wide tables: not freed */
   free(od);
@@ -919,11 +1034,11 @@ static void copy_order_tables(ops_dev *od, int N);
 static void ensure_plan(ops_dev *od, int N)
 {
   if (od->m2l_plan[N] && od->sh_CSup[N]) return;
-  pthread_mutex_lock(&g_ord_mu[N]);
+  mu_lock(&g_ord_mu[N]);
   build_m2l_plan(&g_tab, N);
   build_shift_tables(&g_tab, N);
   copy_order_tables(od, N);
-  pthread_mutex_unlock(&g_ord_mu[N]);
+  mu_unlock(&g_ord_mu[N]);
 }
 
 /* every matrix-core table of order N (FP64 M2L plan and tiles, M2M / L2L
@@ -931,7 +1046,7 @@ static void ensure_plan(ops_dev *od, int N)
 static void ensure_order_tables(ops_dev *od, int N)
 {
   if (od->m2l_have[N]) return;
-  pthread_mutex_lock(&g_ord_mu[N]);
+  mu_lock(&g_ord_mu[N]);
   if (!g_tab.f_A1[N]) {
     build_m2l_plan(&g_tab, N);
     build_lap_m2l_tiles(&g_tab, N);
@@ -941,7 +1056,7 @@ static void ensure_order_tables(ops_dev *od, int N)
   }
   copy_order_tables(od, N);
   od->m2l_have[N] = 1;
-  pthread_mutex_unlock(&g_ord_mu[N]);
+  mu_unlock(&g_ord_mu[N]);
 }
 
 static void copy_order_tables(ops_dev *od, int N)
@@ -1165,7 +1280,7 @@ static uint32_t *all_boxes(unsigned level, unsigned *n)
 {
   unsigned nb = 1u << (3 * level);
   uint32_t *list;
-  pthread_mutex_lock(&g_all_mu);
+  mu_lock(&g_all_mu);
   if (nb > g_all_n) {
     uint32_t *h = malloc(sizeof(uint32_t) * nb), *d = NULL;
     unsigned i;
@@ -1179,7 +1294,7 @@ static uint32_t *all_boxes(unsigned level, unsigned *n)
     g_all_n = nb;
   }
   list = g_all_boxes;
-  pthread_mutex_unlock(&g_all_mu);
+  mu_unlock(&g_all_mu);
   *n = nb;
   return list;
 }
@@ -1570,9 +1685,9 @@ gint wbfmm_target_list_local_coefficients(wbfmm_target_list_t *l, gdouble k,
 }
 
 static void hel_field_run(tree_dev *td, double k, const double *dxt,
-                          const uint32_t *dtid, unsigned n, double *src, gint sstr,
-                          double *d, gint dstr, int fl, double *f, gint fstr,
-                          size_t nf_rows);
+                          const uint32_t *dtid, const uint64_t *dcodes, unsigned n,
+                          double *src, gint sstr, double *d, gint dstr, int fl, double *f,
+                          gint fstr, size_t nf_rows);
 
 /* fused L2P + near field for every target of the list.  Laplace: f holds
    1 (potential) or 3 (gradient) values, 4 for both (potential first);
@@ -1590,8 +1705,8 @@ gint wbfmm_target_list_local_field(wbfmm_target_list_t *l, gdouble *src,
   if (l->t->problem == WBFMM_PROBLEM_HELMHOLTZ) {
     if (fstr < ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0))
       WBFMM_FATAL("field stride %d too small", fstr);
-    hel_field_run(td, ld->hel_k, ld->d_xt, ld->d_tid, l->npoints, src, sstr, d, dstr, fl, f,
-                  fstr, l->npoints);
+    hel_field_run(td, ld->hel_k, ld->d_xt, ld->d_tid, ld->d_codes, l->npoints, src, sstr, d,
+                  dstr, fl, f, fstr, l->npoints);
     return 0;
   }
   if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0))
@@ -1627,6 +1742,8 @@ gint wbfmm_tree_free(wbfmm_tree_t *t)
     wbfmm_hip_free(t->mps[l]);
     wbfmm_hip_free(t->mpr[l]);
     free(t->boxes[l]);
+    free(td->hsc[l]);
+    wbfmm_hip_free(td->d_hsc[l]);
   }
   wbfmm_hip_free(td->d_xyz);
   wbfmm_hip_free(td->d_xyz_raw);
@@ -1650,9 +1767,8 @@ gint wbfmm_hip_synchronize(void) { return wbfmm_hip_sync(); }
    complex per box (box-major on the device) */
 gint wbfmm_tree_coefficient_init(wbfmm_tree_t *t, guint l, guint nr, guint ns)
 {
-  if (TD(t)->prec32) WBFMM_FATAL("FP32 coefficients are available for Laplace trees only");
   tree_dev *td = TD(t);
-  size_t nb = (size_t)1 << (3 * l);
+  size_t nb = (size_t)1 << (3 * l), es = td->prec32 ? sizeof(float) : sizeof(double);
   if ((int)t->nq < 1)
     WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
   if (t->nq != 1) WBFMM_FATAL("only nq == 1 is supported (nq = %u)", t->nq);
@@ -1660,17 +1776,98 @@ gint wbfmm_tree_coefficient_init(wbfmm_tree_t *t, guint l, guint nr, guint ns)
   if (l > t->depth) WBFMM_FATAL("level %u below tree depth %u", l, t->depth);
   if (ns > WBFMM_HIP_MAX_ORDER || nr > WBFMM_HIP_MAX_ORDER)
     WBFMM_FATAL("order above %d", WBFMM_HIP_MAX_ORDER);
+  if (td->prec32 && td->host_mirror)
+    WBFMM_FATAL("FP32 coefficients cannot be mirrored to the host (wbfmm_tree_host_mirror)");
   t->order_s[l] = ns;
   t->order_r[l] = nr;
+  /* mixed precision: FP32 storage until the wavenumber puts the level in
+     FP64 (hel_prepare) */
+  td->h64[l] = 0;
+  td->hk_set = 0;
   if (ns) {
-    alloc_level(td, l, 0, nb * 2 * (ns + 1) * (ns + 1) * sizeof(double));
+    alloc_level(td, l, 0, nb * 2 * (ns + 1) * (ns + 1) * es);
     if (td->host_mirror) set_box_pointers(td, l, 0, 2 * (ns + 1) * (ns + 1));
   }
   if (nr) {
-    alloc_level(td, l, 1, nb * 2 * (nr + 1) * (nr + 1) * sizeof(double));
+    alloc_level(td, l, 1, nb * 2 * (nr + 1) * (nr + 1) * es);
     if (td->host_mirror) set_box_pointers(td, l, 1, 2 * (nr + 1) * (nr + 1));
   }
   return 0;
+}
+
+/* ---- mixed-precision Helmholtz ------------------------------------------
+   FP32 levels hold M~_n = M_n / s_n and L~_n = L_n s_n with
+     s_n = min(1, x^n / (2n+1)!!),  x = k a_l,  a_l = sqrt(3)/2 w_l,
+   the small-argument size of j_n(k r) over the box (j_n(x) ~ x^n/(2n+1)!!
+   for n > x): multipole coefficients of the level's boxes, and local
+   coefficients times the regular functions at their targets, are then
+   O(1) whatever the order, where unscaled FP32 would underflow (j_23(0.2)
+   ~ 1e-45).  Every translation is the FP64 operator with its output degree
+   n and input degree l scaled, rounded to FP32 once; rotations keep the
+   degree and need no scaling.  Levels with k a_l above a threshold (the
+   high-frequency levels near the root, default 2, WBFMM_HEL_FP64_KA) and
+   levels whose order the FP32 matrix-core passes do not serve stay FP64;
+   passes between an FP64 and an FP32 level run in FP64 on a converted
+   copy.  Tables and recurrences stay FP64. */
+static double hel_scale_arg(double k, double D, unsigned l)
+{
+  return k * 0.5 * sqrt(3.0) * D / (double)(1u << l);
+}
+
+static void hel_scales(double x, int nmax, double *sc)
+{
+  double lg = 0.0, lx = x > 1e-300 ? log(x) : log(1e-300);
+  int n;
+  for (n = 0; n <= nmax; n++) {
+    lg += log(2.0 * n + 1.0);
+    sc[n] = n * lx - lg < 0.0 ? exp(fmax(n * lx - lg, -640.0)) : 1.0;
+  }
+}
+
+static double hel_fp64_ka(void)
+{
+  const char *e = getenv("WBFMM_HEL_FP64_KA");
+  return e ? atof(e) : 2.0;
+}
+
+/* levels of a mixed-precision Helmholtz tree for wavenumber k: FP64 where
+   k a_l exceeds the threshold or the orders are not served by the FP32
+   passes (levels moved between precisions are reallocated, zeroed), and
+   the scales of the FP32 levels */
+static void hel_prepare(wbfmm_tree_t *t, double k)
+{
+  tree_dev *td = TD(t);
+  unsigned l;
+  if (!td->prec32 || (td->hk_set && td->hk == k)) return;
+  for (l = 1; l <= t->depth; l++) {
+    const unsigned ns = t->order_s[l], nr = t->order_r[l], N = ns > nr ? ns : nr;
+    const size_t nb = (size_t)1 << (3 * l);
+    const int want64 = hel_scale_arg(k, t->D, l) > hel_fp64_ka() || ns != nr ||
+                       (int)N < M2L_MFMA_MIN || (int)N > HEL_MFMA_MAX;
+    if (want64 != td->h64[l]) {
+      const size_t es = want64 ? sizeof(double) : sizeof(float);
+      if (ns && t->mps[l]) alloc_level(td, l, 0, nb * 2 * (ns + 1) * (ns + 1) * es);
+      if (nr && t->mpr[l]) alloc_level(td, l, 1, nb * 2 * (nr + 1) * (nr + 1) * es);
+      td->h64[l] = want64;
+    }
+    free(td->hsc[l]);
+    td->hsc[l] = malloc(sizeof(double) * (N + 2));
+    if (!td->hsc[l]) WBFMM_FATAL("out of host memory");
+    hel_scales(hel_scale_arg(k, t->D, l), (int)N + 1, td->hsc[l]);
+    wbfmm_hip_free(td->d_hsc[l]);
+    HIPOK(wbfmm_hip_malloc((void **)&td->d_hsc[l], sizeof(double) * (N + 2)));
+    HIPOK(wbfmm_hip_memcpy(td->d_hsc[l], td->hsc[l], sizeof(double) * (N + 2)));
+  }
+  td->hk = k;
+  td->hk_set = 1;
+}
+
+/* coefficients of `level' are FP32 */
+static int level_f32(tree_dev *td, guint level)
+{
+  if (!td->prec32) return 0;
+  if (td->t.problem == WBFMM_PROBLEM_HELMHOLTZ) return !td->h64[level];
+  return 1;
 }
 
 /* src/boxes.c:341-490: P2M at the leaf level; src, dipoles: complex
@@ -1687,12 +1884,19 @@ gint wbfmm_tree_leaf_expansions(wbfmm_tree_t *t, gdouble k, gdouble *src,
     WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
   if (t->problem != WBFMM_PROBLEM_HELMHOLTZ) WBFMM_FATAL("tree is not a Helmholtz tree");
   if (!t->mps[d]) WBFMM_FATAL("coefficients not initialised at level %u", d);
+  hel_prepare(t, k); /* mixed precision: the levels' precision follows k */
   if (zero_expansions) HIPOK(wbfmm_hip_memset(t->mps[d], 0, td->mps_bytes[d]));
   if (src == NULL && dipoles == NULL) return 0;
   if (t->normals == NULL && dipoles != NULL)
     WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
   q = gather_strengths_n(td, src, sstr, 0, 2);
   dp = gather_strengths_n(td, dipoles, dstr, 1, 2);
+  if (level_f32(td, d)) {
+    HIPOK(wbfmm_hip_hel_p2m_f32(td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d],
+                                td->d_xyz, q, dp ? td->d_nrm : NULL, dp, d, ORIGIN(t), t->D,
+                                t->order_s[d], k, td->d_hsc[d], (float *)t->mps[d]));
+    return 0;
+  }
   HIPOK(wbfmm_hip_hel_p2m(td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d],
                           td->d_xyz, q, dp ? td->d_nrm : NULL, dp, d,
                           ORIGIN(t), t->D, t->order_s[d], k,
@@ -1792,6 +1996,10 @@ gint wbfmm_shift_operators_coaxial_reset(wbfmm_shift_operators_t *w)
     od->hel_SS2[n] = NULL;
     free(od->ss_host[n]);
     od->ss_host[n] = NULL;
+    wbfmm_hip_free(od->f_hA2[n]);
+    wbfmm_hip_free(od->f_hUp[n]);
+    wbfmm_hip_free(od->f_hDn[n]);
+    od->f_hA2[n] = od->f_hUp[n] = od->f_hDn[n] = NULL;
   }
   od->hel_k_set = 0;
   return 0;
@@ -1816,14 +2024,14 @@ static void ensure_hel_rotations(ops_dev *od)
   int L = (int)od->op.Lmax;
   hel_rot_set *e = &g_hel_rot[L];
   if (od->d_hel_Hth) return;
-  pthread_mutex_lock(&g_hel_rot_mu);
+  mu_lock(&g_hel_rot_mu);
   while (e->state == 1) pthread_cond_wait(&g_hel_rot_cv, &g_hel_rot_mu);
   if (e->state == 0) {
     double thetas[64], chi[343], rr[16], tud[2];
     int th_idx[343], rix[343], nth, nr;
     hel_rot_set b;
     e->state = 1;
-    pthread_mutex_unlock(&g_hel_rot_mu);
+    mu_unlock(&g_hel_rot_mu);
     wbfmm_hip_offset_geometry(thetas, &nth, th_idx, chi, rr, &nr, rix);
     HIPOK(wbfmm_hip_H_tables(L, nth, thetas, &b.Hth, &b.nrot));
     tud[0] = acos(1.0 / sqrt(3.0));
@@ -1837,7 +2045,7 @@ static void ensure_hel_rotations(ops_dev *od)
     HIPOK(wbfmm_hip_memcpy(b.rix, rix, sizeof(rix)));
     HIPOK(wbfmm_hip_malloc((void **)&b.chi, sizeof(chi)));
     HIPOK(wbfmm_hip_memcpy(b.chi, chi, sizeof(chi)));
-    pthread_mutex_lock(&g_hel_rot_mu);
+    mu_lock(&g_hel_rot_mu);
     b.state = 2;
     *e = b;
     pthread_cond_broadcast(&g_hel_rot_cv);
@@ -1848,7 +2056,7 @@ static void ensure_hel_rotations(ops_dev *od)
   od->d_th_idx = e->th_idx;
   od->d_rix = e->rix;
   od->d_chi = e->chi;
-  pthread_mutex_unlock(&g_hel_rot_mu);
+  mu_unlock(&g_hel_rot_mu);
 }
 
 static int hel_shift_mfma(ops_dev *od, guint Nc, guint Np);
@@ -1856,6 +2064,31 @@ static void ensure_hel_shift_rot_tiles(ops_dev *od, int N);
 static void ensure_hel_shift_coax_tiles(ops_dev *od, guint level, int N);
 
 /* src/pass.c:581-632: M2M from level into level-1 */
+/* FP64 M2M from child expansions src (level `level') into the parents'
+   dst (level-1, overwritten) */
+static void hel_m2m64(wbfmm_tree_t *t, ops_dev *od, guint level, const double *src, double *dst)
+{
+  tree_dev *td = TD(t);
+  guint Nc = t->order_s[level], Np = t->order_s[level - 1];
+  if (hel_shift_mfma(od, Nc, Np)) {
+    int N = (int)Nc;
+    ensure_plan(od, N);
+    ensure_hel_shift_rot_tiles(od, N);
+    ensure_hel_shift_coax_tiles(od, level, N);
+    HIPOK(wbfmm_hip_hel_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                   td->d_start[level], td->d_end[level], NULL, N, src, dst,
+                                   od->m2l_plan_dev[N], od->hel_shA[N], od->hel_SS2[level],
+                                   od->sh_CSup[N], od->m2l_plan[N]));
+    return;
+  }
+  ensure_hel_rotations(od);
+  HIPOK(wbfmm_hip_hel_m2m(td->d_occ[level - 1], td->nocc[level - 1], td->d_start[level],
+                          td->d_end[level], (int)Nc, (int)Np, src, dst, od->d_hel_Hud,
+                          od->hel_nrot, (double *)od->op.SS[level]));
+}
+
+static void hel_upward_mixed(wbfmm_tree_t *t, ops_dev *od, guint level);
+
 gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
                        gdouble *work)
 {
@@ -1872,23 +2105,11 @@ gint wbfmm_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level
   if (!op->SS[level]) WBFMM_FATAL("S|S operators not initialised at level %u", level);
   if ((Nc > Np ? Nc : Np) > od->ss_L[level] || (Nc > Np ? Nc : Np) > op->Lmax)
     WBFMM_FATAL("operators at level %u built to order %u", level, od->ss_L[level]);
-  if (hel_shift_mfma(od, Nc, Np)) {
-    int N = (int)Nc;
-    ensure_plan(od, N);
-    ensure_hel_shift_rot_tiles(od, N);
-    ensure_hel_shift_coax_tiles(od, level, N);
-    HIPOK(wbfmm_hip_hel_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
-                                   td->d_start[level], td->d_end[level], NULL, N,
-                                   (double *)t->mps[level], (double *)t->mps[level - 1],
-                                   od->m2l_plan_dev[N], od->hel_shA[N], od->hel_SS2[level],
-                                   od->sh_CSup[N], od->m2l_plan[N]));
+  if (td->prec32) {
+    hel_upward_mixed(t, od, level);
     return 0;
   }
-  ensure_hel_rotations(od);
-  HIPOK(wbfmm_hip_hel_m2m(td->d_occ[level - 1], td->nocc[level - 1], td->d_start[level],
-                          td->d_end[level], (int)Nc, (int)Np, (double *)t->mps[level],
-                          (double *)t->mps[level - 1], od->d_hel_Hud, od->hel_nrot,
-                          (double *)op->SS[level]));
+  hel_m2m64(t, od, level, (const double *)t->mps[level], (double *)t->mps[level - 1]);
   return 0;
 }
 
@@ -1910,7 +2131,7 @@ static int hel_mfma_enabled(void)
 static void ensure_hel_rot_tiles(ops_dev *od, int N)
 {
   if (od->hel_A[N]) return;
-  pthread_mutex_lock(&g_ord_mu[N]);
+  mu_lock(&g_ord_mu[N]);
   if (!g_tab.hel_A[N]) {
     const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
     size_t n = (size_t)P->NTH * P->KT * 64;
@@ -1923,7 +2144,7 @@ static void ensure_hel_rot_tiles(ops_dev *od, int N)
     g_tab.hel_A[N] = d;
   }
   od->hel_A[N] = g_tab.hel_A[N];
-  pthread_mutex_unlock(&g_ord_mu[N]);
+  mu_unlock(&g_ord_mu[N]);
 }
 
 /* coaxial tiles of the Helmholtz matrix-core M2L at `level' for order N,
@@ -1962,7 +2183,7 @@ static void ensure_hel_coax_tiles(ops_dev *od, guint level, int N)
 static void ensure_hel_shift_rot_tiles(ops_dev *od, int N)
 {
   if (od->hel_shA[N]) return;
-  pthread_mutex_lock(&g_ord_mu[N]);
+  mu_lock(&g_ord_mu[N]);
   if (!g_tab.hel_shA[N]) {
     const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
     size_t n = (size_t)2 * P->KT * 64;
@@ -1976,7 +2197,7 @@ static void ensure_hel_shift_rot_tiles(ops_dev *od, int N)
     g_tab.hel_shA[N] = d;
   }
   od->hel_shA[N] = g_tab.hel_shA[N];
-  pthread_mutex_unlock(&g_ord_mu[N]);
+  mu_unlock(&g_ord_mu[N]);
 }
 
 /* real coaxial tiles of the Helmholtz matrix-core M2M / L2L between
@@ -2005,6 +2226,63 @@ static int hel_shift_mfma(ops_dev *od, guint Nc, guint Np)
   return Nc == Np && (int)Nc >= M2L_MFMA_MIN && (int)Nc <= HEL_MFMA_MAX && hel_mfma_enabled();
 }
 
+/* FP64 M2L at `level' for the nt target boxes tl */
+static void hel_m2l64(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t *tl, unsigned nt)
+{
+  tree_dev *td = TD(t);
+  wbfmm_shift_operators_t *op = &od->op;
+  guint Ns = t->order_s[level], Nr = t->order_r[level];
+  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= HEL_MFMA_MAX && hel_mfma_enabled()) {
+    /* matrix-core form: targets grouped by Morton child digit */
+    int N = (int)Ns;
+    unsigned cstart[8], ccount[8];
+    uint32_t *cls;
+    ensure_plan(od, N);
+    ensure_hel_rot_tiles(od, N);
+    ensure_hel_coax_tiles(od, level, N);
+    HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
+    HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
+    HIPOK(wbfmm_hip_hel_m2l_mfma(cls, cstart, ccount, td->d_start[level], td->d_end[level],
+                                 level, N, (double *)t->mps[level], (double *)t->mpr[level],
+                                 od->m2l_plan_dev[N], od->hel_A[N], od->hel_A2[level],
+                                 od->m2l_CS[N], od->m2l_plan[N]));
+    wbfmm_hip_free(cls);
+  } else {
+    ensure_hel_rotations(od); /* the vector kernels' H tables, on first use */
+    HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
+                            (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
+                            od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
+                            (double *)op->SR[level], coax_entries(op->L[level])));
+  }
+}
+
+/* FP64 L2L from the parents tl (level `level', local expansions src) into
+   their children's dst (level+1, added) */
+static void hel_l2l64(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t *tl,
+                      unsigned nt, const double *src, double *dst)
+{
+  tree_dev *td = TD(t);
+  guint Nr = t->order_r[level], Nc = t->order_r[level + 1];
+  if (hel_shift_mfma(od, Nc, Nr)) {
+    int N = (int)Nc;
+    ensure_plan(od, N);
+    ensure_hel_shift_rot_tiles(od, N);
+    ensure_hel_shift_coax_tiles(od, level + 1, N);
+    HIPOK(wbfmm_hip_hel_shift_mfma(1, tl, nt, NULL, NULL,
+                                   td->have_targets ? td->d_flag[level + 1] : NULL, N, src, dst,
+                                   od->m2l_plan_dev[N], od->hel_shA[N], od->hel_SS2[level + 1],
+                                   od->sh_CSdn[N], od->m2l_plan[N]));
+  } else {
+    ensure_hel_rotations(od);
+    HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL, (int)Nr,
+                            (int)Nc, src, dst, od->d_hel_Hud, od->hel_nrot,
+                            (double *)od->op.SS[level + 1]));
+  }
+}
+
+static void hel_downward_mixed(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t *tl,
+                               unsigned nt);
+
 static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level)
 {
   tree_dev *td = TD(t);
@@ -2028,54 +2306,244 @@ static gint hel_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint lev
   } else {
     tl = all_boxes(level, &nt);
   }
-  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= HEL_MFMA_MAX && hel_mfma_enabled()) {
-    /* matrix-core form: targets grouped by Morton child digit */
+  if (level < t->depth) {
+    guint Nc = t->order_r[level + 1];
+    if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
+    if (!op->SS[level + 1]) WBFMM_FATAL("S|S operators not initialised at level %u", level + 1);
+    if ((Nr > Nc ? Nr : Nc) > od->ss_L[level + 1] || (Nr > Nc ? Nr : Nc) > op->Lmax)
+      WBFMM_FATAL("S|S operators at level %u built to order %u", level + 1, od->ss_L[level + 1]);
+  }
+  if (td->prec32) {
+    hel_downward_mixed(t, od, level, tl, nt);
+    return 0;
+  }
+  hel_m2l64(t, od, level, tl, nt);
+  if (level == t->depth) return 0;
+  hel_l2l64(t, od, level, tl, nt, (const double *)t->mpr[level], (double *)t->mpr[level + 1]);
+  return 0;
+}
+
+/* ---- mixed-precision passes -------------------------------------------- */
+
+/* the FP32 rotation tiles and phases of order N */
+static void ensure_hel_f32_order(ops_dev *od, int N)
+{
+  const wbfmm_m2l_plan_t *P;
+  ensure_plan(od, N);
+  ensure_hel_rot_tiles(od, N);
+  ensure_hel_shift_rot_tiles(od, N);
+  P = od->m2l_plan[N];
+  if (!od->f_hA[N]) od->f_hA[N] = dev_f32(od->hel_A[N], (size_t)P->NTH * P->KT * 64);
+  if (!od->f_hshA[N]) od->f_hshA[N] = dev_f32(od->hel_shA[N], (size_t)2 * P->KT * 64);
+  if (!od->f_hCS[N]) od->f_hCS[N] = dev_f32(od->m2l_CS[N], (size_t)343 * (N + 1) * 2);
+  if (!od->f_hCSup[N]) od->f_hCSup[N] = dev_f32(od->sh_CSup[N], (size_t)8 * (N + 1) * 2);
+  if (!od->f_hCSdn[N]) od->f_hCSdn[N] = dev_f32(od->sh_CSdn[N], (size_t)8 * (N + 1) * 2);
+}
+
+/* scaled FP32 coaxial tiles of the M2L at `level': L~ = s L, M = s M~ */
+static void ensure_hel_f32_m2l_tiles(ops_dev *od, double D, guint level, int N, double k)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  size_t ne = coax_entries(od->op.L[level]), n = (size_t)P->NR * 2 * P->KT * 64;
+  double *E, *h, sc[WBFMM_HIP_MAX_ORDER + 2];
+  int i, j;
+  if (od->f_hA2[level] && od->f_hA2_N[level] == N && od->f_hA2_k[level] == k && od->f_hD == D)
+    return;
+  if (!od->sr_host[level]) WBFMM_FATAL("S|R operators not initialised at level %u", level);
+  if (od->sr_nr != P->NR) WBFMM_FATAL("%d offset lengths, the plan has %d", od->sr_nr, P->NR);
+  E = malloc(sizeof(double) * 2 * ne * P->NR);
+  h = malloc(sizeof(double) * n);
+  if (!E || !h) WBFMM_FATAL("out of host memory");
+  for (i = 0; i < P->NR; i++) {
+    for (j = 0; j < od->sr_nr; j++)
+      if (od->sr_rr[j] == P->rr[i]) break;
+    if (j == od->sr_nr) WBFMM_FATAL("offset length %g not in the S|R tables", P->rr[i]);
+    memcpy(E + (size_t)i * 2 * ne, od->sr_host[level] + (size_t)j * 2 * ne,
+           sizeof(double) * 2 * ne);
+  }
+  hel_scales(hel_scale_arg(k, D, level), N + 1, sc);
+  HIPOK(wbfmm_hip_hel_coax_tiles_scaled(P, (int)od->op.L[level], P->NR, E, sc, sc, h));
+  wbfmm_hip_free(od->f_hA2[level]);
+  od->f_hA2[level] = host_f32(h, n);
+  od->f_hA2_N[level] = N;
+  od->f_hA2_k[level] = k;
+  od->f_hD = D;
+  free(E);
+  free(h);
+}
+
+/* scaled FP32 coaxial tiles of the M2M from `level' into level-1 (up) and
+   of the L2L from level-1 into `level' (down), from the level's R|R table */
+static void ensure_hel_f32_shift_tiles(ops_dev *od, double D, guint level, int N, double k)
+{
+  const wbfmm_m2l_plan_t *P = od->m2l_plan[N];
+  size_t n = (size_t)P->KT * 64;
+  double *h, sc[WBFMM_HIP_MAX_ORDER + 2], sp[WBFMM_HIP_MAX_ORDER + 2];
+  double ip[WBFMM_HIP_MAX_ORDER + 2];
+  int i;
+  if (od->f_hUp[level] && od->f_hUp_N[level] == N && od->f_hUp_k[level] == k &&
+      od->f_hDn[level] && od->f_hDn_N[level] == N && od->f_hDn_k[level] == k && od->f_hD == D)
+    return;
+  if (!od->ss_host[level]) WBFMM_FATAL("S|S operators not initialised at level %u", level);
+  h = malloc(sizeof(double) * 2 * n);
+  if (!h) WBFMM_FATAL("out of host memory");
+  hel_scales(hel_scale_arg(k, D, level), N + 1, sc);
+  hel_scales(hel_scale_arg(k, D, level - 1), N + 1, sp);
+  for (i = 0; i <= N + 1; i++) ip[i] = 1.0 / sp[i];
+  /* M2M: M~_parent = M_parent / s_parent, M_child = s_child M~_child */
+  HIPOK(wbfmm_hip_hel_coax_tiles_scaled(P, (int)od->ss_L[level], 1, od->ss_host[level], ip, sc, h));
+  wbfmm_hip_free(od->f_hUp[level]);
+  od->f_hUp[level] = host_f32(h, n); /* real part */
+  /* L2L: L~_child = s_child L_child, L_parent = L~_parent / s_parent */
+  HIPOK(wbfmm_hip_hel_coax_tiles_scaled(P, (int)od->ss_L[level], 1, od->ss_host[level], sc, ip, h));
+  wbfmm_hip_free(od->f_hDn[level]);
+  od->f_hDn[level] = host_f32(h, n);
+  od->f_hUp_N[level] = od->f_hDn_N[level] = N;
+  od->f_hUp_k[level] = od->f_hDn_k[level] = k;
+  od->f_hD = D;
+  free(h);
+}
+
+/* FP64 copy of an FP32 level (which: 0 mps, 1 mpr) for the listed boxes,
+   unscaled; the rest of the copy is zero */
+static double *hel_level_to64(wbfmm_tree_t *t, guint level, int which, const uint32_t *boxes,
+                              unsigned nbox)
+{
+  tree_dev *td = TD(t);
+  int N = (int)(which ? t->order_r[level] : t->order_s[level]), n;
+  size_t bytes = ((size_t)1 << (3 * level)) * 2 * (N + 1) * (N + 1) * sizeof(double);
+  double *d = NULL, us[WBFMM_HIP_MAX_ORDER + 2], *dus = NULL;
+  HIPOK(wbfmm_hip_malloc((void **)&d, bytes));
+  HIPOK(wbfmm_hip_memset(d, 0, bytes));
+  for (n = 0; n <= N; n++) us[n] = which ? 1.0 / td->hsc[level][n] : td->hsc[level][n];
+  HIPOK(wbfmm_hip_malloc((void **)&dus, sizeof(double) * (N + 1)));
+  HIPOK(wbfmm_hip_memcpy(dus, us, sizeof(double) * (N + 1)));
+  HIPOK(wbfmm_hip_hel_convert(0, 0, boxes, nbox, N, which ? t->mpr[level] : t->mps[level], dus, d));
+  HIPOK(wbfmm_hip_free(dus));
+  return d;
+}
+
+/* an FP64 level array d (unscaled) into an FP32 level, for the listed
+   boxes: overwrite (add 0) or add (add 1) */
+static void hel_level_to32(wbfmm_tree_t *t, guint level, int which, const uint32_t *boxes,
+                           unsigned nbox, const double *d, int add)
+{
+  tree_dev *td = TD(t);
+  int N = (int)(which ? t->order_r[level] : t->order_s[level]), n;
+  double sc[WBFMM_HIP_MAX_ORDER + 2], *dsc = NULL;
+  for (n = 0; n <= N; n++) sc[n] = which ? td->hsc[level][n] : 1.0 / td->hsc[level][n];
+  HIPOK(wbfmm_hip_malloc((void **)&dsc, sizeof(double) * (N + 1)));
+  HIPOK(wbfmm_hip_memcpy(dsc, sc, sizeof(double) * (N + 1)));
+  HIPOK(wbfmm_hip_hel_convert(1, add, boxes, nbox, N, d, dsc, which ? t->mpr[level] : t->mps[level]));
+  HIPOK(wbfmm_hip_free(dsc));
+}
+
+static void hel_require_k(wbfmm_tree_t *t, ops_dev *od)
+{
+  tree_dev *td = TD(t);
+  if (!td->hk_set) WBFMM_FATAL("leaf expansions not formed (the wavenumber sets the precision)");
+  if (od->hel_k_set && od->hel_k != td->hk)
+    WBFMM_FATAL("operators for k = %g, tree formed for k = %g", od->hel_k, td->hk);
+}
+
+static void hel_upward_mixed(wbfmm_tree_t *t, ops_dev *od, guint level)
+{
+  tree_dev *td = TD(t);
+  const int c32 = !td->h64[level], p32 = !td->h64[level - 1];
+  const guint Nc = t->order_s[level], Np = t->order_s[level - 1];
+  hel_require_k(t, od);
+  if (c32 && p32 && Nc == Np) {
+    int N = (int)Nc;
+    ensure_hel_f32_order(od, N);
+    ensure_hel_f32_shift_tiles(od, t->D, level, N, td->hk);
+    HIPOK(wbfmm_hip_hel_shift_mfma_f32(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                       td->d_start[level], td->d_end[level], NULL, N,
+                                       (const float *)t->mps[level], (float *)t->mps[level - 1],
+                                       od->m2l_plan_dev[N], od->f_hshA[N], od->f_hUp[level],
+                                       od->f_hCSup[N], od->m2l_plan[N]));
+    return;
+  }
+  {
+    /* a pass across precisions runs in FP64 on converted copies */
+    double *src = c32 ? hel_level_to64(t, level, 0, td->d_occ[level], td->nocc[level])
+                      : (double *)t->mps[level];
+    double *dst = (double *)t->mps[level - 1];
+    if (p32) {
+      size_t bytes = ((size_t)1 << (3 * (level - 1))) * 2 * (Np + 1) * (Np + 1) * sizeof(double);
+      HIPOK(wbfmm_hip_malloc((void **)&dst, bytes));
+      HIPOK(wbfmm_hip_memset(dst, 0, bytes));
+    }
+    hel_m2m64(t, od, level, src, dst);
+    if (p32) {
+      hel_level_to32(t, level - 1, 0, td->d_occ[level - 1], td->nocc[level - 1], dst, 0);
+      HIPOK(wbfmm_hip_free(dst));
+    }
+    if (c32) HIPOK(wbfmm_hip_free(src));
+  }
+}
+
+static void hel_downward_mixed(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t *tl,
+                               unsigned nt)
+{
+  tree_dev *td = TD(t);
+  const guint Ns = t->order_s[level], Nr = t->order_r[level];
+  hel_require_k(t, od);
+  if (!td->h64[level]) {
     int N = (int)Ns;
     unsigned cstart[8], ccount[8];
     uint32_t *cls;
-    ensure_plan(od, N);
-    ensure_hel_rot_tiles(od, N);
-    ensure_hel_coax_tiles(od, level, N);
+    if (Ns != Nr) WBFMM_FATAL("FP32 level %u with unequal orders", level);
+    ensure_hel_f32_order(od, N);
+    ensure_hel_f32_m2l_tiles(od, t->D, level, N, td->hk);
     HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
     HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
-    HIPOK(wbfmm_hip_hel_m2l_mfma(cls, cstart, ccount, td->d_start[level], td->d_end[level],
-                                 level, N, (double *)t->mps[level], (double *)t->mpr[level],
-                                 od->m2l_plan_dev[N], od->hel_A[N], od->hel_A2[level],
-                                 od->m2l_CS[N], od->m2l_plan[N]));
+    HIPOK(wbfmm_hip_hel_m2l_mfma_f32(cls, cstart, ccount, td->d_start[level], td->d_end[level],
+                                     level, N, (const float *)t->mps[level],
+                                     (float *)t->mpr[level], od->m2l_plan_dev[N], od->f_hA[N],
+                                     od->f_hA2[level], od->f_hCS[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
-    ensure_hel_rotations(od); /* the vector kernels' H tables, on first use */
-    HIPOK(wbfmm_hip_hel_m2l(tl, nt, td->d_start[level], td->d_end[level], level, (int)Ns,
-                            (int)Nr, (double *)t->mps[level], (double *)t->mpr[level],
-                            od->d_hel_Hth, od->hel_nrot, od->d_th_idx, od->d_chi, od->d_rix,
-                            (double *)op->SR[level], coax_entries(op->L[level])));
+    hel_m2l64(t, od, level, tl, nt);
   }
-  if (level == t->depth) return 0;
-  if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
-  if (!op->SS[level + 1]) WBFMM_FATAL("S|S operators not initialised at level %u", level + 1);
+  if (level == t->depth) return;
   {
-    guint Nc = t->order_r[level + 1];
-    if ((Nr > Nc ? Nr : Nc) > od->ss_L[level + 1] || (Nr > Nc ? Nr : Nc) > op->Lmax)
-      WBFMM_FATAL("S|S operators at level %u built to order %u", level + 1, od->ss_L[level + 1]);
-    if (hel_shift_mfma(od, Nc, Nr)) {
+    const int p32 = !td->h64[level], c32 = !td->h64[level + 1];
+    const guint Nc = t->order_r[level + 1];
+    if (p32 && c32 && Nr == Nc) {
       int N = (int)Nc;
-      ensure_plan(od, N);
-      ensure_hel_shift_rot_tiles(od, N);
-      ensure_hel_shift_coax_tiles(od, level + 1, N);
-      HIPOK(wbfmm_hip_hel_shift_mfma(1, tl, nt, NULL, NULL,
-                                     td->have_targets ? td->d_flag[level + 1] : NULL, N,
-                                     (double *)t->mpr[level], (double *)t->mpr[level + 1],
-                                     od->m2l_plan_dev[N], od->hel_shA[N],
-                                     od->hel_SS2[level + 1], od->sh_CSdn[N], od->m2l_plan[N]));
-    } else {
-      ensure_hel_rotations(od);
-      HIPOK(wbfmm_hip_hel_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
-                              (int)Nr, (int)Nc, (double *)t->mpr[level],
-                              (double *)t->mpr[level + 1], od->d_hel_Hud, od->hel_nrot,
-                              (double *)op->SS[level + 1]));
+      ensure_hel_f32_order(od, N);
+      ensure_hel_f32_shift_tiles(od, t->D, level + 1, N, td->hk);
+      HIPOK(wbfmm_hip_hel_shift_mfma_f32(1, tl, nt, NULL, NULL,
+                                         td->have_targets ? td->d_flag[level + 1] : NULL, N,
+                                         (const float *)t->mpr[level], (float *)t->mpr[level + 1],
+                                         od->m2l_plan_dev[N], od->f_hshA[N], od->f_hDn[level + 1],
+                                         od->f_hCSdn[N], od->m2l_plan[N]));
+      return;
+    }
+    {
+      double *src = p32 ? hel_level_to64(t, level, 1, tl, nt) : (double *)t->mpr[level];
+      double *dst = (double *)t->mpr[level + 1];
+      if (c32) {
+        size_t bytes = ((size_t)1 << (3 * (level + 1))) * 2 * (Nc + 1) * (Nc + 1) * sizeof(double);
+        HIPOK(wbfmm_hip_malloc((void **)&dst, bytes));
+        HIPOK(wbfmm_hip_memset(dst, 0, bytes));
+      }
+      hel_l2l64(t, od, level, tl, nt, src, dst);
+      if (c32) {
+        uint32_t *cl;
+        unsigned nc;
+        if (td->have_targets) {
+          cl = td->d_act[level + 1];
+          nc = td->nact[level + 1];
+        } else {
+          cl = all_boxes(level + 1, &nc);
+        }
+        hel_level_to32(t, level + 1, 1, cl, nc, dst, 1);
+        HIPOK(wbfmm_hip_free(dst));
+      }
+      if (p32) HIPOK(wbfmm_hip_free(src));
     }
   }
-  return 0;
 }
 
 /* src/pass.c:448-537 (and the AVX variant, src/pass-avx.c): M2L at level,
@@ -2096,28 +2564,59 @@ gint wbfmm_downward_pass_avx(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   return hel_downward(t, op, level);
 }
 
-/* field values of n staged targets (Helmholtz): fl bit 1 potential (2
-   doubles), bit 2 gradient (6), potential first */
+/* the FP64 field kernel: one thread per target, or one wavefront per
+   target box (WBFMM_HEL_FIELD=box) */
+static int hel_field_box64(void)
+{
+  const char *e = getenv("WBFMM_HEL_FIELD");
+  return e && strcmp(e, "box") == 0;
+}
+
+/* field values of n staged targets (Helmholtz), sorted by their Morton
+   codes dcodes: fl bit 1 potential (2 doubles), bit 2 gradient (6),
+   potential first */
 static void hel_field_run(tree_dev *td, double k, const double *dxt,
-                          const uint32_t *dtid, unsigned n, double *src, gint sstr,
-                          double *d, gint dstr, int fl, double *f, gint fstr,
-                          size_t nf_rows)
+                          const uint32_t *dtid, const uint64_t *dcodes, unsigned n,
+                          double *src, gint sstr, double *d, gint dstr, int fl, double *f,
+                          gint fstr, size_t nf_rows)
 {
   wbfmm_tree_t *t = &td->t;
   unsigned lv = t->depth;
   double *q, *dp, *df;
-  int staged;
+  int staged, f32;
   size_t nelem;
   if (!t->mpr[lv]) WBFMM_FATAL("coefficients not initialised at level %u", lv);
   if (t->normals == NULL && d != NULL)
     WBFMM_FATAL("no normals in tree but dipole strengths specified (d != NULL)");
+  f32 = level_f32(td, lv);
+  if (f32 && (!td->hk_set || td->hk != k))
+    WBFMM_FATAL("tree formed for k = %g, field asked for k = %g", td->hk, k);
   q = gather_strengths_n(td, src, sstr, 0, 2);
   dp = gather_strengths_n(td, d, dstr, 1, 2);
   nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0) : 0;
   df = field_begin(f, nelem, &staged);
-  HIPOK(wbfmm_hip_hel_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
-                            dp ? td->d_nrm : NULL, dp, lv, ORIGIN(t),
-                            t->D, t->order_r[lv], k, (double *)t->mpr[lv], fl, df, fstr));
+  if (f32 || hel_field_box64()) {
+    /* one wavefront per box holding targets */
+    const size_t nb = (size_t)1 << (3 * lv);
+    uint32_t *ts = NULL, *te = NULL, *tb = NULL;
+    unsigned ntb = 0;
+    HIPOK(wbfmm_hip_malloc((void **)&ts, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_malloc((void **)&te, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_malloc((void **)&tb, sizeof(uint32_t) * nb));
+    HIPOK(wbfmm_hip_box_ranges(dcodes, n, lv, ts, te));
+    HIPOK(wbfmm_hip_compact_boxes(ts, te, NULL, lv, tb, &ntb));
+    HIPOK(wbfmm_hip_hel_field_box(tb, ntb, ts, te, dxt, dtid, td->d_start[lv], td->d_end[lv],
+                                  td->d_xyz, q, dp ? td->d_nrm : NULL, dp, lv, ORIGIN(t), t->D,
+                                  t->order_r[lv], k, f32 ? 32 : 64, t->mpr[lv],
+                                  f32 ? td->d_hsc[lv] : NULL, fl, df, fstr));
+    wbfmm_hip_free(ts);
+    wbfmm_hip_free(te);
+    wbfmm_hip_free(tb);
+  } else {
+    HIPOK(wbfmm_hip_hel_field(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
+                              dp ? td->d_nrm : NULL, dp, lv, ORIGIN(t), t->D,
+                              t->order_r[lv], k, (double *)t->mpr[lv], fl, df, fstr));
+  }
   field_finish(f, df, nelem, staged);
 }
 
@@ -2155,7 +2654,7 @@ gint wbfmm_tree_box_local_field(wbfmm_tree_t *t, guint level, guint b, gdouble k
   stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc, ORIGIN(t),
                 t->D, 0);
   if (!eval_neighbours) { src = NULL; d = NULL; }
-  hel_field_run(td, k, dx, did, 1, src, sstr, d, dstr, fl, tmp, 8, 1);
+  hel_field_run(td, k, dx, did, dc, 1, src, sstr, d, dstr, fl, tmp, 8, 1);
   for (i = 0; i < (fl == 1 ? 2 : 6); i++) f[i] += tmp[i];
   wbfmm_hip_free(dx);
   wbfmm_hip_free(did);
@@ -2181,7 +2680,7 @@ gint wbfmm_helmholtz_field_targets(wbfmm_tree_t *t, gdouble k, gdouble *xt,
   if (fstr < ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0)) return -2;
   if (ntgt == 0) return 0;
   stage_targets(xt, tstr, ntgt, &dx, &did, &dc, ORIGIN(t), t->D, 1);
-  hel_field_run(td, k, dx, did, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
+  hel_field_run(td, k, dx, did, dc, ntgt, src, sstr, d, dstr, fl, f, fstr, ntgt);
   HIPOK(wbfmm_hip_sync());
   wbfmm_hip_free(dx);
   wbfmm_hip_free(did);

@@ -1053,6 +1053,14 @@ int wbfmm_hip_hel_m2l_rot_tiles(const wbfmm_m2l_plan_t *P, double *A)
 int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *P, int L, int nr, const double *E,
                                  double *A2)
 {
+  return wbfmm_hip_hel_coax_tiles_scaled(P, L, nr, E, NULL, NULL, A2);
+}
+
+/* the same tiles with output degree n scaled by sout[n] and input degree l
+   by sin[l] (NULL: 1), for coefficients held as M_n / s_n or L_n s_n */
+int wbfmm_hip_hel_coax_tiles_scaled(const wbfmm_m2l_plan_t *P, int L, int nr, const double *E,
+                                    const double *sout, const double *sin, double *A2)
+{
   packing p2;
   int N = P->N, S = 16 * P->NB, i, c, m, part, n, l;
   size_t ne = CXI(L + 1, 0, 0), per = (size_t)P->KT * 64;
@@ -1071,6 +1079,8 @@ int wbfmm_hip_hel_m2l_coax_tiles(const wbfmm_m2l_plan_t *P, int L, int nr, const
               size_t idx = l >= n ? CXI(l, m, n) : CXI(n, m, l);
               double v = E[(size_t)i * 2 * ne + 2 * idx + c];
               if (l < n && ((n + l) & 1)) v = -v;
+              if (sout) v *= sout[n];
+              if (sin) v *= sin[l];
               T[(size_t)slot2(&p2, N, n, m, part) * S + slot2(&p2, N, l, m, part)] = v;
             }
       put_tiles(A2 + (size_t)(2 * i + c) * per, T, P);

@@ -64,6 +64,7 @@ _SIGS = {
     "wbfmm_serve_tune": (_i, [_vp, ctypes.POINTER(Request), ctypes.POINTER(_i), _i,
                               ctypes.POINTER(_i)]),
     "wbfmm_serve_free": (_i, [_vp]),
+    "wbfmm_serve_debug_fail_allocations": (_i, [_vp, _i]),
 }
 _bound = False
 
@@ -149,6 +150,13 @@ class Server:
         if rc:
             raise RuntimeError(f"wbfmm_serve_tune = {rc}")
         return best.value
+
+    def debug_fail_allocations(self, count):
+        """the next submitted request has its first `count` device
+        allocations refused (failure-handling tests)"""
+        rc = _L().wbfmm_serve_debug_fail_allocations(self.s, count)
+        if rc:
+            raise RuntimeError(f"wbfmm_serve_debug_fail_allocations = {rc}")
 
     def stats(self):
         st = Stats()
