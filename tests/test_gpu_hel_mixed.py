@@ -85,7 +85,8 @@ def sphere(seed, n):
 def test_every_level_fp64_gives_fp64_bits(wb, monkeypatch):
     xs, q, xt = cube(701, 3000, 800)
     x0, D = fit_box(xs, xt)
-    monkeypatch.setenv("WBFMM_HEL_FP64_KA", "1e9")
+    # the threshold below every level's k a_l: every level runs FP64
+    monkeypatch.setenv("WBFMM_HEL_FP64_KA", "-1")
     f_mix = solve(wb, xs, q, None, xt, x0, D, 3, [10] * 17, 3.0, 32)
     monkeypatch.delenv("WBFMM_HEL_FP64_KA")
     f64 = solve(wb, xs, q, None, xt, x0, D, 3, [10] * 17, 3.0, 64)
@@ -122,7 +123,7 @@ def test_mixed_sphere_boundary_levels(wb, oracle):
     assert r.max() < TOL, r
     # error against the FP64 direct sum on sampled points, potential
     idx = np.arange(0, n, 97)
-    fd = oracle.hel_direct(k, xs, c2(q), xs[idx], field=1, nrm=xs, d=c2(d), cutoff=1e-6)
+    fd = oracle.hel_direct(k, xs, q, xs[idx], field=1, nrm=xs, d=d, cutoff=1e-6)
     for name, f in (("mixed", fm), ("fp64", f64)):
         fc = f[idx, 0] + 1j * f[idx, 1]
         e = np.linalg.norm(fc - fd[:, 0]) / np.linalg.norm(fd[:, 0])

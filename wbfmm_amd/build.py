@@ -17,7 +17,15 @@ INC = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwbfmm.so")
 
-HIP_SOURCES = ["runtime.hip", "laplace.hip", "m2l_mfma.hip", "hel_m2l_mfma.hip", "helmholtz.hip"]
+# the longest compiles first (the build runs 8 at a time)
+HIP_SOURCES = [
+    # the Helmholtz and Laplace matrix-core kernels, one range of orders per unit
+    "hel_m2l_o26_27.hip", "hel_m2l_o24_25.hip", "hel_m2l_o22_23.hip", "hel_m2l_o19_21.hip",
+    "m2l_lap_o22_24.hip", "m2l_lap_o19_21.hip", "m2l_lap_o16_18.hip", "hel_m2l_o15_18.hip",
+    "m2l_lap_o13_15.hip", "m2l_lap_o11_12.hip", "m2l_lap_o08_10.hip", "hel_m2l_o10_14.hip",
+    "m2l_lap_o02_07.hip", "hel_m2l_o02_09.hip",
+    "helmholtz.hip", "laplace.hip", "m2l_mfma.hip", "hel_m2l_mfma.hip", "runtime.hip",
+]
 C_SOURCES = ["wbfmm_api.c", "wbfmm_tables.c"]
 CXX_SOURCES = ["serve.cpp"]
 

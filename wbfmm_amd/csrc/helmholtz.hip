@@ -598,8 +598,9 @@ __global__ __launch_bounds__(128) void k_hel_field(
 // lanes do not diverge.  T = double: FP64 throughout.  T = float: the
 // local expansion holds L_n^m s_n in FP32 (sc: the level's scales s_n,
 // divided out in FP64), the regular functions and the L2P sums are FP64,
-// and the near-field kernel runs in FP32 (offsets x - y formed in FP64,
-// the 1e-6 cutoff tested in FP64) with compensated (Neumaier) sums.
+// and the near-field kernel runs in FP32 (offsets x - y and the dipole
+// factor n.(x - y) formed in FP64, the 1e-6 cutoff tested in FP64) with
+// compensated (Neumaier) sums.
 // ---------------------------------------------------------------------
 __device__ __forceinline__ void sincos_t(double a, double *s, double *c) { sincos(a, s, c); }
 __device__ __forceinline__ void sincos_t(float a, float *s, float *c) { sincosf(a, s, c); }
@@ -754,7 +755,11 @@ __global__ __launch_bounds__(64) void k_hel_field_box(
               if (dip) {
                 const T dr = (T)sds[j].x, di = (T)sds[j].y;
                 const T nx = (T)sns[j][0], ny = (T)sns[j][1], nz = (T)sns[j][2];
-                const T nr = nx * ux + ny * uy + nz * uz;
+                // n.(x-y) in FP64: for close points on a smooth surface the
+                // normal is nearly orthogonal to x - y, and the FP32 dot
+                // product of unit vectors would cancel
+                const T nr =
+                    (T)(sns[j][0] * rxd + sns[j][1] * ryd + sns[j][2] * rzd) * ir;
                 const T ddr = dr * dGr - di * dGi, ddi = dr * dGi + di * dGr;
                 apot.add(ddr * nr, ddi * nr);
                 // (ik - 1/r)^2 + 1/r^2 = 2/r^2 - k^2 - 2ik/r

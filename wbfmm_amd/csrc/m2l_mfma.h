@@ -218,6 +218,70 @@ __device__ __forceinline__ void stage_wo(const T *__restrict__ a, const T *Rin, 
   }
 }
 
+// Operands held across stages: where a wavefront's k-steps of a stage fit
+// in 16 registers (pairs), av[s] holds the A operand of its step s.  A
+// stage consumes av[s] and at once requests step s of the next stage's
+// table into it, so every stage starts with its operands loaded, or in
+// flight since the stage before, instead of waiting for them after the
+// barrier that precedes it.
+template <int N, int W, int WI>
+struct StageOps {
+  static constexpr WaveSeq WS = make_seq(M2LShape<N>::BN, W, WI);
+  static constexpr int n = WS.n > 0 ? WS.n : 1;
+  static constexpr bool fits = WS.n <= 16;
+};
+
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void ops_load(T *av, const T *__restrict__ a, int lane) {
+  constexpr WaveSeq WS = make_seq(M2LShape<N>::BN, W, WI);
+#pragma unroll
+  for (int s = 0; s < WS.n; s++) av[s] = a[WS.t[s] * 64 + lane];
+}
+
+// stage_w with the operands in av; anext: the next stage's table
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_wp(T *av, const T *__restrict__ anext, const T *R,
+                                         int lane, int grp, int col, typename MF<T>::v4 *y) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      r4 = MF<T>::mma(av[s], R[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
+      av[s] = anext[WS.t[s] * 64 + lane];
+    }
+    y[lb] = r4;
+  }
+}
+
+// stage_wo with the operands in av; anext: the next stage's table
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void stage_wop(T *av, const T *__restrict__ anext, const T *Rin,
+                                          T *Rout, const int4 *tm, int lane, int grp, int col) {
+  constexpr Bins BN = M2LShape<N>::BN;
+  constexpr WaveSeq WS = make_seq(BN, W, WI);
+#pragma unroll
+  for (int lb = 0; lb < WS.nb; lb++) {
+    const int b = WI + W * lb;
+    typename MF<T>::v4 r4 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int j = 0; j < BN.nk[b]; j++) {
+      const int s = WS.start[lb] + j;
+      r4 = MF<T>::mma(av[s], Rin[(BN.kbase[b] + 4 * j + grp) * 16 + col], r4);
+      av[s] = anext[WS.t[s] * 64 + lane];
+    }
+    const int4 d = tm[4 * b + grp];
+    Rout[d.x * 16 + col] = r4[0];
+    Rout[d.y * 16 + col] = r4[1];
+    Rout[d.z * 16 + col] = r4[2];
+    Rout[d.w * 16 + col] = r4[3];
+  }
+}
+
 // the exchange buffer is doubled where the two copies take at most 48 KB,
 // which leaves room for three blocks per CU (FP64 up to order 12, FP32 up
 // to order 17)

@@ -48,12 +48,17 @@ int field_width(int problem, unsigned field) {
   return c * ((p ? 1 : 0) + (g ? 3 : 0));
 }
 
-// Laplace order for a relative accuracy: the error of the octree FMM with
-// one box of separation falls by about 0.3 per order (4.6e-7 potential and
-// 2.3e-6 gradient at order 12 on 1e6 uniform points)
+// Laplace order for a relative accuracy.  The error of the octree FMM with
+// one box of separation falls by a factor between 0.3 and 0.55 per order,
+// depending on the depth and the point distribution: 4.6e-7 (potential) at
+// order 12 on 1e6 uniform points at depth 7, 6.9e-9 at order 17 on 6e4
+// points at depth 4, 1.9e-8 at order 17 on 2192 points at depth 2.  The
+// order is the larger of the estimates from either rate, plus one.
 int laplace_order(double tol) {
   if (!(tol > 0.0) || tol >= 1.0) return 2;
-  int p = (int)std::ceil(std::log(tol) / std::log(0.3)) + 1;
+  const int p_fast = (int)std::ceil(std::log(tol) / std::log(0.3)) + 1;
+  const int p_slow = 12 + (int)std::ceil(std::log(tol / 4.6e-7) / std::log(0.55)) + 1;
+  const int p = std::max(p_fast, p_slow);
   return std::min(std::max(p, 2), WBFMM_HIP_MAX_ORDER);
 }
 
