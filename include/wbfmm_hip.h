@@ -330,6 +330,29 @@ int wbfmm_hip_lap_field(const double *xt, const uint32_t *tid, unsigned n,
                         const double *nrm, const double *dip,
                         unsigned level, const double *x0, double D, int N,
                         const double *mpr, int field, double *f, int fstr);
+/* vector sources, nq components (src/local-laplace.c:333-1336, the curl
+   and curl-gradient of :632-1291): L2P + near field for n staged targets
+   (rows tid[i] of f, fstr doubles apart).  Component q's local expansions
+   are plane q of mpr (plane_stride doubles apart, box-major), its sorted
+   strengths q[s nq + q] (monopoles) and dip[s nq + q] (normal dipoles,
+   normals nrm).  mode: bit 1 the nq potentials, bit 2 their gradients
+   (d_j A_q at 3q + j after the potentials); or bit 4 the curl of the
+   first three components (3 values) and, with bit 2, its gradient
+   (d_j u_i at 3 + 3j + i; 12 values).  nq <= 8; the curl modes need
+   nq >= 3 and take monopoles only, as the reference's. */
+int wbfmm_hip_lap_field_vec(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip, int nq,
+                            unsigned level, const double *x0, double D, int N,
+                            const double *mpr, size_t plane_stride, int mode, double *f,
+                            int fstr);
+/* FP64 direct sums with vector sources (src/direct-laplace.c:87-336):
+   the modes and rows of wbfmm_hip_lap_field_vec for every target t (row t
+   of f) over every source (strengths nq per source in source order),
+   skipping |x - y| <= cutoff */
+int wbfmm_hip_lap_direct_vec(const double *xyz, const double *q, const double *nrm,
+                             const double *dip, int nq, unsigned nsrc, const double *xt,
+                             unsigned ntgt, int mode, double cutoff, double *f, int fstr);
 /* FP64 direct sum on the GPU (src/direct-laplace.c:200-336): sources in
    xyz (4 doubles each), q and/or (nrm,dip); cutoff: pairs with r<=cutoff
    skipped (0: none); f[i*fstr + ...] accumulated as in _lap_field */

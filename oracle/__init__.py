@@ -71,6 +71,7 @@ def lib():
         L.orc_lap_m_eval.restype = ctypes.c_double
         L.orc_lap_m_eval.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_lap_l_eval.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_lap_l_hess.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_lap_coaxial_cfft.restype = ctypes.c_double
         L.orc_lap_coaxial_cfft.argtypes = [ctypes.c_int] * 4
         L.orc_lap_coaxial.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
@@ -153,6 +154,15 @@ def lap_l_eval(N, x0, L, xf, field=3):
     return f
 
 
+def lap_l_hess(N, x0, L, xf):
+    """second derivatives (xx, yy, zz, xy, yz, zx) of the local expansion
+    L (order N, about x0) at xf"""
+    h = np.zeros(6)
+    lib().orc_lap_l_hess(N, P(np.asarray(x0, float)), P(np.ascontiguousarray(L, float)),
+                         P(np.asarray(xf, float)), P(h))
+    return h
+
+
 def lap_coaxial(kind, Ci, Ni, No, t):
     Co = np.zeros((No + 1) ** 2)
     lib().orc_lap_coaxial(kind, P(Co), No, P(np.ascontiguousarray(Ci, float)), Ni, t, 0.0)
@@ -166,9 +176,11 @@ def lap_rotate(Ci, N, H, ph, ch):
 
 
 def lap_direct(xs, q, xt, field=3, nrm=None, d=None, cutoff=0.0):
+    """rows: potential, gradient and, with field bit 4 (monopoles), the
+    second derivatives (xx, yy, zz, xy, yz, zx)"""
     xs = np.ascontiguousarray(xs, float)
     xt = np.ascontiguousarray(xt, float)
-    f = np.zeros((len(xt), 4))
+    f = np.zeros((len(xt), 10 if field & 4 else 4))
     lib().orc_lap_direct(len(xs), P(xs), P(None if q is None else np.ascontiguousarray(q, float)),
                          P(None if nrm is None else np.ascontiguousarray(nrm, float)),
                          P(None if d is None else np.ascontiguousarray(d, float)),
@@ -178,6 +190,27 @@ def lap_direct(xs, q, xt, field=3, nrm=None, d=None, cutoff=0.0):
 
 def _c(a):
     return np.ascontiguousarray(a, float)
+
+
+def curl_from_components(rows):
+    """curl u = curl A and its gradient from the rows of the three
+    components A_l (each [potential, gradient, xx, yy, zz, xy, yz, zx]):
+    returns (n, 12): u, then d_j u_i at 3 + 3j + i, the layout of the
+    reference's curl-gradient field (src/local-laplace.c:147-183)"""
+    J = [r[:, 1:4] for r in rows]            # J[l][:, k] = d_k A_l
+    H = [r[:, 4:10] for r in rows]
+    hi = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (1, 0): 3, (1, 2): 4, (2, 1): 4,
+          (0, 2): 5, (2, 0): 5}
+    n = rows[0].shape[0]
+    out = np.zeros((n, 12))
+    out[:, 0] = J[2][:, 1] - J[1][:, 2]
+    out[:, 1] = J[0][:, 2] - J[2][:, 0]
+    out[:, 2] = J[1][:, 0] - J[0][:, 1]
+    for j in range(3):
+        out[:, 3 + 3 * j + 0] = H[2][:, hi[j, 1]] - H[1][:, hi[j, 2]]
+        out[:, 3 + 3 * j + 1] = H[0][:, hi[j, 2]] - H[2][:, hi[j, 0]]
+        out[:, 3 + 3 * j + 2] = H[1][:, hi[j, 0]] - H[0][:, hi[j, 1]]
+    return out
 
 
 def hel_p2m(k, N, x0, xs, q, M=None):
@@ -265,7 +298,9 @@ def fmm(problem, xs, xt, x0, D, depth, order_s, order_r, q=None, nrm=None, d=Non
         cfg.dump_mpr = P(dumps[1])
     xs = np.ascontiguousarray(xs, float)
     xt = np.ascontiguousarray(xt, float)
-    width = 4 if problem == 1 else 8
+    # Laplace rows: potential, gradient and, with field bit 4, the second
+    # derivatives (xx, yy, zz, xy, yz, zx)
+    width = (10 if field & 4 else 4) if problem == 1 else 8
     f = np.zeros((len(xt), width))
     rc = lib().orc_fmm(ctypes.byref(cfg), len(xs), P(xs),
                        P(None if q is None else np.ascontiguousarray(q, float)),

@@ -563,6 +563,97 @@ void orc_lap_l_eval(int N, const double *x0, const double *L,
 }
 
 /*
+ * Second derivatives of a Laplace local expansion at xf, added into
+ * h[0..5] = (xx, yy, zz, xy, yz, zx): the quantities the reference's
+ * curl-gradient evaluation combines (src/local-laplace.c:752-1011, with
+ * the second-derivative macros of src/wbfmm-derivatives.h).  Restated
+ * with ladder operators on S_n^mu = r^n Pbar_n^|mu| e^{i mu phi}
+ * (S_n^-mu = conj S_n^mu, and L_n^-mu = conj L_n^mu, so that the field
+ * is sum_{n,mu} L_n^mu S_n^mu):
+ *   d/dz S_n^mu = a(n,|mu|) S_{n-1}^mu,   a(n,m) = sqrt(k_n (n-m)(n+m)),
+ *   D+ S_n^mu = s(mu) b(n,mu) S_{n-1}^{mu+1},
+ *               b(n,mu) = sqrt(k_n (n-mu)(n-mu-1)), s = -1 (mu >= 0), +1,
+ * k_n = (2n+1)/(2n-1), D+ = d/dx + i d/dy (the identities the gradient
+ * above uses).  With zz = d2/dz2, V = D+ d/dz and W = D+ D+ of the field:
+ *   xx = (Re W - zz)/2, yy = -(Re W + zz)/2, xy = Im W / 2,
+ *   zx = Re V, yz = Im V   (D+ D- = -d2/dz2 for a harmonic field).
+ */
+static double lad_a(int n, int m)
+{
+  return n >= 1 ? sqrt((2.0 * n + 1) / (2.0 * n - 1) * (double)(n - m) * (n + m)) : 0.0;
+}
+
+static double lad_b(int n, int mu)
+{
+  return n >= 1 ? sqrt((2.0 * n + 1) / (2.0 * n - 1) * (double)(n - mu) * (n - mu - 1)) : 0.0;
+}
+
+void orc_lap_l_hess(int N, const double *x0, const double *L, const double *xf, double *h)
+{
+  double r, C, S, cp, sp, rn, cm, sm, t;
+  double P[(64 + 1) * (64 + 2) / 2];
+  double Rre[(64 + 1) * (64 + 2) / 2], Rim[(64 + 1) * (64 + 2) / 2];
+  double zz = 0.0, vr = 0.0, vi = 0.0, wr = 0.0, wi = 0.0;
+  int n, m, mu;
+  sph(x0, xf, &r, &C, &S, &cp, &sp);
+  orc_legendre(N, C, S, P);
+  rn = 1.0;
+  for (n = 0; n <= N; n++) {
+    cm = 1.0; sm = 0.0;
+    for (m = 0; m <= n; m++) {
+      if (m > 0) { t = cm * cp - sm * sp; sm = sm * cp + cm * sp; cm = t; }
+      Rre[PIDX(n, m)] = rn * P[PIDX(n, m)] * cm;
+      Rim[PIDX(n, m)] = rn * P[PIDX(n, m)] * sm;
+    }
+    rn *= r;
+  }
+  for (n = 2; n <= N; n++) {
+    for (mu = -n; mu <= n; mu++) {
+      const int am = mu < 0 ? -mu : mu;
+      double lr, li;
+      if (am == 0) { lr = L[n * n]; li = 0.0; }
+      else { lr = L[LIDX(n, am)]; li = L[LIDX(n, am) + 1]; }
+      if (mu < 0) li = -li;
+      /* (c, nu): coefficient c of S_{n-2}^nu */
+#define ADD_TERM(cre, cim, nu, ore, oim)                                          \
+      do {                                                                         \
+        const int an = (nu) < 0 ? -(nu) : (nu);                                   \
+        if (an <= n - 2) {                                                         \
+          double sr = Rre[PIDX(n - 2, an)], si = Rim[PIDX(n - 2, an)];             \
+          if ((nu) < 0) si = -si;                                                  \
+          ore += (cre) * sr - (cim) * si;                                          \
+          oim += (cre) * si + (cim) * sr;                                          \
+        }                                                                          \
+      } while (0)
+      {
+        /* d2/dz2: a(n,|mu|) a(n-1,|mu|) */
+        const double c = lad_a(n, am) * lad_a(n - 1, am);
+        double dummy = 0.0;
+        ADD_TERM(c * lr, c * li, mu, zz, dummy);
+      }
+      {
+        /* D+ d/dz: a(n,|mu|) s(mu) b(n-1,mu) */
+        const double c = lad_a(n, am) * (mu >= 0 ? -1.0 : 1.0) * lad_b(n - 1, mu);
+        ADD_TERM(c * lr, c * li, mu + 1, vr, vi);
+      }
+      {
+        /* D+ D+: s(mu) b(n,mu) s(mu+1) b(n-1,mu+1) */
+        const double c = (mu >= 0 ? -1.0 : 1.0) * lad_b(n, mu) * (mu + 1 >= 0 ? -1.0 : 1.0) *
+                         lad_b(n - 1, mu + 1);
+        ADD_TERM(c * lr, c * li, mu + 2, wr, wi);
+      }
+#undef ADD_TERM
+    }
+  }
+  h[0] += 0.5 * (wr - zz);
+  h[1] += -0.5 * (wr + zz);
+  h[2] += zz;
+  h[3] += 0.5 * wi;
+  h[4] += vi;
+  h[5] += vr;
+}
+
+/*
  * closed-form Laplace coaxial coefficients (src/laplace.c:542-569,
  * :630-649, :698-725); kind 0 = SS, 1 = RR, 2 = SR.  The powers of the
  * translation distance are applied by orc_lap_coaxial.
@@ -724,8 +815,9 @@ void orc_lap_direct(int nsrc, const double *xs, const double *q,
 {
   int i, j;
   const double c4 = 0.25 / M_PI;
+  const int fw = (field & 4) ? 10 : 4;
   for (i = 0; i < ntgt; i++) {
-    double phi = 0, gx = 0, gy = 0, gz = 0;
+    double phi = 0, gx = 0, gy = 0, gz = 0, h[6] = {0, 0, 0, 0, 0, 0};
     for (j = 0; j < nsrc; j++) {
       double rx = xt[3 * i] - xs[3 * j], ry = xt[3 * i + 1] - xs[3 * j + 1];
       double rz = xt[3 * i + 2] - xs[3 * j + 2];
@@ -735,6 +827,16 @@ void orc_lap_direct(int nsrc, const double *xs, const double *q,
       if (q) {
         phi += q[j] * ir;
         gx -= q[j] * rx * ir3; gy -= q[j] * ry * ir3; gz -= q[j] * rz * ir3;
+        if (field & 4) {
+          /* second derivatives of q/r: q (3 r_a r_b / r^5 - delta_ab / r^3) */
+          const double ir5 = ir3 * ir * ir;
+          h[0] += q[j] * (3.0 * rx * rx * ir5 - ir3);
+          h[1] += q[j] * (3.0 * ry * ry * ir5 - ir3);
+          h[2] += q[j] * (3.0 * rz * rz * ir5 - ir3);
+          h[3] += q[j] * 3.0 * rx * ry * ir5;
+          h[4] += q[j] * 3.0 * ry * rz * ir5;
+          h[5] += q[j] * 3.0 * rz * rx * ir5;
+        }
       }
       if (d) {
         double nr = nrm[3 * j] * rx + nrm[3 * j + 1] * ry + nrm[3 * j + 2] * rz;
@@ -745,10 +847,12 @@ void orc_lap_direct(int nsrc, const double *xs, const double *q,
         gz += d[j] * (nrm[3 * j + 2] * ir3 - 3.0 * nr * rz * ir5);
       }
     }
-    if (field & 1) f[4 * i] += phi * c4;
+    if (field & 1) f[fw * i] += phi * c4;
     if (field & 2) {
-      f[4 * i + 1] += gx * c4; f[4 * i + 2] += gy * c4; f[4 * i + 3] += gz * c4;
+      f[fw * i + 1] += gx * c4; f[fw * i + 2] += gy * c4; f[fw * i + 3] += gz * c4;
     }
+    if (field & 4)
+      for (j = 0; j < 6; j++) f[fw * i + 4 + j] += h[j] * c4;
   }
 }
 
@@ -1263,6 +1367,7 @@ static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
   int depth = t->depth, l, Lmax = 0, nrot, i;
   double *mps[17] = {0}, *mpr[17] = {0}, *Htab, tm;
   double *Hud[2][8];
+  if ((cfg->field & 4) && d) return -4;  /* second derivatives: monopoles only */
   for (l = 2; l <= depth; l++) {
     if (cfg->order_s[l] > Lmax) Lmax = cfg->order_s[l];
     if (cfg->order_r[l] > Lmax) Lmax = cfg->order_r[l];
@@ -1356,10 +1461,11 @@ static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
       const double *x = xt + 3 * i;
       uint64_t b = orc_point_index(x, t->x0, t->D) >> (3 * (20 - depth));
       uint64_t nb[27];
-      double xc[3], loc[4] = {0, 0, 0, 0};
+      double xc[3], loc[4] = {0, 0, 0, 0}, hes[6] = {0, 0, 0, 0, 0, 0};
       int nn, a, j;
       box_centre(t, depth, b, xc);
       orc_lap_l_eval(Nr, xc, mpr[depth] + b * ncr, x, cfg->field, loc);
+      if (cfg->field & 4) orc_lap_l_hess(Nr, xc, mpr[depth] + b * ncr, x, hes);
       nn = orc_box_neighbours(depth, b, nb);
       for (a = 0; a < nn; a++) {
         for (j = 0; j < t->bcount[depth][nb[a]]; j++) {
@@ -1373,6 +1479,16 @@ static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
             loc[0] += q[s] * g;
             loc[1] -= q[s] * g * rx / r2; loc[2] -= q[s] * g * ry / r2;
             loc[3] -= q[s] * g * rz / r2;
+            if (cfg->field & 4) {
+              /* d_j d_k G = (3 r_j r_k / r^2 - delta_jk) G / r^2 */
+              const double h = q[s] * g / r2;
+              hes[0] += h * (3.0 * rx * rx / r2 - 1.0);
+              hes[1] += h * (3.0 * ry * ry / r2 - 1.0);
+              hes[2] += h * (3.0 * rz * rz / r2 - 1.0);
+              hes[3] += h * 3.0 * rx * ry / r2;
+              hes[4] += h * 3.0 * ry * rz / r2;
+              hes[5] += h * 3.0 * rz * rx / r2;
+            }
           }
           if (d) {
             const double *nv = nrm + 3 * s;
@@ -1384,8 +1500,15 @@ static int lap_solve(orc_fmm_cfg *cfg, orc_tree *t, const double *xs,
           }
         }
       }
-      if (cfg->field & 1) f[4 * i] += loc[0];
-      if (cfg->field & 2) { f[4 * i + 1] += loc[1]; f[4 * i + 2] += loc[2]; f[4 * i + 3] += loc[3]; }
+      {
+        /* rows of 4 (potential, gradient) or, with field bit 4, of 10
+           (and the second derivatives xx, yy, zz, xy, yz, zx) */
+        const int fw = (cfg->field & 4) ? 10 : 4;
+        if (cfg->field & 1) f[fw * i] += loc[0];
+        if (cfg->field & 2) { f[fw * i + 1] += loc[1]; f[fw * i + 2] += loc[2]; f[fw * i + 3] += loc[3]; }
+        if (cfg->field & 4)
+          for (a = 0; a < 6; a++) f[fw * i + 4 + a] += hes[a];
+      }
     }
   }
   cfg->t_phase[4] = now_s() - tm;

@@ -54,6 +54,9 @@ double orc_lap_m_eval(int N, const double *x0, const double *M,
                       const double *xf);
 void orc_lap_l_eval(int N, const double *x0, const double *L,
                     const double *xf, int field, double *f);
+/* second derivatives (xx, yy, zz, xy, yz, zx) of a local expansion at xf,
+   added into h[0..5] */
+void orc_lap_l_hess(int N, const double *x0, const double *L, const double *xf, double *h);
 double orc_lap_coaxial_cfft(int kind, int n, int nu, int m);
 void orc_lap_coaxial(int kind, double *Co, int No, const double *Ci, int Ni,
                      double t, double sc);

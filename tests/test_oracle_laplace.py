@@ -240,3 +240,94 @@ def test_oracle_fmm_dipoles_and_targets(oracle):
     fd = oracle.lap_direct(xs, q, xt, nrm=nrm, d=d, cutoff=1e-6)
     for c in range(4):
         assert np.linalg.norm(f[:, c] - fd[:, c]) / np.linalg.norm(fd[:, c]) < 1e-4
+
+
+# ---- second derivatives and the curl (vector sources) -------------------
+
+@pytest.mark.parametrize("N", [2, 3, 6, 12, 20])
+def test_local_second_derivatives_match_finite_difference(oracle, N):
+    """orc_lap_l_hess against central differences of the local gradient
+    (a wrong ladder sign or index fails at the 1e-3 level)"""
+    rng = np.random.default_rng(100 + N)
+    L = rng.normal(size=(N + 1) ** 2)
+    x0 = np.array([0.2, -0.1, 0.3])
+    for _ in range(3):
+        xf = x0 + rng.uniform(-0.3, 0.3, 3)
+        h = oracle.lap_l_hess(N, x0, L, xf)
+        eps = 1e-5
+        H = np.zeros((3, 3))
+        for j in range(3):
+            e = np.zeros(3)
+            e[j] = eps
+            H[:, j] = (oracle.lap_l_eval(N, x0, L, xf + e, 2)[1:4] -
+                       oracle.lap_l_eval(N, x0, L, xf - e, 2)[1:4]) / (2 * eps)
+        fd = np.array([H[0, 0], H[1, 1], H[2, 2], H[0, 1], H[1, 2], H[2, 0]])
+        assert np.abs(h - fd).max() < 1e-7 * np.abs(fd).max()
+        # the field is harmonic: the trace vanishes
+        assert abs(h[0] + h[1] + h[2]) < 1e-12 * np.abs(h).max()
+
+
+def test_direct_second_derivatives_match_finite_difference(oracle):
+    rng = np.random.default_rng(11)
+    xs = rng.uniform(0, 1, (50, 3))
+    q = rng.normal(size=50)
+    xt = rng.uniform(1.5, 2.0, (6, 3))
+    f = oracle.lap_direct(xs, q, xt, field=7)
+    eps = 1e-5
+    for c, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (2, 0)]):
+        e = np.zeros(3)
+        e[b] = eps
+        fd = (oracle.lap_direct(xs, q, xt + e)[:, 1 + a] -
+              oracle.lap_direct(xs, q, xt - e)[:, 1 + a]) / (2 * eps)
+        assert np.abs(f[:, 4 + c] - fd).max() < 1e-6 * np.abs(f[:, 4:]).max()
+
+
+def reference_curl(xs, w, xt, grad):
+    """the reference's direct curl (src/direct-laplace.c:87-198), written
+    out: grad G x w and d_j grad G x w with G = 1/(4 pi r)"""
+    out = np.zeros((len(xt), 12 if grad else 3))
+    for i, x in enumerate(xt):
+        for y, s in zip(xs, w):
+            dr = x - y
+            r = np.linalg.norm(dr)
+            r3 = r ** 3 * 4 * np.pi
+            r5 = r3 * r * r
+            nR = -dr / r3
+            out[i, :3] += np.cross(nR, s)
+            if grad:
+                for j in range(3):
+                    row = 3 * dr[j] * dr / r5
+                    row[j] -= 1.0 / r3
+                    out[i, 3 + 3 * j:6 + 3 * j] += np.cross(row, s)
+    return out
+
+
+def test_curl_from_components_matches_reference_curl(oracle):
+    """the combination of the three components' gradients and second
+    derivatives equals the reference's closed-form curl and curl-gradient"""
+    rng = np.random.default_rng(12)
+    xs = rng.uniform(0, 1, (40, 3))
+    w = rng.normal(size=(40, 3))
+    xt = rng.uniform(-0.5, 1.5, (9, 3))
+    rows = [oracle.lap_direct(xs, w[:, c], xt, field=7) for c in range(3)]
+    got = oracle.curl_from_components(rows)
+    want = reference_curl(xs, w, xt, True)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("depth,p", [(3, 10), (4, 12)])
+def test_oracle_fmm_curl_converges_to_direct(oracle, depth, p):
+    from conftest import fit_box
+    rng = np.random.default_rng(13 + depth)
+    n = 2500
+    xs = rng.uniform(0, 1, (n, 3))
+    w = rng.normal(size=(n, 3))
+    x0, D = fit_box(xs)
+    rows = [oracle.fmm(1, xs, xs, x0, D, depth, [p] * 17, [p] * 17, q=w[:, c], field=7)[0]
+            for c in range(3)]
+    got = oracle.curl_from_components(rows)
+    drows = [oracle.lap_direct(xs, w[:, c], xs, field=7, cutoff=1e-6) for c in range(3)]
+    want = oracle.curl_from_components(drows)
+    for c in range(12):
+        err = np.linalg.norm(got[:, c] - want[:, c]) / np.linalg.norm(want[:, c])
+        assert err < 1e-4, (c, err)

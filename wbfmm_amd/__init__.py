@@ -275,10 +275,11 @@ class LaplaceSolver:
 
     def __init__(self, xs, x0, D, depth, order_s, order_r, targets=None,
                  pstr=24, tstr=24, ntgt=None, host_mirror=True, nsrc=None,
-                 normals=None, nstr=24, precision=64):
+                 normals=None, nstr=24, precision=64, nq=1):
         L = lib()
         if precision == 32:
             host_mirror = False
+        self.nq = nq   # source components (vector sources: nq = 3)
         self.depth = depth
         self.nsrc = len(xs) if nsrc is None else nsrc
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
@@ -313,7 +314,7 @@ class LaplaceSolver:
         for _ in range(self.depth):
             L.wbfmm_tree_refine(self.t)
         self.tree.contents.problem = WBFMM_PROBLEM_LAPLACE
-        self.tree.contents.nq = 1
+        self.tree.contents.nq = self.nq
         for l in range(1, self.depth + 1):
             L.wbfmm_tree_laplace_coefficient_init(self.t, l, self.order_r[l], self.order_s[l])
         if self.targets is not None:
@@ -343,6 +344,17 @@ class LaplaceSolver:
         L.wbfmm_laplace_target_list_local_coefficients(self.tl, 1, None)
         L.wbfmm_target_list_local_field(self.tl, ptr(q), sstr, ptr(d), dstr, ptr(f), fstr)
 
+    def field_points(self, xt, q, f, field=3, fstr=4, sstr=1, d=None, dstr=1, tstr=24,
+                     ntgt=None):
+        """field at arbitrary points (wbfmm_laplace_field_targets): with nq
+        components, field 1/2/3 gives the potentials and gradients of every
+        component, 4 the curl, 6 the curl and its gradient"""
+        n = len(xt) if ntgt is None else ntgt
+        rc = lib().wbfmm_laplace_field_targets(self.t, ptr(xt), tstr, n, ptr(q), sstr, ptr(d),
+                                               dstr, field, ptr(f), fstr)
+        if rc != 0:
+            raise RuntimeError(f"wbfmm_laplace_field_targets = {rc}")
+
     def solve(self, q, f, field=3, fstr=4, d=None):
         self.build_tree()
         self.passes(q, d=d)
@@ -353,12 +365,14 @@ class LaplaceSolver:
         (8^level, C) array in box-major order."""
         C = (self.order_s[level] + 1) ** 2 if which == 0 else (self.order_r[level] + 1) ** 2
         nb = 8 ** level
-        buf = np.zeros(nb * C)
+        nq = self.nq
+        buf = np.zeros(nb * C * nq)
         rc = lib().wbfmm_tree_coefficients_download(self.t, level, which, ptr(buf))
         if rc != 0:
             raise RuntimeError("download failed")
-        # [group of 8][coef][child] -> [box][coef]
-        return buf.reshape(nb // 8, C, 8).transpose(0, 2, 1).reshape(nb, C)
+        # [group of 8][coef][child][component] -> [box][coef] ([component][box][coef])
+        a = buf.reshape(nb // 8, C, 8, nq).transpose(3, 0, 2, 1).reshape(nq, nb, C)
+        return a[0] if nq == 1 else a
 
     def close(self):
         L = lib()

@@ -445,6 +445,295 @@ __global__ void k_lap_field(const double *xt, const uint32_t *tid,
 }
 
 // ---------------------------------------------------------------------
+// Vector sources (nq components; src/local-laplace.c:333-630 per
+// component, the curl :632-750 and :1013-1089 with near field :38-62, the
+// curl and its gradient :752-1011 and :1229-1291 with near field :147-183).
+// Component q's local expansions are plane q of the level array (plane
+// stride pst).  mode: bit 1 the nq potentials A_q, bit 2 their gradients
+// (d_j A_q at 3q + j, after the potentials); or bit 4: the curl
+// u = curl A of the first three components (3 values) and, with bit 2,
+// its gradient (d_j u_i at 3 + 3j + i, 12 values in all).
+//
+// The derivatives of the local expansion use the ladder identities of
+// S_n^mu = r^n Pbar_n^|mu| e^{i mu phi} (S_n^-mu = conj S_n^mu, L_n^-mu =
+// conj L_n^mu): d/dz S_n^mu = a(n,|mu|) S_{n-1}^mu, (d/dx + i d/dy) S_n^mu
+// = s(mu) b(n,mu) S_{n-1}^{mu+1}, s = -1 for mu >= 0 and +1 below; each
+// derivative of the expansion is an expansion of lower degree whose
+// coefficient at (j, nu) is read from L at degree j+1 or j+2.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double lad_kn(int n) { return (2.0 * n + 1) / (2.0 * n - 1); }
+__device__ __forceinline__ double lad_a(int n, int m) {
+  return n >= 1 ? sqrt(lad_kn(n) * (double)(n - m) * (double)(n + m)) : 0.0;
+}
+__device__ __forceinline__ double lad_b(int n, int mu) {
+  const double v = (double)(n - mu) * (double)(n - mu - 1);
+  return (n >= 1 && v > 0.0) ? sqrt(lad_kn(n) * v) : 0.0;
+}
+__device__ __forceinline__ double lad_s(int mu) { return mu >= 0 ? -1.0 : 1.0; }
+
+// L_n^mu of one box (real-packed: n^2 for mu = 0, n^2 + 2|mu| - 1 and +1)
+__device__ __forceinline__ double2 lcoef_c(const double *L, int N, int n, int mu) {
+  const int am = mu < 0 ? -mu : mu;
+  if (n < 0 || n > N || am > n) return make_double2(0.0, 0.0);
+  if (am == 0) return make_double2(L[n * n], 0.0);
+  const double re = L[n * n + 2 * am - 1], im = L[n * n + 2 * am];
+  return make_double2(re, mu < 0 ? -im : im);
+}
+
+// potential, gradient (want_g) and second derivatives (want_h: xx, yy,
+// zz, xy, yz, zx) of one local expansion L of order N at offset (dx, dy,
+// dz) from its centre
+__device__ void lap_local_derivs(const double *L, int N, double dx, double dy, double dz,
+                                 bool want_g, bool want_h, double &phi, double g[3],
+                                 double h[6]) {
+  const double r = sqrt(dx * dx + dy * dy + dz * dz), rho = sqrt(dx * dx + dy * dy);
+  const double Cth = r > 0.0 ? dz / r : 1.0, Sth = r > 0.0 ? rho / r : 0.0;
+  const double cp = rho > 0.0 ? dx / rho : 1.0, sp = rho > 0.0 ? dy / rho : 0.0;
+  double ph = 0.0, gz = 0.0, gpr = 0.0, gpi = 0.0, zz = 0.0, vr = 0.0, vi = 0.0, wr = 0.0,
+         wi = 0.0;
+  double pmm = 1.0 / sqrt(4.0 * kPi), rm = 1.0, cm = 1.0, sm = 0.0;
+  for (int k = 0; k <= N; k++) {
+    if (k > 0) {
+      pmm *= sqrt((2.0 * k + 1) / (2.0 * k)) * Sth;
+      rm *= r;
+      const double tcm = cm * cp - sm * sp;
+      sm = sm * cp + cm * sp;
+      cm = tcm;
+    }
+    double p1 = 0.0, p0 = pmm, rn = rm;
+    for (int j = k; j <= N; j++) {
+      if (j > k) {
+        const double pn = (Cth * sqrt(2.0 * (j - 1) + 1) * p0 -
+                           sqrt((double)((j - 1) * (j - 1) - k * k) / (2 * (j - 1) - 1)) * p1) *
+                          sqrt((2.0 * (j - 1) + 3) / (double)(j * j - k * k));
+        p1 = p0;
+        p0 = pn;
+        rn *= r;
+      }
+      // S_j^{+k} and, for k > 0, S_j^{-k} = conj
+      for (int sg = 1; sg >= -1; sg -= 2) {
+        if (k == 0 && sg < 0) break;
+        const int nu = sg * k;
+        const double Sr = rn * p0 * cm, Si = sg * rn * p0 * sm;
+        auto acc = [&](double2 c, double &ore, double &oim) {
+          ore += c.x * Sr - c.y * Si;
+          oim += c.x * Si + c.y * Sr;
+        };
+        double dum = 0.0;
+        acc(lcoef_c(L, N, j, nu), ph, dum);
+        if (want_g) {
+          const double2 cz = lcoef_c(L, N, j + 1, nu);
+          const double az = lad_a(j + 1, k);
+          acc(make_double2(cz.x * az, cz.y * az), gz, dum);
+          const double2 cpl = lcoef_c(L, N, j + 1, nu - 1);
+          const double bp = lad_s(nu - 1) * lad_b(j + 1, nu - 1);
+          acc(make_double2(cpl.x * bp, cpl.y * bp), gpr, gpi);
+        }
+        if (want_h) {
+          const double2 c2 = lcoef_c(L, N, j + 2, nu);
+          const double f2 = lad_a(j + 2, k) * lad_a(j + 1, k);
+          acc(make_double2(c2.x * f2, c2.y * f2), zz, dum);
+          const int m1 = nu - 1 < 0 ? 1 - nu : nu - 1;
+          const double2 cv = lcoef_c(L, N, j + 2, nu - 1);
+          const double fv = lad_a(j + 2, m1) * lad_s(nu - 1) * lad_b(j + 1, nu - 1);
+          acc(make_double2(cv.x * fv, cv.y * fv), vr, vi);
+          const double2 cw = lcoef_c(L, N, j + 2, nu - 2);
+          const double fw =
+              lad_s(nu - 2) * lad_b(j + 2, nu - 2) * lad_s(nu - 1) * lad_b(j + 1, nu - 1);
+          acc(make_double2(cw.x * fw, cw.y * fw), wr, wi);
+        }
+      }
+    }
+  }
+  phi = ph;
+  g[0] = gpr;
+  g[1] = gpi;
+  g[2] = gz;
+  h[0] = 0.5 * (wr - zz);
+  h[1] = -0.5 * (wr + zz);
+  h[2] = zz;
+  h[3] = 0.5 * wi;
+  h[4] = vi;
+  h[5] = vr;
+}
+
+// index of d_j d_k in (xx, yy, zz, xy, yz, zx)
+__device__ __forceinline__ int hidx2(int j, int k) {
+  if (j == k) return j;
+  const int s = j + k;  // 1: xy, 3: yz, 2: zx
+  return s == 1 ? 3 : (s == 3 ? 4 : 5);
+}
+
+// curl u_i = e_ikl d_k A_l and d_j u_i from the component Jacobians
+// J[l][k] = d_k A_l and second derivatives H[l][.]
+__device__ __forceinline__ void curl_from(const double J[3][3], const double H[3][6], bool grad,
+                                          double *u) {
+  u[0] += J[2][1] - J[1][2];
+  u[1] += J[0][2] - J[2][0];
+  u[2] += J[1][0] - J[0][1];
+  if (!grad) return;
+  for (int j = 0; j < 3; j++) {
+    u[3 + 3 * j + 0] += H[2][hidx2(j, 1)] - H[1][hidx2(j, 2)];
+    u[3 + 3 * j + 1] += H[0][hidx2(j, 2)] - H[2][hidx2(j, 0)];
+    u[3 + 3 * j + 2] += H[1][hidx2(j, 0)] - H[0][hidx2(j, 1)];
+  }
+}
+
+// near-field terms of one source at r = x - y (|r| above the cutoff):
+// strengths sq[0..nq) (monopoles) and sd (dipoles, normal n)
+__device__ __forceinline__ void near_vec(int mode, int nq, double rx, double ry, double rz,
+                                         const double *sq, const double *sd, double nx,
+                                         double ny, double nz, double *out) {
+  const double c4 = 0.25 / kPi;
+  const double r2 = rx * rx + ry * ry + rz * rz, ir = 1.0 / sqrt(r2), ir3 = ir * ir * ir,
+               ir5 = ir3 * ir * ir;
+  const double rv[3] = {rx, ry, rz};
+  if (mode & 4) {
+    // grad G = -r / (4 pi r^3); d_j grad_k G = (3 r_j r_k / r^5 - delta_jk / r^3) / (4 pi)
+    const double gG[3] = {-c4 * rx * ir3, -c4 * ry * ir3, -c4 * rz * ir3};
+    out[0] += gG[1] * sq[2] - gG[2] * sq[1];
+    out[1] += gG[2] * sq[0] - gG[0] * sq[2];
+    out[2] += gG[0] * sq[1] - gG[1] * sq[0];
+    if (mode & 2)
+      for (int j = 0; j < 3; j++) {
+        double hG[3];
+        for (int k = 0; k < 3; k++)
+          hG[k] = c4 * (3.0 * rv[j] * rv[k] * ir5 - (j == k ? ir3 : 0.0));
+        out[3 + 3 * j + 0] += hG[1] * sq[2] - hG[2] * sq[1];
+        out[3 + 3 * j + 1] += hG[2] * sq[0] - hG[0] * sq[2];
+        out[3 + 3 * j + 2] += hG[0] * sq[1] - hG[1] * sq[0];
+      }
+    return;
+  }
+  const int go = (mode & 1) ? nq : 0;
+  const double nr = nx * rx + ny * ry + nz * rz;
+  for (int q = 0; q < nq; q++) {
+    const double qs = sq ? sq[q] * c4 : 0.0, ds = sd ? sd[q] * c4 : 0.0;
+    if (mode & 1) out[q] += qs * ir + ds * nr * ir3;
+    if (mode & 2) {
+      out[go + 3 * q + 0] += -qs * rx * ir3 + ds * (nx * ir3 - 3.0 * nr * rx * ir5);
+      out[go + 3 * q + 1] += -qs * ry * ir3 + ds * (ny * ir3 - 3.0 * nr * ry * ir5);
+      out[go + 3 * q + 2] += -qs * rz * ir3 + ds * (nz * ir3 - 3.0 * nr * rz * ir5);
+    }
+  }
+}
+
+constexpr int kVecMaxQ = 8;    // components of the vector-source kernels
+constexpr int kVecMaxOut = 32; // values per target row
+
+__host__ __device__ __forceinline__ int vec_width(int mode, int nq) {
+  if (mode & 4) return (mode & 2) ? 12 : 3;
+  return ((mode & 1) ? nq : 0) + ((mode & 2) ? 3 * nq : 0);
+}
+
+// one thread per target: local expansions of the target's leaf box plus
+// the 27 neighbour boxes' sources (strengths nq per point, sorted order)
+__global__ __launch_bounds__(128) void k_lap_field_vec(
+    const double *__restrict__ xt, const uint32_t *__restrict__ tid, unsigned ntgt,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
+    const double *__restrict__ xyz, const double *__restrict__ q,
+    const double *__restrict__ nrm, const double *__restrict__ dip, int nq, int level,
+    double x0, double y0, double z0, double D, int N, const double *__restrict__ mpr,
+    size_t pst, int mode, double *__restrict__ f, int fstr) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntgt) return;
+  const double px = xt[4 * (size_t)t], py = xt[4 * (size_t)t + 1], pz = xt[4 * (size_t)t + 2];
+  const uint64_t b = point_code(px, py, pz, x0, y0, z0, D) >> (3 * (20 - level));
+  const int bi = compact3(b), bj = compact3(b >> 1), bk = compact3(b >> 2);
+  const double w = D / (double)(1u << level);
+  const double dx = px - (x0 + w * (bi + 0.5)), dy = py - (y0 + w * (bj + 0.5)),
+               dz = pz - (z0 + w * (bk + 0.5));
+  const size_t C = (size_t)(N + 1) * (N + 1);
+  double out[kVecMaxOut];
+  const int nout = vec_width(mode, nq);
+  for (int i = 0; i < kVecMaxOut; i++) out[i] = 0.0;
+  if (mode & 4) {
+    double J[3][3], H[3][6];
+    for (int l = 0; l < 3; l++) {
+      double phi;
+      lap_local_derivs(mpr + l * pst + b * C, N, dx, dy, dz, true, (mode & 2) != 0, phi, J[l],
+                       H[l]);
+    }
+    curl_from(J, H, (mode & 2) != 0, out);
+  } else {
+    const int go = (mode & 1) ? nq : 0;
+    for (int l = 0; l < nq; l++) {
+      double phi, g[3], h[6];
+      lap_local_derivs(mpr + l * pst + b * C, N, dx, dy, dz, (mode & 2) != 0, false, phi, g, h);
+      if (mode & 1) out[l] += phi;
+      if (mode & 2)
+        for (int j = 0; j < 3; j++) out[go + 3 * l + j] += g[j];
+    }
+  }
+  const int nbox = 1 << level;
+  for (int ii = bi - 1; ii <= bi + 1; ii++) {
+    if (ii < 0 || ii >= nbox) continue;
+    for (int jj = bj - 1; jj <= bj + 1; jj++) {
+      if (jj < 0 || jj >= nbox) continue;
+      for (int kk = bk - 1; kk <= bk + 1; kk++) {
+        if (kk < 0 || kk >= nbox) continue;
+        const uint64_t nb = spread3(ii) | (spread3(jj) << 1) | (spread3(kk) << 2);
+        for (uint32_t s = start[nb]; s < end[nb]; s++) {
+          const double rx = px - xyz[4 * (size_t)s], ry = py - xyz[4 * (size_t)s + 1],
+                       rz = pz - xyz[4 * (size_t)s + 2];
+          if (rx * rx + ry * ry + rz * rz <= 1e-12) continue;
+          near_vec(mode, nq, rx, ry, rz, q ? q + (size_t)s * nq : nullptr,
+                   dip ? dip + (size_t)s * nq : nullptr, dip ? nrm[4 * (size_t)s] : 0.0,
+                   dip ? nrm[4 * (size_t)s + 1] : 0.0, dip ? nrm[4 * (size_t)s + 2] : 0.0, out);
+        }
+      }
+    }
+  }
+  double *o = f + (size_t)tid[t] * fstr;
+  for (int i = 0; i < nout; i++) o[i] += out[i];
+}
+
+// direct sums with vector sources (src/direct-laplace.c:87-198, :200-336):
+// the same modes and rows as k_lap_field_vec, every source, sources
+// through LDS 128 at a time; strengths nq per source (unsorted order)
+__global__ __launch_bounds__(128) void k_lap_direct_vec(
+    const double *__restrict__ xyz, const double *__restrict__ q,
+    const double *__restrict__ nrm, const double *__restrict__ dip, int nq, unsigned nsrc,
+    const double *__restrict__ xt, unsigned ntgt, int mode, double cut2,
+    double *__restrict__ f, int fstr) {
+  __shared__ double sx[128][4], sq[128][kVecMaxQ], sd[128][kVecMaxQ];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  double px = 0, py = 0, pz = 0;
+  if (t < ntgt) {
+    px = xt[4 * (size_t)t];
+    py = xt[4 * (size_t)t + 1];
+    pz = xt[4 * (size_t)t + 2];
+  }
+  double out[kVecMaxOut];
+  for (int i = 0; i < kVecMaxOut; i++) out[i] = 0.0;
+  for (unsigned s0 = 0; s0 < nsrc; s0 += 128) {
+    const unsigned s = s0 + threadIdx.x;
+    __syncthreads();
+    if (s < nsrc) {
+      for (int c = 0; c < 3; c++) sx[threadIdx.x][c] = xyz[4 * (size_t)s + c];
+      for (int c = 0; c < nq; c++) {
+        sq[threadIdx.x][c] = q ? q[(size_t)s * nq + c] : 0.0;
+        sd[threadIdx.x][c] = dip ? dip[(size_t)s * nq + c] : 0.0;
+      }
+    }
+    __syncthreads();
+    const unsigned ns = nsrc - s0 < 128 ? nsrc - s0 : 128;
+    for (unsigned k = 0; k < ns; k++) {
+      const double rx = px - sx[k][0], ry = py - sx[k][1], rz = pz - sx[k][2];
+      if (rx * rx + ry * ry + rz * rz <= cut2) continue;
+      const size_t sk = s0 + k;
+      near_vec(mode, nq, rx, ry, rz, q ? sq[k] : nullptr, dip ? sd[k] : nullptr,
+               dip ? nrm[4 * sk] : 0.0, dip ? nrm[4 * sk + 1] : 0.0, dip ? nrm[4 * sk + 2] : 0.0,
+               out);
+    }
+  }
+  if (t >= ntgt) return;
+  double *o = f + (size_t)t * fstr;
+  const int nout = vec_width(mode, nq);
+  for (int i = 0; i < nout; i++) o[i] += out[i];
+}
+
+// ---------------------------------------------------------------------
 // FP64 direct sum, LDS-tiled (src/direct-laplace.c:200-336)
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_lap_direct(
@@ -646,6 +935,35 @@ int wbfmm_hip_lap_field_f32(const double *xt, const uint32_t *tid, unsigned n,
 int wbfmm_hip_to_f32(const double *in, float *out, size_t n) {
   if (n == 0) return 0;
   WBFMM_LAUNCH("to_f32", k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, in, out, n);
+  return 0;
+}
+
+int wbfmm_hip_lap_field_vec(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip, int nq,
+                            unsigned level, const double *x0, double D, int N,
+                            const double *mpr, size_t plane_stride, int mode, double *f,
+                            int fstr) {
+  if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
+  if (nq < 1 || nq > kVecMaxQ || ((mode & 4) && nq < 3) || !(mode & 7) ||
+      vec_width(mode, nq) > kVecMaxOut || fstr < vec_width(mode, nq) || ((mode & 4) && dip))
+    return WBFMM_HIP_ERR_ARG;
+  if (n == 0) return 0;
+  WBFMM_LAUNCH("lap_l2p_p2p_vec", k_lap_field_vec, dim3(nblk(n, 128)), dim3(128), 0, xt, tid, n,
+               start, end, xyz, q, nrm, dip, nq, (int)level, x0[0], x0[1], x0[2], D, N, mpr,
+               plane_stride, mode, f, fstr);
+  return 0;
+}
+
+int wbfmm_hip_lap_direct_vec(const double *xyz, const double *q, const double *nrm,
+                             const double *dip, int nq, unsigned nsrc, const double *xt,
+                             unsigned ntgt, int mode, double cutoff, double *f, int fstr) {
+  if (nq < 1 || nq > kVecMaxQ || ((mode & 4) && nq < 3) || !(mode & 7) ||
+      vec_width(mode, nq) > kVecMaxOut || fstr < vec_width(mode, nq) || ((mode & 4) && dip))
+    return WBFMM_HIP_ERR_ARG;
+  if (ntgt == 0) return 0;
+  WBFMM_LAUNCH("lap_direct_vec", k_lap_direct_vec, dim3(nblk(ntgt, 128)), dim3(128), 0, xyz, q,
+               nrm, dip, nq, nsrc, xt, ntgt, mode, cutoff * cutoff, f, fstr);
   return 0;
 }
 

@@ -235,6 +235,8 @@ typedef struct {
 /* orders served by the FP32 passes (the FP64 matrix-core M2M / L2L serve
    M2L_MFMA_MIN..M2L_MFMA_MAX) */
 #define SHIFT_MFMA_MAX 15
+/* source components of a Laplace tree (nq) */
+#define LAP_MAX_NQ 8
 
 static tree_dev *TD(wbfmm_tree_t *t) { return (tree_dev *)t; }
 static int level_f32(tree_dev *td, guint level);
@@ -754,6 +756,17 @@ gint wbfmm_tree_coefficients_download(wbfmm_tree_t *t, guint level,
           host[((g * Cc + c) * 8 + k) * 2] = tmp[(8 * g + k) * C + 2 * c];
           host[((g * Cc + c) * 8 + k) * 2 + 1] = tmp[(8 * g + k) * C + 2 * c + 1];
         }
+  } else if (t->nq > 1) {
+    /* nq components (planes on the device): element (box 8g+k, coefficient
+       c, component q) at ((g C + c) 8 + k) nq + q, the reference's
+       coefficient stride 8 nq */
+    const size_t nq = t->nq, Cq = C / nq;
+    size_t q;
+    for (q = 0; q < nq; q++)
+      for (g = 0; g < nb / 8; g++)
+        for (k = 0; k < 8; k++)
+          for (c = 0; c < Cq; c++)
+            host[((g * Cq + c) * 8 + k) * nq + q] = tmp[(q * nb + 8 * g + k) * Cq + c];
   } else
     for (g = 0; g < nb / 8; g++)
       for (k = 0; k < 8; k++)
@@ -811,7 +824,10 @@ gint wbfmm_tree_laplace_coefficient_init(wbfmm_tree_t *t, guint l, guint nr,
   size_t nb = (size_t)1 << (3 * l);
   if ((int)t->nq < 1)
     WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
-  if (t->nq != 1) WBFMM_FATAL("only nq == 1 is supported (nq = %u)", t->nq);
+  if ((int)t->nq > LAP_MAX_NQ)
+    WBFMM_FATAL("at most %d source components are supported (nq = %u)", LAP_MAX_NQ, t->nq);
+  if (td->prec32 && t->nq != 1)
+    WBFMM_FATAL("single-precision trees take one source component (nq = %u)", t->nq);
   if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
   if (l > t->depth) WBFMM_FATAL("level %u below tree depth %u", l, t->depth);
   if (ns > WBFMM_HIP_MAX_ORDER || nr > WBFMM_HIP_MAX_ORDER)
@@ -821,17 +837,27 @@ gint wbfmm_tree_laplace_coefficient_init(wbfmm_tree_t *t, guint l, guint nr,
   t->order_s[l] = ns;
   t->order_r[l] = nr;
   {
+    /* nq components: nq planes of the box-major level array, plane q at
+       q 8^l (n+1)^2 (box pointers, when mirrored, point into plane 0) */
     const size_t es = td->prec32 ? sizeof(float) : sizeof(double);
     if (ns) {
-      alloc_level(td, l, 0, nb * (ns + 1) * (ns + 1) * es);
+      alloc_level(td, l, 0, t->nq * nb * (ns + 1) * (ns + 1) * es);
       if (td->host_mirror) set_box_pointers(td, l, 0, (ns + 1) * (ns + 1));
     }
     if (nr) {
-      alloc_level(td, l, 1, nb * (nr + 1) * (nr + 1) * es);
+      alloc_level(td, l, 1, t->nq * nb * (nr + 1) * (nr + 1) * es);
       if (td->host_mirror) set_box_pointers(td, l, 1, (nr + 1) * (nr + 1));
     }
   }
   return 0;
+}
+
+/* plane q of a Laplace level array (nq components) */
+static double *lap_plane(wbfmm_tree_t *t, guint l, int which, unsigned q)
+{
+  const unsigned N = which ? t->order_r[l] : t->order_s[l];
+  double *base = which ? (double *)t->mpr[l] : (double *)t->mps[l];
+  return base + (size_t)q * ((size_t)1 << (3 * l)) * (N + 1) * (N + 1);
 }
 
 /* strengths (host or device, element stride) into sorted device order */
@@ -875,18 +901,28 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
   if (src == NULL && dipoles == NULL) return 0;
   if (t->normals == NULL && dipoles != NULL)
     WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
-  q = gather_strengths(td, src, sstr, 0);
-  dp = gather_strengths(td, dipoles, dstr, 1);
-  if (td->prec32)
+  if (td->prec32) {
+    q = gather_strengths(td, src, sstr, 0);
+    dp = gather_strengths(td, dipoles, dstr, 1);
     HIPOK(wbfmm_hip_lap_p2m_f32(td->d_occ[d], td->nocc[d], td->d_start[d],
                                 td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
                                 dp, d, ORIGIN(t), t->D,
                                 t->order_s[d], (float *)t->mps[d]));
-  else
-    HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
-                            td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                            dp, d, ORIGIN(t), t->D,
-                            t->order_s[d], (double *)t->mps[d]));
+    return 0;
+  }
+  {
+    /* component c: strengths src[i sstr + c], dipoles[i dstr + c], into
+       plane c (src/laplace.c:854-989 with nq components) */
+    unsigned c;
+    for (c = 0; c < t->nq; c++) {
+      q = gather_strengths(td, src ? src + c : NULL, sstr, 0);
+      dp = gather_strengths(td, dipoles ? dipoles + c : NULL, dstr, 1);
+      HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
+                              td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
+                              dp, d, ORIGIN(t), t->D,
+                              t->order_s[d], lap_plane(t, d, 0, c)));
+    }
+  }
   return 0;
 }
 
@@ -1219,6 +1255,30 @@ gint wbfmm_laplace_coaxial_translate_init(gint N)
 /* Laplace passes                                                     */
 /* ------------------------------------------------------------------ */
 
+/* FP64 M2M of one component from `level' (in) into level-1 (out) */
+static void lap_m2m_plane(wbfmm_tree_t *t, ops_dev *od, guint level, const double *in,
+                          double *out)
+{
+  tree_dev *td = TD(t);
+  if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
+      (int)t->order_s[level] <= M2L_MFMA_MAX) {
+    /* matrix-core form: 16 parents per wavefront (8 wavefronts above 15) */
+    int N = (int)t->order_s[level];
+    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << level);
+    ensure_shift_tables(od, N);
+    HIPOK(wbfmm_hip_lap_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
+                                   td->d_start[level], td->d_end[level], NULL, N, in, out,
+                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
+                                   mfma_coax(od, N, 0, tt), od->sh_CSup[N], od->m2l_plan[N]));
+    return;
+  }
+  ensure_lap_coax(od);
+  HIPOK(wbfmm_hip_lap_m2m(td->d_occ[level - 1], td->nocc[level - 1],
+                          td->d_start[level], td->d_end[level], level, t->D,
+                          t->order_s[level], t->order_s[level - 1], in, out,
+                          od->d_rot, &od->lay, od->d_lap_coax));
+}
+
 gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                guint level, gdouble *work)
 {
@@ -1244,25 +1304,11 @@ gint wbfmm_laplace_upward_pass(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                        od->f_A2up[N], od->f_CSup[N], od->m2l_plan[N]));
     return 0;
   }
-  if (t->order_s[level] == t->order_s[level - 1] && (int)t->order_s[level] >= M2L_MFMA_MIN &&
-      (int)t->order_s[level] <= M2L_MFMA_MAX) {
-    /* matrix-core form: 16 parents per wavefront (8 wavefronts above 15) */
-    int N = (int)t->order_s[level];
-    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << level);
-    ensure_shift_tables(od, N);
-    HIPOK(wbfmm_hip_lap_shift_mfma(0, td->d_occ[level - 1], td->nocc[level - 1],
-                                   td->d_start[level], td->d_end[level], NULL, N,
-                                   (double *)t->mps[level], (double *)t->mps[level - 1],
-                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
-                                   mfma_coax(od, N, 0, tt), od->sh_CSup[N], od->m2l_plan[N]));
-    return 0;
+  {
+    unsigned c;
+    for (c = 0; c < t->nq; c++) /* each source component's plane */
+      lap_m2m_plane(t, od, level, lap_plane(t, level, 0, c), lap_plane(t, level - 1, 0, c));
   }
-  ensure_lap_coax(od);
-  HIPOK(wbfmm_hip_lap_m2m(td->d_occ[level - 1], td->nocc[level - 1],
-                          td->d_start[level], td->d_end[level], level, t->D,
-                          t->order_s[level], t->order_s[level - 1],
-                          (double *)t->mps[level], (double *)t->mps[level - 1],
-                          od->d_rot, &od->lay, od->d_lap_coax));
   return 0;
 }
 
@@ -1297,6 +1343,29 @@ static uint32_t *all_boxes(unsigned level, unsigned *n)
   mu_unlock(&g_all_mu);
   *n = nb;
   return list;
+}
+
+/* FP64 L2L of one component from the parents tl at `level' (in) into
+   their children at level+1 (out, added) */
+static void lap_l2l_plane(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t *tl,
+                          unsigned nt, const double *in, double *out)
+{
+  tree_dev *td = TD(t);
+  if (t->order_r[level] == t->order_r[level + 1] && (int)t->order_r[level] >= M2L_MFMA_MIN &&
+      (int)t->order_r[level] <= M2L_MFMA_MAX) {
+    int N = (int)t->order_r[level];
+    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << (level + 1));
+    ensure_shift_tables(od, N);
+    HIPOK(wbfmm_hip_lap_shift_mfma(1, tl, nt, NULL, NULL,
+                                   td->have_targets ? td->d_flag[level + 1] : NULL, N, in, out,
+                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
+                                   mfma_coax(od, N, 1, tt), od->sh_CSdn[N], od->m2l_plan[N]));
+    return;
+  }
+  ensure_lap_coax(od);
+  HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
+                          level, t->D, t->order_r[level], t->order_r[level + 1], in, out,
+                          od->d_rot, &od->lay, od->d_lap_coax));
 }
 
 static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
@@ -1345,48 +1414,40 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   }
   if (t->order_s[level] == t->order_r[level] &&
       (int)t->order_s[level] >= M2L_MFMA_MIN && (int)t->order_s[level] <= M2L_MFMA_MAX) {
-    /* matrix-core form: targets grouped by Morton child digit */
+    /* matrix-core form: targets grouped by Morton child digit (once for
+       every component) */
     int N = (int)t->order_s[level];
-    unsigned cstart[8], ccount[8];
+    unsigned cstart[8], ccount[8], c;
     uint32_t *cls;
     ensure_m2l_plan(od, N);
     HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
     HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
-    HIPOK(wbfmm_hip_lap_m2l_mfma(cls, cstart, ccount, td->d_start[level],
-                                 td->d_end[level], level, t->D, N,
-                                 (double *)t->mps[level], (double *)t->mpr[level],
-                                 od->m2l_plan_dev[N], od->m2l_A1[N], od->m2l_A3[N],
-                                 m2l_coax(od, N, t->D / (double)(1u << level)),
-                                 od->m2l_CS[N], od->m2l_plan[N]));
+    for (c = 0; c < t->nq; c++)
+      HIPOK(wbfmm_hip_lap_m2l_mfma(cls, cstart, ccount, td->d_start[level],
+                                   td->d_end[level], level, t->D, N,
+                                   lap_plane(t, level, 0, c), lap_plane(t, level, 1, c),
+                                   od->m2l_plan_dev[N], od->m2l_A1[N], od->m2l_A3[N],
+                                   m2l_coax(od, N, t->D / (double)(1u << level)),
+                                   od->m2l_CS[N], od->m2l_plan[N]));
     wbfmm_hip_free(cls);
   } else {
+    unsigned c;
     ensure_lap_coax(od);
-    HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
-                            t->D, t->order_s[level], t->order_r[level],
-                            (double *)t->mps[level], (double *)t->mpr[level],
-                            od->d_rot, &od->lay, od->d_lap_coax));
+    for (c = 0; c < t->nq; c++)
+      HIPOK(wbfmm_hip_lap_m2l(tl, nt, td->d_start[level], td->d_end[level], level,
+                              t->D, t->order_s[level], t->order_r[level],
+                              lap_plane(t, level, 0, c), lap_plane(t, level, 1, c),
+                              od->d_rot, &od->lay, od->d_lap_coax));
   }
   if (level == t->depth) return 0;
   if (!t->mpr[level + 1]) WBFMM_FATAL("coefficients not initialised at level %u", level + 1);
   if ((int)t->order_r[level + 1] > od->lay.L) WBFMM_FATAL("operators built to order %d", od->lay.L);
-  if (t->order_r[level] == t->order_r[level + 1] && (int)t->order_r[level] >= M2L_MFMA_MIN &&
-      (int)t->order_r[level] <= M2L_MFMA_MAX) {
-    int N = (int)t->order_r[level];
-    double tt = 0.5 * sqrt(3.0) * t->D / (double)(1u << (level + 1));
-    ensure_shift_tables(od, N);
-    HIPOK(wbfmm_hip_lap_shift_mfma(1, tl, nt, NULL, NULL,
-                                   td->have_targets ? td->d_flag[level + 1] : NULL, N,
-                                   (double *)t->mpr[level], (double *)t->mpr[level + 1],
-                                   od->m2l_plan_dev[N], od->sh_A1[N], od->sh_A3[N],
-                                   mfma_coax(od, N, 1, tt), od->sh_CSdn[N], od->m2l_plan[N]));
-    return 0;
+  {
+    unsigned c;
+    for (c = 0; c < t->nq; c++)
+      lap_l2l_plane(t, od, level, tl, nt, lap_plane(t, level, 1, c),
+                    lap_plane(t, level + 1, 1, c));
   }
-  ensure_lap_coax(od);
-  HIPOK(wbfmm_hip_lap_l2l(tl, nt, td->have_targets ? td->d_flag[level + 1] : NULL,
-                          level, t->D,
-                          t->order_r[level], t->order_r[level + 1],
-                          (double *)t->mpr[level], (double *)t->mpr[level + 1],
-                          od->d_rot, &od->lay, od->d_lap_coax));
   return 0;
 }
 
@@ -1450,6 +1511,15 @@ static void field_finish(double *f, double *d, size_t nelem, int staged)
   wbfmm_hip_free(d);
 }
 
+/* values per target of a Laplace field request with nq components:
+   field bit 1 the potentials (nq), bit 2 the gradients (3 nq), or bit 4
+   the curl (3) and, with bit 2, its gradient (9 more) */
+static int lap_vec_width(guint field, int nq)
+{
+  if (field & WBFMM_FIELD_CURL) return (field & WBFMM_FIELD_GRADIENT) ? 12 : 3;
+  return ((field & 1) ? nq : 0) + ((field & 2) ? 3 * nq : 0);
+}
+
 static void lap_field_run(tree_dev *td, const double *dxt,
                           const uint32_t *dtid, unsigned n, double *src,
                           gint sstr, double *d, gint dstr, int field,
@@ -1463,6 +1533,24 @@ static void lap_field_run(tree_dev *td, const double *dxt,
   if (!t->mpr[lv]) WBFMM_FATAL("coefficients not initialised at level %u", lv);
   if (t->normals == NULL && d != NULL)
     WBFMM_FATAL("no normals in tree but dipole strengths specified (d != NULL)");
+  if (t->nq > 1 || (field & 4)) {
+    /* vector sources: nq strengths per point, the potentials and
+       gradients of every component, or the curl (and its gradient) */
+    const int width = lap_vec_width(field, (int)t->nq);
+    if ((field & 4) && d) WBFMM_FATAL("the curl is defined for monopole sources only");
+    q = gather_strengths_n(td, src, sstr, 0, (int)t->nq);
+    dp = gather_strengths_n(td, d, dstr, 1, (int)t->nq);
+    nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + width : 0;
+    df = field_begin(f, nelem, &staged);
+    HIPOK(wbfmm_hip_lap_field_vec(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
+                                  dp ? td->d_nrm : NULL, dp, (int)t->nq, lv, ORIGIN(t), t->D,
+                                  t->order_r[lv], (const double *)t->mpr[lv],
+                                  ((size_t)1 << (3 * lv)) * (t->order_r[lv] + 1) *
+                                      (t->order_r[lv] + 1),
+                                  field, df, fstr));
+    field_finish(f, df, nelem, staged);
+    return;
+  }
   q = gather_strengths(td, src, sstr, 0);
   dp = gather_strengths(td, d, dstr, 1);
   nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((field & 1) ? 1 : 0) + ((field & 2) ? 3 : 0) : 0;
@@ -1489,10 +1577,11 @@ gint wbfmm_laplace_field_targets(wbfmm_tree_t *t, gdouble *xt, gsize tstr,
   double *dx;
   uint32_t *did;
   uint64_t *dc;
-  int fl = field & 3;
+  int fl = field & 7;
   if (t->problem != WBFMM_PROBLEM_LAPLACE) return -2;
-  if (fl == 0 || (field & ~3u)) return -2;
-  if (fstr < ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0)) return -2;
+  if (fl == 0 || (field & ~7u)) return -2;
+  if ((fl & 4) && t->nq < 3) return -2;
+  if (fstr < lap_vec_width(fl, (int)t->nq)) return -2;
   if (ntgt == 0) return 0;
   stage_targets(xt, tstr, ntgt, &dx, &did, &dc, ORIGIN(t),
                 t->D, 1);
@@ -1519,27 +1608,99 @@ gint wbfmm_laplace_box_field(wbfmm_tree_t *t, guint level, guint b,
   (void)b;
   if (t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
   if (level != t->depth) WBFMM_FATAL("field evaluation is at the leaf level only");
+  /* the field types of src/local-laplace.c:1293-1336 */
   switch (field) {
     case WBFMM_FIELD_POTENTIAL: fl = 1; break;
     case WBFMM_FIELD_GRADIENT:
     case WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 2; break;
-    default: WBFMM_FATAL("unsupported field type %u", field);
+    case WBFMM_FIELD_CURL:
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_POTENTIAL: fl = 4; break;
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_GRADIENT:
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 6; break;
+    default: WBFMM_FATAL("unrecognized field type %u", field);
   }
-  if (fl == 2 && fstr < 3) WBFMM_FATAL("field data stride (%d) must be greater than two", fstr);
+  if (fl == 2 && fstr < 3 && t->nq != 1)
+    WBFMM_FATAL("field data stride (%d) must be greater than two", fstr);
+  if ((fl & 4) && t->nq < 3)
+    WBFMM_FATAL("not enough source components (%u) for curl calculation", t->nq);
   stage_targets(x, 3 * sizeof(double), 1, &dx, &did, &dc,
                 ORIGIN(t), t->D, 0);
   if (!eval_neighbours) { src = NULL; d = NULL; }
   {
-    /* f holds 1 (potential) or 3 (gradient) values at unit stride */
-    double tmp[4] = {0, 0, 0, 0};
-    int k;
-    lap_field_run(td, dx, did, 1, src, sstr, d, dstr, fl, tmp, 4, 1);
-    if (fl == 1) f[0] += tmp[0];
-    else for (k = 0; k < 3; k++) f[k] += tmp[k];
+    /* potential: component k at f[k fstr]; gradient: d_j of component k
+       at f[k fstr + j]; curl: f[0..2]; curl and gradient: f[0..11] */
+    double tmp[4 * LAP_MAX_NQ];
+    const int width = lap_vec_width(fl, (int)t->nq);
+    int k, j;
+    memset(tmp, 0, sizeof(tmp));
+    lap_field_run(td, dx, did, 1, src, sstr, d, dstr, fl, tmp, width, 1);
+    if (fl == 1)
+      for (k = 0; k < (int)t->nq; k++) f[k * fstr] += tmp[k];
+    else if (fl == 2)
+      for (k = 0; k < (int)t->nq; k++)
+        for (j = 0; j < 3; j++) f[k * fstr + j] += tmp[3 * k + j];
+    else
+      for (k = 0; k < width; k++) f[k] += tmp[k];
   }
   wbfmm_hip_free(dx);
   wbfmm_hip_free(did);
   wbfmm_hip_free(dc);
+  return 0;
+}
+
+/* the direct sums with vector sources (src/direct-laplace.c:87-198,
+   :200-336): potential f[j fstr], gradient f[j fstr + c] of component j;
+   curl f[0..2] and its gradient f[3..11] (d_c u_i at 3 + 3c + i), the
+   curl terms skipping |x - y| <= 1e-6 as the reference's do */
+static gint lap_direct_vec(gdouble *xs, gint xstride, gdouble *n, gint nstr, gint nsrc,
+                           gdouble *src, gint sstr, gdouble *d, gint dstr, gint nq, int fl,
+                           gdouble *xf, gdouble *f, gint fstr)
+{
+  double *dxs, *dq = NULL, *dd = NULL, *dn = NULL, *dxt, *df, tmp[4 * LAP_MAX_NQ];
+  uint32_t *ids;
+  const int width = lap_vec_width((guint)fl, nq);
+  int j, c;
+  HIPOK(wbfmm_hip_malloc((void **)&dxs, sizeof(double) * 4 * (nsrc ? nsrc : 1)));
+  HIPOK(wbfmm_hip_stage4(xs, xstride * sizeof(double), nsrc, dxs));
+  HIPOK(wbfmm_hip_malloc((void **)&ids, sizeof(uint32_t) * (nsrc ? nsrc : 1)));
+  {
+    uint32_t *h = malloc(sizeof(uint32_t) * (nsrc ? nsrc : 1));
+    gint i2;
+    if (!h) WBFMM_FATAL("out of host memory");
+    for (i2 = 0; i2 < nsrc; i2++) h[i2] = i2;
+    HIPOK(wbfmm_hip_memcpy(ids, h, sizeof(uint32_t) * nsrc));
+    free(h);
+  }
+  if (src) {
+    HIPOK(wbfmm_hip_malloc((void **)&dq, sizeof(double) * nq * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_gather_strided(src, sstr, nq, ids, nsrc, dq));
+  }
+  if (d) {
+    HIPOK(wbfmm_hip_malloc((void **)&dd, sizeof(double) * nq * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_gather_strided(d, dstr, nq, ids, nsrc, dd));
+    HIPOK(wbfmm_hip_malloc((void **)&dn, sizeof(double) * 4 * (nsrc ? nsrc : 1)));
+    HIPOK(wbfmm_hip_stage4(n, nstr * sizeof(double), nsrc, dn));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&dxt, sizeof(double) * 4));
+  {
+    double h[4] = {xf[0], xf[1], xf[2], 0.0};
+    HIPOK(wbfmm_hip_memcpy(dxt, h, sizeof(h)));
+  }
+  HIPOK(wbfmm_hip_malloc((void **)&df, sizeof(double) * width));
+  HIPOK(wbfmm_hip_memset(df, 0, sizeof(double) * width));
+  HIPOK(wbfmm_hip_lap_direct_vec(dxs, dq, dn, dd, nq, (unsigned)nsrc, dxt, 1, fl,
+                                 (fl & 4) ? 1e-6 : 0.0, df, width));
+  HIPOK(wbfmm_hip_memcpy(tmp, df, sizeof(double) * width));
+  if (fl == 1)
+    for (j = 0; j < nq; j++) f[j * fstr] += tmp[j];
+  else if (fl == 2)
+    for (j = 0; j < nq; j++)
+      for (c = 0; c < 3; c++) f[j * fstr + c] += tmp[3 * j + c];
+  else
+    for (j = 0; j < width; j++) f[j] += tmp[j];
+  wbfmm_hip_free(dxs); wbfmm_hip_free(ids);
+  wbfmm_hip_free(dq); wbfmm_hip_free(dd); wbfmm_hip_free(dn);
+  wbfmm_hip_free(dxt); wbfmm_hip_free(df);
   return 0;
 }
 
@@ -1555,14 +1716,28 @@ gint wbfmm_laplace_field_direct(gdouble *xs, gint xstride, gdouble *n,
   uint32_t *ids = NULL;
   int fl, i;
   double tmp[4] = {0, 0, 0, 0};
-  if (nq != 1) WBFMM_FATAL("only nq == 1 is supported");
+  if (src == NULL && d == NULL) return 0;
+  if (n != NULL && d == NULL)
+    WBFMM_FATAL("normals specified but no dipole strengths (d == NULL)");
   switch (field) {
     case WBFMM_FIELD_POTENTIAL: fl = 1; break;
     case WBFMM_FIELD_GRADIENT:
     case WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 2; break;
-    default: WBFMM_FATAL("unsupported field type %u", field);
+    case WBFMM_FIELD_CURL:
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_POTENTIAL: fl = 4; break;
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_GRADIENT:
+    case WBFMM_FIELD_CURL | WBFMM_FIELD_GRADIENT | WBFMM_FIELD_POTENTIAL: fl = 6; break;
+    default: WBFMM_FATAL("unrecognized field type %u", field);
   }
-  (void)fstr;
+  if (nq < 1 || nq > LAP_MAX_NQ)
+    WBFMM_FATAL("%d source components (1 to %d are supported)", nq, LAP_MAX_NQ);
+  if ((fl & 4) && nq < 3)
+    WBFMM_FATAL("not enough source components (%d) for curl calculation", nq);
+  if (fl == 2 && fstr < 3 && nq > 1)
+    WBFMM_FATAL("field data stride (%d) must be greater than two", fstr);
+  if ((fl & 4) && d) WBFMM_FATAL("the curl is defined for monopole sources only");
+  if (nq > 1 || (fl & 4))
+    return lap_direct_vec(xs, xstride, n, nstr, nsrc, src, sstr, d, dstr, nq, fl, xf, f, fstr);
   HIPOK(wbfmm_hip_malloc((void **)&dxs, sizeof(double) * 4 * (nsrc ? nsrc : 1)));
   HIPOK(wbfmm_hip_stage4(xs, xstride * sizeof(double), nsrc, dxs));
   if (src) {
@@ -1670,6 +1845,8 @@ gint wbfmm_laplace_target_list_local_coefficients(wbfmm_target_list_t *l,
   (void)work;
   l->source = source;
   if (l->t->problem != WBFMM_PROBLEM_LAPLACE) WBFMM_FATAL("tree is not a Laplace tree");
+  if (l->t->nq != 1)
+    WBFMM_FATAL("target lists serve one source component (nq = %u)", l->t->nq);
   return 0;
 }
 
