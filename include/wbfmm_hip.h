@@ -204,6 +204,24 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
                            const double *A3, const double *A2,
                            const double *CS,
                            const wbfmm_m2l_plan_t *plan_host);
+/* wbfmm_hip_lap_m2l_mfma without any device allocation: no counterpart
+   in the reference; for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops).  Same arguments, kernels
+   and results as wbfmm_hip_lap_m2l_mfma; its scratch (zero expansion,
+   occupancy bitmap, split partials) is a 64 MiB device array of the
+   library, one per GPU, shared by every call on that GPU, so calls that
+   overlap on one GPU race.  Refused with WBFMM_HIP_ERR_ARG and a message
+   (wbfmm_hip_last_error) before any launch: levels above 8, and levels
+   under 1024 chunks of 16 targets whose split partials
+   (blocks per chunk x targets x (N+1)^2 x 8 bytes) exceed 62 MiB. */
+int wbfmm_hip_lap_m2l_mfma_noalloc(const uint32_t *cls, const unsigned *cstart,
+                                   const unsigned *ccount, const uint32_t *sstart,
+                                   const uint32_t *send, unsigned level, double D,
+                                   int N, const double *mps, double *mpr,
+                                   const void *plan_dev, const double *A1,
+                                   const double *A3, const double *A2,
+                                   const double *CS,
+                                   const wbfmm_m2l_plan_t *plan_host);
 /* tables of the MFMA M2M / L2L for plan P (same bins and slot maps):
    A1, A3: 2 x KT x 64 (polar angles acos(1/sqrt3), acos(-1/sqrt3));
    CSup, CSdn: 8 x (N+1) x 2, cos and sin of m chi of child c's

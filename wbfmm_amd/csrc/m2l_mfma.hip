@@ -146,11 +146,52 @@ static int shift_entry(int mode, const uint32_t *parents, unsigned npar,
                              A1, A3, A2, CS, zero, plan_host);
 }
 
+// Scratch of the allocation-free M2L (wbfmm_hip_lap_m2l_mfma_noalloc):
+// one file-scope device array per GPU, laid out as
+//   [0, 16 KiB)            the zero expansion read in place of empty boxes,
+//   [16 KiB, +2 MiB)       the occupancy bitmap of the source level
+//                          (8^8 bits: levels up to 8),
+//   [16 KiB + 2 MiB, end)  the partial expansions of the split form.
+// Host code reaches it only through the address from hipGetSymbolAddress,
+// taken once per device and kept in g_scratch_dev.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr size_t kScratchOcc = (size_t)16 << 10;
+constexpr size_t kScratchPart = kScratchOcc + ((size_t)2 << 20);
+constexpr unsigned kScratchMaxLevel = 8;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static unsigned char *g_scratch_dev[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(unsigned char **out) {
+  int dev = 0;
+  WBFMM_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail_msg("M2L scratch: device index out of range", WBFMM_HIP_ERR_ARG);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return fail("hipGetSymbolAddress(scratch)", e);
+    g_scratch_dev[dev] = (unsigned char *)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+__global__ void k_fill_zero_u32(uint32_t *p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// ext: scratch of the allocation-free entry point (zero expansion,
+// occupancy bits and partials at the offsets above), or null to allocate
 template <typename T>
 static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned *ccount,
                      const uint32_t *sstart, const uint32_t *send, unsigned level, int N,
                      const T *mps, T *mpr, const void *plan_dev, const T *A1, const T *A3,
-                     const T *A2, const T *CS, const wbfmm_m2l_plan_t *plan_host) {
+                     const T *A2, const T *CS, const wbfmm_m2l_plan_t *plan_host,
+                     unsigned char *ext = nullptr) {
   ClassArgs ca;
   unsigned nch = 0;
   for (int c = 0; c < 8; c++) {
@@ -168,7 +209,36 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
   const T *zero = nullptr;
   uint32_t *occ = nullptr;
   T *part = nullptr;
-  {
+  if (ext) {
+    // every size is checked before anything is launched
+    char msg[256];
+    if (level > kScratchMaxLevel) {
+      snprintf(msg, sizeof(msg),
+               "M2L without allocation: level %u above %u (the occupancy bitmap of "
+               "8^level bits would exceed its %zu-byte scratch region)",
+               level, kScratchMaxLevel, kScratchPart - kScratchOcc);
+      return fail_msg(msg, WBFMM_HIP_ERR_ARG);
+    }
+    unsigned ntot = 0;
+    for (int c = 0; c < 8; c++) ntot += ccount[c];
+    const int nsplit = m2l_nsplit(nch);
+    const size_t need =
+        nsplit > 1 ? sizeof(T) * (size_t)nsplit * ntot * (size_t)((N + 1) * (N + 1)) : 0;
+    if (need > kScratchBytes - kScratchPart) {
+      snprintf(msg, sizeof(msg),
+               "M2L without allocation: %u targets at order %d need %zu bytes of split "
+               "partials, above the %zu-byte scratch region",
+               ntot, N, need, kScratchBytes - kScratchPart);
+      return fail_msg(msg, WBFMM_HIP_ERR_ARG);
+    }
+    // the zero expansion must read zero whatever an earlier call left
+    const size_t zw = kScratchOcc / sizeof(uint32_t);
+    WBFMM_LAUNCH("m2l_scratch_zero", k_fill_zero_u32, dim3((unsigned)((zw + 255) / 256)),
+                 dim3(256), 0, (uint32_t *)ext, zw);
+    zero = (const T *)ext;
+    occ = (uint32_t *)(ext + kScratchOcc);
+    part = nsplit > 1 ? (T *)(ext + kScratchPart) : nullptr;
+  } else {
     const int rz = ensure_zero();
     if (rz) return rz;
     zero = (const T *)g_zero;
@@ -179,7 +249,7 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
                sstart, send, nbox, occ);
   const int rc = lap_m2l_orders<T>(N, cls, ca, nch, occ, (int)level, mps, mpr, plan_dev, A1, A3,
                                    A2, CS, zero, plan_host, part);
-  dfree(occ);
+  if (!ext) dfree(occ);
   return rc;
 }
 
@@ -272,6 +342,22 @@ int wbfmm_hip_lap_m2l_mfma(const uint32_t *cls, const unsigned *cstart,
   (void)D;
   return m2l_entry<double>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev, A1,
                            A3, A2, CS, plan_host);
+}
+
+int wbfmm_hip_lap_m2l_mfma_noalloc(const uint32_t *cls, const unsigned *cstart,
+                                   const unsigned *ccount, const uint32_t *sstart,
+                                   const uint32_t *send, unsigned level, double D,
+                                   int N, const double *mps, double *mpr,
+                                   const void *plan_dev, const double *A1,
+                                   const double *A3, const double *A2,
+                                   const double *CS,
+                                   const wbfmm_m2l_plan_t *plan_host) {
+  (void)D;
+  unsigned char *ext = nullptr;
+  const int rc = scratch_address(&ext);
+  if (rc) return rc;
+  return m2l_entry<double>(cls, cstart, ccount, sstart, send, level, N, mps, mpr, plan_dev, A1,
+                           A3, A2, CS, plan_host, ext);
 }
 
 int wbfmm_hip_lap_m2l_mfma_f32(const uint32_t *cls, const unsigned *cstart,
