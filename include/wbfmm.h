@@ -449,6 +449,39 @@ gint wbfmm_target_list_local_field_f(wbfmm_target_list_t *l, gfloat *src,
                                      gint sstr, gfloat *d, gint dstr,
                                      gfloat *f, gint fstr);
 
+/* The Helmholtz pipeline with float arguments (src/wbfmm.h:938-951,
+   :997-1003, :1059-1082, :1102-1104).  A tree made by wbfmm_tree_new_f
+   runs the mixed-precision Helmholtz path (as after
+   wbfmm_tree_set_precision(t, 32)); complex strengths are pairs of floats
+   at element stride, float outputs are accumulated in float. */
+gint wbfmm_tree_coefficient_init_f(wbfmm_tree_t *t, guint l, guint nr, guint ns);
+gint wbfmm_tree_leaf_expansions_f(wbfmm_tree_t *t, gfloat k, gfloat *src, gint sstr,
+                                  gfloat *dipoles, gint dstr, gboolean zero_expansions,
+                                  gfloat *work);
+gint wbfmm_shift_operators_coaxial_SR_init_f(wbfmm_shift_operators_t *w, gfloat D,
+                                             guint level, guint L, gfloat k, gfloat *work);
+gint wbfmm_shift_operators_coaxial_SS_init_f(wbfmm_shift_operators_t *w, gfloat D,
+                                             guint level, guint L, gfloat k, gfloat *work);
+gint wbfmm_upward_pass_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                         gfloat *work);
+gint wbfmm_downward_pass_ref_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                               gfloat *work, gint nthreads);
+#define wbfmm_downward_pass_f(_t, _op, _level, _work, _nth) \
+  wbfmm_downward_pass_ref_f(_t, _op, _level, _work, _nth)
+gint wbfmm_tree_box_local_field_f(wbfmm_tree_t *t, guint level, guint b, gfloat k,
+                                  gfloat *x, gfloat *f, gint fstr, gfloat *src, gint sstr,
+                                  gfloat *d, gint dstr, gboolean eval_neighbours,
+                                  guint field, gfloat *work);
+gint wbfmm_target_list_local_coefficients_f(wbfmm_target_list_t *l, gfloat k,
+                                            gfloat *work);
+gint wbfmm_total_field_f(gfloat k, gfloat *xs, gint xstride, gfloat *src, gint sstride,
+                         gfloat *normals, gint nstr, gfloat *dipoles, gint dstr, gint nq,
+                         gint nsrc, gfloat *xf, gfloat *field, gint fstr);
+gint wbfmm_total_field_grad_f(gfloat k, gfloat *xs, gint xstride, gfloat *src,
+                              gint sstride, gfloat *normals, gint nstr, gfloat *dipoles,
+                              gint dstr, gint nq, gint nsrc, gfloat *xf, gfloat *field,
+                              gint fstr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -191,3 +191,128 @@ def test_box_field_kernel_fp64_matches_thread_kernel(wb, monkeypatch):
     f_box = solve(wb, xs, q, d, xt, x0, D, 4, orders, k, 64)
     r = colrel(f_box, f_thr)
     assert r.max() < 1e-12, r
+
+
+def _hel_f_api(wb):
+    """ctypes signatures of the Helmholtz single-precision entry points"""
+    L = wb.lib()
+    vp, i, u, fl = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_float
+    sz = ctypes.c_size_t
+    sigs = {
+        "wbfmm_tree_new_f": (ctypes.POINTER(wb.WbfmmTree), [vp, fl, u]),
+        "wbfmm_tree_add_points_f": (i, [vp, vp, sz, vp, sz, u, i]),
+        "wbfmm_tree_refine_f": (i, [vp]),
+        "wbfmm_tree_coefficient_init_f": (i, [vp, u, u, u]),
+        "wbfmm_tree_leaf_expansions_f": (i, [vp, fl, vp, i, vp, i, i, vp]),
+        "wbfmm_shift_angle_table_init_f": (i, []),
+        "wbfmm_shift_operators_new_f": (vp, [u, i, vp]),
+        "wbfmm_shift_operators_coaxial_SR_init_f": (i, [vp, fl, u, u, fl, vp]),
+        "wbfmm_shift_operators_coaxial_SS_init_f": (i, [vp, fl, u, u, fl, vp]),
+        "wbfmm_upward_pass_f": (i, [vp, vp, u, vp]),
+        "wbfmm_downward_pass_ref_f": (i, [vp, vp, u, vp, i]),
+        "wbfmm_tree_box_local_field_f": (i, [vp, u, u, fl, vp, vp, i, vp, i, vp, i, i, u, vp]),
+        "wbfmm_target_list_new_f": (vp, [vp, u]),
+        "wbfmm_target_list_add_points_f": (i, [vp, vp, sz, u]),
+        "wbfmm_target_list_coefficients_init": (i, [vp, u]),
+        "wbfmm_target_list_local_coefficients_f": (i, [vp, fl, vp]),
+        "wbfmm_target_list_local_field_f": (i, [vp, vp, i, vp, i, vp, i]),
+        "wbfmm_target_list_free": (i, [vp]),
+        "wbfmm_total_field_f": (i, [fl, vp, i, vp, i, vp, i, vp, i, i, i, vp, vp, i]),
+        "wbfmm_total_field_grad_f": (i, [fl, vp, i, vp, i, vp, i, vp, i, i, i, vp, vp, i]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def test_single_precision_helmholtz_api(wb, oracle):
+    """the float Helmholtz entry points (src/wbfmm.h:938-951, :997-1003,
+    :1059-1082, :1102-1104) run the mixed-precision tree: with the same
+    float inputs the target-list field equals the double entry points'
+    on a mixed tree, rounded to float; the box field and the direct sums
+    agree with the double ones to float rounding"""
+    L = _hel_f_api(wb)
+    Pf = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rng = np.random.default_rng(777)
+    n, nt, depth, p, k = 3000, 400, 4, 12, 3.0
+    v = rng.normal(size=(n, 3))
+    xs = (v / np.linalg.norm(v, axis=1)[:, None]).astype(np.float32)
+    xt = (0.8 * xs[::7]).astype(np.float32)[:nt]
+    nt = len(xt)
+    q = (rng.normal(size=(n, 2))).astype(np.float32)
+    d = (rng.normal(size=(n, 2))).astype(np.float32)
+    x0 = np.full(3, -1.02, np.float32)
+    D = np.float32(2.04)
+    kf = np.float32(k)
+
+    tp = L.wbfmm_tree_new_f(Pf(x0), D, n)
+    t = ctypes.cast(tp, ctypes.c_void_p).value
+    assert L.wbfmm_tree_add_points_f(t, Pf(xs), 12, Pf(xs), 12, n, 0) == 0
+    for _ in range(depth):
+        L.wbfmm_tree_refine_f(t)
+    tp.contents.problem = 2
+    tp.contents.nq = 1
+    for lv in range(1, depth + 1):
+        L.wbfmm_tree_coefficient_init_f(t, lv, p, p)
+    L.wbfmm_shift_angle_table_init_f()
+    op = L.wbfmm_shift_operators_new_f(p, 0, None)
+    for lv in range(2, depth + 1):
+        assert L.wbfmm_shift_operators_coaxial_SR_init_f(op, D, lv, p, kf, None) == 0
+    for lv in range(3, depth + 1):
+        assert L.wbfmm_shift_operators_coaxial_SS_init_f(op, D, lv, p, kf, None) == 0
+    tl = L.wbfmm_target_list_new_f(t, nt)
+    L.wbfmm_target_list_add_points_f(tl, Pf(xt), 12, nt)
+    L.wbfmm_tree_leaf_expansions_f(t, kf, Pf(q), 2, Pf(d), 2, 1, None)
+    for lv in range(depth, 2, -1):
+        L.wbfmm_upward_pass_f(t, op, lv, None)
+    for lv in range(2, depth + 1):
+        L.wbfmm_downward_pass_ref_f(t, op, lv, None, 0)
+    L.wbfmm_target_list_coefficients_init(tl, 3)
+    L.wbfmm_target_list_local_coefficients_f(tl, kf, None)
+    ff = np.zeros((nt, 8), np.float32)
+    L.wbfmm_target_list_local_field_f(tl, Pf(q), 2, Pf(d), 2, Pf(ff), 8)
+    x5 = np.ascontiguousarray(xt[5])
+    fb = np.zeros(2, np.float32)
+    L.wbfmm_tree_box_local_field_f(t, depth, 0, kf, Pf(x5), Pf(fb), 2, Pf(q), 2, Pf(d), 2, 1, 1,
+                                   None)
+    fdp = np.zeros(2, np.float32)
+    fdg = np.zeros(6, np.float32)
+    L.wbfmm_total_field_f(kf, Pf(xs), 3, Pf(q), 2, Pf(xs), 3, Pf(d), 2, 1, n, Pf(x5), Pf(fdp), 1)
+    L.wbfmm_total_field_grad_f(kf, Pf(xs), 3, Pf(q), 2, Pf(xs), 3, Pf(d), 2, 1, n, Pf(x5),
+                               Pf(fdg), 1)
+    wb.sync()
+    L.wbfmm_target_list_free(tl)
+    wb.lib().wbfmm_shift_operators_free(op)
+    wb.lib().wbfmm_tree_free(t)
+
+    # the double entry points on a mixed tree with the same (float) inputs
+    x64, q64, d64 = xs.astype(np.float64), q.astype(np.float64), d.astype(np.float64)
+    xt64 = xt.astype(np.float64)
+    S = wb.HelmholtzSolver(np.ascontiguousarray(x64), x0.astype(np.float64), float(D), depth,
+                           [p] * 17, [p] * 17, float(kf), normals=np.ascontiguousarray(x64),
+                           targets=np.ascontiguousarray(xt64), precision=32)
+    S.build_tree()
+    S.passes(np.ascontiguousarray(q64), d=np.ascontiguousarray(d64))
+    Ld = wb.lib()
+    tl2 = Ld.wbfmm_target_list_new(S.t, nt)
+    Ld.wbfmm_target_list_add_points(tl2, wb.ptr(np.ascontiguousarray(xt64)), 24, nt)
+    Ld.wbfmm_target_list_coefficients_init(tl2, 3)
+    Ld.wbfmm_target_list_local_coefficients.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                                        ctypes.c_void_p]
+    Ld.wbfmm_target_list_local_coefficients(tl2, float(kf), None)
+    g = np.zeros((nt, 8))
+    Ld.wbfmm_target_list_local_field(tl2, wb.ptr(np.ascontiguousarray(q64)), 2,
+                                     wb.ptr(np.ascontiguousarray(d64)), 2, wb.ptr(g), 8)
+    wb.sync()
+    Ld.wbfmm_target_list_free(tl2)
+    S.close()
+    assert np.array_equal(ff, g.astype(np.float32))
+    assert np.abs(fb - g[5, :2]).max() <= 2e-6 * np.abs(g[5, :2]).max()
+    zq, zd = q64[:, 0] + 1j * q64[:, 1], d64[:, 0] + 1j * d64[:, 1]
+    fd = oracle.hel_direct(float(kf), x64, zq, x5[None].astype(np.float64), field=3, nrm=x64, d=zd)
+    assert np.allclose(fdp, [fd[0, 0].real, fd[0, 0].imag], rtol=2e-6, atol=0)
+    want_g = np.array([fd[0, 1].real, fd[0, 1].imag, fd[0, 2].real, fd[0, 2].imag,
+                       fd[0, 3].real, fd[0, 3].imag])
+    assert np.abs(fdg - want_g).max() <= 2e-6 * np.abs(want_g).max()

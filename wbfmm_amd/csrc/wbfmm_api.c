@@ -3151,16 +3151,16 @@ gint wbfmm_target_list_local_field_f(wbfmm_target_list_t *l, gfloat *src, gint s
 {
   size_t nt = l->npoints, i;
   int fl = (int)(l->field & 3), nv, k;
+  /* Laplace: real strengths, 1 + 3 values; Helmholtz: complex, 2 + 6 */
+  const int cw = l->t->problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
   double *q, *dd, *fd;
-  if (l->t->problem != WBFMM_PROBLEM_LAPLACE)
-    WBFMM_FATAL("single-precision target lists serve Laplace trees");
-  nv = ((fl & 1) ? 1 : 0) + ((fl & 2) ? 3 : 0);
+  nv = ((fl & 1) ? cw : 0) + ((fl & 2) ? 3 * cw : 0);
   if (fstr < nv) WBFMM_FATAL("field stride %d too small", fstr);
-  q = widen(src, sstr, 1, l->t->npoints);
-  dd = widen(d, dstr, 1, l->t->npoints);
+  q = widen(src, sstr, cw, l->t->npoints);
+  dd = widen(d, dstr, cw, l->t->npoints);
   fd = calloc((nt ? nt : 1) * nv, sizeof(double));
   if (!fd) WBFMM_FATAL("out of host memory");
-  wbfmm_target_list_local_field(l, q, 1, dd, 1, fd, nv);
+  wbfmm_target_list_local_field(l, q, cw, dd, cw, fd, nv);
   HIPOK(wbfmm_hip_sync());
   for (i = 0; i < nt; i++)
     for (k = 0; k < nv; k++) f[i * fstr + k] += (gfloat)fd[i * nv + k];
@@ -3168,4 +3168,135 @@ gint wbfmm_target_list_local_field_f(wbfmm_target_list_t *l, gfloat *src, gint s
   free(dd);
   free(fd);
   return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Single-precision entry points of the Helmholtz pipeline              */
+/* (src/wbfmm.h:869-1112).  A tree made by wbfmm_tree_new_f runs the    */
+/* mixed-precision Helmholtz path: FP32 expansions scaled per degree    */
+/* and FP32 matrix-core translations at the levels where k a_l is small */
+/* and the orders are served, FP64 elsewhere, with FP64 tables and      */
+/* recurrences.  Float arguments are host arrays widened to double on   */
+/* the host (complex strengths: 2 floats at element stride); float      */
+/* outputs are accumulated in float.                                    */
+/* ------------------------------------------------------------------ */
+
+gint wbfmm_tree_coefficient_init_f(wbfmm_tree_t *t, guint l, guint nr, guint ns)
+{
+  require_f(t);
+  return wbfmm_tree_coefficient_init(t, l, nr, ns);
+}
+
+gint wbfmm_shift_operators_coaxial_SR_init_f(wbfmm_shift_operators_t *w, gfloat D, guint level,
+                                             guint L, gfloat k, gfloat *work)
+{
+  (void)work;
+  return wbfmm_shift_operators_coaxial_SR_init(w, (double)D, level, L, (double)k, NULL);
+}
+
+gint wbfmm_shift_operators_coaxial_SS_init_f(wbfmm_shift_operators_t *w, gfloat D, guint level,
+                                             guint L, gfloat k, gfloat *work)
+{
+  (void)work;
+  return wbfmm_shift_operators_coaxial_SS_init(w, (double)D, level, L, (double)k, NULL);
+}
+
+gint wbfmm_tree_leaf_expansions_f(wbfmm_tree_t *t, gfloat k, gfloat *src, gint sstr,
+                                  gfloat *dipoles, gint dstr, gboolean zero_expansions,
+                                  gfloat *work)
+{
+  double *q, *d;
+  (void)work;
+  require_f(t);
+  q = widen(src, sstr, 2, t->npoints);
+  d = widen(dipoles, dstr, 2, t->npoints);
+  wbfmm_tree_leaf_expansions(t, (double)k, q, 2, d, 2, zero_expansions, NULL);
+  HIPOK(wbfmm_hip_sync());
+  free(q);
+  free(d);
+  return 0;
+}
+
+gint wbfmm_upward_pass_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                         gfloat *work)
+{
+  (void)work;
+  require_f(t);
+  return wbfmm_upward_pass(t, op, level, NULL);
+}
+
+gint wbfmm_downward_pass_ref_f(wbfmm_tree_t *t, wbfmm_shift_operators_t *op, guint level,
+                               gfloat *work, gint nthreads)
+{
+  (void)work;
+  require_f(t);
+  return wbfmm_downward_pass_ref(t, op, level, NULL, nthreads);
+}
+
+gint wbfmm_tree_box_local_field_f(wbfmm_tree_t *t, guint level, guint b, gfloat k, gfloat *x,
+                                  gfloat *f, gint fstr, gfloat *src, gint sstr, gfloat *d,
+                                  gint dstr, gboolean eval_neighbours, guint field,
+                                  gfloat *work)
+{
+  double *q, *dd, xd[3] = {x[0], x[1], x[2]}, tmp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int nv = (field & WBFMM_FIELD_GRADIENT) ? 6 : 2;
+  int i;
+  (void)work;
+  require_f(t);
+  q = eval_neighbours ? widen(src, sstr, 2, t->npoints) : NULL;
+  dd = eval_neighbours ? widen(d, dstr, 2, t->npoints) : NULL;
+  wbfmm_tree_box_local_field(t, level, b, (double)k, xd, tmp, fstr, q, 2, dd, 2,
+                             eval_neighbours, field, NULL);
+  HIPOK(wbfmm_hip_sync());
+  free(q);
+  free(dd);
+  for (i = 0; i < nv; i++) f[i] += (gfloat)tmp[i];
+  return 0;
+}
+
+gint wbfmm_target_list_local_coefficients_f(wbfmm_target_list_t *l, gfloat k, gfloat *work)
+{
+  (void)work;
+  return wbfmm_target_list_local_coefficients(l, (double)k, NULL);
+}
+
+/* the direct sums at one point (src/direct.c:35-294): 2 (potential) or 6
+   (gradient) floats */
+static gint hel_total_f(gfloat k, gfloat *xs, gint xstride, gfloat *src, gint sstride,
+                        gfloat *normals, gint nstr, gfloat *dipoles, gint dstr, gint nq,
+                        gint nsrc, gfloat *xf, gfloat *field, int grad)
+{
+  double *xd, *nd, *q, *dd, xfd[3] = {xf[0], xf[1], xf[2]}, tmp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int i;
+  if (nq != 1) WBFMM_FATAL("only nq == 1 is supported");
+  xd = widen(xs, xstride, 3, nsrc);
+  nd = widen(normals, nstr, 3, nsrc);
+  q = widen(src, sstride, 2, nsrc);
+  dd = widen(dipoles, dstr, 2, nsrc);
+  hel_total((double)k, xd, 3, q, 2, nd, 3, dd, 2, 1, nsrc, xfd, tmp, grad);
+  HIPOK(wbfmm_hip_sync());
+  free(xd);
+  free(nd);
+  free(q);
+  free(dd);
+  for (i = 0; i < (grad ? 6 : 2); i++) field[i] += (gfloat)tmp[i];
+  return 0;
+}
+
+gint wbfmm_total_field_f(gfloat k, gfloat *xs, gint xstride, gfloat *src, gint sstride,
+                         gfloat *normals, gint nstr, gfloat *dipoles, gint dstr, gint nq,
+                         gint nsrc, gfloat *xf, gfloat *field, gint fstr)
+{
+  (void)fstr;
+  return hel_total_f(k, xs, xstride, src, sstride, normals, nstr, dipoles, dstr, nq, nsrc, xf,
+                     field, 0);
+}
+
+gint wbfmm_total_field_grad_f(gfloat k, gfloat *xs, gint xstride, gfloat *src, gint sstride,
+                              gfloat *normals, gint nstr, gfloat *dipoles, gint dstr, gint nq,
+                              gint nsrc, gfloat *xf, gfloat *field, gint fstr)
+{
+  (void)fstr;
+  return hel_total_f(k, xs, xstride, src, sstride, normals, nstr, dipoles, dstr, nq, nsrc, xf,
+                     field, 1);
 }
