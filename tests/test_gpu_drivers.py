@@ -45,7 +45,7 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("opts", ["", "-l", "-p", "-g", "-g -l"])
+@pytest.mark.parametrize("opts", ["", "-l", "-p", "-g", "-g -l", "-B"])
 def test_laplace_driver_matches_oracle(oracle, tmp_path, opts):
     rng = np.random.default_rng(601)
     xs = rng.uniform(-1, 1, (2500, 3))
@@ -153,13 +153,45 @@ def test_helmholtz_driver_truncation_orders_and_dipoles(wb, oracle, tmp_path):
         assert rel(out[:, 3 + c], fo[:, c]) < PARITY, c
 
 
+@pytest.mark.parametrize("opts", ["", "-c", "-c -g"])
+def test_laplace_driver_vector_sources(oracle, tmp_path, opts):
+    """three-component sources (nq = 3): the potentials, the curl (-c) and
+    the curl with its gradient (-c -g), widths as the reference driver's
+    (tools/wbfmm-laplace-tree-calc.c.in:306-320)"""
+    rng = np.random.default_rng(607)
+    xs = rng.uniform(-1, 1, (2000, 3))
+    w = rng.normal(size=(2000, 3))
+    xf = rng.uniform(-1, 1, (400, 3))
+    write_points(tmp_path / "src.dat", "M", 3, np.column_stack([xs, w]))
+    write_points(tmp_path / "fld.dat", "F", 0, xf)
+    out, _ = run_driver("wbfmm-laplace-tree-calc",
+                        ["-s", str(tmp_path / "src.dat"), "-f", str(tmp_path / "fld.dat"),
+                         "-d", "3", "-S", "9", "-R", "9", "-b", *opts.split()])
+    x0, D = fit_box(xs, xf)
+    rows = [oracle.fmm(1, xs, xf, x0, D, 3, [9] * 17, [9] * 17, q=np.ascontiguousarray(w[:, c]),
+                       field=7)[0] for c in range(3)]
+    if opts == "":
+        want = np.column_stack([r[:, 0] for r in rows])
+    else:
+        want = oracle.curl_from_components(rows)[:, :3 if opts == "-c" else 12]
+    assert out.shape[1] == 3 + want.shape[1]
+    for c in range(want.shape[1]):
+        assert rel(out[:, 3 + c], want[:, c]) < PARITY, (opts, c)
+
+
 def test_drivers_refuse_unsupported_options(tmp_path):
+    """the curl needs three source components; target lists take one"""
     write_points(tmp_path / "src.dat", "M", 1, np.array([[0.5, 0.5, 0.5, 1.0]]))
     write_points(tmp_path / "fld.dat", "F", 0, np.array([[0.2, 0.2, 0.2]]))
     r = subprocess.run([os.path.join(BIN, "wbfmm-laplace-tree-calc"), "-s",
                         str(tmp_path / "src.dat"), "-f", str(tmp_path / "fld.dat"), "-c"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "curl" in r.stderr
+    write_points(tmp_path / "src3.dat", "M", 3, np.array([[0.5, 0.5, 0.5, 1.0, 2.0, 3.0]]))
+    r = subprocess.run([os.path.join(BIN, "wbfmm-laplace-tree-calc"), "-s",
+                        str(tmp_path / "src3.dat"), "-f", str(tmp_path / "fld.dat"), "-l"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "target list" in r.stderr
 
 
 def morton_codes(oracle, pts, x0, D):

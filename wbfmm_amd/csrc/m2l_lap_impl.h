@@ -1327,6 +1327,302 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
   }
 }
 
+// ---------------------------------------------------------------------
+// The packed kernel with a double-buffered exchange buffer.  The stages
+// alternate between R0 and R1 (R0 -> R1 -> R0 -> R1), so an offset's tile
+// needs one barrier per stage and one after the next deposit: 4 instead of
+// 8.  The room for the second buffer comes from the local expansions,
+// which move from LDS to registers: lane l of wavefront w owns target
+// j = l % 32 and the units u = g + 2W i (g = 2w + l / 32), finds its
+// target's column in the tile through pos[k][j], and accumulates there.
+// The 32 lanes of a half-wavefront read 32 targets' columns of one unit,
+// at most 16 distinct columns in distinct banks.  Each target still sums
+// its translations in offset order; the results differ from the chunked
+// and LDS-accumulator kernels by rounding only.
+// ---------------------------------------------------------------------
+template <int N>
+struct PackedDbFits {
+  static constexpr int Sz = (M2LShape<N>::S + 1) * 16 * 8;
+  static constexpr bool value = N <= 12 && 2 * Sz + 2 * 216 * kPackT <= 72 * 1024;
+};
+
+template <int N, int W, int WI>
+__device__ __forceinline__ void m2l_packed_db_loop(
+    double *R0, double *R1, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3,
+    const int4 *uinf, double2 (*cs)[N + 1], const unsigned *anym,
+    const uint8_t (*lists)[kPackT], const uint8_t (*pos)[kPackT], const uint8_t *ncols,
+    const int4 *tco, const uint32_t *tbox, int lane, int px, int py, int pz,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero,
+    const double *__restrict__ mps, double *__restrict__ mpr) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
+  constexpr int G = 2 * W;                    // unit groups of the accumulators
+  constexpr int NAQ = (NU + G - 1) / G;       // accumulator units per lane
+  constexpr bool PF = StageOps<N, W, WI>::fits;
+  const int col = lane & 15, ubase = (lane >> 4) + 4 * WI;
+  const int tj = lane & 31, ug = 2 * WI + (lane >> 5);
+
+  auto offset_of = [&](int k) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+  };
+  auto next_tile = [&](int k, int t0, int &kn, int &tn) {
+    if (k >= 0 && k < 216 && t0 + 16 < ncols[k]) {
+      kn = k;
+      tn = t0 + 16;
+      return;
+    }
+    for (k++; k < 216; k++)
+      if ((anym[k >> 5] >> (k & 31)) & 1u) break;
+    kn = k;
+    tn = 0;
+  };
+  auto source_of = [&](int k, int t0, int p) -> const double * {
+    if (lane <= N) {
+      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      cs[p][lane] = make_double2(t[0], t[1]);
+    }
+    const int sl = t0 + col;
+    const int j = sl < ncols[k] ? lists[k][sl] : -1;
+    if (j < 0) return zero;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int4 tc = tco[j];
+    const uint64_t sb =
+        spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
+    return mps + sb * (uint64_t)C;
+  };
+  double pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const double *src, const int4 *tu, int ub) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const double *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  auto deposit = [&](double *R, const int4 *tu, int ub, int cl, int p) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      if (ub + 4 * W * i >= NU) continue;
+      const int4 inf = tu[ub + 4 * W * i];
+      const double2 e = cs[p][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
+
+  double acc_re[NAQ], acc_im[NAQ];
+#pragma unroll
+  for (int i = 0; i < NAQ; i++) acc_re[i] = acc_im[i] = 0.0;
+
+  int k, t0, p = 0;
+  next_tile(-1, 0, k, t0);
+  k = __builtin_amdgcn_readfirstlane(k);
+  int th = 0, ri = 0;
+  if (k < 216) {
+    th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
+    ri = __builtin_amdgcn_readfirstlane(P->rix[offset_of(k)]);
+    load_sources(source_of(k, t0, 0), uinf, ubase);
+    deposit(R0, uinf, ubase, col, 0);
+  }
+  double av[StageOps<N, W, WI>::n];
+  if constexpr (PF) ops_load<double, N, W, WI>(av, A1 + (size_t)th * KT * 64, lane);
+  __syncthreads();
+  while (k < 216) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+    int kn, tn;
+    next_tile(k, t0, kn, tn);
+    kn = __builtin_amdgcn_readfirstlane(kn);
+    tn = __builtin_amdgcn_readfirstlane(tn);
+    int th_n = 0, ri_n = 0;
+    if (kn < 216) {
+      th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
+      ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
+    }
+    // R0 -> R1 -> R0 -> R1, one barrier per stage
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A2 + (size_t)ri * KT * 64, R0, R1, t1, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A1 + (size_t)th * KT * 64, R0, R1, t1, ln, gp, cl);
+    __syncthreads();
+    if (kn < 216) load_sources(source_of(kn, tn, p ^ 1), tu, ub);
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A3 + (size_t)th * KT * 64, R1, R0, t2, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A2 + (size_t)ri * KT * 64, R1, R0, t2, ln, gp, cl);
+    __syncthreads();
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A1 + (size_t)th_n * KT * 64, R0, R1, t3, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A3 + (size_t)th * KT * 64, R0, R1, t3, ln, gp, cl);
+    __syncthreads();
+    // Zout from R1 into the owning lanes' registers; the next deposit
+    // goes into R0, which the third stage has finished reading
+    {
+      const int c = pos[k][tj];
+      const int cc = c - t0;
+      if (c != 0xff && cc >= 0 && cc < 16) {
+#pragma unroll
+        for (int i = 0; i < NAQ; i++) {
+          const int u = ug + G * i;
+          if (u >= NU) continue;
+          const int m = tu[u].y;
+          const double2 e = cs[p][m];
+          const double tr = R1[u * 16 + cc], tq = R1[(NU + u) * 16 + cc];
+          acc_re[i] += e.x * tr + e.y * tq;
+          acc_im[i] += e.x * tq - e.y * tr;
+        }
+      }
+    }
+    if (kn < 216) deposit(R0, tu, ub, cl, p ^ 1);
+    __syncthreads();
+    p ^= 1;
+    k = kn;
+    t0 = tn;
+    th = th_n;
+    ri = ri_n;
+  }
+  // each lane adds its target's units (m = 0 has no imaginary part)
+  if (tco[tj].w) {
+    double *out = mpr + (size_t)tbox[tj] * C;
+#pragma unroll
+    for (int i = 0; i < NAQ; i++) {
+      const int u = ug + G * i;
+      if (u >= NU) continue;
+      const int4 inf = uinf[u];
+      out[inf.x] += acc_re[i];
+      if (inf.y) out[inf.x + 1] += acc_im[i];
+    }
+  }
+}
+
+// two blocks per CU: at eight wavefronts per block, four per SIMD (at most
+// 128 registers each)
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 8 ? 4 : 2)))
+void k_lap_m2l_packed_db(
+    const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ,
+    int level, const double *__restrict__ mps, double *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  __shared__ double R0[(S + 1) * 16], R1[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[2][N + 1];
+  __shared__ unsigned vm[kPackT][7];
+  __shared__ unsigned anym[7];
+  __shared__ uint8_t lists[216][kPackT];
+  __shared__ uint8_t pos[216][kPackT];
+  __shared__ uint8_t ncols[216];
+  __shared__ int4 tco[kPackT];
+  __shared__ uint32_t tbox[kPackT];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R0[i] = R1[i] = 0.0;
+  for (int i = tid; i < kPackT * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
+  for (int i = tid; i < 216 * kPackT; i += 64 * W) pos[i / kPackT][i % kPackT] = 0xff;
+
+  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x, q8 = nch / 8;
+  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+  int c = 0;
+  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  if (tid < kPackT) {
+    const unsigned kk = (chunk - ca.chunk_prefix[c]) * kPackT + tid;
+    const bool tv = kk < ca.count[c];
+    const uint32_t tb = tv ? cls[ca.start[c] + kk] : 0u;
+    tbox[tid] = tb;
+    tco[tid] = make_int4(compact3(tb), compact3(tb >> 1), compact3(tb >> 2), tv ? 1 : 0);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kPackT * 216; idx += 64 * W) {
+    const int j = idx % kPackT, k = idx / kPackT;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int4 tc = tco[j];
+    const int si = tc.x + dx, sj = tc.y + dy, sk = tc.z + dz;
+    if (!tc.w || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox) continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[j][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  if (tid < 7) {
+    unsigned a = 0;
+    for (int j = 0; j < kPackT; j++) a |= vm[j][tid];
+    anym[tid] = a;
+  }
+  // per offset, the targets with an occupied source (target order) and
+  // each target's column position
+  for (int k = tid; k < 216; k += 64 * W) {
+    int n = 0;
+    for (int j = 0; j < kPackT; j++)
+      if ((vm[j][k >> 5] >> (k & 31)) & 1u) {
+        lists[k][n] = (uint8_t)j;
+        pos[k][j] = (uint8_t)n;
+        n++;
+      }
+    ncols[k] = (uint8_t)n;
+  }
+  __syncthreads();
+
+#define PACKED_DB(WI)                                                                          \
+  m2l_packed_db_loop<N, W, WI>(R0, R1, tmap1, tmap2, tmap3, uinf, cs, anym, lists, pos, ncols, \
+                               tco, tbox, lane, px, py, pz, P, A1, A3, A2, CS, zero, mps, mpr)
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: PACKED_DB(0); break;
+      case 1: PACKED_DB(1); break;
+      case 2: PACKED_DB(2); break;
+      default: PACKED_DB(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: PACKED_DB(0); break;
+      case 1: PACKED_DB(1); break;
+      case 2: PACKED_DB(2); break;
+      case 3: PACKED_DB(3); break;
+      case 4: PACKED_DB(4); break;
+      case 5: PACKED_DB(5); break;
+      case 6: PACKED_DB(6); break;
+      default: PACKED_DB(7); break;
+    }
+  }
+#undef PACKED_DB
+}
+
 // the packed kernel for levels where fewer than 60 % of the boxes hold
 // targets and there are at least 1024 chunks; WBFMM_M2L_PACKED=0 or 1
 // forces the choice at every level and size (read at each call, for
@@ -1369,6 +1665,24 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
       }
       cp.chunk_prefix[8] = nch;
       const char *pw = getenv("WBFMM_PACKED_WAVES");
+      const char *pd = getenv("WBFMM_PACKED_DB");
+      if constexpr (PackedDbFits<N>::value) {
+        if (!(pd && atoi(pd) == 0)) {
+          if (!(pw && atoi(pw) == 4))
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 8>), dim3(nch), dim3(512), 0, cls,
+                         cp, occ, level, (const double *)mps, (double *)mpr,
+                         (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
+                         (const double *)A3, (const double *)A2, (const double *)CS,
+                         (const double *)zero);
+          else
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 4>), dim3(nch), dim3(256), 0, cls,
+                         cp, occ, level, (const double *)mps, (double *)mpr,
+                         (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
+                         (const double *)A3, (const double *)A2, (const double *)CS,
+                         (const double *)zero);
+          return 0;
+        }
+      }
       if (!(pw && atoi(pw) == 4))
         WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 8>), dim3(nch), dim3(512), 0, cls, cp, occ,
                      level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
