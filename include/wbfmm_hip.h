@@ -493,6 +493,16 @@ int wbfmm_hip_hel_shift_mfma_f32(int mode, const uint32_t *parents, unsigned npa
                                  const uint32_t *cflag, int N, const float *in, float *out,
                                  const void *plan_dev, const float *A, const float *A2,
                                  const float *CS, const wbfmm_m2l_plan_t *plan_host);
+/* mixed-precision L2P + near field with one thread per target (the form
+   of wbfmm_hip_hel_field): FP32 local expansions holding L_n^m s_n
+   (scale: the level's s_n, FP64), FP64 regular functions and L2P sums,
+   FP32 near-field kernel with compensated sums (src/local.c:32-310) */
+int wbfmm_hip_hel_field_f32(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip,
+                            unsigned level, const double *x0, double D, int N, double k,
+                            const float *mpr, const double *scale, int field, double *f,
+                            int fstr);
 /* L2P + near field with one wavefront per target box (src/local.c:32-310):
    tboxes lists the nbox leaf boxes holding targets, whose sorted targets
    are [tstart[b], tend[b]) of xt (4 doubles each, caller index tid); the

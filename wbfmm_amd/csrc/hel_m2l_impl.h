@@ -139,9 +139,9 @@ __device__ __forceinline__ void hel_m2l_loop(
       cs[p][lane] = mk2<T>(t[0], t[1]);
     }
     const bool v = (vm[tg][k >> 5] >> (k & 31)) & 1u;
-    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
-                          : 0;
-    return v ? mps + sb * (uint64_t)C : zero;
+    const uint32_t sb = morton_add(tb, dilate_offset(dx, 0, kMortonX),
+                                   dilate_offset(dy, 1, kMortonY), dilate_offset(dz, 2, kMortonZ));
+    return v ? mps + (uint64_t)sb * C : zero;
   };
   T pre_re[NAW], pre_im[NAW];
   // x_{+m} on the real lane, x_{-m} on the imaginary one

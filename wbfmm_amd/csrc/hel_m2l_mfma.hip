@@ -48,6 +48,7 @@ static int hel_m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsi
                          const T *mps, T *mpr, const void *plan_dev, const T *A, const T *A2,
                          const T *CS, const wbfmm_m2l_plan_t *plan_host) {
   ClassArgs ca;
+  ca.qc = 0;
   unsigned nch = 0;
   for (int c = 0; c < 8; c++) {
     ca.start[c] = cstart[c];
@@ -57,7 +58,9 @@ static int hel_m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsi
   }
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
-  if (level < 2 || level > 20) return WBFMM_HIP_ERR_ARG;
+  // source boxes are found by 30-bit Morton arithmetic (levels up to 10;
+  // a dense level array above that exceeds any device memory)
+  if (level < 2 || level > (unsigned)kMortonAddMaxLevel) return WBFMM_HIP_ERR_ARG;
   if (N < 2 || N > 27) return WBFMM_HIP_ERR_ORDER;
   if (!plan_host || plan_host->N != N) return WBFMM_HIP_ERR_ARG;
   {

@@ -798,6 +798,159 @@ __global__ __launch_bounds__(64) void k_hel_field_box(
   }
 }
 
+// ---------------------------------------------------------------------
+// Mixed precision with one thread per target (the form of k_hel_field):
+// the local expansion holds L_n^m s_n in FP32 (sc: the level's scales
+// s_n, divided out in FP64 through 1/s_n in LDS), the regular functions
+// and the L2P sums are FP64, and the near-field kernel runs in FP32 with
+// compensated (Neumaier) sums, the offsets x - y, the 1e-6 cutoff and the
+// dipole factor n.(x - y) in FP64.  Consecutive threads take targets of
+// one box (they arrive sorted), so a wavefront's source loads coincide.
+// On leaves of a few to a few tens of targets this keeps every lane busy,
+// where one wavefront per target box leaves most of its lanes idle.
+// ---------------------------------------------------------------------
+template <int NMAX>
+__global__ __launch_bounds__(128) void k_hel_field_mixed(
+    const double *__restrict__ xt, const uint32_t *__restrict__ tid, unsigned n,
+    const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
+    const double *__restrict__ xyz, const double2 *__restrict__ q,
+    const double *__restrict__ nrm, const double2 *__restrict__ dip,
+    unsigned level, double x0, double y0, double z0, double D, int N, double k,
+    const float2 *__restrict__ mpr, const double *__restrict__ sc, int field,
+    double *__restrict__ f, int fstr) {
+  __shared__ double sinv[NMAX + 2];
+  for (int t = threadIdx.x; t <= N + 1; t += blockDim.x) sinv[t] = 1.0 / sc[t];
+  __syncthreads();
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xt[4 * i], y = xt[4 * i + 1], z = xt[4 * i + 2];
+  const uint64_t code = point_code(x, y, z, x0, y0, z0, D);
+  const uint64_t b = code >> (3 * (20 - level));
+  const double w = D / (double)(1u << level);
+  const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
+               zc = z0 + (compact3(b >> 2) + 0.5) * w;
+  double2 pot = make_double2(0.0, 0.0), gz = pot, dpl = pot, dmi = pot;
+  const float2 *Lc = mpr + b * (size_t)(N + 1) * (N + 1);
+  auto Lg = [&](int nn, int mm) -> double2 {
+    if (nn < 0 || nn > N || mm > nn || -mm > nn) return make_double2(0.0, 0.0);
+    const float2 v = Lc[hidx(nn, mm)];
+    return make_double2((double)v.x * sinv[nn], (double)v.y * sinv[nn]);
+  };
+  {
+    double r, C, S, cp, sp, jn[NMAX + 2];
+    sph(x - xc, y - yc, z - zc, r, C, S, cp, sp);
+    bessel_j(N + 1, k * r, jn);
+    const double phi = atan2(sp, cp);
+    double pmm = 1.0 / sqrt(4.0 * kPi);
+    for (int m = 0; m <= N + 1; m++) {
+      if (m > 0) pmm *= sqrt((2.0 * m + 1) / (2.0 * m)) * S;
+      double sn, cs;
+      sincos((double)m * phi, &sn, &cs);
+      double p = pmm, pprev = 0.0;
+      for (int nn = m; nn <= N + 1; nn++) {
+        const double a = jn[nn] * p;
+        for (int sg = 1; sg >= -1; sg -= 2) {
+          if (m == 0 && sg < 0) break;
+          const int mm = sg * m;
+          const double2 R = make_double2(a * cs, sg * a * sn);
+          if ((field & 1) && nn <= N) pot = cadd(pot, cmul(Lg(nn, mm), R));
+          if (field & 2) {
+            const double2 cz = cadd(cscale(Lg(nn + 1, mm), rec_a(nn, mm)),
+                                    cscale(Lg(nn - 1, mm), -rec_a(nn - 1, mm)));
+            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), rec_b(nn, -mm)),
+                                     cscale(Lg(nn + 1, mm - 1), -rec_b(nn + 1, mm - 1)));
+            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), rec_b(nn, mm)),
+                                     cscale(Lg(nn + 1, mm + 1), -rec_b(nn + 1, -mm - 1)));
+            gz = cadd(gz, cmul(cz, R));
+            dpl = cadd(dpl, cmul(cpz, R));
+            dmi = cadd(dmi, cmul(cmz, R));
+          }
+        }
+        const double pn = pbar_up(nn, m, C, p, pprev);
+        pprev = p;
+        p = pn;
+      }
+    }
+  }
+  double2 gx = cscale(cadd(dpl, dmi), 0.5 * k);
+  const double2 dd = cscale(cadd(dpl, cscale(dmi, -1.0)), 0.5 * k);
+  double2 gy = make_double2(dd.y, -dd.x);
+  gz = cscale(gz, k);
+  // near field in FP32, compensated sums
+  CSum<float> apot, agx, agy, agz;
+  apot.zero();
+  agx.zero();
+  agy.zero();
+  agz.zero();
+  const float kk = (float)k, c4 = (float)(1.0 / (4.0 * kPi * k));
+  const int bi = compact3(b), bj = compact3(b >> 1), bk = compact3(b >> 2), nb = 1 << level;
+  for (int a = -1; a <= 1; a++)
+    for (int bb = -1; bb <= 1; bb++)
+      for (int c = -1; c <= 1; c++) {
+        const int si = bi + a, sj = bj + bb, sk = bk + c;
+        if (si < 0 || si >= nb || sj < 0 || sj >= nb || sk < 0 || sk >= nb) continue;
+        const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+        for (uint32_t s = start[sb]; s < end[sb]; s++) {
+          const double rxd = x - xyz[4 * s], ryd = y - xyz[4 * s + 1], rzd = z - xyz[4 * s + 2];
+          const double r2 = rxd * rxd + ryd * ryd + rzd * rzd;
+          if (r2 <= 1e-12) continue;
+          const float rx = (float)rxd, ry = (float)ryd, rz = (float)rzd;
+          const float r = sqrtf((float)r2), ir = 1.0f / r;
+          float sn, cs;
+          sincosf(kk * r, &sn, &cs);
+          // G = (sin - i cos) / (4 pi k r), G (ik - 1/r), G ((ik - 1/r)^2 + 1/r^2)
+          const float g0 = c4 * ir;
+          const float Gr = sn * g0, Gi = -cs * g0;
+          const float dGr = -Gr * ir - Gi * kk, dGi = Gr * kk - Gi * ir;
+          const float ux = rx * ir, uy = ry * ir, uz = rz * ir;
+          if (q) {
+            const float qr = (float)q[s].x, qi = (float)q[s].y;
+            apot.add(qr * Gr - qi * Gi, qr * Gi + qi * Gr);
+            const float qdr = qr * dGr - qi * dGi, qdi = qr * dGi + qi * dGr;
+            agx.add(qdr * ux, qdi * ux);
+            agy.add(qdr * uy, qdi * uy);
+            agz.add(qdr * uz, qdi * uz);
+          }
+          if (dip) {
+            const float dr = (float)dip[s].x, di = (float)dip[s].y;
+            const double nxd = nrm[4 * s], nyd = nrm[4 * s + 1], nzd = nrm[4 * s + 2];
+            const float nx = (float)nxd, ny = (float)nyd, nz = (float)nzd;
+            // n.(x-y) in FP64 (close points on a surface: n nearly orthogonal)
+            const float nr = (float)(nxd * rxd + nyd * ryd + nzd * rzd) * ir;
+            const float ddr = dr * dGr - di * dGi, ddi = dr * dGi + di * dGr;
+            apot.add(ddr * nr, ddi * nr);
+            // (ik - 1/r)^2 + 1/r^2 = 2/r^2 - k^2 - 2ik/r
+            const float e2r = 2.0f * ir * ir - kk * kk, e2i = -2.0f * kk * ir;
+            const float g2r = Gr * e2r - Gi * e2i, g2i = Gr * e2i + Gi * e2r;
+            const float t1r = (dr * g2r - di * g2i) * nr, t1i = (dr * g2i + di * g2r) * nr;
+            const float t2r = ddr * ir, t2i = ddi * ir;
+            agx.add(t1r * ux + t2r * (nx - nr * ux), t1i * ux + t2i * (nx - nr * ux));
+            agy.add(t1r * uy + t2r * (ny - nr * uy), t1i * uy + t2i * (ny - nr * uy));
+            agz.add(t1r * uz + t2r * (nz - nr * uz), t1i * uz + t2i * (nz - nr * uz));
+          }
+        }
+      }
+  pot = cadd(pot, apot.value());
+  gx = cadd(gx, agx.value());
+  gy = cadd(gy, agy.value());
+  gz = cadd(gz, agz.value());
+  double *o = f + (size_t)tid[i] * fstr;
+  int at = 0;
+  if (field & 1) {
+    o[0] += pot.x;
+    o[1] += pot.y;
+    at = 2;
+  }
+  if (field & 2) {
+    o[at + 0] += gx.x;
+    o[at + 1] += gx.y;
+    o[at + 2] += gy.x;
+    o[at + 3] += gy.y;
+    o[at + 4] += gz.x;
+    o[at + 5] += gz.y;
+  }
+}
+
 // out (FP32, box-major, complex) = in (FP64) with degree n multiplied by
 // sc[n], for the listed boxes; and the reverse (FP32 -> FP64, degree n
 // multiplied by sc[n]: pass 1/s to unscale)
@@ -1043,6 +1196,28 @@ int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
   WBFMM_LAUNCH("hel_l2p_p2p", k_hel_field<NM>, dim3((n + 127) / 128), dim3(128), 0, xt, \
                tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,  \
                level, x0[0], x0[1], x0[2], D, N, k, (const double2 *)mpr, field, f, fstr)
+  if (N <= 16) HF(16);
+  else if (N <= 24) HF(24);
+  else if (N <= 32) HF(32);
+  else if (N <= WBFMM_HIP_MAX_ORDER) HF(WBFMM_HIP_MAX_ORDER);
+  else return WBFMM_HIP_ERR_ORDER;
+#undef HF
+  return 0;
+}
+
+int wbfmm_hip_hel_field_f32(const double *xt, const uint32_t *tid, unsigned n,
+                            const uint32_t *start, const uint32_t *end, const double *xyz,
+                            const double *q, const double *nrm, const double *dip,
+                            unsigned level, const double *x0, double D, int N, double k,
+                            const float *mpr, const double *scale, int field, double *f,
+                            int fstr) {
+  if (n == 0) return 0;
+  if (N < 0 || !scale) return N < 0 ? WBFMM_HIP_ERR_ORDER : WBFMM_HIP_ERR_ARG;
+#define HF(NM)                                                                              \
+  WBFMM_LAUNCH("hel_l2p_p2p_f32", k_hel_field_mixed<NM>, dim3((n + 127) / 128), dim3(128), 0, \
+               xt, tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,   \
+               level, x0[0], x0[1], x0[2], D, N, k, (const float2 *)mpr, scale, field, f,    \
+               fstr)
   if (N <= 16) HF(16);
   else if (N <= 24) HF(24);
   else if (N <= 32) HF(32);

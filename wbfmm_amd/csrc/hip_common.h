@@ -62,6 +62,36 @@ __device__ __forceinline__ uint32_t compact3(uint64_t x) {
   return (uint32_t)x;
 }
 
+// Box indices at levels up to 10 fit in 30 bits.  The index of box
+// (i + di, j + dj, k + dk) follows from that of box (i, j, k) by one
+// dilated addition per axis: the offset is spread onto its axis's bits
+// (dilate_offset; a negative offset as its dilated two's complement) and
+// added to the index with the other axes' bits set, so that carries pass
+// over them.  Exact wherever the result lies inside the level.
+constexpr uint32_t kMortonX = 0x49249249u, kMortonY = 0x92492492u, kMortonZ = 0x24924924u;
+constexpr int kMortonAddMaxLevel = 10;
+
+__device__ __forceinline__ uint32_t dilate10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// offset d (|d| < 1024) on the axis of bit mask m, shift s (0, 1, 2)
+__device__ __forceinline__ uint32_t dilate_offset(int d, int s, uint32_t m) {
+  const uint32_t a = dilate10((uint32_t)(d < 0 ? -d : d)) << s;
+  return d < 0 ? (0u - a) & m : a;
+}
+
+__device__ __forceinline__ uint32_t morton_add(uint32_t t, uint32_t ox, uint32_t oy,
+                                               uint32_t oz) {
+  return (((t | ~kMortonX) + ox) & kMortonX) | (((t | ~kMortonY) + oy) & kMortonY) |
+         (((t | ~kMortonZ) + oz) & kMortonZ);
+}
+
 // Morton code of a point on the 2^20 grid, same arithmetic as the
 // reference's wbfmm_point_index_3d (src/boxes.c:324-339)
 __device__ __forceinline__ uint64_t point_code(double x, double y, double z,

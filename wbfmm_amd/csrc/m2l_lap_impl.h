@@ -16,6 +16,8 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include "hip_common.h"
@@ -78,11 +80,11 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
   // block `split` taking the cube indices k = split (mod nsplit), and write
   // partial sums that k_m2l_reduce adds in a fixed order
   const int split = (int)(blockIdx.x % (unsigned)nsplit);
-  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
-  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
-  int c = 0;
-  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
-  const unsigned k16 = (chunk - ca.chunk_prefix[c]) * 16 + col;
+  int c;
+  unsigned cj;
+  // a block with no chunk leaves before any barrier (uniform over the block)
+  if (!block_chunk(blockIdx.x / (unsigned)nsplit, ca, c, cj)) return;
+  const unsigned k16 = cj * 16 + col;
   const bool tvalid = k16 < ca.count[c];
   const uint32_t tb = tvalid ? cls[ca.start[c] + k16] : 0u;
   const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
@@ -130,9 +132,9 @@ __global__ __launch_bounds__(64) void k_lap_m2l_mfma(
       cs[p][lane] = mk2<T>(t[0], t[1]);
     }
     const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
-    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
-                          : 0;
-    return v ? mps + sb * (uint64_t)C : zero;
+    const uint32_t sb = morton_add(tb, dilate_offset(dx, 0, kMortonX),
+                                   dilate_offset(dy, 1, kMortonY), dilate_offset(dz, 2, kMortonZ));
+    return v ? mps + (uint64_t)sb * C : zero;
   };
   // Units u = grp + 4 i of column col: the lane loads, rotates (Zin) and
   // accumulates the same units.  The last lanes repeat unit NU-1 (same
@@ -328,9 +330,9 @@ __device__ __forceinline__ void m2l_w2_loop(
       cs[p][lane] = mk2<T>(t[0], t[1]);
     }
     const bool v = (vm[col][k >> 5] >> (k & 31)) & 1u;
-    const uint64_t sb = v ? (spread3(ti + dx) | (spread3(tj + dy) << 1) | (spread3(tk + dz) << 2))
-                          : 0;
-    return v ? mps + sb * (uint64_t)C : zero;
+    const uint32_t sb = morton_add(tb, dilate_offset(dx, 0, kMortonX),
+                                   dilate_offset(dy, 1, kMortonY), dilate_offset(dz, 2, kMortonZ));
+    return v ? mps + (uint64_t)sb * C : zero;
   };
   T pre_re[NAW], pre_im[NAW];
   auto load_sources = [&](const T *src, const int4 *tu, int ub, int i0, int i1) {
@@ -402,6 +404,10 @@ __device__ __forceinline__ void m2l_w2_loop(
     if (kn < 216) {
       th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
       ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
+      // the next offset's sources are requested before this offset's
+      // stages: the registers were emptied by the last deposit, and
+      // cs[p ^ 1] was last read before the barrier that ended the last offset
+      if (PREFETCH) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
     }
 
     // double-buffered: R -> R1 -> R -> R1, one barrier per stage; the
@@ -410,7 +416,6 @@ __device__ __forceinline__ void m2l_w2_loop(
     if constexpr (DB && PF) {
       stage_wop<T, N, W, WI>(av, A2 + (size_t)ri * SH::KT * 64, R, R1, t1, ln, gp, cl);
       __syncthreads();
-      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
       stage_wop<T, N, W, WI>(av, A3 + (size_t)th * SH::KT * 64, R1, R, t2, ln, gp, cl);
       __syncthreads();
       stage_wop<T, N, W, WI>(av, A1 + (size_t)th_n * SH::KT * 64, R, R1, t3, ln, gp, cl);
@@ -422,7 +427,6 @@ __device__ __forceinline__ void m2l_w2_loop(
       __syncthreads();
       write_out(t1, y, gp, cl);
       __syncthreads();
-      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
       stage_wp<T, N, W, WI>(av, A3 + (size_t)th * SH::KT * 64, R, ln, gp, cl, y);
       __syncthreads();
       write_out(t2, y, gp, cl);
@@ -434,7 +438,6 @@ __device__ __forceinline__ void m2l_w2_loop(
     } else if constexpr (DB) {
       stage_wo<T, N, W, WI>(A1 + (size_t)th * SH::KT * 64, R, R1, t1, ln, gp, cl);
       __syncthreads();
-      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
       stage_wo<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R1, R, t2, ln, gp, cl);
       __syncthreads();
       stage_wo<T, N, W, WI>(A3 + (size_t)th * SH::KT * 64, R, R1, t3, ln, gp, cl);
@@ -446,7 +449,6 @@ __device__ __forceinline__ void m2l_w2_loop(
       __syncthreads();
       write_out(t1, y, gp, cl);
       __syncthreads();
-      if (PREFETCH && kn < 216) load_sources(source_of(kn, p ^ 1), tu, ub, 0, NAW);
 
       stage_w<T, N, W, WI>(A2 + (size_t)ri * SH::KT * 64, R, ln, gp, cl, y);
       __syncthreads();
@@ -560,11 +562,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
   for (int i = tid; i < 16 * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
 
   const int split = (int)(blockIdx.x % (unsigned)nsplit);
-  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x / (unsigned)nsplit, q8 = nch / 8;
-  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
-  int c = 0;
-  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
-  const unsigned k16 = (chunk - ca.chunk_prefix[c]) * 16 + col;
+  int c;
+  unsigned cj;
+  // a block with no chunk leaves before any barrier (uniform over the block)
+  if (!block_chunk(blockIdx.x / (unsigned)nsplit, ca, c, cj)) return;
+  const unsigned k16 = cj * 16 + col;
   const bool tvalid = k16 < ca.count[c];
   const uint32_t tb = tvalid ? cls[ca.start[c] + k16] : 0u;
   const int ti = compact3(tb), tj = compact3(tb >> 1), tk = compact3(tb >> 2);
@@ -1055,7 +1057,7 @@ template <int N, int W, int WI>
 __device__ __forceinline__ void m2l_packed_loop(
     double *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
     double2 (*cs)[N + 1], const unsigned *anym, const uint8_t (*lists)[kPackT],
-    const uint8_t *ncols, const int4 *tco, double *acc, int lane, int px, int py, int pz,
+    const uint8_t *ncols, const uint32_t *tbox, double *acc, int lane, int px, int py, int pz,
     const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
     const double *__restrict__ A3, const double *__restrict__ A2,
     const double *__restrict__ CS, const double *__restrict__ zero,
@@ -1093,10 +1095,9 @@ __device__ __forceinline__ void m2l_packed_loop(
     j = sl < ncols[k] ? lists[k][sl] : -1;
     if (j < 0) return zero;
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
-    const int4 tc = tco[j];
-    const uint64_t sb =
-        spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
-    return mps + sb * (uint64_t)C;
+    const uint32_t sb = morton_add(tbox[j], dilate_offset(dx, 0, kMortonX),
+                                   dilate_offset(dy, 1, kMortonY), dilate_offset(dz, 2, kMortonZ));
+    return mps + (uint64_t)sb * C;
   };
   double pre_re[NAW], pre_im[NAW];
   auto load_sources = [&](const double *src, const int4 *tu, int ub) {
@@ -1297,7 +1298,7 @@ __global__ __launch_bounds__(64 * W) void k_lap_m2l_packed(
   __syncthreads();
 
 #define PACKED(WI)                                                                          \
-  m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, anym, lists, ncols, tco, acc, \
+  m2l_packed_loop<N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, anym, lists, ncols, tbox, acc, \
                             lane, px, py, pz, P, A1, A3, A2, CS, zero, mps)
   static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
   if constexpr (W == 4) {
@@ -1351,8 +1352,8 @@ __device__ __forceinline__ void m2l_packed_db_loop(
     double *R0, double *R1, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3,
     const int4 *uinf, double2 (*cs)[N + 1], const unsigned *anym,
     const uint8_t (*lists)[kPackT], const uint8_t (*pos)[kPackT], const uint8_t *ncols,
-    const int4 *tco, const uint32_t *tbox, int lane, int px, int py, int pz,
-    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const int4 *tco, const uint32_t *tbox, const int4 *ktab, const int2 *kth, int lane,
+    const double *__restrict__ A1,
     const double *__restrict__ A3, const double *__restrict__ A2,
     const double *__restrict__ CS, const double *__restrict__ zero,
     const double *__restrict__ mps, double *__restrict__ mpr) {
@@ -1365,10 +1366,6 @@ __device__ __forceinline__ void m2l_packed_db_loop(
   const int col = lane & 15, ubase = (lane >> 4) + 4 * WI;
   const int tj = lane & 31, ug = 2 * WI + (lane >> 5);
 
-  auto offset_of = [&](int k) {
-    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
-    return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
-  };
   auto next_tile = [&](int k, int t0, int &kn, int &tn) {
     if (k >= 0 && k < 216 && t0 + 16 < ncols[k]) {
       kn = k;
@@ -1381,18 +1378,16 @@ __device__ __forceinline__ void m2l_packed_db_loop(
     tn = 0;
   };
   auto source_of = [&](int k, int t0, int p) -> const double * {
+    const int4 kt = ktab[k];
     if (lane <= N) {
-      const double *t = CS + ((size_t)offset_of(k) * (N + 1) + lane) * 2;
+      const double *t = CS + ((size_t)kt.w * (N + 1) + lane) * 2;
       cs[p][lane] = make_double2(t[0], t[1]);
     }
     const int sl = t0 + col;
     const int j = sl < ncols[k] ? lists[k][sl] : -1;
     if (j < 0) return zero;
-    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
-    const int4 tc = tco[j];
-    const uint64_t sb =
-        spread3(tc.x + dx) | (spread3(tc.y + dy) << 1) | (spread3(tc.z + dz) << 2);
-    return mps + sb * (uint64_t)C;
+    const uint32_t sb = morton_add(tbox[j], (uint32_t)kt.x, (uint32_t)kt.y, (uint32_t)kt.z);
+    return mps + (uint64_t)sb * C;
   };
   double pre_re[NAW], pre_im[NAW];
   auto load_sources = [&](const double *src, const int4 *tu, int ub) {
@@ -1424,8 +1419,9 @@ __device__ __forceinline__ void m2l_packed_db_loop(
   k = __builtin_amdgcn_readfirstlane(k);
   int th = 0, ri = 0;
   if (k < 216) {
-    th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
-    ri = __builtin_amdgcn_readfirstlane(P->rix[offset_of(k)]);
+    const int2 tr = kth[k];
+    th = __builtin_amdgcn_readfirstlane(tr.x);
+    ri = __builtin_amdgcn_readfirstlane(tr.y);
     load_sources(source_of(k, t0, 0), uinf, ubase);
     deposit(R0, uinf, ubase, col, 0);
   }
@@ -1441,10 +1437,15 @@ __device__ __forceinline__ void m2l_packed_db_loop(
     next_tile(k, t0, kn, tn);
     kn = __builtin_amdgcn_readfirstlane(kn);
     tn = __builtin_amdgcn_readfirstlane(tn);
+    // the next tile's sources are requested before this tile's stages:
+    // the registers were emptied by the last deposit, and cs[p ^ 1] was
+    // last read before the barrier that ended the previous tile
     int th_n = 0, ri_n = 0;
     if (kn < 216) {
-      th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
-      ri_n = __builtin_amdgcn_readfirstlane(P->rix[offset_of(kn)]);
+      const int2 tr = kth[kn];
+      th_n = __builtin_amdgcn_readfirstlane(tr.x);
+      ri_n = __builtin_amdgcn_readfirstlane(tr.y);
+      load_sources(source_of(kn, tn, p ^ 1), tu, ub);
     }
     // R0 -> R1 -> R0 -> R1, one barrier per stage
     if constexpr (PF)
@@ -1452,7 +1453,6 @@ __device__ __forceinline__ void m2l_packed_db_loop(
     else
       stage_wo<double, N, W, WI>(A1 + (size_t)th * KT * 64, R0, R1, t1, ln, gp, cl);
     __syncthreads();
-    if (kn < 216) load_sources(source_of(kn, tn, p ^ 1), tu, ub);
     if constexpr (PF)
       stage_wop<double, N, W, WI>(av, A3 + (size_t)th * KT * 64, R1, R0, t2, ln, gp, cl);
     else
@@ -1527,6 +1527,8 @@ void k_lap_m2l_packed_db(
   __shared__ uint8_t ncols[216];
   __shared__ int4 tco[kPackT];
   __shared__ uint32_t tbox[kPackT];
+  __shared__ int4 ktab[216];  // dilated offsets (x, y, z) and offset index
+  __shared__ int2 kth[216];   // rotation and coaxial table indices
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1554,14 +1556,21 @@ void k_lap_m2l_packed_db(
   for (int i = tid; i < kPackT * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
   for (int i = tid; i < 216 * kPackT; i += 64 * W) pos[i / kPackT][i % kPackT] = 0xff;
 
-  const unsigned nch = ca.chunk_prefix[8], bid = blockIdx.x, q8 = nch / 8;
-  const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
-  int c = 0;
-  while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+  int c;
+  unsigned cj;
+  // a block with no chunk leaves before any barrier (uniform over the block)
+  if (!block_chunk(blockIdx.x, ca, c, cj)) return;
   const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
   const int nbox = 1 << level;
+  for (int k = tid; k < 216; k += 64 * W) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int off = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    ktab[k] = make_int4((int)dilate_offset(dx, 0, kMortonX), (int)dilate_offset(dy, 1, kMortonY),
+                        (int)dilate_offset(dz, 2, kMortonZ), off);
+    kth[k] = make_int2(P->th[off], P->rix[off]);
+  }
   if (tid < kPackT) {
-    const unsigned kk = (chunk - ca.chunk_prefix[c]) * kPackT + tid;
+    const unsigned kk = cj * kPackT + tid;
     const bool tv = kk < ca.count[c];
     const uint32_t tb = tv ? cls[ca.start[c] + kk] : 0u;
     tbox[tid] = tb;
@@ -1600,7 +1609,7 @@ void k_lap_m2l_packed_db(
 
 #define PACKED_DB(WI)                                                                          \
   m2l_packed_db_loop<N, W, WI>(R0, R1, tmap1, tmap2, tmap3, uinf, cs, anym, lists, pos, ncols, \
-                               tco, tbox, lane, px, py, pz, P, A1, A3, A2, CS, zero, mps, mpr)
+                               tco, tbox, ktab, kth, lane, A1, A3, A2, CS, zero, mps, mpr)
   if constexpr (W == 4) {
     switch (wv) {
       case 0: PACKED_DB(0); break;
@@ -1668,14 +1677,25 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
       const char *pd = getenv("WBFMM_PACKED_DB");
       if constexpr (PackedDbFits<N>::value) {
         if (!(pd && atoi(pd) == 0)) {
+          // class order; WBFMM_M2L_XCD=1 interleaves the 8 classes on each
+          // XCD (measured 1.4 % slower on the 1e6-point depth-7 solve)
+          unsigned grid = nch;
+          const char *xe = getenv("WBFMM_M2L_XCD");
+          if (xe && atoi(xe) == 1) {
+            unsigned mx = 0;
+            for (int c = 0; c < 8; c++)
+              mx = std::max(mx, cp.chunk_prefix[c + 1] - cp.chunk_prefix[c]);
+            cp.qc = (mx + 7) / 8;
+            grid = 64 * cp.qc;
+          }
           if (!(pw && atoi(pw) == 4))
-            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 8>), dim3(nch), dim3(512), 0, cls,
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 8>), dim3(grid), dim3(512), 0, cls,
                          cp, occ, level, (const double *)mps, (double *)mpr,
                          (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
                          (const double *)A3, (const double *)A2, (const double *)CS,
                          (const double *)zero);
           else
-            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 4>), dim3(nch), dim3(256), 0, cls,
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_db<N, 4>), dim3(grid), dim3(256), 0, cls,
                          cp, occ, level, (const double *)mps, (double *)mpr,
                          (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
                          (const double *)A3, (const double *)A2, (const double *)CS,
@@ -1697,6 +1717,19 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
     }
   }
   const int nsplit = m2l_nsplit(nchunks);
+  // class order; WBFMM_M2L_XCD=1 interleaves the 8 classes on each XCD at
+  // unsplit levels
+  ClassArgs cx = ca;
+  unsigned grid = nchunks * nsplit;
+  {
+    const char *xe = getenv("WBFMM_M2L_XCD");
+    if (nsplit == 1 && xe && atoi(xe) == 1) {
+      unsigned mx = 0;
+      for (int c = 0; c < 8; c++) mx = std::max(mx, ca.chunk_prefix[c + 1] - ca.chunk_prefix[c]);
+      cx.qc = (mx + 7) / 8;
+      grid = 64 * cx.qc;
+    }
+  }
   T *part = part_ext;
   if (nsplit > 1 && !part_ext &&
       dmalloc((void **)&part, sizeof(T) * (size_t)nsplit * ntot * C))
@@ -1708,27 +1741,27 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   // several bins) have the four-wavefront kernel only
   if constexpr (sizeof(T) == 8 && N >= 9 && N <= 15) {
     if (m2l_waves() == 8) {
-      WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_mfma2<T, N, 8>), dim3(nchunks * nsplit), dim3(512), 0,
-                   cls, ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS,
+      WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_mfma2<T, N, 8>), dim3(grid), dim3(512), 0,
+                   cls, cx, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan, A1, A3, A2, CS,
                    zero, nsplit, part, ntot);
       goto reduce;
     }
   }
   if (N > 15 || m2l_waves() == 4 || m2l_waves() == 8) {
     WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 4>),
-                 dim3(nchunks * nsplit), dim3(256), 0, cls,
-                 ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                 dim3(grid), dim3(256), 0, cls,
+                 cx, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                  A1, A3, A2, CS, zero, nsplit, part, ntot);
   } else if constexpr (N <= 15) {
     if (m2l_waves() >= 2 && (sizeof(T) == 4 || N <= 12))
       WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma2<T, N, 2>),
-                   dim3(nchunks * nsplit), dim3(128), 0, cls,
-                   ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                   dim3(grid), dim3(128), 0, cls,
+                   cx, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                    A1, A3, A2, CS, zero, nsplit, part, ntot);
     else
       WBFMM_LAUNCH(sizeof(T) == 4 ? "lap_m2l_f32" : "lap_m2l", (k_lap_m2l_mfma<T, N>),
-                   dim3(nchunks * nsplit), dim3(64), 0, cls,
-                   ca, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
+                   dim3(grid), dim3(64), 0, cls,
+                   cx, occ, level, mps, mpr, (const wbfmm_m2l_plan_t *)plan,
                    A1, A3, A2, CS, zero, nsplit, part, ntot);
   }
 reduce:

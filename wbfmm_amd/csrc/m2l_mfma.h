@@ -47,7 +47,33 @@ __device__ __forceinline__ typename MF<T>::t2 mk2(T a, T b) {
 struct ClassArgs {
   unsigned start[8], count[8];
   unsigned chunk_prefix[9];
+  // chunks per class and XCD when the blocks interleave the classes (see
+  // block_chunk); 0: chunks in class order
+  unsigned qc;
 };
+
+// The chunk (class c, index j within the class) of block bid, or false
+// for a block with none.  Blocks go round-robin over the 8 XCDs (block
+// bid on XCD bid % 8).  qc == 0: chunks in class order, consecutive chunks
+// of one range on each XCD.  qc > 0: XCD x takes chunks x qc .. x qc + qc
+// - 1 of every class, the 8 classes interleaved, so the blocks an XCD
+// runs at one time cover one compact region of the level and share its
+// source expansions in that XCD's L2 (grid: 64 qc blocks).
+__device__ __forceinline__ bool block_chunk(unsigned bid, const ClassArgs &ca, int &c,
+                                            unsigned &j) {
+  if (ca.qc == 0) {
+    const unsigned nch = ca.chunk_prefix[8], q8 = nch / 8;
+    const unsigned chunk = bid < 8 * q8 ? (bid & 7) * q8 + (bid >> 3) : bid;
+    c = 0;
+    while (c < 7 && chunk >= ca.chunk_prefix[c + 1]) c++;
+    j = chunk - ca.chunk_prefix[c];
+    return chunk < nch;
+  }
+  const unsigned x = bid & 7, local = bid >> 3, jj = local >> 3;
+  c = (int)(local & 7);
+  j = x * ca.qc + jj;
+  return jj < ca.qc && j < ca.chunk_prefix[c + 1] - ca.chunk_prefix[c];
+}
 
 // Compile-time copy of the bin packing of wbfmm_tables.c (pack_bins over
 // the cos/sin blocks, sizes n+1 and n): number of bins and the k steps of

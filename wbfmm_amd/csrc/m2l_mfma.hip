@@ -193,6 +193,7 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
                      const T *A2, const T *CS, const wbfmm_m2l_plan_t *plan_host,
                      unsigned char *ext = nullptr) {
   ClassArgs ca;
+  ca.qc = 0;
   unsigned nch = 0;
   for (int c = 0; c < 8; c++) {
     ca.start[c] = cstart[c];
@@ -202,7 +203,9 @@ static int m2l_entry(const uint32_t *cls, const unsigned *cstart, const unsigned
   }
   ca.chunk_prefix[8] = nch;
   if (nch == 0) return 0;
-  if (level > 20) return WBFMM_HIP_ERR_ARG;
+  // source boxes are found by 30-bit Morton arithmetic (levels up to 10;
+  // a dense level array above that exceeds any device memory)
+  if (level > (unsigned)kMortonAddMaxLevel) return WBFMM_HIP_ERR_ARG;
   if (N < 2 || N > 24) return WBFMM_HIP_ERR_ORDER;
   const uint64_t nbox = (uint64_t)1 << (3 * level);
   const size_t words = (size_t)((nbox + 31) / 32);

@@ -235,6 +235,9 @@ typedef struct {
 /* orders served by the FP32 passes (the FP64 matrix-core M2M / L2L serve
    M2L_MFMA_MIN..M2L_MFMA_MAX) */
 #define SHIFT_MFMA_MAX 15
+/* deepest level served by the matrix-core M2L (30-bit Morton arithmetic
+   on source boxes; a dense level array below it exceeds device memory) */
+#define M2L_MFMA_MAX_LEVEL 10
 /* source components of a Laplace tree (nq) */
 #define LAP_MAX_NQ 8
 
@@ -1394,6 +1397,8 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
     if (t->order_r[level] != t->order_s[level] || !mfma_order(N) ||
         (level < t->depth && t->order_r[level + 1] != t->order_r[level]))
       WBFMM_FATAL("FP32 passes need equal orders in %d..%d", M2L_MFMA_MIN, SHIFT_MFMA_MAX);
+    if (level > M2L_MFMA_MAX_LEVEL)
+      WBFMM_FATAL("FP32 passes serve levels up to %d", M2L_MFMA_MAX_LEVEL);
     ensure_f32_tables(od, N);
     HIPOK(wbfmm_hip_malloc((void **)&cls, sizeof(uint32_t) * (nt ? nt : 1)));
     HIPOK(wbfmm_hip_split_classes(tl, nt, cls, cstart, ccount));
@@ -1412,7 +1417,7 @@ static gint lap_downward(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
                                        od->f_A2dn[N], od->f_CSdn[N], od->m2l_plan[N]));
     return 0;
   }
-  if (t->order_s[level] == t->order_r[level] &&
+  if (t->order_s[level] == t->order_r[level] && level <= M2L_MFMA_MAX_LEVEL &&
       (int)t->order_s[level] >= M2L_MFMA_MIN && (int)t->order_s[level] <= M2L_MFMA_MAX) {
     /* matrix-core form: targets grouped by Morton child digit (once for
        every component) */
@@ -2020,7 +2025,8 @@ static void hel_prepare(wbfmm_tree_t *t, double k)
     const unsigned ns = t->order_s[l], nr = t->order_r[l], N = ns > nr ? ns : nr;
     const size_t nb = (size_t)1 << (3 * l);
     const int want64 = hel_scale_arg(k, t->D, l) > hel_fp64_ka() || ns != nr ||
-                       (int)N < M2L_MFMA_MIN || (int)N > HEL_MFMA_MAX;
+                       (int)N < M2L_MFMA_MIN || (int)N > HEL_MFMA_MAX ||
+                       l > M2L_MFMA_MAX_LEVEL;
     if (want64 != td->h64[l]) {
       const size_t es = want64 ? sizeof(double) : sizeof(float);
       if (ns && t->mps[l]) alloc_level(td, l, 0, nb * 2 * (ns + 1) * (ns + 1) * es);
@@ -2409,7 +2415,8 @@ static void hel_m2l64(wbfmm_tree_t *t, ops_dev *od, guint level, const uint32_t 
   tree_dev *td = TD(t);
   wbfmm_shift_operators_t *op = &od->op;
   guint Ns = t->order_s[level], Nr = t->order_r[level];
-  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= HEL_MFMA_MAX && hel_mfma_enabled()) {
+  if (Ns == Nr && (int)Ns >= M2L_MFMA_MIN && (int)Ns <= HEL_MFMA_MAX && hel_mfma_enabled() &&
+      level <= M2L_MFMA_MAX_LEVEL) {
     /* matrix-core form: targets grouped by Morton child digit */
     int N = (int)Ns;
     unsigned cstart[8], ccount[8];
@@ -2741,8 +2748,9 @@ gint wbfmm_downward_pass_avx(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
   return hel_downward(t, op, level);
 }
 
-/* the FP64 field kernel: one thread per target, or one wavefront per
-   target box (WBFMM_HEL_FIELD=box) */
+/* the field kernels: one thread per target (FP64 or mixed), or one
+   wavefront per target box (WBFMM_HEL_FIELD=box, kept for comparison:
+   slower on leaves of a few tens of targets) */
 static int hel_field_box64(void)
 {
   const char *e = getenv("WBFMM_HEL_FIELD");
@@ -2772,8 +2780,14 @@ static void hel_field_run(tree_dev *td, double k, const double *dxt,
   dp = gather_strengths_n(td, d, dstr, 1, 2);
   nelem = nf_rows ? (nf_rows - 1) * (size_t)fstr + ((fl & 1) ? 2 : 0) + ((fl & 2) ? 6 : 0) : 0;
   df = field_begin(f, nelem, &staged);
-  if (f32 || hel_field_box64()) {
-    /* one wavefront per box holding targets */
+  if (f32 && !hel_field_box64()) {
+    /* mixed precision, one thread per target */
+    HIPOK(wbfmm_hip_hel_field_f32(dxt, dtid, n, td->d_start[lv], td->d_end[lv], td->d_xyz, q,
+                                  dp ? td->d_nrm : NULL, dp, lv, ORIGIN(t), t->D,
+                                  t->order_r[lv], k, (const float *)t->mpr[lv], td->d_hsc[lv],
+                                  fl, df, fstr));
+  } else if (f32 || hel_field_box64()) {
+    /* one wavefront per box holding targets (WBFMM_HEL_FIELD=box) */
     const size_t nb = (size_t)1 << (3 * lv);
     uint32_t *ts = NULL, *te = NULL, *tb = NULL;
     unsigned ntb = 0;
