@@ -510,12 +510,22 @@ __device__ __forceinline__ void m2l_w2_loop(
   }
 }
 
-// two wavefronts per SIMD where the LDS buffer leaves room for them (FP32,
-// and FP64 up to order 15); above, LDS limits the blocks per CU anyway and
-// the registers are not capped
+// Wavefronts per SIMD the register file must hold: at FP64 orders 9..12
+// with eight wavefronts, three blocks per CU (the double LDS buffer of
+// about 50 KB leaves room for three, and 80 registers per lane give six
+// wavefronts per SIMD); otherwise two where the LDS buffer leaves room for
+// them (FP32, and FP64 up to order 15); above, LDS limits the blocks per
+// CU anyway and the registers are not capped
+template <typename T, int N, int W>
+constexpr int m2l_waves_per_eu() {
+  return (sizeof(T) == 8 && W == 8 && N >= 9 && N <= 12) ? 6
+         : (sizeof(T) == 4 || N <= 15)                   ? 2
+                                                         : 1;
+}
+
 template <typename T, int N, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
-    sizeof(T) == 4 || N <= 15 ? 2 : 1))) void k_lap_m2l_mfma2(
+    m2l_waves_per_eu<T, N, W>()))) void k_lap_m2l_mfma2(
     const uint32_t *__restrict__ cls, ClassArgs ca,
     const uint32_t *__restrict__ occ, int level,
     const T *__restrict__ mps, T *__restrict__ mpr,
