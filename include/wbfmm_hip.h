@@ -315,6 +315,21 @@ int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
                       const double *nrm, const double *dip,
                       unsigned level, const double *x0, double D, int N,
                       double *mps);
+/* the same expansions with one wavefront per box, for boxes holding many
+   points (the API picks it at 4 or more points per occupied leaf); _f32:
+   into the FP32 level (M_n / w^n) of wbfmm_hip_lap_p2m_f32 */
+int wbfmm_hip_lap_p2m_wave(const uint32_t *boxes, unsigned nbox,
+                           const uint32_t *start, const uint32_t *end,
+                           const double *xyz, const double *q,
+                           const double *nrm, const double *dip,
+                           unsigned level, const double *x0, double D, int N,
+                           double *mps);
+int wbfmm_hip_lap_p2m_wave_f32(const uint32_t *boxes, unsigned nbox,
+                               const uint32_t *start, const uint32_t *end,
+                               const double *xyz, const double *q,
+                               const double *nrm, const double *dip,
+                               unsigned level, const double *x0, double D, int N,
+                               float *mps);
 /* M2M: parents (list at level-1) from children at level
    (src/pass-laplace.c:433-486, src/shift-laplace.c:225-442) */
 int wbfmm_hip_lap_m2m(const uint32_t *parents, unsigned npar,

@@ -9,9 +9,10 @@
   source points and on interior points (radius 0.5), whose field is all
   far field, so the near-field sum cannot dominate the norm.
 - Batched: 256 problems (N = 10^U(3,5), Laplace / Helmholtz alternating,
-  cube or sphere, tol 1e-6) through the engine at once; every problem's
-  error against the FP64 direct sum is checked and a sample of problems is
-  solved again alone, giving the same bits.
+  cube or sphere, tol 1e-6; depth and precision picked by the engine, so
+  the Helmholtz problems run mixed FP32/FP64) through the engine at once;
+  every problem's error against the FP64 direct sum is checked and a
+  sample of problems is solved again alone, giving the same bits.
 - Serving: a Poisson stream of mixed requests, with Helmholtz at k = 32
   and tol 1e-8 among them, each within its requested accuracy; requests
   rerun alone give the same bits.
@@ -175,7 +176,8 @@ def test_batched_256_problems(wb, S):
     _upload(wb, probs)
     try:
         with S.Server(8) as srv:
-            tk = [srv.submit(_request(S, p, check=100, depth=p["depth"])) for p in probs]
+            # depth and precision picked by the engine, as in the batched bench
+            tk = [srv.submit(_request(S, p, check=100)) for p in probs]
             res = [srv.wait(t) for t in tk]
             assert all(r.status == 0 for r in res), [r.status for r in res if r.status]
             ep = np.array([r.err_potential for r in res])
@@ -189,7 +191,7 @@ def test_batched_256_problems(wb, S):
             batch = {i: _field(probs[i]) for i in pick}
             for i in pick:
                 probs[i]["d_f"].zero()
-                r = srv.wait(srv.submit(_request(S, probs[i], depth=probs[i]["depth"])))
+                r = srv.wait(srv.submit(_request(S, probs[i])))
                 assert r.status == 0
                 assert np.array_equal(_field(probs[i]), batch[i]), i
     finally:

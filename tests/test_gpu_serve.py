@@ -5,7 +5,7 @@ and depth are reproduced here) at rounding level, the engine's error report
 against the oracle's FP64 direct sum, identical bits for a request run
 alone and inside a mix of concurrent requests, admission under a device
 memory budget, device-resident arrays, concurrent submitting threads, the
-autotuner and the refusal of invalid requests."""
+autotuner, resident Helmholtz tables and the refusal of invalid requests."""
 
 import threading
 
@@ -298,6 +298,21 @@ def test_serve_failed_request_reports_status_and_engine_survives(S):
     for a, b in zip(ref, got):
         assert np.array_equal(a, b)
     assert st.completed == len(cases)
+
+
+def test_serve_helmholtz_tables_resident(S):
+    # Helmholtz operator sets stay resident across requests (an engine-wide
+    # cache keyed by k, domain, depth and orders): a problem solved again
+    # after one of another wavenumber builds no tables, on whichever worker
+    # takes it, and gives the same bits
+    a, b = hel_case(95, 3000, 3.0), hel_case(96, 3000, 5.0)
+    with S.Server(2) as srv:
+        fa, ra = solve(S, srv, a, tol=1e-6)
+        fb, rb = solve(S, srv, b, tol=1e-6)
+        fa2, ra2 = solve(S, srv, a, tol=1e-6)
+    assert ra.tables_ms > 0 and rb.tables_ms > 0
+    assert ra2.tables_ms < 0.05 * ra.tables_ms, (ra.tables_ms, ra2.tables_ms)
+    assert np.array_equal(fa, fa2)
 
 
 def test_serve_refuses_request_above_budget(S):

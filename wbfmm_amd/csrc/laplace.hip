@@ -152,6 +152,122 @@ __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
 }
 
 // ---------------------------------------------------------------------
+// The same expansions with one wavefront per box, for boxes that hold many
+// points (clustered or small problems, where one thread per box leaves
+// most of the chip idle and serialises hundreds of points).  The box's
+// points are dealt to the wavefront's 64/GS groups of GS lanes, point
+// start + i to group i mod (64/GS); lane m of a group computes column m of
+// the regular solid harmonics (n = m..N) of its group's points and adds
+// the monopole and dipole terms into the group's copy of the expansion in
+// LDS.  Each of the three dipole terms is a statement of its own, in which
+// every lane writes a distinct coefficient; a wavefront's LDS operations
+// complete in program order.  The groups' copies are added in group order
+// and the sum added to the box's expansion: no atomics, the same bits on
+// every run.
+// ---------------------------------------------------------------------
+template <typename T, int GS>
+__global__ __launch_bounds__(64) void k_lap_p2m_wave(
+    const uint32_t *boxes, unsigned nbox, const uint32_t *start, const uint32_t *end,
+    const double *xyz, const double *q, const double *nrm, const double *dip, int level,
+    double x0, double y0, double z0, double D, int N, double rs, T *mps) {
+  extern __shared__ double pacc[];  // (64 / GS) copies of C coefficients
+  constexpr int G = 64 / GS;
+  const int lane = threadIdx.x, g = lane / GS, m = lane % GS;
+  const int C = (N + 1) * (N + 1);
+  const unsigned t = blockIdx.x;
+  if (t >= nbox) return;
+  for (int i = lane; i < G * C; i += 64) pacc[i] = 0.0;
+  __syncthreads();
+  const uint64_t b = boxes[t];
+  const uint32_t bi = compact3(b), bj = compact3(b >> 1), bk = compact3(b >> 2);
+  const double w = D / (double)(1u << level);
+  const double xc = x0 + w * (bi + 0.5), yc = y0 + w * (bj + 0.5), zc = z0 + w * (bk + 0.5);
+  const double s4pi = 1.0 / sqrt(4.0 * kPi);
+  double *acc = pacc + g * C;
+  auto put = [&](int n, int mm, double re, double im) {
+    if (mm == 0)
+      acc[n * n] += re;
+    else {
+      acc[n * n + 2 * mm - 1] += re;
+      acc[n * n + 2 * mm] -= im;
+    }
+  };
+  for (uint32_t s = start[b] + g; s < end[b]; s += G) {
+    if (m > N) continue;
+    const double dx = xyz[4 * (size_t)s] - xc, dy = xyz[4 * (size_t)s + 1] - yc,
+                 dz = xyz[4 * (size_t)s + 2] - zc;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz), rho = sqrt(dx * dx + dy * dy);
+    const double Cth = r > 0.0 ? dz / r : 1.0, Sth = r > 0.0 ? rho / r : 0.0;
+    const double cp = rho > 0.0 ? dx / rho : 1.0, sp = rho > 0.0 ? dy / rho : 0.0;
+    const double qs = q ? q[s] : 0.0, ds = dip ? dip[s] * rs : 0.0;
+    const double rr = r * rs;
+    double nx = 0, ny = 0, nz = 0;
+    if (dip) {
+      nx = nrm[4 * (size_t)s];
+      ny = nrm[4 * (size_t)s + 1];
+      nz = nrm[4 * (size_t)s + 2];
+    }
+    // P_m^m, r^m and e^{i m phi} of this lane's column
+    double pmm = s4pi, rm = 1.0, cm = 1.0, sm = 0.0;
+    for (int i = 1; i <= m; i++) {
+      pmm *= sqrt((2.0 * i + 1) / (2.0 * i)) * Sth;
+      rm *= rr;
+      const double tcm = cm * cp - sm * sp;
+      sm = sm * cp + cm * sp;
+      cm = tcm;
+    }
+    double p1 = 0.0, p0 = pmm, rn = rm;
+    for (int n = m; n <= N; n++) {
+      if (n > m) {
+        const double pn = (Cth * sqrt(2.0 * (n - 1) + 1) * p0 -
+                           sqrt((double)((n - 1) * (n - 1) - m * m) / (2 * (n - 1) - 1)) * p1) *
+                          sqrt((2.0 * (n - 1) + 3) / (double)(n * n - m * m));
+        p1 = p0;
+        p0 = pn;
+        rn *= rr;
+      }
+      const double Rr = rn * p0 * cm, Ri = rn * p0 * sm;
+      if (qs != 0.0) {
+        const double f = qs / (2 * n + 1);
+        put(n, m, f * Rr, f * Ri);
+      }
+      if (ds != 0.0 && n < N) {
+        // streaming form of M_{n+1} += d conj(nrm . grad R_{n+1})/(2n+3),
+        // as in k_lap_p2m: (a) column m, (b) column m - 1, (c) column m + 1
+        const int np = n + 1;
+        const double fc = sqrt((2.0 * np + 1) / (2.0 * np - 1)) * ds / (2 * np + 1);
+        const double a = fc * sqrt((double)(np - m) * (np + m));
+        put(np, m, nz * a * Rr, nz * a * Ri);
+        if (m >= 1) {
+          const int mm = m - 1;
+          const double b1 = fc * sqrt((double)(np - mm) * (np - mm - 1));
+          const double ur = 0.5 * nx, ui = -0.5 * ny;  // nu+
+          const double vr = ur * Rr - ui * Ri, vi = ur * Ri + ui * Rr;
+          if (mm == 0)
+            acc[np * np] += -b1 * 2.0 * vr;
+          else
+            put(np, mm, -b1 * vr, -b1 * vi);
+        }
+        {
+          const int mp = m + 1;
+          const double b2 = fc * sqrt((double)(np + mp) * (np + mp - 1));
+          const double ur = 0.5 * nx, ui = 0.5 * ny;  // nu-
+          const double vr = ur * Rr - ui * Ri, vi = ur * Ri + ui * Rr;
+          put(np, mp, b2 * vr, b2 * vi);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = lane; c < C; c += 64) {
+    double v = pacc[c];
+#pragma unroll
+    for (int h = 1; h < G; h++) v += pacc[h * C + c];
+    mps[coef_at(b, c, C)] += (T)v;
+  }
+}
+
+// ---------------------------------------------------------------------
 // M2M (src/pass-laplace.c:433-486, src/shift-laplace.c:225-442)
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_lap_m2m(
@@ -810,6 +926,32 @@ __global__ void k_d2f(const double *in, float *out, size_t n) {
 
 using namespace wbfmm;
 
+// one wavefront per box: groups of 16 lanes up to order 15, 32 up to 31,
+// 64 above; LDS holds one expansion per group
+template <typename T>
+static int lap_p2m_wave(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                        const uint32_t *end, const double *xyz, const double *q,
+                        const double *nrm, const double *dip, unsigned level, const double *x0,
+                        double D, int N, double rs, T *mps, const char *name) {
+  if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
+  if (nbox == 0) return 0;
+  const size_t C = (size_t)(N + 1) * (N + 1);
+  if (N <= 15) {
+    WBFMM_LAUNCH(name, (k_lap_p2m_wave<T, 16>), dim3(nbox), dim3(64), 4 * C * sizeof(double),
+                 boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0], x0[1], x0[2], D,
+                 N, rs, mps);
+  } else if (N <= 31) {
+    WBFMM_LAUNCH(name, (k_lap_p2m_wave<T, 32>), dim3(nbox), dim3(64), 2 * C * sizeof(double),
+                 boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0], x0[1], x0[2], D,
+                 N, rs, mps);
+  } else {
+    WBFMM_LAUNCH(name, (k_lap_p2m_wave<T, 64>), dim3(nbox), dim3(64), C * sizeof(double),
+                 boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0], x0[1], x0[2], D,
+                 N, rs, mps);
+  }
+  return 0;
+}
+
 extern "C" {
 
 int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
@@ -837,6 +979,23 @@ int wbfmm_hip_lap_p2m_f32(const uint32_t *boxes, unsigned nbox,
                boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0],
                x0[1], x0[2], D, N, 1.0 / w, mps);
   return 0;
+}
+
+int wbfmm_hip_lap_p2m_wave(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                           const uint32_t *end, const double *xyz, const double *q,
+                           const double *nrm, const double *dip, unsigned level,
+                           const double *x0, double D, int N, double *mps) {
+  return lap_p2m_wave<double>(boxes, nbox, start, end, xyz, q, nrm, dip, level, x0, D, N, 1.0,
+                              mps, "lap_p2m");
+}
+
+int wbfmm_hip_lap_p2m_wave_f32(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                               const uint32_t *end, const double *xyz, const double *q,
+                               const double *nrm, const double *dip, unsigned level,
+                               const double *x0, double D, int N, float *mps) {
+  const double w = D / (double)(1u << level);
+  return lap_p2m_wave<float>(boxes, nbox, start, end, xyz, q, nrm, dip, level, x0, D, N,
+                             1.0 / w, mps, "lap_p2m_f32");
 }
 
 int wbfmm_hip_lap_m2m(const uint32_t *parents, unsigned npar,

@@ -11,6 +11,7 @@
 // changes (the rotation tables depend on the order only).
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -83,18 +84,35 @@ struct Job {
   int fail_allocs = 0;  // device allocations forced to fail (tests)
 };
 
-struct HelOps {
-  wbfmm_shift_operators_t *op = nullptr;
+// Helmholtz operator sets stay resident in an engine-wide cache, one per
+// (largest order, k, domain width, depth, orders of the levels): the
+// coaxial tables depend on k and the box widths, so requests of another
+// wavenumber or domain need their own set.  A set is completed by the
+// first solve that uses it (the passes build some tiles on first use), so
+// a worker holds the set's `use` lock for its whole solve; solves end with
+// a stream synchronisation, so the next holder, on any stream, finds the
+// tables complete.
+struct HelKey {
+  int lmax = 0, depth = 0;
   double k = 0.0, D = 0.0;
-  int depth = 0;
-  int orders[WBFMM_TREE_MAX_DEPTH + 1] = {0};
+  std::array<int, WBFMM_TREE_MAX_DEPTH + 1> orders{};
+  bool operator<(const HelKey &o) const {
+    return std::tie(lmax, depth, k, D, orders) < std::tie(o.lmax, o.depth, o.k, o.D, o.orders);
+  }
+};
+
+struct HelEntry {
+  std::mutex use;
+  wbfmm_shift_operators_t *op = nullptr;
+  bool built = false;  // coaxial tables of every level present
+  size_t bytes = 0;    // estimate (the tables term of the footprint)
+  uint64_t last = 0;   // last use (engine tick), for eviction
 };
 
 struct Worker {
   std::thread th;
   void *stream = nullptr;
   std::map<int, wbfmm_shift_operators_t *> lap_ops;  // by order
-  std::map<int, HelOps> hel_ops;                     // by largest order
 };
 
 }  // namespace
@@ -116,6 +134,10 @@ struct wbfmm_serve_s {
   clk::time_point first_submit, last_done;
   // autotuned depths: (problem, precision, order, log2 n) -> depth
   std::map<std::tuple<int, int, int, int>, int> tuned;
+  // resident Helmholtz operator sets (at most hel_cap bytes of them idle)
+  std::map<HelKey, std::shared_ptr<HelEntry>> hel_cache;
+  size_t hel_bytes = 0, hel_cap = 0;
+  uint64_t tick = 0;
 };
 
 namespace {
@@ -128,13 +150,16 @@ std::tuple<int, int, int, int> tune_key(const wbfmm_serve_request_t &r, int prec
 
 // FP32 (Laplace at orders 2..15; Helmholtz in mixed precision, FP32 at
 // the low-frequency levels) when asked for, or by default when the
-// accuracy asked for is 1e-5 or looser; FP64 otherwise
+// accuracy asked for allows it: 1e-5 or looser for Laplace (every level in
+// FP32), 1e-6 or looser for Helmholtz (FP32 only where k a <= 2, the near
+// field in FP32 with compensated sums); FP64 otherwise
 int choose_precision(const wbfmm_serve_request_t &r, int lap_order) {
   const bool fp32_ok = r.problem == WBFMM_PROBLEM_HELMHOLTZ ||
                        (r.problem == WBFMM_PROBLEM_LAPLACE && lap_order >= 2 && lap_order <= 15);
   if (r.precision == 32) return fp32_ok ? 32 : 0;
   if (r.precision == 64) return 64;
-  return fp32_ok && r.order == 0 && r.tol >= 1e-5 ? 32 : 64;
+  const double loosest = r.problem == WBFMM_PROBLEM_HELMHOLTZ ? 1e-6 : 1e-5;
+  return fp32_ok && r.order == 0 && r.tol >= loosest ? 32 : 64;
 }
 
 // device bytes of a solve:
@@ -148,6 +173,18 @@ int choose_precision(const wbfmm_serve_request_t &r, int lap_order) {
 //    16384 targets' expansions), and the radix sort's temporaries;
 //  - the operator tables a worker builds for this order (rotation and
 //    coaxial tiles; kept by the worker and reused by later requests)
+// tables: rotation tiles of the order (49 angles, one 64-double tile per
+// k-step, KT <= (N+1)^2 / 2), coaxial tiles of every level (15 lengths,
+// complex for Helmholtz) and the Helmholtz host-side coaxial tables
+size_t table_bytes(int problem, const Plan &p) {
+  const size_t cplx = problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
+  const size_t N = (size_t)p.lmax;
+  const size_t kt = (N + 1) * (N + 1) / 2 + 8;
+  size_t tables = 49 * kt * 64 * 8 * 2 + (size_t)p.depth * 15 * 2 * kt * 64 * 8 * cplx;
+  if (cplx == 2) tables += (size_t)p.depth * 15 * 2 * (2 * N + 1) * (2 * N + 2) * (2 * N + 3) / 6 * 8;
+  return tables;
+}
+
 size_t footprint(const wbfmm_serve_request_t &r, const Plan &p) {
   const size_t cplx = r.problem == WBFMM_PROBLEM_HELMHOLTZ ? 2 : 1;
   size_t b = 0, transient = 0;
@@ -167,14 +204,7 @@ size_t footprint(const wbfmm_serve_request_t &r, const Plan &p) {
   b += nt * (32 + 12 + 8 * (size_t)std::max(r.fstr, 1));
   if (r.check) b += (size_t)r.nsrc * (32 + 16 * cplx);
   transient = std::max(transient, (size_t)std::max(r.nsrc, (unsigned)nt) * 24);
-  // tables: rotation tiles of the order (49 angles, one 64-double tile per
-  // k-step, KT <= (N+1)^2 / 2), coaxial tiles of every level (15 lengths,
-  // complex for Helmholtz) and the Helmholtz host-side coaxial tables
-  const size_t N = (size_t)p.lmax;
-  const size_t kt = (N + 1) * (N + 1) / 2 + 8;
-  size_t tables = 49 * kt * 64 * 8 * 2 + (size_t)p.depth * 15 * 2 * kt * 64 * 8 * cplx;
-  if (cplx == 2) tables += (size_t)p.depth * 15 * 2 * (2 * N + 1) * (2 * N + 2) * (2 * N + 3) / 6 * 8;
-  return b + transient + tables + ((size_t)16 << 20);
+  return b + transient + table_bytes(r.problem, p) + ((size_t)16 << 20);
 }
 
 int validate(const wbfmm_serve_request_t &r) {
@@ -306,25 +336,42 @@ int coax_body(void *a) {
   return 0;
 }
 
-// Helmholtz operator set of the worker for this plan; *tables_ms gets the
-// milliseconds spent on tables.  Coaxial tables of the levels are built
-// in parallel host threads that share the worker's stream.  Failures (in
-// any of the threads) return a status.
-int helmholtz_ops(Worker &w, const Plan &p, double k, double *tables_ms,
-                  wbfmm_shift_operators_t **out, char *msg, size_t ml) {
+// Helmholtz operator set of this plan from the engine's cache, locked for
+// the caller's solve (`held`); built when missing: *tables_ms gets the
+// milliseconds spent on tables.  Coaxial tables of the levels are built in
+// parallel host threads that share the worker's stream.  Failures (in any
+// of the threads) return a status.
+int helmholtz_ops(wbfmm_serve_t *s, Worker &w, const Plan &p, double k, double *tables_ms,
+                  std::shared_ptr<HelEntry> &ent, std::unique_lock<std::mutex> &held,
+                  char *msg, size_t ml) {
   const auto t0 = clk::now();
-  HelOps &h = w.hel_ops[p.lmax];
+  HelKey key;
+  key.lmax = p.lmax;
+  key.depth = p.depth;
+  key.k = k;
+  key.D = p.D;
+  for (int l = 0; l <= p.depth; l++) key.orders[(size_t)l] = p.orders[l];
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto &e = s->hel_cache[key];
+    if (!e) {
+      e = std::make_shared<HelEntry>();
+      e->bytes = table_bytes(WBFMM_PROBLEM_HELMHOLTZ, p);
+      s->hel_bytes += e->bytes;
+    }
+    e->last = ++s->tick;
+    ent = e;
+  }
+  held = std::unique_lock<std::mutex>(ent->use);
+  HelEntry &h = *ent;
   if (!h.op) {
     OpsNew x{(guint)p.lmax, nullptr};
     const int rc = wbfmm_guarded_call(ops_new_body, &x, msg, ml);
     if (rc) return rc;
     h.op = x.op;
   }
-  bool same = h.k == k && h.D == p.D && h.depth == p.depth;
-  for (int l = 2; same && l <= p.depth; l++) same = h.orders[l] == p.orders[l];
-  if (!same) {
-    wbfmm_shift_operators_coaxial_reset(h.op);
-    h.depth = -1;  // tables incomplete until every level is built
+  if (!h.built) {
+    wbfmm_shift_operators_coaxial_reset(h.op);  // after a failed build
     std::vector<CoaxJob> jobs((size_t)std::max(p.depth - 1, 0));
     for (int l = 2; l <= p.depth; l++) {
       CoaxJob &c = jobs[(size_t)(l - 2)];
@@ -350,14 +397,32 @@ int helmholtz_ops(Worker &w, const Plan &p, double k, double *tables_ms,
         snprintf(msg, ml, "%s", c.msg);
         return c.rc;
       }
-    h.k = k;
-    h.D = p.D;
-    h.depth = p.depth;
-    for (int l = 0; l <= p.depth; l++) h.orders[l] = p.orders[l];
+    h.built = true;
   }
   *tables_ms = ms_between(t0, clk::now());
-  *out = h.op;
   return 0;
+}
+
+// frees least recently used idle operator sets until the idle ones fit
+// hel_cap (on the calling worker's stream; an entry in use is skipped)
+void hel_evict(wbfmm_serve_t *s) {
+  std::vector<std::shared_ptr<HelEntry>> drop;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    while (s->hel_bytes > s->hel_cap) {
+      auto victim = s->hel_cache.end();
+      for (auto it = s->hel_cache.begin(); it != s->hel_cache.end(); ++it) {
+        if (it->second.use_count() > 1) continue;  // held by a worker
+        if (victim == s->hel_cache.end() || it->second->last < victim->second->last) victim = it;
+      }
+      if (victim == s->hel_cache.end()) break;
+      s->hel_bytes -= victim->second->bytes;
+      drop.push_back(victim->second);
+      s->hel_cache.erase(victim);
+    }
+  }
+  for (auto &e : drop)
+    if (e->op) wbfmm_shift_operators_free(e->op);
 }
 
 // relative l2 errors of the checked targets against the FP64 direct sum
@@ -539,14 +604,24 @@ void run(wbfmm_serve_t *s, Worker &w, Job &j) {
   const bool helm = r.problem == WBFMM_PROBLEM_HELMHOLTZ;
   double tables_ms = 0.0;
   char msg[512] = "";
-  (void)s;
   Solve S{&r, &p, nullptr, nullptr, nullptr, nullptr, field_width(r.problem, r.field),
           r.xt ? r.ntgt : r.nsrc};
   // (Laplace tables are built on first use inside the passes, so their
   // time is part of the first solve of each order on a worker)
-  int rc = helm ? helmholtz_ops(w, p, r.k, &tables_ms, &S.op, msg, sizeof(msg))
+  std::shared_ptr<HelEntry> hent;
+  std::unique_lock<std::mutex> hheld;
+  int rc = helm ? helmholtz_ops(s, w, p, r.k, &tables_ms, hent, hheld, msg, sizeof(msg))
                 : laplace_ops(w, p.lmax, &S.op, msg, sizeof(msg));
+  if (helm && !rc) S.op = hent->op;
   if (!rc) rc = wbfmm_guarded_call(solve_body, &S, msg, sizeof(msg));
+  // the operator set is complete after a solve that ended well; after a
+  // failure it is built again by its next user
+  if (helm && hent) {
+    if (rc) hent->built = false;
+    if (hheld.owns_lock()) hheld.unlock();
+    hent.reset();
+    hel_evict(s);
+  }
   // add into the caller's output
   if (!rc) {
     if (wbfmm_hip_is_device(r.f)) {
@@ -618,7 +693,6 @@ void worker_main(wbfmm_serve_t *s, Worker *w) {
   }
   // release the worker's operator sets on its own stream
   for (auto &kv : w->lap_ops) wbfmm_shift_operators_free(kv.second);
-  for (auto &kv : w->hel_ops) wbfmm_shift_operators_free(kv.second.op);
   wbfmm_hip_sync();
   wbfmm_hip_set_stream(nullptr, nullptr);
 }
@@ -669,6 +743,8 @@ wbfmm_serve_t *wbfmm_serve_new(gint nstreams, gsize budget) {
     if (budget == 0) budget = (size_t)64 << 30;
   }
   s->budget = budget;
+  // idle Helmholtz operator sets kept resident: up to 1/16 of the budget
+  s->hel_cap = budget / 16;
   // process-wide tables of the drop-in API, set once before any worker
   wbfmm_shift_angle_table_init();
   wbfmm_laplace_coaxial_translate_init(WBFMM_HIP_MAX_ORDER + 2);
@@ -823,6 +899,11 @@ gint wbfmm_serve_free(wbfmm_serve_t *s) {
   }
   s->cv_work.notify_all();
   for (auto &w : s->workers) w->th.join();
+  // the resident Helmholtz operator sets (idle now) on the library stream
+  for (auto &kv : s->hel_cache)
+    if (kv.second->op) wbfmm_shift_operators_free(kv.second->op);
+  s->hel_cache.clear();
+  wbfmm_hip_sync();
   for (auto &w : s->workers) wbfmm_hip_stream_destroy(w->stream);
   delete s;
   return 0;

@@ -895,6 +895,7 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
   tree_dev *td = TD(t);
   unsigned d = t->depth;
   double *q, *dp;
+  int wave;
   (void)work;
   if ((int)t->nq < 1)
     WBFMM_FATAL("tree has invalid number of components in source terms (%d)", t->nq);
@@ -904,13 +905,15 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
   if (src == NULL && dipoles == NULL) return 0;
   if (t->normals == NULL && dipoles != NULL)
     WBFMM_FATAL("no normals in tree but dipole strengths specified (dipoles != NULL)");
+  /* one wavefront per box where the occupied leaves hold 4 or more points
+     on average, one thread per box otherwise */
+  wave = t->npoints >= 4u * td->nocc[d];
   if (td->prec32) {
     q = gather_strengths(td, src, sstr, 0);
     dp = gather_strengths(td, dipoles, dstr, 1);
-    HIPOK(wbfmm_hip_lap_p2m_f32(td->d_occ[d], td->nocc[d], td->d_start[d],
-                                td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                                dp, d, ORIGIN(t), t->D,
-                                t->order_s[d], (float *)t->mps[d]));
+    HIPOK((wave ? wbfmm_hip_lap_p2m_wave_f32 : wbfmm_hip_lap_p2m_f32)(
+        td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d], td->d_xyz, q,
+        dp ? td->d_nrm : NULL, dp, d, ORIGIN(t), t->D, t->order_s[d], (float *)t->mps[d]));
     return 0;
   }
   {
@@ -920,10 +923,10 @@ gint wbfmm_tree_laplace_leaf_expansions(wbfmm_tree_t *t, gdouble *src,
     for (c = 0; c < t->nq; c++) {
       q = gather_strengths(td, src ? src + c : NULL, sstr, 0);
       dp = gather_strengths(td, dipoles ? dipoles + c : NULL, dstr, 1);
-      HIPOK(wbfmm_hip_lap_p2m(td->d_occ[d], td->nocc[d], td->d_start[d],
-                              td->d_end[d], td->d_xyz, q, dp ? td->d_nrm : NULL,
-                              dp, d, ORIGIN(t), t->D,
-                              t->order_s[d], lap_plane(t, d, 0, c)));
+      HIPOK((wave ? wbfmm_hip_lap_p2m_wave : wbfmm_hip_lap_p2m)(
+          td->d_occ[d], td->nocc[d], td->d_start[d], td->d_end[d], td->d_xyz, q,
+          dp ? td->d_nrm : NULL, dp, d, ORIGIN(t), t->D, t->order_s[d],
+          lap_plane(t, d, 0, c)));
     }
   }
   return 0;
