@@ -287,6 +287,8 @@ def run_helmholtz(args):
         "kernels": kern,
     }
     S.close()
+    if args.calibrate:
+        calibration_copy(W, L)
     print(json.dumps(out))
 
 
@@ -739,15 +741,24 @@ def main():
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(p)
     if args.calibrate:
-        rows = 8 << 20
-        d_a = W.DeviceBuffer(rows * 24)
-        d_b = W.DeviceBuffer(rows * 32)
-        d_a.zero()
-        W.sync()
-        L.wbfmm_hip_stage4(d_a.ptr, 24, rows, d_b.ptr)
-        W.sync()
-        print("bench: calibration copy done", file=sys.stderr, flush=True)
+        calibration_copy(W, L)
     print(json.dumps(out))
+
+
+def calibration_copy(W, L):
+    """a copy kernel of known size for scaling rocprofv3's FETCH_SIZE and
+    WRITE_SIZE: wbfmm_hip_stage4 over 8Mi rows, 24 B read and 32 B written
+    per row"""
+    rows = 8 << 20
+    d_a = W.DeviceBuffer(rows * 24)
+    d_b = W.DeviceBuffer(rows * 32)
+    d_a.zero()
+    W.sync()
+    L.wbfmm_hip_stage4(d_a.ptr, 24, rows, d_b.ptr)
+    W.sync()
+    d_a.free()
+    d_b.free()
+    print("bench: calibration copy done", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":
