@@ -1642,6 +1642,402 @@ void k_lap_m2l_packed_db(
 #undef PACKED_DB
 }
 
+// ---------------------------------------------------------------------
+// Sparse levels with the offsets packed by class (k_lap_m2l_packed_cls).
+// Offsets with the same dz and the same dx^2 + dy^2 share the polar angle
+// and the distance, hence all three stage tables; only the azimuthal phases
+// differ, and those are applied per column (Zin in the deposit, Zout in the
+// accumulation).  For each class (in the order of the key (dz + 3) 32 +
+// dx^2 + dy^2, members in cube-index order) a block of 32 targets lists its
+// (target, member) pairs with an occupied source in target-major order and
+// deals them round-robin over T = max(ceil(n / 16), most pairs of one
+// target) tiles: position p goes to tile p mod T, column p div T.  A
+// target's pairs are consecutive positions, at most T of them, so a tile
+// holds each target at most once, and the lanes that accumulate a target
+// find its column in tile t from (r, q, cnt) = (s mod T, s div T, count),
+// s its first position.  Fewer, fuller tiles than one offset per tile (on
+// the 1e6-point depth-7 tree 4.19 M instead of 4.93 M at level 7), and
+// consecutive tiles of a class read the same tables.  The exchange buffer
+// is doubled and the expansions stay in registers, as in
+// k_lap_m2l_packed_db.  Each target sums its translations in tile order.
+// ---------------------------------------------------------------------
+constexpr int kMaxOffCls = 64;
+
+struct OffClasses {
+  int n;                        // classes (51 for every parity)
+  uint8_t nm[kMaxOffCls];       // members per class (at most 8)
+  uint8_t k[kMaxOffCls][8];     // members' cube indices, increasing
+};
+
+constexpr OffClasses make_off_classes(int c) {
+  OffClasses O{};
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  for (int key = 0; key < 7 * 32; key++) {
+    int m = 0;
+    for (int k = 0; k < 216; k++) {
+      const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+      const bool nr = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+      if (nr || (dz + 3) * 32 + dx * dx + dy * dy != key) continue;
+      O.k[O.n][m++] = (uint8_t)k;
+    }
+    if (m) O.nm[O.n++] = (uint8_t)m;
+  }
+  return O;
+}
+
+static __constant__ OffClasses kOffCls[8] = {make_off_classes(0), make_off_classes(1),
+                                             make_off_classes(2), make_off_classes(3),
+                                             make_off_classes(4), make_off_classes(5),
+                                             make_off_classes(6), make_off_classes(7)};
+static_assert(make_off_classes(0).n <= kMaxOffCls && make_off_classes(7).n <= kMaxOffCls,
+              "offset classes");
+
+template <int N>
+struct PackedClsFits {
+  static constexpr int Sz = (M2LShape<N>::S + 1) * 16 * 8;
+  static constexpr bool value = N <= 12 && 2 * Sz + 16 * 1024 <= 72 * 1024;
+};
+
+template <int N, int W, int WI>
+__device__ __forceinline__ void m2l_packed_cls_loop(
+    double *R0, double *R1, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3,
+    const int4 *uinf, double2 (*cs)[8][N + 1], int ncls, const uint16_t *gbase,
+    const uint16_t *gcount, const uint8_t *gT, const int2 *gtab, const uint8_t (*gmem)[8],
+    const uint8_t *gnm, const uint8_t *pairs, const uint16_t (*zinfo)[kPackT],
+    const int4 *ktab, const int4 *tco, const uint32_t *tbox, int lane,
+    const double *__restrict__ A1, const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero,
+    const double *__restrict__ mps, double *__restrict__ mpr) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);
+  constexpr int G = 2 * W;                    // unit groups of the accumulators
+  constexpr int NAQ = (NU + G - 1) / G;       // accumulator units per lane
+  constexpr bool PF = StageOps<N, W, WI>::fits;
+  const int col = lane & 15, ubase = (lane >> 4) + 4 * WI;
+  const int tj = lane & 31, ug = 2 * WI + (lane >> 5);
+
+  // the tile after (g, t): the next tile of class g, else the first tile
+  // of the next class with pairs (ncls when there is none)
+  auto next_tile = [&](int g, int t, int &gn, int &tn) {
+    if (g >= 0 && g < ncls && t + 1 < gT[g]) {
+      gn = g;
+      tn = t + 1;
+      return;
+    }
+    for (g++; g < ncls; g++)
+      if (gcount[g]) break;
+    gn = g;
+    tn = 0;
+  };
+  // this lane's column of tile (g, t): its source expansion and member
+  // index (zeros and member 0 for an empty column); the class's phases
+  // into cs[p] (every wavefront writes the same values)
+  auto source_of = [&](int g, int t, int p, int &icol) -> const double * {
+    const int nm = gnm[g];
+    for (int e = lane; e < nm * (N + 1); e += 64) {
+      const int i = e / (N + 1), m = e - i * (N + 1);
+      const double *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
+      cs[p][i][m] = make_double2(tt[0], tt[1]);
+    }
+    const int T = gT[g], pos = t + col * T;
+    icol = 0;
+    if (pos >= gcount[g]) return zero;
+    const int pr = pairs[gbase[g] + pos];
+    icol = pr >> 5;
+    const int4 kt = ktab[gmem[g][icol]];
+    const uint32_t sb = morton_add(tbox[pr & 31], (uint32_t)kt.x, (uint32_t)kt.y, (uint32_t)kt.z);
+    return mps + (uint64_t)sb * C;
+  };
+  double pre_re[NAW], pre_im[NAW];
+  auto load_sources = [&](const double *src, const int4 *tu, int ub) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const double *q = src + tu[u].x;
+      pre_re[i] = q[0];
+      pre_im[i] = q[1];
+    }
+  };
+  auto deposit = [&](double *R, const int4 *tu, int ub, int cl, int p, int ic) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      if (ub + 4 * W * i >= NU) continue;
+      const int4 inf = tu[ub + 4 * W * i];
+      const double2 e = cs[p][ic][inf.y];
+      R[inf.z * 16 + cl] = e.x * pre_re[i] - e.y * pre_im[i];
+      R[inf.w * 16 + cl] = e.y * pre_re[i] + e.x * pre_im[i];
+    }
+  };
+
+  double acc_re[NAQ], acc_im[NAQ];
+#pragma unroll
+  for (int i = 0; i < NAQ; i++) acc_re[i] = acc_im[i] = 0.0;
+
+  int g, t, p = 0, icol = 0, icol_n = 0;
+  next_tile(-1, 0, g, t);
+  g = __builtin_amdgcn_readfirstlane(g);
+  int th = 0, ri = 0;
+  if (g < ncls) {
+    const int2 tr = gtab[g];
+    th = __builtin_amdgcn_readfirstlane(tr.x);
+    ri = __builtin_amdgcn_readfirstlane(tr.y);
+    load_sources(source_of(g, t, 0, icol), uinf, ubase);
+    deposit(R0, uinf, ubase, col, 0, icol);
+  }
+  double av[StageOps<N, W, WI>::n];
+  if constexpr (PF) ops_load<double, N, W, WI>(av, A1 + (size_t)th * KT * 64, lane);
+  __syncthreads();
+  while (g < ncls) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
+    int gn, tn;
+    next_tile(g, t, gn, tn);
+    gn = __builtin_amdgcn_readfirstlane(gn);
+    tn = __builtin_amdgcn_readfirstlane(tn);
+    // the next tile's sources are requested before this tile's stages:
+    // the registers were emptied by the last deposit, and cs[p ^ 1] was
+    // last read before the barrier that ended the previous tile
+    int th_n = 0, ri_n = 0;
+    if (gn < ncls) {
+      const int2 tr = gtab[gn];
+      th_n = __builtin_amdgcn_readfirstlane(tr.x);
+      ri_n = __builtin_amdgcn_readfirstlane(tr.y);
+      load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub);
+    }
+    // R0 -> R1 -> R0 -> R1, one barrier per stage
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A2 + (size_t)ri * KT * 64, R0, R1, t1, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A1 + (size_t)th * KT * 64, R0, R1, t1, ln, gp, cl);
+    __syncthreads();
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A3 + (size_t)th * KT * 64, R1, R0, t2, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A2 + (size_t)ri * KT * 64, R1, R0, t2, ln, gp, cl);
+    __syncthreads();
+    if constexpr (PF)
+      stage_wop<double, N, W, WI>(av, A1 + (size_t)th_n * KT * 64, R0, R1, t3, ln, gp, cl);
+    else
+      stage_wo<double, N, W, WI>(A3 + (size_t)th * KT * 64, R0, R1, t3, ln, gp, cl);
+    __syncthreads();
+    // Zout: this lane's target's column of the tile, if it has one
+    {
+      const int T = gT[g], z = zinfo[g][tj];
+      const int r = z & 15, q = (z >> 4) & 15, cnt = z >> 8;
+      int d = t - r;
+      if (d < 0) d += T;
+      if (d < cnt) {
+        const int cc = q + (r + d >= T ? 1 : 0);
+        const int ic = pairs[gbase[g] + t + cc * T] >> 5;
+#pragma unroll
+        for (int i = 0; i < NAQ; i++) {
+          const int u = ug + G * i;
+          if (u >= NU) continue;
+          const int m = tu[u].y;
+          const double2 e = cs[p][ic][m];
+          const double tr = R1[u * 16 + cc], tq = R1[(NU + u) * 16 + cc];
+          acc_re[i] += e.x * tr + e.y * tq;
+          acc_im[i] += e.x * tq - e.y * tr;
+        }
+      }
+    }
+    if (gn < ncls) deposit(R0, tu, ub, cl, p ^ 1, icol_n);
+    __syncthreads();
+    p ^= 1;
+    g = gn;
+    t = tn;
+    th = th_n;
+    ri = ri_n;
+    icol = icol_n;
+  }
+  (void)icol;
+  // each lane adds its target's units (m = 0 has no imaginary part)
+  if (tco[tj].w) {
+    double *out = mpr + (size_t)tbox[tj] * C;
+#pragma unroll
+    for (int i = 0; i < NAQ; i++) {
+      const int u = ug + G * i;
+      if (u >= NU) continue;
+      const int4 inf = uinf[u];
+      out[inf.x] += acc_re[i];
+      if (inf.y) out[inf.x + 1] += acc_im[i];
+    }
+  }
+}
+
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 8 ? 4 : 2)))
+void k_lap_m2l_packed_cls(
+    const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ,
+    int level, const double *__restrict__ mps, double *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const double *__restrict__ A1,
+    const double *__restrict__ A3, const double *__restrict__ A2,
+    const double *__restrict__ CS, const double *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  __shared__ double R0[(S + 1) * 16], R1[(S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ double2 cs[2][8][N + 1];
+  __shared__ unsigned vm[kPackT][7];
+  __shared__ uint8_t pairs[kPackT * 189];
+  __shared__ uint16_t zinfo[kMaxOffCls][kPackT];  // r | q << 4 | cnt << 8
+  __shared__ uint8_t ccnt[kMaxOffCls][kPackT];
+  __shared__ uint16_t gbase[kMaxOffCls], gcount[kMaxOffCls];
+  __shared__ uint8_t gT[kMaxOffCls], gnm[kMaxOffCls], gmem[kMaxOffCls][8];
+  __shared__ int2 gtab[kMaxOffCls];
+  __shared__ int4 ktab[216];  // dilated offsets (x, y, z) and offset index
+  __shared__ int4 tco[kPackT];
+  __shared__ uint32_t tbox[kPackT];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<double>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * n + (m ? 2 * m - 1 : 0), m, P->s1[u] & 0xffff,
+                        (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (S + 1) * 16; i += 64 * W) R0[i] = R1[i] = 0.0;
+  for (int i = tid; i < kPackT * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
+
+  int c;
+  unsigned cj;
+  // a block with no chunk leaves before any barrier (uniform over the block)
+  if (!block_chunk(blockIdx.x, ca, c, cj)) return;
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  const int ncls = kOffCls[c].n;
+  for (int k = tid; k < 216; k += 64 * W) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int off = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    ktab[k] = make_int4((int)dilate_offset(dx, 0, kMortonX), (int)dilate_offset(dy, 1, kMortonY),
+                        (int)dilate_offset(dz, 2, kMortonZ), off);
+  }
+  for (int g = tid; g < ncls; g += 64 * W) {
+    const int nm = kOffCls[c].nm[g];
+    gnm[g] = (uint8_t)nm;
+    for (int i = 0; i < 8; i++) gmem[g][i] = kOffCls[c].k[g][i];
+    // the members share the polar angle and distance: the first one's
+    const int k0 = kOffCls[c].k[g][0];
+    const int dx = k0 % 6 - 2 - px, dy = (k0 / 6) % 6 - 2 - py, dz = k0 / 36 - 2 - pz;
+    const int off = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    gtab[g] = make_int2(P->th[off], P->rix[off]);
+  }
+  if (tid < kPackT) {
+    const unsigned kk = cj * kPackT + tid;
+    const bool tv = kk < ca.count[c];
+    const uint32_t tb = tv ? cls[ca.start[c] + kk] : 0u;
+    tbox[tid] = tb;
+    tco[tid] = make_int4(compact3(tb), compact3(tb >> 1), compact3(tb >> 2), tv ? 1 : 0);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kPackT * 216; idx += 64 * W) {
+    const int j = idx % kPackT, k = idx / kPackT;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int4 tc = tco[j];
+    const int si = tc.x + dx, sj = tc.y + dy, sk = tc.z + dz;
+    if (!tc.w || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox) continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[j][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  // pairs per (class, target)
+  for (int idx = tid; idx < ncls * kPackT; idx += 64 * W) {
+    const int g = idx / kPackT, j = idx % kPackT;
+    int n = 0;
+    for (int i = 0; i < gnm[g]; i++) {
+      const int k = gmem[g][i];
+      n += (vm[j][k >> 5] >> (k & 31)) & 1u;
+    }
+    ccnt[g][j] = (uint8_t)n;
+  }
+  __syncthreads();
+  // per class: pair count, tiles, and each target's (r, q, cnt)
+  for (int g = tid; g < ncls; g += 64 * W) {
+    int n = 0, mx = 0;
+    for (int j = 0; j < kPackT; j++) {
+      n += ccnt[g][j];
+      mx = max(mx, (int)ccnt[g][j]);
+    }
+    const int T = n ? max((n + 15) / 16, mx) : 0;
+    gcount[g] = (uint16_t)n;
+    gT[g] = (uint8_t)T;
+    int s = 0;
+    for (int j = 0; j < kPackT; j++) {
+      const int cn = ccnt[g][j];
+      // (a target without pairs may start at s = n = 16 T: stored as 0)
+      zinfo[g][j] = (uint16_t)(cn ? ((s % T) | ((s / T) << 4) | (cn << 8)) : 0);
+      s += cn;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int b = 0;
+    for (int g = 0; g < ncls; g++) {
+      gbase[g] = (uint16_t)b;
+      b += gcount[g];
+    }
+  }
+  __syncthreads();
+  // pairs (target | member << 5), target-major within each class
+  for (int idx = tid; idx < ncls * kPackT; idx += 64 * W) {
+    const int g = idx / kPackT, j = idx % kPackT;
+    if (!ccnt[g][j]) continue;
+    int s = gbase[g];
+    for (int jj = 0; jj < j; jj++) s += ccnt[g][jj];
+    for (int i = 0; i < gnm[g]; i++) {
+      const int k = gmem[g][i];
+      if ((vm[j][k >> 5] >> (k & 31)) & 1u) pairs[s++] = (uint8_t)(j | (i << 5));
+    }
+  }
+  __syncthreads();
+
+#define PACKED_CLS(WI)                                                                        \
+  m2l_packed_cls_loop<N, W, WI>(R0, R1, tmap1, tmap2, tmap3, uinf, cs, ncls, gbase, gcount,   \
+                                gT, gtab, gmem, gnm, pairs, zinfo, ktab, tco, tbox, lane, A1, \
+                                A3, A2, CS, zero, mps, mpr)
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: PACKED_CLS(0); break;
+      case 1: PACKED_CLS(1); break;
+      case 2: PACKED_CLS(2); break;
+      default: PACKED_CLS(3); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: PACKED_CLS(0); break;
+      case 1: PACKED_CLS(1); break;
+      case 2: PACKED_CLS(2); break;
+      case 3: PACKED_CLS(3); break;
+      case 4: PACKED_CLS(4); break;
+      case 5: PACKED_CLS(5); break;
+      case 6: PACKED_CLS(6); break;
+      default: PACKED_CLS(7); break;
+    }
+  }
+#undef PACKED_CLS
+}
+
 // the packed kernel for levels where fewer than 60 % of the boxes hold
 // targets and there are at least 1024 chunks; WBFMM_M2L_PACKED=0 or 1
 // forces the choice at every level and size (read at each call, for
@@ -1685,6 +2081,25 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
       cp.chunk_prefix[8] = nch;
       const char *pw = getenv("WBFMM_PACKED_WAVES");
       const char *pd = getenv("WBFMM_PACKED_DB");
+      const char *pcl = getenv("WBFMM_PACKED_CLS");
+      // offsets packed by class (WBFMM_PACKED_CLS=0: one offset per tile)
+      if constexpr (PackedClsFits<N>::value) {
+        if (!(pcl && atoi(pcl) == 0) && !(pd && atoi(pd) == 0)) {
+          if (!(pw && atoi(pw) == 4))
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, 8>), dim3(nch), dim3(512), 0, cls,
+                         cp, occ, level, (const double *)mps, (double *)mpr,
+                         (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
+                         (const double *)A3, (const double *)A2, (const double *)CS,
+                         (const double *)zero);
+          else
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, 4>), dim3(nch), dim3(256), 0, cls,
+                         cp, occ, level, (const double *)mps, (double *)mpr,
+                         (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
+                         (const double *)A3, (const double *)A2, (const double *)CS,
+                         (const double *)zero);
+          return 0;
+        }
+      }
       if constexpr (PackedDbFits<N>::value) {
         if (!(pd && atoi(pd) == 0)) {
           // class order; WBFMM_M2L_XCD=1 interleaves the 8 classes on each
