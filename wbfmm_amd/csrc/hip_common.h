@@ -71,18 +71,11 @@ __device__ __forceinline__ uint32_t compact3(uint64_t x) {
 constexpr uint32_t kMortonX = 0x49249249u, kMortonY = 0x92492492u, kMortonZ = 0x24924924u;
 constexpr int kMortonAddMaxLevel = 10;
 
-__device__ __forceinline__ uint32_t dilate10(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-// offset d (|d| < 1024) on the axis of bit mask m, shift s (0, 1, 2)
+// offset d (|d| < 8: the list-4 offsets are within 3) on the axis of bit
+// mask m, shift s (0, 1, 2)
 __device__ __forceinline__ uint32_t dilate_offset(int d, int s, uint32_t m) {
-  const uint32_t a = dilate10((uint32_t)(d < 0 ? -d : d)) << s;
+  const uint32_t u = (uint32_t)(d < 0 ? -d : d);
+  const uint32_t a = ((u & 1u) | ((u & 2u) << 2) | ((u & 4u) << 4)) << s;
   return d < 0 ? (0u - a) & m : a;
 }
 

@@ -313,10 +313,28 @@ __device__ __forceinline__ void m2l_w2_loop(
   constexpr bool PREFETCH = true;
   const int ubase = grp + 4 * WI;
 
+  // the used offsets of this block's split as scalar bit words; the next
+  // one after k by a scan of the words (216 when there is none)
+  unsigned am[7];
+#pragma unroll
+  for (int w = 0; w < 7; w++) {
+    unsigned v = __builtin_amdgcn_readfirstlane(anym[w]);
+    if (nsplit > 1)
+      for (int b = 0; b < 32; b++)
+        if ((32 * w + b) % nsplit != split) v &= ~(1u << b);
+    am[w] = v;
+  }
   auto next_k = [&](int k) {
-    for (k++; k < 216; k++)
-      if (((anym[k >> 5] >> (k & 31)) & 1u) && k % nsplit == split) break;
-    return k;
+    int kn = k + 1;
+#pragma unroll
+    for (int w = 0; w < 7; w++) {
+      if (kn < 32 * (w + 1)) {
+        const unsigned bits = am[w] & (kn > 32 * w ? ~0u << (kn - 32 * w) : ~0u);
+        if (bits) return 32 * w + (int)__builtin_ctz(bits);
+        kn = 32 * (w + 1);
+      }
+    }
+    return 216;
   };
   auto offset_of = [&](int k) {
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
