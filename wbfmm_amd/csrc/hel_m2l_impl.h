@@ -312,8 +312,17 @@ __device__ __forceinline__ void hel_m2l_loop(
   }
 }
 
+// FP32 from order 16 (eight wavefronts): held to 128 registers so that two
+// blocks share a CU (the LDS buffer leaves room for two; at 159 registers
+// only one fits)
 template <typename T, int N, int W>
-__global__ __launch_bounds__(64 * W) void k_hel_m2l_mfma(
+constexpr int hel_m2l_waves_per_eu() {
+  return sizeof(T) == 4 && W == 8 && N >= 16 ? 4 : 1;
+}
+
+template <typename T, int N, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
+    hel_m2l_waves_per_eu<T, N, W>()))) void k_hel_m2l_mfma(
     const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ, int level,
     const T2t<T> *__restrict__ mps, T2t<T> *__restrict__ mpr,
     const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A,
