@@ -16,7 +16,9 @@ levels stay FP64.  Checked here:
   path's truncation error is well below it;
 - the coefficients of FP32 levels, downloaded in the reference's scaling,
   against the FP64 tree's;
-- the one-wavefront-per-box FP64 field kernel against the thread kernel.
+- the one-wavefront-per-box FP64 field kernel against the thread kernel;
+- the class-packed FP32 M2L (forced at every level) against the chunked
+  kernel, the FP64 solve and the direct sum.
 """
 
 import ctypes
@@ -129,6 +131,49 @@ def test_mixed_sphere_boundary_levels(wb, oracle):
         e = np.linalg.norm(fc - fd[:, 0]) / np.linalg.norm(fd[:, 0])
         print(f"sphere {name}: potential rel L2 error {e:.2e}", flush=True)
         assert e < 1e-6
+
+
+@pytest.mark.parametrize("p,k", [(10, 3.0), (23, 1.5)])
+def test_packed_fp32_m2l_matches_chunked(wb, monkeypatch, p, k):
+    # the class-packed FP32 M2L (forced at every level) against the chunked
+    # kernel and the FP64 solve on the same tree: FP32 rounding, summed in
+    # another order
+    xs, q, xt = cube(730 + p, 4000, 1000)
+    x0, D = fit_box(xs, xt)
+    monkeypatch.setenv("WBFMM_HEL_FP64_KA", "100")
+    monkeypatch.setenv("WBFMM_HEL_PACKED", "0")
+    fc = solve(wb, xs, q, None, xt, x0, D, 4, [p] * 17, k, 32)
+    monkeypatch.setenv("WBFMM_HEL_PACKED", "1")
+    fp = solve(wb, xs, q, None, xt, x0, D, 4, [p] * 17, k, 32)
+    monkeypatch.delenv("WBFMM_HEL_PACKED")
+    monkeypatch.delenv("WBFMM_HEL_FP64_KA")
+    f64 = solve(wb, xs, q, None, xt, x0, D, 4, [p] * 17, k, 64)
+    r, r64 = colrel(fp, fc), colrel(fp, f64)
+    print(f"p={p}: packed vs chunked {r.max():.2e}, packed vs FP64 {r64.max():.2e}", flush=True)
+    assert r.max() < TOL and r64.max() < TOL
+    assert not np.array_equal(fp, fc)  # the packed kernel ran
+
+
+def test_packed_fp32_m2l_sphere_with_dipoles(wb, oracle, monkeypatch):
+    # the mixed sphere of the boundary test with the packed FP32 M2L forced
+    # at its FP32 levels: against the FP64 solve and the direct sum
+    n, k, depth = 20_000, 16.0, 5
+    xs, q, d = sphere(721, n)
+    x0, D = fit_box(xs)
+    orders = trunc_orders(wb, D, k, 1e-6, depth)
+    monkeypatch.setenv("WBFMM_HEL_PACKED", "1")
+    fm = solve(wb, xs, q, d, xs, x0, D, depth, orders, k, 32)
+    monkeypatch.delenv("WBFMM_HEL_PACKED")
+    f64 = solve(wb, xs, q, d, xs, x0, D, depth, orders, k, 64)
+    r = colrel(fm, f64)
+    print(f"sphere packed mixed vs FP64: max rel diff per component {r.max():.2e}", flush=True)
+    assert r.max() < TOL, r
+    idx = np.arange(0, n, 97)
+    fd = oracle.hel_direct(k, xs, q, xs[idx], field=1, nrm=xs, d=d, cutoff=1e-6)
+    fc = fm[idx, 0] + 1j * fm[idx, 1]
+    e = np.linalg.norm(fc - fd[:, 0]) / np.linalg.norm(fd[:, 0])
+    print(f"sphere packed mixed: potential rel L2 error {e:.2e}", flush=True)
+    assert e < 1e-6
 
 
 def test_mixed_target_list_and_dipoles(wb):

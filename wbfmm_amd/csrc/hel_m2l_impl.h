@@ -9,6 +9,7 @@
 #define WBFMM_HEL_M2L_IMPL_H
 
 #include <cstdio>
+#include <cstdlib>
 
 #include <hip/hip_runtime.h>
 
@@ -426,6 +427,435 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
 #undef HEL_LOOP
 }
 
+// ---------------------------------------------------------------------
+// Sparse levels with the offsets packed by class (FP32 levels).  A block
+// takes 16 targets of one Morton class.  For each offset class (offsets
+// sharing dz and dx^2 + dy^2, hence all three stage tables) it lists the
+// (target, member) pairs with an occupied source in target-major order and
+// deals them round-robin over T = max(ceil(n / 8), most pairs of one
+// target) tiles of 8 pairs: position p goes to tile p mod T, column pair
+// p div T (columns 2q, 2q + 1 are the pair's real and imaginary lanes, as
+// in the chunked kernel).  A target's pairs are consecutive positions, at
+// most T of them, so a tile holds each target at most once.  The chunked
+// kernel runs the union of its 8 targets' offsets; on the 1e6-point sphere
+// half of that work has no source.  The local expansions stay in
+// registers on owner lanes: lane L = 64 w + lane holds L^{+nu} (sign 0) or
+// L^{-nu} (sign 1) of target L & 15, sign (L >> 4) & 1, for the units
+// (L >> 5) + 2W i; after each tile's third stage it reads its target's
+// column pair, found from (r, q, cnt) = (s mod T, s div T, count), s the
+// target's first position.  Each target sums its translations in tile
+// order.
+// ---------------------------------------------------------------------
+constexpr int kHelPackT = 16;
+
+template <typename T, int N>
+struct HelPackedFits {
+  static constexpr bool value = sizeof(T) == 4 && N >= 8 && N <= 27;
+};
+
+template <typename T, int N, int W, int WI>
+__device__ __forceinline__ void hel_packed_loop(
+    T *R, const int4 *tmap1, const int4 *tmap2, const int4 *tmap3, const int4 *uinf,
+    T2t<T> (*cs)[8][N + 1], int ncls, const uint16_t *gbase, const uint16_t *gcount,
+    const uint8_t *gT, const int2 *gtab, const uint8_t (*gmem)[8], const uint8_t *gnm,
+    const uint8_t *pairs, const uint16_t (*zinfo)[kHelPackT], const int4 *ktab,
+    const int4 *tco, const uint32_t *tbox, int lane, int col, int grp,
+    const T *__restrict__ A, const T *__restrict__ A2, const T *__restrict__ CS,
+    const T2t<T> *__restrict__ zero, const T2t<T> *__restrict__ mps,
+    T2t<T> *__restrict__ mpr) {
+  using SH = M2LShape<N>;
+  constexpr int C = SH::C, NU = SH::NU, KT = SH::KT;
+  constexpr int NAW = (NU + 4 * W - 1) / (4 * W);  // units per lane (deposit)
+  constexpr WaveSeq WS = make_seq(SH::BN, W, WI);
+  constexpr int NBW = WS.nb;
+  constexpr bool DB = M2LDouble<T, N>::value;
+  constexpr int G = 2 * W;                       // unit groups of the owners
+  constexpr int NAQ = (NU + G - 1) / G;          // owner units per lane
+  T *R1 = R + (SH::S + 1) * 16;  // the second buffer (DB only)
+  const int ubase = grp + 4 * WI;
+  const int LL = 64 * WI + lane, ot = LL & 15, osg = (LL >> 4) & 1, og = LL >> 5;
+
+  auto next_tile = [&](int g, int t, int &gn, int &tn) {
+    if (g >= 0 && g < ncls && t + 1 < gT[g]) {
+      gn = g;
+      tn = t + 1;
+      return;
+    }
+    for (g++; g < ncls; g++)
+      if (gcount[g]) break;
+    gn = g;
+    tn = 0;
+  };
+  // this lane's column pair of tile (g, t): its source expansion and
+  // member index (zeros and member 0 for an empty pair); the class's
+  // phases into cs[p] (every wavefront writes the same values)
+  auto source_of = [&](int g, int t, int p, int &icol) -> const T2t<T> * {
+    const int nm = gnm[g];
+    for (int e = lane; e < nm * (N + 1); e += 64) {
+      const int i = e / (N + 1), m = e - i * (N + 1);
+      const T *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
+      cs[p][i][m] = mk2<T>(tt[0], tt[1]);
+    }
+    const int Tt = gT[g], pos = t + (col >> 1) * Tt;
+    icol = 0;
+    if (pos >= gcount[g]) return zero;
+    const int pr = pairs[gbase[g] + pos];
+    icol = pr >> 4;
+    const int4 kt = ktab[gmem[g][icol]];
+    const uint32_t sb = morton_add(tbox[pr & 15], (uint32_t)kt.x, (uint32_t)kt.y, (uint32_t)kt.z);
+    return mps + (uint64_t)sb * C;
+  };
+  T pre_re[NAW], pre_im[NAW];
+  // x_{+m} on the real lane, x_{-m} on the imaginary one
+  auto load_sources = [&](const T2t<T> *src, const int4 *tu, int ub, int sp) {
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const T2t<T> q = src[inf.x + (sp ? -inf.y : inf.y)];
+      pre_re[i] = q.x;
+      pre_im[i] = q.y;
+    }
+  };
+  // x'_{+-m} = e^{+-i m chi} x_{+-m}; the partner lane holds the other sign
+  auto deposit = [&](const int4 *tu, int ub, int cl, int p, int ic) {
+    const int sp = cl & 1;
+#pragma unroll
+    for (int i = 0; i < NAW; i++) {
+      const int u = min(ub + 4 * W * i, NU - 1);
+      const int4 inf = tu[u];
+      const T2t<T> e = cs[p][ic][inf.y];
+      const T es = sp ? -e.y : e.y;
+      const T xr = e.x * pre_re[i] - es * pre_im[i];
+      const T xi = es * pre_re[i] + e.x * pre_im[i];
+      const T yr = __shfl_xor(xr, 1), yi = __shfl_xor(xi, 1);
+      if (ub + 4 * W * i >= NU) continue;
+      R[inf.z * 16 + cl] = sp ? yi + xi : xr + yr;
+      R[inf.w * 16 + cl] = sp ? yi - xi : xr - yr;
+    }
+  };
+  auto write_out = [&](const int4 *tm, const V4t<T> *y, int gp, int cl) {
+#pragma unroll
+    for (int lb = 0; lb < NBW; lb++) {
+      const int4 d = tm[4 * (WI + W * lb) + gp];
+      R[d.x * 16 + cl] = y[lb][0];
+      R[d.y * 16 + cl] = y[lb][1];
+      R[d.z * 16 + cl] = y[lb][2];
+      R[d.w * 16 + cl] = y[lb][3];
+    }
+  };
+
+  T acc_re[NAQ], acc_im[NAQ];
+#pragma unroll
+  for (int i = 0; i < NAQ; i++) acc_re[i] = acc_im[i] = T(0);
+
+  int g, t, p = 0, icol = 0, icol_n = 0;
+  next_tile(-1, 0, g, t);
+  g = __builtin_amdgcn_readfirstlane(g);
+  int th = 0, ri = 0;
+  if (g < ncls) {
+    const int2 tr = gtab[g];
+    th = __builtin_amdgcn_readfirstlane(tr.x);
+    ri = __builtin_amdgcn_readfirstlane(tr.y);
+    load_sources(source_of(g, t, 0, icol), uinf, ubase, col & 1);
+    deposit(uinf, ubase, col, 0, icol);
+  }
+  __syncthreads();
+  while (g < ncls) {
+    int z0;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
+    const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
+    const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI, sp = cl & 1;
+    int gn, tn;
+    next_tile(g, t, gn, tn);
+    gn = __builtin_amdgcn_readfirstlane(gn);
+    tn = __builtin_amdgcn_readfirstlane(tn);
+    int th_n = 0, ri_n = 0;
+    if (gn < ncls) {
+      const int2 tr = gtab[gn];
+      th_n = __builtin_amdgcn_readfirstlane(tr.x);
+      ri_n = __builtin_amdgcn_readfirstlane(tr.y);
+    }
+    // the next tile's sources are requested after the first stage, as in
+    // the chunked kernel (their registers are not held across all three)
+    T *RZ = R;
+    if constexpr (DB) {
+      stage_wo<T, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t1, ln, gp, cl);
+      __syncthreads();
+      if (gn < ncls) load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub, sp);
+      stage_cx_wo<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+                               R1, R, t2, ln, gp, cl);
+      __syncthreads();
+      stage_wo<T, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t3, ln, gp, cl);
+      __syncthreads();
+      RZ = R1;
+    } else {
+      V4t<T> y[NBW > 0 ? NBW : 1];
+      stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t1, y, gp, cl);
+      __syncthreads();
+      if (gn < ncls) load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub, sp);
+      stage_cx<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
+                            R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t2, y, gp, cl);
+      __syncthreads();
+      stage_w<T, N, W, WI>(A + (size_t)th * KT * 64, R, ln, gp, cl, y);
+      __syncthreads();
+      write_out(t3, y, gp, cl);
+      __syncthreads();
+    }
+    // owners: y_{+-nu} = (P +- Q)/2 of the target's column pair, then
+    // L^{+-nu} += e^{-+i nu chi} y
+    {
+      const int Tt = gT[g], z = zinfo[g][ot];
+      const int r = z & 15, q = (z >> 4) & 15, cnt = z >> 8;
+      int d = t - r;
+      if (d < 0) d += Tt;
+      if (d < cnt) {
+        const int cc = q + (r + d >= Tt ? 1 : 0);
+        const int ic = pairs[gbase[g] + t + cc * Tt] >> 4;
+        const int c0 = 2 * cc;
+#pragma unroll
+        for (int i = 0; i < NAQ; i++) {
+          const int u = og + G * i;
+          if (u >= NU) continue;
+          const int nu = tu[u].y;
+          const T2t<T> e = cs[p][ic][nu];
+          const T es = osg ? -e.y : e.y;
+          const T pr = RZ[u * 16 + c0], pi = RZ[u * 16 + c0 + 1];
+          const T qr = nu ? RZ[(NU + u) * 16 + c0] : T(0);
+          const T qi = nu ? RZ[(NU + u) * 16 + c0 + 1] : T(0);
+          const T yr = T(0.5) * (osg ? pr - qr : pr + qr);
+          const T yi = T(0.5) * (osg ? pi - qi : pi + qi);
+          acc_re[i] += e.x * yr + es * yi;
+          acc_im[i] += e.x * yi - es * yr;
+        }
+      }
+    }
+    if constexpr (!DB) __syncthreads();  // every read of R done before the next deposit
+    if (gn < ncls) deposit(tu, ub, cl, p ^ 1, icol_n);
+    __syncthreads();
+    p ^= 1;
+    g = gn;
+    t = tn;
+    th = th_n;
+    ri = ri_n;
+    icol = icol_n;
+  }
+  (void)icol;
+  if (!tco[ot].w) return;
+  T2t<T> *dst = mpr + (size_t)tbox[ot] * C;
+#pragma unroll
+  for (int i = 0; i < NAQ; i++) {
+    const int u = og + G * i;
+    if (u >= NU) continue;
+    const int4 inf = uinf[u];
+    if (osg && inf.y == 0) continue;  // L^0 is the sign-0 owner's
+    const int r = inf.x + (osg ? -inf.y : inf.y);
+    const T2t<T> o = dst[r];
+    dst[r] = mk2<T>(o.x + acc_re[i], o.y + acc_im[i]);
+  }
+}
+
+// sixteen wavefronts from order 16 (one block per CU, four wavefronts per
+// SIMD at 128 registers); eight below, two blocks per CU
+template <typename T, int N, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(
+    W == 16 ? 4 : hel_m2l_waves_per_eu<T, N, W>()))) void k_hel_m2l_packed(
+    const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ, int level,
+    const T2t<T> *__restrict__ mps, T2t<T> *__restrict__ mpr,
+    const wbfmm_m2l_plan_t *__restrict__ P, const T *__restrict__ A,
+    const T *__restrict__ A2, const T *__restrict__ CS, const T2t<T> *__restrict__ zero) {
+  using SH = M2LShape<N>;
+  constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
+  static_assert(W == 4 || W == 8 || W == 16, "four, eight or sixteen wavefronts per block");
+  __shared__ T R[(M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16];
+  __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
+  __shared__ int4 uinf[NU];
+  __shared__ T2t<T> cs[2][8][N + 1];
+  __shared__ unsigned vm[kHelPackT][7];
+  __shared__ uint8_t pairs[kHelPackT * 189];
+  __shared__ uint16_t zinfo[kMaxOffCls][kHelPackT];  // r | q << 4 | cnt << 8
+  __shared__ uint8_t ccnt[kMaxOffCls][kHelPackT];
+  __shared__ uint16_t gbase[kMaxOffCls], gcount[kMaxOffCls];
+  __shared__ uint8_t gT[kMaxOffCls], gnm[kMaxOffCls], gmem[kMaxOffCls][8];
+  __shared__ int2 gtab[kMaxOffCls];
+  __shared__ int4 ktab[216];  // dilated offsets (x, y, z) and offset index
+  __shared__ int4 tco[kHelPackT];
+  __shared__ uint32_t tbox[kHelPackT];
+
+  const int tid = threadIdx.x, lane = tid & 63, col = lane & 15, grp = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int i = tid; i < 4 * NB; i += 64 * W) {
+    const int b = i >> 2, g = i & 3;
+    int t1[4], t2[4], t3[4];
+    for (int q = 0; q < 4; q++) {
+      const int row = 16 * b + MF<T>::drow(g, q);
+      t1[q] = P->map1[row] >> 6;
+      t2[q] = P->map2[row] >> 6;
+      t3[q] = P->map3[row] >> 6;
+    }
+    tmap1[i] = make_int4(t1[0], t1[1], t1[2], t1[3]);
+    tmap2[i] = make_int4(t2[0], t2[1], t2[2], t2[3]);
+    tmap3[i] = make_int4(t3[0], t3[1], t3[2], t3[3]);
+  }
+  // unit u = (n, m): index of x_n^0, m, the slots of P_m and Q_m
+  for (int u = tid; u < NU; u += 64 * W) {
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= u) n++;
+    const int m = u - n * (n + 1) / 2;
+    uinf[u] = make_int4(n * (n + 1), m, P->s1[u] & 0xffff, (P->s1[u] >> 16) & 0xffff);
+  }
+  for (int i = tid; i < (M2LDouble<T, N>::value ? 2 : 1) * (S + 1) * 16; i += 64 * W)
+    R[i] = T(0);
+  for (int i = tid; i < kHelPackT * 7; i += 64 * W) vm[i / 7][i % 7] = 0u;
+
+  int c;
+  unsigned cj;
+  // a block with no chunk leaves before any barrier (uniform over the block)
+  if (!block_chunk(blockIdx.x, ca, c, cj)) return;
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  const int nbox = 1 << level;
+  const int ncls = kOffCls[c].n;
+  for (int k = tid; k < 216; k += 64 * W) {
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    const int off = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    ktab[k] = make_int4((int)dilate_offset(dx, 0, kMortonX), (int)dilate_offset(dy, 1, kMortonY),
+                        (int)dilate_offset(dz, 2, kMortonZ), off);
+  }
+  for (int g = tid; g < ncls; g += 64 * W) {
+    gnm[g] = kOffCls[c].nm[g];
+    for (int i = 0; i < 8; i++) gmem[g][i] = kOffCls[c].k[g][i];
+    // the members share the polar angle and distance: the first one's
+    const int k0 = kOffCls[c].k[g][0];
+    const int dx = k0 % 6 - 2 - px, dy = (k0 / 6) % 6 - 2 - py, dz = k0 / 36 - 2 - pz;
+    const int off = (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
+    gtab[g] = make_int2(P->th[off], P->rix[off]);
+  }
+  if (tid < kHelPackT) {
+    const unsigned kk = cj * kHelPackT + tid;
+    const bool tv = kk < ca.count[c];
+    const uint32_t tb = tv ? cls[ca.start[c] + kk] : 0u;
+    tbox[tid] = tb;
+    tco[tid] = make_int4(compact3(tb), compact3(tb >> 1), compact3(tb >> 2), tv ? 1 : 0);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kHelPackT * 216; idx += 64 * W) {
+    const int j = idx % kHelPackT, k = idx / kHelPackT;
+    const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+    if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) continue;
+    const int4 tc = tco[j];
+    const int si = tc.x + dx, sj = tc.y + dy, sk = tc.z + dz;
+    if (!tc.w || si < 0 || si >= nbox || sj < 0 || sj >= nbox || sk < 0 || sk >= nbox) continue;
+    const uint64_t sb = spread3(si) | (spread3(sj) << 1) | (spread3(sk) << 2);
+    if ((occ[sb >> 5] >> (sb & 31)) & 1u) atomicOr(&vm[j][k >> 5], 1u << (k & 31));
+  }
+  __syncthreads();
+  for (int idx = tid; idx < ncls * kHelPackT; idx += 64 * W) {
+    const int g = idx / kHelPackT, j = idx % kHelPackT;
+    int n = 0;
+    for (int i = 0; i < gnm[g]; i++) {
+      const int k = gmem[g][i];
+      n += (vm[j][k >> 5] >> (k & 31)) & 1u;
+    }
+    ccnt[g][j] = (uint8_t)n;
+  }
+  __syncthreads();
+  // per class: pair count, tiles of 8 pairs, each target's (r, q, cnt)
+  for (int g = tid; g < ncls; g += 64 * W) {
+    int n = 0, mx = 0;
+    for (int j = 0; j < kHelPackT; j++) {
+      n += ccnt[g][j];
+      mx = max(mx, (int)ccnt[g][j]);
+    }
+    const int Tt = n ? max((n + 7) / 8, mx) : 0;
+    gcount[g] = (uint16_t)n;
+    gT[g] = (uint8_t)Tt;
+    int s = 0;
+    for (int j = 0; j < kHelPackT; j++) {
+      const int cn = ccnt[g][j];
+      // (a target without pairs may start at s = n = 8 T: stored as 0)
+      zinfo[g][j] = (uint16_t)(cn ? ((s % Tt) | ((s / Tt) << 4) | (cn << 8)) : 0);
+      s += cn;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int b = 0;
+    for (int g = 0; g < ncls; g++) {
+      gbase[g] = (uint16_t)b;
+      b += gcount[g];
+    }
+  }
+  __syncthreads();
+  // pairs (target | member << 4), target-major within each class
+  for (int idx = tid; idx < ncls * kHelPackT; idx += 64 * W) {
+    const int g = idx / kHelPackT, j = idx % kHelPackT;
+    if (!ccnt[g][j]) continue;
+    int s = gbase[g];
+    for (int jj = 0; jj < j; jj++) s += ccnt[g][jj];
+    for (int i = 0; i < gnm[g]; i++) {
+      const int k = gmem[g][i];
+      if ((vm[j][k >> 5] >> (k & 31)) & 1u) pairs[s++] = (uint8_t)(j | (i << 4));
+    }
+  }
+  __syncthreads();
+
+#define HEL_PACKED(WI)                                                                       \
+  hel_packed_loop<T, N, W, WI>(R, tmap1, tmap2, tmap3, uinf, cs, ncls, gbase, gcount, gT,    \
+                               gtab, gmem, gnm, pairs, zinfo, ktab, tco, tbox, lane, col, grp, \
+                               A, A2, CS, zero, mps, mpr)
+  if constexpr (W == 4) {
+    switch (wv) {
+      case 0: HEL_PACKED(0); break;
+      case 1: HEL_PACKED(1); break;
+      case 2: HEL_PACKED(2); break;
+      default: HEL_PACKED(3); break;
+    }
+  } else if constexpr (W == 8) {
+    switch (wv) {
+      case 0: HEL_PACKED(0); break;
+      case 1: HEL_PACKED(1); break;
+      case 2: HEL_PACKED(2); break;
+      case 3: HEL_PACKED(3); break;
+      case 4: HEL_PACKED(4); break;
+      case 5: HEL_PACKED(5); break;
+      case 6: HEL_PACKED(6); break;
+      default: HEL_PACKED(7); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: HEL_PACKED(0); break;
+      case 1: HEL_PACKED(1); break;
+      case 2: HEL_PACKED(2); break;
+      case 3: HEL_PACKED(3); break;
+      case 4: HEL_PACKED(4); break;
+      case 5: HEL_PACKED(5); break;
+      case 6: HEL_PACKED(6); break;
+      case 7: HEL_PACKED(7); break;
+      case 8: HEL_PACKED(8); break;
+      case 9: HEL_PACKED(9); break;
+      case 10: HEL_PACKED(10); break;
+      case 11: HEL_PACKED(11); break;
+      case 12: HEL_PACKED(12); break;
+      case 13: HEL_PACKED(13); break;
+      case 14: HEL_PACKED(14); break;
+      default: HEL_PACKED(15); break;
+    }
+  }
+#undef HEL_PACKED
+}
+
+// the packed kernel (FP32 levels) where fewer than 60 % of the level's
+// boxes hold targets and there are at least 512 chunks of 16 targets;
+// WBFMM_HEL_PACKED=0 or 1 forces the choice at every level and size
+static int hel_packed_choice() {
+  const char *e = getenv("WBFMM_HEL_PACKED");
+  return e ? atoi(e) : -1;
+}
+
 template <typename T, int N>
 static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks,
                           const uint32_t *occ, int level, const T *mps, T *mpr,
@@ -442,6 +872,25 @@ static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nch
   constexpr int C = (N + 1) * (N + 1);
   unsigned ntot = 0;
   for (int c = 0; c < 8; c++) ntot += ca.count[c];
+  if constexpr (HelPackedFits<T, N>::value) {
+    ClassArgs cp = ca;
+    unsigned nch = 0;
+    for (int c = 0; c < 8; c++) {
+      cp.chunk_prefix[c] = nch;
+      nch += (ca.count[c] + kHelPackT - 1) / kHelPackT;
+    }
+    cp.chunk_prefix[8] = nch;
+    cp.qc = 0;
+    const int pc = hel_packed_choice();
+    const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
+    if (nch > 0 && (pc == 1 || (pc == -1 && nch >= 512 && fill < 0.6))) {
+      constexpr int W = N >= 16 ? 16 : 4;
+      WBFMM_LAUNCH("hel_m2l_f32", (k_hel_m2l_packed<T, N, W>), dim3(nch), dim3(64 * W), 0, cls,
+                   cp, occ, level, (const T2t<T> *)mps, (T2t<T> *)mpr,
+                   (const wbfmm_m2l_plan_t *)plan, A, A2, CS, (const T2t<T> *)zero);
+      return 0;
+    }
+  }
   const int nsplit = nchunks >= 1024 ? 1 : (int)(2048 / nchunks < 8 ? 2048 / nchunks : 8);
   T2t<T> *part = nullptr;
   if (nsplit > 1 && dmalloc((void **)&part, sizeof(T2t<T>) * (size_t)nsplit * ntot * C))

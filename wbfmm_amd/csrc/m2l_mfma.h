@@ -348,6 +348,43 @@ const double *zero_expansion();
 int occupancy_bits(const uint32_t *sstart, const uint32_t *send, unsigned level,
                    uint32_t **occ);
 
+// Offset classes of list 4 for the packed sparse-level M2L kernels (Laplace
+// and Helmholtz): offsets with the same dz and the same dx^2 + dy^2 share
+// the polar angle and the distance, hence all three stage tables.  Per
+// parity c = px | py << 1 | pz << 2 of the targets' child digit, the classes
+// in the order of the key (dz + 3) 32 + dx^2 + dy^2, members in cube-index
+// order (k = 36 (dz+2+pz) + 6 (dy+2+py) + dx+2+px).
+constexpr int kMaxOffCls = 64;
+
+struct OffClasses {
+  int n;                        // classes (51 for every parity)
+  uint8_t nm[kMaxOffCls];       // members per class (at most 8)
+  uint8_t k[kMaxOffCls][8];     // members' cube indices, increasing
+};
+
+constexpr OffClasses make_off_classes(int c) {
+  OffClasses O{};
+  const int px = c & 1, py = (c >> 1) & 1, pz = (c >> 2) & 1;
+  for (int key = 0; key < 7 * 32; key++) {
+    int m = 0;
+    for (int k = 0; k < 216; k++) {
+      const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
+      const bool nr = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+      if (nr || (dz + 3) * 32 + dx * dx + dy * dy != key) continue;
+      O.k[O.n][m++] = (uint8_t)k;
+    }
+    if (m) O.nm[O.n++] = (uint8_t)m;
+  }
+  return O;
+}
+
+static __constant__ OffClasses kOffCls[8] = {make_off_classes(0), make_off_classes(1),
+                                             make_off_classes(2), make_off_classes(3),
+                                             make_off_classes(4), make_off_classes(5),
+                                             make_off_classes(6), make_off_classes(7)};
+static_assert(make_off_classes(0).n <= kMaxOffCls && make_off_classes(7).n <= kMaxOffCls,
+              "offset classes");
+
 }  // namespace wbfmm
 
 #endif
