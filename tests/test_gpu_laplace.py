@@ -372,6 +372,22 @@ def test_packed_m2l_matches_oracle(wb, oracle, monkeypatch, name, p, waves):
         assert rel(fg[:, c], fo[:, c]) < PARITY, (name, p, waves, c, rel(fg[:, c], fo[:, c]))
 
 
+@pytest.mark.parametrize("p", [13, 14, 15])
+@pytest.mark.parametrize("name", sorted(PACKED_CASES))
+def test_packed_m2l_orders_13_15_match_oracle(wb, oracle, monkeypatch, name, p):
+    # orders 13..15: the class-packed sparse-level M2L with sixteen
+    # wavefronts per 32 targets (forced at every level) against the oracle
+    xs, q, xt, depth, nrm, d = PACKED_CASES[name]
+    x0, D = fit_box(xs, xt)
+    print(f"packed case {name} p={p}", flush=True)
+    fo, _, _ = oracle.fmm(1, xs, xt, x0, D, depth, [p] * 17, [p] * 17, q=q, nrm=nrm, d=d,
+                          field=3, nthreads=8)
+    monkeypatch.setenv("WBFMM_M2L_PACKED", "1")
+    fg = gpu_fmm(wb, xs, q, xt, x0, D, depth, p, field=3, nrm=nrm, d=d)
+    for c in range(4):
+        assert rel(fg[:, c], fo[:, c]) < PARITY, (name, p, c, rel(fg[:, c], fo[:, c]))
+
+
 def test_packed_m2l_chosen_by_size_matches_oracle(wb, oracle):
     # 25,000 uniform points at depth 5: level 5 has about 17,500 occupied
     # boxes (53 %) in more than 1024 chunks, so the sparse-level kernel is

@@ -1679,10 +1679,15 @@ void k_lap_m2l_packed_db(
 // is doubled and the expansions stay in registers, as in
 // k_lap_m2l_packed_db.  Each target sums its translations in tile order.
 // ---------------------------------------------------------------------
+// up to order 12 eight wavefronts (two blocks per CU); orders 13..15 sixteen
+// (one block per CU, the larger double buffer; registers stay under 128
+// with the units and bins spread over twice the wavefronts)
 template <int N>
 struct PackedClsFits {
   static constexpr int Sz = (M2LShape<N>::S + 1) * 16 * 8;
-  static constexpr bool value = N <= 12 && 2 * Sz + 16 * 1024 <= 72 * 1024;
+  static constexpr bool value = (N <= 12 && 2 * Sz + 16 * 1024 <= 72 * 1024) ||
+                                (N >= 13 && N <= 15 && 2 * Sz + 32 * 1024 <= 150 * 1024);
+  static constexpr int W = N <= 12 ? 8 : 16;
 };
 
 template <int N, int W, int WI>
@@ -1856,7 +1861,7 @@ __device__ __forceinline__ void m2l_packed_cls_loop(
 }
 
 template <int N, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 8 ? 4 : 2)))
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W >= 8 ? 4 : 2)))
 void k_lap_m2l_packed_cls(
     const uint32_t *__restrict__ cls, ClassArgs ca, const uint32_t *__restrict__ occ,
     int level, const double *__restrict__ mps, double *__restrict__ mpr,
@@ -1865,7 +1870,7 @@ void k_lap_m2l_packed_cls(
     const double *__restrict__ CS, const double *__restrict__ zero) {
   using SH = M2LShape<N>;
   constexpr int NB = SH::NB, NU = SH::NU, S = SH::S;
-  static_assert(W == 4 || W == 8, "four or eight wavefronts per block");
+  static_assert(W == 4 || W == 8 || W == 16, "four, eight or sixteen wavefronts per block");
   __shared__ double R0[(S + 1) * 16], R1[(S + 1) * 16];
   __shared__ int4 tmap1[4 * NB], tmap2[4 * NB], tmap3[4 * NB];
   __shared__ int4 uinf[NU];
@@ -2010,7 +2015,7 @@ void k_lap_m2l_packed_cls(
       case 2: PACKED_CLS(2); break;
       default: PACKED_CLS(3); break;
     }
-  } else {
+  } else if constexpr (W == 8) {
     switch (wv) {
       case 0: PACKED_CLS(0); break;
       case 1: PACKED_CLS(1); break;
@@ -2020,6 +2025,25 @@ void k_lap_m2l_packed_cls(
       case 5: PACKED_CLS(5); break;
       case 6: PACKED_CLS(6); break;
       default: PACKED_CLS(7); break;
+    }
+  } else {
+    switch (wv) {
+      case 0: PACKED_CLS(0); break;
+      case 1: PACKED_CLS(1); break;
+      case 2: PACKED_CLS(2); break;
+      case 3: PACKED_CLS(3); break;
+      case 4: PACKED_CLS(4); break;
+      case 5: PACKED_CLS(5); break;
+      case 6: PACKED_CLS(6); break;
+      case 7: PACKED_CLS(7); break;
+      case 8: PACKED_CLS(8); break;
+      case 9: PACKED_CLS(9); break;
+      case 10: PACKED_CLS(10); break;
+      case 11: PACKED_CLS(11); break;
+      case 12: PACKED_CLS(12); break;
+      case 13: PACKED_CLS(13); break;
+      case 14: PACKED_CLS(14); break;
+      default: PACKED_CLS(15); break;
     }
   }
 #undef PACKED_CLS
@@ -2053,7 +2077,7 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   constexpr int C = (N + 1) * (N + 1);
   unsigned ntot = 0;
   for (int c = 0; c < 8; c++) ntot += ca.count[c];
-  if constexpr (sizeof(T) == 8 && PackedFits<N>::value) {
+  if constexpr (sizeof(T) == 8 && (PackedFits<N>::value || PackedClsFits<N>::value)) {
     const int pc = m2l_packed_choice();
     const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
     // forced on, the kernel runs at any size (it needs no minimum chunk
@@ -2072,18 +2096,20 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
       // offsets packed by class (WBFMM_PACKED_CLS=0: one offset per tile)
       if constexpr (PackedClsFits<N>::value) {
         if (!(pcl && atoi(pcl) == 0) && !(pd && atoi(pd) == 0)) {
-          if (!(pw && atoi(pw) == 4))
-            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, 8>), dim3(nch), dim3(512), 0, cls,
-                         cp, occ, level, (const double *)mps, (double *)mpr,
+          constexpr int WC = PackedClsFits<N>::W;
+          if (WC == 16 || !(pw && atoi(pw) == 4)) {
+            WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, WC>), dim3(nch), dim3(64 * WC), 0,
+                         cls, cp, occ, level, (const double *)mps, (double *)mpr,
                          (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
                          (const double *)A3, (const double *)A2, (const double *)CS,
                          (const double *)zero);
-          else
+          } else if constexpr (WC == 8) {
             WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, 4>), dim3(nch), dim3(256), 0, cls,
                          cp, occ, level, (const double *)mps, (double *)mpr,
                          (const wbfmm_m2l_plan_t *)plan, (const double *)A1,
                          (const double *)A3, (const double *)A2, (const double *)CS,
                          (const double *)zero);
+          }
           return 0;
         }
       }
@@ -2115,17 +2141,20 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
           return 0;
         }
       }
-      if (!(pw && atoi(pw) == 4))
-        WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 8>), dim3(nch), dim3(512), 0, cls, cp, occ,
-                     level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
-                     (const double *)A1, (const double *)A3, (const double *)A2,
-                     (const double *)CS, (const double *)zero);
-      else
-        WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 4>), dim3(nch), dim3(256), 0, cls, cp, occ,
-                     level, (const double *)mps, (double *)mpr, (const wbfmm_m2l_plan_t *)plan,
-                     (const double *)A1, (const double *)A3, (const double *)A2,
-                     (const double *)CS, (const double *)zero);
-      return 0;
+      if constexpr (PackedFits<N>::value) {
+        if (!(pw && atoi(pw) == 4))
+          WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 8>), dim3(nch), dim3(512), 0, cls, cp,
+                       occ, level, (const double *)mps, (double *)mpr,
+                       (const wbfmm_m2l_plan_t *)plan, (const double *)A1, (const double *)A3,
+                       (const double *)A2, (const double *)CS, (const double *)zero);
+        else
+          WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed<N, 4>), dim3(nch), dim3(256), 0, cls, cp,
+                       occ, level, (const double *)mps, (double *)mpr,
+                       (const wbfmm_m2l_plan_t *)plan, (const double *)A1, (const double *)A3,
+                       (const double *)A2, (const double *)CS, (const double *)zero);
+        return 0;
+      }
+      // orders 13..15 with the class-packed kernel switched off: chunked
     }
   }
   const int nsplit = m2l_nsplit(nchunks);
