@@ -336,6 +336,16 @@ __device__ __forceinline__ void m2l_w2_loop(
     }
     return 216;
   };
+  // class order (ca.ord): the position after pos in kOffOrder[c] whose
+  // offset is used (189 when there is none)
+  auto next_pos = [&](int pos) {
+    for (pos++; pos < 189; pos++) {
+      const int k = kOffOrder[c].k[pos];
+      if (((anym[k >> 5] >> (k & 31)) & 1u) && (nsplit == 1 || k % nsplit == split)) break;
+    }
+    return pos;
+  };
+  const bool cord = ca.ord != 0;
   auto offset_of = [&](int k) {
     const int dx = k % 6 - 2 - px, dy = (k / 6) % 6 - 2 - py, dz = k / 36 - 2 - pz;
     return (dx + 3) * 49 + (dy + 3) * 7 + dz + 3;
@@ -400,7 +410,10 @@ __device__ __forceinline__ void m2l_w2_loop(
 
   // k is the same in every lane (it comes from the shared anym): held as a
   // scalar, and the next offset's table indices are read one offset ahead
-  int k = __builtin_amdgcn_readfirstlane(next_k(-1)), p = 0;
+  int pos = cord ? __builtin_amdgcn_readfirstlane(next_pos(-1)) : 0;
+  int k = cord ? (pos < 189 ? (int)kOffOrder[c].k[pos] : 216)
+               : __builtin_amdgcn_readfirstlane(next_k(-1));
+  int p = 0;
   int th = 0, ri = 0;
   if (k < 216) {
     th = __builtin_amdgcn_readfirstlane(P->th[offset_of(k)]);
@@ -417,7 +430,13 @@ __device__ __forceinline__ void m2l_w2_loop(
     asm volatile("s_mov_b32 %0, 0" : "=s"(z0));
     const int4 *t1 = tmap1 + z0, *t2 = tmap2 + z0, *t3 = tmap3 + z0, *tu = uinf + z0;
     const int ln = lane + z0, cl = ln & 15, gp = ln >> 4, ub = gp + 4 * WI;
-    const int kn = __builtin_amdgcn_readfirstlane(next_k(k));
+    int pos_n = 0, kn;
+    if (cord) {
+      pos_n = __builtin_amdgcn_readfirstlane(next_pos(pos));
+      kn = pos_n < 189 ? (int)kOffOrder[c].k[pos_n] : 216;
+    } else {
+      kn = __builtin_amdgcn_readfirstlane(next_k(k));
+    }
     int th_n = 0, ri_n = 0;
     if (kn < 216) {
       th_n = __builtin_amdgcn_readfirstlane(P->th[offset_of(kn)]);
@@ -498,6 +517,7 @@ __device__ __forceinline__ void m2l_w2_loop(
     __syncthreads();
     p ^= 1;
     k = kn;
+    pos = pos_n;
     th = th_n;
     ri = ri_n;
   }
@@ -2162,6 +2182,10 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   // unsplit levels
   ClassArgs cx = ca;
   unsigned grid = nchunks * nsplit;
+  {
+    const char *oe = getenv("WBFMM_M2L_CLSORDER");
+    cx.ord = oe && atoi(oe) == 1 ? 1u : 0u;
+  }
   {
     const char *xe = getenv("WBFMM_M2L_XCD");
     if (nsplit == 1 && xe && atoi(xe) == 1) {

@@ -50,6 +50,9 @@ struct ClassArgs {
   // chunks per class and XCD when the blocks interleave the classes (see
   // block_chunk); 0: chunks in class order
   unsigned qc;
+  // the chunked Laplace M2L's order of the offsets: 0 cube index, 1 offset
+  // class (kOffOrder)
+  unsigned ord = 0;
 };
 
 // The chunk (class c, index j within the class) of block bid, or false
@@ -384,6 +387,26 @@ static __constant__ OffClasses kOffCls[8] = {make_off_classes(0), make_off_class
                                              make_off_classes(6), make_off_classes(7)};
 static_assert(make_off_classes(0).n <= kMaxOffCls && make_off_classes(7).n <= kMaxOffCls,
               "offset classes");
+
+// the far offsets of each parity in class order (the members of
+// kOffCls[c] one class after the other): 189 cube indices
+struct OffOrder {
+  uint8_t k[189];
+};
+
+constexpr OffOrder make_off_order(int c) {
+  OffOrder O{};
+  const OffClasses C = make_off_classes(c);
+  int n = 0;
+  for (int g = 0; g < C.n; g++)
+    for (int i = 0; i < C.nm[g]; i++) O.k[n++] = C.k[g][i];
+  return O;
+}
+
+static __constant__ OffOrder kOffOrder[8] = {make_off_order(0), make_off_order(1),
+                                             make_off_order(2), make_off_order(3),
+                                             make_off_order(4), make_off_order(5),
+                                             make_off_order(6), make_off_order(7)};
 
 }  // namespace wbfmm
 
