@@ -29,6 +29,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "hip_common.h"
 #include "m2l_mfma.h"
 
@@ -139,6 +141,50 @@ __device__ __forceinline__ double pbar_mm(int m, double S) {
 __device__ __forceinline__ double pbar_up(int n, int m, double C, double pn, double pnm1) {
   const double v = C * sqrt(2.0 * n + 1) * pn - sqrt((double)(n * n - m * m) / (2 * n - 1)) * pnm1;
   return v / sqrt((double)((n + 1) * (n + 1) - m * m) / (2 * n + 3));
+}
+
+// The factors of these recurrences per (n, m), as one table per process
+// built once on the device by the functions above (the same values), for
+// the field kernels: their loops over (n, m) are the same in every lane,
+// so the table is read with scalar loads instead of a square root and a
+// division per factor and target.
+constexpr int kRT = WBFMM_HIP_MAX_ORDER + 3;
+struct RecTab {
+  double ra[kRT][kRT];          // rec_a(i - 1, m)
+  double rb[kRT][2 * kRT + 1];  // rec_b(i, m - kRT)
+  double pa[kRT];               // sqrt(2 i + 1)
+  double pb[kRT][kRT];          // sqrt((i^2 - m^2) / (2 i - 1))
+  double pd[kRT][kRT];          // sqrt(((i + 1)^2 - m^2) / (2 i + 3))
+};
+
+__global__ void k_rec_tab(RecTab *T) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < kRT * kRT) {
+    const int i = t / kRT, m = t % kRT;
+    T->ra[i][m] = rec_a(i - 1, m);
+    T->pb[i][m] = sqrt((double)(i * i - m * m) / (2 * i - 1));
+    T->pd[i][m] = sqrt((double)((i + 1) * (i + 1) - m * m) / (2 * i + 3));
+  }
+  if (t < kRT * (2 * kRT + 1)) {
+    const int i = t / (2 * kRT + 1), m = t % (2 * kRT + 1) - kRT;
+    T->rb[i][m + kRT] = rec_b(i, m);
+  }
+  if (t < kRT) T->pa[t] = sqrt(2.0 * t + 1);
+}
+
+// rec_a(n, m) for n >= -1, |m| <= n + 1 <= WBFMM_HIP_MAX_ORDER + 1
+__device__ __forceinline__ double tab_a(const RecTab *__restrict__ T, int n, int m) {
+  return T->ra[n + 1][m < 0 ? -m : m];
+}
+// rec_b(n, m) for 0 <= n <= WBFMM_HIP_MAX_ORDER + 2, |m| <= n + 1
+__device__ __forceinline__ double tab_b(const RecTab *__restrict__ T, int n, int m) {
+  return T->rb[n][m + kRT];
+}
+// pbar_up with the table's factors (0 <= m <= n <= WBFMM_HIP_MAX_ORDER + 1)
+__device__ __forceinline__ double tab_pbar_up(const RecTab *__restrict__ T, int n, int m, double C,
+                                              double pn, double pnm1) {
+  const double v = C * T->pa[n] * pn - T->pb[n][m] * pnm1;
+  return v / T->pd[n][m];
 }
 
 // ---------------------------------------------------------------------
@@ -477,7 +523,8 @@ __global__ __launch_bounds__(128) void k_hel_field(
     const double *__restrict__ xyz, const double2 *__restrict__ q,
     const double *__restrict__ nrm, const double2 *__restrict__ dip,
     unsigned level, double x0, double y0, double z0, double D, int N, double k,
-    const double2 *__restrict__ mpr, int field, double *__restrict__ f, int fstr) {
+    const double2 *__restrict__ mpr, int field, double *__restrict__ f, int fstr,
+    const RecTab *__restrict__ RT) {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double x = xt[4 * i], y = xt[4 * i + 1], z = xt[4 * i + 2];
@@ -511,18 +558,18 @@ __global__ __launch_bounds__(128) void k_hel_field(
           const double2 R = make_double2(a * cs, sg * a * sn);
           if ((field & 1) && nn <= N) pot = cadd(pot, cmul(Lg(nn, mm), R));
           if (field & 2) {
-            const double2 cz = cadd(cscale(Lg(nn + 1, mm), rec_a(nn, mm)),
-                                    cscale(Lg(nn - 1, mm), -rec_a(nn - 1, mm)));
-            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), rec_b(nn, -mm)),
-                                     cscale(Lg(nn + 1, mm - 1), -rec_b(nn + 1, mm - 1)));
-            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), rec_b(nn, mm)),
-                                     cscale(Lg(nn + 1, mm + 1), -rec_b(nn + 1, -mm - 1)));
+            const double2 cz = cadd(cscale(Lg(nn + 1, mm), tab_a(RT, nn, mm)),
+                                    cscale(Lg(nn - 1, mm), -tab_a(RT, nn - 1, mm)));
+            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), tab_b(RT, nn, -mm)),
+                                     cscale(Lg(nn + 1, mm - 1), -tab_b(RT, nn + 1, mm - 1)));
+            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), tab_b(RT, nn, mm)),
+                                     cscale(Lg(nn + 1, mm + 1), -tab_b(RT, nn + 1, -mm - 1)));
             gz = cadd(gz, cmul(cz, R));
             dpl = cadd(dpl, cmul(cpz, R));
             dmi = cadd(dmi, cmul(cmz, R));
           }
         }
-        const double pn = pbar_up(nn, m, C, p, pprev);
+        const double pn = tab_pbar_up(RT, nn, m, C, p, pprev);
         pprev = p;
         p = pn;
       }
@@ -817,7 +864,7 @@ __global__ __launch_bounds__(128) void k_hel_field_mixed(
     const double *__restrict__ nrm, const double2 *__restrict__ dip,
     unsigned level, double x0, double y0, double z0, double D, int N, double k,
     const float2 *__restrict__ mpr, const double *__restrict__ sc, int field,
-    double *__restrict__ f, int fstr) {
+    double *__restrict__ f, int fstr, const RecTab *__restrict__ RT) {
   __shared__ double sinv[NMAX + 2];
   for (int t = threadIdx.x; t <= N + 1; t += blockDim.x) sinv[t] = 1.0 / sc[t];
   __syncthreads();
@@ -855,18 +902,18 @@ __global__ __launch_bounds__(128) void k_hel_field_mixed(
           const double2 R = make_double2(a * cs, sg * a * sn);
           if ((field & 1) && nn <= N) pot = cadd(pot, cmul(Lg(nn, mm), R));
           if (field & 2) {
-            const double2 cz = cadd(cscale(Lg(nn + 1, mm), rec_a(nn, mm)),
-                                    cscale(Lg(nn - 1, mm), -rec_a(nn - 1, mm)));
-            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), rec_b(nn, -mm)),
-                                     cscale(Lg(nn + 1, mm - 1), -rec_b(nn + 1, mm - 1)));
-            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), rec_b(nn, mm)),
-                                     cscale(Lg(nn + 1, mm + 1), -rec_b(nn + 1, -mm - 1)));
+            const double2 cz = cadd(cscale(Lg(nn + 1, mm), tab_a(RT, nn, mm)),
+                                    cscale(Lg(nn - 1, mm), -tab_a(RT, nn - 1, mm)));
+            const double2 cpz = cadd(cscale(Lg(nn - 1, mm - 1), tab_b(RT, nn, -mm)),
+                                     cscale(Lg(nn + 1, mm - 1), -tab_b(RT, nn + 1, mm - 1)));
+            const double2 cmz = cadd(cscale(Lg(nn - 1, mm + 1), tab_b(RT, nn, mm)),
+                                     cscale(Lg(nn + 1, mm + 1), -tab_b(RT, nn + 1, -mm - 1)));
             gz = cadd(gz, cmul(cz, R));
             dpl = cadd(dpl, cmul(cpz, R));
             dmi = cadd(dmi, cmul(cmz, R));
           }
         }
-        const double pn = pbar_up(nn, m, C, p, pprev);
+        const double pn = tab_pbar_up(RT, nn, m, C, p, pprev);
         pprev = p;
         p = pn;
       }
@@ -1066,6 +1113,24 @@ __global__ __launch_bounds__(256) void k_hel_direct(
 
 using namespace wbfmm;
 
+// the process's recurrence table (built once, on the first call, on the
+// calling stream; other streams read it once it is published)
+static RecTab *g_rectab = nullptr;
+static std::mutex g_rectab_mu;
+
+static int rec_tab(const RecTab **out) {
+  std::lock_guard<std::mutex> lk(g_rectab_mu);
+  if (!g_rectab) {
+    RecTab *t = nullptr;
+    WBFMM_CHECK(hipMalloc((void **)&t, sizeof(RecTab)));
+    WBFMM_LAUNCH("rec_tab", k_rec_tab, dim3((kRT * (2 * kRT + 1) + 255) / 256), dim3(256), 0, t);
+    WBFMM_CHECK(hipStreamSynchronize(stream()));
+    g_rectab = t;
+  }
+  *out = g_rectab;
+  return 0;
+}
+
 extern "C" {
 
 static size_t lds_vectors(int L, int nvec) {
@@ -1192,10 +1257,15 @@ int wbfmm_hip_hel_field(const double *xt, const uint32_t *tid, unsigned n,
                         const double *mpr, int field, double *f, int fstr) {
   if (n == 0) return 0;
   if (N < 0) return WBFMM_HIP_ERR_ORDER;
+  const RecTab *RT = nullptr;
+  {
+    const int rc = rec_tab(&RT);
+    if (rc) return rc;
+  }
 #define HF(NM)                                                                          \
   WBFMM_LAUNCH("hel_l2p_p2p", k_hel_field<NM>, dim3((n + 127) / 128), dim3(128), 0, xt, \
                tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,  \
-               level, x0[0], x0[1], x0[2], D, N, k, (const double2 *)mpr, field, f, fstr)
+               level, x0[0], x0[1], x0[2], D, N, k, (const double2 *)mpr, field, f, fstr, RT)
   if (N <= 16) HF(16);
   else if (N <= 24) HF(24);
   else if (N <= 32) HF(32);
@@ -1213,11 +1283,16 @@ int wbfmm_hip_hel_field_f32(const double *xt, const uint32_t *tid, unsigned n,
                             int fstr) {
   if (n == 0) return 0;
   if (N < 0 || !scale) return N < 0 ? WBFMM_HIP_ERR_ORDER : WBFMM_HIP_ERR_ARG;
+  const RecTab *RT = nullptr;
+  {
+    const int rc = rec_tab(&RT);
+    if (rc) return rc;
+  }
 #define HF(NM)                                                                              \
   WBFMM_LAUNCH("hel_l2p_p2p_f32", k_hel_field_mixed<NM>, dim3((n + 127) / 128), dim3(128), 0, \
                xt, tid, n, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip,   \
                level, x0[0], x0[1], x0[2], D, N, k, (const float2 *)mpr, scale, field, f,    \
-               fstr)
+               fstr, RT)
   if (N <= 16) HF(16);
   else if (N <= 24) HF(24);
   else if (N <= 32) HF(32);
