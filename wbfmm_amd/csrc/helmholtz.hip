@@ -277,7 +277,8 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
     const double *__restrict__ xyz, const double2 *__restrict__ q,
     const double *__restrict__ nrm, const double2 *__restrict__ dip,
     unsigned level, double x0, double y0, double z0, double D, int N,
-    double k, int pb, const double *__restrict__ sc, O *__restrict__ mps) {
+    double k, int pb, const double *__restrict__ sc, O *__restrict__ mps,
+    const RecTab *__restrict__ RT) {
   extern __shared__ double2 smem[];
   const int N1 = N + 1, tid = threadIdx.x, NT = (N1 + 1) * (N1 + 2) / 2;
   double2 *R = smem;                        // pb x NT: R_n^m, n <= N+1, m >= 0
@@ -315,7 +316,7 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
       double pv = pbar_mm(m, S), pprev = 0.0;
       for (int n = m; n <= N1; n++) {
         Rp[n * (n + 1) / 2 + m] = make_double2(j[n] * pv * cs, j[n] * pv * sn);
-        const double pn = (n + 1 <= N1) ? pbar_up(n, m, C, pv, pprev) : 0.0;
+        const double pn = (n + 1 <= N1) ? tab_pbar_up(RT, n, m, C, pv, pprev) : 0.0;
         pprev = pv;
         pv = pn;
       }
@@ -327,9 +328,9 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
       if (c >= N1 * N1) break;
       const int n = isqrt_deg(c), mm = c - n * (n + 1);
       // G&D 2004 eq 3.7 coefficients of this (n, mm)
-      const double a0 = rec_a(n - 1, mm), a1 = -rec_a(n, mm);
-      const double bp0 = rec_b(n + 1, -mm - 1), bp1 = -rec_b(n, mm);
-      const double bm0 = rec_b(n + 1, mm - 1), bm1 = -rec_b(n, -mm);
+      const double a0 = tab_a(RT, n - 1, mm), a1 = -tab_a(RT, n, mm);
+      const double bp0 = tab_b(RT, n + 1, -mm - 1), bp1 = -tab_b(RT, n, mm);
+      const double bm0 = tab_b(RT, n + 1, mm - 1), bm1 = -tab_b(RT, n, -mm);
       double2 ac = acc[i];
       for (int p = 0; p < np; p++) {
         const uint32_t s = sb + p;
@@ -1147,9 +1148,14 @@ int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *star
   const size_t per = sizeof(double2) * (size_t)(N + 2) * (N + 3) / 2 + sizeof(double) * (N + 2);
   int pb = (int)((60 * 1024) / per);
   pb = pb < 1 ? 1 : (pb > kP2MBatch ? kP2MBatch : pb);
+  const RecTab *RT = nullptr;
+  {
+    const int rc = rec_tab(&RT);
+    if (rc) return rc;
+  }
   WBFMM_LAUNCH("hel_p2m", k_hel_p2m<double2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
                nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
-               x0[0], x0[1], x0[2], D, N, k, pb, (const double *)nullptr, (double2 *)mps);
+               x0[0], x0[1], x0[2], D, N, k, pb, (const double *)nullptr, (double2 *)mps, RT);
   return 0;
 }
 
@@ -1163,9 +1169,14 @@ int wbfmm_hip_hel_p2m_f32(const uint32_t *boxes, unsigned nbox, const uint32_t *
   const size_t per = sizeof(double2) * (size_t)(N + 2) * (N + 3) / 2 + sizeof(double) * (N + 2);
   int pb = (int)((60 * 1024) / per);
   pb = pb < 1 ? 1 : (pb > kP2MBatch ? kP2MBatch : pb);
+  const RecTab *RT = nullptr;
+  {
+    const int rc = rec_tab(&RT);
+    if (rc) return rc;
+  }
   WBFMM_LAUNCH("hel_p2m_f32", k_hel_p2m<float2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
                nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
-               x0[0], x0[1], x0[2], D, N, k, pb, scale, (float2 *)mps);
+               x0[0], x0[1], x0[2], D, N, k, pb, scale, (float2 *)mps, RT);
   return 0;
 }
 
