@@ -152,6 +152,134 @@ __global__ void k_lap_p2m(const uint32_t *boxes, unsigned nbox,
 }
 
 // ---------------------------------------------------------------------
+// Monopole P2M for leaves of a few points.  A group of L = N/2 + 1 lanes
+// takes one box; lane j computes columns m = j and m = N - j of the regular
+// solid harmonics (the middle column alone when N is even), so that every
+// lane runs about N + 2 steps of the recurrence in n per point.  Each
+// coefficient has one owner lane, which adds the box's points in source
+// order in registers and then adds that sum to the box's expansion once.
+// The arithmetic per point is that of k_lap_p2m (the square-root factors
+// come from an LDS table that the block fills with the same expressions);
+// what goes is k_lap_p2m's read-modify-write of every coefficient in
+// global memory per point, a chain of dependent memory round trips.
+// ---------------------------------------------------------------------
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void k_lap_p2m_cols(
+    const uint32_t *boxes, unsigned nbox, const uint32_t *start, const uint32_t *end,
+    const double *xyz, const double *q, int level, double x0, double y0, double z0,
+    double D, int N, double rs, T *mps) {
+  constexpr int K = NB + 1;
+  __shared__ double tA[K], tS[K], tB[K * K], tC[K * K];
+  for (int i = threadIdx.x; i < K * K; i += blockDim.x) {
+    const int k = i / K, m = i - (i / K) * K;
+    if (m <= k) {
+      // the factors of P_{k+1}^m from P_k^m and P_{k-1}^m
+      tB[i] = sqrt((double)(k * k - m * m) / (2 * k - 1));
+      tC[i] = sqrt((2.0 * k + 3) / (double)((k + 1) * (k + 1) - m * m));
+    }
+    if (i < K) {
+      tA[i] = sqrt(2.0 * i + 1);
+      tS[i] = i > 0 ? sqrt((2.0 * i + 1) / (2.0 * i)) : 0.0;
+    }
+  }
+  __syncthreads();
+  const int L = N / 2 + 1, BPW = 64 / L;
+  const int lane = threadIdx.x & 63, g = lane / L, j = lane - g * L;
+  const unsigned t = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * BPW + g;
+  if (g >= BPW || t >= nbox) return;
+  const int m1 = j, m2 = N - j;
+  const bool pair = m2 > m1;
+  const int sw = N - j + 1;               // first step of column m2
+  const int last = pair ? N + 1 : N - j;  // last step of the lane
+  const uint64_t b = boxes[t];
+  const uint32_t bi = compact3(b), bj = compact3(b >> 1), bk = compact3(b >> 2);
+  const double w = D / (double)(1u << level);
+  const double xc = x0 + w * (bi + 0.5), yc = y0 + w * (bj + 0.5), zc = z0 + w * (bk + 0.5);
+  const double s4pi = 1.0 / sqrt(4.0 * kPi);
+  double ar[NB + 2], ai[NB + 2];
+#pragma unroll
+  for (int i = 0; i < NB + 2; i++) ar[i] = ai[i] = 0.0;
+  for (uint32_t s = start[b]; s < end[b]; s++) {
+    const double qs = q[s];
+    if (qs == 0.0) continue;
+    const double dx = xyz[4 * (size_t)s] - xc, dy = xyz[4 * (size_t)s + 1] - yc,
+                 dz = xyz[4 * (size_t)s + 2] - zc;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz), rho = sqrt(dx * dx + dy * dy);
+    const double Cth = r > 0.0 ? dz / r : 1.0, Sth = r > 0.0 ? rho / r : 0.0;
+    const double cp = rho > 0.0 ? dx / rho : 1.0, sp = rho > 0.0 ? dy / rho : 0.0;
+    const double rr = r * rs;
+    // P_m^m, r^m and e^{i m phi} up to m2, kept at m1 on the way
+    double pmm = s4pi, rm = 1.0, cm = 1.0, sm = 0.0;
+    double pa = pmm, ra = rm, ca = cm, sa = sm;
+#pragma unroll
+    for (int i = 1; i <= NB; i++) {
+      if (i <= m2) {
+        pmm *= tS[i] * Sth;
+        rm *= rr;
+        const double tcm = cm * cp - sm * sp;
+        sm = sm * cp + cm * sp;
+        cm = tcm;
+        if (i == m1) {
+          pa = pmm;
+          ra = rm;
+          ca = cm;
+          sa = sm;
+        }
+      }
+    }
+    double p1 = 0.0, p0 = pa, rn = ra, c = ca, sn = sa;
+    int n = m1, m = m1;
+#pragma unroll
+    for (int st = 0; st < NB + 2; st++) {
+      if (st > last) break;
+      if (st > 0) {
+        if (pair && st == sw) {
+          m = m2;
+          n = m2;
+          p1 = 0.0;
+          p0 = pmm;
+          rn = rm;
+          c = cm;
+          sn = sm;
+        } else {
+          const int k = n;
+          n++;
+          const double pn = (Cth * tA[k] * p0 - tB[k * K + m] * p1) * tC[k * K + m];
+          p1 = p0;
+          p0 = pn;
+          rn *= rr;
+        }
+      }
+      const double Rr = rn * p0 * c, Ri = rn * p0 * sn;
+      const double f = qs / (2 * n + 1);
+      ar[st] += f * Rr;
+      ai[st] += f * Ri;
+    }
+  }
+  const int C = (N + 1) * (N + 1);
+  T *out = mps + (size_t)b * C;
+  int n = m1, m = m1;
+#pragma unroll
+  for (int st = 0; st < NB + 2; st++) {
+    if (st > last) break;
+    if (st > 0) {
+      if (pair && st == sw) {
+        m = m2;
+        n = m2;
+      } else {
+        n++;
+      }
+    }
+    if (m == 0) {
+      out[n * n] += ar[st];
+    } else {
+      out[n * n + 2 * m - 1] += ar[st];
+      out[n * n + 2 * m] -= ai[st];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------
 // The same expansions with one wavefront per box, for boxes that hold many
 // points (clustered or small problems, where one thread per box leaves
 // most of the chip idle and serialises hundreds of points).  The box's
@@ -952,6 +1080,27 @@ static int lap_p2m_wave(const uint32_t *boxes, unsigned nbox, const uint32_t *st
   return 0;
 }
 
+// monopole P2M with a group of lanes per box (orders up to 16)
+template <typename T>
+static int lap_p2m_cols(const uint32_t *boxes, unsigned nbox, const uint32_t *start,
+                        const uint32_t *end, const double *xyz, const double *q,
+                        unsigned level, const double *x0, double D, int N, double rs, T *mps,
+                        const char *name) {
+  const int L = N / 2 + 1, per = 4 * (64 / L);  // boxes per block of four wavefronts
+  const dim3 grid((nbox + per - 1) / per);
+#define WBFMM_P2M_COLS(NB)                                                                 \
+  WBFMM_LAUNCH(name, (k_lap_p2m_cols<T, NB>), grid, dim3(256), 0, boxes, nbox, start, end, \
+               xyz, q, (int)level, x0[0], x0[1], x0[2], D, N, rs, mps)
+  if (N <= 8)
+    WBFMM_P2M_COLS(8);
+  else if (N <= 12)
+    WBFMM_P2M_COLS(12);
+  else
+    WBFMM_P2M_COLS(16);
+#undef WBFMM_P2M_COLS
+  return 0;
+}
+
 extern "C" {
 
 int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
@@ -961,6 +1110,8 @@ int wbfmm_hip_lap_p2m(const uint32_t *boxes, unsigned nbox,
                       double D, int N, double *mps) {
   if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
   if (nbox == 0) return 0;
+  if (!dip && q && N <= 16) return lap_p2m_cols<double>(boxes, nbox, start, end, xyz, q, level, x0, D, N,
+                                             1.0, mps, "lap_p2m");
   WBFMM_LAUNCH("lap_p2m", k_lap_p2m<double>, dim3(nblk(nbox, 64)), dim3(64), 0,
                boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0],
                x0[1], x0[2], D, N, 1.0, mps);
@@ -975,6 +1126,8 @@ int wbfmm_hip_lap_p2m_f32(const uint32_t *boxes, unsigned nbox,
   if (N < 0 || N > kMaxN) return WBFMM_HIP_ERR_ORDER;
   if (nbox == 0) return 0;
   const double w = D / (double)(1u << level);
+  if (!dip && q && N <= 16) return lap_p2m_cols<float>(boxes, nbox, start, end, xyz, q, level, x0, D, N,
+                                            1.0 / w, mps, "lap_p2m_f32");
   WBFMM_LAUNCH("lap_p2m_f32", k_lap_p2m<float>, dim3(nblk(nbox, 64)), dim3(64), 0,
                boxes, nbox, start, end, xyz, q, nrm, dip, (int)level, x0[0],
                x0[1], x0[2], D, N, 1.0 / w, mps);
