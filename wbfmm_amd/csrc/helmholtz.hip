@@ -270,8 +270,10 @@ constexpr int kP2MCoef =
 
 // O: double2 (FP64 coefficients) or float2 (FP32 coefficients, each degree
 // n divided by sc[n] in FP64 before rounding; sc NULL: no scaling)
-template <typename O>
-__global__ __launch_bounds__(256) void k_hel_p2m(
+// NC: coefficients per thread, ceil((N+1)^2 / 256); three blocks per CU
+// (the LDS of a batch is at most 60 KiB)
+template <typename O, int NC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_hel_p2m(
     const uint32_t *__restrict__ boxes, unsigned nbox,
     const uint32_t *__restrict__ start, const uint32_t *__restrict__ end,
     const double *__restrict__ xyz, const double2 *__restrict__ q,
@@ -290,9 +292,9 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
   const double xc = x0 + (compact3(b) + 0.5) * w, yc = y0 + (compact3(b >> 1) + 0.5) * w,
                zc = z0 + (compact3(b >> 2) + 0.5) * w;
   const uint32_t s0 = start[b], s1 = end[b];
-  double2 acc[kP2MCoef];
+  double2 acc[NC];
 #pragma unroll
-  for (int i = 0; i < kP2MCoef; i++) acc[i] = make_double2(0.0, 0.0);
+  for (int i = 0; i < NC; i++) acc[i] = make_double2(0.0, 0.0);
 
   for (uint32_t sb = s0; sb < s1; sb += (uint32_t)pb) {
     const int np = (int)min((uint32_t)pb, s1 - sb);
@@ -323,7 +325,7 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kP2MCoef; i++) {
+    for (int i = 0; i < NC; i++) {
       const int c = tid + kP2MThreads * i;
       if (c >= N1 * N1) break;
       const int n = isqrt_deg(c), mm = c - n * (n + 1);
@@ -331,25 +333,41 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
       const double a0 = tab_a(RT, n - 1, mm), a1 = -tab_a(RT, n, mm);
       const double bp0 = tab_b(RT, n + 1, -mm - 1), bp1 = -tab_b(RT, n, mm);
       const double bm0 = tab_b(RT, n + 1, mm - 1), bm1 = -tab_b(RT, n, -mm);
+      // the LDS slots of the R_{n'}^{m'} this coefficient reads, with
+      // R^{-m'} = conj R^{m'} as a sign on the imaginary part and a weight 0
+      // (slot 0) for (n', m') outside the table: (n, mm), (n - 1, mm),
+      // (n + 1, mm), (n + 1, mm + 1), (n - 1, mm + 1), (n + 1, mm - 1),
+      // (n - 1, mm - 1)
+      int ix[7];
+      double sx[7], sy[7];
+      {
+        const int nqs[7] = {n, n - 1, n + 1, n + 1, n - 1, n + 1, n - 1};
+        const int mqs[7] = {mm, mm, mm, mm + 1, mm + 1, mm - 1, mm - 1};
+#pragma unroll
+        for (int e = 0; e < 7; e++) {
+          const int nq = nqs[e], mq = mqs[e];
+          const bool ok = !(nq < 0 || nq > N1 || mq > nq || -mq > nq);
+          ix[e] = ok ? nq * (nq + 1) / 2 + (mq < 0 ? -mq : mq) : 0;
+          sx[e] = ok ? 1.0 : 0.0;
+          sy[e] = ok ? (mq >= 0 ? 1.0 : -1.0) : 0.0;
+        }
+      }
       double2 ac = acc[i];
       for (int p = 0; p < np; p++) {
         const uint32_t s = sb + p;
         const double2 *Rp = R + (size_t)p * NT;
-        auto Rg = [&](int nq, int mq) -> double2 {
-          if (nq < 0 || nq > N1 || mq > nq || -mq > nq) return make_double2(0.0, 0.0);
-          const double2 v = Rp[nq * (nq + 1) / 2 + (mq < 0 ? -mq : mq)];
-          return mq >= 0 ? v : make_double2(v.x, -v.y);
+        auto Rg = [&](int e) -> double2 {
+          const double2 v = Rp[ix[e]];
+          return make_double2(v.x * sx[e], v.y * sy[e]);
         };
         double2 v = make_double2(0.0, 0.0);
-        if (q) v = cmulc(q[s], Rg(n, mm));
+        if (q) v = cmulc(q[s], Rg(0));
         if (dip) {
           const double2 dd = dip[s];
           const double nx = nrm[4 * s], ny = nrm[4 * s + 1], nz = nrm[4 * s + 2];
-          const double2 dz = cscale(cadd(cscale(Rg(n - 1, mm), a0), cscale(Rg(n + 1, mm), a1)), k);
-          const double2 dp =
-              cscale(cadd(cscale(Rg(n + 1, mm + 1), bp0), cscale(Rg(n - 1, mm + 1), bp1)), k);
-          const double2 dm =
-              cscale(cadd(cscale(Rg(n + 1, mm - 1), bm0), cscale(Rg(n - 1, mm - 1), bm1)), k);
+          const double2 dz = cscale(cadd(cscale(Rg(1), a0), cscale(Rg(2), a1)), k);
+          const double2 dp = cscale(cadd(cscale(Rg(3), bp0), cscale(Rg(4), bp1)), k);
+          const double2 dm = cscale(cadd(cscale(Rg(5), bm0), cscale(Rg(6), bm1)), k);
           // gx = (dp + dm)/2, gy = -i (dp - dm)/2
           const double2 gx = cscale(cadd(dp, dm), 0.5);
           const double2 dd2 = cscale(cadd(dp, cscale(dm, -1.0)), 0.5);
@@ -366,7 +384,7 @@ __global__ __launch_bounds__(256) void k_hel_p2m(
   }
   O *out = mps + (size_t)b * N1 * N1;
 #pragma unroll
-  for (int i = 0; i < kP2MCoef; i++) {
+  for (int i = 0; i < NC; i++) {
     const int c = tid + kP2MThreads * i;
     if (c < N1 * N1) {
       const double s = sc ? 1.0 / sc[isqrt_deg(c)] : 1.0;
@@ -1153,9 +1171,23 @@ int wbfmm_hip_hel_p2m(const uint32_t *boxes, unsigned nbox, const uint32_t *star
     const int rc = rec_tab(&RT);
     if (rc) return rc;
   }
-  WBFMM_LAUNCH("hel_p2m", k_hel_p2m<double2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
-               nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
-               x0[0], x0[1], x0[2], D, N, k, pb, (const double *)nullptr, (double2 *)mps, RT);
+  {
+    const int nc = ((N + 1) * (N + 1) + kP2MThreads - 1) / kP2MThreads;
+#define HP2M(NCV) \
+    WBFMM_LAUNCH("hel_p2m", (k_hel_p2m<double2, NCV>), dim3(nbox), dim3(kP2MThreads), per * pb, boxes, \
+                 nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level, \
+                 x0[0], x0[1], x0[2], D, N, k, pb, (const double *)nullptr, (double2 *)mps, RT)
+    switch (nc) {
+      case 1: HP2M(1); break;
+      case 2: HP2M(2); break;
+      case 3: HP2M(3); break;
+      case 4: HP2M(4); break;
+      case 5: HP2M(5); break;
+      case 6: HP2M(6); break;
+      default: HP2M(kP2MCoef); break;
+    }
+#undef HP2M
+  }
   return 0;
 }
 
@@ -1174,9 +1206,23 @@ int wbfmm_hip_hel_p2m_f32(const uint32_t *boxes, unsigned nbox, const uint32_t *
     const int rc = rec_tab(&RT);
     if (rc) return rc;
   }
-  WBFMM_LAUNCH("hel_p2m_f32", k_hel_p2m<float2>, dim3(nbox), dim3(kP2MThreads), per * pb, boxes,
-               nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level,
-               x0[0], x0[1], x0[2], D, N, k, pb, scale, (float2 *)mps, RT);
+  {
+    const int nc = ((N + 1) * (N + 1) + kP2MThreads - 1) / kP2MThreads;
+#define HP2M(NCV) \
+    WBFMM_LAUNCH("hel_p2m_f32", (k_hel_p2m<float2, NCV>), dim3(nbox), dim3(kP2MThreads), per * pb, boxes, \
+                 nbox, start, end, xyz, (const double2 *)q, nrm, (const double2 *)dip, level, \
+                 x0[0], x0[1], x0[2], D, N, k, pb, scale, (float2 *)mps, RT)
+    switch (nc) {
+      case 1: HP2M(1); break;
+      case 2: HP2M(2); break;
+      case 3: HP2M(3); break;
+      case 4: HP2M(4); break;
+      case 5: HP2M(5); break;
+      case 6: HP2M(6); break;
+      default: HP2M(kP2MCoef); break;
+    }
+#undef HP2M
+  }
   return 0;
 }
 
