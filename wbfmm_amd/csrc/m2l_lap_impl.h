@@ -1743,15 +1743,16 @@ __device__ __forceinline__ void m2l_packed_cls_loop(
     tn = 0;
   };
   // this lane's column of tile (g, t): its source expansion and member
-  // index (zeros and member 0 for an empty column); the class's phases
-  // into cs[p] (every wavefront writes the same values)
-  auto source_of = [&](int g, int t, int p, int &icol) -> const double * {
+  // index (zeros and member 0 for an empty column); with fill, the class's
+  // phases into cs[p] (every wavefront writes the same values)
+  auto source_of = [&](int g, int t, int p, bool fill, int &icol) -> const double * {
     const int nm = gnm[g];
-    for (int e = lane; e < nm * (N + 1); e += 64) {
-      const int i = e / (N + 1), m = e - i * (N + 1);
-      const double *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
-      cs[p][i][m] = make_double2(tt[0], tt[1]);
-    }
+    if (fill)
+      for (int e = lane; e < nm * (N + 1); e += 64) {
+        const int i = e / (N + 1), m = e - i * (N + 1);
+        const double *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
+        cs[p][i][m] = make_double2(tt[0], tt[1]);
+      }
     const int T = gT[g], pos = t + col * T;
     icol = 0;
     if (pos >= gcount[g]) return zero;
@@ -1794,7 +1795,7 @@ __device__ __forceinline__ void m2l_packed_cls_loop(
     const int2 tr = gtab[g];
     th = __builtin_amdgcn_readfirstlane(tr.x);
     ri = __builtin_amdgcn_readfirstlane(tr.y);
-    load_sources(source_of(g, t, 0, icol), uinf, ubase);
+    load_sources(source_of(g, t, 0, true, icol), uinf, ubase);
     deposit(R0, uinf, ubase, col, 0, icol);
   }
   double av[StageOps<N, W, WI>::n];
@@ -1810,14 +1811,18 @@ __device__ __forceinline__ void m2l_packed_cls_loop(
     gn = __builtin_amdgcn_readfirstlane(gn);
     tn = __builtin_amdgcn_readfirstlane(tn);
     // the next tile's sources are requested before this tile's stages:
-    // the registers were emptied by the last deposit, and cs[p ^ 1] was
-    // last read before the barrier that ended the previous tile
+    // the registers were emptied by the last deposit.  The phase buffer
+    // changes with the class: a new class's phases go into cs[p ^ 1],
+    // last read (by the previous class) before the barrier that ended the
+    // previous tile; the tiles of one class share cs[p]
+    const bool newc = gn != g;
+    const int pn = newc ? p ^ 1 : p;
     int th_n = 0, ri_n = 0;
     if (gn < ncls) {
       const int2 tr = gtab[gn];
       th_n = __builtin_amdgcn_readfirstlane(tr.x);
       ri_n = __builtin_amdgcn_readfirstlane(tr.y);
-      load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub);
+      load_sources(source_of(gn, tn, pn, newc, icol_n), tu, ub);
     }
     // R0 -> R1 -> R0 -> R1, one barrier per stage
     if constexpr (PF)
@@ -1856,9 +1861,9 @@ __device__ __forceinline__ void m2l_packed_cls_loop(
         }
       }
     }
-    if (gn < ncls) deposit(R0, tu, ub, cl, p ^ 1, icol_n);
+    if (gn < ncls) deposit(R0, tu, ub, cl, pn, icol_n);
     __syncthreads();
-    p ^= 1;
+    p = pn;
     g = gn;
     t = tn;
     th = th_n;
