@@ -487,15 +487,16 @@ __device__ __forceinline__ void hel_packed_loop(
     tn = 0;
   };
   // this lane's column pair of tile (g, t): its source expansion and
-  // member index (zeros and member 0 for an empty pair); the class's
-  // phases into cs[p] (every wavefront writes the same values)
-  auto source_of = [&](int g, int t, int p, int &icol) -> const T2t<T> * {
+  // member index (zeros and member 0 for an empty pair); with fill, the
+  // class's phases into cs[p] (every wavefront writes the same values)
+  auto source_of = [&](int g, int t, int p, bool fill, int &icol) -> const T2t<T> * {
     const int nm = gnm[g];
-    for (int e = lane; e < nm * (N + 1); e += 64) {
-      const int i = e / (N + 1), m = e - i * (N + 1);
-      const T *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
-      cs[p][i][m] = mk2<T>(tt[0], tt[1]);
-    }
+    if (fill)
+      for (int e = lane; e < nm * (N + 1); e += 64) {
+        const int i = e / (N + 1), m = e - i * (N + 1);
+        const T *tt = CS + ((size_t)ktab[gmem[g][i]].w * (N + 1) + m) * 2;
+        cs[p][i][m] = mk2<T>(tt[0], tt[1]);
+      }
     const int Tt = gT[g], pos = t + (col >> 1) * Tt;
     icol = 0;
     if (pos >= gcount[g]) return zero;
@@ -557,7 +558,7 @@ __device__ __forceinline__ void hel_packed_loop(
     const int2 tr = gtab[g];
     th = __builtin_amdgcn_readfirstlane(tr.x);
     ri = __builtin_amdgcn_readfirstlane(tr.y);
-    load_sources(source_of(g, t, 0, icol), uinf, ubase, col & 1);
+    load_sources(source_of(g, t, 0, true, icol), uinf, ubase, col & 1);
     deposit(uinf, ubase, col, 0, icol);
   }
   __syncthreads();
@@ -570,6 +571,11 @@ __device__ __forceinline__ void hel_packed_loop(
     next_tile(g, t, gn, tn);
     gn = __builtin_amdgcn_readfirstlane(gn);
     tn = __builtin_amdgcn_readfirstlane(tn);
+    // the phase buffer changes with the class: a new class's phases go
+    // into cs[p ^ 1], last read (by the previous class) before the barrier
+    // that ended the previous tile; the tiles of one class share cs[p]
+    const bool newc = gn != g;
+    const int pn = newc ? p ^ 1 : p;
     int th_n = 0, ri_n = 0;
     if (gn < ncls) {
       const int2 tr = gtab[gn];
@@ -582,7 +588,7 @@ __device__ __forceinline__ void hel_packed_loop(
     if constexpr (DB) {
       stage_wo<T, N, W, WI>(A + (size_t)th * KT * 64, R, R1, t1, ln, gp, cl);
       __syncthreads();
-      if (gn < ncls) load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub, sp);
+      if (gn < ncls) load_sources(source_of(gn, tn, pn, newc, icol_n), tu, ub, sp);
       stage_cx_wo<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
                                R1, R, t2, ln, gp, cl);
       __syncthreads();
@@ -595,7 +601,7 @@ __device__ __forceinline__ void hel_packed_loop(
       __syncthreads();
       write_out(t1, y, gp, cl);
       __syncthreads();
-      if (gn < ncls) load_sources(source_of(gn, tn, p ^ 1, icol_n), tu, ub, sp);
+      if (gn < ncls) load_sources(source_of(gn, tn, pn, newc, icol_n), tu, ub, sp);
       stage_cx<T, N, W, WI>(A2 + (size_t)(2 * ri) * KT * 64, A2 + (size_t)(2 * ri + 1) * KT * 64,
                             R, ln, gp, cl, y);
       __syncthreads();
@@ -635,9 +641,9 @@ __device__ __forceinline__ void hel_packed_loop(
       }
     }
     if constexpr (!DB) __syncthreads();  // every read of R done before the next deposit
-    if (gn < ncls) deposit(tu, ub, cl, p ^ 1, icol_n);
+    if (gn < ncls) deposit(tu, ub, cl, pn, icol_n);
     __syncthreads();
-    p ^= 1;
+    p = pn;
     g = gn;
     t = tn;
     th = th_n;
