@@ -46,6 +46,22 @@ int wbfmm_hip_stream_create(void **s);
 int wbfmm_hip_stream_destroy(void *s);
 int wbfmm_hip_set_stream(void *s, void **prev);
 void *wbfmm_hip_get_stream(void);
+/* kernel variants of the calling thread (no counterpart in the reference:
+   the serving engine's autotuner times them per request class and sets
+   the winner around each solve).  value -1 restores the library's own
+   choice; other values force a variant where the library would use it:
+   - WBFMM_VARIANT_M2L, at sparse levels (fewer than 60 % of the boxes
+     hold targets): 0 chunks of targets running every offset, 1 tiles of
+     one offset (Laplace FP64; Helmholtz: as 2), 2 tiles of one offset
+     class (the default);
+   - WBFMM_VARIANT_FIELD (Helmholtz L2P + P2P): 0 one thread per target
+     (the default), 1 one wavefront per target box with the neighbours'
+     sources staged through LDS. */
+#define WBFMM_VARIANT_M2L 0
+#define WBFMM_VARIANT_FIELD 1
+#define WBFMM_VARIANT_COUNT 2
+int wbfmm_hip_set_variant(int which, int value);
+int wbfmm_hip_get_variant(int which);
 /* free and total device memory */
 int wbfmm_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* copy of any kind (host/device either side), ordered on the stream;

@@ -115,10 +115,17 @@ gint wbfmm_serve_reset_stats(wbfmm_serve_t *s);
 gint wbfmm_serve_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t *r,
                       gint *depth, gint *order, gsize *footprint);
 /* autotuner: solve r (into a scratch output) at each of the ndepth
-   candidate depths, keep the fastest for requests of the same class
-   (problem, precision, order, log2 of the point count) and return it */
+   candidate depths; at the fastest, time the kernel variants one kind at a
+   time (wbfmm_hip_set_variant: the sparse-level M2L tiles, and for
+   Helmholtz the field kernel), and keep the fastest depth and variants for
+   requests of the same class (problem, precision, order, log2 of the point
+   count); returns the depth in *best */
 gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r,
                       const gint *depths, gint ndepth, gint *best);
+/* the autotuner's choice for r's class: depth (0 when not tuned) and the
+   variants (-1: the library's choice) */
+gint wbfmm_serve_tuned(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, gint *depth,
+                       gint *var_m2l, gint *var_field);
 /* for failure-handling tests: the next request submitted has its first
    count device allocations refused (as when memory runs out), so that it
    ends with WBFMM_HIP_ERR_NOMEM in its result */

@@ -245,6 +245,36 @@ def test_serve_autotuner(S):
         assert srv.stats().completed == 0  # tuning runs are not counted
 
 
+def test_serve_autotuner_kernel_variants(S):
+    # the autotuner also times the kernel variants at its best depth (the
+    # sparse-level M2L tiles, the Helmholtz field kernel) and records them
+    # for the class; a later request of the class runs with them and gives
+    # the field of the library's own choice to within the precision used
+    case = hel_case(93, 40000, 8.0, sphere=True, dipoles=True)
+    with S.Server(2) as srv:
+        f0, r0 = solve(S, srv, case, tol=1e-6, depth=6, check=300)
+        r = request(S, case, out_array(S, case), tol=1e-6)
+        best = srv.tune(r, [5, 6])
+        d, vm, vf = srv.tuned(r)
+        assert d == best and best in (5, 6)
+        assert vm in (-1, 0, 2) and vf in (-1, 0, 1)
+        f1, r1 = solve(S, srv, case, tol=1e-6, check=300)
+        assert r1.depth == best
+        assert 0 < r1.err_potential < 1e-5, r1.as_dict()
+        if best == 6:
+            rel = np.linalg.norm(f1 - f0) / np.linalg.norm(f0)
+            assert rel < 1e-5, rel
+    # a Laplace class: the M2L variants only (no field variant)
+    lcase = lap_case(94, 30000)
+    with S.Server(1) as srv:
+        r = request(S, lcase, out_array(S, lcase), tol=1e-6)
+        best = srv.tune(r, [4])
+        d, vm, vf = srv.tuned(r)
+        assert d == best == 4 and vm in (-1, 0, 1, 2) and vf == -1
+        f, res = solve(S, srv, lcase, tol=1e-6, check=300)
+        assert 0 < res.err_potential < 1e-5, res.as_dict()
+
+
 def test_serve_rejects_invalid(S):
     case = lap_case(71, 100)
     with S.Server(1) as srv:

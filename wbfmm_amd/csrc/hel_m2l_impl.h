@@ -887,7 +887,10 @@ static int launch_hel_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nch
     }
     cp.chunk_prefix[8] = nch;
     cp.qc = 0;
-    const int pc = hel_packed_choice();
+    // the calling thread's variant (wbfmm_hip_set_variant): 0 chunks at
+    // the sparse levels, 1 or 2 the packed kernel there
+    const int var = wbfmm_hip_get_variant(WBFMM_VARIANT_M2L);
+    const int pc = var == 0 ? 0 : hel_packed_choice();
     const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
     if (nch > 0 && (pc == 1 || (pc == -1 && nch >= 512 && fill < 0.6))) {
       constexpr int W = N >= 16 ? 16 : 4;

@@ -2102,8 +2102,12 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
   constexpr int C = (N + 1) * (N + 1);
   unsigned ntot = 0;
   for (int c = 0; c < 8; c++) ntot += ca.count[c];
+  // the calling thread's variant (wbfmm_hip_set_variant): the kernel of
+  // the sparse levels, or -1 for the choice below
+  const int var = wbfmm_hip_get_variant(WBFMM_VARIANT_M2L);
+  (void)var;
   if constexpr (sizeof(T) == 8 && (PackedFits<N>::value || PackedClsFits<N>::value)) {
-    const int pc = m2l_packed_choice();
+    const int pc = var == 0 ? 0 : m2l_packed_choice();
     const double fill = (double)ntot / (double)((uint64_t)1 << (3 * level));
     // forced on, the kernel runs at any size (it needs no minimum chunk
     // count); chosen automatically, only where it pays
@@ -2120,7 +2124,7 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
       const char *pcl = getenv("WBFMM_PACKED_CLS");
       // offsets packed by class (WBFMM_PACKED_CLS=0: one offset per tile)
       if constexpr (PackedClsFits<N>::value) {
-        if (!(pcl && atoi(pcl) == 0) && !(pd && atoi(pd) == 0)) {
+        if (var == 2 || (var == -1 && !(pcl && atoi(pcl) == 0) && !(pd && atoi(pd) == 0))) {
           constexpr int WC = PackedClsFits<N>::W;
           if (WC == 16 || !(pw && atoi(pw) == 4)) {
             WBFMM_LAUNCH("lap_m2l", (k_lap_m2l_packed_cls<N, WC>), dim3(nch), dim3(64 * WC), 0,
@@ -2139,7 +2143,7 @@ static int launch_m2l(const uint32_t *cls, const ClassArgs &ca, unsigned nchunks
         }
       }
       if constexpr (PackedDbFits<N>::value) {
-        if (!(pd && atoi(pd) == 0)) {
+        if (var >= 1 || !(pd && atoi(pd) == 0)) {
           // class order; WBFMM_M2L_XCD=1 interleaves the 8 classes on each
           // XCD (measured 1.4 % slower on the 1e6-point depth-7 solve)
           unsigned grid = nch;

@@ -316,6 +316,19 @@ int wbfmm_hip_stream_destroy(void *s) {
   return 0;
 }
 
+static thread_local int t_variant[WBFMM_VARIANT_COUNT] = {-1, -1};
+
+int wbfmm_hip_set_variant(int which, int value) {
+  if (which < 0 || which >= WBFMM_VARIANT_COUNT || value < -1 || value > 2)
+    return fail_msg("unknown kernel variant", WBFMM_HIP_ERR_ARG);
+  t_variant[which] = value;
+  return 0;
+}
+
+int wbfmm_hip_get_variant(int which) {
+  return which >= 0 && which < WBFMM_VARIANT_COUNT ? t_variant[which] : -1;
+}
+
 int wbfmm_hip_set_stream(void *s, void **prev) {
   if (prev) *prev = (void *)t_stream;
   t_stream = (hipStream_t)s;

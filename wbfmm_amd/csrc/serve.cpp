@@ -66,6 +66,9 @@ int laplace_order(double tol) {
 struct Plan {
   int precision = 64;
   int depth = 0;
+  // kernel variants set on the worker's thread for the solve (-1: the
+  // library's choice); from the autotuner's table
+  int var[WBFMM_VARIANT_COUNT] = {-1, -1};
   int orders[WBFMM_TREE_MAX_DEPTH + 1] = {0};
   int lmax = 0;
   double x0[3] = {0, 0, 0};
@@ -92,6 +95,12 @@ struct Job {
 // a worker holds the set's `use` lock for its whole solve; solves end with
 // a stream synchronisation, so the next holder, on any stream, finds the
 // tables complete.
+// the autotuner's choice for a request class
+struct Tuned {
+  int depth = 0;
+  int var[WBFMM_VARIANT_COUNT] = {-1, -1};
+};
+
 struct HelKey {
   int lmax = 0, depth = 0;
   double k = 0.0, D = 0.0;
@@ -132,8 +141,9 @@ struct wbfmm_serve_s {
   uint64_t targets = 0;
   bool have_first = false;
   clk::time_point first_submit, last_done;
-  // autotuned depths: (problem, precision, order, log2 n) -> depth
-  std::map<std::tuple<int, int, int, int>, int> tuned;
+  // autotuned depth and kernel variants per request class:
+  // (problem, precision, order, log2 n) -> Tuned
+  std::map<std::tuple<int, int, int, int>, Tuned> tuned;
   // resident Helmholtz operator sets (at most hel_cap bytes of them idle)
   std::map<HelKey, std::shared_ptr<HelEntry>> hel_cache;
   size_t hel_bytes = 0, hel_cap = 0;
@@ -261,13 +271,14 @@ int make_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t &r, Plan &p) {
   const int lap_order = order_fixed ? order_fixed : laplace_order(r.tol);
   p.precision = choose_precision(r, lap_order);
   if (p.precision == 0) return WBFMM_HIP_ERR_ARG;
-  if (r.depth) {
-    p.depth = r.depth;
-  } else {
+  {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->tuned.find(
         tune_key(r, p.precision, r.problem == WBFMM_PROBLEM_LAPLACE ? lap_order : order_fixed));
-    p.depth = it != s->tuned.end() ? it->second : default_depth(r);
+    const bool have = it != s->tuned.end();
+    p.depth = r.depth ? r.depth : (have && it->second.depth > 0 ? it->second.depth : default_depth(r));
+    if (have)
+      for (int v = 0; v < WBFMM_VARIANT_COUNT; v++) p.var[v] = it->second.var[v];
   }
   p.lmax = 0;
   for (int l = 0; l <= p.depth; l++) {
@@ -613,7 +624,9 @@ void run(wbfmm_serve_t *s, Worker &w, Job &j) {
   int rc = helm ? helmholtz_ops(s, w, p, r.k, &tables_ms, hent, hheld, msg, sizeof(msg))
                 : laplace_ops(w, p.lmax, &S.op, msg, sizeof(msg));
   if (helm && !rc) S.op = hent->op;
+  for (int v = 0; v < WBFMM_VARIANT_COUNT; v++) wbfmm_hip_set_variant(v, p.var[v]);
   if (!rc) rc = wbfmm_guarded_call(solve_body, &S, msg, sizeof(msg));
+  for (int v = 0; v < WBFMM_VARIANT_COUNT; v++) wbfmm_hip_set_variant(v, -1);
   // the operator set is complete after a solve that ended well; after a
   // failure it is built again by its next user
   if (helm && hent) {
@@ -698,13 +711,18 @@ void worker_main(wbfmm_serve_t *s, Worker *w) {
 }
 
 // plan on the caller's thread (the bounds pass runs on the library stream)
-int64_t enqueue(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, bool tune) {
+// var (an autotuner run): the kernel variants to time, in place of the
+// table's
+int64_t enqueue(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, bool tune,
+                const int *var = nullptr) {
   auto j = std::make_unique<Job>();
   j->req = *r;
   j->tune = tune;
   j->t_submit = clk::now();
   int rc = make_plan(s, *r, j->plan);
   if (rc) return rc;
+  if (var)
+    for (int v = 0; v < WBFMM_VARIANT_COUNT; v++) j->plan.var[v] = var[v];
   j->res.depth = j->plan.depth;
   j->res.order = j->plan.lmax;
   j->res.precision = j->plan.precision;
@@ -838,32 +856,63 @@ gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gi
   double *scratch = nullptr;
   rc = wbfmm_hip_malloc((void **)&scratch, sizeof(double) * fw * std::max(ntgt, 1u));
   if (rc) return rc;
-  double best_ms = INFINITY;
-  int best_d = -1, order = 0;
-  for (int i = 0; i < ndepth; i++) {
+  // solve time of q at depth d with the kernel variants var (two runs:
+  // the first builds the worker's tables)
+  auto timed = [&](int d, const int *var, double &ms) {
     wbfmm_serve_request_t q = *r;
-    q.depth = depths[i];
+    q.depth = d;
     q.f = scratch;
     q.fstr = fw;
     q.check = 0;
-    // two runs: the first builds the worker's tables
-    double ms = INFINITY;
+    ms = INFINITY;
     for (int rep = 0; rep < 2; rep++) {
-      const int64_t id = enqueue(s, &q, true);
-      if (id < 0) {
-        rc = (int)id;
-        break;
-      }
+      const int64_t id = enqueue(s, &q, true, var);
+      if (id < 0) return (int)id;
       wbfmm_serve_result_t res;
-      rc = wbfmm_serve_wait(s, id, &res);
-      if (rc) break;
+      const int e = wbfmm_serve_wait(s, id, &res);
+      if (e) return e;
+      if (res.status) return res.status;
       ms = res.solve_ms - res.tables_ms;
-      order = res.order;
     }
-    if (rc) break;
-    if (ms < best_ms) {
+    return 0;
+  };
+  const int dflt[WBFMM_VARIANT_COUNT] = {-1, -1};
+  Tuned t;
+  double best_ms = INFINITY;
+  // the depth with the library's kernels, then each kernel variant at that
+  // depth, one kind at a time: the sparse-level M2L tiles (chunks, offset
+  // tiles, class tiles; Helmholtz chunks or class tiles), the Helmholtz
+  // field (thread per target, wavefront per target box)
+  for (int i = 0; i < ndepth && !rc; i++) {
+    double ms;
+    rc = timed(depths[i], dflt, ms);
+    if (!rc && ms < best_ms) {
       best_ms = ms;
-      best_d = depths[i];
+      t.depth = depths[i];
+    }
+  }
+  const bool helm = r->problem == WBFMM_PROBLEM_HELMHOLTZ;
+  for (int kind = 0; kind < WBFMM_VARIANT_COUNT && !rc; kind++) {
+    if (kind == WBFMM_VARIANT_FIELD && !helm) continue;
+    const int cand_m2l[] = {0, 1, 2}, cand_hm2l[] = {0, 2}, cand_field[] = {0, 1};
+    const int *cand = kind == WBFMM_VARIANT_M2L ? (helm ? cand_hm2l : cand_m2l) : cand_field;
+    const int nc = kind == WBFMM_VARIANT_M2L ? (helm ? 2 : 3) : 2;
+    int bv = -1;
+    double bms = best_ms;
+    for (int c = 0; c < nc && !rc; c++) {
+      int var[WBFMM_VARIANT_COUNT];
+      for (int v = 0; v < WBFMM_VARIANT_COUNT; v++) var[v] = t.var[v];
+      var[kind] = cand[c];
+      double ms;
+      rc = timed(t.depth, var, ms);
+      if (!rc && ms < bms) {
+        bms = ms;
+        bv = cand[c];
+      }
+    }
+    if (bv >= 0) {
+      t.var[kind] = bv;
+      best_ms = bms;
     }
   }
   wbfmm_hip_sync();
@@ -877,10 +926,27 @@ gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gi
     const int key_order = r->problem == WBFMM_PROBLEM_LAPLACE
                               ? (r->order ? r->order : laplace_order(r->tol))
                               : r->order;
-    (void)order;
-    s->tuned[tune_key(*r, p.precision, key_order)] = best_d;
+    s->tuned[tune_key(*r, p.precision, key_order)] = t;
   }
-  if (best) *best = best_d;
+  if (best) *best = t.depth;
+  return 0;
+}
+
+gint wbfmm_serve_tuned(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, gint *depth,
+                       gint *var_m2l, gint *var_field) {
+  if (!s || !r) return WBFMM_HIP_ERR_ARG;
+  Plan p;
+  const int rc = make_plan(s, *r, p);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int key_order = r->problem == WBFMM_PROBLEM_LAPLACE
+                            ? (r->order ? r->order : laplace_order(r->tol))
+                            : r->order;
+  auto it = s->tuned.find(tune_key(*r, p.precision, key_order));
+  const Tuned t = it != s->tuned.end() ? it->second : Tuned{};
+  if (depth) *depth = t.depth;
+  if (var_m2l) *var_m2l = t.var[WBFMM_VARIANT_M2L];
+  if (var_field) *var_field = t.var[WBFMM_VARIANT_FIELD];
   return 0;
 }
 

@@ -2756,7 +2756,11 @@ gint wbfmm_downward_pass_avx(wbfmm_tree_t *t, wbfmm_shift_operators_t *op,
    slower on leaves of a few tens of targets) */
 static int hel_field_box64(void)
 {
-  const char *e = getenv("WBFMM_HEL_FIELD");
+  /* the calling thread's variant (wbfmm_hip_set_variant) first */
+  const int v = wbfmm_hip_get_variant(WBFMM_VARIANT_FIELD);
+  const char *e;
+  if (v >= 0) return v == 1;
+  e = getenv("WBFMM_HEL_FIELD");
   return e && strcmp(e, "box") == 0;
 }
 

@@ -63,6 +63,8 @@ _SIGS = {
                               ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
     "wbfmm_serve_tune": (_i, [_vp, ctypes.POINTER(Request), ctypes.POINTER(_i), _i,
                               ctypes.POINTER(_i)]),
+    "wbfmm_serve_tuned": (_i, [_vp, ctypes.POINTER(Request), ctypes.POINTER(_i),
+                               ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "wbfmm_serve_free": (_i, [_vp]),
     "wbfmm_serve_debug_fail_allocations": (_i, [_vp, _i]),
 }
@@ -150,6 +152,17 @@ class Server:
         if rc:
             raise RuntimeError(f"wbfmm_serve_tune = {rc}")
         return best.value
+
+    def tuned(self, req):
+        """the autotuner's choice for req's class: (depth, M2L variant,
+        field variant); depth 0 and variants -1 when the class is not
+        tuned"""
+        d, vm, vf = _i(), _i(), _i()
+        rc = _L().wbfmm_serve_tuned(self.s, ctypes.byref(req), ctypes.byref(d), ctypes.byref(vm),
+                                    ctypes.byref(vf))
+        if rc:
+            raise RuntimeError(f"wbfmm_serve_tuned = {rc}")
+        return d.value, vm.value, vf.value
 
     def debug_fail_allocations(self, count):
         """the next submitted request has its first `count` device
