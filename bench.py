@@ -377,6 +377,25 @@ def run_batched(args):
     ntot = sum(p["n"] for p in probs)
     print(f"bench: {len(probs)} problems ready, {ntot} targets", file=sys.stderr, flush=True)
     srv = S.Server(args.streams)
+    tuned = {}
+    tune_s = 0.0
+    if args.batch_depth == "tuned":
+        # one autotuner run per request class (kernel, log2 N; the engine's
+        # class also holds the precision and order, the same here for every
+        # problem of a kernel): depths around the engine's own, then the
+        # kernel variants at the best depth; untimed, like the table builds
+        t_tune = time.perf_counter()
+        for p in probs:
+            key = ("helmholtz" if p["helm"] else "laplace", int(np.floor(np.log2(p["n"]))))
+            if key in tuned:
+                continue
+            r = _request(S, p, precision=PRECISION[args.precision])
+            d0 = srv.plan(r)[0]
+            best = srv.tune(r, sorted({max(2, d0 - 1), d0, min(10, d0 + 1)}))
+            _, vm, vf = srv.tuned(r)
+            tuned[key] = (best, vm, vf)
+        tune_s = time.perf_counter() - t_tune
+        print(f"bench: tuned {len(tuned)} classes in {tune_s:.1f} s", file=sys.stderr, flush=True)
 
     def step(check):
         tk = [srv.submit(_request(S, p, depth=p["depth"] if args.batch_depth == "formula" else 0,
@@ -423,6 +442,8 @@ def run_batched(args):
         "config": {"workload": f"batched: {len(probs)} problems, N = 10^U(3,5), Laplace / "
                                "Helmholtz alternating (k ~ U(1,20)), cube or sphere, depth "
                                + ("max(2, ceil(log8(N/20)))" if args.batch_depth == "formula"
+                                  else "and kernel variants from the engine's autotuner per "
+                                       "class" if args.batch_depth == "tuned"
                                   else "from N by the engine (about 16 points per leaf for "
                                        "Laplace, 32 for Helmholtz)")
                                + ", orders and precision from tol 1e-6 (precision auto: "
@@ -435,6 +456,11 @@ def run_batched(args):
         "solve_ms_p50": float(np.percentile(solve, 50)),
         "solve_ms_p99": float(np.percentile(solve, 99)),
         "tables_ms_total": float(tables.sum()),
+        **({"autotuner": {"seconds": tune_s,
+                          "classes": {f"{k[0]} log2N={k[1]}": {"depth": v[0], "m2l_variant": v[1],
+                                                               "field_variant": v[2]}
+                                      for k, v in sorted(tuned.items())}}}
+           if tuned else {}),
         "error_vs_fp64_direct": {"targets_checked_per_problem": args.check_targets,
                                  "rel_l2_potential_median": float(np.median(ep)),
                                  "rel_l2_potential_max": float(ep.max()),
@@ -635,9 +661,10 @@ def main():
                          "independent problems per step; serving: Poisson request stream")
     ap.add_argument("--streams", type=int, default=8, help="serving engine streams")
     ap.add_argument("--batch", type=int, default=256, help="problems per batched step")
-    ap.add_argument("--batch-depth", default="engine", choices=["engine", "formula"],
-                    help="batched: depth picked by the engine from N, or the fixed formula "
-                         "max(2, ceil(log8(N/20)))")
+    ap.add_argument("--batch-depth", default="engine", choices=["engine", "formula", "tuned"],
+                    help="batched: depth picked by the engine from N, the fixed formula "
+                         "max(2, ceil(log8(N/20))), or the engine's autotuner (depth and "
+                         "kernel variants per request class, tuned before the timed steps)")
     ap.add_argument("--requests", type=int, default=48, help="serving requests")
     ap.add_argument("--qps", type=float, default=4.0, help="serving arrival rate")
     ap.add_argument("--precision", default="auto", choices=["auto", "32", "64"],

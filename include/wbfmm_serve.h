@@ -114,9 +114,10 @@ gint wbfmm_serve_reset_stats(wbfmm_serve_t *s);
 /* depth, largest order and estimated device bytes the engine would use */
 gint wbfmm_serve_plan(wbfmm_serve_t *s, const wbfmm_serve_request_t *r,
                       gint *depth, gint *order, gsize *footprint);
-/* autotuner: solve r (into a scratch output) at each of the ndepth
-   candidate depths; at the fastest, time the kernel variants one kind at a
-   time (wbfmm_hip_set_variant: the sparse-level M2L tiles, and for
+/* autotuner: solve r (into scratch outputs, as many copies at once as the
+   engine has streams, the way a batch shares the GPU) at each of the
+   ndepth candidate depths; at the fastest, time the kernel variants one
+   kind at a time (wbfmm_hip_set_variant: the sparse-level M2L tiles, and for
    Helmholtz the field kernel), and keep the fastest depth and variants for
    requests of the same class (problem, precision, order, log2 of the point
    count); returns the depth in *best */

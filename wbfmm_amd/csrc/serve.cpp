@@ -853,26 +853,48 @@ gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gi
   if (rc) return rc;
   const int fw = field_width(r->problem, r->field);
   const unsigned ntgt = r->xt ? r->ntgt : r->nsrc;
-  double *scratch = nullptr;
-  rc = wbfmm_hip_malloc((void **)&scratch, sizeof(double) * fw * std::max(ntgt, 1u));
-  if (rc) return rc;
-  // solve time of q at depth d with the kernel variants var (two runs:
-  // the first builds the worker's tables)
+  // as many copies as the engine has workers, each with its own scratch
+  // output: a class is timed the way a batch runs it, its solves sharing
+  // the GPU (alone on an idle GPU, fewer and larger launches would win)
+  const int ncopy = (int)s->workers.size();
+  std::vector<double *> scratch(ncopy, nullptr);
+  for (int c = 0; c < ncopy && !rc; c++)
+    rc = wbfmm_hip_malloc((void **)&scratch[c], sizeof(double) * fw * std::max(ntgt, 1u));
+  auto release = [&]() {
+    wbfmm_hip_sync();
+    for (double *p : scratch)
+      if (p) wbfmm_hip_free(p);
+  };
+  if (rc) {
+    release();
+    return rc;
+  }
+  // wall time per solve of q at depth d with the kernel variants var, the
+  // copies submitted at once (two rounds: the first builds the workers'
+  // tables)
   auto timed = [&](int d, const int *var, double &ms) {
-    wbfmm_serve_request_t q = *r;
-    q.depth = d;
-    q.f = scratch;
-    q.fstr = fw;
-    q.check = 0;
     ms = INFINITY;
     for (int rep = 0; rep < 2; rep++) {
-      const int64_t id = enqueue(s, &q, true, var);
-      if (id < 0) return (int)id;
-      wbfmm_serve_result_t res;
-      const int e = wbfmm_serve_wait(s, id, &res);
-      if (e) return e;
-      if (res.status) return res.status;
-      ms = res.solve_ms - res.tables_ms;
+      std::vector<int64_t> ids;
+      const auto t0 = clk::now();
+      for (int c = 0; c < ncopy; c++) {
+        wbfmm_serve_request_t q = *r;
+        q.depth = d;
+        q.f = scratch[c];
+        q.fstr = fw;
+        q.check = 0;
+        const int64_t id = enqueue(s, &q, true, var);
+        if (id < 0) return (int)id;
+        ids.push_back(id);
+      }
+      int err = 0;
+      for (int64_t id : ids) {
+        wbfmm_serve_result_t res;
+        const int e = wbfmm_serve_wait(s, id, &res);
+        if (!err) err = e ? e : res.status;
+      }
+      if (err) return err;
+      ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count() / ncopy;
     }
     return 0;
   };
@@ -915,8 +937,7 @@ gint wbfmm_serve_tune(wbfmm_serve_t *s, const wbfmm_serve_request_t *r, const gi
       best_ms = bms;
     }
   }
-  wbfmm_hip_sync();
-  wbfmm_hip_free(scratch);
+  release();
   if (rc) return rc;
   {
     Plan p;
