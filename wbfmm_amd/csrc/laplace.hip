@@ -559,6 +559,17 @@ __device__ __forceinline__ void lcoef(const T *mpr, uint64_t b, int C,
   }
 }
 
+// lcoef without a branch on m (both reads issued; the imaginary part 0
+// for m = 0)
+template <typename T>
+__device__ __forceinline__ void lcoef_nb(const T *mpr, uint64_t b, int C, int n, int m,
+                                         double &re, double &im) {
+  const int ir = m ? n * n + 2 * m - 1 : n * n, ii = m ? n * n + 2 * m : n * n;
+  re = mpr[coef_at(b, ir, C)];
+  const double v = mpr[coef_at(b, ii, C)];
+  im = m ? v : 0.0;
+}
+
 // T: coefficient type; rs as in k_lap_p2m (1/w for coefficients scaled by
 // the box width, L_n w^(n+1): the expansion then gives phi w and grad w^2)
 template <typename T>
@@ -611,29 +622,33 @@ __global__ void k_lap_field(const double *xt, const uint32_t *tid,
       }
       double Rr = rn * p0 * cm, Ri = rn * p0 * sm;
       double lr, li;
-      lcoef(mpr, b, C, j, k, lr, li);
+      lcoef_nb(mpr, b, C, j, k, lr, li);
       phi += wk * (lr * Rr - li * Ri);
       if (want_g && j < N) {
         int n1 = j + 1;
         double fc = sqrt((2.0 * n1 + 1) / (2.0 * n1 - 1));
+        // the three coefficients of degree j + 1 read without branches
+        // (the conditions are the same in every lane), so that their
+        // loads issue together: L^k, L^{k-1} (weight 0 at k = 0, read at
+        // order 0 there), L^{k+1} (k + 1 <= j + 1 always)
+        double l0r, l0i, lmr, lmi, lpr, lpi;
+        const int mm = k >= 1 ? k - 1 : 0, mp = k + 1;
+        lcoef_nb(mpr, b, C, n1, k, l0r, l0i);
+        lcoef_nb(mpr, b, C, n1, mm, lmr, lmi);
+        lcoef_nb(mpr, b, C, n1, mp, lpr, lpi);
         // dz: a_{j+1}^k L_{j+1}^k R_j^k
-        lcoef(mpr, b, C, n1, k, lr, li);
-        gz += wk * fc * sqrt((double)(n1 - k) * (n1 + k)) * (lr * Rr - li * Ri);
+        gz += wk * fc * sqrt((double)(n1 - k) * (n1 + k)) * (l0r * Rr - l0i * Ri);
         // D+: -b1_{j+1}^{k-1} L_{j+1}^{k-1} R_j^k
-        if (k >= 1) {
-          int mm = k - 1;
-          double b1 = fc * sqrt((double)(n1 - mm) * (n1 - mm - 1));
-          lcoef(mpr, b, C, n1, mm, lr, li);
-          gpr -= b1 * (lr * Rr - li * Ri);
-          gpi -= b1 * (lr * Ri + li * Rr);
+        {
+          double b1 = k >= 1 ? fc * sqrt((double)(n1 - mm) * (n1 - mm - 1)) : 0.0;
+          gpr -= b1 * (lmr * Rr - lmi * Ri);
+          gpi -= b1 * (lmr * Ri + lmi * Rr);
         }
         // D+: b2_{j+1}^{k+1} conj(L_{j+1}^{k+1} R_j^k)
-        if (k + 1 <= n1) {
-          int mp = k + 1;
+        {
           double b2 = fc * sqrt((double)(n1 + mp) * (n1 + mp - 1));
-          lcoef(mpr, b, C, n1, mp, lr, li);
-          gpr += b2 * (lr * Rr - li * Ri);
-          gpi -= b2 * (lr * Ri + li * Rr);
+          gpr += b2 * (lpr * Rr - lpi * Ri);
+          gpi -= b2 * (lpr * Ri + lpi * Rr);
         }
       }
     }
