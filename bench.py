@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per lap_m2l launch (mean over the 6 launches of one solve) from
 # rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a copy kernel of known size;
 # measured with the commands in profiles/lap_m2l_laplace_1e6_depth7_p12.md
-M2L_TRAFFIC_BYTES_PER_LAUNCH = 22.16e9
+M2L_TRAFFIC_BYTES_PER_LAUNCH = 21.89e9
 
 
 def splitmix64(seed, n):
