@@ -421,6 +421,32 @@ def test_packed_m2l_orders_13_15_match_oracle(wb, oracle, monkeypatch, name, p):
         assert rel(fg[:, c], fo[:, c]) < PARITY, (name, p, c, rel(fg[:, c], fo[:, c]))
 
 
+def test_m2l_variants_match_oracle(wb, oracle):
+    # the calling thread's M2L variant (wbfmm_hip_set_variant, as the
+    # serving engine's autotuner sets it) at the sparse level of a tree
+    # where the sparse-level kernel is chosen (25,000 uniform points at
+    # depth 5, as below): chunks, offset tiles and class tiles each against
+    # the oracle on the same tree
+    rng = np.random.default_rng(81)
+    n = 25_000
+    xs = rng.uniform(0, 1, (n, 3))
+    q = rng.normal(size=n)
+    x0, D = fit_box(xs)
+    fo, _, _ = oracle.fmm(1, xs, xs, x0, D, 5, [12] * 17, [12] * 17, q=q, field=3, nthreads=8)
+    L = wb.lib()
+    try:
+        for v in (0, 1, 2):
+            assert L.wbfmm_hip_set_variant(0, v) == 0
+            assert L.wbfmm_hip_get_variant(0) == v
+            print(f"m2l variant {v}", flush=True)
+            fg = gpu_fmm(wb, xs, q, xs, x0, D, 5, 12)
+            for c in range(4):
+                assert rel(fg[:, c], fo[:, c]) < PARITY, (v, c, rel(fg[:, c], fo[:, c]))
+    finally:
+        L.wbfmm_hip_set_variant(0, -1)
+    assert L.wbfmm_hip_set_variant(0, 3) != 0  # unknown variant refused
+
+
 def test_packed_m2l_chosen_by_size_matches_oracle(wb, oracle):
     # 25,000 uniform points at depth 5: level 5 has about 17,500 occupied
     # boxes (53 %) in more than 1024 chunks, so the sparse-level kernel is

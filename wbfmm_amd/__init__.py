@@ -135,6 +135,8 @@ _SIGS = {
     "wbfmm_truncation_number": (_i, [_vp, _d, _u, _d]),
     "wbfmm_hip_synchronize": (_i, []),
     "wbfmm_hip_sync": (_i, []),
+    "wbfmm_hip_set_variant": (_i, [_i, _i]),
+    "wbfmm_hip_get_variant": (_i, [_i]),
     "wbfmm_hip_init": (_i, []),
     "wbfmm_hip_timing_enable": (_i, [_i]),
     "wbfmm_hip_timing_reset": (_i, []),
