@@ -37,3 +37,23 @@ def test_headers_compile_as_c(tmp_path):
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                         "-c", str(src), "-o", str(tmp_path / "t.o")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_kernel_variant_switch(wb):
+    # host-side state of the calling thread only (no device work): the
+    # M2L variant takes -1..2, the field variant -1..1, anything else is
+    # refused and leaves the setting as it was
+    lib = ctypes.CDLL(wb.LIB_PATH)
+    try:
+        for which, vmax in ((0, 2), (1, 1)):
+            for v in range(-1, vmax + 1):
+                assert lib.wbfmm_hip_set_variant(which, v) == 0
+                assert lib.wbfmm_hip_get_variant(which) == v
+            assert lib.wbfmm_hip_set_variant(which, vmax + 1) != 0
+            assert lib.wbfmm_hip_set_variant(which, -2) != 0
+            assert lib.wbfmm_hip_get_variant(which) == vmax
+        assert lib.wbfmm_hip_set_variant(2, 0) != 0
+        assert lib.wbfmm_hip_get_variant(2) == -1
+    finally:
+        lib.wbfmm_hip_set_variant(0, -1)
+        lib.wbfmm_hip_set_variant(1, -1)

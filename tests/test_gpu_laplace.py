@@ -444,7 +444,9 @@ def test_m2l_variants_match_oracle(wb, oracle):
                 assert rel(fg[:, c], fo[:, c]) < PARITY, (v, c, rel(fg[:, c], fo[:, c]))
     finally:
         L.wbfmm_hip_set_variant(0, -1)
-    assert L.wbfmm_hip_set_variant(0, 3) != 0  # unknown variant refused
+    assert L.wbfmm_hip_set_variant(0, 3) != 0  # unknown variants refused
+    assert L.wbfmm_hip_set_variant(1, 2) != 0
+    assert L.wbfmm_hip_set_variant(2, 0) != 0
 
 
 def test_packed_m2l_chosen_by_size_matches_oracle(wb, oracle):

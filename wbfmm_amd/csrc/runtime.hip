@@ -319,7 +319,9 @@ int wbfmm_hip_stream_destroy(void *s) {
 static thread_local int t_variant[WBFMM_VARIANT_COUNT] = {-1, -1};
 
 int wbfmm_hip_set_variant(int which, int value) {
-  if (which < 0 || which >= WBFMM_VARIANT_COUNT || value < -1 || value > 2)
+  // M2L: -1..2; field: -1..1
+  const int vmax = which == WBFMM_VARIANT_FIELD ? 1 : 2;
+  if (which < 0 || which >= WBFMM_VARIANT_COUNT || value < -1 || value > vmax)
     return fail_msg("unknown kernel variant", WBFMM_HIP_ERR_ARG);
   t_variant[which] = value;
   return 0;
